@@ -1,0 +1,164 @@
+/*
+ * mcg.h -- C ABI of libmcg, the MI355X (gfx950) clustering hot path.
+ *
+ * Every entry point returns an int status (MCG_OK == 0, negative on error);
+ * the message of the last error on the calling thread is in mcg_last_error().
+ * Nothing throws across this boundary.
+ *
+ * Pointer arguments may be host or device memory; the library classifies
+ * each one (hipPointerGetAttributes) and copies host buffers as needed.
+ * The caller owns every buffer it passes in and keeps it alive for the call
+ * (a dataset built from a device buffer borrows it for the dataset's life,
+ * like the reference's CSparseMatrix borrows numpy buffers,
+ * python/pycsparse.h:33-70).  The library owns its device buffers inside an
+ * mcg_ctx (one HIP stream per context).  A context is not safe for use by
+ * two threads at once.
+ *
+ * Layout conventions: dense matrices are row-major float32, n rows x d
+ * columns (leading dimension ld >= d).  Centers are dense row-major float32,
+ * k x d.  CSR matrices: values, column indices (sorted within each row),
+ * row pointers of length n+1.
+ *
+ * Costs follow the reference's argument order msr(first = data row,
+ * second = center) (include/minicore/dist/applicator.h:1632).
+ */
+#ifndef MCG_H
+#define MCG_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define MCG_OK            0
+#define MCG_EINVAL       -1   /* invalid argument (reference: std::invalid_argument) */
+#define MCG_ENOMEM       -2   /* device allocation failed */
+#define MCG_EDEVICE      -3   /* HIP runtime / device error */
+#define MCG_EUNSUPPORTED -4   /* measure / dtype / shape without a GPU kernel */
+#define MCG_ENAN         -5   /* NaN center (centroid.h:771-775 raises runtime_error) */
+
+/* element types */
+#define MCG_F32 0
+#define MCG_F64 1
+#define MCG_U8  2
+#define MCG_U16 3
+#define MCG_U32 4
+#define MCG_U64 5
+#define MCG_I32 6
+#define MCG_I64 7
+
+/* measure codes: include/minicore/dist/distance.h:24-64 */
+#define MCG_SQRL2        2
+#define MCG_JSD          4
+#define MCG_MKL          5
+#define MCG_REVERSE_MKL 11
+
+/* cross-term precision for dense SQRL2 */
+#define MCG_PREC_FP32  0   /* FP32 MFMA cross term, FP32 compare ("plain FP32 form") */
+#define MCG_PREC_BF16  1   /* BF16 MFMA cross term + FP32 re-score of near ties */
+#define MCG_PREC_EXACT 2   /* direct FP32 sum of squared differences per pair */
+
+/* coreset sensitivity methods: include/minicore/coreset/coreset.h:34-48 */
+#define MCG_SENS_BFL 0
+#define MCG_SENS_FL  1
+#define MCG_SENS_VX  3
+#define MCG_SENS_LBK 4
+
+typedef struct mcg_ctx  mcg_ctx;
+typedef struct mcg_data mcg_data;
+
+/* ---- library / context -------------------------------------------------- */
+const char *mcg_last_error(void);
+int mcg_device_count(int *count);
+int mcg_ctx_create(int device, mcg_ctx **out);
+int mcg_ctx_destroy(mcg_ctx *ctx);
+int mcg_ctx_synchronize(mcg_ctx *ctx);
+/* The HIP stream of the context (as void*), for callers that enqueue their own work. */
+int mcg_ctx_stream(mcg_ctx *ctx, void **stream);
+
+/* Kernel timing on the context's stream (HIP events around each launch of the
+ * timed kernels).  which: 0 = E-step cross-term kernel, 1 = M-step partial-sum
+ * kernel.  mcg_timing_read returns the summed milliseconds and launch count
+ * since the last reset. */
+int mcg_timing_enable(mcg_ctx *ctx, int on);
+int mcg_timing_reset(mcg_ctx *ctx);
+int mcg_timing_read(mcg_ctx *ctx, int which, double *total_ms, uint64_t *count);
+
+/* ---- datasets ----------------------------------------------------------- */
+/* Dense rows.  Replaces the numpy-array view that python/pycluster_dense.cpp:5-30
+ * and python/pydense.cpp:34-77 build (c_style|forcecast arrays of float/double).
+ * dtype MCG_F32 or MCG_F64 (converted to float32 on the device). */
+int mcg_data_dense(mcg_ctx *ctx, const void *x, int dtype, uint64_t n, uint64_t d,
+                   uint64_t ld, mcg_data **out);
+/* CSR rows.  Replaces PyCSparseMatrix (python/pycsparse.h:33-244):
+ * data f32/f64/u16, indices u16/u32/i32/u64/i64, indptr u32/u64/i32/i64. */
+int mcg_data_csr(mcg_ctx *ctx, const void *data, int data_dtype, const void *indices,
+                 int idx_dtype, const void *indptr, int ptr_dtype, uint64_t n, uint64_t d,
+                 uint64_t nnz, mcg_data **out);
+int mcg_data_free(mcg_data *ds);
+int mcg_data_shape(const mcg_data *ds, uint64_t *n, uint64_t *d, uint64_t *nnz);
+
+/* ---- distance / assignment ---------------------------------------------- */
+/* Full n x k cost matrix (float32, row-major).  Replaces the per-pair
+ * msr_with_prior loops of python/pycmp.cpp:16-246 and pyarrcmp.cpp:126-145
+ * (cmp) and pycluster_soft.h:38-40 (soft costs). */
+int mcg_cost_matrix(mcg_data *ds, int msr, double prior, const float *centers, uint64_t k,
+                    float *out);
+/* Hard E-step: asn[i] = argmin_c msr(row i, center c) (lowest index on ties),
+ * costs[i] = that minimum (nullable).  Replaces assign_points_hard,
+ * include/minicore/clustering/solve.h:234-285.  precision applies to dense SQRL2. */
+int mcg_assign_hard(mcg_data *ds, int msr, double prior, const float *centers, uint64_t k,
+                    int precision, uint32_t *asn, float *costs);
+/* Hard M-step: weighted mean of each cluster's rows, FP64 accumulation in a
+ * fixed order; empty clusters keep their center.  Replaces set_centroids_hard
+ * without the restart (solve.h:184-232, centroid.h:81-178).  weights: f64[n] or NULL.
+ * n_empty (nullable) receives the number of empty clusters. */
+int mcg_centroids_hard(mcg_data *ds, const uint32_t *asn, const double *weights, uint64_t k,
+                       float *centers, uint64_t *n_empty);
+/* Lloyd driver: perform_hard_clustering, solve.h:98-175, with the empty-cluster
+ * restart of centroid.h:687-752.  centers (k x d) in/out; asn/costs outputs
+ * (costs nullable).  Returns the reference's (initcost, finalcost, numiter). */
+int mcg_lloyd(mcg_data *ds, int msr, double prior, float *centers, uint64_t k,
+              const double *weights, double eps, uint64_t maxiter, int precision,
+              uint32_t *asn, float *costs, double *initcost, double *finalcost,
+              uint64_t *numiter);
+/* One Lloyd iteration on device-resident state: asn = argmin over centers
+ * (E-step), then centers = weighted cluster means (M-step, in place; empty
+ * clusters keep their center); *cost = the assignment's cost against the input
+ * centers (FP64; for dense SQRL2 measured in the M-step pass, otherwise the
+ * weighted sum of the E-step minima).  No restart, no convergence test.
+ * Replaces one pass of lloyd_iteration, include/minicore/optim/kmeans.h:359-479.
+ * centers and asn must be device pointers. */
+int mcg_lloyd_step(mcg_data *ds, int msr, double prior, float *centers, uint64_t k,
+                   const double *weights, int precision, uint32_t *asn, double *cost);
+/* Soft EM driver: perform_soft_clustering, solve.h:287-374 with
+ * set_centroids_full_mean (soft), centroid.h:781-963.  costs and resp are
+ * n x k float32 outputs (nullable). */
+int mcg_soft_em(mcg_data *ds, int msr, double prior, double temp, float *centers, uint64_t k,
+                const double *weights, double eps, uint64_t maxiter, float *costs, float *resp,
+                double *initcost, double *finalcost, uint64_t *numiter);
+
+/* ---- seeding and coresets ------------------------------------------------ */
+/* D^2 seeding: coresets::kmeanspp, include/minicore/optim/kmeans.h:49-215,
+ * as bound by python/smw.h:179-410 (n_local_trials = 1, lspp = 0).
+ * centers u32[k] (row indices), asn u32[n], costs f32[n].  weights nullable. */
+int mcg_kmeanspp(mcg_data *ds, int msr, double prior, uint64_t k, uint64_t seed,
+                 const double *weights, uint32_t *centers, uint32_t *asn, float *costs);
+/* Sensitivities: CoresetSampler::make_sampler, coreset.h:386-578.
+ * costs f64[n], asn u32[n], weights f64[n] or NULL; probs f32[n]. */
+int mcg_coreset_probs(mcg_ctx *ctx, int sens, uint64_t n, uint64_t ncenters,
+                      const double *costs, const uint32_t *asn, const double *weights,
+                      float *probs);
+/* m independent draws from probs: CoresetSampler::sample, coreset.h:625-651.
+ * Draw t uses the (first_draw + t)-th output of the seed's random stream.
+ * indices u64[m], out_weights f32[m] = w_i / (m * p_i). */
+int mcg_coreset_sample(mcg_ctx *ctx, uint64_t n, const float *probs, const double *weights,
+                       uint64_t m, uint64_t seed, uint64_t first_draw, uint64_t *indices,
+                       float *out_weights);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

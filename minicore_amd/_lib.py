@@ -1,0 +1,281 @@
+"""ctypes binding of the libmcg C ABI (include/mcg.h).
+
+The product path: every call goes to the HIP library.  There is no CPU
+fallback; a missing or unloadable libmcg.so raises immediately.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libmcg.so")
+
+MCG_F32, MCG_F64, MCG_U8, MCG_U16, MCG_U32, MCG_U64, MCG_I32, MCG_I64 = range(8)
+SQRL2, JSD, MKL, REVERSE_MKL = 2, 4, 5, 11
+PREC_FP32, PREC_BF16, PREC_EXACT = 0, 1, 2
+SENS_BFL, SENS_FL, SENS_VX, SENS_LBK = 0, 1, 3, 4
+
+EXPORTS = [
+    "mcg_last_error", "mcg_device_count", "mcg_ctx_create", "mcg_ctx_destroy", "mcg_ctx_synchronize",
+    "mcg_ctx_stream", "mcg_data_dense", "mcg_data_csr", "mcg_data_free", "mcg_data_shape",
+    "mcg_cost_matrix", "mcg_assign_hard", "mcg_centroids_hard", "mcg_lloyd", "mcg_soft_em",
+    "mcg_kmeanspp", "mcg_coreset_probs", "mcg_coreset_sample", "mcg_lloyd_step", "mcg_timing_enable",
+    "mcg_timing_reset", "mcg_timing_read",
+]
+
+_lib = None
+
+
+class MCGError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libmcg error {code}: {msg}")
+        self.code = code
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise RuntimeError(f"{LIBPATH} is missing: build it with `python -m minicore_amd.build`")
+    L = C.CDLL(LIBPATH)
+    P, U64, D, I = C.c_void_p, C.c_uint64, C.c_double, C.c_int
+    sig = {
+        "mcg_last_error": (C.c_char_p, []),
+        "mcg_device_count": (I, [P]),
+        "mcg_ctx_create": (I, [I, P]),
+        "mcg_ctx_destroy": (I, [P]),
+        "mcg_ctx_synchronize": (I, [P]),
+        "mcg_ctx_stream": (I, [P, P]),
+        "mcg_data_dense": (I, [P, P, I, U64, U64, U64, P]),
+        "mcg_data_csr": (I, [P, P, I, P, I, P, I, U64, U64, U64, P]),
+        "mcg_data_free": (I, [P]),
+        "mcg_data_shape": (I, [P, P, P, P]),
+        "mcg_cost_matrix": (I, [P, I, D, P, U64, P]),
+        "mcg_assign_hard": (I, [P, I, D, P, U64, I, P, P]),
+        "mcg_centroids_hard": (I, [P, P, P, U64, P, P]),
+        "mcg_lloyd": (I, [P, I, D, P, U64, P, D, U64, I, P, P, P, P, P]),
+        "mcg_soft_em": (I, [P, I, D, D, P, U64, P, D, U64, P, P, P, P, P]),
+        "mcg_kmeanspp": (I, [P, I, D, U64, U64, P, P, P, P]),
+        "mcg_coreset_probs": (I, [P, I, U64, U64, P, P, P, P]),
+        "mcg_coreset_sample": (I, [P, U64, P, P, U64, U64, U64, P, P]),
+        "mcg_lloyd_step": (I, [P, I, D, P, U64, P, I, P, P]),
+        "mcg_timing_enable": (I, [P, I]),
+        "mcg_timing_reset": (I, [P]),
+        "mcg_timing_read": (I, [P, I, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        msg = load().mcg_last_error()
+        raise MCGError(rc, msg.decode() if msg else "")
+
+
+def ptr(a):
+    """Address of a numpy array or a torch tensor (device pointers pass straight through)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data_as(C.c_void_p)
+    if hasattr(a, "data_ptr"):
+        return C.c_void_p(a.data_ptr())
+    raise TypeError(f"unsupported buffer type {type(a)}")
+
+
+def device_count():
+    n = C.c_int(0)
+    check(load().mcg_device_count(C.byref(n)))
+    return n.value
+
+
+class Context:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(load().mcg_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            load().mcg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        check(load().mcg_ctx_synchronize(self.h))
+
+    def timing(self, on=True):
+        check(load().mcg_timing_enable(self.h, 1 if on else 0))
+        check(load().mcg_timing_reset(self.h))
+
+    def timing_read(self, which):
+        ms, cnt = C.c_double(), C.c_uint64()
+        check(load().mcg_timing_read(self.h, which, C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value
+
+
+_default = None
+
+
+def default_context():
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default
+
+
+def _dtype_code(dt):
+    dt = np.dtype(dt)
+    table = {np.float32: MCG_F32, np.float64: MCG_F64, np.uint8: MCG_U8, np.uint16: MCG_U16,
+             np.uint32: MCG_U32, np.uint64: MCG_U64, np.int32: MCG_I32, np.int64: MCG_I64}
+    for k, v in table.items():
+        if dt == np.dtype(k):
+            return v
+    raise TypeError(f"unsupported dtype {dt}")
+
+
+class Dataset:
+    """Rows resident on the device (dense float32 or canonical CSR)."""
+
+    def __init__(self, ctx, h, keep=()):
+        self.ctx = ctx
+        self.h = h
+        self._keep = keep
+        n, d, nnz = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(load().mcg_data_shape(h, C.byref(n), C.byref(d), C.byref(nnz)))
+        self.n, self.d, self.nnz = n.value, d.value, nnz.value
+
+    @classmethod
+    def dense(cls, X, ctx=None):
+        ctx = ctx or default_context()
+        h = C.c_void_p()
+        if isinstance(X, np.ndarray):
+            if X.dtype not in (np.float32, np.float64):
+                X = X.astype(np.float32)
+            X = np.ascontiguousarray(X)
+            n, d = X.shape
+            check(load().mcg_data_dense(ctx.h, ptr(X), _dtype_code(X.dtype), n, d, d, C.byref(h)))
+            return cls(ctx, h)
+        # torch tensor on the device: borrowed
+        n, d = X.shape
+        code = MCG_F32 if str(X.dtype) == "torch.float32" else MCG_F64
+        check(load().mcg_data_dense(ctx.h, ptr(X), code, n, d, X.stride(0), C.byref(h)))
+        return cls(ctx, h, keep=(X,))
+
+    @classmethod
+    def csr(cls, data, indices, indptr, shape, ctx=None):
+        ctx = ctx or default_context()
+        h = C.c_void_p()
+        data = np.ascontiguousarray(data)
+        indices = np.ascontiguousarray(indices)
+        indptr = np.ascontiguousarray(indptr)
+        n, d = int(shape[0]), int(shape[1])
+        check(load().mcg_data_csr(ctx.h, ptr(data), _dtype_code(data.dtype), ptr(indices),
+                                  _dtype_code(indices.dtype), ptr(indptr), _dtype_code(indptr.dtype),
+                                  n, d, len(data), C.byref(h)))
+        return cls(ctx, h)
+
+    def close(self):
+        if self.h:
+            load().mcg_data_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- operations (host numpy in/out) ----
+    def cost_matrix(self, msr, centers, prior=0.):
+        centers = np.ascontiguousarray(centers, dtype=np.float32)
+        out = np.empty((self.n, centers.shape[0]), dtype=np.float32)
+        check(load().mcg_cost_matrix(self.h, msr, prior, ptr(centers), centers.shape[0], ptr(out)))
+        return out
+
+    def assign(self, msr, centers, prior=0., precision=PREC_BF16, with_costs=True):
+        centers = np.ascontiguousarray(centers, dtype=np.float32)
+        asn = np.empty(self.n, dtype=np.uint32)
+        costs = np.empty(self.n, dtype=np.float32) if with_costs else None
+        check(load().mcg_assign_hard(self.h, msr, prior, ptr(centers), centers.shape[0], precision,
+                                     ptr(asn), ptr(costs)))
+        return asn, costs
+
+    def centroids(self, asn, k, centers_prev, weights=None):
+        asn = np.ascontiguousarray(asn, dtype=np.uint32)
+        C_ = np.ascontiguousarray(centers_prev, dtype=np.float32).copy()
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        ne = C.c_uint64()
+        check(load().mcg_centroids_hard(self.h, ptr(asn), ptr(w), k, ptr(C_), C.byref(ne)))
+        return C_, ne.value
+
+    def lloyd(self, msr, centers, prior=0., weights=None, eps=1e-10, maxiter=100, precision=PREC_BF16):
+        Cc = np.ascontiguousarray(centers, dtype=np.float32).copy()
+        k = Cc.shape[0]
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        asn = np.empty(self.n, dtype=np.uint32)
+        costs = np.empty(self.n, dtype=np.float32)
+        ic, fc, it = C.c_double(), C.c_double(), C.c_uint64()
+        check(load().mcg_lloyd(self.h, msr, prior, ptr(Cc), k, ptr(w), eps, maxiter, precision, ptr(asn),
+                               ptr(costs), C.byref(ic), C.byref(fc), C.byref(it)))
+        return dict(centers=Cc, asn=asn, costs=costs, initcost=ic.value, finalcost=fc.value, numiter=it.value)
+
+    def lloyd_step(self, msr, centers_dev, asn_dev, prior=0., weights=None, precision=PREC_BF16):
+        """One E+M iteration on device tensors (torch) in place; returns the FP64 cost."""
+        cost = C.c_double()
+        check(load().mcg_lloyd_step(self.h, msr, prior, ptr(centers_dev), centers_dev.shape[0], ptr(weights),
+                                    precision, ptr(asn_dev), C.byref(cost)))
+        return cost.value
+
+    def soft(self, msr, centers, prior=0., temp=1., weights=None, eps=1e-8, maxiter=1000):
+        Cc = np.ascontiguousarray(centers, dtype=np.float32).copy()
+        k = Cc.shape[0]
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        costs = np.empty((self.n, k), dtype=np.float32)
+        resp = np.empty((self.n, k), dtype=np.float32)
+        ic, fc, it = C.c_double(), C.c_double(), C.c_uint64()
+        check(load().mcg_soft_em(self.h, msr, prior, temp, ptr(Cc), k, ptr(w), eps, maxiter, ptr(costs),
+                                 ptr(resp), C.byref(ic), C.byref(fc), C.byref(it)))
+        return dict(centers=Cc, costs=costs, asn=resp, initcost=ic.value, finalcost=fc.value, numiter=it.value)
+
+    def kmeanspp(self, msr, k, prior=0., seed=0, weights=None):
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        kk = min(k, self.n)
+        centers = np.empty(kk, dtype=np.uint32)
+        asn = np.empty(self.n, dtype=np.uint32)
+        costs = np.empty(self.n, dtype=np.float32)
+        check(load().mcg_kmeanspp(self.h, msr, prior, k, seed, ptr(w), ptr(centers), ptr(asn), ptr(costs)))
+        return centers, asn, costs
+
+
+def coreset_probs(sens, ncenters, costs, asn, weights=None, ctx=None):
+    ctx = ctx or default_context()
+    costs = np.ascontiguousarray(costs, dtype=np.float64)
+    asn = np.ascontiguousarray(asn, dtype=np.uint32)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    probs = np.empty(len(costs), dtype=np.float32)
+    check(load().mcg_coreset_probs(ctx.h, sens, len(costs), ncenters, ptr(costs), ptr(asn), ptr(w), ptr(probs)))
+    return probs
+
+
+def coreset_sample(probs, m, seed, weights=None, first_draw=0, ctx=None):
+    ctx = ctx or default_context()
+    probs = np.ascontiguousarray(probs, dtype=np.float32)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    idx = np.empty(max(m, 1), dtype=np.uint64)
+    wt = np.empty(max(m, 1), dtype=np.float32)
+    check(load().mcg_coreset_sample(ctx.h, len(probs), ptr(probs), ptr(w), m, seed, first_draw, ptr(idx), ptr(wt)))
+    return wt[:m], idx[:m]

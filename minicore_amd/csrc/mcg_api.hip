@@ -1,0 +1,869 @@
+// C ABI of libmcg: contexts, datasets, and the host-side drivers of the
+// clustering hot path (Lloyd, soft EM, seeding, coresets).  Kernels live in
+// the other translation units.
+#include "mcg_internal.h"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <memory>
+
+namespace mcg {
+
+static thread_local std::string g_err;
+void set_error(const std::string &m) { g_err = m; }
+
+template <class F>
+static int guard(F &&f) {
+    try {
+        f();
+        return MCG_OK;
+    } catch (const Error &e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed");
+        return MCG_ENOMEM;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return MCG_EDEVICE;
+    }
+}
+
+bool is_device_ptr(const void *p) {
+    if (!p) return false;
+    hipPointerAttribute_t a;
+    hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+void Ctx::tic(int which, hipEvent_t &a) {
+    a = nullptr;
+    if (!timing) return;
+    (void)which;
+    HIP_CHECK(hipEventCreate(&a));
+    HIP_CHECK(hipEventRecord(a, stream));
+}
+
+void Ctx::toc(int which, hipEvent_t a) {
+    if (!timing || !a) return;
+    hipEvent_t b;
+    HIP_CHECK(hipEventCreate(&b));
+    HIP_CHECK(hipEventRecord(b, stream));
+    pending[which].push_back({a, b});
+}
+
+void Ctx::collect() {
+    for (int w = 0; w < 2; ++w) {
+        for (auto &pr : pending[w]) {
+            HIP_CHECK(hipEventSynchronize(pr.second));
+            float ms = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&ms, pr.first, pr.second));
+            timed_ms[w] += ms;
+            timed_n[w] += 1;
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+        pending[w].clear();
+    }
+}
+
+Ctx::~Ctx() {
+    for (int w = 0; w < 2; ++w)
+        for (auto &pr : pending[w]) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+    if (stream) (void)hipStreamDestroy(stream);
+    if (host_pinned) (void)hipHostFree(host_pinned);
+}
+
+void *Ctx::pinned(size_t nb) {
+    if (nb > host_pinned_bytes) {
+        if (host_pinned) (void)hipHostFree(host_pinned);
+        host_pinned = nullptr;
+        HIP_CHECK(hipHostMalloc(&host_pinned, nb));
+        host_pinned_bytes = nb;
+    }
+    return host_pinned;
+}
+
+// Device view of a caller buffer (borrow device memory, stage host memory).
+static const void *in_dev(Ctx &c, const void *p, size_t bytes, DBuf &stage) {
+    if (!p) return nullptr;
+    if (is_device_ptr(p)) return p;
+    stage.reserve(bytes);
+    HIP_CHECK(hipMemcpyAsync(stage.p, p, bytes, hipMemcpyHostToDevice, c.stream));
+    return stage.p;
+}
+
+// Output target: device pointer used directly; host pointer gets a device
+// scratch that out_finish() copies back.
+struct Out {
+    void *user = nullptr;
+    void *dev = nullptr;
+    size_t bytes = 0;
+    bool host = false;
+};
+static Out out_dev(Ctx &, void *p, size_t bytes, DBuf &stage) {
+    Out o;
+    o.user = p;
+    o.bytes = bytes;
+    if (!p) return o;
+    if (is_device_ptr(p)) { o.dev = p; return o; }
+    o.host = true;
+    o.dev = stage.reserve(bytes);
+    return o;
+}
+static void out_finish(Ctx &c, Out &o) {
+    if (o.host && o.user) HIP_CHECK(hipMemcpyAsync(o.user, o.dev, o.bytes, hipMemcpyDeviceToHost, c.stream));
+}
+
+static void sync(Ctx &c) {
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    c.collect();
+}
+
+// ---------------------------------------------------------------------------
+// element-type conversion on the host
+// ---------------------------------------------------------------------------
+template <class T>
+static double rd(const void *p, uint64_t i) { return (double)((const T *)p)[i]; }
+static double read_num(const void *p, int dt, uint64_t i) {
+    switch (dt) {
+    case MCG_F32: return rd<float>(p, i);
+    case MCG_F64: return rd<double>(p, i);
+    case MCG_U8: return rd<uint8_t>(p, i);
+    case MCG_U16: return rd<uint16_t>(p, i);
+    case MCG_U32: return rd<uint32_t>(p, i);
+    case MCG_U64: return rd<uint64_t>(p, i);
+    case MCG_I32: return rd<int32_t>(p, i);
+    case MCG_I64: return rd<int64_t>(p, i);
+    }
+    MCG_THROW(MCG_EINVAL, "unknown dtype");
+}
+static uint64_t read_idx(const void *p, int dt, uint64_t i) {
+    switch (dt) {
+    case MCG_U16: return ((const uint16_t *)p)[i];
+    case MCG_U32: return ((const uint32_t *)p)[i];
+    case MCG_U64: return ((const uint64_t *)p)[i];
+    case MCG_I32: { int32_t v = ((const int32_t *)p)[i]; REQUIRE(v >= 0, "negative index"); return (uint64_t)v; }
+    case MCG_I64: { int64_t v = ((const int64_t *)p)[i]; REQUIRE(v >= 0, "negative index"); return (uint64_t)v; }
+    }
+    MCG_THROW(MCG_EINVAL, "unsupported index dtype");
+}
+static size_t dt_size(int dt) {
+    switch (dt) {
+    case MCG_F32: case MCG_U32: case MCG_I32: return 4;
+    case MCG_F64: case MCG_U64: case MCG_I64: return 8;
+    case MCG_U16: return 2;
+    case MCG_U8: return 1;
+    }
+    MCG_THROW(MCG_EINVAL, "unknown dtype");
+}
+
+__global__ void k_f64_to_f32(const double *__restrict__ in, uint64_t n, uint64_t d, uint64_t ld,
+                             float *__restrict__ out) {
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n * d; t += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t i = t / d, j = t % d;
+        out[t] = (float)in[i * ld + j];
+    }
+}
+
+// per-row FP64 sums of the stored values
+__global__ void k_rowsum_dense(const float *__restrict__ X, uint64_t n, uint32_t d, double *__restrict__ out) {
+    const uint64_t row = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    double s = 0.;
+    for (uint32_t j = lane; j < d; j += 64) s += (double)X[row * d + j];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) out[row] = s;
+}
+__global__ void k_rowsum_csr(const float *__restrict__ val, const uint64_t *__restrict__ ptr, uint64_t n,
+                             double *__restrict__ out) {
+    const uint64_t row = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    double s = 0.;
+    for (uint64_t p = ptr[row] + lane; p < ptr[row + 1]; p += 64) s += (double)val[p];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) out[row] = s;
+}
+
+void rowsums(Data &ds, hipStream_t st) {
+    if (ds.rowsum.p) return;
+    ds.rowsum.reserve(ds.n * sizeof(double));
+    if (ds.csr)
+        hipLaunchKernelGGL(k_rowsum_csr, dim3((uint32_t)cdiv(ds.n, 4)), dim3(256), 0, st, ds.val.as<const float>(),
+                           ds.ptr.as<const uint64_t>(), ds.n, ds.rowsum.as<double>());
+    else
+        hipLaunchKernelGGL(k_rowsum_dense, dim3((uint32_t)cdiv(ds.n, 4)), dim3(256), 0, st, ds.X, ds.n,
+                           (uint32_t)ds.d, ds.rowsum.as<double>());
+}
+
+// weighted FP64 total of per-point costs (fixed order)
+__global__ void k_wsum(const float *__restrict__ cost, const double *__restrict__ w, uint64_t n,
+                       double *__restrict__ out) {
+    __shared__ double sm[1024];
+    double s = 0.;
+    const uint64_t per = (n + 1023) / 1024;
+    const uint64_t b = threadIdx.x * per, e = min<uint64_t>(n, b + per);
+    for (uint64_t i = b; i < e; ++i) s += w ? w[i] * (double)cost[i] : (double)cost[i];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sm[0];
+}
+
+__global__ void k_wsum_d(const double *__restrict__ v, uint64_t n, double *__restrict__ out) {
+    __shared__ double sm[1024];
+    double s = 0.;
+    const uint64_t per = (n + 1023) / 1024;
+    const uint64_t b = threadIdx.x * per, e = min<uint64_t>(n, b + per);
+    for (uint64_t i = b; i < e; ++i) s += v[i];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sm[0];
+}
+
+__global__ void k_any_nan(const float *__restrict__ p, uint64_t n, uint32_t *__restrict__ flag) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        if (p[i] != p[i]) *flag = 1u;
+}
+
+bool any_nan(const float *p, uint64_t n, hipStream_t st, Ctx &c) {
+    uint32_t *f = (uint32_t *)c.s[S_MISC3].reserve(sizeof(uint32_t));
+    fill_u32(f, 0, 1, st);
+    hipLaunchKernelGGL(k_any_nan, dim3((uint32_t)std::min<uint64_t>(cdiv(n, 256), 1024)), dim3(256), 0, st, p, n, f);
+    uint32_t h = 0;
+    HIP_CHECK(hipMemcpyAsync(&h, f, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    return h != 0;
+}
+
+// copy data rows into center slots: C[ids[q]] = row rows[q]
+__global__ void k_rows_to_centers_dense(const float *__restrict__ X, uint32_t d, const uint64_t *__restrict__ rows,
+                                        const uint32_t *__restrict__ ids, uint64_t m, float *__restrict__ C) {
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < m * d; t += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t q = t / d, j = t % d;
+        C[(uint64_t)ids[q] * d + j] = X[rows[q] * d + j];
+    }
+}
+__global__ void k_rows_to_centers_csr(const float *__restrict__ val, const uint32_t *__restrict__ idx,
+                                      const uint64_t *__restrict__ ptr, uint32_t d,
+                                      const uint64_t *__restrict__ rows, const uint32_t *__restrict__ ids, uint64_t m,
+                                      float *__restrict__ C) {
+    const uint64_t q = blockIdx.x;
+    if (q >= m) return;
+    float *c = C + (uint64_t)ids[q] * d;
+    for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) c[j] = 0.f;
+    __syncthreads();
+    for (uint64_t p = ptr[rows[q]] + threadIdx.x; p < ptr[rows[q] + 1]; p += blockDim.x) c[idx[p]] = val[p];
+}
+
+void copy_rows_to_centers(Data &ds, const uint64_t *rows_h, const uint32_t *ids_h, uint64_t m, float *C,
+                          hipStream_t st) {
+    if (!m) return;
+    Ctx &c = *ds.ctx;
+    char *buf = (char *)c.s[S_MISC2].reserve(m * (sizeof(uint64_t) + sizeof(uint32_t)));
+    HIP_CHECK(hipMemcpyAsync(buf, rows_h, m * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(buf + m * sizeof(uint64_t), ids_h, m * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    const uint64_t *rows = (const uint64_t *)buf;
+    const uint32_t *ids = (const uint32_t *)(buf + m * sizeof(uint64_t));
+    if (ds.csr)
+        hipLaunchKernelGGL(k_rows_to_centers_csr, dim3((uint32_t)m), dim3(256), 0, st, ds.val.as<const float>(),
+                           ds.idx.as<const uint32_t>(), ds.ptr.as<const uint64_t>(), (uint32_t)ds.d, rows, ids, m, C);
+    else
+        hipLaunchKernelGGL(k_rows_to_centers_dense, dim3((uint32_t)std::min<uint64_t>(cdiv(m * ds.d, 256), 4096)),
+                           dim3(256), 0, st, ds.X, (uint32_t)ds.d, rows, ids, m, C);
+    HIP_CHECK(hipStreamSynchronize(st));  // the host arrays may go away
+}
+
+__global__ void k_force(uint32_t *__restrict__ asn, float *__restrict__ cost, const uint64_t *__restrict__ rows,
+                        const uint32_t *__restrict__ ids, uint64_t m) {
+    uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    const uint64_t r = rows[q];
+    if (asn[r] != ids[q]) { asn[r] = ids[q]; if (cost) cost[r] = 0.f; }
+}
+
+// ---------------------------------------------------------------------------
+// E-step dispatch: asn (device) and per-point best costs (device, may be null
+// for dense SQRL2 where the M-step measures the cost).
+// ---------------------------------------------------------------------------
+static bool dense_l2(const Data &ds, int msr) { return !ds.csr && msr == MCG_SQRL2; }
+
+static void estep(Data &ds, int msr, double prior, const float *C, uint64_t k, int precision, uint32_t *asn,
+                  float *best, hipStream_t st) {
+    if (dense_l2(ds, msr)) {
+        if (best) dense_sqrl2_assign_exact(ds, C, k, asn, best, st);
+        else dense_sqrl2_assign(ds, C, k, precision, asn, st);
+        return;
+    }
+    generic_costs(ds, msr, prior, C, k, nullptr, asn, best, st);
+}
+
+static void check_msr(int msr) {
+    REQUIRE(msr == MCG_SQRL2 || msr == MCG_JSD || msr == MCG_MKL || msr == MCG_REVERSE_MKL,
+            "measure must be SQRL2 (2), JSD (4), MKL (5) or REVERSE_MKL (11)");
+}
+
+}  // namespace mcg
+
+using namespace mcg;
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *mcg_last_error(void) { return g_err.c_str(); }
+
+int mcg_device_count(int *count) {
+    return guard([&] {
+        REQUIRE(count, "null count");
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess) { (void)hipGetLastError(); n = 0; }
+        *count = n;
+    });
+}
+
+int mcg_ctx_create(int device, mcg_ctx **out) {
+    return guard([&] {
+        REQUIRE(out, "null out");
+        int n = 0;
+        HIP_CHECK(hipGetDeviceCount(&n));
+        REQUIRE(device >= 0 && device < n, "invalid device ordinal");
+        HIP_CHECK(hipSetDevice(device));
+        std::unique_ptr<mcg_ctx> c(new mcg_ctx);
+        c->c.device = device;
+        HIP_CHECK(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
+        *out = c.release();
+    });
+}
+
+int mcg_ctx_destroy(mcg_ctx *ctx) {
+    return guard([&] {
+        if (!ctx) return;
+        (void)hipSetDevice(ctx->c.device);
+        (void)hipStreamSynchronize(ctx->c.stream);
+        delete ctx;
+    });
+}
+
+int mcg_ctx_synchronize(mcg_ctx *ctx) {
+    return guard([&] { REQUIRE(ctx, "null ctx"); HIP_CHECK(hipStreamSynchronize(ctx->c.stream)); });
+}
+
+int mcg_ctx_stream(mcg_ctx *ctx, void **stream) {
+    return guard([&] { REQUIRE(ctx && stream, "null argument"); *stream = (void *)ctx->c.stream; });
+}
+
+int mcg_data_dense(mcg_ctx *ctx, const void *x, int dtype, uint64_t n, uint64_t d, uint64_t ld, mcg_data **out) {
+    return guard([&] {
+        REQUIRE(ctx && out && x, "null argument");
+        REQUIRE(n > 0 && d > 0, "empty matrix");
+        REQUIRE(dtype == MCG_F32 || dtype == MCG_F64, "dense data must be float32 or float64");
+        if (ld == 0) ld = d;
+        REQUIRE(ld >= d, "ld < d");
+        Ctx &c = ctx->c;
+        HIP_CHECK(hipSetDevice(c.device));
+        std::unique_ptr<mcg_data> ds(new mcg_data);
+        Data &D = ds->d;
+        D.ctx = &c;
+        D.n = n;
+        D.d = d;
+        D.nnz = n * d;
+        const bool dev = is_device_ptr(x);
+        if (dev && dtype == MCG_F32 && ld == d) {
+            D.X = (float *)x;  // borrowed
+        } else {
+            D.X = (float *)D.Xown.reserve(n * d * sizeof(float));
+            if (dtype == MCG_F32) {
+                HIP_CHECK(hipMemcpy2DAsync(D.X, d * 4, x, ld * 4, d * 4, n,
+                                           dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c.stream));
+            } else {
+                DBuf tmp;
+                const void *src = x;
+                if (!dev) {
+                    tmp.reserve(n * ld * 8);
+                    HIP_CHECK(hipMemcpyAsync(tmp.p, x, n * ld * 8, hipMemcpyHostToDevice, c.stream));
+                    src = tmp.p;
+                }
+                hipLaunchKernelGGL(k_f64_to_f32, dim3((uint32_t)std::min<uint64_t>(cdiv(n * d, 256), 65536)), dim3(256),
+                                   0, c.stream, (const double *)src, n, d, ld, D.X);
+                sync(c);
+            }
+        }
+        sync(c);
+        *out = ds.release();
+    });
+}
+
+int mcg_data_csr(mcg_ctx *ctx, const void *data, int data_dtype, const void *indices, int idx_dtype,
+                 const void *indptr, int ptr_dtype, uint64_t n, uint64_t d, uint64_t nnz, mcg_data **out) {
+    return guard([&] {
+        REQUIRE(ctx && out && indptr, "null argument");
+        REQUIRE(n > 0 && d > 0, "empty matrix");
+        REQUIRE(d < (1ull << 32), "d must fit in uint32");
+        Ctx &c = ctx->c;
+        HIP_CHECK(hipSetDevice(c.device));
+        // bring the three arrays to the host (they are small next to the work)
+        auto host_copy = [&](const void *p, size_t bytes, std::vector<char> &buf) -> const void * {
+            if (!p) return nullptr;
+            if (!is_device_ptr(p)) return p;
+            buf.resize(bytes);
+            HIP_CHECK(hipMemcpy(buf.data(), p, bytes, hipMemcpyDeviceToHost));
+            return buf.data();
+        };
+        std::vector<char> b0, b1, b2;
+        const void *hp = host_copy(indptr, (n + 1) * dt_size(ptr_dtype), b2);
+        const void *hd = host_copy(data, nnz * dt_size(data_dtype), b0);
+        const void *hi = host_copy(indices, nnz * dt_size(idx_dtype), b1);
+        std::vector<uint64_t> P(n + 1);
+        for (uint64_t i = 0; i <= n; ++i) P[i] = read_idx(hp, ptr_dtype, i);
+        REQUIRE(P[0] == 0 && P[n] == nnz, "indptr must start at 0 and end at nnz");
+        for (uint64_t i = 0; i < n; ++i) REQUIRE(P[i] <= P[i + 1], "indptr must be non-decreasing");
+        std::vector<float> V(nnz);
+        std::vector<uint32_t> J(nnz);
+        for (uint64_t p = 0; p < nnz; ++p) {
+            V[p] = (float)read_num(hd, data_dtype, p);
+            uint64_t j = read_idx(hi, idx_dtype, p);
+            REQUIRE(j < d, "column index out of range");
+            J[p] = (uint32_t)j;
+        }
+        for (uint64_t i = 0; i < n; ++i)
+            for (uint64_t p = P[i] + 1; p < P[i + 1]; ++p)
+                REQUIRE(J[p - 1] < J[p], "column indices must be sorted and unique within a row");
+        std::unique_ptr<mcg_data> ds(new mcg_data);
+        Data &D = ds->d;
+        D.ctx = &c;
+        D.csr = true;
+        D.n = n;
+        D.d = d;
+        D.nnz = nnz;
+        D.val.reserve(nnz * 4);
+        D.idx.reserve(nnz * 4);
+        D.ptr.reserve((n + 1) * 8);
+        if (nnz) {
+            HIP_CHECK(hipMemcpy(D.val.p, V.data(), nnz * 4, hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpy(D.idx.p, J.data(), nnz * 4, hipMemcpyHostToDevice));
+        }
+        HIP_CHECK(hipMemcpy(D.ptr.p, P.data(), (n + 1) * 8, hipMemcpyHostToDevice));
+        *out = ds.release();
+    });
+}
+
+int mcg_data_free(mcg_data *ds) {
+    return guard([&] {
+        if (!ds) return;
+        (void)hipStreamSynchronize(ds->d.ctx->stream);
+        delete ds;
+    });
+}
+
+int mcg_data_shape(const mcg_data *ds, uint64_t *n, uint64_t *d, uint64_t *nnz) {
+    return guard([&] {
+        REQUIRE(ds, "null dataset");
+        if (n) *n = ds->d.n;
+        if (d) *d = ds->d.d;
+        if (nnz) *nnz = ds->d.nnz;
+    });
+}
+
+int mcg_cost_matrix(mcg_data *ds, int msr, double prior, const float *centers, uint64_t k, float *out) {
+    return guard([&] {
+        REQUIRE(ds && centers && out, "null argument");
+        REQUIRE(k > 0, "k must be positive");
+        check_msr(msr);
+        Data &D = ds->d;
+        Ctx &c = *D.ctx;
+        HIP_CHECK(hipSetDevice(c.device));
+        DBuf sc, so;
+        const float *C = (const float *)in_dev(c, centers, k * D.d * 4, sc);
+        Out o = out_dev(c, out, D.n * k * 4, so);
+        generic_costs(D, msr, prior, C, k, (float *)o.dev, nullptr, nullptr, c.stream);
+        out_finish(c, o);
+        sync(c);
+    });
+}
+
+int mcg_assign_hard(mcg_data *ds, int msr, double prior, const float *centers, uint64_t k, int precision,
+                    uint32_t *asn, float *costs) {
+    return guard([&] {
+        REQUIRE(ds && centers && asn, "null argument");
+        REQUIRE(k > 0, "k must be positive");
+        check_msr(msr);
+        Data &D = ds->d;
+        Ctx &c = *D.ctx;
+        HIP_CHECK(hipSetDevice(c.device));
+        DBuf sc, sa, sk;
+        const float *C = (const float *)in_dev(c, centers, k * D.d * 4, sc);
+        Out oa = out_dev(c, asn, D.n * 4, sa);
+        Out ok = out_dev(c, costs, D.n * 4, sk);
+        if (dense_l2(D, msr)) {
+            dense_sqrl2_assign(D, C, k, precision, (uint32_t *)oa.dev, c.stream);
+            if (ok.dev) dense_sqrl2_costs(D, C, (const uint32_t *)oa.dev, (float *)ok.dev, c.stream);
+        } else {
+            generic_costs(D, msr, prior, C, k, nullptr, (uint32_t *)oa.dev, (float *)ok.dev, c.stream);
+        }
+        out_finish(c, oa);
+        out_finish(c, ok);
+        sync(c);
+    });
+}
+
+int mcg_centroids_hard(mcg_data *ds, const uint32_t *asn, const double *weights, uint64_t k, float *centers,
+                       uint64_t *n_empty) {
+    return guard([&] {
+        REQUIRE(ds && asn && centers, "null argument");
+        REQUIRE(k > 0, "k must be positive");
+        Data &D = ds->d;
+        Ctx &c = *D.ctx;
+        HIP_CHECK(hipSetDevice(c.device));
+        DBuf sa, sw, sc;
+        {
+            std::vector<uint32_t> h(D.n);
+            if (is_device_ptr(asn)) {
+                HIP_CHECK(hipMemcpy(h.data(), asn, D.n * 4, hipMemcpyDeviceToHost));
+            } else {
+                std::memcpy(h.data(), asn, D.n * 4);
+            }
+            for (uint64_t i = 0; i < D.n; ++i) REQUIRE(h[i] < k, "assignment out of range");
+        }
+        const uint32_t *A = (const uint32_t *)in_dev(c, asn, D.n * 4, sa);
+        const double *W = (const double *)in_dev(c, weights, D.n * 8, sw);
+        float *C;
+        bool host_c = !is_device_ptr(centers);
+        if (host_c) {
+            C = (float *)sc.reserve(k * D.d * 4);
+            HIP_CHECK(hipMemcpyAsync(C, centers, k * D.d * 4, hipMemcpyHostToDevice, c.stream));
+        } else {
+            C = centers;
+        }
+        MStepOut mo = D.csr ? csr_mstep(D, A, W, k, C, c.stream) : dense_mstep(D, A, W, k, C, C, false, c.stream);
+        std::vector<uint32_t> cnt(k);
+        HIP_CHECK(hipMemcpyAsync(cnt.data(), mo.counts, k * 4, hipMemcpyDeviceToHost, c.stream));
+        if (host_c) HIP_CHECK(hipMemcpyAsync(centers, C, k * D.d * 4, hipMemcpyDeviceToHost, c.stream));
+        sync(c);
+        if (n_empty) *n_empty = (uint64_t)std::count(cnt.begin(), cnt.end(), 0u);
+    });
+}
+
+// perform_hard_clustering (solve.h:98-175).  For dense SQRL2 the M-step pass also
+// measures cost(asn, C) in FP64; elsewhere the E-step's per-point minima are summed.
+int mcg_lloyd(mcg_data *ds, int msr, double prior, float *centers, uint64_t k, const double *weights, double eps,
+              uint64_t maxiter, int precision, uint32_t *asn, float *costs, double *initcost, double *finalcost,
+              uint64_t *numiter) {
+    return guard([&] {
+        REQUIRE(ds && centers && asn, "null argument");
+        REQUIRE(k > 0, "k must be positive");
+        check_msr(msr);
+        Data &D = ds->d;
+        Ctx &c = *D.ctx;
+        hipStream_t st = c.stream;
+        HIP_CHECK(hipSetDevice(c.device));
+        const uint64_t n = D.n, d = D.d;
+        const bool l2 = dense_l2(D, msr);
+        DBuf sw, sa_out, sk_out;
+        const double *W = (const double *)in_dev(c, weights, n * 8, sw);
+        // device state: accepted centers C, candidate Cn, next means Cm; two assignment buffers
+        float *C = (float *)c.s[S_CENT_A].reserve(k * d * 4);
+        float *Cn = (float *)c.s[S_CENT_B].reserve(k * d * 4);
+        float *Cm = (float *)c.s[S_CENT_C].reserve(k * d * 4);
+        uint32_t *A = (uint32_t *)c.s[S_ASN_A].reserve(n * 4);
+        uint32_t *An = (uint32_t *)c.s[S_ASN_B].reserve(n * 4);
+        float *pc = (float *)c.s[S_COSTS].reserve(2 * n * 4);  // per-point costs for A / An
+        float *pcn = pc + n;
+        double *tot = (double *)c.s[S_TOT].reserve(4 * sizeof(double));
+        HIP_CHECK(hipMemcpyAsync(C, centers, k * d * 4, is_device_ptr(centers) ? hipMemcpyDeviceToDevice
+                                                                               : hipMemcpyHostToDevice, st));
+        std::vector<uint32_t> cnt(k);
+        // Measure cost(asn, Cref) and the means of asn into Cmeans.  Returns FP64 cost.
+        auto mpass = [&](const uint32_t *a, const float *cost_pts, const float *Cref, float *Cmeans) -> double {
+            double cost;
+            HIP_CHECK(hipMemcpyAsync(Cmeans, Cref, k * d * 4, hipMemcpyDeviceToDevice, st));
+            MStepOut mo;
+            if (l2) {
+                mo = dense_mstep(D, a, W, k, Cref, Cmeans, true, st);
+                HIP_CHECK(hipMemcpyAsync(&cost, mo.total, sizeof(double), hipMemcpyDeviceToHost, st));
+            } else {
+                hipLaunchKernelGGL(k_wsum, dim3(1), dim3(1024), 0, st, cost_pts, W, n, tot);
+                mo = D.csr ? csr_mstep(D, a, W, k, Cmeans, st) : dense_mstep(D, a, W, k, Cref, Cmeans, false, st);
+                HIP_CHECK(hipMemcpyAsync(&cost, tot, sizeof(double), hipMemcpyDeviceToHost, st));
+            }
+            HIP_CHECK(hipMemcpyAsync(cnt.data(), mo.counts, k * 4, hipMemcpyDeviceToHost, st));
+            sync(c);
+            return cost;
+        };
+        auto do_estep = [&](const float *Cx, uint32_t *a, float *cp) {
+            estep(D, msr, prior, Cx, k, precision, a, l2 ? nullptr : cp, st);
+        };
+        // Centers from assignment a (w.r.t. Cprev) with the empty-cluster restart
+        // of centroid.h:687-752.  Cm already holds the plain means of a.
+        auto next_centers = [&](uint32_t *a, float *cp, const float *Cprev, float *out) -> bool {
+            const uint64_t ne = (uint64_t)std::count(cnt.begin(), cnt.end(), 0u);
+            if (ne == 0) {
+                HIP_CHECK(hipMemcpyAsync(out, Cm, k * d * 4, hipMemcpyDeviceToDevice, st));
+                return false;
+            }
+            if (l2) dense_sqrl2_costs(D, Cprev, a, cp, st);
+            std::vector<uint64_t> rows(ne);
+            std::vector<uint32_t> ids;
+            for (uint64_t q = 0; q < k; ++q) if (!cnt[q]) ids.push_back((uint32_t)q);
+            restart_draws(c, cp, n, ne, rows.data(), st);
+            HIP_CHECK(hipMemcpyAsync(out, Cprev, k * d * 4, hipMemcpyDeviceToDevice, st));
+            copy_rows_to_centers(D, rows.data(), ids.data(), ne, out, st);
+            // full re-assignment against the mixed centers, then force the drawn rows
+            if (l2) dense_sqrl2_assign_exact(D, out, k, a, cp, st);
+            else generic_costs(D, msr, prior, out, k, nullptr, a, cp, st);
+            {
+                char *buf = (char *)c.s[S_MISC2].reserve(ne * 12);
+                HIP_CHECK(hipMemcpyAsync(buf, rows.data(), ne * 8, hipMemcpyHostToDevice, st));
+                HIP_CHECK(hipMemcpyAsync(buf + ne * 8, ids.data(), ne * 4, hipMemcpyHostToDevice, st));
+                hipLaunchKernelGGL(k_force, dim3((uint32_t)cdiv(ne, 256)), dim3(256), 0, st, a, cp,
+                                   (const uint64_t *)buf, (const uint32_t *)(buf + ne * 8), ne);
+            }
+            MStepOut mo = D.csr ? csr_mstep(D, a, W, k, out, st) : dense_mstep(D, a, W, k, out, out, false, st);
+            (void)mo;
+            sync(c);
+            return true;
+        };
+
+        do_estep(C, A, pc);
+        double cost64 = mpass(A, pc, C, Cm);
+        float cost = (float)cost64;
+        const double init = cost;
+        uint64_t iter = 0;
+        if (cost == 0.f) {
+            *initcost = 0.;
+            *finalcost = 0.;
+            *numiter = 0;
+        } else {
+            for (;;) {
+                const bool res = next_centers(A, pc, C, Cn);
+                if (any_nan(Cn, k * d, st, c)) MCG_THROW(MCG_ENAN, "Unexpected nan in ctr");
+                do_estep(Cn, An, pcn);
+                const float newcost = (float)mpass(An, pcn, Cn, Cm);
+                if (newcost > cost && !res) break;  // keep C and A (solve.h:160-164)
+                std::swap(C, Cn);
+                std::swap(A, An);
+                std::swap(pc, pcn);
+                ++iter;
+                const float old = cost;
+                cost = newcost;
+                const float diff = old - newcost;
+                if ((double)diff < eps * std::max((double)newcost, (double)old) || iter > maxiter) break;
+            }
+            *initcost = init;
+            *finalcost = cost;
+            *numiter = iter;
+        }
+        // outputs
+        Out oa = out_dev(c, asn, n * 4, sa_out);
+        Out ok = out_dev(c, costs, n * 4, sk_out);
+        HIP_CHECK(hipMemcpyAsync(oa.dev, A, n * 4, hipMemcpyDeviceToDevice, st));
+        if (ok.dev) {
+            if (l2) dense_sqrl2_costs(D, C, A, (float *)ok.dev, st);
+            else HIP_CHECK(hipMemcpyAsync(ok.dev, pc, n * 4, hipMemcpyDeviceToDevice, st));
+        }
+        out_finish(c, oa);
+        out_finish(c, ok);
+        HIP_CHECK(hipMemcpyAsync(centers, C, k * d * 4, is_device_ptr(centers) ? hipMemcpyDeviceToDevice
+                                                                               : hipMemcpyDeviceToHost, st));
+        sync(c);
+        // keep the scratch slots consistent for the next call
+        if (C != c.s[S_CENT_A].as<float>()) { /* buffers are swapped pointers into fixed slots; nothing to do */ }
+    });
+}
+
+// perform_soft_clustering (solve.h:287-336) with set_centroids_soft (:342-374).
+int mcg_soft_em(mcg_data *ds, int msr, double prior, double temp, float *centers, uint64_t k, const double *weights,
+                double eps, uint64_t maxiter, float *costs, float *resp, double *initcost, double *finalcost,
+                uint64_t *numiter) {
+    return guard([&] {
+        REQUIRE(ds && centers, "null argument");
+        REQUIRE(k > 0, "k must be positive");
+        check_msr(msr);
+        Data &D = ds->d;
+        Ctx &c = *D.ctx;
+        hipStream_t st = c.stream;
+        HIP_CHECK(hipSetDevice(c.device));
+        const uint64_t n = D.n, d = D.d;
+        DBuf sw, so1, so2;
+        const double *W = (const double *)in_dev(c, weights, n * 8, sw);
+        float *C = (float *)c.s[S_CENT_A].reserve(k * d * 4);
+        float *Cc = (float *)c.s[S_CENT_B].reserve(k * d * 4);
+        float *K = (float *)c.s[S_STAGE0].reserve(n * k * 4);
+        float *R = (float *)c.s[S_STAGE1].reserve(n * k * 4);
+        double *rowdot = (double *)c.s[S_W].reserve(n * 8);
+        double *tot = (double *)c.s[S_TOT].reserve(4 * sizeof(double));
+        HIP_CHECK(hipMemcpyAsync(C, centers, k * d * 4, is_device_ptr(centers) ? hipMemcpyDeviceToDevice
+                                                                               : hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(Cc, C, k * d * 4, hipMemcpyDeviceToDevice, st));
+        generic_costs(D, msr, prior, Cc, k, K, nullptr, nullptr, st);
+        double cost = 1.7976931348623157e308, init = -1.;
+        uint64_t iter = 0;
+        for (;;) {
+            const double oldcost = cost;
+            softmax_rows(K, n, k, (float)temp, R, W, rowdot, st);
+            hipLaunchKernelGGL(k_wsum_d, dim3(1), dim3(1024), 0, st, rowdot, n, tot);
+            soft_mstep(D, R, W, k, Cc, st);
+            generic_costs(D, msr, prior, Cc, k, K, nullptr, nullptr, st);
+            HIP_CHECK(hipMemcpyAsync(&cost, tot, sizeof(double), hipMemcpyDeviceToHost, st));
+            sync(c);
+            if (init < 0.) init = cost;
+            if (oldcost >= cost) HIP_CHECK(hipMemcpyAsync(C, Cc, k * d * 4, hipMemcpyDeviceToDevice, st));
+            if (oldcost - cost <= eps * std::max(oldcost, cost) || ++iter == maxiter) break;
+        }
+        Out ok = out_dev(c, costs, n * k * 4, so1);
+        Out orr = out_dev(c, resp, n * k * 4, so2);
+        if (ok.dev) HIP_CHECK(hipMemcpyAsync(ok.dev, K, n * k * 4, hipMemcpyDeviceToDevice, st));
+        if (orr.dev) HIP_CHECK(hipMemcpyAsync(orr.dev, R, n * k * 4, hipMemcpyDeviceToDevice, st));
+        out_finish(c, ok);
+        out_finish(c, orr);
+        HIP_CHECK(hipMemcpyAsync(centers, C, k * d * 4, is_device_ptr(centers) ? hipMemcpyDeviceToDevice
+                                                                               : hipMemcpyDeviceToHost, st));
+        sync(c);
+        if (initcost) *initcost = init;
+        if (finalcost) *finalcost = cost;
+        if (numiter) *numiter = iter;
+    });
+}
+
+int mcg_kmeanspp(mcg_data *ds, int msr, double prior, uint64_t k, uint64_t seed, const double *weights,
+                 uint32_t *centers, uint32_t *asn, float *costs) {
+    return guard([&] {
+        REQUIRE(ds && centers, "null argument");
+        REQUIRE(k > 0, "k must be positive");
+        check_msr(msr);
+        Data &D = ds->d;
+        Ctx &c = *D.ctx;
+        hipStream_t st = c.stream;
+        HIP_CHECK(hipSetDevice(c.device));
+        const uint64_t n = D.n;
+        if (k > n) k = n;  // kmeans.h:59-62
+        DBuf sw, sa, sk;
+        const double *W = (const double *)in_dev(c, weights, n * 8, sw);
+        Out oa = out_dev(c, asn, n * 4, sa);
+        Out ok = out_dev(c, costs, n * 4, sk);
+        uint32_t *A = oa.dev ? (uint32_t *)oa.dev : (uint32_t *)c.s[S_ASN_A].reserve(n * 4);
+        float *Dd = ok.dev ? (float *)ok.dev : (float *)c.s[S_COSTS].reserve(n * 4);
+        std::vector<uint32_t> ch(k);
+        kmeanspp_run(D, msr, prior, k, seed, W, ch.data(), A, Dd, st);
+        if (is_device_ptr(centers)) HIP_CHECK(hipMemcpyAsync(centers, ch.data(), k * 4, hipMemcpyHostToDevice, st));
+        else std::memcpy(centers, ch.data(), k * 4);
+        out_finish(c, oa);
+        out_finish(c, ok);
+        sync(c);
+    });
+}
+
+int mcg_coreset_probs(mcg_ctx *ctx, int sens, uint64_t n, uint64_t ncenters, const double *costs,
+                      const uint32_t *asn, const double *weights, float *probs) {
+    return guard([&] {
+        REQUIRE(ctx && costs && asn && probs, "null argument");
+        REQUIRE(n > 0 && ncenters > 0, "empty input");
+        Ctx &c = ctx->c;
+        HIP_CHECK(hipSetDevice(c.device));
+        DBuf s1, s2, s3, s4;
+        const double *K = (const double *)in_dev(c, costs, n * 8, s1);
+        const uint32_t *A = (const uint32_t *)in_dev(c, asn, n * 4, s2);
+        const double *W = (const double *)in_dev(c, weights, n * 8, s3);
+        {
+            std::vector<uint32_t> h(n);
+            HIP_CHECK(hipMemcpyAsync(h.data(), A, n * 4, hipMemcpyDeviceToHost, c.stream));
+            sync(c);
+            for (uint64_t i = 0; i < n; ++i) REQUIRE(h[i] < ncenters, "assignment out of range");
+        }
+        Out o = out_dev(c, probs, n * 4, s4);
+        coreset_probs_run(c, sens, n, ncenters, K, A, W, (float *)o.dev, c.stream);
+        out_finish(c, o);
+        sync(c);
+    });
+}
+
+int mcg_coreset_sample(mcg_ctx *ctx, uint64_t n, const float *probs, const double *weights, uint64_t m,
+                       uint64_t seed, uint64_t first_draw, uint64_t *indices, float *out_weights) {
+    return guard([&] {
+        REQUIRE(ctx && probs && indices && out_weights, "null argument");
+        REQUIRE(n > 0, "empty input");
+        Ctx &c = ctx->c;
+        HIP_CHECK(hipSetDevice(c.device));
+        DBuf s1, s2, s3, s4;
+        const float *P = (const float *)in_dev(c, probs, n * 4, s1);
+        const double *W = (const double *)in_dev(c, weights, n * 8, s2);
+        Out oi = out_dev(c, indices, (m ? m : 1) * 8, s3);
+        Out ow = out_dev(c, out_weights, (m ? m : 1) * 4, s4);
+        coreset_sample_from(c, n, P, W, m, seed, first_draw, (uint64_t *)oi.dev, (float *)ow.dev, c.stream);
+        out_finish(c, oi);
+        out_finish(c, ow);
+        sync(c);
+    });
+}
+
+int mcg_timing_enable(mcg_ctx *ctx, int on) {
+    return guard([&] { REQUIRE(ctx, "null ctx"); ctx->c.timing = on != 0; });
+}
+
+int mcg_timing_reset(mcg_ctx *ctx) {
+    return guard([&] {
+        REQUIRE(ctx, "null ctx");
+        sync(ctx->c);
+        for (int w = 0; w < 2; ++w) { ctx->c.timed_ms[w] = 0.; ctx->c.timed_n[w] = 0; }
+    });
+}
+
+int mcg_timing_read(mcg_ctx *ctx, int which, double *total_ms, uint64_t *count) {
+    return guard([&] {
+        REQUIRE(ctx && (which == 0 || which == 1), "invalid argument");
+        sync(ctx->c);
+        if (total_ms) *total_ms = ctx->c.timed_ms[which];
+        if (count) *count = ctx->c.timed_n[which];
+    });
+}
+
+// lloyd_iteration (kmeans.h:359-479): one E-step and one M-step on device state.
+int mcg_lloyd_step(mcg_data *ds, int msr, double prior, float *centers, uint64_t k, const double *weights,
+                   int precision, uint32_t *asn, double *cost) {
+    return guard([&] {
+        REQUIRE(ds && centers && asn, "null argument");
+        REQUIRE(k > 0, "k must be positive");
+        check_msr(msr);
+        REQUIRE(is_device_ptr(centers) && is_device_ptr(asn), "mcg_lloyd_step needs device centers and asn");
+        Data &D = ds->d;
+        Ctx &c = *D.ctx;
+        hipStream_t st = c.stream;
+        HIP_CHECK(hipSetDevice(c.device));
+        DBuf sw;
+        const double *W = (const double *)in_dev(c, weights, D.n * 8, sw);
+        double h = 0.;
+        if (dense_l2(D, msr)) {
+            dense_sqrl2_assign(D, centers, k, precision, asn, st);
+            MStepOut mo = dense_mstep(D, asn, W, k, centers, centers, true, st);
+            HIP_CHECK(hipMemcpyAsync(&h, mo.total, sizeof(double), hipMemcpyDeviceToHost, st));
+        } else {
+            float *pc = (float *)c.s[S_COSTS].reserve(D.n * 4);
+            double *tot = (double *)c.s[S_TOT].reserve(4 * sizeof(double));
+            generic_costs(D, msr, prior, centers, k, nullptr, asn, pc, st);
+            hipLaunchKernelGGL(k_wsum, dim3(1), dim3(1024), 0, st, pc, W, D.n, tot);
+            if (D.csr) csr_mstep(D, asn, W, k, centers, st);
+            else dense_mstep(D, asn, W, k, centers, centers, false, st);
+            HIP_CHECK(hipMemcpyAsync(&h, tot, sizeof(double), hipMemcpyDeviceToHost, st));
+        }
+        sync(c);
+        if (cost) *cost = h;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,591 @@
+// Dense squared-L2 hard assignment for gfx950.
+//
+// Replaces the n x k loop of msr_with_prior(SQRL2) calls in assign_points_hard
+// (include/minicore/clustering/solve.h:234-285; per pair
+// include/minicore/dist/applicator.h:1680-1685).
+//
+// Three forms:
+//  * BF16: ||x-c||^2 = ||x-mu||^2 + ||c-mu||^2 - 2 (x-mu).(c-mu) with the cross term on
+//    v_mfma_f32_32x32x16_bf16 (FP32 accumulate).  Points are the MFMA B operand and stay
+//    in registers for the whole pass over the centers; centers stream through LDS in
+//    tiles of 64.  Each lane keeps a running best/second of its column with the center's
+//    tile-local position packed into the low 6 mantissa bits (3 VALU ops per value).
+//    Points whose best/second gap is not provably larger than the BF16 error bound are
+//    re-scored exactly (k_rescore).
+//  * FP32: the same GEMM form on v_mfma_f32_32x32x2f32, plain compare/select.
+//  * EXACT: sum of squared differences per pair as a sequential FMA chain, FP32.
+#include "mcg_internal.h"
+#include <cfloat>
+#include <cmath>
+
+namespace mcg {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ uint16_t f2bf_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// ---------------------------------------------------------------------------
+// Dataset preparation: column mean mu (FP64 sums), BF16 copy of X - mu padded
+// to dpad columns, ||x - mu|| per row (rounded up), max over rows.
+// ---------------------------------------------------------------------------
+__global__ void k_colsum_partial(const float *__restrict__ X, uint64_t n, uint32_t d,
+                                 uint32_t rows_per, double *__restrict__ part) {
+    const uint64_t r0 = (uint64_t)blockIdx.y * rows_per;
+    const uint64_t r1 = min<uint64_t>(n, r0 + rows_per);
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < d; j += gridDim.x * blockDim.x) {
+        double s = 0.;
+        for (uint64_t r = r0; r < r1; ++r) s += (double)X[r * d + j];
+        part[(uint64_t)blockIdx.y * d + j] = s;
+    }
+}
+
+__global__ void k_colsum_final(const double *__restrict__ part, uint32_t nparts, uint32_t d,
+                               uint64_t n, float *__restrict__ mu) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d) return;
+    double s = 0.;
+    for (uint32_t p = 0; p < nparts; ++p) s += part[(uint64_t)p * d + j];
+    mu[j] = (float)(s / (double)n);
+}
+
+// one wave per row
+__global__ void k_make_xb(const float *__restrict__ X, uint64_t n, uint32_t d, uint32_t dpad,
+                          const float *__restrict__ mu, uint16_t *__restrict__ Xb,
+                          float *__restrict__ xnorm) {
+    const uint64_t row = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    double s = 0.;
+    for (uint32_t j = lane; j < dpad; j += 64) {
+        float v = j < d ? X[row * d + j] - mu[j] : 0.f;
+        Xb[row * dpad + j] = f2bf_rne(v);
+        s += (double)v * (double)v;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) xnorm[row] = (float)(sqrt(s) * (1. + 1e-6));
+}
+
+__global__ void k_max_f32(const float *__restrict__ v, uint64_t n, float *__restrict__ out) {
+    __shared__ float sm[256];
+    float m = 0.f;
+    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) m = fmaxf(m, v[i]);
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sm[threadIdx.x] = fmaxf(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sm[0];
+}
+
+static uint32_t pad_dims(uint64_t d) {
+    if (d <= 64) return 64;
+    if (d <= 128) return 128;
+    if (d <= 256) return 256;
+    return 512;
+}
+
+void prepare_bf16(Data &ds) {
+    if (ds.bf16_ready) return;
+    Ctx &c = *ds.ctx;
+    hipStream_t st = c.stream;
+    const uint64_t n = ds.n;
+    const uint32_t d = (uint32_t)ds.d;
+    REQUIRE(d <= 512, "BF16 form supports d <= 512");
+    ds.dpad = pad_dims(d);
+    const uint32_t rows_per = 4096;
+    const uint32_t nparts = (uint32_t)cdiv(n, rows_per);
+    DBuf part;
+    part.reserve((size_t)nparts * d * sizeof(double));
+    ds.mu.reserve(d * sizeof(float));
+    dim3 g1((uint32_t)cdiv(d, 256), nparts);
+    hipLaunchKernelGGL(k_colsum_partial, g1, dim3(256), 0, st, ds.X, n, d, rows_per, part.as<double>());
+    hipLaunchKernelGGL(k_colsum_final, dim3((uint32_t)cdiv(d, 256)), dim3(256), 0, st, part.as<double>(),
+                       nparts, d, n, ds.mu.as<float>());
+    ds.Xb.reserve((size_t)n * ds.dpad * sizeof(uint16_t));
+    ds.xnorm.reserve(n * sizeof(float));
+    hipLaunchKernelGGL(k_make_xb, dim3((uint32_t)cdiv(n, 4)), dim3(256), 0, st, ds.X, n, d, ds.dpad,
+                       ds.mu.as<float>(), ds.Xb.as<uint16_t>(), ds.xnorm.as<float>());
+    DBuf mx;
+    mx.reserve(sizeof(float));
+    hipLaunchKernelGGL(k_max_f32, dim3(1), dim3(256), 0, st, ds.xnorm.as<float>(), n, mx.as<float>());
+    HIP_CHECK(hipMemcpyAsync(&ds.xmax, mx.p, sizeof(float), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    ds.bf16_ready = true;
+}
+
+// ---------------------------------------------------------------------------
+// Center preparation (per E-step).
+// ---------------------------------------------------------------------------
+// one wave per (padded) center: Cb = bf16(-2 (c - mu)), cn = ||c - mu||^2 (FP64)
+__global__ void k_prep_centers_bf16(const float *__restrict__ C, uint64_t k, uint64_t kpad, uint32_t d,
+                                    uint32_t dpad, const float *__restrict__ mu,
+                                    uint16_t *__restrict__ Cb, double *__restrict__ cn) {
+    const uint64_t c = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (c >= kpad) return;
+    double s = 0.;
+    for (uint32_t j = lane; j < dpad; j += 64) {
+        float v = (c < k && j < d) ? C[c * d + j] - mu[j] : 0.f;
+        Cb[c * dpad + j] = f2bf_rne(-2.f * v);
+        s += (double)v * (double)v;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) cn[c] = s;
+}
+
+// single block: cmax, offset, near-tie coefficients, cadd[c] = off + cn[c]
+__global__ void k_prep_params(const double *__restrict__ cn, uint64_t k, uint64_t kpad, float xmax,
+                              uint32_t d, float *__restrict__ cadd, EParams *__restrict__ prm) {
+    __shared__ double sm[256];
+    double m = 0.;
+    for (uint64_t i = threadIdx.x; i < k; i += blockDim.x) m = fmax(m, cn[i]);
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const double cmax = sqrt(sm[0]) * (1. + 1e-6);
+    const double off = 2.25 * (double)xmax * cmax + 1.;
+    // Largest approximate value: off + cmax^2 + 2.1 xmax cmax.
+    const double vmax = off + cmax * cmax + 2.1 * (double)xmax * cmax;
+    const double u = 1. / 256.;                       // BF16 unit roundoff
+    const double e_abs = 400. * vmax * 0x1p-24        // accumulation, offset, packing
+                         + 4. * 0x1p-24 * ((double)xmax + cmax) * ((double)xmax + cmax);
+    if (threadIdx.x == 0) {
+        EParams p;
+        p.off = (float)off;
+        p.cmax = (float)cmax;
+        p.tau1 = (float)(4. * (2. * u + u * u) * cmax * (1. + 1e-3));
+        p.tau0 = (float)(2. * e_abs);
+        p.taur = (float)(1e-4 + 4. * (double)d * 0x1p-24);
+        p.pad[0] = p.pad[1] = p.pad[2] = 0.f;
+        *prm = p;
+    }
+    __syncthreads();
+    for (uint64_t i = threadIdx.x; i < kpad; i += blockDim.x)
+        cadd[i] = i < k ? (float)(off + cn[i]) : 3.0e38f;
+}
+
+// ---------------------------------------------------------------------------
+// BF16 MFMA E-step.  KS = dpad / 16 k-steps; NB n-blocks of 32 points per wave.
+// K-permutation: lane half h supplies dims [h*dpad/2 + 8s, +8) at step s for both
+// operands, so each lane reads contiguous 16-byte chunks.
+// ---------------------------------------------------------------------------
+template <int KS, int NB>
+__global__ __launch_bounds__(256) void k_estep_bf16(
+    const uint4 *__restrict__ Xb, const uint4 *__restrict__ Cb, const float *__restrict__ cadd,
+    const float *__restrict__ xnorm, const EParams *__restrict__ prm, uint32_t n, uint32_t ntiles, uint32_t kreal,
+    uint32_t *__restrict__ asn, uint32_t *__restrict__ flags, uint32_t *__restrict__ nflag) {
+    constexpr int CH = KS * 2;               // 16-byte chunks per row
+    constexpr int ROWB = CH * 16 + 16;       // padded LDS row stride (bytes)
+    constexpr int CPT = (64 * CH) / 256;     // tile chunks per thread
+    __shared__ __attribute__((aligned(16))) char lds[2][64 * ROWB];
+    __shared__ float ladd[2][64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 5, r32 = lane & 31;
+    const uint32_t pbase = blockIdx.x * (4u * NB * 32u) + wave * (NB * 32u);
+
+    bf16x8 b[NB][KS];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+        uint32_t p = pbase + nb * 32 + r32;
+        p = p < n ? p : n - 1;
+        const uint4 *row = Xb + (size_t)p * CH + h * KS;
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            uint4 v = row[s];
+            b[nb][s] = __builtin_bit_cast(bf16x8, v);
+        }
+    }
+
+    uint4 pf[CPT];
+    float addn = 0.f;
+    auto load_tile = [&](uint32_t t) {
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) {
+            int g = tid + q * 256;
+            pf[q] = Cb[(size_t)(t * 64 + g / CH) * CH + (g % CH)];
+        }
+        if (tid < 64) addn = cadd[t * 64 + tid];
+    };
+    auto store_tile = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) {
+            int g = tid + q * 256;
+            *(uint4 *)(lds[buf] + (g / CH) * ROWB + (g % CH) * 16) = pf[q];
+        }
+        if (tid < 64) ladd[buf][tid] = addn;
+    };
+
+    float best[NB], sec[NB];
+    uint32_t btile[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) { best[nb] = FLT_MAX; sec[nb] = FLT_MAX; btile[nb] = 0; }
+
+    load_tile(0);
+    store_tile(0);
+    __syncthreads();
+    for (uint32_t t = 0; t < ntiles; ++t) {
+        const int buf = t & 1;
+        if (t + 1 < ntiles) load_tile(t + 1);
+        float bstart[NB];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) bstart[nb] = best[nb];
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb) {
+            floatx16 acc[NB];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                float v = ladd[buf][mb * 32 + 8 * (e >> 2) + 4 * h + (e & 3)];
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) acc[nb][e] = v;
+            }
+            const char *arow = lds[buf] + (mb * 32 + r32) * ROWB + h * KS * 16;
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                bf16x8 a = *(const bf16x8 *)(arow + s * 16);
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb)
+                    acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[nb][s], acc[nb], 0, 0, 0);
+            }
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const uint32_t pos = mb * 32 + 8 * (e >> 2) + (e & 3);
+                    float v = __uint_as_float((__float_as_uint(acc[nb][e]) & 0xFFFFFFC0u) | pos);
+                    sec[nb] = __builtin_amdgcn_fmed3f(best[nb], v, sec[nb]);
+                    best[nb] = fminf(best[nb], v);
+                }
+            }
+        }
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+            if (__float_as_uint(best[nb]) != __float_as_uint(bstart[nb])) btile[nb] = t;
+        if (t + 1 < ntiles) store_tile(buf ^ 1);
+        __syncthreads();
+    }
+
+    const EParams P = *prm;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+        const uint32_t bb = __float_as_uint(best[nb]);
+        const uint32_t gi = btile[nb] * 64u + (bb & 63u) + 4u * h;
+        const float bv = __uint_as_float(bb & 0xFFFFFFC0u);
+        const float sv = __uint_as_float(__float_as_uint(sec[nb]) & 0xFFFFFFC0u);
+        const float obv = __shfl_xor(bv, 32);
+        const uint32_t ogi = __shfl_xor(gi, 32);
+        const float osv = __shfl_xor(sv, 32);
+        const bool other = (obv < bv) || (obv == bv && ogi < gi);
+        const float fb = other ? obv : bv;
+        const uint32_t fi = other ? ogi : gi;
+        const float fs = fminf(fmaxf(bv, obv), fminf(sv, osv));
+        const uint32_t p = pbase + nb * 32 + r32;
+        if (h == 0 && p < n) {
+            // a non-finite row can leave no real center selected: re-score it exactly
+            const bool ok = fi < kreal;
+            asn[p] = ok ? fi : 0u;
+            const float xn = xnorm[p];
+            const float dbest = xn * xn + (fb - P.off);
+            const float tau = P.tau1 * xn + P.tau0 + P.taur * fmaxf(dbest, 0.f);
+            if (!ok || !(fs - fb > tau)) flags[atomicAdd(nflag, 1u)] = p;
+        }
+    }
+}
+
+// Exact FP32 argmin for the listed points: one wave per point, lanes over centers,
+// each pair a sequential FMA chain over the dims (lowest index wins ties).
+__global__ void k_rescore(const float *__restrict__ X, uint32_t d, const float *__restrict__ C, uint32_t k,
+                          const uint32_t *__restrict__ list, const uint32_t *__restrict__ nlist,
+                          uint32_t *__restrict__ asn) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t cnt = *nlist;
+    for (uint32_t w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); w < cnt;
+         w += gridDim.x * (blockDim.x / 64)) {
+        const uint32_t p = list[w];
+        const float *x = X + (uint64_t)p * d;
+        float best = 0.f;
+        uint32_t bi = 0xFFFFFFFFu;
+        for (uint32_t c = lane; c < k; c += 64) {
+            const float *cc = C + (uint64_t)c * d;
+            float s = 0.f;
+            for (uint32_t j = 0; j < d; ++j) { float t = x[j] - cc[j]; s = fmaf(t, t, s); }
+            if (bi == 0xFFFFFFFFu || s < best) { best = s; bi = c; }
+        }
+        // wave argmin, lowest index on ties
+        for (int o = 32; o > 0; o >>= 1) {
+            float ob = __shfl_xor(best, o);
+            uint32_t oi = __shfl_xor(bi, o);
+            bool take = (oi != 0xFFFFFFFFu) &&
+                        (bi == 0xFFFFFFFFu || ob < best || (ob == best && oi < bi));
+            if (take) { best = ob; bi = oi; }
+        }
+        if (lane == 0) asn[p] = bi;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 MFMA E-step ("plain FP32"): v_mfma_f32_32x32x2f32, 32 points per wave,
+// tiles of 32 centers.  KS = dpad/2 steps; lane half h supplies dim h*dpad/2 + s.
+// ---------------------------------------------------------------------------
+template <int DP>
+__global__ __launch_bounds__(256) void k_estep_f32(const float *__restrict__ X, uint32_t d,
+                                                   const float *__restrict__ mu,
+                                                   const float *__restrict__ Cc,   // kpad x DP, (c - mu) * -2
+                                                   const float *__restrict__ cadd, // kpad
+                                                   uint32_t n, uint32_t ntiles,
+                                                   uint32_t *__restrict__ asn) {
+    constexpr int KS = DP / 2;
+    constexpr int ROWF = DP + 4;            // padded LDS row stride (floats)
+    constexpr int CPT = (32 * DP / 4) / 256; // float4 chunks per thread per tile
+    __shared__ __attribute__((aligned(16))) float lds[2][32 * ROWF];
+    __shared__ float ladd[2][32];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 5, r32 = lane & 31;
+    const uint32_t p = blockIdx.x * 128u + wave * 32u + r32;
+    const uint32_t pc = p < n ? p : n - 1;
+
+    float b[KS];
+    const float *xr = X + (uint64_t)pc * d;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        const uint32_t j = h * KS + s;
+        b[s] = j < d ? xr[j] - mu[j] : 0.f;
+    }
+
+    float4 pf[CPT > 0 ? CPT : 1];
+    float addn = 0.f;
+    auto load_tile = [&](uint32_t t) {
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) {
+            int g = tid + q * 256;
+            pf[q] = ((const float4 *)Cc)[(size_t)(t * 32 + g / (DP / 4)) * (DP / 4) + (g % (DP / 4))];
+        }
+        if (tid < 32) addn = cadd[t * 32 + tid];
+    };
+    auto store_tile = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) {
+            int g = tid + q * 256;
+            *(float4 *)(&lds[buf][(g / (DP / 4)) * ROWF + (g % (DP / 4)) * 4]) = pf[q];
+        }
+        if (tid < 32) ladd[buf][tid] = addn;
+    };
+
+    float best = 0.f;
+    uint32_t bi = 0xFFFFFFFFu;
+    load_tile(0);
+    store_tile(0);
+    __syncthreads();
+    for (uint32_t t = 0; t < ntiles; ++t) {
+        const int buf = t & 1;
+        if (t + 1 < ntiles) load_tile(t + 1);
+        floatx16 acc;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[e] = ladd[buf][8 * (e >> 2) + 4 * h + (e & 3)];
+        const float *arow = &lds[buf][r32 * ROWF + h * KS];
+#pragma unroll
+        for (int s = 0; s < KS; s += 4) {
+            float4 a = *(const float4 *)(arow + s);
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b[s + 0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b[s + 1], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, b[s + 2], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, b[s + 3], acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const uint32_t c = t * 32u + 8u * (e >> 2) + 4u * h + (e & 3);
+            if (bi == 0xFFFFFFFFu || acc[e] < best) { best = acc[e]; bi = c; }
+        }
+        if (t + 1 < ntiles) store_tile(buf ^ 1);
+        __syncthreads();
+    }
+    const float ob = __shfl_xor(best, 32);
+    const uint32_t oi = __shfl_xor(bi, 32);
+    if (ob < best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+    if (h == 0 && p < n) asn[p] = bi;
+}
+
+__global__ void k_prep_centers_f32(const float *__restrict__ C, uint64_t k, uint64_t kpad, uint32_t d,
+                                   uint32_t dpad, const float *__restrict__ mu, float *__restrict__ Cc,
+                                   float *__restrict__ cadd) {
+    const uint64_t c = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (c >= kpad) return;
+    double s = 0.;
+    for (uint32_t j = lane; j < dpad; j += 64) {
+        float v = (c < k && j < d) ? C[c * d + j] - mu[j] : 0.f;
+        Cc[c * dpad + j] = -2.f * v;
+        s += (double)v * (double)v;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) cadd[c] = c < k ? (float)s : FLT_MAX;
+}
+
+// ---------------------------------------------------------------------------
+// EXACT form and per-point costs: sequential FMA chain per pair, FP32.
+// 256 points per block; X staged through LDS in 32-dim slabs (coalesced reads).
+// ---------------------------------------------------------------------------
+constexpr int EX_TC = 16;  // centers per register tile
+
+__global__ __launch_bounds__(256) void k_assign_exact(const float *__restrict__ X, uint64_t n, uint32_t d,
+                                                      const float *__restrict__ C, uint32_t k,
+                                                      uint32_t *__restrict__ asn, float *__restrict__ cost) {
+    __shared__ float xs[256][33];
+    __shared__ float cs[EX_TC][32];
+    const int tid = threadIdx.x;
+    const uint64_t p0 = (uint64_t)blockIdx.x * 256;
+    const uint64_t p = p0 + tid;
+    float best = 0.f;
+    uint32_t bi = 0xFFFFFFFFu;
+    for (uint32_t c0 = 0; c0 < k; c0 += EX_TC) {
+        float acc[EX_TC];
+#pragma unroll
+        for (int c = 0; c < EX_TC; ++c) acc[c] = 0.f;
+        for (uint32_t j0 = 0; j0 < d; j0 += 32) {
+            __syncthreads();
+            // stage X[p0..p0+256)[j0..j0+32): 8192 floats, 32 per thread, rows contiguous
+            for (int q = tid; q < 256 * 32; q += 256) {
+                int r = q >> 5, jj = q & 31;
+                uint64_t pr = p0 + r;
+                xs[r][jj] = (pr < n && j0 + jj < d) ? X[pr * d + j0 + jj] : 0.f;
+            }
+            for (int q = tid; q < EX_TC * 32; q += 256) {
+                int c = q >> 5, jj = q & 31;
+                cs[c][jj] = (c0 + c < k && j0 + jj < d) ? C[(uint64_t)(c0 + c) * d + j0 + jj] : 0.f;
+            }
+            __syncthreads();
+            const int jn = (int)min<uint32_t>(32, d - j0);
+            for (int jj = 0; jj < jn; ++jj) {
+                const float xv = xs[tid][jj];
+#pragma unroll
+                for (int c = 0; c < EX_TC; ++c) { float t = xv - cs[c][jj]; acc[c] = fmaf(t, t, acc[c]); }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < EX_TC; ++c) {
+            if (c0 + c < k && (bi == 0xFFFFFFFFu || acc[c] < best)) { best = acc[c]; bi = c0 + c; }
+        }
+    }
+    if (p < n) { asn[p] = bi; if (cost) cost[p] = best; }
+}
+
+__global__ __launch_bounds__(256) void k_point_costs(const float *__restrict__ X, uint64_t n, uint32_t d,
+                                                     const float *__restrict__ C,
+                                                     const uint32_t *__restrict__ asn,
+                                                     float *__restrict__ cost) {
+    __shared__ float xs[256][33];
+    const int tid = threadIdx.x;
+    const uint64_t p0 = (uint64_t)blockIdx.x * 256;
+    const uint64_t p = p0 + tid;
+    const float *cr = C + (uint64_t)(p < n ? asn[p] : 0) * d;
+    float s = 0.f;
+    for (uint32_t j0 = 0; j0 < d; j0 += 32) {
+        __syncthreads();
+        for (int q = tid; q < 256 * 32; q += 256) {
+            int r = q >> 5, jj = q & 31;
+            uint64_t pr = p0 + r;
+            xs[r][jj] = (pr < n && j0 + jj < d) ? X[pr * d + j0 + jj] : 0.f;
+        }
+        __syncthreads();
+        const int jn = (int)min<uint32_t>(32, d - j0);
+        for (int jj = 0; jj < jn; ++jj) { float t = xs[tid][jj] - cr[j0 + jj]; s = fmaf(t, t, s); }
+    }
+    if (p < n) cost[p] = s;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <int KS, int NB>
+static void launch_bf16(Data &ds, Ctx &c, uint64_t kpad, uint64_t kreal, uint32_t *asn, hipStream_t st) {
+    const uint32_t n = (uint32_t)ds.n;
+    const uint32_t per = 4u * NB * 32u;
+    hipEvent_t ev;
+    c.tic(0, ev);
+    hipLaunchKernelGGL((k_estep_bf16<KS, NB>), dim3((uint32_t)cdiv(n, per)), dim3(256), 0, st,
+                       ds.Xb.as<const uint4>(), c.s[S_CB].as<const uint4>(), c.s[S_CADD].as<const float>(),
+                       ds.xnorm.as<const float>(), c.s[S_PARAMS].as<const EParams>(), n,
+                       (uint32_t)(kpad / 64), (uint32_t)kreal, asn, c.s[S_FLAGS].as<uint32_t>(),
+                       c.s[S_FLAGCNT].as<uint32_t>());
+    c.toc(0, ev);
+}
+
+template <int DP>
+static void launch_f32(Data &ds, Ctx &c, uint64_t kpad, uint32_t *asn, hipStream_t st) {
+    const uint32_t n = (uint32_t)ds.n;
+    hipEvent_t ev;
+    c.tic(0, ev);
+    hipLaunchKernelGGL((k_estep_f32<DP>), dim3((uint32_t)cdiv(n, 128)), dim3(256), 0, st, ds.X,
+                       (uint32_t)ds.d, ds.mu.as<const float>(), c.s[S_CB].as<const float>(),
+                       c.s[S_CADD].as<const float>(), n, (uint32_t)(kpad / 32), asn);
+    c.toc(0, ev);
+}
+
+void dense_sqrl2_assign_exact(Data &ds, const float *C, uint64_t k, uint32_t *asn, float *costs,
+                              hipStream_t st) {
+    hipEvent_t ev;
+    ds.ctx->tic(0, ev);
+    hipLaunchKernelGGL(k_assign_exact, dim3((uint32_t)cdiv(ds.n, 256)), dim3(256), 0, st, ds.X, ds.n,
+                       (uint32_t)ds.d, C, (uint32_t)k, asn, costs);
+    ds.ctx->toc(0, ev);
+}
+
+void dense_sqrl2_costs(Data &ds, const float *C, const uint32_t *asn, float *costs, hipStream_t st) {
+    hipLaunchKernelGGL(k_point_costs, dim3((uint32_t)cdiv(ds.n, 256)), dim3(256), 0, st, ds.X, ds.n,
+                       (uint32_t)ds.d, C, asn, costs);
+}
+
+void dense_sqrl2_assign(Data &ds, const float *C, uint64_t k, int precision, uint32_t *asn,
+                        hipStream_t st) {
+    Ctx &c = *ds.ctx;
+    REQUIRE(ds.n < (1ull << 31), "dense rows must be < 2^31");
+    if (precision == MCG_PREC_EXACT || ds.d > 512 || (precision == MCG_PREC_FP32 && ds.d > 256)) {
+        dense_sqrl2_assign_exact(ds, C, k, asn, nullptr, st);
+        return;
+    }
+    prepare_bf16(ds);  // mu, dpad (and the BF16 copy) are shared by both MFMA forms
+    const uint32_t d = (uint32_t)ds.d, dpad = ds.dpad;
+    if (precision == MCG_PREC_FP32) {
+        const uint64_t kpad = cdiv(k, 32) * 32;
+        c.s[S_CB].reserve(kpad * dpad * sizeof(float));
+        c.s[S_CADD].reserve(kpad * sizeof(float));
+        hipLaunchKernelGGL(k_prep_centers_f32, dim3((uint32_t)cdiv(kpad, 4)), dim3(256), 0, st, C, k, kpad, d,
+                           dpad, ds.mu.as<const float>(), c.s[S_CB].as<float>(), c.s[S_CADD].as<float>());
+        switch (dpad) {
+        case 64: launch_f32<64>(ds, c, kpad, asn, st); break;
+        case 128: launch_f32<128>(ds, c, kpad, asn, st); break;
+        default: launch_f32<256>(ds, c, kpad, asn, st); break;
+        }
+        return;
+    }
+    const uint64_t kpad = cdiv(k, 64) * 64;
+    c.s[S_CB].reserve(kpad * dpad * sizeof(uint16_t));
+    c.s[S_CADD].reserve(kpad * sizeof(float));
+    c.s[S_CNORM].reserve(kpad * sizeof(double));
+    c.s[S_PARAMS].reserve(sizeof(EParams));
+    c.s[S_FLAGS].reserve(ds.n * sizeof(uint32_t));
+    c.s[S_FLAGCNT].reserve(sizeof(uint32_t));
+    hipLaunchKernelGGL(k_prep_centers_bf16, dim3((uint32_t)cdiv(kpad, 4)), dim3(256), 0, st, C, k, kpad, d, dpad,
+                       ds.mu.as<const float>(), c.s[S_CB].as<uint16_t>(), c.s[S_CNORM].as<double>());
+    hipLaunchKernelGGL(k_prep_params, dim3(1), dim3(256), 0, st, c.s[S_CNORM].as<const double>(), k, kpad,
+                       ds.xmax, d, c.s[S_CADD].as<float>(), c.s[S_PARAMS].as<EParams>());
+    fill_u32(c.s[S_FLAGCNT].as<uint32_t>(), 0u, 1, st);
+    switch (dpad) {
+    case 64: launch_bf16<4, 2>(ds, c, kpad, k, asn, st); break;
+    case 128: launch_bf16<8, 2>(ds, c, kpad, k, asn, st); break;
+    case 256: launch_bf16<16, 2>(ds, c, kpad, k, asn, st); break;
+    default: launch_bf16<32, 1>(ds, c, kpad, k, asn, st); break;
+    }
+    hipLaunchKernelGGL(k_rescore, dim3(512), dim3(256), 0, st, ds.X, d, C, (uint32_t)k,
+                       c.s[S_FLAGS].as<const uint32_t>(), c.s[S_FLAGCNT].as<const uint32_t>(), asn);
+}
+
+}  // namespace mcg

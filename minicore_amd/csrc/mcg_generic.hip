@@ -1,0 +1,399 @@
+// Point-to-center costs for every measure on the hot path, dense or CSR rows
+// against dense centers, FP32 arithmetic with FP64 per-center/per-row terms.
+//
+// Replaces msr_with_prior (include/minicore/dist/applicator.h:1632-1915) in the
+// n x k loops of assign_points_hard (solve.h:234-285), the soft cost matrix
+// (solve.h:368-371, python/pycluster_soft.h:38-40) and cmp (python/pycmp.cpp).
+//
+// With x~ = (x+pv)/L, L = sum(x)+d*pv, li = pv/L (the value of x~ off the row's
+// support) and c~ = (c+pv)/M, M = sum(c)+d*pv, each cost splits into per-row
+// terms, per-center terms and a sum over the row's stored entries only:
+//   SQRL2       = ||c||^2 + sum_supp x (x - 2c)
+//   MKL         = H_c - S_c (log pv - log L) - sum_supp c~ log1p(x/pv)
+//                 (H_c = sum c~ log c~, S_c = sum c~)
+//   REVERSE_MKL = H_x - li * sum_j log c~_j - (1/L) sum_supp x log c~
+//   JSD         = G_c(li) + sum_supp [h(x~, c~) - h(li, c~)]
+//                 (h = the per-coordinate JSD term, G_c(l) = sum_j h(l, c~_j))
+// pv is the prior when prior >= 1e-20 (then applicator.h:1644-1648 always
+// returns it); a smaller prior takes the per-pair kernel k_pairs_direct, which
+// follows the reference formula coordinate by coordinate.
+#include "mcg_internal.h"
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+
+namespace mcg {
+
+constexpr double DIRECT_PRIOR = 1e-20;
+
+// ---- center tables -----------------------------------------------------------
+// One block per center.  CT is d x kpad (transposed so lanes over centers read
+// contiguous memory); cs = {a, b} per-center scalars (FP64).
+__global__ void k_center_tables(int msr, const float *__restrict__ C, uint32_t k, uint32_t kpad, uint32_t d,
+                                double pv, float *__restrict__ CT, double *__restrict__ cs) {
+    __shared__ double red[2][256];
+    const uint32_t c = blockIdx.x;
+    const float *cr = C + (uint64_t)c * d;
+    // center sum
+    double s = 0.;
+    for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) s += (double)cr[j];
+    red[0][threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[0][threadIdx.x] += red[0][threadIdx.x + o];
+        __syncthreads();
+    }
+    const double csum = red[0][0];
+    __syncthreads();
+    const double M = csum + (double)d * pv, Mi = 1. / M;
+    double a = 0., b = 0.;
+    for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) {
+        const double cj = (double)cr[j];
+        float t;
+        if (msr == MCG_SQRL2) {
+            t = (float)cj;
+            a += cj * cj;
+        } else {
+            const double ct = (cj + pv) * Mi;
+            if (msr == MCG_REVERSE_MKL) {
+                const double l = log(ct);
+                t = (float)l;
+                a += l;
+            } else {
+                t = (float)ct;
+                a += ct * log(ct);
+                b += ct;
+            }
+        }
+        CT[(uint64_t)j * kpad + c] = t;
+    }
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = b;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + o];
+            red[1][threadIdx.x] += red[1][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        cs[2 * c] = red[0][0];
+        cs[2 * c + 1] = red[1][0];
+    }
+}
+
+// per-row terms: {L, log L, H_x, li}
+__global__ void k_row_terms(int msr, const float *__restrict__ X, const float *__restrict__ val,
+                            const uint64_t *__restrict__ ptr, uint64_t n, uint32_t d, const double *__restrict__ rsum,
+                            double pv, float4 *__restrict__ out) {
+    const uint64_t row = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const double L = rsum[row] + (double)d * pv, Li = 1. / L, li = pv * Li;
+    double h = 0.;
+    uint64_t cnt = 0;
+    if (msr == MCG_REVERSE_MKL || msr == MCG_JSD) {
+        if (val) {
+            for (uint64_t p = ptr[row] + lane; p < ptr[row + 1]; p += 64) {
+                const double xt = ((double)val[p] + pv) * Li;
+                h += xt * log(xt);
+                ++cnt;
+            }
+        } else {
+            for (uint32_t j = lane; j < d; j += 64) {
+                const double xt = ((double)X[row * d + j] + pv) * Li;
+                h += xt * log(xt);
+                ++cnt;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) { h += __shfl_xor(h, o); cnt += __shfl_xor(cnt, o); }
+        h += (double)(d - cnt) * li * log(li);
+    }
+    if (lane == 0) out[row] = make_float4((float)L, (float)log(L), (float)h, (float)li);
+}
+
+// h(a, b): one coordinate of JSD, 1/2 [a log(2a/(a+b)) + b log(2b/(a+b))]
+__device__ __forceinline__ float jsd_h(float a, float b) {
+    const float m = a + b;
+    return 0.5f * (a * __logf(2.f * a / m) + b * __logf(2.f * b / m));
+}
+
+// G[u][c] = sum_j h(li_u, c~_j), FP64 accumulation.  One wave per (u, c).
+__global__ void k_jsd_g(const float *__restrict__ CT, uint32_t kpad, uint32_t k, uint32_t d,
+                        const float *__restrict__ lis, uint32_t U, double *__restrict__ G) {
+    const uint64_t wid = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (wid >= (uint64_t)U * k) return;
+    const uint32_t u = (uint32_t)(wid / k), c = (uint32_t)(wid % k);
+    const double l = lis[u];
+    double s = 0.;
+    for (uint32_t j = lane; j < d; j += 64) {
+        const double b = CT[(uint64_t)j * kpad + c];
+        const double m = l + b;
+        s += 0.5 * (l * log(2. * l / m) + b * log(2. * b / m));
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) G[(uint64_t)u * kpad + c] = s;
+}
+
+__device__ __forceinline__ float post_kl(float r) {
+    r = r < 0.f ? 0.f : r;
+    if (r == INFINITY) r = FLT_MAX;
+    return r;
+}
+
+// One wave per row; lanes over centers in chunks of 64; row entries in batches
+// of 64 broadcast with shuffles.
+template <int MSR>
+__global__ __launch_bounds__(256) void k_row_costs(const float *__restrict__ X, const float *__restrict__ val,
+                                                   const uint32_t *__restrict__ idx, const uint64_t *__restrict__ ptr,
+                                                   uint64_t n, uint32_t d, float pv, float logpv,
+                                                   const float4 *__restrict__ rt, const float *__restrict__ CT,
+                                                   uint32_t kpad, uint32_t k, const double *__restrict__ cs,
+                                                   const double *__restrict__ G, const uint32_t *__restrict__ slot,
+                                                   float *__restrict__ out, uint32_t *__restrict__ asn,
+                                                   float *__restrict__ best_out) {
+    const uint64_t row = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    uint64_t b, e;
+    if (val) { b = ptr[row]; e = ptr[row + 1]; }
+    else { b = row * d; e = b + d; }
+    float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (MSR != MCG_SQRL2) r = rt[row];
+    const float L = r.x, logL = r.y, Hx = r.z, li = r.w;
+    const float Li = 1.f / L;
+    const float hli = li * __logf(li);
+    float best = 0.f;
+    uint32_t bi = 0xFFFFFFFFu;
+    for (uint32_t c0 = 0; c0 < k; c0 += 64) {
+        const uint32_t c = c0 + lane;
+        const uint32_t cc = c < kpad ? c : kpad - 1;
+        float acc = 0.f;
+        for (uint64_t p0 = b; p0 < e; p0 += 64) {
+            const uint64_t p = p0 + lane;
+            uint32_t jq = 0;
+            float tq = 0.f, tq2 = 0.f;
+            if (p < e) {
+                const float x = val ? val[p] : X[p];
+                jq = val ? idx[p] : (uint32_t)(p - b);
+                if (MSR == MCG_SQRL2) { tq = x; }
+                else if (MSR == MCG_MKL) { tq = log1pf(x / pv); }
+                else if (MSR == MCG_REVERSE_MKL) { tq = x; }
+                else { const float xt = (x + pv) * Li; tq = xt; tq2 = xt * __logf(xt); }
+            }
+            const int cnt = (int)min<uint64_t>(64, e - p0);
+            for (int t = 0; t < cnt; ++t) {
+                const uint32_t j = __shfl(jq, t);
+                const float xv = __shfl(tq, t);
+                const float ct = CT[(uint64_t)j * kpad + cc];
+                if (MSR == MCG_SQRL2) acc = fmaf(xv, xv - 2.f * ct, acc);
+                else if (MSR == MCG_MKL || MSR == MCG_REVERSE_MKL) acc = fmaf(xv, ct, acc);
+                else {
+                    const float xl = __shfl(tq2, t);
+                    const float m1 = xv + ct, m0 = li + ct;
+                    acc += 0.5f * ((xl - hli) - m1 * __logf(0.5f * m1) + m0 * __logf(0.5f * m0));
+                }
+            }
+        }
+        float v;
+        if (MSR == MCG_SQRL2) {
+            v = (float)(cs[2 * cc] + (double)acc);
+            v = v < 0.f ? 0.f : v;
+        } else if (MSR == MCG_MKL) {
+            v = (float)(cs[2 * cc] - cs[2 * cc + 1] * (double)(logpv - logL) - (double)acc);
+            v = post_kl(v);
+        } else if (MSR == MCG_REVERSE_MKL) {
+            v = (float)((double)Hx - (double)li * cs[2 * cc] - (double)acc * (double)Li);
+            v = post_kl(v);
+        } else {
+            v = (float)(G[(uint64_t)slot[row] * kpad + cc] + (double)acc);
+            v = post_kl(v);
+        }
+        if (c < k) {
+            if (out) out[row * k + c] = v;
+            if (bi == 0xFFFFFFFFu || v < best) { best = v; bi = c; }
+        }
+    }
+    if (asn) {
+        for (int o = 32; o > 0; o >>= 1) {
+            const float ob = __shfl_xor(best, o);
+            const uint32_t oi = __shfl_xor(bi, o);
+            const bool take = (oi != 0xFFFFFFFFu) && (bi == 0xFFFFFFFFu || ob < best || (ob == best && oi < bi));
+            if (take) { best = ob; bi = oi; }
+        }
+        if (lane == 0) {
+            asn[row] = bi;
+            if (best_out) best_out[row] = best;
+        }
+    }
+}
+
+// ---- reference-order per-pair kernel (tiny priors) -----------------------------
+// FP32 (FT=float) following applicator.h:1642-1760 coordinate by coordinate, with
+// pv = max(prior, 7.0069e-42 (lhsum + rhsum)) for each pair.  One wave per row.
+__global__ __launch_bounds__(256) void k_pairs_direct(int msr, const float *__restrict__ X,
+                                                      const float *__restrict__ val,
+                                                      const uint32_t *__restrict__ idx,
+                                                      const uint64_t *__restrict__ ptr, uint64_t n, uint32_t d,
+                                                      const double *__restrict__ rsum, const float *__restrict__ C,
+                                                      uint32_t k, const double *__restrict__ csum, float prior,
+                                                      float *__restrict__ out, uint32_t *__restrict__ asn,
+                                                      float *__restrict__ best_out) {
+    const uint64_t row = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    float best = 0.f;
+    uint32_t bi = 0xFFFFFFFFu;
+    const float xs = (float)rsum[row];
+    for (uint32_t c0 = 0; c0 < k; c0 += 64) {
+        const uint32_t c = c0 + lane;
+        const uint32_t cc = c < k ? c : k - 1;
+        const float cs = (float)csum[cc];
+        const float psum0 = prior * (float)d;
+        float lhsum = cs + psum0, rhsum = xs + psum0;
+        const float smallest = 7.0069e-42f * (lhsum + rhsum);
+        const float pv = fmaxf(prior, smallest);
+        const float psum = pv * (float)d;
+        lhsum = cs + psum;
+        rhsum = xs + psum;
+        const float lhrsi = 1.f / lhsum, rhrsi = 1.f / rhsum;
+        float lhinc = (pv != 0.f && lhsum != 0.f) ? pv * lhrsi : 0.f;
+        float rhinc = (pv != 0.f && rhsum != 0.f) ? pv * rhrsi : 0.f;
+        if (lhinc != lhinc) lhinc = 0.f;
+        if (rhinc != rhinc) rhinc = 0.f;
+        float acc = 0.f;
+        uint64_t p = val ? ptr[row] : 0, pe = val ? ptr[row + 1] : 0;
+        const float *cr = C + (uint64_t)cc * d;
+        for (uint32_t j = 0; j < d; ++j) {
+            float x;
+            if (val) { x = (p < pe && idx[p] == j) ? val[p++] : 0.f; }
+            else x = X[row * d + j];
+            const float cj = cr[j];
+            if (msr == MCG_SQRL2) { const float t = cj - x; acc = fmaf(t, t, acc); continue; }
+            const float a = cj * lhrsi + lhinc, bb = x * rhrsi + rhinc;
+            if (msr == MCG_MKL) acc += a * logf(a / bb);
+            else if (msr == MCG_REVERSE_MKL) acc += bb * logf(bb / a);
+            else { const float m = a + bb; acc += 0.5f * (a * logf(2.f * a / m) + bb * logf(2.f * bb / m)); }
+        }
+        float v = acc;
+        if (msr != MCG_SQRL2) v = post_kl(v);
+        if (c < k) {
+            if (out) out[row * k + c] = v;
+            if (bi == 0xFFFFFFFFu || v < best) { best = v; bi = c; }
+        }
+    }
+    if (asn) {
+        for (int o = 32; o > 0; o >>= 1) {
+            const float ob = __shfl_xor(best, o);
+            const uint32_t oi = __shfl_xor(bi, o);
+            const bool take = (oi != 0xFFFFFFFFu) && (bi == 0xFFFFFFFFu || ob < best || (ob == best && oi < bi));
+            if (take) { best = ob; bi = oi; }
+        }
+        if (lane == 0) { asn[row] = bi; if (best_out) best_out[row] = best; }
+    }
+}
+
+__global__ void k_center_sums(const float *__restrict__ C, uint32_t k, uint32_t d, double *__restrict__ out) {
+    const uint64_t c = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (c >= k) return;
+    double s = 0.;
+    for (uint32_t j = lane; j < d; j += 64) s += (double)C[c * d + j];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) out[c] = s;
+}
+
+// Per-prior row transforms: row terms and (JSD) the distinct li values.
+void csr_transforms(Data &ds, double pv, hipStream_t st) {
+    if (ds.tf_prior == pv && ds.rowinfo.p) return;
+    rowsums(ds, st);
+    ds.rowinfo.reserve(ds.n * sizeof(float4));
+    // row terms for REVERSE_MKL/JSD need H_x; computing it for every measure is cheap
+    hipLaunchKernelGGL(k_row_terms, dim3((uint32_t)cdiv(ds.n, 4)), dim3(256), 0, st, (int)MCG_JSD, ds.X,
+                       ds.csr ? ds.val.as<const float>() : nullptr, ds.csr ? ds.ptr.as<const uint64_t>() : nullptr,
+                       ds.n, (uint32_t)ds.d, ds.rowsum.as<const double>(), pv, ds.rowinfo.as<float4>());
+    // distinct li (float) values for JSD
+    std::vector<float4> ri(ds.n);
+    HIP_CHECK(hipMemcpyAsync(ri.data(), ds.rowinfo.p, ds.n * sizeof(float4), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    std::vector<float> lis(ds.n);
+    for (uint64_t i = 0; i < ds.n; ++i) lis[i] = ri[i].w;
+    std::vector<float> u = lis;
+    std::sort(u.begin(), u.end());
+    u.erase(std::unique(u.begin(), u.end()), u.end());
+    std::vector<uint32_t> sl(ds.n);
+    for (uint64_t i = 0; i < ds.n; ++i)
+        sl[i] = (uint32_t)(std::lower_bound(u.begin(), u.end(), lis[i]) - u.begin());
+    ds.li_unique = u;
+    ds.li_slot.reserve(ds.n * 4);
+    HIP_CHECK(hipMemcpy(ds.li_slot.p, sl.data(), ds.n * 4, hipMemcpyHostToDevice));
+    ds.tf_prior = pv;
+}
+
+void generic_costs(Data &ds, int msr, double prior, const float *C, uint64_t k, float *out_nk, uint32_t *asn,
+                   float *best, hipStream_t st) {
+    Ctx &c = *ds.ctx;
+    const uint32_t d = (uint32_t)ds.d;
+    const uint64_t n = ds.n;
+    const float *X = ds.csr ? nullptr : ds.X;
+    const float *val = ds.csr ? ds.val.as<const float>() : nullptr;
+    const uint32_t *idx = ds.csr ? ds.idx.as<const uint32_t>() : nullptr;
+    const uint64_t *ptr = ds.csr ? ds.ptr.as<const uint64_t>() : nullptr;
+    REQUIRE(k < (1ull << 31), "k too large");
+    if (msr != MCG_SQRL2 && prior < DIRECT_PRIOR) {
+        rowsums(ds, st);
+        double *csum = (double *)c.s[S_CPARAM].reserve(k * sizeof(double));
+        hipLaunchKernelGGL(k_center_sums, dim3((uint32_t)cdiv(k, 4)), dim3(256), 0, st, C, (uint32_t)k, d, csum);
+        hipLaunchKernelGGL(k_pairs_direct, dim3((uint32_t)cdiv(n, 4)), dim3(256), 0, st, msr, X, val, idx, ptr, n, d,
+                           ds.rowsum.as<const double>(), C, (uint32_t)k, csum, (float)prior, out_nk, asn, best);
+        return;
+    }
+    const double pv = msr == MCG_SQRL2 ? 0. : prior;
+    const uint32_t kpad = (uint32_t)cdiv(k, 64) * 64;
+    float *CT = (float *)c.s[S_CT].reserve((uint64_t)d * kpad * sizeof(float));
+    double *cs = (double *)c.s[S_CPARAM].reserve(2 * (uint64_t)kpad * sizeof(double));
+    hipLaunchKernelGGL(k_center_tables, dim3((uint32_t)k), dim3(256), 0, st, msr, C, (uint32_t)k, kpad, d, pv, CT, cs);
+    const float4 *rt = nullptr;
+    const double *G = nullptr;
+    const uint32_t *slot = nullptr;
+    if (msr != MCG_SQRL2) {
+        csr_transforms(ds, pv, st);
+        rt = ds.rowinfo.as<const float4>();
+    }
+    if (msr == MCG_JSD) {
+        const uint32_t U = (uint32_t)ds.li_unique.size();
+        REQUIRE(U <= (1u << 20), "JSD: too many distinct row sums");
+        float *lis = (float *)c.s[S_LIS].reserve(U * sizeof(float));
+        HIP_CHECK(hipMemcpyAsync(lis, ds.li_unique.data(), U * sizeof(float), hipMemcpyHostToDevice, st));
+        double *Gb = (double *)c.s[S_JSDG].reserve((uint64_t)U * kpad * sizeof(double));
+        hipLaunchKernelGGL(k_jsd_g, dim3((uint32_t)cdiv((uint64_t)U * k, 4)), dim3(256), 0, st, CT, kpad,
+                           (uint32_t)k, d, lis, U, Gb);
+        G = Gb;
+        slot = ds.li_slot.as<const uint32_t>();
+        HIP_CHECK(hipStreamSynchronize(st));  // lis staging
+    }
+    const dim3 grid((uint32_t)cdiv(n, 4)), blk(256);
+    const float fpv = (float)pv, lpv = (float)log(pv > 0 ? pv : 1.);
+    switch (msr) {
+    case MCG_SQRL2:
+        hipLaunchKernelGGL(k_row_costs<MCG_SQRL2>, grid, blk, 0, st, X, val, idx, ptr, n, d, fpv, lpv, rt, CT, kpad,
+                           (uint32_t)k, cs, G, slot, out_nk, asn, best);
+        break;
+    case MCG_MKL:
+        hipLaunchKernelGGL(k_row_costs<MCG_MKL>, grid, blk, 0, st, X, val, idx, ptr, n, d, fpv, lpv, rt, CT, kpad,
+                           (uint32_t)k, cs, G, slot, out_nk, asn, best);
+        break;
+    case MCG_REVERSE_MKL:
+        hipLaunchKernelGGL(k_row_costs<MCG_REVERSE_MKL>, grid, blk, 0, st, X, val, idx, ptr, n, d, fpv, lpv, rt, CT,
+                           kpad, (uint32_t)k, cs, G, slot, out_nk, asn, best);
+        break;
+    default:
+        hipLaunchKernelGGL(k_row_costs<MCG_JSD>, grid, blk, 0, st, X, val, idx, ptr, n, d, fpv, lpv, rt, CT, kpad,
+                           (uint32_t)k, cs, G, slot, out_nk, asn, best);
+        break;
+    }
+}
+
+}  // namespace mcg

@@ -1,0 +1,190 @@
+// Internal declarations shared by the libmcg translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+#include "../../include/mcg.h"
+
+namespace mcg {
+
+// ---- error plumbing -------------------------------------------------------
+void set_error(const std::string &msg);
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define MCG_THROW(code, msg) throw ::mcg::Error{(code), (msg)}
+#define HIP_CHECK(expr)                                                                      \
+    do {                                                                                     \
+        hipError_t _e = (expr);                                                              \
+        if (_e != hipSuccess)                                                                \
+            MCG_THROW(_e == hipErrorOutOfMemory ? MCG_ENOMEM : MCG_EDEVICE,                  \
+                      std::string(#expr) + ": " + hipGetErrorString(_e));                    \
+    } while (0)
+#define REQUIRE(cond, msg) do { if (!(cond)) MCG_THROW(MCG_EINVAL, (msg)); } while (0)
+
+// ---- device buffers -------------------------------------------------------
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release() { if (p) (void)hipFree(p); p = nullptr; bytes = 0; }
+    // grow-only; contents are not preserved
+    void *reserve(size_t nb) {
+        if (nb > bytes) {
+            release();
+            HIP_CHECK(hipMalloc(&p, nb ? nb : 1));
+            bytes = nb;
+        }
+        return p;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Host-or-device classification of a caller pointer.
+bool is_device_ptr(const void *p);
+
+struct Ctx;
+
+// A caller buffer made available on the device: borrowed when it already is
+// device memory, else staged through a context scratch buffer.
+struct DevView {
+    const void *dptr = nullptr;
+    DBuf staged;
+};
+
+// ---- dataset ----------------------------------------------------------------
+struct Data {
+    Ctx *ctx = nullptr;
+    bool csr = false;
+    uint64_t n = 0, d = 0, nnz = 0;
+    // dense rows (float32, row-major, stride d)
+    float *X = nullptr;
+    DBuf Xown;
+    // CSR rows (canonical: f32 data, u32 indices, u64 indptr)
+    DBuf val, idx, ptr;
+    // per-row FP64 sums (csc.h:1008-1016 / blaze sum<rowwise>)
+    DBuf rowsum;
+    // BF16 copy for the MFMA cross term: rows centered on the column mean mu,
+    // padded to dpad columns; ||x - mu|| per row; max over rows.
+    bool bf16_ready = false;
+    uint32_t dpad = 0;
+    DBuf Xb, mu, xnorm;
+    float xmax = 0.f;
+    // per-prior CSR transforms (cached for the last prior seen)
+    double tf_prior = -1.;
+    DBuf tf_logx;      // log1p(x / pv) per stored value (MKL)
+    DBuf rowinfo;      // per-row float4 {L, log L, Hx, li}
+    std::vector<float> li_unique;  // distinct li values (JSD)
+    DBuf li_slot;      // per-row index into li_unique
+    // column-major copy of the CSR rows (soft M-step), built on first use
+    DBuf csc_ptr, csc_row, csc_val;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // named scratch buffers
+    DBuf s[48];
+    DBuf pinned_dummy;
+    void *host_pinned = nullptr;
+    size_t host_pinned_bytes = 0;
+    void *pinned(size_t nb);
+    // kernel timing: pending (start, stop) event pairs per timed kernel
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[2];
+    double timed_ms[2] = {0., 0.};
+    uint64_t timed_n[2] = {0, 0};
+    void tic(int which, hipEvent_t &a);
+    void toc(int which, hipEvent_t a);
+    void collect();   // after a stream synchronize: fold pending pairs into the totals
+    ~Ctx();
+};
+
+// scratch slot ids
+enum Slot {
+    S_CB = 0, S_CADD, S_CNORM, S_PARAMS, S_FLAGS, S_FLAGCNT, S_ASN_A, S_ASN_B,
+    S_HIST, S_BASE, S_PERM, S_CNT, S_ITEMOFF, S_PART, S_PARTW, S_PARTQ,
+    S_CCOST, S_TOTAL, S_CENT_A, S_CENT_B, S_CENT_C, S_COSTS, S_CT, S_CPARAM,
+    S_MISC0, S_MISC1, S_MISC2, S_MISC3, S_W, S_STAGE0, S_STAGE1, S_STAGE2,
+    S_LIS, S_TOT, S_JSDG, S_NSLOTS
+};
+static_assert(S_NSLOTS <= 48, "scratch slots");
+
+// ---- kernels (launchers) ----------------------------------------------------
+// E-step parameters for the BF16 near-tie test, written on the device.
+struct EParams {
+    float off;      // added to every approximate value to keep it positive
+    float cmax;     // max_c ||c - mu||
+    float tau1;     // coefficient of ||x - mu|| in the tie threshold
+    float tau0;     // absolute part of the tie threshold
+    float taur;     // relative part (of the approximate best distance)
+    float pad[3];
+};
+
+void prepare_bf16(Data &ds);  // builds Xb, mu, xnorm, xmax (once per dataset)
+// Dense SQRL2 hard assignment into asn (device), by precision.
+void dense_sqrl2_assign(Data &ds, const float *C, uint64_t k, int precision, uint32_t *asn,
+                        hipStream_t st);
+// Exact per-point costs ||x_i - c_asn(i)||^2 (FP32 sequential FMA) into costs (device).
+void dense_sqrl2_costs(Data &ds, const float *C, const uint32_t *asn, float *costs, hipStream_t st);
+// Dense SQRL2 assignment of every point against every center, FP32 sequential FMA.
+void dense_sqrl2_assign_exact(Data &ds, const float *C, uint64_t k, uint32_t *asn, float *costs,
+                              hipStream_t st);
+
+// M-step: stable counting sort by asn, FP64 segmented sums; new centers and
+// per-cluster costs against Cold.  Writes counts (u32[k]) and total cost
+// (f64 at S_TOTAL) on the device.
+struct MStepOut {
+    uint32_t *counts;
+    double *total;   // [0] = cost vs Cold (SQRL2 dense only), [1] = total weight
+};
+MStepOut dense_mstep(Data &ds, const uint32_t *asn, const double *w, uint64_t k, const float *Cold,
+                     float *Cnew, bool want_cost, hipStream_t st);
+MStepOut csr_mstep(Data &ds, const uint32_t *asn, const double *w, uint64_t k, float *Cnew,
+                   hipStream_t st);
+
+// Generic-measure kernels (dense or CSR rows vs dense centers), FP32.
+void generic_costs(Data &ds, int msr, double prior, const float *C, uint64_t k, float *out_nk,
+                   uint32_t *asn, float *best, hipStream_t st);
+
+// Soft EM pieces.
+void softmax_rows(const float *costs, uint64_t n, uint64_t k, float temp, float *resp,
+                  const double *w, double *rowdot, hipStream_t st);
+void soft_mstep(Data &ds, const float *resp, const double *w, uint64_t k, float *Cnew,
+                hipStream_t st);
+
+// Sampling.
+void kmeanspp_run(Data &ds, int msr, double prior, uint64_t k, uint64_t seed, const double *w,
+                  uint32_t *centers_h, uint32_t *asn_d, float *dist_d, hipStream_t st);
+void coreset_probs_run(Ctx &ctx, int sens, uint64_t n, uint64_t ncenters, const double *costs_d,
+                       const uint32_t *asn_d, const double *w_d, float *probs_d, hipStream_t st);
+void coreset_sample_run(Ctx &ctx, uint64_t n, const float *probs_d, const double *w_d, uint64_t m,
+                        uint64_t seed, uint64_t *idx_d, float *wout_d, hipStream_t st);
+void coreset_sample_from(Ctx &ctx, uint64_t n, const float *probs_d, const double *w_d, uint64_t m,
+                         uint64_t seed, uint64_t first, uint64_t *idx_d, float *wout_d, hipStream_t st);
+void restart_draws(Ctx &ctx, const float *costs_d, uint64_t n, uint64_t nempty, uint64_t *rows_h,
+                   hipStream_t st);
+
+// Small helpers.
+void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
+void copy_rows_to_centers(Data &ds, const uint64_t *rows_h, const uint32_t *ids_h, uint64_t m,
+                          float *C, hipStream_t st);
+bool any_nan(const float *p, uint64_t n, hipStream_t st, Ctx &ctx);
+void rowsums(Data &ds, hipStream_t st);
+// CSR per-prior transforms
+void csr_transforms(Data &ds, double prior, hipStream_t st);
+
+inline uint64_t cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+}  // namespace mcg
+
+struct mcg_ctx { mcg::Ctx c; };
+struct mcg_data { mcg::Data d; };

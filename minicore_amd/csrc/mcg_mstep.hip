@@ -1,0 +1,424 @@
+// Hard M-step: weighted means of each cluster's rows, FP64, deterministic, no
+// floating-point atomics.
+//
+// Replaces set_centroids_full_mean / set_center (include/minicore/clustering/
+// centroid.h:649-779 and 81-178): the reference collects each cluster's members
+// under per-center mutexes, radix-sorts them and sums in data precision
+// (dense, blaze::mean) or with OMP atomics into a double buffer (CSR).
+//
+// Here: (1) a stable counting sort of point ids by assignment (per-block
+// histograms, an exclusive scan, an in-order scatter), so each cluster's
+// members are contiguous and ascending; (2) the members are cut into chunks of
+// MCHUNK, each chunk summed sequentially in FP64 by one block (a thread per
+// dim), and (3) chunk sums are added in order.  The association is fixed, so
+// results are bit-reproducible and equal the oracle's restatement.
+#include "mcg_internal.h"
+#include <cfloat>
+
+namespace mcg {
+
+constexpr uint32_t SORT_BLK = 1024;  // points per histogram block
+constexpr uint32_t MCHUNK = 256;     // members per partial-sum chunk
+constexpr uint32_t MAX_SORT_K = 16384;
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st) {
+    if (!n) return;
+    uint32_t g = (uint32_t)std::min<uint64_t>(cdiv(n, 256), 4096);
+    hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(256), 0, st, p, v, n);
+}
+
+// Points whose assignment is >= k (never produced by the E-step kernels) are
+// left out of every cluster.
+__global__ void k_hist(const uint32_t *__restrict__ asn, uint64_t n, uint32_t nblk, uint32_t k,
+                       uint32_t *__restrict__ hist) {
+    const uint64_t b = blockIdx.x;
+    const uint64_t i0 = b * SORT_BLK;
+    for (uint32_t q = threadIdx.x; q < SORT_BLK; q += blockDim.x) {
+        uint64_t i = i0 + q;
+        if (i < n) {
+            const uint32_t a = asn[i];
+            if (a < k) atomicAdd(&hist[(uint64_t)a * nblk + b], 1u);
+        }
+    }
+}
+
+// --- exclusive scan of u32 (length L), three phases ------------------------
+constexpr uint32_t SCAN_PER = 16;  // elements per thread
+__global__ void k_scan_reduce(const uint32_t *__restrict__ in, uint64_t L, uint32_t *__restrict__ bsum) {
+    __shared__ uint32_t sm[256];
+    const uint64_t base = (uint64_t)blockIdx.x * 256 * SCAN_PER;
+    uint32_t s = 0;
+    for (uint32_t q = 0; q < SCAN_PER; ++q) {
+        uint64_t i = base + (uint64_t)q * 256 + threadIdx.x;
+        if (i < L) s += in[i];
+    }
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = sm[0];
+}
+
+// single block, exclusive scan in place of m values (m arbitrary); total -> *tot
+__global__ void k_scan_single(uint32_t *__restrict__ v, uint64_t m, uint32_t *__restrict__ tot) {
+    __shared__ uint32_t sm[1024];
+    const uint64_t per = (m + 1023) / 1024;
+    const uint64_t b = threadIdx.x * per, e = min<uint64_t>(m, b + per);
+    uint32_t s = 0;
+    for (uint64_t i = b; i < e; ++i) s += v[i];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        uint32_t t = threadIdx.x >= (unsigned)o ? sm[threadIdx.x - o] : 0u;
+        __syncthreads();
+        sm[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? sm[threadIdx.x - 1] : 0u;
+    for (uint64_t i = b; i < e; ++i) { uint32_t t = v[i]; v[i] = run; run += t; }
+    if (threadIdx.x == 1023 && tot) *tot = sm[1023];
+}
+
+__global__ void k_scan_apply(const uint32_t *__restrict__ in, uint64_t L, const uint32_t *__restrict__ boff,
+                             uint32_t *__restrict__ out) {
+    __shared__ uint32_t sm[256];
+    const uint64_t base = (uint64_t)blockIdx.x * 256 * SCAN_PER;
+    // thread t owns elements [base + t*SCAN_PER, +SCAN_PER)
+    const uint64_t b = base + (uint64_t)threadIdx.x * SCAN_PER;
+    uint32_t s = 0;
+    for (uint32_t q = 0; q < SCAN_PER; ++q) if (b + q < L) s += in[b + q];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        uint32_t t = threadIdx.x >= (unsigned)o ? sm[threadIdx.x - o] : 0u;
+        __syncthreads();
+        sm[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = boff[blockIdx.x] + (threadIdx.x ? sm[threadIdx.x - 1] : 0u);
+    for (uint32_t q = 0; q < SCAN_PER; ++q) {
+        if (b + q < L) { uint32_t t = in[b + q]; out[b + q] = run; run += t; }
+    }
+}
+
+static void scan_exclusive(const uint32_t *in, uint32_t *out, uint64_t L, DBuf &tmp, hipStream_t st) {
+    const uint64_t per_blk = 256ull * SCAN_PER;
+    const uint64_t nb = cdiv(L, per_blk);
+    tmp.reserve((nb + 1) * sizeof(uint32_t));
+    hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(256), 0, st, in, L, tmp.as<uint32_t>());
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, tmp.as<uint32_t>(), nb, (uint32_t *)nullptr);
+    hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)nb), dim3(256), 0, st, in, L, tmp.as<const uint32_t>(), out);
+}
+
+// counts[c] = sum_b hist[c][b]; off[c] = base[c][0]; items[c] = ceil(counts/MCHUNK)
+__global__ void k_counts(const uint32_t *__restrict__ hist, const uint32_t *__restrict__ base, uint64_t k,
+                         uint32_t nblk, uint32_t *__restrict__ counts, uint32_t *__restrict__ off,
+                         uint32_t *__restrict__ items) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= k) return;
+    const uint32_t b0 = base[c * nblk];
+    const uint32_t last = base[c * nblk + nblk - 1] + hist[c * nblk + nblk - 1];
+    counts[c] = last - b0;
+    off[c] = b0;
+    items[c] = (last - b0 + MCHUNK - 1) / MCHUNK;
+}
+
+// Stable scatter: one wave per histogram block, LDS cursors for all clusters.
+__global__ void k_scatter(const uint32_t *__restrict__ asn, uint64_t n, uint32_t nblk, uint32_t k,
+                          const uint32_t *__restrict__ base, uint32_t *__restrict__ perm) {
+    extern __shared__ uint32_t cur[];
+    __shared__ uint32_t keys[64];
+    const uint32_t b = blockIdx.x;
+    const int lane = threadIdx.x;
+    const uint64_t i0 = (uint64_t)b * SORT_BLK;
+    for (uint32_t q = lane; q < SORT_BLK; q += 64) {
+        uint64_t i = i0 + q;
+        if (i < n) { uint32_t a = asn[i]; if (a < k) cur[a] = base[(uint64_t)a * nblk + b]; }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    for (uint32_t q0 = 0; q0 < SORT_BLK; q0 += 64) {
+        const uint64_t i = i0 + q0 + lane;
+        const uint32_t a = i < n ? asn[i] : 0xFFFFFFFFu;
+        const bool valid = a < k;
+        const uint32_t key = valid ? a : 0xFFFFFFFFu;
+        keys[lane] = key;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        uint32_t rank = 0, tot = 0;
+        for (int j = 0; j < 64; ++j) {
+            const uint32_t kj = keys[j];
+            const bool eq = kj == key;
+            rank += (eq && j < lane) ? 1u : 0u;
+            tot += eq ? 1u : 0u;
+        }
+        uint32_t pos = 0;
+        if (valid) { pos = cur[key] + rank; perm[pos] = (uint32_t)i; }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        if (valid && rank + 1 == tot) cur[key] = pos + 1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+}
+
+// Partial sums: one block per (cluster, chunk) item; a thread per dim.
+// part[item*d + j] = sum_m w*x (FP64, sequential); partw[item] = sum_m w;
+// partq[item] = sum_m w * ||x - cold||^2 (FP64) when cold != nullptr.
+__global__ __launch_bounds__(256) void k_partial(const float *__restrict__ X, uint32_t d,
+                                                 const uint32_t *__restrict__ perm,
+                                                 const uint32_t *__restrict__ counts,
+                                                 const uint32_t *__restrict__ off,
+                                                 const uint32_t *__restrict__ itemoff, uint32_t k,
+                                                 const double *__restrict__ w, const float *__restrict__ cold,
+                                                 double *__restrict__ part, double *__restrict__ partw,
+                                                 double *__restrict__ partq) {
+    __shared__ uint32_t mem[MCHUNK];
+    __shared__ double wl[MCHUNK];
+    __shared__ double red[256];
+    const uint32_t item = blockIdx.x;
+    const uint32_t total_items = itemoff[k];
+    if (item >= total_items) return;
+    // find cluster c with itemoff[c] <= item < itemoff[c+1]
+    uint32_t lo = 0, hi = k;
+    while (hi - lo > 1) { uint32_t mid = (lo + hi) >> 1; if (itemoff[mid] <= item) lo = mid; else hi = mid; }
+    const uint32_t c = lo;
+    const uint32_t chunk = item - itemoff[c];
+    const uint32_t mb = off[c] + chunk * MCHUNK;
+    const uint32_t me = min(off[c] + counts[c], mb + MCHUNK);
+    const uint32_t nm = me - mb;
+    for (uint32_t q = threadIdx.x; q < nm; q += blockDim.x) {
+        uint32_t pnt = perm[mb + q];
+        mem[q] = pnt;
+        wl[q] = w ? w[pnt] : 1.;
+    }
+    __syncthreads();
+    double qsum = 0.;
+    for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) {
+        double s = 0.;
+        const double cj = cold ? (double)cold[(uint64_t)c * d + j] : 0.;
+        if (w) {
+            for (uint32_t q = 0; q < nm; ++q) {
+                const double x = (double)X[(uint64_t)mem[q] * d + j];
+                s = __fma_rn(wl[q], x, s);
+                if (cold) { double t = x - cj; qsum += wl[q] * (t * t); }
+            }
+        } else {
+            for (uint32_t q = 0; q < nm; ++q) {
+                const double x = (double)X[(uint64_t)mem[q] * d + j];
+                s = __dadd_rn(s, x);
+                if (cold) { double t = x - cj; qsum += t * t; }
+            }
+        }
+        part[(uint64_t)item * d + j] = s;
+    }
+    if (threadIdx.x == 0) {
+        double ws = 0.;
+        for (uint32_t q = 0; q < nm; ++q) ws = __dadd_rn(ws, wl[q]);
+        partw[item] = ws;
+    }
+    if (cold) {
+        red[threadIdx.x] = qsum;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partq[item] = red[0];
+    }
+}
+
+// One block per cluster: add chunk partials in order; mean = sum / weight.
+__global__ __launch_bounds__(256) void k_finalize(uint32_t d, const uint32_t *__restrict__ counts,
+                                                  const uint32_t *__restrict__ itemoff,
+                                                  const double *__restrict__ part,
+                                                  const double *__restrict__ partw,
+                                                  const double *__restrict__ partq, const float *__restrict__ cold,
+                                                  float *__restrict__ cnew, double *__restrict__ ccost,
+                                                  double *__restrict__ cw) {
+    const uint32_t c = blockIdx.x;
+    const uint32_t i0 = itemoff[c], i1 = itemoff[c + 1];
+    double wt = 0.;
+    for (uint32_t it = i0; it < i1; ++it) wt = __dadd_rn(wt, partw[it]);
+    for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) {
+        if (counts[c] == 0) {
+            if (cold && cnew != cold) cnew[(uint64_t)c * d + j] = cold[(uint64_t)c * d + j];
+            continue;
+        }
+        double s = 0.;
+        for (uint32_t it = i0; it < i1; ++it) s = __dadd_rn(s, part[(uint64_t)it * d + j]);
+        cnew[(uint64_t)c * d + j] = (float)__ddiv_rn(s, wt);
+    }
+    if (threadIdx.x == 0) {
+        double q = 0.;
+        if (partq) for (uint32_t it = i0; it < i1; ++it) q += partq[it];
+        ccost[c] = q;
+        cw[c] = wt;
+    }
+}
+
+// Single block: total[0] = sum_c ccost[c], total[1] = sum_c cw[c] (fixed order);
+// total[2] (as u32 bit pattern in a double slot) unused.
+__global__ void k_total(const double *__restrict__ ccost, const double *__restrict__ cw, uint64_t k,
+                        double *__restrict__ total) {
+    __shared__ double a[256], b[256];
+    double s = 0., t = 0.;
+    for (uint64_t i = threadIdx.x; i < k; i += blockDim.x) { s += ccost[i]; t += cw[i]; }
+    a[threadIdx.x] = s;
+    b[threadIdx.x] = t;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) { a[threadIdx.x] += a[threadIdx.x + o]; b[threadIdx.x] += b[threadIdx.x + o]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { total[0] = a[0]; total[1] = b[0]; }
+}
+
+struct SortOut {
+    uint32_t *perm, *counts, *off, *itemoff;
+};
+
+static SortOut sort_by_cluster(Ctx &c, const uint32_t *asn, uint64_t n, uint64_t k, hipStream_t st) {
+    REQUIRE(k <= MAX_SORT_K, "k must be <= 16384 for the GPU M-step");
+    REQUIRE(n < (1ull << 32) - 1, "n must fit in uint32");
+    const uint32_t nblk = (uint32_t)cdiv(n, SORT_BLK);
+    const uint64_t L = (uint64_t)k * nblk;
+    uint32_t *hist = (uint32_t *)c.s[S_HIST].reserve(L * sizeof(uint32_t));
+    uint32_t *base = (uint32_t *)c.s[S_BASE].reserve(L * sizeof(uint32_t));
+    uint32_t *perm = (uint32_t *)c.s[S_PERM].reserve(n * sizeof(uint32_t));
+    uint32_t *cnt = (uint32_t *)c.s[S_CNT].reserve(3 * (k + 1) * sizeof(uint32_t));
+    uint32_t *off = cnt + (k + 1), *items = cnt + 2 * (k + 1);
+    fill_u32(hist, 0, L, st);
+    hipLaunchKernelGGL(k_hist, dim3(nblk), dim3(256), 0, st, asn, n, nblk, (uint32_t)k, hist);
+    scan_exclusive(hist, base, L, c.s[S_MISC0], st);
+    hipLaunchKernelGGL(k_counts, dim3((uint32_t)cdiv(k, 256)), dim3(256), 0, st, hist, base, k, nblk, cnt, off,
+                       items);
+    // items -> exclusive prefix (k+1 entries: the last one becomes the total)
+    fill_u32(items + k, 0, 1, st);
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, items, k + 1, (uint32_t *)nullptr);
+    hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(64), (size_t)k * sizeof(uint32_t), st, asn, n, nblk,
+                       (uint32_t)k, base, perm);
+    return SortOut{perm, cnt, off, items};
+}
+
+MStepOut dense_mstep(Data &ds, const uint32_t *asn, const double *w, uint64_t k, const float *Cold,
+                     float *Cnew, bool want_cost, hipStream_t st) {
+    Ctx &c = *ds.ctx;
+    const uint64_t n = ds.n;
+    const uint32_t d = (uint32_t)ds.d;
+    SortOut so = sort_by_cluster(c, asn, n, k, st);
+    const uint64_t max_items = n / MCHUNK + k + 1;
+    double *part = (double *)c.s[S_PART].reserve(max_items * d * sizeof(double));
+    double *partw = (double *)c.s[S_PARTW].reserve(max_items * sizeof(double));
+    double *partq = (double *)c.s[S_PARTQ].reserve(max_items * sizeof(double));
+    double *ccost = (double *)c.s[S_CCOST].reserve(2 * k * sizeof(double));
+    double *total = (double *)c.s[S_TOTAL].reserve(4 * sizeof(double));
+    hipEvent_t ev;
+    c.tic(1, ev);
+    hipLaunchKernelGGL(k_partial, dim3((uint32_t)max_items), dim3(256), 0, st, ds.X, d, so.perm, so.counts, so.off,
+                       so.itemoff, (uint32_t)k, w, want_cost ? Cold : nullptr, part, partw, partq);
+    c.toc(1, ev);
+    hipLaunchKernelGGL(k_finalize, dim3((uint32_t)k), dim3(256), 0, st, d, so.counts, so.itemoff, part, partw,
+                       want_cost ? partq : nullptr, Cold, Cnew, ccost, ccost + k);
+    hipLaunchKernelGGL(k_total, dim3(1), dim3(256), 0, st, ccost, ccost + k, k, total);
+    return MStepOut{so.counts, total};
+}
+
+// ---------------------------------------------------------------------------
+// CSR M-step.  After the same sort, one block per (cluster, column window of
+// CSR_CW columns); 4 waves each own an LDS FP64 accumulator and take members
+// q = wave, wave+4, ...; a wave walks its rows in order (row entries have
+// distinct columns, so lanes never collide), and the 4 accumulators are added
+// in a fixed order.
+// ---------------------------------------------------------------------------
+constexpr uint32_t CSR_CW = 2048;
+
+__global__ __launch_bounds__(256) void k_csr_mean(const float *__restrict__ val, const uint32_t *__restrict__ idx,
+                                                  const uint64_t *__restrict__ ptr, uint32_t d,
+                                                  const uint32_t *__restrict__ perm,
+                                                  const uint32_t *__restrict__ counts,
+                                                  const uint32_t *__restrict__ off, const double *__restrict__ w,
+                                                  float *__restrict__ cnew, double *__restrict__ cw) {
+    __shared__ double acc[4][CSR_CW];
+    const uint32_t c = blockIdx.y;
+    const uint32_t col0 = blockIdx.x * CSR_CW;
+    const uint32_t col1 = min(d, col0 + CSR_CW);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (uint32_t j = tid; j < 4 * CSR_CW; j += 256) (&acc[0][0])[j] = 0.;
+    __syncthreads();
+    const uint32_t m0 = off[c], nm = counts[c];
+    for (uint32_t g = wave * 64; g < nm; g += 256) {
+        // lanes find the first entry >= col0 of 64 member rows at once
+        const uint32_t q = g + lane;
+        uint64_t rb = 0, re = 0;
+        double wq = 1.;
+        if (q < nm) {
+            const uint32_t r = perm[m0 + q];
+            uint64_t lo = ptr[r], hi = ptr[r + 1];
+            re = hi;
+            while (lo < hi) { uint64_t mid = (lo + hi) >> 1; if (idx[mid] < col0) lo = mid + 1; else hi = mid; }
+            rb = lo;
+            if (w) wq = w[r];
+        }
+        const uint32_t nrow = min(64u, nm - g);
+        for (uint32_t t = 0; t < nrow; ++t) {
+            const uint64_t b = __shfl(rb, t), e = __shfl(re, t);
+            const double wt = __shfl(wq, t);
+            for (uint64_t p = b + lane; p < e; p += 64) {
+                const uint32_t j = idx[p];
+                if (j >= col1) break;
+                double *a = &acc[wave][j - col0];
+                *a = __dadd_rn(*a, __dmul_rn(wt, (double)val[p]));
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        }
+    }
+    __syncthreads();
+    if (nm == 0) return;
+    const double wt = cw[c];
+    for (uint32_t j = col0 + tid; j < col1; j += 256) {
+        const uint32_t jj = j - col0;
+        double s = __dadd_rn(__dadd_rn(acc[0][jj], acc[1][jj]), __dadd_rn(acc[2][jj], acc[3][jj]));
+        cnew[(uint64_t)c * d + j] = (float)__ddiv_rn(s, wt);
+    }
+}
+
+// per-cluster weight (sequential over members) -> cw[c]
+__global__ void k_cluster_weight(const uint32_t *__restrict__ perm, const uint32_t *__restrict__ counts,
+                                 const uint32_t *__restrict__ off, const double *__restrict__ w, uint64_t k,
+                                 double *__restrict__ cw) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= k) return;
+    double s = 0.;
+    const uint32_t m0 = off[c], nm = counts[c];
+    for (uint32_t q = 0; q < nm; ++q) s = __dadd_rn(s, w ? w[perm[m0 + q]] : 1.);
+    cw[c] = s;
+}
+
+MStepOut csr_mstep(Data &ds, const uint32_t *asn, const double *w, uint64_t k, float *Cnew, hipStream_t st) {
+    Ctx &c = *ds.ctx;
+    SortOut so = sort_by_cluster(c, asn, ds.n, k, st);
+    double *ccost = (double *)c.s[S_CCOST].reserve(2 * k * sizeof(double));
+    double *total = (double *)c.s[S_TOTAL].reserve(4 * sizeof(double));
+    hipLaunchKernelGGL(k_cluster_weight, dim3((uint32_t)cdiv(k, 256)), dim3(256), 0, st, so.perm, so.counts, so.off,
+                       w, k, ccost + k);
+    dim3 g((uint32_t)cdiv(ds.d, CSR_CW), (uint32_t)k);
+    hipLaunchKernelGGL(k_csr_mean, g, dim3(256), 0, st, ds.val.as<const float>(), ds.idx.as<const uint32_t>(),
+                       ds.ptr.as<const uint64_t>(), (uint32_t)ds.d, so.perm, so.counts, so.off, w, Cnew, ccost + k);
+    return MStepOut{so.counts, total};
+}
+
+}  // namespace mcg

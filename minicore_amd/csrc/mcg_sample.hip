@@ -1,0 +1,552 @@
+// D^2 seeding and coreset sampling.
+//
+// Replaces coresets::kmeanspp (include/minicore/optim/kmeans.h:49-215), the
+// empty-cluster restart draw (clustering/centroid.h:687-752) and
+// CoresetSampler::make_probs_* / sample (coreset/coreset.h:340-699).
+//
+// Sampling "i with probability w_i / sum(w)" is an inverse CDF over integers:
+// w_i -> floor(w_i * 2^s) (s from max(w) and n so the total fits in 63 bits),
+// u = mulhi(r, T) for a raw 64-bit draw r, and the answer is the first i whose
+// inclusive prefix sum exceeds u.  Integer sums are exact, so the index is the
+// same for any summation order and equals the host restatement.  Draws come
+// from the wyrand stream in counter form: draw j = wyrand output j for seed s.
+// Sums in the sensitivity formulas are exact fixed-point integer sums for the
+// same reason.
+#include "mcg_internal.h"
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+
+#pragma clang fp contract(off)
+
+namespace mcg {
+
+__host__ __device__ inline uint64_t wyrand_at(uint64_t seed, uint64_t j) {
+    const uint64_t s = seed + (j + 1) * 0xa0761d6478bd642fULL;
+    const uint64_t b = s ^ 0xe7037ed1a0b428dbULL;
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul64hi(s, b) ^ (s * b);
+#else
+    const unsigned __int128 p = (unsigned __int128)s * b;
+    return (uint64_t)(p >> 64) ^ (uint64_t)p;
+#endif
+}
+
+__host__ __device__ inline uint64_t mulhi64(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul64hi(a, b);
+#else
+    return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+__host__ __device__ inline int bitlen64(uint64_t v) {
+    int b = 0;
+    while (v) { ++b; v >>= 1; }
+    return b;
+}
+
+// shift so that n * max * 2^s < 2^62
+__host__ __device__ inline int fixed_shift(double maxw, uint64_t n) {
+    if (!(maxw > 0.) || !(maxw <= DBL_MAX)) return 0;
+    int e;
+    frexp(maxw, &e);
+    return 62 - e - bitlen64(n);
+}
+
+__host__ __device__ inline uint64_t to_fixed(double w, int s) {
+    return w > 0. ? (uint64_t)floor(ldexp(w, s)) : 0ull;
+}
+
+__host__ __device__ inline int64_t to_fixed_signed(double v, int s) {
+    const double t = ldexp(v, s);
+    return (int64_t)(t < 0. ? -floor(-t) : floor(t));
+}
+
+__host__ __device__ inline double from_fixed(int64_t v, int s) { return ldexp((double)v, -s); }
+
+// ---- reductions --------------------------------------------------------------
+constexpr uint32_t FX_BLK = 1024;   // elements per block sum
+
+// max of |a_i| (a_i = w_i * v_i when w != null, v as float or double)
+template <class T>
+__global__ void k_absmax(const T *__restrict__ v, const double *__restrict__ w, uint64_t n,
+                         unsigned long long *__restrict__ out_bits) {
+    __shared__ double sm[256];
+    double m = 0.;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        double a = (double)v[i];
+        if (w) a = w[i] * a;
+        a = fabs(a);
+        if (a > m) m = a;
+    }
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]);
+        __syncthreads();
+    }
+    // non-negative doubles order like their bit patterns
+    if (threadIdx.x == 0) atomicMax(out_bits, (unsigned long long)__double_as_longlong(sm[0]));
+}
+
+// block sums of floor(ldexp(w_i v_i, s)); s derived from *maxbits
+template <class T>
+__global__ void k_fixed_blocksum(const T *__restrict__ v, const double *__restrict__ w, uint64_t n,
+                                 const unsigned long long *__restrict__ maxbits, uint64_t *__restrict__ bsum) {
+    __shared__ unsigned long long sm[256];
+    const int s = fixed_shift(__longlong_as_double((long long)*maxbits), n);
+    const uint64_t b0 = (uint64_t)blockIdx.x * FX_BLK;
+    unsigned long long acc = 0;
+    for (uint32_t q = threadIdx.x; q < FX_BLK; q += blockDim.x) {
+        const uint64_t i = b0 + q;
+        if (i < n) {
+            double a = (double)v[i];
+            if (w) a = w[i] * a;
+            acc += to_fixed(a, s);
+        }
+    }
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = sm[0];
+}
+
+// Single block: choose the index for raw draw r; writes *out.  If the fixed total
+// is 0 the result is 0.
+template <class T>
+__global__ void k_fixed_select(const T *__restrict__ v, const double *__restrict__ w, uint64_t n,
+                               const unsigned long long *__restrict__ maxbits, const uint64_t *__restrict__ bsum,
+                               uint32_t nblk, uint64_t r, uint64_t *__restrict__ out) {
+    __shared__ unsigned long long part[1024];
+    __shared__ uint64_t sel_block, sel_u;
+    const int s = fixed_shift(__longlong_as_double((long long)*maxbits), n);
+    // total and block prefix: thread t owns blocks [t*per, ...)
+    const uint32_t per = (nblk + 1023) / 1024;
+    const uint32_t b0 = threadIdx.x * per, b1 = min(nblk, b0 + per);
+    unsigned long long acc = 0;
+    for (uint32_t b = b0; b < b1; ++b) acc += bsum[b];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        for (int t = 0; t < 1024; ++t) tot += part[t];
+        if (tot == 0) { sel_block = 0xFFFFFFFFFFFFFFFFull; }
+        else {
+            const uint64_t u = mulhi64(r, tot);
+            unsigned long long run = 0;
+            uint64_t blk = 0;
+            for (int t = 0; t < 1024; ++t) {
+                if (run + part[t] > u) {
+                    const uint32_t bb = t * per, be = min(nblk, bb + per);
+                    for (uint32_t b = bb; b < be; ++b) {
+                        if (run + bsum[b] > u) { blk = b; break; }
+                        run += bsum[b];
+                    }
+                    break;
+                }
+                run += part[t];
+            }
+            sel_block = blk;
+            sel_u = u - run;
+        }
+    }
+    __syncthreads();
+    if (sel_block == 0xFFFFFFFFFFFFFFFFull) {
+        if (threadIdx.x == 0) *out = 0;
+        return;
+    }
+    // inside the block: sequential scan by thread 0 over at most FX_BLK elements
+    if (threadIdx.x == 0) {
+        const uint64_t i0 = sel_block * FX_BLK;
+        unsigned long long run = 0;
+        uint64_t ans = min<uint64_t>(n - 1, i0 + FX_BLK - 1);
+        for (uint64_t i = i0; i < min<uint64_t>(n, i0 + FX_BLK); ++i) {
+            double a = (double)v[i];
+            if (w) a = w[i] * a;
+            run += to_fixed(a, s);
+            if (run > sel_u) { ans = i; break; }
+        }
+        *out = ans;
+    }
+}
+
+// Host-side draw from device values (used for the rare restart path).
+static uint64_t draw_host(const std::vector<double> &wv, uint64_t r) {
+    const uint64_t n = wv.size();
+    double mx = 0.;
+    for (double x : wv) if (x > mx) mx = x;
+    const int s = fixed_shift(mx, n);
+    uint64_t T = 0;
+    for (double x : wv) T += to_fixed(x, s);
+    if (T == 0) return 0;
+    const uint64_t u = mulhi64(r, T);
+    uint64_t acc = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        acc += to_fixed(wv[i], s);
+        if (acc > u) return i;
+    }
+    return n - 1;
+}
+
+void restart_draws(Ctx &c, const float *costs_d, uint64_t n, uint64_t nempty, uint64_t *rows_h, hipStream_t st) {
+    std::vector<float> cf(n);
+    HIP_CHECK(hipMemcpyAsync(cf.data(), costs_d, n * 4, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    std::vector<double> wv(cf.begin(), cf.end());
+    for (uint64_t e = 0; e < nempty; ++e) rows_h[e] = draw_host(wv, wyrand_at(n, e));
+    (void)c;
+}
+
+// ---- D^2 seeding ---------------------------------------------------------------
+// dist update for dense SQRL2: sequential FMA over dims of (x_i - x_c)^2.
+__global__ __launch_bounds__(256) void k_pp_update_l2(const float *__restrict__ X, uint64_t n, uint32_t d,
+                                                      const uint64_t *__restrict__ newc_p, uint32_t ci,
+                                                      float *__restrict__ dist, uint32_t *__restrict__ asn,
+                                                      int first) {
+    __shared__ float xs[256][33];
+    __shared__ float cs[32];
+    const int tid = threadIdx.x;
+    const uint64_t newc = *newc_p;
+    const uint64_t p0 = (uint64_t)blockIdx.x * 256, p = p0 + tid;
+    const float *cr = X + newc * d;
+    float s = 0.f;
+    for (uint32_t j0 = 0; j0 < d; j0 += 32) {
+        __syncthreads();
+        for (int q = tid; q < 256 * 32; q += 256) {
+            const int r = q >> 5, jj = q & 31;
+            const uint64_t pr = p0 + r;
+            xs[r][jj] = (pr < n && j0 + jj < d) ? X[pr * d + j0 + jj] : 0.f;
+        }
+        if (tid < 32) cs[tid] = j0 + tid < d ? cr[j0 + tid] : 0.f;
+        __syncthreads();
+        const int jn = (int)min<uint32_t>(32, d - j0);
+        for (int jj = 0; jj < jn; ++jj) { const float t = xs[tid][jj] - cs[jj]; s = fmaf(t, t, s); }
+    }
+    if (p >= n) return;
+    if (first) {
+        dist[p] = (p == newc) ? 0.f : s;
+        asn[p] = 0;
+        return;
+    }
+    if (p == newc) { dist[p] = 0.f; asn[p] = ci; return; }
+    const float dv = dist[p];
+    if (dv <= 0.f) return;
+    if (s < dv) { dist[p] = s; asn[p] = ci; }
+}
+
+// dist update from precomputed costs to the new center (generic measures)
+__global__ void k_pp_update_from(const float *__restrict__ cost, uint64_t n, const uint64_t *__restrict__ newc_p,
+                                 uint32_t ci, float *__restrict__ dist, uint32_t *__restrict__ asn, int first) {
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint64_t newc = *newc_p;
+    const float s = cost[p];
+    if (first) { dist[p] = (p == newc) ? 0.f : s; asn[p] = 0; return; }
+    if (p == newc) { dist[p] = 0.f; asn[p] = ci; return; }
+    const float dv = dist[p];
+    if (dv <= 0.f) return;
+    if (s < dv) { dist[p] = s; asn[p] = ci; }
+}
+
+// if dist[newc] == 0, take the first index of the maximum distance (kmeans.h:179-186)
+__global__ void k_pp_fallback(const float *__restrict__ dist, uint64_t n, uint64_t *__restrict__ newc_p,
+                              uint32_t *__restrict__ centers, uint32_t ci) {
+    __shared__ float bv[1024];
+    __shared__ uint64_t bi[1024];
+    const uint64_t nc = *newc_p;
+    if (dist[nc] != 0.f) {
+        if (threadIdx.x == 0) centers[ci] = (uint32_t)nc;
+        return;
+    }
+    float m = -1.f;
+    uint64_t mi = 0;
+    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x)
+        if (dist[i] > m) { m = dist[i]; mi = i; }
+    bv[threadIdx.x] = m;
+    bi[threadIdx.x] = mi;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            const float ov = bv[threadIdx.x + o];
+            const uint64_t oi = bi[threadIdx.x + o];
+            if (ov > bv[threadIdx.x] || (ov == bv[threadIdx.x] && oi < bi[threadIdx.x])) {
+                bv[threadIdx.x] = ov;
+                bi[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { *newc_p = bi[0]; centers[ci] = (uint32_t)bi[0]; }
+}
+
+__global__ void k_set_u64(uint64_t *p, uint64_t v) { *p = v; }
+
+// one dense center from data row *newc_p (CSR or dense)
+__global__ void k_row_to_dense(const float *__restrict__ X, const float *__restrict__ val,
+                               const uint32_t *__restrict__ idx, const uint64_t *__restrict__ ptr, uint32_t d,
+                               const uint64_t *__restrict__ newc_p, float *__restrict__ out) {
+    const uint64_t r = *newc_p;
+    if (!val) {
+        for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) out[j] = X[r * d + j];
+        return;
+    }
+    for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) out[j] = 0.f;
+    __syncthreads();
+    for (uint64_t p = ptr[r] + threadIdx.x; p < ptr[r + 1]; p += blockDim.x) out[idx[p]] = val[p];
+}
+
+void kmeanspp_run(Data &ds, int msr, double prior, uint64_t k, uint64_t seed, const double *w, uint32_t *centers_h,
+                  uint32_t *asn_d, float *dist_d, hipStream_t st) {
+    Ctx &c = *ds.ctx;
+    const uint64_t n = ds.n;
+    const uint32_t d = (uint32_t)ds.d;
+    const bool l2 = !ds.csr && msr == MCG_SQRL2;
+    const uint32_t nblk = (uint32_t)cdiv(n, FX_BLK);
+    char *misc = (char *)c.s[S_MISC0].reserve(64 + nblk * 8 + k * 4);
+    uint64_t *newc = (uint64_t *)misc;
+    unsigned long long *mx = (unsigned long long *)(misc + 8);
+    uint64_t *bsum = (uint64_t *)(misc + 64);
+    uint32_t *cent = (uint32_t *)(misc + 64 + nblk * 8);
+    float *row = nullptr, *cost1 = nullptr;
+    uint32_t *asn1 = nullptr;
+    if (!l2) {
+        row = (float *)c.s[S_STAGE0].reserve(d * sizeof(float));
+        cost1 = (float *)c.s[S_STAGE1].reserve(n * sizeof(float));
+        asn1 = (uint32_t *)c.s[S_FLAGS].reserve(n * sizeof(uint32_t));
+    }
+    auto update = [&](uint32_t ci, int first) {
+        if (l2) {
+            hipLaunchKernelGGL(k_pp_update_l2, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, ds.X, n, d, newc, ci,
+                               dist_d, asn_d, first);
+        } else {
+            hipLaunchKernelGGL(k_row_to_dense, dim3(1), dim3(256), 0, st, ds.X,
+                               ds.csr ? ds.val.as<const float>() : nullptr,
+                               ds.csr ? ds.idx.as<const uint32_t>() : nullptr,
+                               ds.csr ? ds.ptr.as<const uint64_t>() : nullptr, d, newc, row);
+            generic_costs(ds, msr, prior, row, 1, nullptr, asn1, cost1, st);
+            hipLaunchKernelGGL(k_pp_update_from, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, cost1, n, newc, ci,
+                               dist_d, asn_d, first);
+        }
+    };
+    uint64_t j = 0;
+    const uint64_t fc = wyrand_at(seed, j++) % n;
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, newc, fc);
+    update(0, 1);
+    centers_h[0] = (uint32_t)fc;
+    fill_u32(cent, (uint32_t)fc, 1, st);
+    for (uint64_t ci = 1; ci < k; ++ci) {
+        const uint64_t r = wyrand_at(seed, j++);
+        fill_u32((uint32_t *)mx, 0, 2, st);
+        hipLaunchKernelGGL(k_absmax<float>, dim3((uint32_t)std::min<uint64_t>(cdiv(n, 256), 1024)), dim3(256), 0, st,
+                           (const float *)dist_d, w, n, mx);
+        hipLaunchKernelGGL(k_fixed_blocksum<float>, dim3(nblk), dim3(256), 0, st, (const float *)dist_d, w, n,
+                           (const unsigned long long *)mx, bsum);
+        hipLaunchKernelGGL(k_fixed_select<float>, dim3(1), dim3(1024), 0, st, (const float *)dist_d, w, n,
+                           (const unsigned long long *)mx, (const uint64_t *)bsum, nblk, r, newc);
+        hipLaunchKernelGGL(k_pp_fallback, dim3(1), dim3(1024), 0, st, (const float *)dist_d, n, newc, cent,
+                           (uint32_t)ci);
+        update((uint32_t)ci, 0);
+    }
+    HIP_CHECK(hipMemcpyAsync(centers_h, cent, k * 4, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+}
+
+// ---- coreset sensitivities ----------------------------------------------------
+struct CsSums {
+    unsigned long long maxw, maxc;           // |w|, |w*cost| maxima (bit patterns)
+    unsigned long long wtot, ctot;           // fixed-point totals (two's complement)
+    unsigned long long maxp, ptot;
+};
+
+// wf_i = w ? (double)(float)w_i : 1 (weights are stored as float in CoresetSampler<float>)
+__device__ __forceinline__ double wf_of(const double *w, uint64_t i) { return w ? (double)(float)w[i] : 1.; }
+
+__global__ void k_cs_max(const double *__restrict__ cost, const double *__restrict__ w, uint64_t n,
+                         CsSums *__restrict__ S) {
+    __shared__ double a[256], b[256];
+    double ma = 0., mb = 0.;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const double wf = wf_of(w, i);
+        ma = fmax(ma, fabs(wf));
+        mb = fmax(mb, fabs(wf * cost[i]));
+    }
+    a[threadIdx.x] = ma;
+    b[threadIdx.x] = mb;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            a[threadIdx.x] = fmax(a[threadIdx.x], a[threadIdx.x + o]);
+            b[threadIdx.x] = fmax(b[threadIdx.x], b[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicMax(&S->maxw, (unsigned long long)__double_as_longlong(a[0]));
+        atomicMax(&S->maxc, (unsigned long long)__double_as_longlong(b[0]));
+    }
+}
+
+__global__ void k_cs_sums(const double *__restrict__ cost, const uint32_t *__restrict__ asn,
+                          const double *__restrict__ w, uint64_t n, CsSums *__restrict__ S,
+                          unsigned long long *__restrict__ ws, unsigned long long *__restrict__ cs,
+                          unsigned long long *__restrict__ cnt) {
+    const int sw = fixed_shift(__longlong_as_double((long long)S->maxw), n);
+    const int sc = fixed_shift(__longlong_as_double((long long)S->maxc), n);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const double wf = wf_of(w, i);
+        const int64_t a = to_fixed_signed(wf, sw), b = to_fixed_signed(wf * cost[i], sc);
+        const uint32_t c = asn[i];
+        atomicAdd(&ws[c], (unsigned long long)a);
+        atomicAdd(&cs[c], (unsigned long long)b);
+        atomicAdd(&cnt[c], 1ull);
+        atomicAdd(&S->wtot, (unsigned long long)a);
+        atomicAdd(&S->ctot, (unsigned long long)b);
+    }
+}
+
+// unnormalised sensitivities p_i (double), plus max |p|
+__global__ void k_cs_sens(int sens, const double *__restrict__ cost, const uint32_t *__restrict__ asn,
+                          const double *__restrict__ w, uint64_t n, double alpha, CsSums *__restrict__ S,
+                          const unsigned long long *__restrict__ ws, const unsigned long long *__restrict__ cs,
+                          const unsigned long long *__restrict__ cnt, double *__restrict__ p, uint64_t nfl,
+                          float *__restrict__ probs_fl) {
+    __shared__ double red[256];
+    const int sw = fixed_shift(__longlong_as_double((long long)S->maxw), n);
+    const int sc = fixed_shift(__longlong_as_double((long long)S->maxc), n);
+    const double W = from_fixed((int64_t)S->wtot, sw), TC = from_fixed((int64_t)S->ctot, sc);
+    double m = 0.;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t a = asn[i];
+        const double wf = wf_of(w, i);
+        const double wc = wf * cost[i];
+        double v;
+        if (sens == MCG_SENS_LBK) {
+            const double mean = TC / W, tcinv = alpha / mean, alpha2 = 2. * alpha;
+            const double Wc = from_fixed((int64_t)ws[a], sw), Sc = from_fixed((int64_t)cs[a], sc);
+            const double adj = alpha2 * Sc / (Wc * mean) + 4. * W / Wc;
+            v = tcinv * cost[i] + adj;
+        } else if (sens == MCG_SENS_VX) {
+            v = wc / TC + 1. / (double)cnt[a];
+        } else if (sens == MCG_SENS_BFL) {
+            const double Wc = from_fixed((int64_t)ws[a], sw);
+            v = .5 * (wc / TC + wf / (Wc * (double)cnt[a]));
+        } else {  // FL: not normalised
+            probs_fl[i] = w ? (float)(wc / TC) : (float)(ceil((double)nfl * cost[i] / TC) + 1.);
+            continue;
+        }
+        p[i] = v;
+        m = fmax(m, fabs(v));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(&S->maxp, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+__global__ void k_cs_psum(const double *__restrict__ p, uint64_t n, CsSums *__restrict__ S) {
+    const int s = fixed_shift(__longlong_as_double((long long)S->maxp), n);
+    __shared__ unsigned long long red[256];
+    unsigned long long acc = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        acc += (unsigned long long)to_fixed_signed(p[i], s);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(&S->ptot, red[0]);
+}
+
+__global__ void k_cs_norm(const double *__restrict__ p, uint64_t n, const CsSums *__restrict__ S,
+                          float *__restrict__ probs) {
+    const int s = fixed_shift(__longlong_as_double((long long)S->maxp), n);
+    const double P = from_fixed((int64_t)S->ptot, s);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        probs[i] = (float)(p[i] / P);
+}
+
+void coreset_probs_run(Ctx &c, int sens, uint64_t n, uint64_t ncenters, const double *cost, const uint32_t *asn,
+                       const double *w, float *probs, hipStream_t st) {
+    REQUIRE(sens == MCG_SENS_LBK || sens == MCG_SENS_VX || sens == MCG_SENS_BFL || sens == MCG_SENS_FL,
+            "sensitivity method must be BFL (0), FL (1), VX (3) or LBK (4)");
+    const uint64_t bytes = sizeof(CsSums) + 3 * ncenters * 8;
+    char *buf = (char *)c.s[S_MISC0].reserve(bytes);
+    HIP_CHECK(hipMemsetAsync(buf, 0, bytes, st));
+    CsSums *S = (CsSums *)buf;
+    unsigned long long *ws = (unsigned long long *)(buf + sizeof(CsSums));
+    unsigned long long *cs = ws + ncenters, *cnt = cs + ncenters;
+    double *p = (double *)c.s[S_STAGE0].reserve(n * 8);
+    const uint32_t g = (uint32_t)std::min<uint64_t>(cdiv(n, 256), 2048);
+    hipLaunchKernelGGL(k_cs_max, dim3(g), dim3(256), 0, st, cost, w, n, S);
+    hipLaunchKernelGGL(k_cs_sums, dim3(g), dim3(256), 0, st, cost, asn, w, n, S, ws, cs, cnt);
+    const double alpha = 16. * std::log((double)ncenters) + 32.;
+    hipLaunchKernelGGL(k_cs_sens, dim3(g), dim3(256), 0, st, sens, cost, asn, w, n, alpha, S, ws, cs, cnt, p, n,
+                       probs);
+    if (sens == MCG_SENS_FL) return;
+    hipLaunchKernelGGL(k_cs_psum, dim3(g), dim3(256), 0, st, p, n, S);
+    hipLaunchKernelGGL(k_cs_norm, dim3(g), dim3(256), 0, st, p, n, S, probs);
+}
+
+// ---- coreset draws ----------------------------------------------------------------
+// inclusive prefix of fixed weights: per-block sums -> host scan (nblk small) -> per-block
+// prefixes; each draw binary-searches blocks then walks its block.
+__global__ void k_draws(const float *__restrict__ probs, const double *__restrict__ w, uint64_t n,
+                        const unsigned long long *__restrict__ maxbits, const uint64_t *__restrict__ bpre,
+                        uint32_t nblk, uint64_t T, uint64_t m, uint64_t seed, uint64_t first,
+                        uint64_t *__restrict__ idx, float *__restrict__ wout) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    const int s = fixed_shift(__longlong_as_double((long long)*maxbits), n);
+    const uint64_t u = mulhi64(wyrand_at(seed, first + t), T);
+    // first block with inclusive prefix > u
+    uint32_t lo = 0, hi = nblk;
+    while (lo < hi) { const uint32_t mid = (lo + hi) >> 1; if (bpre[mid] > u) hi = mid; else lo = mid + 1; }
+    const uint32_t b = lo < nblk ? lo : nblk - 1;
+    uint64_t run = b ? bpre[b - 1] : 0;
+    uint64_t ans = min<uint64_t>(n - 1, (uint64_t)(b + 1) * FX_BLK - 1);
+    for (uint64_t i = (uint64_t)b * FX_BLK; i < min<uint64_t>(n, (uint64_t)(b + 1) * FX_BLK); ++i) {
+        run += to_fixed((double)probs[i], s);
+        if (run > u) { ans = i; break; }
+    }
+    idx[t] = ans;
+    const double wi = w ? (double)(float)w[ans] : 1.;
+    wout[t] = (float)(wi / ((double)m * (double)probs[ans]));
+}
+
+void coreset_sample_run(Ctx &c, uint64_t n, const float *probs, const double *w, uint64_t m, uint64_t seed,
+                        uint64_t *idx, float *wout, hipStream_t st) {
+    coreset_sample_from(c, n, probs, w, m, seed, 0, idx, wout, st);
+}
+
+void coreset_sample_from(Ctx &c, uint64_t n, const float *probs, const double *w, uint64_t m, uint64_t seed,
+                         uint64_t first, uint64_t *idx, float *wout, hipStream_t st) {
+    const uint32_t nblk = (uint32_t)cdiv(n, FX_BLK);
+    char *buf = (char *)c.s[S_MISC0].reserve(64 + nblk * 8);
+    unsigned long long *mx = (unsigned long long *)buf;
+    uint64_t *bsum = (uint64_t *)(buf + 64);
+    fill_u32((uint32_t *)mx, 0, 2, st);
+    hipLaunchKernelGGL(k_absmax<float>, dim3((uint32_t)std::min<uint64_t>(cdiv(n, 256), 1024)), dim3(256), 0, st,
+                       probs, (const double *)nullptr, n, mx);
+    hipLaunchKernelGGL(k_fixed_blocksum<float>, dim3(nblk), dim3(256), 0, st, probs, (const double *)nullptr, n,
+                       (const unsigned long long *)mx, bsum);
+    std::vector<uint64_t> h(nblk);
+    HIP_CHECK(hipMemcpyAsync(h.data(), bsum, nblk * 8, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    uint64_t T = 0;
+    for (uint32_t b = 0; b < nblk; ++b) { T += h[b]; h[b] = T; }
+    REQUIRE(T > 0, "all sampling probabilities are zero");
+    HIP_CHECK(hipMemcpyAsync(bsum, h.data(), nblk * 8, hipMemcpyHostToDevice, st));
+    if (m) hipLaunchKernelGGL(k_draws, dim3((uint32_t)cdiv(m, 256)), dim3(256), 0, st, probs, w, n,
+                              (const unsigned long long *)mx, (const uint64_t *)bsum, nblk, T, m, seed, first, idx, wout);
+    HIP_CHECK(hipStreamSynchronize(st));
+}
+
+}  // namespace mcg

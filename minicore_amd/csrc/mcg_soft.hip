@@ -1,0 +1,205 @@
+// Soft EM: responsibilities and the soft M-step.
+//
+// Replaces set_centroids_full_mean (soft) and correct_softmax
+// (include/minicore/clustering/centroid.h:781-963): R = softmax(-temp * costs)
+// per row (no max subtraction, as blaze's softmax; a row with a NaN becomes
+// uniform over its argmin ties), cost = sum_i w_i <costs_i, R_i>, and centers
+// = sum_i R_ic w_i x_i / sum_i R_ic w_i (dense) or sum_i R_ic x_i /
+// sum_i R_ic w_i (CSR, the reference omits w in that numerator).
+// All sums are FP64 in a fixed order (row splits added in order).
+#include "mcg_internal.h"
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+
+namespace mcg {
+
+// one wave per row
+__global__ void k_softmax(const float *__restrict__ costs, uint64_t n, uint32_t k, float nt,
+                          float *__restrict__ resp, const double *__restrict__ w, double *__restrict__ rowdot) {
+    const uint64_t row = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const float *cr = costs + row * k;
+    float *r = resp + row * k;
+    float s = 0.f;
+    for (uint32_t c = lane; c < k; c += 64) { const float e = expf(cr[c] * nt); r[c] = e; s += e; }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    int bad = 0;
+    for (uint32_t c = lane; c < k; c += 64) { const float v = r[c] / s; r[c] = v; bad |= (v != v); }
+    for (int o = 32; o > 0; o >>= 1) bad |= __shfl_xor(bad, o);
+    if (bad) {
+        float m = FLT_MAX;
+        for (uint32_t c = lane; c < k; c += 64) m = fminf(m, cr[c]);
+        for (int o = 32; o > 0; o >>= 1) m = fminf(m, __shfl_xor(m, o));
+        int ties = 0;
+        for (uint32_t c = lane; c < k; c += 64) ties += (cr[c] == m);
+        for (int o = 32; o > 0; o >>= 1) ties += __shfl_xor(ties, o);
+        const float per = (float)(1. / (double)ties);
+        for (uint32_t c = lane; c < k; c += 64) r[c] = (cr[c] == m) ? per : 0.f;
+    }
+    float dt = 0.f;
+    for (uint32_t c = lane; c < k; c += 64) dt += cr[c] * r[c];
+    for (int o = 32; o > 0; o >>= 1) dt += __shfl_xor(dt, o);
+    if (lane == 0) rowdot[row] = (double)dt * (w ? w[row] : 1.);
+}
+
+void softmax_rows(const float *costs, uint64_t n, uint64_t k, float temp, float *resp, const double *w,
+                  double *rowdot, hipStream_t st) {
+    hipLaunchKernelGGL(k_softmax, dim3((uint32_t)cdiv(n, 4)), dim3(256), 0, st, costs, n, (uint32_t)k, -temp, resp, w,
+                       rowdot);
+}
+
+// ---- dense soft M-step -----------------------------------------------------------
+constexpr int SM_TC = 32, SM_TD = 64, SM_TR = 64;
+
+__global__ __launch_bounds__(256) void k_soft_dense_part(const float *__restrict__ X, uint64_t n, uint32_t d,
+                                                         const float *__restrict__ R, uint32_t k,
+                                                         const double *__restrict__ w, uint64_t rows_per,
+                                                         double *__restrict__ part, double *__restrict__ dpart) {
+    __shared__ double rs[SM_TR][SM_TC + 1];
+    __shared__ float xs[SM_TR][SM_TD + 1];
+    const uint32_t c0 = blockIdx.x * SM_TC, j0 = blockIdx.y * SM_TD;
+    const uint64_t r0 = (uint64_t)blockIdx.z * rows_per, r1 = min<uint64_t>(n, r0 + rows_per);
+    const int tid = threadIdx.x, cc = tid >> 3, jj = (tid & 7) * 8;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double den = 0.;
+    for (uint64_t rb = r0; rb < r1; rb += SM_TR) {
+        __syncthreads();
+        for (int q = tid; q < SM_TR * SM_TC; q += 256) {
+            const int r = q / SM_TC, c = q % SM_TC;
+            const uint64_t i = rb + r;
+            rs[r][c] = (i < r1 && c0 + c < k) ? (double)R[i * k + c0 + c] * (w ? w[i] : 1.) : 0.;
+        }
+        for (int q = tid; q < SM_TR * SM_TD; q += 256) {
+            const int r = q / SM_TD, j = q % SM_TD;
+            const uint64_t i = rb + r;
+            xs[r][j] = (i < r1 && j0 + j < d) ? X[i * d + j0 + j] : 0.f;
+        }
+        __syncthreads();
+        const int nr = (int)min<uint64_t>(SM_TR, r1 - rb);
+        for (int r = 0; r < nr; ++r) {
+            const double rw = rs[r][cc];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc[q] += rw * (double)xs[r][jj + q];
+            den += rw;
+        }
+    }
+    const uint32_t c = c0 + cc;
+    if (c >= k) return;
+    for (int q = 0; q < 8; ++q) {
+        const uint32_t j = j0 + jj + q;
+        if (j < d) part[((uint64_t)blockIdx.z * k + c) * d + j] = acc[q];
+    }
+    if (blockIdx.y == 0 && jj == 0) dpart[(uint64_t)blockIdx.z * k + c] = den;
+}
+
+__global__ void k_soft_dense_final(const double *__restrict__ part, const double *__restrict__ dpart, uint32_t S,
+                                   uint32_t k, uint32_t d, float *__restrict__ C) {
+    const uint32_t c = blockIdx.x;
+    double den = 0.;
+    for (uint32_t s = 0; s < S; ++s) den += dpart[(uint64_t)s * k + c];
+    for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) {
+        double num = 0.;
+        for (uint32_t s = 0; s < S; ++s) num += part[((uint64_t)s * k + c) * d + j];
+        C[(uint64_t)c * d + j] = (float)(num / den);
+    }
+}
+
+// ---- CSR soft M-step (column-major walk: one wave per column) ---------------------
+__global__ void k_soft_den_part(const float *__restrict__ R, uint64_t n, uint32_t k, const double *__restrict__ w,
+                                uint64_t rows_per, double *__restrict__ dpart) {
+    const uint64_t r0 = (uint64_t)blockIdx.x * rows_per, r1 = min<uint64_t>(n, r0 + rows_per);
+    for (uint32_t c = threadIdx.x; c < k; c += blockDim.x) {
+        double s = 0.;
+        for (uint64_t i = r0; i < r1; ++i) s += (double)R[i * k + c] * (w ? w[i] : 1.);
+        dpart[(uint64_t)blockIdx.x * k + c] = s;
+    }
+}
+
+__global__ void k_soft_den_final(const double *__restrict__ dpart, uint32_t S, uint32_t k, double *__restrict__ den) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= k) return;
+    double s = 0.;
+    for (uint32_t q = 0; q < S; ++q) s += dpart[(uint64_t)q * k + c];
+    den[c] = s;
+}
+
+__global__ void k_soft_csc(const uint64_t *__restrict__ cptr, const uint32_t *__restrict__ crow,
+                           const float *__restrict__ cval, uint32_t d, const float *__restrict__ R, uint32_t k,
+                           const double *__restrict__ den, float *__restrict__ C) {
+    const uint64_t j = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (j >= d) return;
+    const uint64_t b = cptr[j], e = cptr[j + 1];
+    for (uint32_t c0 = 0; c0 < k; c0 += 64) {
+        const uint32_t c = c0 + lane;
+        double acc = 0.;
+        if (c < k)
+            for (uint64_t p = b; p < e; ++p) acc += (double)R[(uint64_t)crow[p] * k + c] * (double)cval[p];
+        if (c < k) C[(uint64_t)c * d + j] = (float)(acc / den[c]);
+    }
+}
+
+static void build_csc(Data &ds) {
+    if (ds.csc_ptr.p) return;
+    const uint64_t n = ds.n, d = ds.d, nnz = ds.nnz;
+    std::vector<uint64_t> P(n + 1);
+    std::vector<uint32_t> J(nnz);
+    std::vector<float> V(nnz);
+    HIP_CHECK(hipMemcpy(P.data(), ds.ptr.p, (n + 1) * 8, hipMemcpyDeviceToHost));
+    if (nnz) {
+        HIP_CHECK(hipMemcpy(J.data(), ds.idx.p, nnz * 4, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(V.data(), ds.val.p, nnz * 4, hipMemcpyDeviceToHost));
+    }
+    std::vector<uint64_t> cp(d + 1, 0);
+    for (uint64_t p = 0; p < nnz; ++p) cp[J[p] + 1]++;
+    for (uint64_t j = 0; j < d; ++j) cp[j + 1] += cp[j];
+    std::vector<uint64_t> cur(cp.begin(), cp.end() - 1);
+    std::vector<uint32_t> rows(nnz);
+    std::vector<float> cv(nnz);
+    for (uint64_t i = 0; i < n; ++i)
+        for (uint64_t p = P[i]; p < P[i + 1]; ++p) {
+            const uint64_t q = cur[J[p]]++;
+            rows[q] = (uint32_t)i;
+            cv[q] = V[p];
+        }
+    ds.csc_ptr.reserve((d + 1) * 8);
+    ds.csc_row.reserve(nnz * 4 + 4);
+    ds.csc_val.reserve(nnz * 4 + 4);
+    HIP_CHECK(hipMemcpy(ds.csc_ptr.p, cp.data(), (d + 1) * 8, hipMemcpyHostToDevice));
+    if (nnz) {
+        HIP_CHECK(hipMemcpy(ds.csc_row.p, rows.data(), nnz * 4, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(ds.csc_val.p, cv.data(), nnz * 4, hipMemcpyHostToDevice));
+    }
+}
+
+void soft_mstep(Data &ds, const float *R, const double *w, uint64_t k, float *Cnew, hipStream_t st) {
+    Ctx &c = *ds.ctx;
+    const uint64_t n = ds.n;
+    const uint32_t d = (uint32_t)ds.d;
+    if (!ds.csr) {
+        const uint32_t tc = (uint32_t)cdiv(k, SM_TC), td = (uint32_t)cdiv(d, SM_TD);
+        uint32_t S = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cdiv(n, 1024), cdiv(2048, (uint64_t)tc * td)));
+        const uint64_t rows_per = cdiv(n, S);
+        S = (uint32_t)cdiv(n, rows_per);
+        double *part = (double *)c.s[S_PART].reserve((uint64_t)S * k * d * 8);
+        double *dpart = (double *)c.s[S_PARTW].reserve((uint64_t)S * k * 8);
+        hipLaunchKernelGGL(k_soft_dense_part, dim3(tc, td, S), dim3(256), 0, st, ds.X, n, d, R, (uint32_t)k, w,
+                           rows_per, part, dpart);
+        hipLaunchKernelGGL(k_soft_dense_final, dim3((uint32_t)k), dim3(256), 0, st, part, dpart, S, (uint32_t)k, d,
+                           Cnew);
+        return;
+    }
+    build_csc(ds);
+    const uint64_t rows_per = 4096;
+    const uint32_t S = (uint32_t)cdiv(n, rows_per);
+    double *dpart = (double *)c.s[S_PARTW].reserve((uint64_t)S * k * 8);
+    double *den = (double *)c.s[S_PARTQ].reserve(k * 8);
+    hipLaunchKernelGGL(k_soft_den_part, dim3(S), dim3(256), 0, st, R, n, (uint32_t)k, w, rows_per, dpart);
+    hipLaunchKernelGGL(k_soft_den_final, dim3((uint32_t)cdiv(k, 256)), dim3(256), 0, st, dpart, S, (uint32_t)k, den);
+    hipLaunchKernelGGL(k_soft_csc, dim3((uint32_t)cdiv(d, 4)), dim3(256), 0, st, ds.csc_ptr.as<const uint64_t>(),
+                       ds.csc_row.as<const uint32_t>(), ds.csc_val.as<const float>(), d, R, (uint32_t)k, den, Cnew);
+}
+
+}  // namespace mcg

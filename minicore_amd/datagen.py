@@ -1,0 +1,53 @@
+"""Synthetic inputs of the benchmark workloads (numpy default_rng, seeded).
+
+  gaussian_mixture   dense rows: component means U[lo, hi]^d, per-component
+                     scales, equal weights (the dense k-means workloads).
+  poisson_csr        sparse count rows: nnz_i ~ Poisson(density*d) distinct
+                     uniform columns, values Poisson(lambda_j) (zeros -> 1),
+                     lambda_j ~ LogNormal(0, sigma), optionally row-normalised.
+"""
+import numpy as np
+
+
+def gaussian_mixture(n, d, ncomp, seed, lo=-10., hi=10., scale=(1., 1.), dtype=np.float32):
+    rng = np.random.default_rng(seed)
+    means = rng.uniform(lo, hi, size=(ncomp, d))
+    scales = rng.uniform(scale[0], scale[1], size=ncomp) if scale[0] != scale[1] else np.full(ncomp, scale[0])
+    labels = rng.integers(0, ncomp, size=n)
+    X = means[labels] + rng.standard_normal((n, d)) * scales[labels, None]
+    return X.astype(dtype), labels.astype(np.uint32), means.astype(dtype)
+
+
+def poisson_csr(n, d, density, seed, sigma=1.5, normalize=False):
+    """Returns (data f32, indices u32, indptr u64) with sorted column indices."""
+    rng = np.random.default_rng(seed)
+    lam = rng.lognormal(0., sigma, size=d)
+    nnz_row = np.clip(rng.poisson(density * d, size=n), 1, d)
+    indptr = np.zeros(n + 1, dtype=np.uint64)
+    indptr[1:] = np.cumsum(nnz_row)
+    nnz = int(indptr[-1])
+    indices = np.empty(nnz, dtype=np.uint32)
+    for i in range(n):
+        b, e = int(indptr[i]), int(indptr[i + 1])
+        cols = rng.choice(d, size=e - b, replace=False)
+        cols.sort()
+        indices[b:e] = cols
+    vals = rng.poisson(lam[indices]).astype(np.float64)
+    vals[vals == 0] = 1.
+    if normalize:
+        rs = np.add.reduceat(vals, indptr[:-1].astype(np.int64))
+        vals = vals / np.repeat(rs, nnz_row)
+    return vals.astype(np.float32), indices, indptr
+
+
+def gaussian_mixture_device(n, d, ncomp, seed, lo=-10., hi=10., scale=(0.5, 2.), device="cuda"):
+    """Same construction on the device with torch's generator (for the 10^6-row workload)."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    means = (torch.rand((ncomp, d), generator=g, device=device, dtype=torch.float32) * (hi - lo) + lo)
+    scales = torch.rand((ncomp,), generator=g, device=device, dtype=torch.float32) * (scale[1] - scale[0]) + scale[0]
+    labels = torch.randint(0, ncomp, (n,), generator=g, device=device)
+    X = torch.randn((n, d), generator=g, device=device, dtype=torch.float32)
+    X.mul_(scales[labels, None]).add_(means[labels])
+    return X, labels, means

@@ -1,0 +1,724 @@
+/*
+ * mco.c -- CPU oracle for the minicore clustering hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Loaded by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg; never by the
+ * product library.  Every function cites the reference lines it restates
+ * (paths relative to the dnbaker/minicore tree).
+ *
+ * Built with -ffp-contract=off so that every a*b+c below is two roundings
+ * unless the code calls fma() explicitly: the GPU library follows the same
+ * operation sequence where results are compared bit for bit.
+ */
+#include "mco.h"
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ======================================================================
+ * Random numbers.  The reference draws from wy::WyRand (aesctr/wy.h, an
+ * un-vendored submodule; shared.h:5).  This restates the published wyrand
+ * generator (Wang Yi, wyhash): state += 0xa0761d6478bd642f; output =
+ * mum(state, state ^ 0xe7037ed1a0b428db), mum = hi64 ^ lo64 of the 128-bit
+ * product.  The j-th output is a pure function of (seed, j).
+ * ====================================================================== */
+uint64_t mco_mulhi64(uint64_t a, uint64_t b) {
+    return (uint64_t)(((unsigned __int128)a * b) >> 64);
+}
+uint64_t mco_wyrand_at(uint64_t seed, uint64_t j) {
+    uint64_t s = seed + (j + 1) * 0xa0761d6478bd642fULL;
+    unsigned __int128 p = (unsigned __int128)s * (s ^ 0xe7037ed1a0b428dbULL);
+    return (uint64_t)(p >> 64) ^ (uint64_t)p;
+}
+
+/* ======================================================================
+ * Fixed-point inverse-CDF sampler.  The reference samples index i with
+ * probability w_i / sum(w) through reservoir_simd::sample (kmeans.h:123,130;
+ * centroid.h:709) and an alias table (coreset.h:382-385, 641); both live in
+ * un-vendored submodules.  Restated here as: w_i -> floor(w_i * 2^s) in
+ * uint64 with s chosen from max(w) and n so the total fits in 63 bits;
+ * draw u = mulhi(r, T) for a raw 64-bit RNG output r; return the first i
+ * whose inclusive integer prefix sum exceeds u.  Integer sums are exact, so
+ * the result does not depend on summation order.
+ * ====================================================================== */
+static int bitlen64(uint64_t v) { int b = 0; while (v) { ++b; v >>= 1; } return b; }
+
+int mco_fixed_shift(double maxw, uint64_t n) {
+    int e;
+    if (!(maxw > 0.) || !isfinite(maxw)) return 0;
+    frexp(maxw, &e);                 /* maxw < 2^e */
+    return 62 - e - bitlen64(n);     /* n * maxw * 2^s < 2^62 */
+}
+
+void mco_fixed_weights(const double *w, uint64_t n, uint64_t *out) {
+    double mx = 0.;
+    for (uint64_t i = 0; i < n; ++i) if (w[i] > mx) mx = w[i];
+    int s = mco_fixed_shift(mx, n);
+    for (uint64_t i = 0; i < n; ++i) {
+        double v = w[i] > 0. ? floor(ldexp(w[i], s)) : 0.;
+        out[i] = (uint64_t)v;
+    }
+}
+
+uint64_t mco_fixed_draw(const uint64_t *fw, uint64_t n, uint64_t r) {
+    uint64_t T = 0;
+    for (uint64_t i = 0; i < n; ++i) T += fw[i];
+    if (T == 0) return 0;
+    uint64_t u = mco_mulhi64(r, T), acc = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        acc += fw[i];
+        if (acc > u) return i;
+    }
+    return n - 1;
+}
+
+/* ======================================================================
+ * Dissimilarity with additive prior: applicator.h:1632-1915.
+ * first = data row x ("ctr" in the reference's parameter list), second =
+ * center c ("mr"); lhsum/lhinc belong to the center, rhsum/rhinc to the row.
+ * ====================================================================== */
+typedef struct { double pv, lhrsi, rhrsi, lhinc, rhinc; } smooth_t;
+
+/* applicator.h:1642-1656 */
+static smooth_t smoothing(double prior, uint64_t d, double xsum, double csum) {
+    smooth_t s;
+    double prior_sum = prior * (double)d;
+    double lhsum = csum + prior_sum, rhsum = xsum + prior_sum;
+    double smallest = 7.0069e-42 * (lhsum + rhsum);
+    s.pv = prior > smallest ? prior : smallest;
+    prior_sum = s.pv * (double)d;
+    lhsum = csum + prior_sum; rhsum = xsum + prior_sum;
+    s.lhrsi = 1. / lhsum; s.rhrsi = 1. / rhsum;
+    s.lhinc = (s.pv != 0. && lhsum != 0.) ? s.pv * s.lhrsi : 0.;
+    s.rhinc = (s.pv != 0. && rhsum != 0.) ? s.pv * s.rhrsi : 0.;
+    if (isnan(s.lhinc)) s.lhinc = 0.;
+    if (isnan(s.rhinc)) s.rhinc = 0.;
+    return s;
+}
+
+/* One coordinate's contribution given smoothed center value a and smoothed
+ * row value b (applicator.h:1732-1736: MKL = kl(a||b), REVERSE_MKL =
+ * kl(b||a), JSD = half the symmetric sum against the midpoint). */
+static double coord_term(int msr, double a, double b) {
+    switch (msr) {
+    case MCO_MKL:         return a * log(a / b);
+    case MCO_REVERSE_MKL: return b * log(b / a);
+    case MCO_JSD: {
+        double m = a + b;
+        return 0.5 * (a * log(2. * a / m) + b * log(2. * b / m));
+    }
+    }
+    return NAN;
+}
+
+/* applicator.h:1750-1760 / 1876-1891: clamp at 0; +inf -> DBL_MAX; -inf -> 0 */
+static double post_kl(double r) {
+    r = (r < 0.) ? 0. : r;   /* std::max(ret, 0.) keeps NaN */
+    if (r == INFINITY) r = DBL_MAX;
+    return r;
+}
+
+double mco_msr_dense(int msr, const double *x, const double *c, uint64_t d,
+                     double prior, double xsum, double csum) {
+    if (msr == MCO_SQRL2) {                       /* applicator.h:1680-1685 */
+        double s = 0.;
+        for (uint64_t j = 0; j < d; ++j) { double t = c[j] - x[j]; s += t * t; }
+        return s;
+    }
+    smooth_t sm = smoothing(prior, d, xsum, csum);
+    double r = 0.;
+    for (uint64_t j = 0; j < d; ++j) {
+        double a = c[j] * sm.lhrsi + sm.lhinc;    /* smoothed center */
+        double b = x[j] * sm.rhrsi + sm.rhinc;    /* smoothed row */
+        r += coord_term(msr, a, b);
+    }
+    return post_kl(r);
+}
+
+/* merge.h:26-65: ordered union walk; returns the number of coordinates absent from both. */
+uint64_t mco_shared_zeros(const uint32_t *a, uint64_t an, const uint32_t *b, uint64_t bn, uint64_t d) {
+    uint64_t i = 0, j = 0, u = 0;
+    while (i < an && j < bn) {
+        if (a[i] == b[j]) { ++i; ++j; }
+        else if (a[i] < b[j]) ++i;
+        else ++j;
+        ++u;
+    }
+    u += (an - i) + (bn - j);
+    return d - u;
+}
+
+/* applicator.h:1762-1897 sparse-sparse branch. */
+double mco_msr_sparse(int msr, const double *xv, const uint32_t *xi, uint64_t xn,
+                      const double *cv, const uint32_t *ci, uint64_t cn, uint64_t d,
+                      double prior, double xsum, double csum, uint64_t *sharednz_out) {
+    uint64_t sharednz = mco_shared_zeros(xi, xn, ci, cn, d);
+    if (sharednz_out) *sharednz_out = sharednz;
+    uint64_t i = 0, j = 0;
+    if (msr == MCO_SQRL2) {                        /* :1797-1806 */
+        double s = 0.;
+        while (i < xn || j < cn) {
+            double xval, cval;
+            if (j >= cn || (i < xn && xi[i] < ci[j])) { xval = xv[i++]; cval = 0.; }
+            else if (i >= xn || ci[j] < xi[i])        { xval = 0.; cval = cv[j++]; }
+            else                                      { xval = xv[i++]; cval = cv[j++]; }
+            double t = cval - xval; s += t * t;
+        }
+        if (isnan(s)) return -1.;
+        return s < 0. ? 0. : s;
+    }
+    smooth_t sm = smoothing(prior, d, xsum, csum);
+    double klc = 0.;
+    while (i < xn || j < cn) {
+        double xval, cval;
+        if (j >= cn || (i < xn && xi[i] < ci[j])) { xval = xv[i++]; cval = 0.; }
+        else if (i >= xn || ci[j] < xi[i])        { xval = 0.; cval = cv[j++]; }
+        else                                      { xval = xv[i++]; cval = cv[j++]; }
+        klc += coord_term(msr, cval * sm.lhrsi + sm.lhinc, xval * sm.rhrsi + sm.rhinc);
+    }
+    double lhl = sm.lhinc != 0. ? log(sm.lhinc) : 0., rhl = sm.rhinc != 0. ? log(sm.rhinc) : 0.;
+    double zc;
+    if (msr == MCO_MKL)              zc = (double)sharednz * sm.lhinc * (lhl - rhl);   /* :1847-1849 */
+    else if (msr == MCO_REVERSE_MKL) zc = (double)sharednz * sm.rhinc * (rhl - lhl);   /* :1853-1855 */
+    else {                                                                              /* :1850-1852 */
+        double shl = log((sm.lhinc + sm.rhinc) * .5);
+        double shincl = (sm.lhinc + sm.rhinc) > 0. ? (sm.lhinc + sm.rhinc) * shl : 0.;
+        zc = ((lhl * sm.lhinc + rhl * sm.rhinc - shincl) * (double)sharednz) * .5;
+    }
+    return post_kl(klc + zc);
+}
+
+float mco_sqrl2_f32(const float *x, const float *c, uint64_t d) {
+    float s = 0.f;
+    for (uint64_t j = 0; j < d; ++j) { float t = x[j] - c[j]; s = fmaf(t, t, s); }
+    return s;
+}
+
+/* Cost of CSR row i against a dense center, walking all d coordinates. */
+static double csr_row_cost(int msr, const float *val, const uint32_t *idx, uint64_t b, uint64_t e,
+                           uint64_t d, const float *c, double prior, double xsum, double csum) {
+    if (msr == MCO_SQRL2) {
+        double s = 0.; uint64_t p = b;
+        for (uint64_t j = 0; j < d; ++j) {
+            double xv = (p < e && idx[p] == j) ? (double)val[p++] : 0.;
+            double t = (double)c[j] - xv; s += t * t;
+        }
+        return s;
+    }
+    smooth_t sm = smoothing(prior, d, xsum, csum);
+    double r = 0.; uint64_t p = b;
+    for (uint64_t j = 0; j < d; ++j) {
+        double xv = (p < e && idx[p] == j) ? (double)val[p++] : 0.;
+        r += coord_term(msr, (double)c[j] * sm.lhrsi + sm.lhinc, xv * sm.rhrsi + sm.rhinc);
+    }
+    return post_kl(r);
+}
+
+static double row_sum_dense(const float *x, uint64_t d) {
+    double s = 0.; for (uint64_t j = 0; j < d; ++j) s += x[j]; return s;
+}
+static double row_sum_csr(const float *val, uint64_t b, uint64_t e) {
+    double s = 0.; for (uint64_t p = b; p < e; ++p) s += val[p]; return s;
+}
+
+static double pair_cost(int msr, const float *X, const float *val, const uint32_t *idx,
+                        const uint64_t *indptr, uint64_t i, uint64_t d, const float *c,
+                        double prior, double xsum, double csum, double *xbuf, double *cbuf) {
+    if (val) return csr_row_cost(msr, val, idx, indptr[i], indptr[i + 1], d, c, prior, xsum, csum);
+    for (uint64_t j = 0; j < d; ++j) { xbuf[j] = X[i * d + j]; cbuf[j] = c[j]; }
+    return mco_msr_dense(msr, xbuf, cbuf, d, prior, xsum, csum);
+}
+
+void mco_cost_matrix_dense(int msr, const float *X, uint64_t n, uint64_t d,
+                           const float *C, uint64_t k, double prior, double *out) {
+    double *cs = (double *)malloc(k * sizeof(double));
+    for (uint64_t c = 0; c < k; ++c) cs[c] = row_sum_dense(C + c * d, d);
+    #pragma omp parallel
+    {
+        double *xb = (double *)malloc(d * sizeof(double)), *cb = (double *)malloc(d * sizeof(double));
+        #pragma omp for schedule(static)
+        for (int64_t i = 0; i < (int64_t)n; ++i) {
+            double xs = row_sum_dense(X + i * d, d);
+            for (uint64_t c = 0; c < k; ++c)
+                out[i * k + c] = pair_cost(msr, X, NULL, NULL, NULL, i, d, C + c * d, prior, xs, cs[c], xb, cb);
+        }
+        free(xb); free(cb);
+    }
+    free(cs);
+}
+
+void mco_cost_matrix_csr(int msr, const float *val, const uint32_t *idx, const uint64_t *indptr,
+                         uint64_t n, uint64_t d, const float *C, uint64_t k, double prior, double *out) {
+    double *cs = (double *)malloc(k * sizeof(double));
+    for (uint64_t c = 0; c < k; ++c) cs[c] = row_sum_dense(C + c * d, d);
+    #pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        double xs = row_sum_csr(val, indptr[i], indptr[i + 1]);
+        for (uint64_t c = 0; c < k; ++c)
+            out[i * k + c] = csr_row_cost(msr, val, idx, indptr[i], indptr[i + 1], d, C + c * d, prior, xs, cs[c]);
+    }
+    free(cs);
+}
+
+/* ======================================================================
+ * Hard assignment: solve.h:234-285 (start from center 0, replace on strict <).
+ * ====================================================================== */
+void mco_assign_sqrl2_f32(const float *X, uint64_t n, uint64_t d, const float *C, uint64_t k,
+                          uint32_t *asn, float *cost, double *gap_out, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const float *x = X + (uint64_t)i * d;
+        float best = mco_sqrl2_f32(x, C, d), second = INFINITY;
+        uint32_t bi = 0;
+        for (uint64_t c = 1; c < k; ++c) {
+            float v = mco_sqrl2_f32(x, C + c * d, d);
+            if (v < best) { second = best; best = v; bi = (uint32_t)c; }
+            else if (v < second) second = v;
+        }
+        asn[i] = bi; cost[i] = best;
+        if (gap_out) gap_out[i] = ((double)second - (double)best) / ((double)best > 0. ? (double)best : 1.);
+    }
+}
+
+void mco_assign_f64(int msr, const float *X, const float *val, const uint32_t *idx,
+                    const uint64_t *indptr, uint64_t n, uint64_t d, const float *C, uint64_t k,
+                    double prior, uint32_t *asn, double *cost, double *gap_out) {
+    double *cs = (double *)malloc(k * sizeof(double));
+    for (uint64_t c = 0; c < k; ++c) cs[c] = row_sum_dense(C + c * d, d);
+    #pragma omp parallel
+    {
+        double *xb = (double *)malloc(d * sizeof(double)), *cb = (double *)malloc(d * sizeof(double));
+        #pragma omp for schedule(dynamic, 16)
+        for (int64_t i = 0; i < (int64_t)n; ++i) {
+            double xs = val ? row_sum_csr(val, indptr[i], indptr[i + 1]) : row_sum_dense(X + i * d, d);
+            double best = pair_cost(msr, X, val, idx, indptr, i, d, C, prior, xs, cs[0], xb, cb), second = INFINITY;
+            uint32_t bi = 0;
+            for (uint64_t c = 1; c < k; ++c) {
+                double v = pair_cost(msr, X, val, idx, indptr, i, d, C + c * d, prior, xs, cs[c], xb, cb);
+                if (v < best) { second = best; best = v; bi = (uint32_t)c; }
+                else if (v < second) second = v;
+            }
+            asn[i] = bi; cost[i] = best;
+            if (gap_out) gap_out[i] = (second - best) / (best > 0. ? best : 1.);
+        }
+        free(xb); free(cb);
+    }
+    free(cs);
+}
+
+/* ======================================================================
+ * M-step (hard): centroid.h:137-178 (dense mean), 81-112 (CSR mean).  Sums
+ * are FP64 in a fixed order: members in ascending index, `chunk` members
+ * summed sequentially (w*x as one fma when weighted), chunk sums added in
+ * order; mean = sum / total weight, rounded once to float.
+ * ====================================================================== */
+uint64_t mco_centroids_dense(const float *X, uint64_t n, uint64_t d, const uint32_t *asn,
+                             const double *w, uint64_t k, uint64_t chunk, float *C) {
+    uint64_t *cnt = (uint64_t *)calloc(k + 1, sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; ++i) cnt[asn[i] + 1]++;
+    for (uint64_t c = 0; c < k; ++c) cnt[c + 1] += cnt[c];
+    uint64_t *perm = (uint64_t *)malloc((n ? n : 1) * sizeof(uint64_t));
+    uint64_t *cur = (uint64_t *)malloc(k * sizeof(uint64_t));
+    memcpy(cur, cnt, k * sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; ++i) perm[cur[asn[i]]++] = i;
+    uint64_t empty = 0;
+    if (chunk == 0) chunk = 256;
+    #pragma omp parallel for schedule(dynamic, 1) reduction(+:empty)
+    for (int64_t c = 0; c < (int64_t)k; ++c) {
+        uint64_t b = cnt[c], e = cnt[c + 1];
+        if (b == e) { ++empty; continue; }
+        double *tot = (double *)calloc(d, sizeof(double)), *part = (double *)malloc(d * sizeof(double));
+        double wt = 0., wpart;
+        for (uint64_t s = b; s < e; s += chunk) {
+            uint64_t se = s + chunk < e ? s + chunk : e;
+            memset(part, 0, d * sizeof(double)); wpart = 0.;
+            for (uint64_t p = s; p < se; ++p) {
+                const float *x = X + perm[p] * d;
+                double wi = w ? w[perm[p]] : 1.;
+                for (uint64_t j = 0; j < d; ++j) part[j] = w ? fma(wi, (double)x[j], part[j]) : part[j] + (double)x[j];
+                wpart += wi;
+            }
+            for (uint64_t j = 0; j < d; ++j) tot[j] += part[j];
+            wt += wpart;
+        }
+        for (uint64_t j = 0; j < d; ++j) C[c * d + j] = (float)(tot[j] / wt);
+        free(tot); free(part);
+    }
+    free(cnt); free(perm); free(cur);
+    return empty;
+}
+
+uint64_t mco_centroids_csr(const float *val, const uint32_t *idx, const uint64_t *indptr,
+                           uint64_t n, uint64_t d, const uint32_t *asn, const double *w,
+                           uint64_t k, float *C) {
+    double *acc = (double *)calloc(k * d, sizeof(double));
+    double *wt = (double *)calloc(k, sizeof(double));
+    uint64_t *cnt = (uint64_t *)calloc(k, sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; ++i) {
+        uint32_t c = asn[i];
+        double wi = w ? w[i] : 1.;
+        for (uint64_t p = indptr[i]; p < indptr[i + 1]; ++p) acc[c * d + idx[p]] += wi * (double)val[p];
+        wt[c] += wi; cnt[c]++;
+    }
+    uint64_t empty = 0;
+    for (uint64_t c = 0; c < k; ++c) {
+        if (!cnt[c]) { ++empty; continue; }
+        for (uint64_t j = 0; j < d; ++j) C[c * d + j] = (float)(acc[c * d + j] / wt[c]);
+    }
+    free(acc); free(wt); free(cnt);
+    return empty;
+}
+
+/* Per-point cost for the Lloyd control flow: SQRL2 on dense rows uses the
+ * FP32 sequential-FMA value (what the GPU library reports); other cases FP64. */
+static void assign_generic(int msr, const float *X, const float *val, const uint32_t *idx,
+                           const uint64_t *indptr, uint64_t n, uint64_t d, const float *C, uint64_t k,
+                           double prior, uint32_t *asn, double *cost, int nthreads) {
+    if (msr == MCO_SQRL2 && !val) {
+        float *cf = (float *)malloc((n ? n : 1) * sizeof(float));
+        mco_assign_sqrl2_f32(X, n, d, C, k, asn, cf, NULL, nthreads);
+        for (uint64_t i = 0; i < n; ++i) cost[i] = cf[i];
+        free(cf);
+    } else {
+        mco_assign_f64(msr, X, val, idx, indptr, n, d, C, k, prior, asn, cost, NULL);
+    }
+}
+
+static double total_cost(const double *cost, const double *w, uint64_t n) {
+    double s = 0.;
+    for (uint64_t i = 0; i < n; ++i) s += w ? w[i] * cost[i] : cost[i];
+    return s;
+}
+
+static int has_nan(const float *C, uint64_t m) {
+    for (uint64_t i = 0; i < m; ++i) if (isnan(C[i])) return 1;
+    return 0;
+}
+
+/* set_centroids_full_mean (hard) with the empty-cluster restart of
+ * centroid.h:687-752: each empty cluster (ascending id) takes the row drawn
+ * with probability proportional to its cost (RNG seeded by n), all points
+ * are re-assigned, and each drawn row is forced onto its new center. */
+static int centroids_with_restart(int msr, const float *X, const float *val, const uint32_t *idx,
+                                  const uint64_t *indptr, uint64_t n, uint64_t d, float *C, uint64_t k,
+                                  double prior, const double *w, uint32_t *asn, double *cost, int nthreads) {
+    uint64_t *cnt = (uint64_t *)calloc(k, sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; ++i) cnt[asn[i]]++;
+    int restarted = 0; uint64_t draw = 0;
+    uint32_t *rs = NULL, *sa = NULL; uint64_t ne = 0;
+    for (uint64_t c = 0; c < k; ++c) if (!cnt[c]) ++ne;
+    if (ne) {
+        restarted = 1;
+        rs = (uint32_t *)malloc(ne * sizeof(uint32_t)); sa = (uint32_t *)malloc(ne * sizeof(uint32_t));
+        uint64_t *fw = (uint64_t *)malloc(n * sizeof(uint64_t));
+        mco_fixed_weights(cost, n, fw);
+        uint64_t e = 0;
+        for (uint64_t c = 0; c < k; ++c) if (!cnt[c]) {
+            uint64_t r = mco_fixed_draw(fw, n, mco_wyrand_at(n, draw++));
+            rs[e] = (uint32_t)r; sa[e] = (uint32_t)c; ++e;
+            if (val) {
+                memset(C + c * d, 0, d * sizeof(float));
+                for (uint64_t p = indptr[r]; p < indptr[r + 1]; ++p) C[c * d + idx[p]] = val[p];
+            } else memcpy(C + c * d, X + r * d, d * sizeof(float));
+        }
+        free(fw);
+        assign_generic(msr, X, val, idx, indptr, n, d, C, k, prior, asn, cost, nthreads);
+        for (uint64_t i = 0; i < ne; ++i) if (asn[rs[i]] != sa[i]) { asn[rs[i]] = sa[i]; cost[rs[i]] = 0.; }
+        free(rs); free(sa);
+    }
+    free(cnt);
+    if (val) mco_centroids_csr(val, idx, indptr, n, d, asn, w, k, C);
+    else     mco_centroids_dense(X, n, d, asn, w, k, 256, C);
+    return restarted;
+}
+
+/* perform_hard_clustering, solve.h:98-175.  FT is float for float data, so
+ * every cost in the control flow is rounded to float (solve.h:120-123). */
+int mco_lloyd(int msr, const float *X, const float *val, const uint32_t *idx,
+              const uint64_t *indptr, uint64_t n, uint64_t d, float *C, uint64_t k,
+              double prior, const double *w, double eps, uint64_t maxiter,
+              uint32_t *asn, double *cost, double *initcost, double *finalcost,
+              uint64_t *numiter, int nthreads) {
+    assign_generic(msr, X, val, idx, indptr, n, d, C, k, prior, asn, cost, nthreads);
+    float fcost = (float)total_cost(cost, w, n);
+    *initcost = fcost;
+    if (fcost == 0.f) { *initcost = 0.; *finalcost = 0.; *numiter = 0; return 0; }
+    float *Cc = (float *)malloc(k * d * sizeof(float));
+    memcpy(Cc, C, k * d * sizeof(float));
+    uint64_t iter = 0;
+    int rc = 0;
+    for (;;) {
+        int res = centroids_with_restart(msr, X, val, idx, indptr, n, d, Cc, k, prior, w, asn, cost, nthreads);
+        if (has_nan(Cc, k * d)) { rc = -1; break; }
+        assign_generic(msr, X, val, idx, indptr, n, d, Cc, k, prior, asn, cost, nthreads);
+        float newcost = (float)total_cost(cost, w, n);
+        if (newcost > fcost && !res) {
+            assign_generic(msr, X, val, idx, indptr, n, d, C, k, prior, asn, cost, nthreads);
+            break;
+        }
+        memcpy(C, Cc, k * d * sizeof(float));
+        ++iter;
+        float old = fcost; fcost = newcost;
+        float diff = old - newcost;
+        if ((double)diff < eps * ((double)newcost > (double)old ? (double)newcost : (double)old) || iter > maxiter) break;
+    }
+    free(Cc);
+    *finalcost = fcost; *numiter = iter;
+    return rc;
+}
+
+/* ======================================================================
+ * Soft clustering.  centroid.h:781-793 (correct_softmax) and 795-963;
+ * solve.h:287-374.  softmax is exp(-temp*cost) / row sum in float without
+ * max subtraction; a row with any NaN becomes uniform over its argmin ties.
+ * ====================================================================== */
+void mco_softmax_rows(const float *costs, uint64_t n, uint64_t k, double temp, float *resp) {
+    float nt = (float)(-temp);
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const float *cr = costs + i * k; float *r = resp + i * k;
+        float s = 0.f;
+        for (uint64_t c = 0; c < k; ++c) { r[c] = expf(cr[c] * nt); s += r[c]; }
+        int bad = 0;
+        for (uint64_t c = 0; c < k; ++c) { r[c] = r[c] / s; if (isnan(r[c])) bad = 1; }
+        if (bad) {
+            uint64_t b = 0; for (uint64_t c = 1; c < k; ++c) if (cr[c] < cr[b]) b = c;
+            uint64_t ties = 0; for (uint64_t c = 0; c < k; ++c) if (cr[c] == cr[b]) ++ties;
+            float per = (float)(1. / (double)ties);
+            for (uint64_t c = 0; c < k; ++c) r[c] = (cr[c] == cr[b]) ? per : 0.f;
+        }
+    }
+}
+
+static void soft_costs(int msr, const float *X, const float *val, const uint32_t *idx, const uint64_t *indptr,
+                       uint64_t n, uint64_t d, const float *C, uint64_t k, double prior, float *costs) {
+    double *tmp = (double *)malloc(n * k * sizeof(double));
+    if (val) mco_cost_matrix_csr(msr, val, idx, indptr, n, d, C, k, prior, tmp);
+    else     mco_cost_matrix_dense(msr, X, n, d, C, k, prior, tmp);
+    for (uint64_t i = 0; i < n * k; ++i) costs[i] = (float)tmp[i];
+    free(tmp);
+}
+
+int mco_soft(int msr, const float *X, const float *val, const uint32_t *idx,
+             const uint64_t *indptr, uint64_t n, uint64_t d, float *C, uint64_t k,
+             double prior, double temp, const double *w, double eps, uint64_t maxiter,
+             float *costs, float *resp, double *initcost, double *finalcost, uint64_t *numiter) {
+    float *Cc = (float *)malloc(k * d * sizeof(float));
+    memcpy(Cc, C, k * d * sizeof(float));
+    soft_costs(msr, X, val, idx, indptr, n, d, Cc, k, prior, costs);
+    double cost = DBL_MAX, init = -1.;
+    uint64_t iter = 0;
+    double *num = (double *)malloc(k * d * sizeof(double)), *den = (double *)malloc(k * sizeof(double));
+    for (;;) {
+        double oldcost = cost;
+        mco_softmax_rows(costs, n, k, temp, resp);
+        double ret = 0.;
+        for (uint64_t i = 0; i < n; ++i) {
+            double dt = 0.;
+            for (uint64_t c = 0; c < k; ++c) dt += (double)costs[i * k + c] * (double)resp[i * k + c];
+            ret += dt * (w ? w[i] : 1.);
+        }
+        cost = ret;
+        /* M-step: dense numerator carries w (centroid.h:840-873); CSR omits it (:925-939). */
+        memset(num, 0, k * d * sizeof(double)); memset(den, 0, k * sizeof(double));
+        for (uint64_t i = 0; i < n; ++i) {
+            double wi = w ? w[i] : 1.;
+            for (uint64_t c = 0; c < k; ++c) {
+                double r = resp[i * k + c];
+                den[c] += r * wi;
+                if (val) {
+                    for (uint64_t p = indptr[i]; p < indptr[i + 1]; ++p) num[c * d + idx[p]] += r * (double)val[p];
+                } else {
+                    double rw = r * wi;
+                    for (uint64_t j = 0; j < d; ++j) num[c * d + j] += rw * (double)X[i * d + j];
+                }
+            }
+        }
+        for (uint64_t c = 0; c < k; ++c)
+            for (uint64_t j = 0; j < d; ++j) Cc[c * d + j] = (float)(num[c * d + j] / den[c]);
+        soft_costs(msr, X, val, idx, indptr, n, d, Cc, k, prior, costs);
+        if (init < 0.) init = cost;
+        if (oldcost >= cost) memcpy(C, Cc, k * d * sizeof(float));
+        double mx = oldcost > cost ? oldcost : cost;
+        if (oldcost - cost <= eps * mx || ++iter == maxiter) break;
+    }
+    free(num); free(den); free(Cc);
+    *initcost = init; *finalcost = cost; *numiter = iter;
+    return 0;
+}
+
+/* ======================================================================
+ * D^2 seeding: coresets::kmeanspp, kmeans.h:49-215 with n_local_samples=1.
+ * oracle(center, i) = msr(first = row i, second = row center) (smw.h:370-374).
+ * Distances are the FT=float values: SQRL2 uses the FP32 sequential-FMA sum.
+ * ====================================================================== */
+static double pair_rows(int msr, const float *X, const float *val, const uint32_t *idx,
+                        const uint64_t *indptr, uint64_t d, uint64_t i, uint64_t c,
+                        const double *rsum, double prior) {
+    if (!val) {
+        if (msr == MCO_SQRL2) return (double)mco_sqrl2_f32(X + i * d, X + c * d, d);
+        double *xb = (double *)malloc(d * sizeof(double)), *cb = (double *)malloc(d * sizeof(double));
+        for (uint64_t j = 0; j < d; ++j) { xb[j] = X[i * d + j]; cb[j] = X[c * d + j]; }
+        double v = (double)(float)mco_msr_dense(msr, xb, cb, d, prior, rsum[i], rsum[c]);
+        free(xb); free(cb);
+        return v;
+    }
+    uint64_t xb = indptr[i], xe = indptr[i + 1], cb = indptr[c], ce = indptr[c + 1];
+    double *xv = (double *)malloc((xe - xb + 1) * sizeof(double)), *cv = (double *)malloc((ce - cb + 1) * sizeof(double));
+    for (uint64_t p = xb; p < xe; ++p) xv[p - xb] = val[p];
+    for (uint64_t p = cb; p < ce; ++p) cv[p - cb] = val[p];
+    double v = mco_msr_sparse(msr, xv, idx + xb, xe - xb, cv, idx + cb, ce - cb, d, prior, rsum[i], rsum[c], NULL);
+    if (msr != MCO_SQRL2) v = (double)(float)v;
+    free(xv); free(cv);
+    return v;
+}
+
+uint64_t mco_kmeanspp(int msr, const float *X, const float *val, const uint32_t *idx,
+                      const uint64_t *indptr, uint64_t n, uint64_t d, uint64_t k,
+                      double prior, uint64_t seed, const double *w,
+                      uint32_t *centers, uint32_t *asn, float *distf, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    if (n < k) k = n;
+    if (k == 0) return 0;
+    double *rsum = (double *)malloc(n * sizeof(double));
+    for (uint64_t i = 0; i < n; ++i)
+        rsum[i] = val ? row_sum_csr(val, indptr[i], indptr[i + 1]) : row_sum_dense(X + i * d, d);
+    double *dist = (double *)malloc(n * sizeof(double)), *rv = (double *)malloc(n * sizeof(double));
+    uint64_t *fw = (uint64_t *)malloc(n * sizeof(uint64_t));
+    uint64_t j = 0;
+    uint64_t fc = mco_wyrand_at(seed, j++) % n;                    /* kmeans.h:79 */
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i)
+        dist[i] = ((uint64_t)i == fc) ? 0. : pair_rows(msr, X, val, idx, indptr, d, i, fc, rsum, prior);
+    for (uint64_t i = 0; i < n; ++i) asn[i] = 0;
+    centers[0] = (uint32_t)fc;
+    for (uint64_t ci = 1; ci < k; ++ci) {
+        uint64_t r = mco_wyrand_at(seed, j++);
+        for (uint64_t i = 0; i < n; ++i) rv[i] = w ? w[i] * dist[i] : dist[i];
+        mco_fixed_weights(rv, n, fw);
+        uint64_t newc = mco_fixed_draw(fw, n, r);
+        if (dist[newc] == 0.) {                                      /* kmeans.h:179-186 */
+            uint64_t b = 0; for (uint64_t i = 1; i < n; ++i) if (dist[i] > dist[b]) b = i;
+            newc = b;
+        }
+        asn[newc] = (uint32_t)ci; centers[ci] = (uint32_t)newc;
+        #pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < (int64_t)n; ++i) {                  /* kmeans.h:189-201 */
+            if ((uint64_t)i == newc || dist[i] <= 0.) continue;
+            double v = pair_rows(msr, X, val, idx, indptr, d, i, newc, rsum, prior);
+            if (v < dist[i]) { asn[i] = (uint32_t)ci; dist[i] = v; }
+        }
+        dist[newc] = 0.;
+    }
+    for (uint64_t i = 0; i < n; ++i) distf[i] = (float)dist[i];
+    free(rsum); free(dist); free(rv); free(fw);
+    return k;
+}
+
+/* ======================================================================
+ * Coreset sensitivities, coreset.h:423-578.  Sums over points are exact:
+ * each term is mapped to fixed point with one shift for the whole vector
+ * (from its max) and added as integers, so the result does not depend on
+ * the order of addition; probabilities are then formed in FP64 and rounded
+ * to float (CoresetSampler<float, uint32_t>, pyfgc.h:36).
+ * ====================================================================== */
+typedef struct { int shift; } fx_t;
+
+static fx_t fx_from(const double *t, uint64_t n) {
+    double mx = 0.;
+    for (uint64_t i = 0; i < n; ++i) { double a = fabs(t[i]); if (a > mx) mx = a; }
+    fx_t f; f.shift = mco_fixed_shift(mx, n);
+    return f;
+}
+static int64_t fx_of(double v, fx_t f) {
+    double s = ldexp(v, f.shift);
+    return (int64_t)(s < 0. ? -floor(-s) : floor(s));
+}
+static double fx_back(int64_t v, fx_t f) { return ldexp((double)v, -f.shift); }
+
+int mco_coreset_probs(int sens, uint64_t n, uint64_t ncenters, const double *costs,
+                      const uint32_t *asn, const double *w, float *probs) {
+    double *wf = (double *)malloc(n * sizeof(double)), *wc = (double *)malloc(n * sizeof(double));
+    for (uint64_t i = 0; i < n; ++i) {
+        wf[i] = w ? (double)(float)w[i] : 1.;      /* weights_ is DynamicVector<float> */
+        wc[i] = wf[i] * costs[i];
+    }
+    fx_t fw = fx_from(wf, n), fc = fx_from(wc, n);
+    int64_t *ws = (int64_t *)calloc(ncenters, sizeof(int64_t)), *cs = (int64_t *)calloc(ncenters, sizeof(int64_t));
+    uint64_t *cnt = (uint64_t *)calloc(ncenters, sizeof(uint64_t));
+    int64_t wtot = 0, ctot = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        int64_t a = fx_of(wf[i], fw), b = fx_of(wc[i], fc);
+        ws[asn[i]] += a; cs[asn[i]] += b; cnt[asn[i]]++;
+        wtot += a; ctot += b;
+    }
+    double W = fx_back(wtot, fw), TC = fx_back(ctot, fc);
+    double *p = (double *)malloc(n * sizeof(double));
+    int rc = 0;
+    if (sens == 4) {                                    /* LBK, :487-533 */
+        double alpha = 16. * log((double)ncenters) + 32., alpha2 = 2. * alpha;
+        double mean = TC / W, tcinv = alpha / mean;
+        double *adj = (double *)malloc(ncenters * sizeof(double));
+        for (uint64_t c = 0; c < ncenters; ++c) {
+            double Wc = fx_back(ws[c], fw), Sc = fx_back(cs[c], fc);
+            adj[c] = alpha2 * Sc / (Wc * mean) + 4. * W / Wc;
+        }
+        for (uint64_t i = 0; i < n; ++i) p[i] = tcinv * costs[i] + adj[asn[i]];
+        free(adj);
+    } else if (sens == 3) {                             /* VX, :423-455 */
+        for (uint64_t i = 0; i < n; ++i) p[i] = wc[i] / TC + 1. / (double)cnt[asn[i]];
+    } else if (sens == 0) {                             /* BFL, :534-578 */
+        for (uint64_t i = 0; i < n; ++i) {
+            uint32_t a = asn[i];
+            p[i] = .5 * (wc[i] / TC + wf[i] / (fx_back(ws[a], fw) * (double)cnt[a]));
+        }
+    } else if (sens == 1) {                             /* FL, :456-486 (not normalised) */
+        for (uint64_t i = 0; i < n; ++i)
+            probs[i] = w ? (float)(wc[i] / TC) : (float)(ceil((double)n * costs[i] / TC) + 1.);
+        goto done;
+    } else { rc = -1; goto done; }
+    {
+        fx_t fp = fx_from(p, n);
+        int64_t ptot = 0;
+        for (uint64_t i = 0; i < n; ++i) ptot += fx_of(p[i], fp);
+        double P = fx_back(ptot, fp);
+        for (uint64_t i = 0; i < n; ++i) probs[i] = (float)(p[i] / P);
+    }
+done:
+    free(wf); free(wc); free(ws); free(cs); free(cnt); free(p);
+    return rc;
+}
+
+/* coreset.h:625-651: m i.i.d. draws; weight = w_i / (m * p_i). */
+void mco_coreset_sample(uint64_t n, const float *probs, const double *w, uint64_t m,
+                        uint64_t seed, uint64_t *indices, float *weights) {
+    double *pd = (double *)malloc(n * sizeof(double));
+    uint64_t *fw = (uint64_t *)malloc(n * sizeof(uint64_t)), *pre = (uint64_t *)malloc(n * sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; ++i) pd[i] = probs[i];
+    mco_fixed_weights(pd, n, fw);
+    uint64_t T = 0;
+    for (uint64_t i = 0; i < n; ++i) { T += fw[i]; pre[i] = T; }
+    for (uint64_t t = 0; t < m; ++t) {
+        uint64_t u = mco_mulhi64(mco_wyrand_at(seed, t), T);
+        uint64_t lo = 0, hi = n;                   /* first i with pre[i] > u */
+        while (lo < hi) { uint64_t mid = (lo + hi) / 2; if (pre[mid] > u) hi = mid; else lo = mid + 1; }
+        if (lo >= n) lo = n - 1;
+        indices[t] = lo;
+        double wi = w ? (double)(float)w[lo] : 1.;
+        weights[t] = (float)(wi / ((double)m * (double)probs[lo]));
+    }
+    free(pd); free(fw); free(pre);
+}

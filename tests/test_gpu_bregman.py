@@ -1,0 +1,157 @@
+"""Bregman measures (MKL, REVERSE_MKL, JSD) and SQRL2 over CSR and dense rows,
+hard and soft clustering, on the GPU against the FP64 oracle."""
+import os
+
+import numpy as np
+import pytest
+
+from minicore_amd.datagen import poisson_csr
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+MEASURES = [("SQRL2", 2), ("JSD", 4), ("MKL", 5), ("REVERSE_MKL", 11)]
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def csr_data(n, d, density, seed, normalize):
+    data, idx, ptr = poisson_csr(n, d, density, seed, sigma=1.5, normalize=normalize)
+    return data, idx, ptr
+
+
+def centers_from_rows(mco, data, idx, ptr, n, d, k, seed):
+    rng = np.random.default_rng(seed)
+    rows = rng.choice(n, k, replace=False)
+    csr = mco.CSR(data, idx, ptr, (n, d))
+    dense = csr.todense()
+    return dense[rows].copy()
+
+
+def test_pair26183_gpu(mcg, ctx):
+    say("26183-dim pair")
+    import json
+    z = np.load(os.path.join(GOLD, "testmsr_pair.npz"))
+    ref = json.load(open(os.path.join(GOLD, "reference_cases.json")))["msr_pair26183"]
+    d = int(z["d"])
+    c = np.zeros((1, d), np.float32); c[0, z["rhs_idx"]] = z["rhs_val"]
+    ds = mcg.Dataset.csr(z["lhs_val"], z["lhs_idx"], np.array([0, len(z["lhs_idx"])], np.uint64), (1, d), ctx)
+    for name, code in MEASURES:
+        v = ds.cost_matrix(code, c, prior=ref["prior"])[0, 0]
+        assert v == pytest.approx(ref[name], rel=2e-4), name
+
+
+@pytest.mark.parametrize("normalize", [False, True])
+@pytest.mark.parametrize("name,code", MEASURES)
+def test_cost_matrix_csr(mcg, ctx, mco, name, code, normalize):
+    say(f"csr cost matrix {name} normalize={normalize}")
+    n, d, k = 600, 500, 9
+    data, idx, ptr = csr_data(n, d, 0.03, 1, normalize)
+    prior = 1. / d
+    C = centers_from_rows(mco, data, idx, ptr, n, d, k, 2)
+    C = (C + 0.5 * C.mean(0)).astype(np.float32)   # dense-ish centers, like cluster means
+    ref = mco.cost_matrix(code, mco.CSR(data, idx, ptr, (n, d)), C, prior)
+    ds = mcg.Dataset.csr(data, idx, ptr, (n, d), ctx)
+    got = ds.cost_matrix(code, C, prior=prior)
+    scale = np.abs(ref).max()
+    np.testing.assert_allclose(got, ref, rtol=2e-4, atol=2e-5 * scale)
+
+
+@pytest.mark.parametrize("name,code", MEASURES)
+def test_cost_matrix_dense_rows(mcg, ctx, mco, name, code):
+    say(f"dense-row cost matrix {name}")
+    rng = np.random.default_rng(4)
+    X = rng.poisson(1.5, (400, 60)).astype(np.float32)
+    C = rng.poisson(1.5, (7, 60)).astype(np.float32) + 0.25
+    ref = mco.cost_matrix(code, X, C, 0.1)
+    got = mcg.Dataset.dense(X, ctx).cost_matrix(code, C, prior=0.1)
+    np.testing.assert_allclose(got, ref, rtol=2e-4, atol=2e-5 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("name,code", MEASURES[1:])
+def test_tiny_prior_direct_path(mcg, ctx, mco, name, code):
+    """prior 0 takes the reference's per-pair smoothing (7.0069e-42 floor)."""
+    say(f"prior 0 {name}")
+    rng = np.random.default_rng(6)
+    X = (rng.poisson(2., (100, 30)) + 1).astype(np.float32)
+    C = (rng.poisson(2., (5, 30)) + 1).astype(np.float32)
+    ref = mco.cost_matrix(code, X, C, 0.)
+    got = mcg.Dataset.dense(X, ctx).cost_matrix(code, C, prior=0.)
+    np.testing.assert_allclose(got, ref, rtol=5e-4, atol=1e-5 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("name,code", MEASURES)
+def test_assign_csr_non_ties(mcg, ctx, mco, name, code):
+    say(f"csr assign {name}")
+    n, d, k = 3000, 800, 12
+    data, idx, ptr = csr_data(n, d, 0.02, 3, True)
+    prior = 1. / d
+    C = centers_from_rows(mco, data, idx, ptr, n, d, k, 5) + 1e-3
+    a_ref, c_ref, gap = mco.assign_f64(code, mco.CSR(data, idx, ptr, (n, d)), C, prior)
+    ds = mcg.Dataset.csr(data, idx, ptr, (n, d), ctx)
+    a, c = ds.assign(code, C, prior=prior)
+    bad = a != a_ref
+    assert np.all(gap[bad] < 1e-4), gap[bad][:10]
+    np.testing.assert_allclose(c, c_ref, rtol=2e-4, atol=1e-6)
+
+
+@pytest.mark.parametrize("name,code", [("JSD", 4), ("MKL", 5)])
+def test_lloyd_csr(mcg, ctx, mco, name, code):
+    say(f"csr lloyd {name}")
+    n, d, k = 2000, 400, 6
+    data, idx, ptr = csr_data(n, d, 0.03, 7, True)
+    prior = 1. / d
+    C0 = centers_from_rows(mco, data, idx, ptr, n, d, k, 8)
+    ref = mco.lloyd(code, mco.CSR(data, idx, ptr, (n, d)), C0, prior=prior, eps=1e-10, maxiter=8)
+    got = mcg.Dataset.csr(data, idx, ptr, (n, d), ctx).lloyd(code, C0, prior=prior, eps=1e-10, maxiter=8)
+    assert got["initcost"] == pytest.approx(ref["initcost"], rel=1e-4)
+    assert got["finalcost"] == pytest.approx(ref["finalcost"], rel=1e-4)
+    assert np.mean(got["asn"] == ref["asn"]) > 0.995
+    np.testing.assert_allclose(got["centers"], ref["centers"], rtol=1e-3, atol=1e-6)
+
+
+def test_csr_centroids(mcg, ctx, mco):
+    say("csr centroids")
+    n, d, k = 3000, 5000, 11
+    data, idx, ptr = csr_data(n, d, 0.01, 9, False)
+    asn = np.random.default_rng(1).integers(0, k, n).astype(np.uint32)
+    w = np.random.default_rng(2).uniform(0.5, 2, n)
+    prev = np.zeros((k, d), np.float32)
+    for ww in (None, w):
+        ref, _ = mco.centroids(mco.CSR(data, idx, ptr, (n, d)), asn, k, prev, ww)
+        got, _ = mcg.Dataset.csr(data, idx, ptr, (n, d), ctx).centroids(asn, k, prev, ww)
+        np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-9)
+    # reference fixture: test_centroid.cpp stripes
+    data2 = 2. * (np.arange(50) + 1.)
+    idx2 = (2 * (np.arange(50) % 5)).astype(np.uint32)
+    got, _ = mcg.Dataset.csr(data2.astype(np.float32), idx2, (np.arange(11) * 5).astype(np.uint64), (10, 10),
+                             ctx).centroids(np.zeros(10, np.uint32), 1, np.zeros((1, 10)))
+    assert list(got[0, ::2].astype(int)) == [47, 49, 51, 53, 55]
+
+
+@pytest.mark.parametrize("sparse", [False, True])
+@pytest.mark.parametrize("name,code", [("SQRL2", 2), ("REVERSE_MKL", 11), ("JSD", 4)])
+def test_soft_em(mcg, ctx, mco, name, code, sparse):
+    say(f"soft {name} sparse={sparse}")
+    if sparse:
+        n, d, k = 800, 300, 5
+        data, idx, ptr = csr_data(n, d, 0.05, 10, False)
+        X = mco.CSR(data, idx, ptr, (n, d))
+        C0 = X.todense()[:k] + 0.5
+        ds = mcg.Dataset.csr(data, idx, ptr, (n, d), ctx)
+        prior = 1e-3
+    else:
+        rng = np.random.default_rng(11)
+        X = (rng.poisson(3., (700, 20)) + rng.integers(0, 6, (700, 1))).astype(np.float32)
+        C0 = X[:4].copy() + 0.5
+        ds = mcg.Dataset.dense(X, ctx)
+        prior = 0.5
+    temp = 0.05 if code == 2 else 1.
+    ref = mco.soft(code, X, C0, prior=prior, temp=temp, maxiter=6)
+    got = ds.soft(code, C0, prior=prior, temp=temp, maxiter=6)
+    assert got["numiter"] == ref["numiter"]
+    assert got["initcost"] == pytest.approx(ref["initcost"], rel=2e-4)
+    assert got["finalcost"] == pytest.approx(ref["finalcost"], rel=2e-3)
+    np.testing.assert_allclose(got["asn"], ref["asn"], atol=2e-3)
+    np.testing.assert_allclose(got["centers"], ref["centers"], rtol=2e-3, atol=1e-4)

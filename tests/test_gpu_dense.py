@@ -1,0 +1,162 @@
+"""Dense squared-L2 hot path on the GPU against the CPU oracle (through the C ABI)."""
+import numpy as np
+import pytest
+
+from minicore_amd.datagen import gaussian_mixture
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def mixture(n, d, k, seed, lo=-10., hi=10., scale=(1., 1.)):
+    X, lab, means = gaussian_mixture(n, d, k, seed, lo, hi, scale)
+    rng = np.random.default_rng(seed + 100)
+    C = (means + 0.3 * rng.standard_normal(means.shape)).astype(np.float32)
+    return X, C
+
+
+CASES = [(20000, 50, 25, 0), (3000, 17, 3, 1), (4000, 64, 64, 2), (5000, 100, 65, 3), (2500, 256, 130, 4),
+         (1500, 300, 7, 5), (777, 33, 1, 6)]
+
+
+@pytest.mark.parametrize("n,d,k,seed", CASES)
+def test_assign_exact_and_bf16_bitexact(mcg, ctx, mco, n, d, k, seed):
+    say(f"assign n={n} d={d} k={k} seed={seed}")
+    X, C = mixture(n, d, k, seed)
+    a_ref, c_ref, gap = mco.assign_sqrl2_f32(X, C)
+    ds = mcg.Dataset.dense(X, ctx)
+    for prec in (mcg.PREC_EXACT, mcg.PREC_BF16):
+        a, c = ds.assign(mcg.SQRL2, C, precision=prec)
+        assert np.array_equal(a, a_ref), (prec, np.nonzero(a != a_ref)[0][:10])
+        assert np.array_equal(c, c_ref), prec
+    a, _ = ds.assign(mcg.SQRL2, C, precision=mcg.PREC_FP32)
+    bad = a != a_ref
+    # plain FP32 may differ only where the oracle's top-2 gap is below 1e-6 relative
+    assert np.all(gap[bad] < 1e-6), gap[bad][:10]
+    ds.close()
+
+
+def test_bf16_near_ties_are_rescored(mcg, ctx, mco):
+    """Points equidistant (in exact arithmetic) from two centers, plus tiny offsets."""
+    say("near ties")
+    rng = np.random.default_rng(0)
+    d, k = 128, 40
+    C = rng.uniform(-5, 5, (k, d)).astype(np.float32)
+    pts = []
+    for t in range(3000):
+        i, j = rng.choice(k, 2, replace=False)
+        lam = 0.5 + rng.choice([0., 1e-7, -1e-7, 1e-5, 1e-3])
+        pts.append(lam * C[i] + (1 - lam) * C[j] + 1e-4 * rng.standard_normal(d))
+    X = np.array(pts, dtype=np.float32)
+    C[5] = C[4]   # exact duplicate center: lowest index must win
+    a_ref, c_ref, _ = mco.assign_sqrl2_f32(X, C)
+    ds = mcg.Dataset.dense(X, ctx)
+    a, c = ds.assign(mcg.SQRL2, C, precision=mcg.PREC_BF16)
+    assert np.array_equal(a, a_ref)
+    assert np.array_equal(c, c_ref)
+    assert not np.any(a == 5)
+
+
+def test_far_from_origin(mcg, ctx, mco):
+    """Data offset far from the origin: centering keeps the BF16 cross term usable."""
+    say("offset data")
+    X, C = mixture(4000, 64, 16, 9, lo=1000., hi=1010.)
+    a_ref, c_ref, _ = mco.assign_sqrl2_f32(X, C)
+    ds = mcg.Dataset.dense(X, ctx)
+    a, c = ds.assign(mcg.SQRL2, C, precision=mcg.PREC_BF16)
+    assert np.array_equal(a, a_ref)
+    assert np.array_equal(c, c_ref)
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_centroids_bitexact(mcg, ctx, mco, weighted):
+    say(f"centroids weighted={weighted}")
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((9000, 70)).astype(np.float32)
+    k = 37
+    asn = rng.integers(0, k, 9000).astype(np.uint32)
+    asn[asn == 11] = 12  # one empty cluster
+    asn[:700] = 3        # one big cluster (several 256-member chunks)
+    w = rng.uniform(0.1, 3., 9000) if weighted else None
+    prev = rng.standard_normal((k, 70)).astype(np.float32)
+    ref, e_ref = mco.centroids(X, asn, k, prev, w)
+    ds = mcg.Dataset.dense(X, ctx)
+    got, e = ds.centroids(asn, k, prev, w)
+    assert e == e_ref == 1
+    assert np.array_equal(got, ref)
+
+
+def test_centroid_reference_means_gpu(mcg, ctx):
+    say("centroid reference means")
+    dm = np.eye(10, dtype=np.float32) * 10
+    ds = mcg.Dataset.dense(dm, ctx)
+    got, _ = ds.centroids(np.zeros(10, np.uint32), 1, np.zeros((1, 10)))
+    assert np.all(got == 1.)
+    dm2 = np.zeros((500, 5), np.float32); dm2[:, 0] = 1; dm2[:, 3] = -1.4
+    ds = mcg.Dataset.dense(dm2, ctx)
+    got, _ = ds.centroids(np.zeros(500, np.uint32), 1, np.zeros((1, 5)))
+    np.testing.assert_allclose(got[0], [1, 0, 0, -1.4, 0], rtol=1e-7)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp32"])
+def test_lloyd_matches_oracle(mcg, ctx, mco, prec):
+    say(f"lloyd config-1 shape prec={prec}")
+    X, _, means = gaussian_mixture(20000, 50, 25, 0)
+    C0 = X[np.random.default_rng(0).choice(20000, 25, replace=False)].copy()
+    ref = mco.lloyd(mco.SQRL2, X, C0, eps=1e-10, maxiter=100)
+    ds = mcg.Dataset.dense(X, ctx)
+    p = mcg.PREC_BF16 if prec == "bf16" else mcg.PREC_FP32
+    got = ds.lloyd(mcg.SQRL2, C0, eps=1e-10, maxiter=100, precision=p)
+    assert got["numiter"] == ref["numiter"]
+    assert got["initcost"] == pytest.approx(ref["initcost"], rel=1e-6)
+    assert got["finalcost"] == pytest.approx(ref["finalcost"], rel=1e-6)
+    assert np.array_equal(got["asn"], ref["asn"])
+    np.testing.assert_allclose(got["centers"], ref["centers"], rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(got["costs"], ref["costs"], rtol=1e-6)
+
+
+def test_lloyd_weighted_and_fixed_iterations(mcg, ctx, mco):
+    say("lloyd weighted eps=0")
+    X, _, _ = gaussian_mixture(6000, 20, 8, 4, lo=-2, hi=2)
+    w = np.random.default_rng(1).uniform(0.5, 2., 6000)
+    C0 = X[:8].copy()
+    ref = mco.lloyd(mco.SQRL2, X, C0, w=w, eps=0., maxiter=5)
+    got = mcg.Dataset.dense(X, ctx).lloyd(mcg.SQRL2, C0, weights=w, eps=0., maxiter=5)
+    assert got["numiter"] == ref["numiter"]
+    assert np.array_equal(got["asn"], ref["asn"])
+    assert got["finalcost"] == pytest.approx(ref["finalcost"], rel=1e-6)
+
+
+def test_lloyd_empty_cluster_restart(mcg, ctx, mco):
+    say("lloyd restart")
+    X, _, _ = gaussian_mixture(3000, 10, 4, 5, lo=-5, hi=5)
+    C0 = np.concatenate([X[:4], np.full((2, 10), 1e3, np.float32)])   # two centers far away: empty
+    ref = mco.lloyd(mco.SQRL2, X, C0, eps=1e-10, maxiter=20)
+    got = mcg.Dataset.dense(X, ctx).lloyd(mcg.SQRL2, C0, eps=1e-10, maxiter=20)
+    assert np.array_equal(got["asn"], ref["asn"])
+    assert got["numiter"] == ref["numiter"]
+    assert got["finalcost"] == pytest.approx(ref["finalcost"], rel=1e-6)
+
+
+def test_cost_matrix_sqrl2(mcg, ctx, mco):
+    say("cost matrix sqrl2")
+    X, C = mixture(1000, 40, 30, 7)
+    ref = mco.cost_matrix(mco.SQRL2, X, C, 0.)
+    got = mcg.Dataset.dense(X, ctx).cost_matrix(mcg.SQRL2, C)
+    np.testing.assert_allclose(got, ref, rtol=2e-5, atol=1e-3)
+
+
+def test_device_resident_input(mcg, ctx, mco):
+    """A torch tensor on the device is borrowed, not copied; results are identical."""
+    import torch
+    say("device-resident dataset")
+    X, C = mixture(5000, 64, 20, 8)
+    Xt = torch.from_numpy(X).to("cuda")
+    torch.cuda.synchronize()
+    ds = mcg.Dataset.dense(Xt, ctx)
+    a, c = ds.assign(mcg.SQRL2, C)
+    a_ref, c_ref, _ = mco.assign_sqrl2_f32(X, C)
+    assert np.array_equal(a, a_ref) and np.array_equal(c, c_ref)

@@ -1,0 +1,58 @@
+"""D^2 seeding and coreset sampling on the GPU: bit-exact against the oracle
+when fed the same inputs."""
+import numpy as np
+import pytest
+
+from minicore_amd.datagen import gaussian_mixture, poisson_csr
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_kmeanspp_dense_bitexact(mcg, ctx, mco, weighted):
+    say(f"kmeans++ dense weighted={weighted}")
+    X, _, _ = gaussian_mixture(20000, 50, 25, 0)
+    w = np.random.default_rng(3).uniform(0.5, 2, 20000) if weighted else None
+    c_ref, a_ref, d_ref = mco.kmeanspp(mco.SQRL2, X, 25, seed=13, w=w)
+    c, a, dd = mcg.Dataset.dense(X, ctx).kmeanspp(mcg.SQRL2, 25, seed=13, weights=w)
+    assert np.array_equal(c, c_ref)
+    assert np.array_equal(a, a_ref)
+    assert np.array_equal(dd, d_ref)
+
+
+@pytest.mark.parametrize("code", [4, 5])
+def test_kmeanspp_csr_consistent(mcg, ctx, mco, code):
+    say(f"kmeans++ csr msr={code}")
+    n, d = 2000, 300
+    data, idx, ptr = poisson_csr(n, d, 0.04, 2, normalize=True)
+    X = mco.CSR(data, idx, ptr, (n, d))
+    c, a, dist = mcg.Dataset.csr(data, idx, ptr, (n, d), ctx).kmeanspp(code, 10, prior=1. / d, seed=7)
+    assert c[0] == mco.wyrand_at(7, 0) % n and len(set(c.tolist())) == 10
+    Cd = X.todense()[c]
+    ref = mco.cost_matrix(code, X, Cd, 1. / d)
+    m = dist > 0
+    np.testing.assert_allclose(dist[m], ref.min(1)[m], rtol=2e-4, atol=1e-6)
+
+
+@pytest.mark.parametrize("sens", [0, 1, 3, 4])
+@pytest.mark.parametrize("weighted", [False, True])
+def test_coreset_probs_and_sample_bitexact(mcg, ctx, mco, sens, weighted):
+    say(f"coreset sens={sens} weighted={weighted}")
+    rng = np.random.default_rng(sens + 10 * weighted)
+    n, k = 50000, 37
+    asn = rng.integers(0, k, n).astype(np.uint32)
+    costs = rng.exponential(3., n)
+    w = rng.uniform(0.2, 4., n) if weighted else None
+    p_ref = mco.coreset_probs(sens, k, costs, asn, w)
+    p = mcg.coreset_probs(sens, k, costs, asn, w, ctx=ctx)
+    assert np.array_equal(p, p_ref)
+    if sens == 1:
+        return
+    wt_ref, idx_ref = mco.coreset_sample(p_ref, 5000, 99, w)
+    wt, idx = mcg.coreset_sample(p, 5000, 99, w, ctx=ctx)
+    assert np.array_equal(idx, idx_ref)
+    assert np.array_equal(wt, wt_ref)

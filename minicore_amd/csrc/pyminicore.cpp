@@ -1,0 +1,677 @@
+// pybind11 module `_pyminicore`: the pyminicore entry points of the clustering
+// hot path (hcluster, scluster, cmp, kmeanspp, CoresetSampler, CSparseMatrix),
+// with the reference's names, argument order, defaults and return values,
+// running on libmcg (include/mcg.h).
+//
+// Reference bindings mirrored (paths relative to the dnbaker/minicore tree):
+//   hcluster   python/pycluster_dense.cpp:69-110, pycluster_csr.cpp:5-30
+//   scluster   python/pycluster_softdense.cpp:65-91, pycluster_softcsr.cpp:4-29
+//   cmp        python/pycmp.cpp:147-248, pyarrcmp.cpp:126-145
+//   kmeanspp   python/pydense.cpp:34-77, pycsparse.cpp:100-104 (smw.h:179-410)
+//   CoresetSampler python/pycs.cpp:4-58
+//   CSparseMatrix  python/pycsparse.cpp:15-89, pycsparse.h:33-70
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mcg.h"
+
+namespace py = pybind11;
+using namespace pybind11::literals;
+
+namespace {
+
+mcg_ctx *g_ctx = nullptr;
+int g_sqrl2_precision = MCG_PREC_BF16;
+
+void check(int rc) {
+    if (rc == MCG_OK) return;
+    std::string msg = mcg_last_error();
+    if (rc == MCG_EINVAL) throw std::invalid_argument(msg);
+    if (rc == MCG_EUNSUPPORTED) throw py::value_error("unsupported on the GPU path: " + msg);
+    throw std::runtime_error("libmcg error " + std::to_string(rc) + ": " + msg);
+}
+
+mcg_ctx *ctx() {
+    if (!g_ctx) check(mcg_ctx_create(0, &g_ctx));
+    return g_ctx;
+}
+
+struct DataHandle {
+    mcg_data *h = nullptr;
+    ~DataHandle() { if (h) mcg_data_free(h); }
+};
+using DataPtr = std::shared_ptr<DataHandle>;
+
+// measure codes: distance.h:24-64 (names as in distance.h / msr2str)
+int measure_code(py::object msr) {
+    int code;
+    if (py::isinstance<py::str>(msr)) {
+        std::string s = msr.cast<std::string>();
+        std::transform(s.begin(), s.end(), s.begin(), ::toupper);
+        static const std::map<std::string, int> names = {
+            {"SQRL2", 2}, {"JSD", 4}, {"PSD", 4}, {"MKL", 5}, {"REVERSE_MKL", 11}, {"RMKL", 11}};
+        auto it = names.find(s);
+        if (it == names.end()) throw std::invalid_argument("measure " + s + " has no GPU kernel (supported: SQRL2, JSD, MKL, REVERSE_MKL)");
+        code = it->second;
+    } else {
+        code = msr.cast<int>();
+    }
+    if (code != 2 && code != 4 && code != 5 && code != 11)
+        throw std::invalid_argument("measure " + std::to_string(code) + " has no GPU kernel (supported: 2, 4, 5, 11)");
+    return code;
+}
+
+// ---- CSparseMatrix: borrowed scipy CSR arrays (pycsparse.h:33-70) ------------------
+int dtype_code(const py::array &a) {
+    const char k = a.dtype().kind();
+    const auto sz = a.itemsize();
+    if (k == 'f') return sz == 4 ? MCG_F32 : MCG_F64;
+    if (k == 'u') return sz == 1 ? MCG_U8 : sz == 2 ? MCG_U16 : sz == 4 ? MCG_U32 : MCG_U64;
+    if (k == 'i') {
+        if (sz == 4) return MCG_I32;
+        if (sz == 8) return MCG_I64;
+    }
+    throw std::invalid_argument(std::string("unsupported dtype ") + py::str(a.dtype()).cast<std::string>());
+}
+
+struct PyCSparseMatrix {
+    py::array data, indices, indptr;
+    uint64_t nr = 0, nc = 0, nnz = 0;
+    DataPtr dev;  // device-resident copy, built on first use
+
+    PyCSparseMatrix(py::array d, py::array i, py::array p, uint64_t r, uint64_t c, uint64_t z)
+        : data(py::array::ensure(d, py::array::c_style)), indices(py::array::ensure(i, py::array::c_style)),
+          indptr(py::array::ensure(p, py::array::c_style)), nr(r), nc(c), nnz(z) {}
+    explicit PyCSparseMatrix(py::object obj)
+        : PyCSparseMatrix(obj.attr("data"), obj.attr("indices"), obj.attr("indptr"),
+                          py::cast<py::sequence>(obj.attr("shape"))[0].cast<uint64_t>(),
+                          py::cast<py::sequence>(obj.attr("shape"))[1].cast<uint64_t>(),
+                          obj.attr("nnz").cast<uint64_t>()) {}
+
+    mcg_data *device() {
+        if (!dev) {
+            auto h = std::make_shared<DataHandle>();
+            check(mcg_data_csr(ctx(), data.data(), dtype_code(data), indices.data(), dtype_code(indices),
+                               indptr.data(), dtype_code(indptr), nr, nc, nnz, &h->h));
+            dev = h;
+        }
+        return dev->h;
+    }
+    // dense float32 copy of selected rows
+    std::vector<float> dense_rows(const std::vector<int64_t> &rows) const {
+        std::vector<float> out(rows.size() * nc, 0.f);
+        auto P = [&](uint64_t i) -> uint64_t {
+            switch (dtype_code(indptr)) {
+            case MCG_U32: return ((const uint32_t *)indptr.data())[i];
+            case MCG_I32: return (uint64_t)((const int32_t *)indptr.data())[i];
+            case MCG_U64: return ((const uint64_t *)indptr.data())[i];
+            case MCG_I64: return (uint64_t)((const int64_t *)indptr.data())[i];
+            }
+            throw std::invalid_argument("indptr dtype");
+        };
+        auto J = [&](uint64_t p) -> uint64_t {
+            switch (dtype_code(indices)) {
+            case MCG_U16: return ((const uint16_t *)indices.data())[p];
+            case MCG_U32: return ((const uint32_t *)indices.data())[p];
+            case MCG_I32: return (uint64_t)((const int32_t *)indices.data())[p];
+            case MCG_U64: return ((const uint64_t *)indices.data())[p];
+            case MCG_I64: return (uint64_t)((const int64_t *)indices.data())[p];
+            }
+            throw std::invalid_argument("indices dtype");
+        };
+        auto V = [&](uint64_t p) -> float {
+            switch (dtype_code(data)) {
+            case MCG_F32: return ((const float *)data.data())[p];
+            case MCG_F64: return (float)((const double *)data.data())[p];
+            case MCG_U16: return (float)((const uint16_t *)data.data())[p];
+            case MCG_U32: return (float)((const uint32_t *)data.data())[p];
+            case MCG_I32: return (float)((const int32_t *)data.data())[p];
+            case MCG_U8: return (float)((const uint8_t *)data.data())[p];
+            case MCG_U64: return (float)((const uint64_t *)data.data())[p];
+            case MCG_I64: return (float)((const int64_t *)data.data())[p];
+            }
+            throw std::invalid_argument("data dtype");
+        };
+        for (size_t q = 0; q < rows.size(); ++q) {
+            const int64_t r = rows[q];
+            if (r < 0 || (uint64_t)r >= nr) throw std::invalid_argument("center index out of range");
+            for (uint64_t p = P(r); p < P(r + 1); ++p) out[q * nc + J(p)] = V(p);
+        }
+        return out;
+    }
+    std::vector<float> dense_all() const {
+        std::vector<int64_t> rows(nr);
+        for (uint64_t i = 0; i < nr; ++i) rows[i] = (int64_t)i;
+        return dense_rows(rows);
+    }
+};
+
+// ---- dense inputs ------------------------------------------------------------------
+struct Dense {
+    py::array arr;        // kept alive (float32 or float64, C order)
+    uint64_t n = 0, d = 0;
+    bool f64 = false;
+    DataPtr dev;
+};
+
+Dense dense_of(py::array a) {
+    Dense D;
+    if (a.ndim() != 2) throw std::invalid_argument("Expected 2 dimensions");
+    if (a.dtype().kind() == 'f' && a.itemsize() == 8) {
+        D.arr = py::array_t<double, py::array::c_style | py::array::forcecast>::ensure(a);
+        D.f64 = true;
+    } else {
+        D.arr = py::array_t<float, py::array::c_style | py::array::forcecast>::ensure(a);
+    }
+    D.n = (uint64_t)D.arr.shape(0);
+    D.d = (uint64_t)D.arr.shape(1);
+    auto h = std::make_shared<DataHandle>();
+    check(mcg_data_dense(ctx(), D.arr.data(), D.f64 ? MCG_F64 : MCG_F32, D.n, D.d, D.d, &h->h));
+    D.dev = h;
+    return D;
+}
+
+std::vector<float> row_f32(const Dense &D, int64_t r) {
+    if (r < 0 || (uint64_t)r >= D.n) throw std::invalid_argument("center index out of range");
+    std::vector<float> v(D.d);
+    if (D.f64) {
+        const double *p = (const double *)D.arr.data() + (uint64_t)r * D.d;
+        for (uint64_t j = 0; j < D.d; ++j) v[j] = (float)p[j];
+    } else {
+        std::memcpy(v.data(), (const float *)D.arr.data() + (uint64_t)r * D.d, D.d * 4);
+    }
+    return v;
+}
+
+// weights: any numeric dtype -> float64 (pycluster_dense.cpp:43-61)
+py::object weights_f64(py::object w, uint64_t n) {
+    if (w.is_none()) return py::none();
+    auto a = py::array_t<double, py::array::c_style | py::array::forcecast>::ensure(w);
+    if (!a || (uint64_t)a.size() != n) throw std::invalid_argument("weights must have one entry per row");
+    return a;
+}
+const double *wptr(const py::object &w) {
+    return w.is_none() ? nullptr : (const double *)py::cast<py::array>(w).data();
+}
+
+bool is_index_list(py::object centers) {
+    if (py::isinstance<py::array>(centers)) {
+        auto a = py::cast<py::array>(centers);
+        return a.ndim() == 1;
+    }
+    if (py::isinstance<py::sequence>(centers)) {
+        auto s = py::cast<py::sequence>(centers);
+        return s.size() > 0 && !py::isinstance<py::array>(s[0]) && !py::isinstance<py::sequence>(s[0]);
+    }
+    return false;
+}
+
+std::vector<int64_t> index_list(py::object centers) {
+    auto a = py::array_t<int64_t, py::array::c_style | py::array::forcecast>::ensure(centers);
+    if (!a) throw std::invalid_argument("Invalid: expected numpy array or list of numpy arrays or list of center ids");
+    return std::vector<int64_t>(a.data(), a.data() + a.size());
+}
+
+// k x d center block from an array, a list of arrays or indices (pyhelpers.h:122-218)
+std::vector<float> center_block(py::object centers, uint64_t d, const std::function<std::vector<float>(const std::vector<int64_t> &)> &by_index,
+                                uint64_t *k_out) {
+    std::vector<float> C;
+    if (is_index_list(centers)) {
+        auto ids = index_list(centers);
+        C = by_index(ids);
+        *k_out = ids.size();
+        return C;
+    }
+    if (py::isinstance<py::array>(centers)) {
+        auto a = py::array_t<float, py::array::c_style | py::array::forcecast>::ensure(centers);
+        if (!a || a.ndim() != 2 || (uint64_t)a.shape(1) != d)
+            throw std::invalid_argument("centers must be k x d (d = number of columns)");
+        *k_out = (uint64_t)a.shape(0);
+        C.assign(a.data(), a.data() + a.size());
+        return C;
+    }
+    if (py::isinstance<py::sequence>(centers)) {
+        uint64_t k = 0;
+        for (auto item : py::cast<py::sequence>(centers)) {
+            auto a = py::array_t<float, py::array::c_style | py::array::forcecast>::ensure(item);
+            if (!a || (uint64_t)a.size() != d) throw std::invalid_argument("each center must have d entries");
+            C.insert(C.end(), a.data(), a.data() + d);
+            ++k;
+        }
+        *k_out = k;
+        return C;
+    }
+    throw std::invalid_argument("centers must be a 2d numpy array or sequence containing numpy arrays of the full vectors or center ids");
+}
+
+// centers2pylist (pyhelpers.h:66-112): ((data f32, idx u32, indptr u64), shape u64[2]) of the non-zeros
+py::object centers_csr_tuple(const std::vector<float> &C, uint64_t k, uint64_t d) {
+    std::vector<uint64_t> ip(k + 1, 0);
+    for (uint64_t c = 0; c < k; ++c) {
+        uint64_t z = 0;
+        for (uint64_t j = 0; j < d; ++j) z += C[c * d + j] != 0.f;
+        ip[c + 1] = ip[c] + z;
+    }
+    py::array_t<float> data(ip[k]);
+    py::array_t<uint32_t> idx(ip[k]);
+    py::array_t<uint64_t> indptr(k + 1);
+    float *dp = data.mutable_data();
+    uint32_t *jp = idx.mutable_data();
+    std::memcpy(indptr.mutable_data(), ip.data(), (k + 1) * 8);
+    uint64_t q = 0;
+    for (uint64_t c = 0; c < k; ++c)
+        for (uint64_t j = 0; j < d; ++j)
+            if (C[c * d + j] != 0.f) { dp[q] = C[c * d + j]; jp[q] = (uint32_t)j; ++q; }
+    py::array_t<uint64_t> shape(2);
+    shape.mutable_data()[0] = k;
+    shape.mutable_data()[1] = d;
+    return py::make_tuple(py::make_tuple(data, idx, indptr), shape);
+}
+
+void no_minibatch(py::ssize_t mbsize, bool cs) {
+    if (mbsize >= 0 || cs)
+        throw py::value_error("minibatch / coreset-minibatch clustering (mbsize >= 0, cs=True) is not on the GPU path");
+}
+
+// ---- hcluster -------------------------------------------------------------------------
+py::dict run_hard(mcg_data *h, uint64_t n, uint64_t d, std::vector<float> &C, uint64_t k, double prior,
+                  int msr, py::object w, double eps, uint64_t maxiter) {
+    py::array_t<uint32_t> asn(n);
+    py::array_t<float> costs(n);
+    double ic = 0, fc = 0;
+    uint64_t it = 0;
+    check(mcg_lloyd(h, msr, prior, C.data(), k, wptr(w), eps, maxiter, g_sqrl2_precision, asn.mutable_data(),
+                    costs.mutable_data(), &ic, &fc, &it));
+    return py::dict("initcost"_a = ic, "finalcost"_a = fc, "numiter"_a = it, "costs"_a = costs, "asn"_a = asn);
+}
+
+py::object hcluster_dense(py::array dataset, py::object centers, double prior, py::object msr, py::object weights,
+                          double eps, uint64_t maxiter, uint64_t, py::ssize_t mbsize, py::ssize_t, py::ssize_t,
+                          bool, bool cs) {
+    no_minibatch(mbsize, cs);
+    const int m = measure_code(msr);
+    Dense D = dense_of(dataset);
+    uint64_t k = 0;
+    auto C = center_block(centers, D.d, [&](const std::vector<int64_t> &ids) {
+        std::vector<float> out;
+        for (auto r : ids) { auto v = row_f32(D, r); out.insert(out.end(), v.begin(), v.end()); }
+        return out;
+    }, &k);
+    if (k == 0) throw std::invalid_argument("no centers");
+    auto w = weights_f64(weights, D.n);
+    py::dict r = run_hard(D.dev->h, D.n, D.d, C, k, prior, m, w, eps, maxiter);
+    if (D.f64) {
+        py::array_t<double> ctr({(py::ssize_t)k, (py::ssize_t)D.d});
+        for (uint64_t q = 0; q < k * D.d; ++q) ctr.mutable_data()[q] = C[q];
+        r["centers"] = ctr;
+    } else {
+        py::array_t<float> ctr({(py::ssize_t)k, (py::ssize_t)D.d});
+        std::memcpy(ctr.mutable_data(), C.data(), k * D.d * 4);
+        r["centers"] = ctr;
+    }
+    return r;
+}
+
+py::object hcluster_csr(PyCSparseMatrix &smw, py::object centers, double prior, py::object msr, py::object weights,
+                        double eps, uint64_t maxiter, uint64_t, py::ssize_t mbsize, py::ssize_t, py::ssize_t, bool,
+                        bool cs) {
+    no_minibatch(mbsize, cs);
+    const int m = measure_code(msr);
+    uint64_t k = 0;
+    auto C = center_block(centers, smw.nc, [&](const std::vector<int64_t> &ids) { return smw.dense_rows(ids); }, &k);
+    if (k == 0) throw std::invalid_argument("no centers");
+    auto w = weights_f64(weights, smw.nr);
+    py::dict r = run_hard(smw.device(), smw.nr, smw.nc, C, k, prior, m, w, eps, maxiter);
+    r["centers"] = centers_csr_tuple(C, k, smw.nc);
+    return r;
+}
+
+// ---- scluster -------------------------------------------------------------------------
+py::dict run_soft(mcg_data *h, uint64_t n, uint64_t d, std::vector<float> &C, uint64_t k, double prior, int msr,
+                  double temp, uint64_t maxiter, const std::string &savepref, py::object w) {
+    py::object costs, asns;
+    if (!savepref.empty()) {
+        auto memmap = py::module_::import("numpy").attr("memmap");
+        std::vector<py::ssize_t> shape{(py::ssize_t)n, (py::ssize_t)k};
+        costs = memmap(py::str(savepref + ".costs.f32.npy"), "dtype"_a = "f", "mode"_a = "w+", "shape"_a = shape);
+        asns = memmap(py::str(savepref + ".asns.f32.npy"), "dtype"_a = "f", "mode"_a = "w+", "shape"_a = shape);
+    } else {
+        costs = py::array_t<float>({(py::ssize_t)n, (py::ssize_t)k});
+        asns = py::array_t<float>({(py::ssize_t)n, (py::ssize_t)k});
+    }
+    double ic = 0, fc = 0;
+    uint64_t it = 0;
+    check(mcg_soft_em(h, msr, prior, temp, C.data(), k, wptr(w), 1e-8, maxiter,
+                      (float *)py::cast<py::array>(costs).mutable_data(), (float *)py::cast<py::array>(asns).mutable_data(),
+                      &ic, &fc, &it));
+    return py::dict("initcost"_a = ic, "finalcost"_a = fc, "numiter"_a = it, "centers"_a = centers_csr_tuple(C, k, d),
+                    "costs"_a = costs, "asn"_a = asns);
+}
+
+py::object scluster_dense(py::array mat, py::object centers, py::object msr, double prior, double temp,
+                          uint64_t maxiter, py::ssize_t mbsize, py::ssize_t, std::string savepref, py::object weights) {
+    if (prior < 0.) prior = 0.;  // pycluster_softdense.cpp:71
+    if (mbsize > 0) throw py::value_error("minibatch soft clustering is not on the GPU path");
+    const int m = measure_code(msr);
+    Dense D = dense_of(mat);
+    uint64_t k = 0;
+    // centers given by index keep only the positive entries of the rows (pyhelpers.h:150-157)
+    auto C = center_block(centers, D.d, [&](const std::vector<int64_t> &ids) {
+        std::vector<float> out;
+        for (auto r : ids) {
+            auto v = row_f32(D, r);
+            for (auto &x : v) if (!(x > 0.f)) x = 0.f;
+            out.insert(out.end(), v.begin(), v.end());
+        }
+        return out;
+    }, &k);
+    if (k == 0) throw std::invalid_argument("no centers");
+    return run_soft(D.dev->h, D.n, D.d, C, k, prior, m, temp, maxiter, savepref, weights_f64(weights, D.n));
+}
+
+py::object scluster_csr(PyCSparseMatrix &smw, py::object centers, py::object msr, double prior, double temp,
+                        uint64_t maxiter, py::ssize_t mbsize, py::ssize_t, std::string savepref, py::object weights) {
+    if (mbsize > 0) throw py::value_error("minibatch soft clustering is not on the GPU path");
+    const int m = measure_code(msr);
+    uint64_t k = 0;
+    auto C = center_block(centers, smw.nc, [&](const std::vector<int64_t> &ids) { return smw.dense_rows(ids); }, &k);
+    if (k == 0) throw std::invalid_argument("no centers");
+    return run_soft(smw.device(), smw.nr, smw.nc, C, k, prior, m, temp, maxiter, savepref,
+                    weights_f64(weights, smw.nr));
+}
+
+// ---- cmp ----------------------------------------------------------------------------------
+// out (n_data x k) = msr(first = data row, second = center row)
+py::array_t<float> cost_block(mcg_data *h, uint64_t n, const std::vector<float> &C, uint64_t k, int msr, double prior) {
+    py::array_t<float> out({(py::ssize_t)n, (py::ssize_t)k});
+    check(mcg_cost_matrix(h, msr, prior, C.data(), k, out.mutable_data()));
+    return out;
+}
+
+py::array transpose_copy(py::array_t<float> a) {
+    return py::cast<py::array>(py::module_::import("numpy").attr("ascontiguousarray")(a.attr("T")));
+}
+
+py::object cmp_csr_array(PyCSparseMatrix &lhs, py::array arr, py::object msr, py::object betaprior, py::object reverse) {
+    const int m = measure_code(msr);
+    const double prior = betaprior.cast<double>();
+    const bool rev = reverse.cast<bool>();
+    auto a = py::array_t<float, py::array::c_style | py::array::forcecast>::ensure(arr);
+    if (a.ndim() == 1) {
+        if ((uint64_t)a.size() != lhs.nc) throw std::invalid_argument("Array must be of the same dimensionality as the matrix");
+        std::vector<float> sv(a.data(), a.data() + a.size());
+        if (rev) {  // msr(first = matrix row, second = array)
+            auto out = cost_block(lhs.device(), lhs.nr, sv, 1, m, prior);
+            return out.attr("reshape")(lhs.nr);
+        }
+        // msr(first = array, second = matrix row)
+        Dense D = dense_of(a.reshape({(py::ssize_t)1, (py::ssize_t)lhs.nc}));
+        auto out = cost_block(D.dev->h, 1, lhs.dense_all(), lhs.nr, m, prior);
+        return out.attr("reshape")(lhs.nr);
+    }
+    if (a.ndim() != 2) throw std::invalid_argument("NumPy array expected to have 1 or two dimensions.");
+    if ((uint64_t)a.shape(1) != lhs.nc) throw std::invalid_argument("Array must be of the same dimensionality as the matrix");
+    // out[x][y] = msr(first = data row y, second = matrix row x)   (pycmp.cpp:193-197)
+    Dense D = dense_of(a);
+    return transpose_copy(cost_block(D.dev->h, D.n, lhs.dense_all(), lhs.nr, m, prior));
+}
+
+py::object cmp_csr_csr(PyCSparseMatrix &lhs, PyCSparseMatrix &rhs, py::object msr, py::object betaprior) {
+    const int m = measure_code(msr);
+    if (lhs.nc != rhs.nc) throw std::invalid_argument("mismatched # columns");
+    // out[l][r] = msr(first = rhs row r, second = lhs row l)   (pycmp.cpp:244)
+    return transpose_copy(cost_block(rhs.device(), rhs.nr, lhs.dense_all(), lhs.nr, m, betaprior.cast<double>()));
+}
+
+py::object cmp_arrays(py::array lhs, py::array rhs, py::object msr, double prior, bool reverse, int use_double) {
+    const int m = measure_code(msr);
+    if (lhs.ndim() != rhs.ndim() || lhs.ndim() < 1 || lhs.ndim() > 2) throw std::invalid_argument("arrcmp requires arrays of 1 or 2d");
+    if (use_double < 0) use_double = std::max(lhs.itemsize(), rhs.itemsize()) > 4;
+    const bool one = lhs.ndim() == 1;
+    py::array L = one ? py::array(lhs.attr("reshape")(1, -1)) : lhs;
+    py::array R = one ? py::array(rhs.attr("reshape")(1, -1)) : rhs;
+    auto Lf = py::array_t<float, py::array::c_style | py::array::forcecast>::ensure(L);
+    auto Rf = py::array_t<float, py::array::c_style | py::array::forcecast>::ensure(R);
+    if (Lf.shape(1) != Rf.shape(1)) throw std::invalid_argument("mismatched # columns");
+    py::array out;
+    if (reverse) {  // out[i][j] = msr(first = rhs_j, second = lhs_i)   (pyarrcmp.cpp:16-20)
+        Dense D = dense_of(Rf);
+        std::vector<float> C(Lf.data(), Lf.data() + Lf.size());
+        out = transpose_copy(cost_block(D.dev->h, D.n, C, (uint64_t)Lf.shape(0), m, prior));
+    } else {
+        Dense D = dense_of(Lf);
+        std::vector<float> C(Rf.data(), Rf.data() + Rf.size());
+        out = cost_block(D.dev->h, D.n, C, (uint64_t)Rf.shape(0), m, prior);
+    }
+    if (use_double) out = py::cast<py::array>(out.attr("astype")("float64"));
+    if (one) return out.attr("item")(0);
+    return out;
+}
+
+// ---- kmeanspp (smw.h:179-410) --------------------------------------------------------------
+py::object asn_array(const std::vector<uint32_t> &a, uint64_t k) {
+    const uint64_t n = a.size();
+    if (k <= 256) { py::array_t<uint8_t> r(n); for (uint64_t i = 0; i < n; ++i) r.mutable_data()[i] = (uint8_t)a[i]; return r; }
+    if (k <= 63356) { py::array_t<uint16_t> r(n); for (uint64_t i = 0; i < n; ++i) r.mutable_data()[i] = (uint16_t)a[i]; return r; }
+    py::array_t<uint32_t> r(n);
+    std::memcpy(r.mutable_data(), a.data(), n * 4);
+    return r;
+}
+
+struct PPResult {
+    std::vector<uint32_t> centers, asn;
+    std::vector<float> costs;
+    double total = 0.;
+};
+
+PPResult run_pp(mcg_data *h, uint64_t n, int m, uint64_t k, double prior, uint64_t seed, const double *w) {
+    PPResult r;
+    const uint64_t kk = std::min(k, n);
+    r.centers.resize(kk);
+    r.asn.resize(n);
+    r.costs.resize(n);
+    check(mcg_kmeanspp(h, m, prior, k, seed, w, r.centers.data(), r.asn.data(), r.costs.data()));
+    for (float c : r.costs) r.total += c;
+    return r;
+}
+
+void check_pp_args(py::ssize_t lspp, bool expskips, py::ssize_t n_local_trials) {
+    if (lspp > 0) throw py::value_error("local search++ (lspp > 0) is not on the GPU path");
+    if (n_local_trials > 1) throw py::value_error("n_local_trials > 1 is not on the GPU path");
+    (void)expskips;  // sampling method choice; the GPU path has one exact sampler
+}
+
+uint64_t pp_seed(uint64_t seed) { return seed ? seed : std::mt19937_64(std::rand())(); }
+
+py::object kmeanspp_dense(py::array arr, int k, py::object msr, py::object prior, py::object seed, py::object ntimes,
+                          py::ssize_t lspp, py::object weights, bool expskips, py::ssize_t n_local_trials) {
+    check_pp_args(lspp, expskips, n_local_trials);
+    const int m = measure_code(msr);
+    Dense D = dense_of(arr);
+    double pr = prior.cast<double>();
+    if (pr < 0.) pr = 1. / (double)D.d;
+    uint64_t s = pp_seed(seed.cast<uint64_t>());
+    auto w = weights_f64(weights, D.n);
+    PPResult best = run_pp(D.dev->h, D.n, m, (uint64_t)k, pr, s, wptr(w));
+    const unsigned nt = ntimes.cast<unsigned>();
+    for (unsigned t = 0; t < nt; ++t) {  // extra runs keep the cheapest (smw.h:385-393)
+        PPResult r = run_pp(D.dev->h, D.n, m, (uint64_t)k, pr, s + 0x9e3779b97f4a7c15ULL * (t + 1), wptr(w));
+        if (r.total < best.total) best = std::move(r);
+    }
+    py::array_t<uint32_t> ctr(best.centers.size());
+    std::memcpy(ctr.mutable_data(), best.centers.data(), best.centers.size() * 4);
+    py::array_t<float> costs(D.n);
+    std::memcpy(costs.mutable_data(), best.costs.data(), D.n * 4);
+    return py::make_tuple(ctr, asn_array(best.asn, (uint64_t)k), costs);
+}
+
+py::object kmeanspp_csr(PyCSparseMatrix &smw, py::object msr, py::int_ k, double prior, uint64_t seed, unsigned,
+                        py::ssize_t lspp, bool expskips, py::ssize_t n_local_trials, py::object weights) {
+    // Extra runs (ntimes) never replace the first in the reference (smw.h:248-256 compares the
+    // first run's cost with itself), so the first run is the result.
+    check_pp_args(lspp, expskips, n_local_trials);
+    const int m = measure_code(msr);
+    if (prior < 0.) prior = 1. / (double)smw.nc;
+    auto w = weights_f64(weights, smw.nr);
+    const uint64_t kk = k.cast<uint64_t>();
+    PPResult r = run_pp(smw.device(), smw.nr, m, kk, prior, pp_seed(seed), wptr(w));
+    py::array_t<uint32_t> ctr(r.centers.size());
+    std::memcpy(ctr.mutable_data(), r.centers.data(), r.centers.size() * 4);
+    py::array_t<double> costs(smw.nr);
+    for (uint64_t i = 0; i < smw.nr; ++i) costs.mutable_data()[i] = r.costs[i];
+    return py::make_tuple(ctr, asn_array(r.asn, kk), costs);
+}
+
+// ---- CoresetSampler (coreset.h:228-701, pycs.cpp) ----------------------------------------
+struct CoresetSampler {
+    std::vector<float> probs;
+    std::vector<double> weights;  // empty if unweighted
+    uint64_t np = 0, b = 0;
+    int sens = 0;
+    bool ready = false;
+
+    void make_sampler(size_t ncenters, py::array costs, py::array assignments, py::object w, uint64_t, int sens_) {
+        if (costs.ndim() != 1) throw std::invalid_argument("costs must be 1-dimensional");
+        if (costs.dtype().kind() != 'f') throw std::invalid_argument("Costs is not double or float");
+        auto c = py::array_t<double, py::array::c_style | py::array::forcecast>::ensure(costs);
+        auto a = py::array_t<uint32_t, py::array::c_style | py::array::forcecast>::ensure(assignments);
+        np = (uint64_t)c.size();
+        if ((uint64_t)a.size() != np) throw std::invalid_argument("assignments must match costs");
+        weights.clear();
+        if (!w.is_none()) {
+            auto wa = py::cast<py::array>(w);
+            if (wa.ndim() != 1) throw std::invalid_argument("weightss must be 1-dimensional");
+            if (wa.dtype().kind() != 'f') throw std::invalid_argument("Weights can only be double or float");
+            auto wd = py::array_t<double, py::array::c_style | py::array::forcecast>::ensure(wa);
+            weights.assign(wd.data(), wd.data() + wd.size());
+        }
+        sens = sens_;
+        b = ncenters;
+        probs.resize(np);
+        check(mcg_coreset_probs(ctx(), sens, np, ncenters, c.data(), a.data(), weights.empty() ? nullptr : weights.data(),
+                                probs.data()));
+        ready = true;
+    }
+
+    py::array_t<float> get_probs() const {
+        py::array_t<float> r(probs.size());
+        std::memcpy(r.mutable_data(), probs.data(), probs.size() * 4);
+        return r;
+    }
+
+    py::tuple sample(py::ssize_t size, py::ssize_t seed, bool unique, double) {
+        if (!ready) throw std::invalid_argument("Can't sample without created sampler. Call make_ampler");
+        if (seed == 0) seed = std::rand();
+        const uint64_t n = (uint64_t)size;
+        // FL draws n - b points directly (coreset.h:637)
+        const uint64_t direct = sens == MCG_SENS_FL ? (n > b ? n - b : 0) : n;
+        py::array_t<float> rf(size);
+        py::array_t<uint64_t> ri(size);
+        std::fill(rf.mutable_data(), rf.mutable_data() + size, 0.f);
+        std::fill(ri.mutable_data(), ri.mutable_data() + size, 0ull);
+        const double *W = weights.empty() ? nullptr : weights.data();
+        if (!unique) {
+            if (direct)
+                check(mcg_coreset_sample(ctx(), np, probs.data(), W, direct, (uint64_t)seed, 0, ri.mutable_data(),
+                                         rf.mutable_data()));
+            return py::make_tuple(rf, ri);
+        }
+        if (direct > np) throw std::invalid_argument("cannot draw more unique points than there are points");
+        // draw until `direct` distinct indices (coreset.h:640-672); weight = count * w_i / (n p_i)
+        std::map<uint64_t, uint32_t> ctr;
+        uint64_t drawn = 0;
+        std::vector<uint64_t> idx;
+        std::vector<float> wt;
+        while (ctr.size() < direct) {
+            const uint64_t batch = std::max<uint64_t>(1024, 2 * (direct - ctr.size()));
+            idx.resize(batch);
+            wt.resize(batch);
+            check(mcg_coreset_sample(ctx(), np, probs.data(), W, batch, (uint64_t)seed, drawn, idx.data(), wt.data()));
+            for (uint64_t t = 0; t < batch && ctr.size() < direct; ++t) ++ctr[idx[t]];
+            drawn += batch;
+        }
+        uint64_t q = 0;
+        for (auto &kv : ctr) {
+            const double wi = W ? (double)(float)W[kv.first] : 1.;
+            ri.mutable_data()[q] = kv.first;
+            rf.mutable_data()[q] = (float)((double)kv.second * (wi / ((double)n * (double)probs[kv.first])));
+            ++q;
+        }
+        return py::make_tuple(rf, ri);
+    }
+};
+
+}  // namespace
+
+PYBIND11_MODULE(_pyminicore, m) {
+    m.doc() = "pyminicore entry points of the clustering hot path on MI355X (libmcg)";
+
+    py::class_<PyCSparseMatrix>(m, "CSparseMatrix")
+        .def(py::init<py::object>(), py::arg("sparray"))
+        .def("__str__", [](const PyCSparseMatrix &x) {
+            return "CSparseMatrix, " + std::to_string(x.nr) + "x" + std::to_string(x.nc) + ", " + std::to_string(x.nnz);
+        })
+        .def("columns", [](const PyCSparseMatrix &x) { return x.nc; })
+        .def("rows", [](const PyCSparseMatrix &x) { return x.nr; })
+        .def("nnz", [](const PyCSparseMatrix &x) { return x.nnz; })
+        .def_static("from_items", [](py::array data, py::array idx, py::array indptr, py::sequence shape) {
+            return PyCSparseMatrix(data, idx, indptr, shape[0].cast<uint64_t>(), shape[1].cast<uint64_t>(),
+                                   (uint64_t)data.size());
+        }, py::arg("data"), py::arg("indices"), py::arg("indptr"), py::arg("shape"));
+
+    m.def("hcluster", &hcluster_csr, py::arg("smw"), py::arg("centers"), py::arg("prior") = 0., py::arg("msr") = 2,
+          py::arg("weights") = py::none(), py::arg("eps") = 1e-10, py::arg("maxiter") = 100, py::arg("seed") = 0,
+          py::arg("mbsize") = py::ssize_t(-1), py::arg("ncheckins") = py::ssize_t(-1),
+          py::arg("reseed_count") = py::ssize_t(5), py::arg("with_rep") = false, py::arg("cs") = false);
+    m.def("hcluster", &hcluster_dense, py::arg("dataset"), py::arg("centers"), py::arg("prior") = 0., py::arg("msr") = 2,
+          py::arg("weights") = py::none(), py::arg("eps") = 1e-10, py::arg("maxiter") = 100, py::arg("seed") = 0,
+          py::arg("mbsize") = py::ssize_t(-1), py::arg("ncheckins") = py::ssize_t(-1),
+          py::arg("reseed_count") = py::ssize_t(5), py::arg("with_rep") = false, py::arg("cs") = false,
+          "Clusters the rows using the centers provided; Lloyd / EM until convergence");
+
+    m.def("scluster", &scluster_csr, py::arg("smw"), py::arg("centers"), py::arg("msr") = 2, py::arg("prior") = 0.,
+          py::arg("temp") = 1., py::arg("maxiter") = 1000, py::arg("mbsize") = py::ssize_t(-1),
+          py::arg("mbn") = py::ssize_t(-1), py::arg("savepref") = "", py::arg("weights") = py::none());
+    m.def("scluster", &scluster_dense, py::arg("matrix"), py::arg("centers"), py::arg("msr") = 2, py::arg("prior") = 0.,
+          py::arg("temp") = 1., py::arg("maxiter") = 1000, py::arg("mbsize") = py::ssize_t(-1),
+          py::arg("mbn") = py::ssize_t(-1), py::arg("savepref") = "", py::arg("weights") = py::none());
+
+    m.def("cmp", &cmp_csr_array, py::arg("matrix"), py::arg("data"), py::arg("msr") = 2, py::arg("prior") = 0.,
+          py::arg("reverse") = false);
+    m.def("cmp", &cmp_csr_csr, py::arg("matrix"), py::arg("data"), py::arg("msr") = 2, py::arg("prior") = 0.);
+    m.def("cmp", &cmp_arrays, py::arg("lhs"), py::arg("rhs"), py::arg("msr") = 2, py::arg("prior") = 0.,
+          py::arg("reverse") = true, py::arg("use_double") = -1);
+
+    m.def("kmeanspp", &kmeanspp_dense, py::arg("matrix"), py::arg("k"), py::arg("msr") = py::int_(2),
+          py::arg("prior") = 0., py::arg("seed") = 0, py::arg("ntimes") = 0, py::arg("lspp") = 0,
+          py::arg("weights") = py::none(), py::arg("expskips") = false, py::arg("n_local_trials") = 1);
+    m.def("kmeanspp", &kmeanspp_csr, py::arg("smw"), py::arg("msr"), py::arg("k"), py::arg("prior") = 0.,
+          py::arg("seed") = 0, py::arg("ntimes") = 1, py::arg("lspp") = 0, py::arg("expskips") = false,
+          py::arg("n_local_trials") = 1, py::arg("weights") = py::none());
+
+    py::class_<CoresetSampler>(m, "CoresetSampler")
+        .def(py::init<>())
+        .def("make_sampler", &CoresetSampler::make_sampler, py::arg("ncenters"), py::arg("costs"),
+             py::arg("assignments"), py::arg("weights") = py::none(), py::arg("seed") = 13, py::arg("sens") = 0)
+        .def("get_probs", &CoresetSampler::get_probs)
+        .def("sample", &CoresetSampler::sample, py::arg("size"), py::arg("seed") = 0, py::arg("unique") = false,
+             py::arg("eps") = 0.1);
+
+    m.def("set_sqrl2_precision", [](const std::string &p) {
+        if (p == "bf16") g_sqrl2_precision = MCG_PREC_BF16;
+        else if (p == "fp32") g_sqrl2_precision = MCG_PREC_FP32;
+        else if (p == "exact") g_sqrl2_precision = MCG_PREC_EXACT;
+        else throw std::invalid_argument("precision must be bf16, fp32 or exact");
+    }, py::arg("precision"), "Cross-term form for dense squared-L2 clustering (default bf16, exact argmin)");
+}

@@ -51,10 +51,11 @@ def time_form(ds, ctx, mcg, C0, asn, prec, steps, warmup):
     t1 = time.perf_counter()
     e_ms, e_n = ctx.timing_read(0)
     m_ms, m_n = ctx.timing_read(1)
+    pairs, full = ctx.refine_counts()
     ctx.timing(False)
     lat.sort()
     return dict(sec_per_step=(t1 - t0) / steps, estep_ms=e_ms / max(e_n, 1), estep_launches=e_n,
-                mstep_partial_ms=m_ms / max(m_n, 1), cost=cost,
+                mstep_partial_ms=m_ms / max(m_n, 1), cost=cost, refine_pairs=pairs, refine_full=full,
                 p50_ms=1e3 * lat[len(lat) // 2], p99_ms=1e3 * lat[min(len(lat) - 1, int(0.99 * len(lat)))])
 
 
@@ -150,6 +151,7 @@ def main():
                      "kernel": "k_estep_bf16<16,2>", "kernel_ms": bf["estep_ms"],
                      "algorithmic": "2*n*k*d flop per launch"},
         "latency_ms": {"p50": bf["p50_ms"], "p99": bf["p99_ms"]},
+        "near_ties_last_step": {"pairs": bf["refine_pairs"], "full": bf["refine_full"]},
         "mstep": {"partial_kernel_ms": bf["mstep_partial_ms"],
                   "hbm_GBs": mstep_bytes / (bf["mstep_partial_ms"] * 1e-3) / 1e9},
         "cpu_baseline": cpu,

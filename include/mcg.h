@@ -86,6 +86,11 @@ int mcg_timing_enable(mcg_ctx *ctx, int on);
 int mcg_timing_reset(mcg_ctx *ctx);
 int mcg_timing_read(mcg_ctx *ctx, int which, double *total_ms, uint64_t *count);
 
+/* Points of the last BF16 E-step on this context that needed an exact re-score:
+ * `pairs` chose between their two nearest centers, `full` were re-scored over
+ * all centers (no counterpart in the reference; for measurement). */
+int mcg_refine_counts(mcg_ctx *ctx, uint64_t *pairs, uint64_t *full);
+
 /* ---- datasets ----------------------------------------------------------- */
 /* Dense rows.  Replaces the numpy-array view that python/pycluster_dense.cpp:5-30
  * and python/pydense.cpp:34-77 build (c_style|forcecast arrays of float/double).

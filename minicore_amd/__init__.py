@@ -11,6 +11,8 @@ There is no CPU fallback: importing the wrappers needs the built extension.
 import numpy as np
 
 try:
+    from ._lib import _share_hip_runtime
+    _share_hip_runtime()
     from . import _pyminicore
 except ImportError as e:  # pragma: no cover - exercised when the extension is not built
     _pyminicore = None

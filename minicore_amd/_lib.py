@@ -21,10 +21,21 @@ EXPORTS = [
     "mcg_ctx_stream", "mcg_data_dense", "mcg_data_csr", "mcg_data_free", "mcg_data_shape",
     "mcg_cost_matrix", "mcg_assign_hard", "mcg_centroids_hard", "mcg_lloyd", "mcg_soft_em",
     "mcg_kmeanspp", "mcg_coreset_probs", "mcg_coreset_sample", "mcg_lloyd_step", "mcg_timing_enable",
-    "mcg_timing_reset", "mcg_timing_read",
+    "mcg_timing_reset", "mcg_timing_read", "mcg_refine_counts",
 ]
 
 _lib = None
+
+
+def _share_hip_runtime():
+    """torch ships its own libamdhip64 (same SONAME as /opt/rocm's).  If libmcg
+    loaded first, torch would later load a second HIP runtime into the process,
+    which cannot see the GPU.  Importing torch first makes libmcg bind to the
+    runtime already loaded."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
 
 
 class MCGError(RuntimeError):
@@ -39,6 +50,7 @@ def load():
         return _lib
     if not os.path.exists(LIBPATH):
         raise RuntimeError(f"{LIBPATH} is missing: build it with `python -m minicore_amd.build`")
+    _share_hip_runtime()
     L = C.CDLL(LIBPATH)
     P, U64, D, I = C.c_void_p, C.c_uint64, C.c_double, C.c_int
     sig = {
@@ -64,6 +76,7 @@ def load():
         "mcg_timing_enable": (I, [P, I]),
         "mcg_timing_reset": (I, [P]),
         "mcg_timing_read": (I, [P, I, P, P]),
+        "mcg_refine_counts": (I, [P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -120,6 +133,11 @@ class Context:
     def timing(self, on=True):
         check(load().mcg_timing_enable(self.h, 1 if on else 0))
         check(load().mcg_timing_reset(self.h))
+
+    def refine_counts(self):
+        a, b = C.c_uint64(), C.c_uint64()
+        check(load().mcg_refine_counts(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def timing_read(self, which):
         ms, cnt = C.c_double(), C.c_uint64()

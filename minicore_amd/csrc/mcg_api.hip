@@ -72,7 +72,16 @@ void Ctx::collect() {
     }
 }
 
+uint32_t *Ctx::refine_host() {
+    if (!refine_counts) {
+        HIP_CHECK(hipHostMalloc((void **)&refine_counts, 4 * sizeof(uint32_t)));
+        refine_counts[0] = refine_counts[1] = 0;
+    }
+    return refine_counts;
+}
+
 Ctx::~Ctx() {
+    if (refine_counts) (void)hipHostFree(refine_counts);
     for (int w = 0; w < 2; ++w)
         for (auto &pr : pending[w]) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
     if (stream) (void)hipStreamDestroy(stream);
@@ -863,6 +872,16 @@ int mcg_lloyd_step(mcg_data *ds, int msr, double prior, float *centers, uint64_t
         }
         sync(c);
         if (cost) *cost = h;
+    });
+}
+
+int mcg_refine_counts(mcg_ctx *ctx, uint64_t *pairs, uint64_t *full) {
+    return guard([&] {
+        REQUIRE(ctx, "null ctx");
+        sync(ctx->c);
+        const uint32_t *r = ctx->c.refine_host();
+        if (pairs) *pairs = r[0];
+        if (full) *full = r[1];
     });
 }
 

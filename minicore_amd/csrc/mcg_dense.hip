@@ -11,7 +11,7 @@
 //    tiles of 64.  Each lane keeps a running best/second of its column with the center's
 //    tile-local position packed into the low 6 mantissa bits (3 VALU ops per value).
 //    Points whose best/second gap is not provably larger than the BF16 error bound are
-//    re-scored exactly (k_rescore).
+//    re-scored exactly (k_refine_pairs, k_rescore_full).
 //  * FP32: the same GEMM form on v_mfma_f32_32x32x2f32, plain compare/select.
 //  * EXACT: sum of squared differences per pair as a sequential FMA chain, FP32.
 #include "mcg_internal.h"
@@ -177,16 +177,33 @@ __global__ void k_prep_params(const double *__restrict__ cn, uint64_t k, uint64_
 // BF16 MFMA E-step.  KS = dpad / 16 k-steps; NB n-blocks of 32 points per wave.
 // K-permutation: lane half h supplies dims [h*dpad/2 + 8s, +8) at step s for both
 // operands, so each lane reads contiguous 16-byte chunks.
+//
+// Center tiles (64 rows) go global -> LDS directly (global_load_lds_dwordx4), with
+// the 16-byte chunk index XOR-swizzled by the row so that the 16 rows read by one
+// ds_read_b128 phase hit distinct banks.  Each lane keeps the three smallest packed
+// values of its point column and the tiles of the best two; the halves are merged
+// at the end and every point is classified:
+//   certain  : second - best > tau                 -> asn = best
+//   pair     : third - best > tau >= second - best -> exact re-score of {best, second}
+//   full     : otherwise                           -> exact re-score over all centers
 // ---------------------------------------------------------------------------
+struct PairEntry {
+    uint32_t p, c1, c2, pad;
+};
+
+template <int CH>
+__device__ __forceinline__ int swz(int row) { return row & ((CH < 16 ? CH : 16) - 1); }
+
 template <int KS, int NB>
-__global__ __launch_bounds__(256) void k_estep_bf16(
+__global__ __launch_bounds__(256, 2) void k_estep_bf16(
     const uint4 *__restrict__ Xb, const uint4 *__restrict__ Cb, const float *__restrict__ cadd,
     const float *__restrict__ xnorm, const EParams *__restrict__ prm, uint32_t n, uint32_t ntiles, uint32_t kreal,
-    uint32_t *__restrict__ asn, uint32_t *__restrict__ flags, uint32_t *__restrict__ nflag) {
-    constexpr int CH = KS * 2;               // 16-byte chunks per row
-    constexpr int ROWB = CH * 16 + 16;       // padded LDS row stride (bytes)
-    constexpr int CPT = (64 * CH) / 256;     // tile chunks per thread
-    __shared__ __attribute__((aligned(16))) char lds[2][64 * ROWB];
+    uint32_t *__restrict__ asn, PairEntry *__restrict__ pairs, uint32_t *__restrict__ npairs,
+    uint32_t *__restrict__ full, uint32_t *__restrict__ nfull) {
+    constexpr int CH = KS * 2;                  // 16-byte chunks per row
+    constexpr int TILE = 64 * CH * 16;          // bytes per tile
+    constexpr int INSTR = CH / 4;               // global_load_lds per wave per tile
+    __shared__ __attribute__((aligned(16))) char lds[2][TILE];
     __shared__ float ladd[2][64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int h = lane >> 5, r32 = lane & 31;
@@ -199,58 +216,59 @@ __global__ __launch_bounds__(256) void k_estep_bf16(
         p = p < n ? p : n - 1;
         const uint4 *row = Xb + (size_t)p * CH + h * KS;
 #pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            uint4 v = row[s];
-            b[nb][s] = __builtin_bit_cast(bf16x8, v);
-        }
+        for (int s = 0; s < KS; ++s) b[nb][s] = __builtin_bit_cast(bf16x8, row[s]);
     }
 
-    uint4 pf[CPT];
-    float addn = 0.f;
-    auto load_tile = [&](uint32_t t) {
+    // direct global->LDS copy of tile t into buffer buf (swizzled)
+    auto issue_tile = [&](uint32_t t, int buf) {
 #pragma unroll
-        for (int q = 0; q < CPT; ++q) {
-            int g = tid + q * 256;
-            pf[q] = Cb[(size_t)(t * 64 + g / CH) * CH + (g % CH)];
+        for (int j = 0; j < INSTR; ++j) {
+            const int P = (j * 4 + wave) * 64 + lane;   // physical chunk slot in the tile
+            const int r = P / CH, q = P % CH;
+            const uint4 *src = Cb + (size_t)(t * 64 + r) * CH + (q ^ swz<CH>(r));
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                             (__attribute__((address_space(3))) void *)(lds[buf] + (j * 4 + wave) * 64 * 16),
+                                             16, 0, 0);
         }
-        if (tid < 64) addn = cadd[t * 64 + tid];
-    };
-    auto store_tile = [&](int buf) {
-#pragma unroll
-        for (int q = 0; q < CPT; ++q) {
-            int g = tid + q * 256;
-            *(uint4 *)(lds[buf] + (g / CH) * ROWB + (g % CH) * 16) = pf[q];
-        }
-        if (tid < 64) ladd[buf][tid] = addn;
     };
 
-    float best[NB], sec[NB];
-    uint32_t btile[NB];
+    float best[NB], sec[NB], thr[NB];
+    uint32_t btile[NB], stile[NB];
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb) { best[nb] = FLT_MAX; sec[nb] = FLT_MAX; btile[nb] = 0; }
+    for (int nb = 0; nb < NB; ++nb) {
+        best[nb] = FLT_MAX; sec[nb] = FLT_MAX; thr[nb] = FLT_MAX; btile[nb] = 0; stile[nb] = 0;
+    }
 
-    load_tile(0);
-    store_tile(0);
+    issue_tile(0, 0);
+    if (tid < 64) ladd[0][tid] = cadd[tid];
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     for (uint32_t t = 0; t < ntiles; ++t) {
         const int buf = t & 1;
-        if (t + 1 < ntiles) load_tile(t + 1);
-        float bstart[NB];
+        float addn = 0.f;
+        if (t + 1 < ntiles) {
+            issue_tile(t + 1, buf ^ 1);
+            if (tid < 64) addn = cadd[(t + 1) * 64 + tid];
+        }
+        float bstart[NB], sstart[NB];
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) bstart[nb] = best[nb];
-#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) { bstart[nb] = best[nb]; sstart[nb] = sec[nb]; }
+        const char *tb = lds[buf];
+#pragma unroll 1
         for (int mb = 0; mb < 2; ++mb) {
             floatx16 acc[NB];
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
-                float v = ladd[buf][mb * 32 + 8 * (e >> 2) + 4 * h + (e & 3)];
+                const float v = ladd[buf][mb * 32 + 8 * (e >> 2) + 4 * h + (e & 3)];
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb) acc[nb][e] = v;
             }
-            const char *arow = lds[buf] + (mb * 32 + r32) * ROWB + h * KS * 16;
+            const int row = mb * 32 + r32;
+            const char *arow = tb + row * (CH * 16);
+            const int sw = swz<CH>(row);
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
-                bf16x8 a = *(const bf16x8 *)(arow + s * 16);
+                const bf16x8 a = *(const bf16x8 *)(arow + (((h * KS + s) ^ sw) * 16));
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb)
                     acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[nb][s], acc[nb], 0, 0, 0);
@@ -260,74 +278,135 @@ __global__ __launch_bounds__(256) void k_estep_bf16(
 #pragma unroll
                 for (int e = 0; e < 16; ++e) {
                     const uint32_t pos = mb * 32 + 8 * (e >> 2) + (e & 3);
-                    float v = __uint_as_float((__float_as_uint(acc[nb][e]) & 0xFFFFFFC0u) | pos);
+                    const float v = __uint_as_float((__float_as_uint(acc[nb][e]) & 0xFFFFFFC0u) | pos);
+                    thr[nb] = __builtin_amdgcn_fmed3f(sec[nb], v, thr[nb]);
                     sec[nb] = __builtin_amdgcn_fmed3f(best[nb], v, sec[nb]);
                     best[nb] = fminf(best[nb], v);
                 }
             }
         }
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb)
-            if (__float_as_uint(best[nb]) != __float_as_uint(bstart[nb])) btile[nb] = t;
-        if (t + 1 < ntiles) store_tile(buf ^ 1);
+        for (int nb = 0; nb < NB; ++nb) {
+            const uint32_t bb = __float_as_uint(best[nb]), b0 = __float_as_uint(bstart[nb]);
+            const uint32_t ss = __float_as_uint(sec[nb]), s0 = __float_as_uint(sstart[nb]);
+            if (bb != b0) {
+                stile[nb] = (ss == b0) ? btile[nb] : t;
+                btile[nb] = t;
+            } else if (ss != s0) {
+                stile[nb] = t;
+            }
+        }
+        if (t + 1 < ntiles && tid < 64) ladd[buf ^ 1][tid] = addn;
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
     }
 
     const EParams P = *prm;
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
-        const uint32_t bb = __float_as_uint(best[nb]);
-        const uint32_t gi = btile[nb] * 64u + (bb & 63u) + 4u * h;
-        const float bv = __uint_as_float(bb & 0xFFFFFFC0u);
-        const float sv = __uint_as_float(__float_as_uint(sec[nb]) & 0xFFFFFFC0u);
-        const float obv = __shfl_xor(bv, 32);
-        const uint32_t ogi = __shfl_xor(gi, 32);
-        const float osv = __shfl_xor(sv, 32);
-        const bool other = (obv < bv) || (obv == bv && ogi < gi);
-        const float fb = other ? obv : bv;
-        const uint32_t fi = other ? ogi : gi;
-        const float fs = fminf(fmaxf(bv, obv), fminf(sv, osv));
+        const uint32_t bb = __float_as_uint(best[nb]), ss = __float_as_uint(sec[nb]);
+        const uint32_t gb = btile[nb] * 64u + (bb & 63u) + 4u * h;
+        const uint32_t gs = stile[nb] * 64u + (ss & 63u) + 4u * h;
+        const float b = __uint_as_float(bb & 0xFFFFFFC0u);
+        const float s = __uint_as_float(ss & 0xFFFFFFC0u);
+        const float r = __uint_as_float(__float_as_uint(thr[nb]) & 0xFFFFFFC0u);
+        const float ob = __shfl_xor(b, 32), os = __shfl_xor(s, 32);
+        const uint32_t ogb = __shfl_xor(gb, 32), ogs = __shfl_xor(gs, 32);
+        const float orr = __shfl_xor(r, 32);
+        const bool oth = (ob < b) || (ob == b && ogb < gb);
+        const float v1 = oth ? ob : b;
+        const uint32_t c1 = oth ? ogb : gb;
+        const float sA = oth ? os : s, rA = oth ? orr : r, bB = oth ? b : ob, sB = oth ? s : os;
+        const uint32_t gsA = oth ? ogs : gs, gB = oth ? gb : ogb;
+        const bool secB = (bB < sA) || (bB == sA && gB < gsA);
+        const float v2 = secB ? bB : sA;
+        const uint32_t c2 = secB ? gB : gsA;
+        const float v3 = secB ? fminf(sA, sB) : fminf(rA, bB);
         const uint32_t p = pbase + nb * 32 + r32;
         if (h == 0 && p < n) {
-            // a non-finite row can leave no real center selected: re-score it exactly
-            const bool ok = fi < kreal;
-            asn[p] = ok ? fi : 0u;
             const float xn = xnorm[p];
-            const float dbest = xn * xn + (fb - P.off);
+            const float dbest = xn * xn + (v1 - P.off);
             const float tau = P.tau1 * xn + P.tau0 + P.taur * fmaxf(dbest, 0.f);
-            if (!ok || !(fs - fb > tau)) flags[atomicAdd(nflag, 1u)] = p;
+            const bool ok1 = c1 < kreal;
+            asn[p] = ok1 ? c1 : 0u;
+            if (ok1 && v2 - v1 > tau) {
+                // certain
+            } else if (ok1 && c2 < kreal && v3 - v1 > tau) {
+                pairs[atomicAdd(npairs, 1u)] = PairEntry{p, c1, c2, 0u};
+            } else {
+                full[atomicAdd(nfull, 1u)] = p;
+            }
         }
     }
 }
 
-// Exact FP32 argmin for the listed points: one wave per point, lanes over centers,
-// each pair a sequential FMA chain over the dims (lowest index wins ties).
-__global__ void k_rescore(const float *__restrict__ X, uint32_t d, const float *__restrict__ C, uint32_t k,
-                          const uint32_t *__restrict__ list, const uint32_t *__restrict__ nlist,
-                          uint32_t *__restrict__ asn) {
-    const int lane = threadIdx.x & 63;
+// Exact choice between two candidates (sequential FMA chains; lowest index on ties).
+__global__ void k_refine_pairs(const float *__restrict__ X, uint32_t d, const float *__restrict__ C,
+                               const PairEntry *__restrict__ pairs, const uint32_t *__restrict__ npairs,
+                               uint32_t *__restrict__ asn) {
+    const uint32_t cnt = *npairs;
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < cnt; w += gridDim.x * blockDim.x) {
+        const PairEntry e = pairs[w];
+        const float *x = X + (uint64_t)e.p * d;
+        const float *a = C + (uint64_t)e.c1 * d, *bq = C + (uint64_t)e.c2 * d;
+        float s1 = 0.f, s2 = 0.f;
+        for (uint32_t j = 0; j < d; ++j) {
+            const float xv = x[j];
+            const float t1 = xv - a[j], t2 = xv - bq[j];
+            s1 = fmaf(t1, t1, s1);
+            s2 = fmaf(t2, t2, s2);
+        }
+        const bool take2 = (s2 < s1) || (s2 == s1 && e.c2 < e.c1);
+        asn[e.p] = take2 ? e.c2 : e.c1;
+    }
+}
+
+// Exact argmin over all centers for the listed points: 256 listed points per block,
+// their rows staged through LDS in 32-dim slabs, 16 centers per register tile.
+__global__ __launch_bounds__(256) void k_rescore_full(const float *__restrict__ X, uint32_t d,
+                                                      const float *__restrict__ C, uint32_t k,
+                                                      const uint32_t *__restrict__ list,
+                                                      const uint32_t *__restrict__ nlist,
+                                                      uint32_t *__restrict__ asn) {
+    __shared__ float xs[256][33];
+    __shared__ float cs[16][32];
+    __shared__ uint32_t pts[256];
     const uint32_t cnt = *nlist;
-    for (uint32_t w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); w < cnt;
-         w += gridDim.x * (blockDim.x / 64)) {
-        const uint32_t p = list[w];
-        const float *x = X + (uint64_t)p * d;
+    const int tid = threadIdx.x;
+    for (uint32_t base = blockIdx.x * 256u; base < cnt; base += gridDim.x * 256u) {
+        __syncthreads();
+        pts[tid] = base + tid < cnt ? list[base + tid] : 0xFFFFFFFFu;
+        __syncthreads();
         float best = 0.f;
         uint32_t bi = 0xFFFFFFFFu;
-        for (uint32_t c = lane; c < k; c += 64) {
-            const float *cc = C + (uint64_t)c * d;
-            float s = 0.f;
-            for (uint32_t j = 0; j < d; ++j) { float t = x[j] - cc[j]; s = fmaf(t, t, s); }
-            if (bi == 0xFFFFFFFFu || s < best) { best = s; bi = c; }
+        for (uint32_t c0 = 0; c0 < k; c0 += 16) {
+            float acc[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) acc[c] = 0.f;
+            for (uint32_t j0 = 0; j0 < d; j0 += 32) {
+                __syncthreads();
+                for (int q = tid; q < 256 * 32; q += 256) {
+                    const int r = q >> 5, jj = q & 31;
+                    const uint32_t pr = pts[r];
+                    xs[r][jj] = (pr != 0xFFFFFFFFu && j0 + jj < d) ? X[(uint64_t)pr * d + j0 + jj] : 0.f;
+                }
+                for (int q = tid; q < 16 * 32; q += 256) {
+                    const int c = q >> 5, jj = q & 31;
+                    cs[c][jj] = (c0 + c < k && j0 + jj < d) ? C[(uint64_t)(c0 + c) * d + j0 + jj] : 0.f;
+                }
+                __syncthreads();
+                const int jn = (int)min<uint32_t>(32, d - j0);
+                for (int jj = 0; jj < jn; ++jj) {
+                    const float xv = xs[tid][jj];
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) { const float t = xv - cs[c][jj]; acc[c] = fmaf(t, t, acc[c]); }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+                if (c0 + c < k && (bi == 0xFFFFFFFFu || acc[c] < best)) { best = acc[c]; bi = c0 + c; }
         }
-        // wave argmin, lowest index on ties
-        for (int o = 32; o > 0; o >>= 1) {
-            float ob = __shfl_xor(best, o);
-            uint32_t oi = __shfl_xor(bi, o);
-            bool take = (oi != 0xFFFFFFFFu) &&
-                        (bi == 0xFFFFFFFFu || ob < best || (ob == best && oi < bi));
-            if (take) { best = ob; bi = oi; }
-        }
-        if (lane == 0) asn[p] = bi;
+        if (pts[tid] != 0xFFFFFFFFu) asn[pts[tid]] = bi;
     }
 }
 
@@ -513,8 +592,9 @@ static void launch_bf16(Data &ds, Ctx &c, uint64_t kpad, uint64_t kreal, uint32_
     hipLaunchKernelGGL((k_estep_bf16<KS, NB>), dim3((uint32_t)cdiv(n, per)), dim3(256), 0, st,
                        ds.Xb.as<const uint4>(), c.s[S_CB].as<const uint4>(), c.s[S_CADD].as<const float>(),
                        ds.xnorm.as<const float>(), c.s[S_PARAMS].as<const EParams>(), n,
-                       (uint32_t)(kpad / 64), (uint32_t)kreal, asn, c.s[S_FLAGS].as<uint32_t>(),
-                       c.s[S_FLAGCNT].as<uint32_t>());
+                       (uint32_t)(kpad / 64), (uint32_t)kreal, asn, c.s[S_PAIRS].as<PairEntry>(),
+                       c.s[S_FLAGCNT].as<uint32_t>(), c.s[S_FLAGS].as<uint32_t>(),
+                       c.s[S_FLAGCNT].as<uint32_t>() + 1);
     c.toc(0, ev);
 }
 
@@ -572,20 +652,24 @@ void dense_sqrl2_assign(Data &ds, const float *C, uint64_t k, int precision, uin
     c.s[S_CNORM].reserve(kpad * sizeof(double));
     c.s[S_PARAMS].reserve(sizeof(EParams));
     c.s[S_FLAGS].reserve(ds.n * sizeof(uint32_t));
-    c.s[S_FLAGCNT].reserve(sizeof(uint32_t));
+    c.s[S_PAIRS].reserve(ds.n * sizeof(PairEntry));
+    c.s[S_FLAGCNT].reserve(4 * sizeof(uint32_t));
     hipLaunchKernelGGL(k_prep_centers_bf16, dim3((uint32_t)cdiv(kpad, 4)), dim3(256), 0, st, C, k, kpad, d, dpad,
                        ds.mu.as<const float>(), c.s[S_CB].as<uint16_t>(), c.s[S_CNORM].as<double>());
     hipLaunchKernelGGL(k_prep_params, dim3(1), dim3(256), 0, st, c.s[S_CNORM].as<const double>(), k, kpad,
                        ds.xmax, d, c.s[S_CADD].as<float>(), c.s[S_PARAMS].as<EParams>());
-    fill_u32(c.s[S_FLAGCNT].as<uint32_t>(), 0u, 1, st);
+    fill_u32(c.s[S_FLAGCNT].as<uint32_t>(), 0u, 4, st);
     switch (dpad) {
     case 64: launch_bf16<4, 2>(ds, c, kpad, k, asn, st); break;
     case 128: launch_bf16<8, 2>(ds, c, kpad, k, asn, st); break;
     case 256: launch_bf16<16, 2>(ds, c, kpad, k, asn, st); break;
     default: launch_bf16<32, 1>(ds, c, kpad, k, asn, st); break;
     }
-    hipLaunchKernelGGL(k_rescore, dim3(512), dim3(256), 0, st, ds.X, d, C, (uint32_t)k,
-                       c.s[S_FLAGS].as<const uint32_t>(), c.s[S_FLAGCNT].as<const uint32_t>(), asn);
+    hipLaunchKernelGGL(k_refine_pairs, dim3(1024), dim3(256), 0, st, ds.X, d, C, c.s[S_PAIRS].as<const PairEntry>(),
+                       c.s[S_FLAGCNT].as<const uint32_t>(), asn);
+    hipLaunchKernelGGL(k_rescore_full, dim3(512), dim3(256), 0, st, ds.X, d, C, (uint32_t)k,
+                       c.s[S_FLAGS].as<const uint32_t>(), c.s[S_FLAGCNT].as<const uint32_t>() + 1, asn);
+    HIP_CHECK(hipMemcpyAsync(c.refine_host(), c.s[S_FLAGCNT].p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
 }
 
 }  // namespace mcg

@@ -97,8 +97,11 @@ struct Ctx {
     void *host_pinned = nullptr;
     size_t host_pinned_bytes = 0;
     void *pinned(size_t nb);
+    uint32_t *refine_host();
     // kernel timing: pending (start, stop) event pairs per timed kernel
     bool timing = false;
+    // near-tie statistics of the last BF16 E-step {pairs, full re-scores}, pinned host memory
+    uint32_t *refine_counts = nullptr;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[2];
     double timed_ms[2] = {0., 0.};
     uint64_t timed_n[2] = {0, 0};
@@ -114,7 +117,7 @@ enum Slot {
     S_HIST, S_BASE, S_PERM, S_CNT, S_ITEMOFF, S_PART, S_PARTW, S_PARTQ,
     S_CCOST, S_TOTAL, S_CENT_A, S_CENT_B, S_CENT_C, S_COSTS, S_CT, S_CPARAM,
     S_MISC0, S_MISC1, S_MISC2, S_MISC3, S_W, S_STAGE0, S_STAGE1, S_STAGE2,
-    S_LIS, S_TOT, S_JSDG, S_NSLOTS
+    S_LIS, S_TOT, S_JSDG, S_PAIRS, S_NSLOTS
 };
 static_assert(S_NSLOTS <= 48, "scratch slots");
 

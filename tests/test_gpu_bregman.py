@@ -147,10 +147,13 @@ def test_soft_em(mcg, ctx, mco, name, code, sparse):
         C0 = X[:4].copy() + 0.5
         ds = mcg.Dataset.dense(X, ctx)
         prior = 0.5
-    temp = 0.05 if code == 2 else 1.
-    ref = mco.soft(code, X, C0, prior=prior, temp=temp, maxiter=6)
-    got = ds.soft(code, C0, prior=prior, temp=temp, maxiter=6)
-    assert got["numiter"] == ref["numiter"]
+    # a temperature at the cost scale keeps the responsibilities soft (no underflow to argmin-uniform)
+    K0 = mco.cost_matrix(code, X, C0, prior)
+    temp = float(2. / np.median(K0)) if code == 2 else 1.
+    # eps < 0: both run exactly maxiter iterations
+    ref = mco.soft(code, X, C0, prior=prior, temp=temp, eps=-1., maxiter=6)
+    got = ds.soft(code, C0, prior=prior, temp=temp, eps=-1., maxiter=6)
+    assert got["numiter"] == ref["numiter"] == 6
     assert got["initcost"] == pytest.approx(ref["initcost"], rel=2e-4)
     assert got["finalcost"] == pytest.approx(ref["finalcost"], rel=2e-3)
     np.testing.assert_allclose(got["asn"], ref["asn"], atol=2e-3)

@@ -15,6 +15,7 @@
 //  * FP32: the same GEMM form on v_mfma_f32_32x32x2f32, plain compare/select.
 //  * EXACT: sum of squared differences per pair as a sequential FMA chain, FP32.
 #include "mcg_internal.h"
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 
@@ -180,26 +181,64 @@ __global__ void k_prep_params(const double *__restrict__ cn, uint64_t k, uint64_
 //
 // Center tiles (64 rows) go global -> LDS directly (global_load_lds_dwordx4), with
 // the 16-byte chunk index XOR-swizzled by the row so that the 16 rows read by one
-// ds_read_b128 phase hit distinct banks.  Each lane keeps the three smallest packed
-// values of its point column and the tiles of the best two; the halves are merged
-// at the end and every point is classified:
-//   certain  : second - best > tau                 -> asn = best
-//   pair     : third - best > tau >= second - best -> exact re-score of {best, second}
-//   full     : otherwise                           -> exact re-score over all centers
+// ds_read_b128 phase hit distinct banks.
+//
+// Main pass (COLLECT = false): each lane keeps the three smallest packed values of
+// its point column and the tiles of the best two; the halves are merged at the end
+// and every point is classified:
+//   certain : second - best > tau                 -> asn = best
+//   pair    : third - best > tau >= second - best -> candidates {best, second}
+//   full    : otherwise                           -> listed with threshold best + tau
+// Collect pass (COLLECT = true) over the listed points: every center whose value is
+// within the threshold becomes a candidate.  Candidates are then scored exactly.
 // ---------------------------------------------------------------------------
-struct PairEntry {
-    uint32_t p, c1, c2, pad;
+struct Cand {
+    uint32_t p, c;
+};
+
+struct RefineBufs {
+    Cand *cand;          // candidate (point, center) pairs
+    uint32_t cap;        // capacity of cand
+    uint32_t *cnt;       // [0] candidates, [1] listed (full) points, [2] overflow points, [3] pair points
+    uint32_t *full;      // listed points
+    float *thresh;       // per listed point: best + tau (packed-value space)
+    uint32_t *over;      // points whose candidates did not fit (each listed once)
+    uint32_t *oflag;     // per point: 1 while listed in `over` (reset by the full re-score)
+    unsigned long long *key;  // per point: packed (exact cost bits << 32 | center), min-reduced
 };
 
 template <int CH>
 __device__ __forceinline__ int swz(int row) { return row & ((CH < 16 ? CH : 16) - 1); }
 
-template <int KS, int NB>
-__global__ __launch_bounds__(256, 2) void k_estep_bf16(
+__device__ __forceinline__ uint32_t lane_prefix(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Wave-aggregated append of `cnt_per_lane` (0, 1 or 2) candidates per lane: one
+// atomic per wave.  Slots past the capacity send the point to the overflow list.
+__device__ __forceinline__ void push_cands(const RefineBufs &R, bool want, uint32_t per, uint32_t p, uint32_t c1,
+                                           uint32_t c2) {
+    const uint64_t m = __ballot(want);
+    if (!m) return;
+    const int leader = __builtin_ctzll(m);
+    uint32_t base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(&R.cnt[0], per * (uint32_t)__builtin_popcountll(m));
+    base = __shfl(base, leader);
+    if (!want) return;
+    const uint32_t i = base + per * lane_prefix(m);
+    if (i + per <= R.cap) {
+        R.cand[i] = Cand{p, c1};
+        if (per == 2) R.cand[i + 1] = Cand{p, c2};
+    } else if (atomicExch(&R.oflag[p], 1u) == 0u) {
+        R.over[atomicAdd(&R.cnt[2], 1u)] = p;
+    }
+}
+
+template <int KS, int NB, bool COLLECT>
+__global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep_bf16(
     const uint4 *__restrict__ Xb, const uint4 *__restrict__ Cb, const float *__restrict__ cadd,
     const float *__restrict__ xnorm, const EParams *__restrict__ prm, uint32_t n, uint32_t ntiles, uint32_t kreal,
-    uint32_t *__restrict__ asn, PairEntry *__restrict__ pairs, uint32_t *__restrict__ npairs,
-    uint32_t *__restrict__ full, uint32_t *__restrict__ nfull) {
+    uint32_t *__restrict__ asn, RefineBufs R) {
     constexpr int CH = KS * 2;                  // 16-byte chunks per row
     constexpr int TILE = 64 * CH * 16;          // bytes per tile
     constexpr int INSTR = CH / 4;               // global_load_lds per wave per tile
@@ -207,13 +246,24 @@ __global__ __launch_bounds__(256, 2) void k_estep_bf16(
     __shared__ float ladd[2][64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int h = lane >> 5, r32 = lane & 31;
-    const uint32_t pbase = blockIdx.x * (4u * NB * 32u) + wave * (NB * 32u);
+    const uint32_t blk0 = blockIdx.x * (4u * NB * 32u);
+    uint32_t count = n;
+    if (COLLECT) {
+        count = R.cnt[1];
+        if (blk0 >= count) return;   // uniform over the block
+    }
+    const uint32_t pbase = blk0 + wave * (NB * 32u);
 
     bf16x8 b[NB][KS];
+    uint32_t pt[NB];
+    float thr_in[NB];
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
-        uint32_t p = pbase + nb * 32 + r32;
-        p = p < n ? p : n - 1;
+        uint32_t q = pbase + nb * 32 + r32;
+        q = q < count ? q : count - 1;
+        const uint32_t p = COLLECT ? R.full[q] : q;
+        pt[nb] = p;
+        thr_in[nb] = COLLECT ? R.thresh[q] : 0.f;
         const uint4 *row = Xb + (size_t)p * CH + h * KS;
 #pragma unroll
         for (int s = 0; s < KS; ++s) b[nb][s] = __builtin_bit_cast(bf16x8, row[s]);
@@ -239,16 +289,20 @@ __global__ __launch_bounds__(256, 2) void k_estep_bf16(
         best[nb] = FLT_MAX; sec[nb] = FLT_MAX; thr[nb] = FLT_MAX; btile[nb] = 0; stile[nb] = 0;
     }
 
-    issue_tile(0, 0);
-    if (tid < 64) ladd[0][tid] = cadd[tid];
+    // the collect pass spreads the tiles over gridDim.y blocks (no state crosses tiles)
+    const uint32_t t0 = COLLECT ? blockIdx.y : 0u, tstep = COLLECT ? gridDim.y : 1u;
+    if (t0 >= ntiles) return;
+    issue_tile(t0, 0);
+    if (tid < 64) ladd[0][tid] = cadd[t0 * 64 + tid];
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    for (uint32_t t = 0; t < ntiles; ++t) {
-        const int buf = t & 1;
+    int buf = 0;
+    for (uint32_t t = t0; t < ntiles; t += tstep, buf ^= 1) {
         float addn = 0.f;
-        if (t + 1 < ntiles) {
-            issue_tile(t + 1, buf ^ 1);
-            if (tid < 64) addn = cadd[(t + 1) * 64 + tid];
+        const bool more = t + tstep < ntiles;
+        if (more) {
+            issue_tile(t + tstep, buf ^ 1);
+            if (tid < 64) addn = cadd[(t + tstep) * 64 + tid];
         }
         float bstart[NB], sstart[NB];
 #pragma unroll
@@ -278,28 +332,38 @@ __global__ __launch_bounds__(256, 2) void k_estep_bf16(
 #pragma unroll
                 for (int e = 0; e < 16; ++e) {
                     const uint32_t pos = mb * 32 + 8 * (e >> 2) + (e & 3);
-                    const float v = __uint_as_float((__float_as_uint(acc[nb][e]) & 0xFFFFFFC0u) | pos);
-                    thr[nb] = __builtin_amdgcn_fmed3f(sec[nb], v, thr[nb]);
-                    sec[nb] = __builtin_amdgcn_fmed3f(best[nb], v, sec[nb]);
-                    best[nb] = fminf(best[nb], v);
+                    if (COLLECT) {
+                        const float v = __uint_as_float(__float_as_uint(acc[nb][e]) & 0xFFFFFFC0u);
+                        const uint32_t c = t * 64u + pos + 4u * h;
+                        const bool want = v <= thr_in[nb] && c < kreal && pbase + nb * 32 + r32 < count;
+                        push_cands(R, want, 1u, pt[nb], c, 0u);
+                    } else {
+                        const float v = __uint_as_float((__float_as_uint(acc[nb][e]) & 0xFFFFFFC0u) | pos);
+                        thr[nb] = __builtin_amdgcn_fmed3f(sec[nb], v, thr[nb]);
+                        sec[nb] = __builtin_amdgcn_fmed3f(best[nb], v, sec[nb]);
+                        best[nb] = fminf(best[nb], v);
+                    }
                 }
             }
         }
+        if (!COLLECT) {
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            const uint32_t bb = __float_as_uint(best[nb]), b0 = __float_as_uint(bstart[nb]);
-            const uint32_t ss = __float_as_uint(sec[nb]), s0 = __float_as_uint(sstart[nb]);
-            if (bb != b0) {
-                stile[nb] = (ss == b0) ? btile[nb] : t;
-                btile[nb] = t;
-            } else if (ss != s0) {
-                stile[nb] = t;
+            for (int nb = 0; nb < NB; ++nb) {
+                const uint32_t bb = __float_as_uint(best[nb]), b0 = __float_as_uint(bstart[nb]);
+                const uint32_t ss = __float_as_uint(sec[nb]), s0 = __float_as_uint(sstart[nb]);
+                if (bb != b0) {
+                    stile[nb] = (ss == b0) ? btile[nb] : t;
+                    btile[nb] = t;
+                } else if (ss != s0) {
+                    stile[nb] = t;
+                }
             }
         }
-        if (t + 1 < ntiles && tid < 64) ladd[buf ^ 1][tid] = addn;
+        if (more && tid < 64) ladd[buf ^ 1][tid] = addn;
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
     }
+    if (COLLECT) return;
 
     const EParams P = *prm;
 #pragma unroll
@@ -323,51 +387,108 @@ __global__ __launch_bounds__(256, 2) void k_estep_bf16(
         const uint32_t c2 = secB ? gB : gsA;
         const float v3 = secB ? fminf(sA, sB) : fminf(rA, bB);
         const uint32_t p = pbase + nb * 32 + r32;
-        if (h == 0 && p < n) {
+        const bool mine = h == 0 && p < n;
+        bool is_pair = false, is_full = false;
+        float thresh = FLT_MAX;
+        if (mine) {
             const float xn = xnorm[p];
             const float dbest = xn * xn + (v1 - P.off);
             const float tau = P.tau1 * xn + P.tau0 + P.taur * fmaxf(dbest, 0.f);
             const bool ok1 = c1 < kreal;
             asn[p] = ok1 ? c1 : 0u;
-            if (ok1 && v2 - v1 > tau) {
-                // certain
-            } else if (ok1 && c2 < kreal && v3 - v1 > tau) {
-                pairs[atomicAdd(npairs, 1u)] = PairEntry{p, c1, c2, 0u};
-            } else {
-                full[atomicAdd(nfull, 1u)] = p;
+            if (!(ok1 && v2 - v1 > tau)) {
+                R.key[p] = ~0ull;
+                is_pair = ok1 && c2 < kreal && v3 - v1 > tau;
+                is_full = !is_pair;
+                thresh = ok1 ? v1 + tau : FLT_MAX;   // no finite best: every center is a candidate
+            }
+        }
+        push_cands(R, is_pair, 2u, p, c1, c2);
+        const uint64_t mf = __ballot(is_full);
+        if (mf) {
+            const int leader = __builtin_ctzll(mf);
+            uint32_t base = 0;
+            if ((int)__lane_id() == leader) base = atomicAdd(&R.cnt[1], (uint32_t)__builtin_popcountll(mf));
+            base = __shfl(base, leader);
+            if (is_full) {
+                const uint32_t i = base + lane_prefix(mf);
+                R.full[i] = p;
+                R.thresh[i] = thresh;
             }
         }
     }
 }
 
-// Exact choice between two candidates (sequential FMA chains; lowest index on ties).
-__global__ void k_refine_pairs(const float *__restrict__ X, uint32_t d, const float *__restrict__ C,
-                               const PairEntry *__restrict__ pairs, const uint32_t *__restrict__ npairs,
-                               uint32_t *__restrict__ asn) {
-    const uint32_t cnt = *npairs;
-    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < cnt; w += gridDim.x * blockDim.x) {
-        const PairEntry e = pairs[w];
-        const float *x = X + (uint64_t)e.p * d;
-        const float *a = C + (uint64_t)e.c1 * d, *bq = C + (uint64_t)e.c2 * d;
-        float s1 = 0.f, s2 = 0.f;
-        for (uint32_t j = 0; j < d; ++j) {
-            const float xv = x[j];
-            const float t1 = xv - a[j], t2 = xv - bq[j];
-            s1 = fmaf(t1, t1, s1);
-            s2 = fmaf(t2, t2, s2);
+// Exact cost of each candidate (sequential FMA chain over the dims, the oracle's
+// order), min-reduced per point as (cost bits << 32 | center): lowest cost, then
+// lowest index; the result does not depend on the order of the atomics.
+// 256 candidates per block; their two rows staged through LDS in 32-dim slabs.
+__global__ __launch_bounds__(256) void k_exact_cands(const float *__restrict__ X, uint32_t d,
+                                                     const float *__restrict__ C, RefineBufs R) {
+    __shared__ float xs[256][33];
+    __shared__ float cs[256][33];
+    __shared__ Cand cd[256];
+    const uint32_t cnt = min(R.cnt[0], R.cap);
+    const int tid = threadIdx.x;
+    const bool vec = (d & 3u) == 0;
+    for (uint32_t base = blockIdx.x * 256u; base < cnt; base += gridDim.x * 256u) {
+        __syncthreads();
+        cd[tid] = base + tid < cnt ? R.cand[base + tid] : Cand{0xFFFFFFFFu, 0u};
+        __syncthreads();
+        float s = 0.f;
+        for (uint32_t j0 = 0; j0 < d; j0 += 32) {
+            if (j0) __syncthreads();
+            if (vec) {
+                // 8 float4 per 32-dim row slab
+                for (int q = tid; q < 256 * 8; q += 256) {
+                    const int r = q >> 3, jj = (q & 7) * 4;
+                    const Cand e = cd[r];
+                    float4 xv = make_float4(0.f, 0.f, 0.f, 0.f), cv = xv;
+                    if (e.p != 0xFFFFFFFFu && j0 + jj < d) {
+                        xv = *(const float4 *)(X + (uint64_t)e.p * d + j0 + jj);
+                        cv = *(const float4 *)(C + (uint64_t)e.c * d + j0 + jj);
+                    }
+                    xs[r][jj] = xv.x; xs[r][jj + 1] = xv.y; xs[r][jj + 2] = xv.z; xs[r][jj + 3] = xv.w;
+                    cs[r][jj] = cv.x; cs[r][jj + 1] = cv.y; cs[r][jj + 2] = cv.z; cs[r][jj + 3] = cv.w;
+                }
+            } else {
+                for (int q = tid; q < 256 * 32; q += 256) {
+                    const int r = q >> 5, jj = q & 31;
+                    const Cand e = cd[r];
+                    const bool ok = e.p != 0xFFFFFFFFu && j0 + jj < d;
+                    xs[r][jj] = ok ? X[(uint64_t)e.p * d + j0 + jj] : 0.f;
+                    cs[r][jj] = ok ? C[(uint64_t)e.c * d + j0 + jj] : 0.f;
+                }
+            }
+            __syncthreads();
+            const int jn = (int)min<uint32_t>(32, d - j0);
+            for (int jj = 0; jj < jn; ++jj) { const float t = xs[tid][jj] - cs[tid][jj]; s = fmaf(t, t, s); }
         }
-        const bool take2 = (s2 < s1) || (s2 == s1 && e.c2 < e.c1);
-        asn[e.p] = take2 ? e.c2 : e.c1;
+        const Cand e = cd[tid];
+        if (e.p != 0xFFFFFFFFu) {
+            // costs are >= 0 (or NaN), so their bit patterns order like their values
+            const unsigned long long key = ((unsigned long long)__float_as_uint(s) << 32) | e.c;
+            atomicMin(&R.key[e.p], key);
+        }
     }
 }
 
-// Exact argmin over all centers for the listed points: 256 listed points per block,
-// their rows staged through LDS in 32-dim slabs, 16 centers per register tile.
+__global__ void k_apply_keys(RefineBufs R, uint32_t *__restrict__ asn) {
+    const uint32_t cnt = min(R.cnt[0], R.cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+        const uint32_t p = R.cand[i].p;
+        asn[p] = (uint32_t)(R.key[p] & 0xFFFFFFFFull);
+    }
+}
+
+// Exact argmin over all centers for the listed points (candidate overflow only):
+// 256 listed points per block, rows staged through LDS in 32-dim slabs, 16 centers
+// per register tile.
 __global__ __launch_bounds__(256) void k_rescore_full(const float *__restrict__ X, uint32_t d,
-                                                      const float *__restrict__ C, uint32_t k,
-                                                      const uint32_t *__restrict__ list,
-                                                      const uint32_t *__restrict__ nlist,
+                                                      const float *__restrict__ C, uint32_t k, RefineBufs R,
                                                       uint32_t *__restrict__ asn) {
+    const uint32_t *list = R.over;
+    const uint32_t *nlist = R.cnt + 2;
     __shared__ float xs[256][33];
     __shared__ float cs[16][32];
     __shared__ uint32_t pts[256];
@@ -406,7 +527,10 @@ __global__ __launch_bounds__(256) void k_rescore_full(const float *__restrict__ 
             for (int c = 0; c < 16; ++c)
                 if (c0 + c < k && (bi == 0xFFFFFFFFu || acc[c] < best)) { best = acc[c]; bi = c0 + c; }
         }
-        if (pts[tid] != 0xFFFFFFFFu) asn[pts[tid]] = bi;
+        if (pts[tid] != 0xFFFFFFFFu) {
+            asn[pts[tid]] = bi;
+            R.oflag[pts[tid]] = 0u;
+        }
     }
 }
 
@@ -580,22 +704,34 @@ __global__ __launch_bounds__(256) void k_point_costs(const float *__restrict__ X
     if (p < n) cost[p] = s;
 }
 
+__global__ void k_copy_counts(const uint32_t *__restrict__ cnt, uint32_t *__restrict__ out) {
+    out[0] = cnt[0] / 2;   // after the main pass every candidate belongs to a pair point
+    out[1] = cnt[1];
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 template <int KS, int NB>
-static void launch_bf16(Data &ds, Ctx &c, uint64_t kpad, uint64_t kreal, uint32_t *asn, hipStream_t st) {
+static void launch_bf16(Data &ds, Ctx &c, uint64_t kpad, uint64_t kreal, uint32_t *asn, const RefineBufs &R,
+                        hipStream_t st) {
     const uint32_t n = (uint32_t)ds.n;
     const uint32_t per = 4u * NB * 32u;
+    const dim3 grid((uint32_t)cdiv(n, per));
     hipEvent_t ev;
     c.tic(0, ev);
-    hipLaunchKernelGGL((k_estep_bf16<KS, NB>), dim3((uint32_t)cdiv(n, per)), dim3(256), 0, st,
-                       ds.Xb.as<const uint4>(), c.s[S_CB].as<const uint4>(), c.s[S_CADD].as<const float>(),
-                       ds.xnorm.as<const float>(), c.s[S_PARAMS].as<const EParams>(), n,
-                       (uint32_t)(kpad / 64), (uint32_t)kreal, asn, c.s[S_PAIRS].as<PairEntry>(),
-                       c.s[S_FLAGCNT].as<uint32_t>(), c.s[S_FLAGS].as<uint32_t>(),
-                       c.s[S_FLAGCNT].as<uint32_t>() + 1);
+    hipLaunchKernelGGL((k_estep_bf16<KS, NB, false>), grid, dim3(256), 0, st, ds.Xb.as<const uint4>(),
+                       c.s[S_CB].as<const uint4>(), c.s[S_CADD].as<const float>(), ds.xnorm.as<const float>(),
+                       c.s[S_PARAMS].as<const EParams>(), n, (uint32_t)(kpad / 64), (uint32_t)kreal, asn, R);
     c.toc(0, ev);
+    // statistics: {pair points, listed points} before the collect pass adds candidates
+    hipLaunchKernelGGL(k_copy_counts, dim3(1), dim3(1), 0, st, R.cnt, R.cnt + 4);
+    // collect pass over the listed points (blocks beyond the list exit at once); the
+    // center tiles are spread over grid.y so the few listed points fill the chip
+    const uint32_t ysplit = (uint32_t)std::min<uint64_t>(kpad / 64, 16);
+    hipLaunchKernelGGL((k_estep_bf16<KS, NB, true>), dim3(grid.x, ysplit), dim3(256), 0, st, ds.Xb.as<const uint4>(),
+                       c.s[S_CB].as<const uint4>(), c.s[S_CADD].as<const float>(), ds.xnorm.as<const float>(),
+                       c.s[S_PARAMS].as<const EParams>(), n, (uint32_t)(kpad / 64), (uint32_t)kreal, asn, R);
 }
 
 template <int DP>
@@ -647,29 +783,41 @@ void dense_sqrl2_assign(Data &ds, const float *C, uint64_t k, int precision, uin
         return;
     }
     const uint64_t kpad = cdiv(k, 64) * 64;
+    const uint64_t n = ds.n;
     c.s[S_CB].reserve(kpad * dpad * sizeof(uint16_t));
     c.s[S_CADD].reserve(kpad * sizeof(float));
     c.s[S_CNORM].reserve(kpad * sizeof(double));
     c.s[S_PARAMS].reserve(sizeof(EParams));
-    c.s[S_FLAGS].reserve(ds.n * sizeof(uint32_t));
-    c.s[S_PAIRS].reserve(ds.n * sizeof(PairEntry));
-    c.s[S_FLAGCNT].reserve(4 * sizeof(uint32_t));
+    // refinement buffers: candidates (8 per point), listed points, thresholds, keys
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(8 * n + 1024, 0xFFFFFFF0ull);
+    RefineBufs R;
+    R.cand = (Cand *)c.s[S_PAIRS].reserve((uint64_t)cap * sizeof(Cand));
+    R.cap = cap;
+    R.cnt = (uint32_t *)c.s[S_FLAGCNT].reserve(8 * sizeof(uint32_t));
+    R.full = (uint32_t *)c.s[S_FLAGS].reserve(n * sizeof(uint32_t));
+    R.thresh = (float *)c.s[S_THRESH].reserve(n * sizeof(float));
+    R.over = (uint32_t *)c.s[S_OVER].reserve(n * sizeof(uint32_t));
+    R.key = (unsigned long long *)c.s[S_KEYS].reserve(n * sizeof(unsigned long long));
+    {
+        const size_t before = c.s[S_OFLAG].bytes;
+        R.oflag = (uint32_t *)c.s[S_OFLAG].reserve(n * sizeof(uint32_t));
+        if (c.s[S_OFLAG].bytes != before) fill_u32(R.oflag, 0u, c.s[S_OFLAG].bytes / 4, st);
+    }
     hipLaunchKernelGGL(k_prep_centers_bf16, dim3((uint32_t)cdiv(kpad, 4)), dim3(256), 0, st, C, k, kpad, d, dpad,
                        ds.mu.as<const float>(), c.s[S_CB].as<uint16_t>(), c.s[S_CNORM].as<double>());
     hipLaunchKernelGGL(k_prep_params, dim3(1), dim3(256), 0, st, c.s[S_CNORM].as<const double>(), k, kpad,
                        ds.xmax, d, c.s[S_CADD].as<float>(), c.s[S_PARAMS].as<EParams>());
-    fill_u32(c.s[S_FLAGCNT].as<uint32_t>(), 0u, 4, st);
+    fill_u32(R.cnt, 0u, 4, st);
     switch (dpad) {
-    case 64: launch_bf16<4, 2>(ds, c, kpad, k, asn, st); break;
-    case 128: launch_bf16<8, 2>(ds, c, kpad, k, asn, st); break;
-    case 256: launch_bf16<16, 2>(ds, c, kpad, k, asn, st); break;
-    default: launch_bf16<32, 1>(ds, c, kpad, k, asn, st); break;
+    case 64: launch_bf16<4, 2>(ds, c, kpad, k, asn, R, st); break;
+    case 128: launch_bf16<8, 2>(ds, c, kpad, k, asn, R, st); break;
+    case 256: launch_bf16<16, 2>(ds, c, kpad, k, asn, R, st); break;
+    default: launch_bf16<32, 1>(ds, c, kpad, k, asn, R, st); break;
     }
-    hipLaunchKernelGGL(k_refine_pairs, dim3(1024), dim3(256), 0, st, ds.X, d, C, c.s[S_PAIRS].as<const PairEntry>(),
-                       c.s[S_FLAGCNT].as<const uint32_t>(), asn);
-    hipLaunchKernelGGL(k_rescore_full, dim3(512), dim3(256), 0, st, ds.X, d, C, (uint32_t)k,
-                       c.s[S_FLAGS].as<const uint32_t>(), c.s[S_FLAGCNT].as<const uint32_t>() + 1, asn);
-    HIP_CHECK(hipMemcpyAsync(c.refine_host(), c.s[S_FLAGCNT].p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(k_exact_cands, dim3(2048), dim3(256), 0, st, ds.X, d, C, R);
+    hipLaunchKernelGGL(k_apply_keys, dim3(1024), dim3(256), 0, st, R, asn);
+    hipLaunchKernelGGL(k_rescore_full, dim3(256), dim3(256), 0, st, ds.X, d, C, (uint32_t)k, R, asn);
+    HIP_CHECK(hipMemcpyAsync(c.refine_host(), R.cnt + 4, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
 }
 
 }  // namespace mcg

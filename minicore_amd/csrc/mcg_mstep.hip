@@ -173,69 +173,77 @@ __global__ void k_scatter(const uint32_t *__restrict__ asn, uint64_t n, uint32_t
     }
 }
 
-// Partial sums: one block per (cluster, chunk) item; a thread per dim.
-// part[item*d + j] = sum_m w*x (FP64, sequential); partw[item] = sum_m w;
+// Partial sums: one wave per (cluster, chunk) item.  Lane l owns dims
+// [4l, 4l+4) of each 256-dim pass (16-byte loads when d % 4 == 0, else one dim
+// per lane).  Members are walked in order, so each dim's FP64 sum is sequential:
+// part[item*d + j] = sum_m w*x (fma) or sum_m x;  partw[item] = sum_m w;
 // partq[item] = sum_m w * ||x - cold||^2 (FP64) when cold != nullptr.
-__global__ __launch_bounds__(256) void k_partial(const float *__restrict__ X, uint32_t d,
-                                                 const uint32_t *__restrict__ perm,
-                                                 const uint32_t *__restrict__ counts,
-                                                 const uint32_t *__restrict__ off,
-                                                 const uint32_t *__restrict__ itemoff, uint32_t k,
-                                                 const double *__restrict__ w, const float *__restrict__ cold,
-                                                 double *__restrict__ part, double *__restrict__ partw,
-                                                 double *__restrict__ partq) {
+template <bool VEC4>
+__global__ __launch_bounds__(64) void k_partial(const float *__restrict__ X, uint32_t d,
+                                                const uint32_t *__restrict__ perm,
+                                                const uint32_t *__restrict__ counts,
+                                                const uint32_t *__restrict__ off,
+                                                const uint32_t *__restrict__ itemoff, uint32_t k,
+                                                const double *__restrict__ w, const float *__restrict__ cold,
+                                                double *__restrict__ part, double *__restrict__ partw,
+                                                double *__restrict__ partq) {
     __shared__ uint32_t mem[MCHUNK];
     __shared__ double wl[MCHUNK];
-    __shared__ double red[256];
     const uint32_t item = blockIdx.x;
     const uint32_t total_items = itemoff[k];
     if (item >= total_items) return;
-    // find cluster c with itemoff[c] <= item < itemoff[c+1]
     uint32_t lo = 0, hi = k;
-    while (hi - lo > 1) { uint32_t mid = (lo + hi) >> 1; if (itemoff[mid] <= item) lo = mid; else hi = mid; }
+    while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (itemoff[mid] <= item) lo = mid; else hi = mid; }
     const uint32_t c = lo;
-    const uint32_t chunk = item - itemoff[c];
-    const uint32_t mb = off[c] + chunk * MCHUNK;
+    const uint32_t mb = off[c] + (item - itemoff[c]) * MCHUNK;
     const uint32_t me = min(off[c] + counts[c], mb + MCHUNK);
     const uint32_t nm = me - mb;
-    for (uint32_t q = threadIdx.x; q < nm; q += blockDim.x) {
-        uint32_t pnt = perm[mb + q];
+    const int lane = threadIdx.x;
+    for (uint32_t q = lane; q < nm; q += 64) {
+        const uint32_t pnt = perm[mb + q];
         mem[q] = pnt;
         wl[q] = w ? w[pnt] : 1.;
     }
     __syncthreads();
     double qsum = 0.;
-    for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) {
-        double s = 0.;
-        const double cj = cold ? (double)cold[(uint64_t)c * d + j] : 0.;
-        if (w) {
-            for (uint32_t q = 0; q < nm; ++q) {
-                const double x = (double)X[(uint64_t)mem[q] * d + j];
-                s = __fma_rn(wl[q], x, s);
-                if (cold) { double t = x - cj; qsum += wl[q] * (t * t); }
+    constexpr uint32_t DPL = VEC4 ? 4 : 1;   // dims per lane
+    for (uint32_t j0 = 0; j0 < d; j0 += 64 * DPL) {
+        const uint32_t j = j0 + lane * DPL;
+        if (j >= d) break;
+        double s[DPL], cj[DPL];
+#pragma unroll
+        for (uint32_t i = 0; i < DPL; ++i) {
+            s[i] = 0.;
+            cj[i] = cold ? (double)cold[(uint64_t)c * d + j + i] : 0.;
+        }
+#pragma unroll 4
+        for (uint32_t q = 0; q < nm; ++q) {
+            float xv[DPL];
+            if (VEC4) {
+                const float4 v = *(const float4 *)(X + (uint64_t)mem[q] * d + j);
+                xv[0] = v.x; xv[1] = v.y; xv[2] = v.z; xv[3] = v.w;
+            } else {
+                xv[0] = X[(uint64_t)mem[q] * d + j];
             }
-        } else {
-            for (uint32_t q = 0; q < nm; ++q) {
-                const double x = (double)X[(uint64_t)mem[q] * d + j];
-                s = __dadd_rn(s, x);
-                if (cold) { double t = x - cj; qsum += t * t; }
+            const double wq = wl[q];
+#pragma unroll
+            for (uint32_t i = 0; i < DPL; ++i) {
+                const double x = (double)xv[i];
+                s[i] = w ? __fma_rn(wq, x, s[i]) : __dadd_rn(s[i], x);
+                if (cold) { const double t = x - cj[i]; qsum += w ? wq * (t * t) : t * t; }
             }
         }
-        part[(uint64_t)item * d + j] = s;
+#pragma unroll
+        for (uint32_t i = 0; i < DPL; ++i) part[(uint64_t)item * d + j + i] = s[i];
     }
-    if (threadIdx.x == 0) {
+    if (cold) {
+        for (int o = 32; o > 0; o >>= 1) qsum += __shfl_xor(qsum, o);
+        if (lane == 0) partq[item] = qsum;
+    }
+    if (lane == 0) {
         double ws = 0.;
         for (uint32_t q = 0; q < nm; ++q) ws = __dadd_rn(ws, wl[q]);
         partw[item] = ws;
-    }
-    if (cold) {
-        red[threadIdx.x] = qsum;
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) partq[item] = red[0];
     }
 }
 
@@ -326,8 +334,12 @@ MStepOut dense_mstep(Data &ds, const uint32_t *asn, const double *w, uint64_t k,
     double *total = (double *)c.s[S_TOTAL].reserve(4 * sizeof(double));
     hipEvent_t ev;
     c.tic(1, ev);
-    hipLaunchKernelGGL(k_partial, dim3((uint32_t)max_items), dim3(256), 0, st, ds.X, d, so.perm, so.counts, so.off,
-                       so.itemoff, (uint32_t)k, w, want_cost ? Cold : nullptr, part, partw, partq);
+    if (d % 4 == 0)
+        hipLaunchKernelGGL(k_partial<true>, dim3((uint32_t)max_items), dim3(64), 0, st, ds.X, d, so.perm, so.counts,
+                           so.off, so.itemoff, (uint32_t)k, w, want_cost ? Cold : nullptr, part, partw, partq);
+    else
+        hipLaunchKernelGGL(k_partial<false>, dim3((uint32_t)max_items), dim3(64), 0, st, ds.X, d, so.perm, so.counts,
+                           so.off, so.itemoff, (uint32_t)k, w, want_cost ? Cold : nullptr, part, partw, partq);
     c.toc(1, ev);
     hipLaunchKernelGGL(k_finalize, dim3((uint32_t)k), dim3(256), 0, st, d, so.counts, so.itemoff, part, partw,
                        want_cost ? partq : nullptr, Cold, Cnew, ccost, ccost + k);

@@ -68,6 +68,23 @@ def test_bf16_near_ties_are_rescored(mcg, ctx, mco):
     assert not np.any(a == 5) and not np.any(a == 9) and not np.any(a == 10)
 
 
+def test_bf16_candidate_overflow(mcg, ctx, mco):
+    """40 identical centers give every point 40 exact-tie candidates, more than the
+    candidate buffer holds (8 per point): the overflow path re-scores over all centers."""
+    say("candidate overflow")
+    rng = np.random.default_rng(1)
+    d = 64
+    base = rng.uniform(-3, 3, d).astype(np.float32)
+    C = np.concatenate([np.tile(base, (40, 1)), rng.uniform(-3, 3, (5, d)).astype(np.float32)])
+    X = (base + 0.5 * rng.standard_normal((2000, d))).astype(np.float32)
+    a_ref, c_ref, _ = mco.assign_sqrl2_f32(X, C)
+    ds = mcg.Dataset.dense(X, ctx)
+    a, c = ds.assign(mcg.SQRL2, C, precision=mcg.PREC_BF16)
+    assert np.array_equal(a, a_ref) and np.array_equal(c, c_ref)
+    a2, _ = ds.assign(mcg.SQRL2, C, precision=mcg.PREC_BF16)   # flags were reset
+    assert np.array_equal(a2, a_ref)
+
+
 def test_far_from_origin(mcg, ctx, mco):
     """Data offset far from the origin: centering keeps the BF16 cross term usable."""
     say("offset data")

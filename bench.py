@@ -59,6 +59,63 @@ def time_form(ds, ctx, mcg, C0, asn, prec, steps, warmup):
                 p50_ms=1e3 * lat[len(lat) // 2], p99_ms=1e3 * lat[min(len(lat) - 1, int(0.99 * len(lat)))])
 
 
+def serving_mix(R, seed):
+    """Request mix: n ~ logU[1e3, 5e5]; d = 64 or 256 (dense Gaussian mixture with
+    k components, SQRL2) or d = 2000 sparse at 2% density (Poisson counts, rows
+    normalised, JSD with prior 1/d); k ~ logU[5, 500]."""
+    import numpy as np
+    from minicore_amd.datagen import gaussian_mixture_device, poisson_csr_fast
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(R):
+        n = int(round(np.exp(rng.uniform(np.log(1e3), np.log(5e5)))))
+        k = int(round(np.exp(rng.uniform(np.log(5), np.log(500)))))
+        kind = int(rng.integers(3))
+        if kind < 2:
+            d = (64, 256)[kind]
+            X, _, _ = gaussian_mixture_device(n, d, k, seed * 1000 + i)
+            reqs.append(dict(X=X, n=n, d=d, k=k, msr=2, prior=0.))
+        else:
+            d = 2000
+            data, idx, ptr = poisson_csr_fast(n, d, 0.02, seed * 1000 + i, normalize=True)
+            reqs.append(dict(csr=(data, idx, ptr, (n, d)), n=n, d=d, k=k, msr=4, prior=1. / d))
+    return reqs
+
+
+def _submit(srv, mcg, q, prec, seed):
+    kw = dict(k=q["k"], msr=q["msr"], prior=q["prior"], seed=seed, precision=prec, with_costs=False)
+    if "csr" in q:
+        return srv.submit(csr=q["csr"], **kw)
+    return srv.submit(q["X"], **kw)
+
+
+def serving(mcg, reqs, prec, rate, seed, workers):
+    """Poisson arrivals at `rate` requests/s on a `workers`-slot server; returns
+    per-request end-to-end latencies (submit -> done) in ms."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, size=len(reqs))
+    lat = []
+    with mcg.Server(0, nworkers=workers) as srv:
+        t0 = time.perf_counter()
+        tickets = []
+        at = 0.
+        for i, q in enumerate(reqs):
+            at += gaps[i]
+            dt = t0 + at - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            tickets.append(_submit(srv, mcg, q, prec, i + 1))
+        for t in tickets:
+            r = srv.wait(t, 600.)
+            if r["state"] != mcg.REQ_DONE:
+                raise RuntimeError(f"request failed: {r}")
+            lat.append(r["queue_ms"] + r["run_ms"])
+            srv.release(t)
+        wall = time.perf_counter() - t0
+    return np.array(lat), wall
+
+
 def cpu_baseline(X_host, C_host, n_full, nthreads):
     from oracle import mco
     ns = X_host.shape[0]
@@ -85,6 +142,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-workers", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path would shard rows "
@@ -128,6 +188,38 @@ def main():
         cpu = cpu_baseline(X[:ns].cpu().numpy(), C0.cpu().numpy(), n, nthreads)
         log(json.dumps(cpu))
 
+    serve = None
+    if not args.no_serve:
+        log(f"serving: generating {args.serve_requests} requests")
+        reqs = serving_mix(args.serve_requests, args.seed + 7)
+        torch.cuda.synchronize()
+        # offered load: each request alone on one worker gives its service time;
+        # arrivals at 0.7 / mean service time keep the device ~70% busy
+        svc = []
+        with mcg.Server(0, nworkers=1) as srv:
+            for i, q in enumerate(reqs):
+                t = _submit(srv, mcg, q, mcg.PREC_BF16, i + 1)
+                r = srv.wait(t, 600.)
+                if r["state"] != mcg.REQ_DONE:
+                    raise RuntimeError(f"request failed: {r}")
+                svc.append(r["run_ms"])
+                srv.release(t)
+        svc_ms = float(np.mean(svc))
+        rate = 0.7 / (svc_ms * 1e-3)
+        log(f"serving: mean service {svc_ms:.2f} ms -> {rate:.1f} req/s")
+        lat_bf, wall_bf = serving(mcg, reqs, mcg.PREC_BF16, rate, args.seed + 11, args.serve_workers)
+        lat_fp, wall_fp = serving(mcg, reqs, mcg.PREC_FP32, rate, args.seed + 11, args.serve_workers)
+        pct = lambda a, q: float(np.percentile(a, q))
+        serve = {"requests": len(reqs), "workers": args.serve_workers, "rate_per_s": rate,
+                 "mean_service_ms_alone": svc_ms, "offered_load": 0.7,
+                 "p50_ms": pct(lat_bf, 50), "p99_ms": pct(lat_bf, 99), "throughput_per_s": len(reqs) / wall_bf,
+                 "fp32_p50_ms": pct(lat_fp, 50), "fp32_p99_ms": pct(lat_fp, 99),
+                 "fp32_throughput_per_s": len(reqs) / wall_fp,
+                 "pipeline": "D2 seeding -> LBK sensitivities -> 1% coreset (>= k rows) -> 10 weighted Lloyd "
+                             "iterations on the coreset -> assignment of all rows",
+                 "mix": "n ~ logU[1e3,5e5], k ~ logU[5,500]; dense d=64/256 SQRL2 or sparse d=2000 2% JSD"}
+        log(json.dumps(serve))
+
     value = 1.0 / bf["sec_per_step"]
     out = {
         "metric": METRIC,
@@ -155,6 +247,7 @@ def main():
         "mstep": {"partial_kernel_ms": bf["mstep_partial_ms"],
                   "hbm_GBs": mstep_bytes / (bf["mstep_partial_ms"] * 1e-3) / 1e9},
         "cpu_baseline": cpu,
+        "serving": serve,
     }
     if fp is not None:
         fa = flops / (fp["estep_ms"] * 1e-3) / 1e12

@@ -162,6 +162,92 @@ int mcg_coreset_probs(mcg_ctx *ctx, int sens, uint64_t n, uint64_t ncenters,
 int mcg_coreset_sample(mcg_ctx *ctx, uint64_t n, const float *probs, const double *weights,
                        uint64_t m, uint64_t seed, uint64_t first_draw, uint64_t *indices,
                        float *out_weights);
+/* A new dataset on ds's context holding rows idx[0..m) of ds in that order
+ * (repeats allowed); idx u64, host or device.  Replaces the coreset row
+ * selection rows(mat, coreset.indices_) / set_center(row(smat, i), ...) of
+ * hmb_coreset_clustering, include/minicore/clustering/solve.h:748-762. */
+int mcg_data_rows(const mcg_data *ds, const uint64_t *idx, uint64_t m, mcg_data **out);
+
+/* ---- serving ------------------------------------------------------------ */
+/* A server runs clustering requests from many callers on one GPU: worker
+ * threads, each with its own context and HIP stream, take requests in
+ * arrival order as soon as the request's device footprint fits in the HBM
+ * budget, so requests of different callers run concurrently on the device.
+ * Each request runs the pipeline of one round of the reference's coreset
+ * clustering (solve.h:716-762): D^2 seeding on all rows (kmeans.h:49-215),
+ * sensitivities (coreset.h:386-578), a weighted coreset of
+ * m = min(n, max(ceil(coreset_frac * n), k)) rows (coreset.h:625-651), weighted
+ * Lloyd on the coreset from the seeded centers (solve.h:98-175), then one hard
+ * assignment of all rows to the resulting centers (solve.h:234-285).  A
+ * request's result does not depend on what else runs on the server.
+ * (The reference has no serving layer; these entry points have no counterpart
+ * there.) */
+typedef struct mcg_server mcg_server;
+
+/* request states */
+#define MCG_REQ_QUEUED    0
+#define MCG_REQ_RUNNING   1
+#define MCG_REQ_DONE      2
+#define MCG_REQ_FAILED    3
+#define MCG_REQ_CANCELLED 4
+
+typedef struct mcg_request {
+    /* rows: dense when indices == NULL (data: float32/float64, ld >= d, 0 = d),
+     * else CSR (data, indices, indptr with their dtypes as in mcg_data_csr).
+     * Host or device buffers; they must stay valid until the request ends. */
+    const void *data;
+    int data_dtype;
+    const void *indices;
+    int idx_dtype;
+    const void *indptr;
+    int ptr_dtype;
+    uint64_t n, d, nnz, ld;
+    const double *weights;   /* f64[n] or NULL */
+    int msr;
+    double prior;
+    uint64_t k;
+    uint64_t seed;           /* seeding and coreset draws */
+    int sens;                /* MCG_SENS_* */
+    double coreset_frac;     /* in (0, 1] */
+    double eps;              /* Lloyd stop rule on the coreset (solve.h:165-170) */
+    uint64_t maxiter;        /* Lloyd maxiter on the coreset (maxiter + 1 passes at most) */
+    int precision;           /* MCG_PREC_* for dense SQRL2 */
+    int fail_at_stage;       /* 0; 1..4 makes the request fail at that stage (tests of error paths) */
+    /* outputs: caller-owned host or device buffers, written when the request is done */
+    float *centers;          /* k x d */
+    uint32_t *asn;           /* n */
+    float *costs;            /* n, nullable */
+} mcg_request;
+
+typedef struct mcg_result {
+    int state;               /* MCG_REQ_* */
+    int status;              /* MCG_OK or the error code of a failed request */
+    double cost;             /* sum_i w_i cost_i of the final assignment (FP64) */
+    double coreset_initcost, coreset_finalcost;
+    uint64_t coreset_m;      /* rows in the coreset */
+    uint64_t numiter;        /* Lloyd iterations on the coreset */
+    double queue_ms;         /* submit -> start */
+    double run_ms;           /* start -> end */
+    char error[256];         /* message of a failed request */
+} mcg_result;
+
+/* nworkers >= 1 concurrent requests; hbm_budget bytes of device memory for the
+ * requests' working sets (0 = 80% of the free device memory at creation). */
+int mcg_server_create(int device, int nworkers, uint64_t hbm_budget, mcg_server **out);
+/* Cancels queued requests, waits for running ones, joins the workers. */
+int mcg_server_destroy(mcg_server *s);
+/* Queues a copy of *req; *ticket identifies it.  MCG_ENOMEM if the request's
+ * footprint exceeds the whole budget; MCG_EINVAL for malformed requests. */
+int mcg_submit(mcg_server *s, const mcg_request *req, uint64_t *ticket);
+/* Non-blocking: the request's state and, once it ended, its result. */
+int mcg_poll(mcg_server *s, uint64_t ticket, mcg_result *res);
+/* Blocks until the request ends or timeout_s passes (< 0: no limit); then as mcg_poll. */
+int mcg_wait(mcg_server *s, uint64_t ticket, double timeout_s, mcg_result *res);
+/* A queued request is dropped; a running one stops at its next stage boundary.
+ * The outputs of a cancelled request are unspecified. */
+int mcg_cancel(mcg_server *s, uint64_t ticket);
+/* Forgets an ended request (its ticket becomes invalid). */
+int mcg_release(mcg_server *s, uint64_t ticket);
 
 #ifdef __cplusplus
 }

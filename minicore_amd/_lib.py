@@ -21,8 +21,38 @@ EXPORTS = [
     "mcg_ctx_stream", "mcg_data_dense", "mcg_data_csr", "mcg_data_free", "mcg_data_shape",
     "mcg_cost_matrix", "mcg_assign_hard", "mcg_centroids_hard", "mcg_lloyd", "mcg_soft_em",
     "mcg_kmeanspp", "mcg_coreset_probs", "mcg_coreset_sample", "mcg_lloyd_step", "mcg_timing_enable",
-    "mcg_timing_reset", "mcg_timing_read", "mcg_refine_counts",
+    "mcg_timing_reset", "mcg_timing_read", "mcg_refine_counts", "mcg_data_rows", "mcg_server_create",
+    "mcg_server_destroy", "mcg_submit", "mcg_poll", "mcg_wait", "mcg_cancel", "mcg_release",
 ]
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = range(5)
+
+
+class Request(C.Structure):
+    """struct mcg_request (include/mcg.h)."""
+    _fields_ = [
+        ("data", C.c_void_p), ("data_dtype", C.c_int),
+        ("indices", C.c_void_p), ("idx_dtype", C.c_int),
+        ("indptr", C.c_void_p), ("ptr_dtype", C.c_int),
+        ("n", C.c_uint64), ("d", C.c_uint64), ("nnz", C.c_uint64), ("ld", C.c_uint64),
+        ("weights", C.c_void_p),
+        ("msr", C.c_int), ("prior", C.c_double),
+        ("k", C.c_uint64), ("seed", C.c_uint64),
+        ("sens", C.c_int), ("coreset_frac", C.c_double), ("eps", C.c_double), ("maxiter", C.c_uint64),
+        ("precision", C.c_int), ("fail_at_stage", C.c_int),
+        ("centers", C.c_void_p), ("asn", C.c_void_p), ("costs", C.c_void_p),
+    ]
+
+
+class Result(C.Structure):
+    """struct mcg_result (include/mcg.h)."""
+    _fields_ = [
+        ("state", C.c_int), ("status", C.c_int),
+        ("cost", C.c_double), ("coreset_initcost", C.c_double), ("coreset_finalcost", C.c_double),
+        ("coreset_m", C.c_uint64), ("numiter", C.c_uint64),
+        ("queue_ms", C.c_double), ("run_ms", C.c_double),
+        ("error", C.c_char * 256),
+    ]
 
 _lib = None
 
@@ -77,6 +107,14 @@ def load():
         "mcg_timing_reset": (I, [P]),
         "mcg_timing_read": (I, [P, I, P, P]),
         "mcg_refine_counts": (I, [P, P, P]),
+        "mcg_data_rows": (I, [P, P, U64, P]),
+        "mcg_server_create": (I, [I, I, U64, P]),
+        "mcg_server_destroy": (I, [P]),
+        "mcg_submit": (I, [P, C.POINTER(Request), P]),
+        "mcg_poll": (I, [P, U64, C.POINTER(Result)]),
+        "mcg_wait": (I, [P, U64, D, C.POINTER(Result)]),
+        "mcg_cancel": (I, [P, U64]),
+        "mcg_release": (I, [P, U64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -269,6 +307,13 @@ class Dataset:
                                  ptr(resp), C.byref(ic), C.byref(fc), C.byref(it)))
         return dict(centers=Cc, costs=costs, asn=resp, initcost=ic.value, finalcost=fc.value, numiter=it.value)
 
+    def rows(self, idx):
+        """A new dataset of rows idx (in order) of this one."""
+        idx = np.ascontiguousarray(idx, dtype=np.uint64)
+        h = C.c_void_p()
+        check(load().mcg_data_rows(self.h, ptr(idx), len(idx), C.byref(h)))
+        return Dataset(self.ctx, h)
+
     def kmeanspp(self, msr, k, prior=0., seed=0, weights=None):
         w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
         kk = min(k, self.n)
@@ -297,3 +342,117 @@ def coreset_sample(probs, m, seed, weights=None, first_draw=0, ctx=None):
     wt = np.empty(max(m, 1), dtype=np.float32)
     check(load().mcg_coreset_sample(ctx.h, len(probs), ptr(probs), ptr(w), m, seed, first_draw, ptr(idx), ptr(wt)))
     return wt[:m], idx[:m]
+
+
+class Server:
+    """Concurrent clustering requests on one GPU (mcg_server_*, include/mcg.h).
+
+    Each request: D^2 seeding, coreset sensitivities, a weighted coreset, weighted
+    Lloyd on the coreset, then a hard assignment of all rows.  submit() keeps the
+    request's input and output arrays alive until the request is released."""
+
+    def __init__(self, device=0, nworkers=4, hbm_budget=0):
+        h = C.c_void_p()
+        check(load().mcg_server_create(device, nworkers, hbm_budget, C.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def close(self):
+        if self.h:
+            check(load().mcg_server_destroy(self.h))
+            self.h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, X=None, k=8, *, csr=None, msr=SQRL2, prior=0., weights=None, seed=0, sens=SENS_LBK,
+               coreset_frac=0.01, eps=0., maxiter=9, precision=PREC_BF16, fail_at_stage=0, with_costs=True):
+        """X: dense float32/float64 (n, d) numpy array or torch tensor (device tensors are
+        borrowed); or csr=(data, indices, indptr, (n, d)).
+        Returns a ticket.  Outputs land in self.outputs(ticket) when the request is done."""
+        q = Request()
+        keep = []
+        if csr is not None:
+            data, indices, indptr, shape = csr
+            data = np.ascontiguousarray(data)
+            indices = np.ascontiguousarray(indices)
+            indptr = np.ascontiguousarray(indptr)
+            keep += [data, indices, indptr]
+            n, d = int(shape[0]), int(shape[1])
+            q.data, q.data_dtype = ptr(data).value, _dtype_code(data.dtype)
+            q.indices, q.idx_dtype = ptr(indices).value, _dtype_code(indices.dtype)
+            q.indptr, q.ptr_dtype = ptr(indptr).value, _dtype_code(indptr.dtype)
+            q.nnz = len(data)
+        elif hasattr(X, "data_ptr") and not isinstance(X, np.ndarray):
+            # torch tensor: rows stay where they are (device rows are borrowed)
+            if not X.is_contiguous():
+                raise ValueError("tensor rows must be contiguous")
+            keep.append(X)
+            n, d = X.shape
+            q.data = X.data_ptr()
+            q.data_dtype = MCG_F32 if str(X.dtype) == "torch.float32" else MCG_F64
+            q.ld = d
+        else:
+            X = np.ascontiguousarray(X)
+            if X.dtype not in (np.float32, np.float64):
+                X = X.astype(np.float32)
+            keep.append(X)
+            n, d = X.shape
+            q.data, q.data_dtype = ptr(X).value, _dtype_code(X.dtype)
+            q.ld = d
+        q.n, q.d = n, d
+        if weights is not None:
+            w = np.ascontiguousarray(weights, dtype=np.float64)
+            keep.append(w)
+            q.weights = ptr(w).value
+        kk = min(k, n)
+        centers = np.empty((kk, d), dtype=np.float32)
+        asn = np.empty(n, dtype=np.uint32)
+        costs = np.empty(n, dtype=np.float32) if with_costs else None
+        keep += [centers, asn, costs]
+        q.msr, q.prior, q.k, q.seed, q.sens = msr, prior, k, seed, sens
+        q.coreset_frac, q.eps, q.maxiter, q.precision, q.fail_at_stage = coreset_frac, eps, maxiter, precision, \
+            fail_at_stage
+        q.centers, q.asn = ptr(centers).value, ptr(asn).value
+        q.costs = ptr(costs).value if costs is not None else None
+        t = C.c_uint64()
+        check(load().mcg_submit(self.h, C.byref(q), C.byref(t)))
+        self._keep[t.value] = dict(keep=keep, centers=centers, asn=asn, costs=costs)
+        return t.value
+
+    @staticmethod
+    def _res(r):
+        return dict(state=r.state, status=r.status, cost=r.cost, coreset_initcost=r.coreset_initcost,
+                    coreset_finalcost=r.coreset_finalcost, coreset_m=r.coreset_m, numiter=r.numiter,
+                    queue_ms=r.queue_ms, run_ms=r.run_ms, error=r.error.decode(errors="replace"))
+
+    def poll(self, ticket):
+        r = Result()
+        check(load().mcg_poll(self.h, ticket, C.byref(r)))
+        return self._res(r)
+
+    def wait(self, ticket, timeout=-1.0):
+        r = Result()
+        check(load().mcg_wait(self.h, ticket, timeout, C.byref(r)))
+        return self._res(r)
+
+    def cancel(self, ticket):
+        check(load().mcg_cancel(self.h, ticket))
+
+    def outputs(self, ticket):
+        o = self._keep[ticket]
+        return dict(centers=o["centers"], asn=o["asn"], costs=o["costs"])
+
+    def release(self, ticket):
+        check(load().mcg_release(self.h, ticket))
+        self._keep.pop(ticket, None)

@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libmcg.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MCG_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["mcg_api.hip", "mcg_dense.hip", "mcg_mstep.hip", "mcg_generic.hip", "mcg_sample.hip",
+SOURCES = ["mcg_api.hip", "mcg_dense.hip", "mcg_mstep.hip", "mcg_generic.hip", "mcg_sample.hip", "mcg_serve.hip",
            "mcg_soft.hip"]
 
 HIP_FLAGS = [

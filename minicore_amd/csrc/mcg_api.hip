@@ -445,11 +445,17 @@ int mcg_data_csr(mcg_ctx *ctx, const void *data, int data_dtype, const void *ind
         for (uint64_t i = 0; i < n; ++i) REQUIRE(P[i] <= P[i + 1], "indptr must be non-decreasing");
         std::vector<float> V(nnz);
         std::vector<uint32_t> J(nnz);
-        for (uint64_t p = 0; p < nnz; ++p) {
-            V[p] = (float)read_num(hd, data_dtype, p);
-            uint64_t j = read_idx(hi, idx_dtype, p);
-            REQUIRE(j < d, "column index out of range");
-            J[p] = (uint32_t)j;
+        if (data_dtype == MCG_F32 && nnz) std::memcpy(V.data(), hd, nnz * 4);
+        else for (uint64_t p = 0; p < nnz; ++p) V[p] = (float)read_num(hd, data_dtype, p);
+        if (idx_dtype == MCG_U32 && nnz) {
+            std::memcpy(J.data(), hi, nnz * 4);
+            for (uint64_t p = 0; p < nnz; ++p) REQUIRE(J[p] < d, "column index out of range");
+        } else {
+            for (uint64_t p = 0; p < nnz; ++p) {
+                uint64_t j = read_idx(hi, idx_dtype, p);
+                REQUIRE(j < d, "column index out of range");
+                J[p] = (uint32_t)j;
+            }
         }
         for (uint64_t i = 0; i < n; ++i)
             for (uint64_t p = P[i] + 1; p < P[i + 1]; ++p)

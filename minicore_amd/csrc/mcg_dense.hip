@@ -8,10 +8,11 @@
 //  * BF16: ||x-c||^2 = ||x-mu||^2 + ||c-mu||^2 - 2 (x-mu).(c-mu) with the cross term on
 //    v_mfma_f32_32x32x16_bf16 (FP32 accumulate).  Points are the MFMA B operand and stay
 //    in registers for the whole pass over the centers; centers stream through LDS in
-//    tiles of 64.  Each lane keeps a running best/second of its column with the center's
-//    tile-local position packed into the low 6 mantissa bits (3 VALU ops per value).
-//    Points whose best/second gap is not provably larger than the BF16 error bound are
-//    re-scored exactly (k_refine_pairs, k_rescore_full).
+//    tiles of 64.  Each lane keeps a running best/second/third of its column with the
+//    center's position in its 32-center sub-tile packed into the low 5 mantissa bits
+//    (4 VALU ops per value: v_and_or, two v_med3_u32, v_min_u32).  Points whose
+//    best/second gap is not provably larger than the BF16 error bound get their
+//    candidate centers re-scored exactly (k_collect_bf16, k_exact_cands, k_rescore_full).
 //  * FP32: the same GEMM form on v_mfma_f32_32x32x2f32, plain compare/select.
 //  * EXACT: sum of squared differences per pair as a sequential FMA chain, FP32.
 #include "mcg_internal.h"
@@ -71,17 +72,19 @@ __global__ void k_make_xb(const float *__restrict__ X, uint64_t n, uint32_t d, u
     if (lane == 0) xnorm[row] = (float)(sqrt(s) * (1. + 1e-6));
 }
 
-__global__ void k_max_f32(const float *__restrict__ v, uint64_t n, float *__restrict__ out) {
+// max of non-negative floats (their bit patterns order like their values)
+__global__ void k_max_f32(const float *__restrict__ v, uint64_t n, unsigned int *__restrict__ out_bits) {
     __shared__ float sm[256];
     float m = 0.f;
-    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) m = fmaxf(m, v[i]);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        m = fmaxf(m, v[i]);
     sm[threadIdx.x] = m;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) sm[threadIdx.x] = fmaxf(sm[threadIdx.x], sm[threadIdx.x + s]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = sm[0];
+    if (threadIdx.x == 0) atomicMax(out_bits, __float_as_uint(sm[0]));
 }
 
 static uint32_t pad_dims(uint64_t d) {
@@ -99,7 +102,7 @@ void prepare_bf16(Data &ds) {
     const uint32_t d = (uint32_t)ds.d;
     REQUIRE(d <= 512, "BF16 form supports d <= 512");
     ds.dpad = pad_dims(d);
-    const uint32_t rows_per = 4096;
+    const uint32_t rows_per = 512;
     const uint32_t nparts = (uint32_t)cdiv(n, rows_per);
     DBuf part;
     part.reserve((size_t)nparts * d * sizeof(double));
@@ -114,7 +117,9 @@ void prepare_bf16(Data &ds) {
                        ds.mu.as<float>(), ds.Xb.as<uint16_t>(), ds.xnorm.as<float>());
     DBuf mx;
     mx.reserve(sizeof(float));
-    hipLaunchKernelGGL(k_max_f32, dim3(1), dim3(256), 0, st, ds.xnorm.as<float>(), n, mx.as<float>());
+    fill_u32(mx.as<uint32_t>(), 0u, 1, st);
+    hipLaunchKernelGGL(k_max_f32, dim3((uint32_t)std::min<uint64_t>(cdiv(n, 256), 2048)), dim3(256), 0, st,
+                       ds.xnorm.as<const float>(), n, mx.as<unsigned int>());
     HIP_CHECK(hipMemcpyAsync(&ds.xmax, mx.p, sizeof(float), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
     ds.bf16_ready = true;
@@ -183,14 +188,14 @@ __global__ void k_prep_params(const double *__restrict__ cn, uint64_t k, uint64_
 // the 16-byte chunk index XOR-swizzled by the row so that the 16 rows read by one
 // ds_read_b128 phase hit distinct banks.
 //
-// Main pass (COLLECT = false): each lane keeps the three smallest packed values of
+// Main pass: each lane keeps the three smallest packed values of
 // its point column and the tiles of the best two; the halves are merged at the end
 // and every point is classified:
 //   certain : second - best > tau                 -> asn = best
 //   pair    : third - best > tau >= second - best -> candidates {best, second}
 //   full    : otherwise                           -> listed with threshold best + tau
-// Collect pass (COLLECT = true) over the listed points: every center whose value is
-// within the threshold becomes a candidate.  Candidates are then scored exactly.
+// The collect pass (k_collect_bf16) then gathers, for each listed point, every center
+// whose value is within the threshold; all candidates are scored exactly.
 // ---------------------------------------------------------------------------
 struct Cand {
     uint32_t p, c;
@@ -210,31 +215,62 @@ struct RefineBufs {
 template <int CH>
 __device__ __forceinline__ int swz(int row) { return row & ((CH < 16 ? CH : 16) - 1); }
 
-__device__ __forceinline__ uint32_t lane_prefix(uint64_t m) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
-
-// Wave-aggregated append of `cnt_per_lane` (0, 1 or 2) candidates per lane: one
-// atomic per wave.  Slots past the capacity send the point to the overflow list.
-__device__ __forceinline__ void push_cands(const RefineBufs &R, bool want, uint32_t per, uint32_t p, uint32_t c1,
-                                           uint32_t c2) {
-    const uint64_t m = __ballot(want);
-    if (!m) return;
-    const int leader = __builtin_ctzll(m);
-    uint32_t base = 0;
-    if ((int)__lane_id() == leader) base = atomicAdd(&R.cnt[0], per * (uint32_t)__builtin_popcountll(m));
-    base = __shfl(base, leader);
-    if (!want) return;
-    const uint32_t i = base + per * lane_prefix(m);
-    if (i + per <= R.cap) {
-        R.cand[i] = Cand{p, c1};
-        if (per == 2) R.cand[i + 1] = Cand{p, c2};
-    } else if (atomicExch(&R.oflag[p], 1u) == 0u) {
-        R.over[atomicAdd(&R.cnt[2], 1u)] = p;
+// Block-wide appends.  Every thread of a 256-thread block calls these together
+// with its own count; each returns the thread's first slot.  One global atomic per
+// list per block (a wave-level scan, then per-wave offsets through LDS), so the
+// appends of the whole grid do not serialize on one counter.  `sm`: 8 u32 of LDS.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
     }
+    return x;
 }
 
-template <int KS, int NB, bool COLLECT>
+// two lists at once: cnt2 = count A (low 16 bits) | count B << 16, block totals < 2^16
+__device__ __forceinline__ uint2 block_append2(uint32_t cnt2, uint32_t *ctr_a, uint32_t *ctr_b, uint32_t *sm) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t x = wave_incl_scan(cnt2);
+    if (lane == 63) sm[wave] = x;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t run = 0;
+        for (int w = 0; w < 4; ++w) { const uint32_t t = sm[w]; sm[w] = run; run += t; }
+        const uint32_t ta = run & 0xFFFFu, tb = run >> 16;
+        sm[4] = ta ? atomicAdd(ctr_a, ta) : 0u;
+        sm[5] = tb ? atomicAdd(ctr_b, tb) : 0u;
+    }
+    __syncthreads();
+    const uint32_t excl = sm[wave] + x - cnt2;
+    const uint2 r = make_uint2(sm[4] + (excl & 0xFFFFu), sm[5] + (excl >> 16));
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ uint32_t block_append1(uint32_t cnt, uint32_t *ctr, uint32_t *sm) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t x = wave_incl_scan(cnt);
+    if (lane == 63) sm[wave] = x;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t run = 0;
+        for (int w = 0; w < 4; ++w) { const uint32_t t = sm[w]; sm[w] = run; run += t; }
+        sm[4] = run ? atomicAdd(ctr, run) : 0u;
+    }
+    __syncthreads();
+    const uint32_t r = sm[4] + sm[wave] + x - cnt;
+    __syncthreads();
+    return r;
+}
+
+// a point whose candidates did not fit goes to the overflow list (once)
+__device__ __forceinline__ void cand_overflow(const RefineBufs &R, uint32_t p) {
+    if (atomicExch(&R.oflag[p], 1u) == 0u) R.over[atomicAdd(&R.cnt[2], 1u)] = p;
+}
+
+template <int KS, int NB>
 __global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep_bf16(
     const uint4 *__restrict__ Xb, const uint4 *__restrict__ Cb, const float *__restrict__ cadd,
     const float *__restrict__ xnorm, const EParams *__restrict__ prm, uint32_t n, uint32_t ntiles, uint32_t kreal,
@@ -242,138 +278,140 @@ __global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep_bf16(
     constexpr int CH = KS * 2;                  // 16-byte chunks per row
     constexpr int TILE = 64 * CH * 16;          // bytes per tile
     constexpr int INSTR = CH / 4;               // global_load_lds per wave per tile
-    __shared__ __attribute__((aligned(16))) char lds[2][TILE];
-    __shared__ float ladd[2][64];
+    // Two distinct LDS objects (not one indexed array): the tile loop alternates
+    // between them statically, so the compiler can tell that the copy into one does
+    // not alias the reads of the other and need not wait for it.
+    __shared__ __attribute__((aligned(16))) char lds0[TILE];
+    __shared__ __attribute__((aligned(16))) char lds1[TILE];
+    __shared__ float ladd0[64], ladd1[64];
+    __shared__ uint32_t sm_app[8];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int h = lane >> 5, r32 = lane & 31;
-    const uint32_t blk0 = blockIdx.x * (4u * NB * 32u);
-    uint32_t count = n;
-    if (COLLECT) {
-        count = R.cnt[1];
-        if (blk0 >= count) return;   // uniform over the block
-    }
-    const uint32_t pbase = blk0 + wave * (NB * 32u);
+    const uint32_t pbase = blockIdx.x * (4u * NB * 32u) + wave * (NB * 32u);
 
     bf16x8 b[NB][KS];
-    uint32_t pt[NB];
-    float thr_in[NB];
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
-        uint32_t q = pbase + nb * 32 + r32;
-        q = q < count ? q : count - 1;
-        const uint32_t p = COLLECT ? R.full[q] : q;
-        pt[nb] = p;
-        thr_in[nb] = COLLECT ? R.thresh[q] : 0.f;
+        uint32_t p = pbase + nb * 32 + r32;
+        p = p < n ? p : n - 1;
         const uint4 *row = Xb + (size_t)p * CH + h * KS;
 #pragma unroll
         for (int s = 0; s < KS; ++s) b[nb][s] = __builtin_bit_cast(bf16x8, row[s]);
     }
 
-    // direct global->LDS copy of tile t into buffer buf (swizzled)
-    auto issue_tile = [&](uint32_t t, int buf) {
+    // Direct global->LDS copy of tile t into dst.  Instruction j of this wave fills
+    // physical chunk slots P = (4j + wave) * 64 + lane: row P / CH, swizzled chunk
+    // P % CH, whose source chunk is (P % CH) ^ swz(row).  The swizzle repeats every PER
+    // instructions, so the source is a uniform base + j * 4 KiB + voff[j % PER].
+    constexpr int PER = CH == 64 ? 4 : (CH == 32 ? 2 : 1);
+    uint32_t voff[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const uint32_t P = (uint32_t)((i * 4 + wave) * 64 + lane);
+        const uint32_t r = P / CH, q = P % CH;
+        voff[i] = (r * CH + (q ^ swz<CH>((int)r))) * 16u - (uint32_t)i * (4u * 64u * 16u);
+    }
+    auto issue_tile = [&](uint32_t t, char *dst) {
+        const char *tbase = (const char *)Cb + (size_t)t * TILE;
 #pragma unroll
         for (int j = 0; j < INSTR; ++j) {
-            const int P = (j * 4 + wave) * 64 + lane;   // physical chunk slot in the tile
-            const int r = P / CH, q = P % CH;
-            const uint4 *src = Cb + (size_t)(t * 64 + r) * CH + (q ^ swz<CH>(r));
+            const char *src = tbase + j * (4 * 64 * 16) + voff[j % PER];
             __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
-                                             (__attribute__((address_space(3))) void *)(lds[buf] + (j * 4 + wave) * 64 * 16),
+                                             (__attribute__((address_space(3))) void *)(dst + (j * 4 + wave) * 64 * 16),
                                              16, 0, 0);
         }
     };
 
-    float best[NB], sec[NB], thr[NB];
+    // Running best / second / third per lane as packed values: positive floats whose
+    // low 5 mantissa bits hold the position within a 32-center sub-tile, compared as
+    // unsigned integers (bit patterns of positive floats order like their values).
+    // btile / stile: sub-tile (32 centers) of the best and second.
+    uint32_t best[NB], sec[NB], thr[NB];
     uint32_t btile[NB], stile[NB];
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
-        best[nb] = FLT_MAX; sec[nb] = FLT_MAX; thr[nb] = FLT_MAX; btile[nb] = 0; stile[nb] = 0;
+        best[nb] = 0x7F7FFFFFu; sec[nb] = 0x7F7FFFFFu; thr[nb] = 0x7F7FFFFFu; btile[nb] = 0; stile[nb] = 0;
     }
+    // Lane-constant part of the A-fragment address: the chunk of step s is
+    // (h KS + s) ^ sw = ((h KS) ^ sw) ^ s (s < KS, KS a power of two), so the byte
+    // address is base ^ (16 s); row r32 + 32 has the same swizzle as r32.
+    const uint32_t abase0 = (uint32_t)(r32 * (CH * 16) + ((h * KS) ^ swz<CH>(r32)) * 16);
 
-    // the collect pass spreads the tiles over gridDim.y blocks (no state crosses tiles)
-    const uint32_t t0 = COLLECT ? blockIdx.y : 0u, tstep = COLLECT ? gridDim.y : 1u;
-    if (t0 >= ntiles) return;
-    issue_tile(t0, 0);
-    if (tid < 64) ladd[0][tid] = cadd[t0 * 64 + tid];
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    int buf = 0;
-    for (uint32_t t = t0; t < ntiles; t += tstep, buf ^= 1) {
+    auto tile_step = [&](uint32_t t, const char *cur, const float *cadd_cur, char *nxt, float *cadd_nxt) {
         float addn = 0.f;
-        const bool more = t + tstep < ntiles;
+        const bool more = t + 1 < ntiles;
         if (more) {
-            issue_tile(t + tstep, buf ^ 1);
-            if (tid < 64) addn = cadd[(t + tstep) * 64 + tid];
+            issue_tile(t + 1, nxt);
+            if (tid < 64) addn = cadd[(t + 1) * 64 + tid];
         }
-        float bstart[NB], sstart[NB];
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) { bstart[nb] = best[nb]; sstart[nb] = sec[nb]; }
-        const char *tb = lds[buf];
 #pragma unroll 1
-        for (int mb = 0; mb < 2; ++mb) {
+        for (uint32_t mb = 0; mb < 2; ++mb) {
             floatx16 acc[NB];
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
-                const float v = ladd[buf][mb * 32 + 8 * (e >> 2) + 4 * h + (e & 3)];
+                const float v = cadd_cur[mb * 32 + 8 * (e >> 2) + 4 * h + (e & 3)];
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb) acc[nb][e] = v;
             }
-            const int row = mb * 32 + r32;
-            const char *arow = tb + row * (CH * 16);
-            const int sw = swz<CH>(row);
+            // rows 32..63 of the tile: row + 32 keeps the swizzle (32 & 15 == 0)
+            const char *arow = cur + abase0 + mb * (32u * CH * 16u);
+            bf16x8 a = *(const bf16x8 *)arow;
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
-                const bf16x8 a = *(const bf16x8 *)(arow + (((h * KS + s) ^ sw) * 16));
+                bf16x8 an = a;
+                if (s + 1 < KS) an = *(const bf16x8 *)(cur + ((abase0 ^ (uint32_t)((s + 1) * 16)) + mb * (32u * CH * 16u)));
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb)
                     acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[nb][s], acc[nb], 0, 0, 0);
+                a = an;
             }
+            const uint32_t u = 2u * t + mb;
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) {
+                const uint32_t b0 = best[nb], s0 = sec[nb];
 #pragma unroll
                 for (int e = 0; e < 16; ++e) {
-                    const uint32_t pos = mb * 32 + 8 * (e >> 2) + (e & 3);
-                    if (COLLECT) {
-                        const float v = __uint_as_float(__float_as_uint(acc[nb][e]) & 0xFFFFFFC0u);
-                        const uint32_t c = t * 64u + pos + 4u * h;
-                        const bool want = v <= thr_in[nb] && c < kreal && pbase + nb * 32 + r32 < count;
-                        push_cands(R, want, 1u, pt[nb], c, 0u);
-                    } else {
-                        const float v = __uint_as_float((__float_as_uint(acc[nb][e]) & 0xFFFFFFC0u) | pos);
-                        thr[nb] = __builtin_amdgcn_fmed3f(sec[nb], v, thr[nb]);
-                        sec[nb] = __builtin_amdgcn_fmed3f(best[nb], v, sec[nb]);
-                        best[nb] = fminf(best[nb], v);
-                    }
+                    const uint32_t v = (__float_as_uint(acc[nb][e]) & 0xFFFFFFE0u) | (uint32_t)(8 * (e >> 2) + (e & 3));
+                    // third = med3(second, v, third); second = med3(best, v, second); best = min
+                    thr[nb] = max(min(sec[nb], v), min(max(sec[nb], v), thr[nb]));
+                    sec[nb] = max(min(best[nb], v), min(max(best[nb], v), sec[nb]));
+                    best[nb] = min(best[nb], v);
                 }
+                // the new best came from this sub-tile; the second is either the old best
+                // (its sub-tile moves over) or from this sub-tile
+                const bool nbest = best[nb] != b0, nsec = sec[nb] != s0;
+                const uint32_t st_new = (nbest && sec[nb] == b0) ? btile[nb] : u;
+                stile[nb] = (nbest || nsec) ? st_new : stile[nb];
+                btile[nb] = nbest ? u : btile[nb];
             }
         }
-        if (!COLLECT) {
-#pragma unroll
-            for (int nb = 0; nb < NB; ++nb) {
-                const uint32_t bb = __float_as_uint(best[nb]), b0 = __float_as_uint(bstart[nb]);
-                const uint32_t ss = __float_as_uint(sec[nb]), s0 = __float_as_uint(sstart[nb]);
-                if (bb != b0) {
-                    stile[nb] = (ss == b0) ? btile[nb] : t;
-                    btile[nb] = t;
-                } else if (ss != s0) {
-                    stile[nb] = t;
-                }
-            }
-        }
-        if (more && tid < 64) ladd[buf ^ 1][tid] = addn;
+        if (more && tid < 64) cadd_nxt[tid] = addn;
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-    }
-    if (COLLECT) return;
+    };
 
+    issue_tile(0, lds0);
+    if (tid < 64) ladd0[tid] = cadd[tid];
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    for (uint32_t t = 0; t < ntiles; t += 2) {
+        tile_step(t, lds0, ladd0, lds1, ladd1);
+        if (t + 1 >= ntiles) break;
+        tile_step(t + 1, lds1, ladd1, lds0, ladd0);
+    }
     const EParams P = *prm;
+    bool is_pair[NB], is_full[NB];
+    uint32_t pc1[NB], pc2[NB];
+    float pthr[NB];
+    uint32_t cnt2 = 0;
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
-        const uint32_t bb = __float_as_uint(best[nb]), ss = __float_as_uint(sec[nb]);
-        const uint32_t gb = btile[nb] * 64u + (bb & 63u) + 4u * h;
-        const uint32_t gs = stile[nb] * 64u + (ss & 63u) + 4u * h;
-        const float b = __uint_as_float(bb & 0xFFFFFFC0u);
-        const float s = __uint_as_float(ss & 0xFFFFFFC0u);
-        const float r = __uint_as_float(__float_as_uint(thr[nb]) & 0xFFFFFFC0u);
+        const uint32_t bb = best[nb], ss = sec[nb];
+        const uint32_t gb = btile[nb] * 32u + (bb & 31u) + 4u * h;
+        const uint32_t gs = stile[nb] * 32u + (ss & 31u) + 4u * h;
+        const float b = __uint_as_float(bb & 0xFFFFFFE0u);
+        const float s = __uint_as_float(ss & 0xFFFFFFE0u);
+        const float r = __uint_as_float(thr[nb] & 0xFFFFFFE0u);
         const float ob = __shfl_xor(b, 32), os = __shfl_xor(s, 32);
         const uint32_t ogb = __shfl_xor(gb, 32), ogs = __shfl_xor(gs, 32);
         const float orr = __shfl_xor(r, 32);
@@ -388,8 +426,11 @@ __global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep_bf16(
         const float v3 = secB ? fminf(sA, sB) : fminf(rA, bB);
         const uint32_t p = pbase + nb * 32 + r32;
         const bool mine = h == 0 && p < n;
-        bool is_pair = false, is_full = false;
-        float thresh = FLT_MAX;
+        is_pair[nb] = false;
+        is_full[nb] = false;
+        pthr[nb] = FLT_MAX;
+        pc1[nb] = c1;
+        pc2[nb] = c2;
         if (mine) {
             const float xn = xnorm[p];
             const float dbest = xn * xn + (v1 - P.off);
@@ -398,23 +439,112 @@ __global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep_bf16(
             asn[p] = ok1 ? c1 : 0u;
             if (!(ok1 && v2 - v1 > tau)) {
                 R.key[p] = ~0ull;
-                is_pair = ok1 && c2 < kreal && v3 - v1 > tau;
-                is_full = !is_pair;
-                thresh = ok1 ? v1 + tau : FLT_MAX;   // no finite best: every center is a candidate
+                is_pair[nb] = ok1 && c2 < kreal && v3 - v1 > tau;
+                is_full[nb] = !is_pair[nb];
+                pthr[nb] = ok1 ? v1 + tau : FLT_MAX;   // no finite best: every center is a candidate
             }
         }
-        push_cands(R, is_pair, 2u, p, c1, c2);
-        const uint64_t mf = __ballot(is_full);
-        if (mf) {
-            const int leader = __builtin_ctzll(mf);
-            uint32_t base = 0;
-            if ((int)__lane_id() == leader) base = atomicAdd(&R.cnt[1], (uint32_t)__builtin_popcountll(mf));
-            base = __shfl(base, leader);
-            if (is_full) {
-                const uint32_t i = base + lane_prefix(mf);
-                R.full[i] = p;
-                R.thresh[i] = thresh;
+        cnt2 += (is_pair[nb] ? 2u : 0u) + (is_full[nb] ? (1u << 16) : 0u);
+    }
+    // pairs -> two candidates each; the rest -> the list of the collect pass
+    const uint2 base = block_append2(cnt2, &R.cnt[0], &R.cnt[1], sm_app);
+    uint32_t ia = base.x, ib = base.y;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+        const uint32_t p = pbase + nb * 32 + r32;
+        if (is_pair[nb]) {
+            if (ia + 2u <= R.cap) {
+                R.cand[ia] = Cand{p, pc1[nb]};
+                R.cand[ia + 1] = Cand{p, pc2[nb]};
+            } else {
+                cand_overflow(R, p);
             }
+            ia += 2u;
+        }
+        if (is_full[nb]) {
+            R.full[ib] = p;
+            R.thresh[ib] = pthr[nb];
+            ib += 1u;
+        }
+    }
+}
+
+// Collect pass over the listed points: a persistent grid walks (point block, tile)
+// work items; every center whose approximate value is within the point's
+// threshold becomes a candidate.  No state crosses tiles.
+template <int KS>
+__global__ __launch_bounds__(256) void k_collect_bf16(const uint4 *__restrict__ Xb, const uint4 *__restrict__ Cb,
+                                                      const float *__restrict__ cadd, uint32_t ntiles,
+                                                      uint32_t kreal, RefineBufs R) {
+    constexpr int CH = KS * 2;
+    __shared__ __attribute__((aligned(16))) char lds[64 * CH * 16];
+    __shared__ float ladd[64];
+    __shared__ uint32_t sm_app[8];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 5, r32 = lane & 31;
+    const uint32_t count = R.cnt[1];
+    const uint32_t npb = (count + 127) / 128;         // 128 points per block: 4 waves x 32
+    const uint32_t nwork = npb * ntiles;
+    uint32_t cur_pb = 0xFFFFFFFFu;
+    bf16x8 b[KS];
+    uint32_t p = 0;
+    float thr = 0.f;
+    bool valid = false;
+    // each block takes a contiguous run of work items, so its points change rarely
+    const uint32_t per = (nwork + gridDim.x - 1) / gridDim.x;
+    const uint32_t w0 = blockIdx.x * per, w1 = min(nwork, w0 + per);
+    for (uint32_t w = w0; w < w1; ++w) {
+        const uint32_t pb = w / ntiles, t = w % ntiles;
+        if (pb != cur_pb) {
+            cur_pb = pb;
+            const uint32_t q = pb * 128 + wave * 32 + r32;
+            valid = q < count;
+            const uint32_t qq = valid ? q : count - 1;
+            p = R.full[qq];
+            thr = R.thresh[qq];
+            const uint4 *row = Xb + (size_t)p * CH + h * KS;
+#pragma unroll
+            for (int s = 0; s < KS; ++s) b[s] = __builtin_bit_cast(bf16x8, row[s]);
+        }
+        __syncthreads();
+        for (int g = tid; g < 64 * CH; g += 256) {
+            const int r = g / CH, q = g % CH;
+            *(uint4 *)(lds + (r * CH + (q ^ swz<CH>(r))) * 16) = Cb[(size_t)(t * 64 + r) * CH + q];
+        }
+        if (tid < 64) ladd[tid] = cadd[t * 64 + tid];
+        __syncthreads();
+#pragma unroll 1
+        for (int mb = 0; mb < 2; ++mb) {
+            floatx16 acc;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[e] = ladd[mb * 32 + 8 * (e >> 2) + 4 * h + (e & 3)];
+            const int row = mb * 32 + r32;
+            const char *arow = lds + row * (CH * 16);
+            const int sw = swz<CH>(row);
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const bf16x8 a = *(const bf16x8 *)(arow + (((h * KS + s) ^ sw) * 16));
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[s], acc, 0, 0, 0);
+            }
+            uint32_t want = 0;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const float v = __uint_as_float(__float_as_uint(acc[e]) & 0xFFFFFFE0u);
+                const uint32_t c = t * 64u + mb * 32u + 8u * (e >> 2) + 4u * h + (e & 3);
+                want |= (valid && v <= thr && c < kreal) ? (1u << e) : 0u;
+            }
+            uint32_t slot = block_append1((uint32_t)__builtin_popcount(want), &R.cnt[0], sm_app);
+            bool over = false;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                if (want & (1u << e)) {
+                    const uint32_t c = t * 64u + mb * 32u + 8u * (e >> 2) + 4u * h + (e & 3);
+                    if (slot < R.cap) R.cand[slot] = Cand{p, c};
+                    else over = true;
+                    ++slot;
+                }
+            }
+            if (over) cand_overflow(R, p);
         }
     }
 }
@@ -720,18 +850,16 @@ static void launch_bf16(Data &ds, Ctx &c, uint64_t kpad, uint64_t kreal, uint32_
     const dim3 grid((uint32_t)cdiv(n, per));
     hipEvent_t ev;
     c.tic(0, ev);
-    hipLaunchKernelGGL((k_estep_bf16<KS, NB, false>), grid, dim3(256), 0, st, ds.Xb.as<const uint4>(),
+    hipLaunchKernelGGL((k_estep_bf16<KS, NB>), grid, dim3(256), 0, st, ds.Xb.as<const uint4>(),
                        c.s[S_CB].as<const uint4>(), c.s[S_CADD].as<const float>(), ds.xnorm.as<const float>(),
                        c.s[S_PARAMS].as<const EParams>(), n, (uint32_t)(kpad / 64), (uint32_t)kreal, asn, R);
     c.toc(0, ev);
     // statistics: {pair points, listed points} before the collect pass adds candidates
     hipLaunchKernelGGL(k_copy_counts, dim3(1), dim3(1), 0, st, R.cnt, R.cnt + 4);
-    // collect pass over the listed points (blocks beyond the list exit at once); the
-    // center tiles are spread over grid.y so the few listed points fill the chip
-    const uint32_t ysplit = (uint32_t)std::min<uint64_t>(kpad / 64, 16);
-    hipLaunchKernelGGL((k_estep_bf16<KS, NB, true>), dim3(grid.x, ysplit), dim3(256), 0, st, ds.Xb.as<const uint4>(),
-                       c.s[S_CB].as<const uint4>(), c.s[S_CADD].as<const float>(), ds.xnorm.as<const float>(),
-                       c.s[S_PARAMS].as<const EParams>(), n, (uint32_t)(kpad / 64), (uint32_t)kreal, asn, R);
+    // collect pass over the listed points
+    hipLaunchKernelGGL((k_collect_bf16<KS>), dim3(1024), dim3(256), 0, st, ds.Xb.as<const uint4>(),
+                       c.s[S_CB].as<const uint4>(), c.s[S_CADD].as<const float>(), (uint32_t)(kpad / 64),
+                       (uint32_t)kreal, R);
 }
 
 template <int DP>

@@ -702,9 +702,10 @@ done:
     return rc;
 }
 
-/* coreset.h:625-651: m i.i.d. draws; weight = w_i / (m * p_i). */
+/* coreset.h:625-651: m i.i.d. draws; weight = w_i / (m * p_i).  Draw t uses output
+ * first + t of the seed's counter-based stream. */
 void mco_coreset_sample(uint64_t n, const float *probs, const double *w, uint64_t m,
-                        uint64_t seed, uint64_t *indices, float *weights) {
+                        uint64_t seed, uint64_t first, uint64_t *indices, float *weights) {
     double *pd = (double *)malloc(n * sizeof(double));
     uint64_t *fw = (uint64_t *)malloc(n * sizeof(uint64_t)), *pre = (uint64_t *)malloc(n * sizeof(uint64_t));
     for (uint64_t i = 0; i < n; ++i) pd[i] = probs[i];
@@ -712,7 +713,7 @@ void mco_coreset_sample(uint64_t n, const float *probs, const double *w, uint64_
     uint64_t T = 0;
     for (uint64_t i = 0; i < n; ++i) { T += fw[i]; pre[i] = T; }
     for (uint64_t t = 0; t < m; ++t) {
-        uint64_t u = mco_mulhi64(mco_wyrand_at(seed, t), T);
+        uint64_t u = mco_mulhi64(mco_wyrand_at(seed, first + t), T);
         uint64_t lo = 0, hi = n;                   /* first i with pre[i] > u */
         while (lo < hi) { uint64_t mid = (lo + hi) / 2; if (pre[mid] > u) hi = mid; else lo = mid + 1; }
         if (lo >= n) lo = n - 1;

@@ -105,7 +105,7 @@ uint64_t mco_kmeanspp(int msr, const float *X, const float *val, const uint32_t 
 int  mco_coreset_probs(int sens, uint64_t n, uint64_t ncenters, const double *costs,
                        const uint32_t *asn, const double *w, float *probs);
 void mco_coreset_sample(uint64_t n, const float *probs, const double *w, uint64_t m,
-                        uint64_t seed, uint64_t *indices, float *weights);
+                        uint64_t seed, uint64_t first, uint64_t *indices, float *weights);
 
 #ifdef __cplusplus
 }

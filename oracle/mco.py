@@ -59,7 +59,7 @@ def _setup(L):
         "mco_softmax_rows": (None, [P, U64, U64, D, P]),
         "mco_kmeanspp": (U64, [I, P, P, P, P, U64, U64, U64, D, U64, P, P, P, P, I]),
         "mco_coreset_probs": (I, [I, U64, U64, P, P, P, P]),
-        "mco_coreset_sample": (None, [U64, P, P, U64, U64, P, P]),
+        "mco_coreset_sample": (None, [U64, P, P, U64, U64, U64, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -263,10 +263,10 @@ def coreset_probs(sens, ncenters, costs, asn, w=None):
     return probs
 
 
-def coreset_sample(probs, m, seed, w=None):
+def coreset_sample(probs, m, seed, w=None, first_draw=0):
     probs = _f32(probs)
     w = _f64(w)
     idx = np.empty(m, dtype=np.uint64)
     wt = np.empty(m, dtype=np.float32)
-    lib().mco_coreset_sample(len(probs), _p(probs), _p(w), m, seed, _p(idx), _p(wt))
+    lib().mco_coreset_sample(len(probs), _p(probs), _p(w), m, seed, first_draw, _p(idx), _p(wt))
     return wt, idx

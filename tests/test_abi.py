@@ -42,3 +42,30 @@ def test_error_path_without_device():
     rc = L.mcg_ctx_create(12345, ctypes.byref(ctypes.c_void_p()))
     assert rc != 0
     assert L.mcg_last_error()
+
+
+def test_request_structs_match_header(tmp_path):
+    """The ctypes mirrors of mcg_request / mcg_result have the C layout (gcc offsetof)."""
+    import shutil
+    import subprocess
+    from minicore_amd import _lib
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    fields = {"mcg_request": _lib.Request, "mcg_result": _lib.Result}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{ROOT}/include/mcg.h"', "int main(void) {"]
+    for cname, py in fields.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in fields.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)

@@ -220,6 +220,15 @@ def main():
                  "mix": "n ~ logU[1e3,5e5], k ~ logU[5,500]; dense d=64/256 SQRL2 or sparse d=2000 2% JSD"}
         log(json.dumps(serve))
 
+    # Untimed reference copy of known size (256 MiB in, 256 MiB out): profiler
+    # counter runs of this command scale the E-step's FETCH_SIZE / WRITE_SIZE by
+    # what the same counters read for this plain copy kernel.
+    ref_src = torch.empty(64 << 20, dtype=torch.float32, device="cuda").fill_(1.0)
+    ref_dst = torch.empty_like(ref_src)
+    ref_dst.copy_(ref_src)
+    torch.cuda.synchronize()
+    del ref_src, ref_dst
+
     value = 1.0 / bf["sec_per_step"]
     out = {
         "metric": METRIC,

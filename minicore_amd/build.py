@@ -54,6 +54,12 @@ def build_lib(jobs=None):
                 print(out)
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    # A kernel template whose body the host pass cannot instantiate loses its launch
+    # stub without a compile error; the library would then fail to load.
+    undef = [l.split()[-1] for l in _run(["nm", "-u", "-C", tmp]).splitlines() if "mcg::" in l]
+    if undef:
+        os.remove(tmp)
+        raise RuntimeError("libmcg has undefined symbols of its own:\n" + "\n".join(undef[:20]))
     os.replace(tmp, LIB)
     return LIB
 

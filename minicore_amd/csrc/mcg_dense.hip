@@ -215,6 +215,15 @@ struct RefineBufs {
 template <int CH>
 __device__ __forceinline__ int swz(int row) { return row & ((CH < 16 ? CH : 16) - 1); }
 
+// 16 bytes per lane from base + voff + soff into LDS at dst + 16 * lane, through a
+// buffer resource over [base, base + nbytes) (loads past the end return 0).
+// Word 3 0x00020000: raw 32-bit data format of the gfx9 buffer descriptor.
+__device__ __forceinline__ void lds_dma16(const void *base, uint32_t nbytes, char *dst, uint32_t voff,
+                                          uint32_t soff) {
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)nbytes, 0x00020000);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void *)dst, 16, voff, soff, 0, 0);
+}
+
 // Block-wide appends.  Every thread of a 256-thread block calls these together
 // with its own count; each returns the thread's first slot.  One global atomic per
 // list per block (a wave-level scan, then per-wave offsets through LDS), so the
@@ -276,8 +285,8 @@ __global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep_bf16(
     const float *__restrict__ xnorm, const EParams *__restrict__ prm, uint32_t n, uint32_t ntiles, uint32_t kreal,
     uint32_t *__restrict__ asn, RefineBufs R) {
     constexpr int CH = KS * 2;                  // 16-byte chunks per row
-    constexpr int TILE = 64 * CH * 16;          // bytes per tile
-    constexpr int INSTR = CH / 4;               // global_load_lds per wave per tile
+    constexpr int TILE = 64 * CH * 16;          // bytes per tile (64 rows)
+    constexpr int INSTR = CH / 4;               // LDS loads per wave per tile
     // Two distinct LDS objects (not one indexed array): the tile loop alternates
     // between them statically, so the compiler can tell that the copy into one does
     // not alias the reads of the other and need not wait for it.
@@ -299,10 +308,11 @@ __global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep_bf16(
         for (int s = 0; s < KS; ++s) b[nb][s] = __builtin_bit_cast(bf16x8, row[s]);
     }
 
-    // Direct global->LDS copy of tile t into dst.  Instruction j of this wave fills
-    // physical chunk slots P = (4j + wave) * 64 + lane: row P / CH, swizzled chunk
-    // P % CH, whose source chunk is (P % CH) ^ swz(row).  The swizzle repeats every PER
-    // instructions, so the source is a uniform base + j * 4 KiB + voff[j % PER].
+    // Global->LDS copy of tile t into dst through a buffer resource over the center
+    // array (loads past its end return 0).  Load j of this wave fills physical chunk
+    // slots P = (4j + wave) * 64 + lane: row P / CH, swizzled chunk P % CH, whose
+    // source chunk is (P % CH) ^ swz(row).  The swizzle repeats every PER loads, so
+    // the lane's offset is voff[j % PER] and the rest (tile, j * 4 KiB) is scalar.
     constexpr int PER = CH == 64 ? 4 : (CH == 32 ? 2 : 1);
     uint32_t voff[PER];
 #pragma unroll
@@ -311,15 +321,11 @@ __global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep_bf16(
         const uint32_t r = P / CH, q = P % CH;
         voff[i] = (r * CH + (q ^ swz<CH>((int)r))) * 16u - (uint32_t)i * (4u * 64u * 16u);
     }
-    auto issue_tile = [&](uint32_t t, char *dst) {
-        const char *tbase = (const char *)Cb + (size_t)t * TILE;
+    auto issue_tile = [&](uint32_t t, char *dst) __attribute__((always_inline)) {
 #pragma unroll
-        for (int j = 0; j < INSTR; ++j) {
-            const char *src = tbase + j * (4 * 64 * 16) + voff[j % PER];
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
-                                             (__attribute__((address_space(3))) void *)(dst + (j * 4 + wave) * 64 * 16),
-                                             16, 0, 0);
-        }
+        for (int j = 0; j < INSTR; ++j)
+            lds_dma16(Cb, ntiles * (uint32_t)TILE, dst + (j * 4 + wave) * 64 * 16, voff[j % PER],
+                      t * (uint32_t)TILE + (uint32_t)j * (4u * 64u * 16u));
     };
 
     // Running best / second / third per lane as packed values: positive floats whose
@@ -334,57 +340,103 @@ __global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep_bf16(
     }
     // Lane-constant part of the A-fragment address: the chunk of step s is
     // (h KS + s) ^ sw = ((h KS) ^ sw) ^ s (s < KS, KS a power of two), so the byte
-    // address is base ^ (16 s); row r32 + 32 has the same swizzle as r32.
+    // address is abase ^ (16 s); row r32 + 32 has the same swizzle as r32.
     const uint32_t abase0 = (uint32_t)(r32 * (CH * 16) + ((h * KS) ^ swz<CH>(r32)) * 16);
 
-    auto tile_step = [&](uint32_t t, const char *cur, const float *cadd_cur, char *nxt, float *cadd_nxt) {
+    // Software pipeline over 32-center sub-tiles with two accumulator sets in fixed
+    // roles: sub-tile 2t (rows 0-31 of tile t) goes to accA while the argmin epilogue
+    // of sub-tile 2t - 1 (accB) is interleaved with its MFMAs; sub-tile 2t + 1 goes
+    // to accB while accA's epilogue runs.  accB starts as +inf (an epilogue that
+    // changes nothing).
+    constexpr int E = 16 * NB;                  // epilogue elements per sub-tile
+    floatx16 accA[NB], accB[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) accB[nb][e] = __uint_as_float(0x7F800000u);
+    uint32_t b0[NB], s0[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) { b0[nb] = best[nb]; s0[nb] = sec[nb]; }
+
+    auto epi_elem = [&](const floatx16 (&pv)[NB], int i) __attribute__((always_inline)) {
+        const int nb = i / 16, e = i % 16;
+        const uint32_t v = (__float_as_uint(pv[nb][e]) & 0xFFFFFFE0u) | (uint32_t)(8 * (e >> 2) + (e & 3));
+        // third = med3(second, v, third); second = med3(best, v, second); best = min
+        thr[nb] = max(min(sec[nb], v), min(max(sec[nb], v), thr[nb]));
+        sec[nb] = max(min(best[nb], v), min(max(best[nb], v), sec[nb]));
+        best[nb] = min(best[nb], v);
+    };
+    // sub-tile bookkeeping once all elements of sub-tile u went in
+    auto epi_close = [&](uint32_t u) __attribute__((always_inline)) {
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            // the new best came from sub-tile u; the second is either the old best
+            // (its sub-tile moves over) or from sub-tile u
+            const bool nbest = best[nb] != b0[nb], nsec = sec[nb] != s0[nb];
+            const uint32_t st_new = (nbest && sec[nb] == b0[nb]) ? btile[nb] : u;
+            stile[nb] = (nbest || nsec) ? st_new : stile[nb];
+            btile[nb] = nbest ? u : btile[nb];
+            b0[nb] = best[nb];
+            s0[nb] = sec[nb];
+            // consume the results here: keeps the epilogue inside this sub-tile's
+            // block, where it is interleaved with the MFMAs (instead of being sunk
+            // past the end of the tile and run as one serial stretch)
+            asm volatile("" ::"v"(thr[nb]), "v"(btile[nb]), "v"(stile[nb]), "v"(b0[nb]), "v"(s0[nb]));
+        }
+    };
+
+    // MFMA chain of one sub-tile into acc, with the epilogue of pv interleaved.
+    auto subtile = [&](const char *cur, const float *cadd_cur, uint32_t mb, floatx16 (&acc)[NB],
+                       const floatx16 (&pv)[NB], uint32_t upv) __attribute__((always_inline)) {
+        floatx16 init;   // C operand of the first MFMA of each chain (no copies)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) init[e] = cadd_cur[mb * 32 + 8 * (e >> 2) + 4 * h + (e & 3)];
+        // recomputed per sub-tile (the empty asm keeps the compiler from holding all
+        // KS addresses in registers across the loop)
+        uint32_t ab = abase0;
+        asm volatile("" : "+v"(ab));
+        const char *tb = cur + mb * (32u * CH * 16u);
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const bf16x8 a = *(const bf16x8 *)(tb + (ab ^ (uint32_t)(s * 16)));
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb)
+                acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[nb][s], s == 0 ? init : acc[nb], 0, 0, 0);
+#pragma unroll
+            for (int i = (s * E) / KS; i < ((s + 1) * E) / KS; ++i) epi_elem(pv, i);
+        }
+        // order: per step one fragment read, then the step's MFMAs, each followed by
+        // its share of the VALU work (epilogue: 4 ops per element; 1 address op)
+        // The 4 reads of the chain's initial values and fragments 0..DEPTH-1 go first;
+        // then each step issues the read of fragment s + DEPTH, so a fragment is in
+        // flight for DEPTH steps before its MFMAs.
+        constexpr int VPS = (4 * E) / KS + 1;   // VALU ops per step
+        constexpr int VA = (VPS + NB - 1) / NB, VB = VPS - VA > 0 ? VPS - VA : 1;
+        constexpr int DEPTH = 3;
+        __builtin_amdgcn_sched_group_barrier(0x100, 4 + DEPTH, 0);
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, VA, 0);
+            if constexpr (NB == 2) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x002, VB, 0);
+            }
+            if (s + DEPTH < KS) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        }
+        epi_close(upv);
+    };
+
+    auto tile_step = [&](uint32_t t, const char *cur, const float *cadd_cur, char *nxt,
+                         float *cadd_nxt) __attribute__((always_inline)) {
         float addn = 0.f;
         const bool more = t + 1 < ntiles;
         if (more) {
             issue_tile(t + 1, nxt);
             if (tid < 64) addn = cadd[(t + 1) * 64 + tid];
         }
-#pragma unroll 1
-        for (uint32_t mb = 0; mb < 2; ++mb) {
-            floatx16 acc[NB];
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const float v = cadd_cur[mb * 32 + 8 * (e >> 2) + 4 * h + (e & 3)];
-#pragma unroll
-                for (int nb = 0; nb < NB; ++nb) acc[nb][e] = v;
-            }
-            // rows 32..63 of the tile: row + 32 keeps the swizzle (32 & 15 == 0)
-            const char *arow = cur + abase0 + mb * (32u * CH * 16u);
-            bf16x8 a = *(const bf16x8 *)arow;
-#pragma unroll
-            for (int s = 0; s < KS; ++s) {
-                bf16x8 an = a;
-                if (s + 1 < KS) an = *(const bf16x8 *)(cur + ((abase0 ^ (uint32_t)((s + 1) * 16)) + mb * (32u * CH * 16u)));
-#pragma unroll
-                for (int nb = 0; nb < NB; ++nb)
-                    acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[nb][s], acc[nb], 0, 0, 0);
-                a = an;
-            }
-            const uint32_t u = 2u * t + mb;
-#pragma unroll
-            for (int nb = 0; nb < NB; ++nb) {
-                const uint32_t b0 = best[nb], s0 = sec[nb];
-#pragma unroll
-                for (int e = 0; e < 16; ++e) {
-                    const uint32_t v = (__float_as_uint(acc[nb][e]) & 0xFFFFFFE0u) | (uint32_t)(8 * (e >> 2) + (e & 3));
-                    // third = med3(second, v, third); second = med3(best, v, second); best = min
-                    thr[nb] = max(min(sec[nb], v), min(max(sec[nb], v), thr[nb]));
-                    sec[nb] = max(min(best[nb], v), min(max(best[nb], v), sec[nb]));
-                    best[nb] = min(best[nb], v);
-                }
-                // the new best came from this sub-tile; the second is either the old best
-                // (its sub-tile moves over) or from this sub-tile
-                const bool nbest = best[nb] != b0, nsec = sec[nb] != s0;
-                const uint32_t st_new = (nbest && sec[nb] == b0) ? btile[nb] : u;
-                stile[nb] = (nbest || nsec) ? st_new : stile[nb];
-                btile[nb] = nbest ? u : btile[nb];
-            }
-        }
+        subtile(cur, cadd_cur, 0u, accA, accB, 2u * t - 1u);
+        subtile(cur, cadd_cur, 1u, accB, accA, 2u * t);
         if (more && tid < 64) cadd_nxt[tid] = addn;
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
@@ -399,6 +451,11 @@ __global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep_bf16(
         if (t + 1 >= ntiles) break;
         tile_step(t + 1, lds1, ladd1, lds0, ladd0);
     }
+    // epilogue of the last sub-tile (accB)
+#pragma unroll
+    for (int i = 0; i < E; ++i) epi_elem(accB, i);
+    epi_close(2u * ntiles - 1u);
+
     const EParams P = *prm;
     bool is_pair[NB], is_full[NB];
     uint32_t pc1[NB], pc2[NB];
@@ -469,137 +526,146 @@ __global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep_bf16(
     }
 }
 
-// Collect pass over the listed points: a persistent grid walks (point block, tile)
-// work items; every center whose approximate value is within the point's
-// threshold becomes a candidate.  No state crosses tiles.
+// Collect pass over the listed points: every center whose approximate value is
+// within the point's threshold becomes a candidate.  Blocks walk point blocks of
+// 128 listed points (4 waves x 32, the MFMA B operand in registers) over the tiles
+// of their share of the centers (blockIdx.y), streaming the tiles through two LDS
+// buffers with the main pass's DMA; one block-wide append per tile.
 template <int KS>
 __global__ __launch_bounds__(256) void k_collect_bf16(const uint4 *__restrict__ Xb, const uint4 *__restrict__ Cb,
                                                       const float *__restrict__ cadd, uint32_t ntiles,
                                                       uint32_t kreal, RefineBufs R) {
     constexpr int CH = KS * 2;
-    __shared__ __attribute__((aligned(16))) char lds[64 * CH * 16];
-    __shared__ float ladd[64];
+    constexpr int TILE = 64 * CH * 16;
+    constexpr int INSTR = CH / 4;
+    __shared__ __attribute__((aligned(16))) char lds0[TILE];
+    __shared__ __attribute__((aligned(16))) char lds1[TILE];
+    __shared__ float ladd0[64], ladd1[64];
     __shared__ uint32_t sm_app[8];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int h = lane >> 5, r32 = lane & 31;
     const uint32_t count = R.cnt[1];
-    const uint32_t npb = (count + 127) / 128;         // 128 points per block: 4 waves x 32
-    const uint32_t nwork = npb * ntiles;
-    uint32_t cur_pb = 0xFFFFFFFFu;
+    const uint32_t t_begin = blockIdx.y * ntiles / gridDim.y, t_end = (blockIdx.y + 1) * ntiles / gridDim.y;
+    if (t_begin >= t_end) return;   // uniform over the block
+
+    constexpr int PER = CH == 64 ? 4 : (CH == 32 ? 2 : 1);
+    uint32_t voff[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const uint32_t P = (uint32_t)((i * 4 + wave) * 64 + lane);
+        const uint32_t r = P / CH, q = P % CH;
+        voff[i] = (r * CH + (q ^ swz<CH>((int)r))) * 16u - (uint32_t)i * (4u * 64u * 16u);
+    }
+    auto issue_tile = [&](uint32_t t, char *dst) __attribute__((always_inline)) {
+#pragma unroll
+        for (int j = 0; j < INSTR; ++j)
+            lds_dma16(Cb, ntiles * (uint32_t)TILE, dst + (j * 4 + wave) * 64 * 16, voff[j % PER],
+                      t * (uint32_t)TILE + (uint32_t)j * (4u * 64u * 16u));
+    };
+    const uint32_t abase0 = (uint32_t)(r32 * (CH * 16) + ((h * KS) ^ swz<CH>(r32)) * 16);
+
     bf16x8 b[KS];
     uint32_t p = 0;
     float thr = 0.f;
     bool valid = false;
-    // each block takes a contiguous run of work items, so its points change rarely
-    const uint32_t per = (nwork + gridDim.x - 1) / gridDim.x;
-    const uint32_t w0 = blockIdx.x * per, w1 = min(nwork, w0 + per);
-    for (uint32_t w = w0; w < w1; ++w) {
-        const uint32_t pb = w / ntiles, t = w % ntiles;
-        if (pb != cur_pb) {
-            cur_pb = pb;
-            const uint32_t q = pb * 128 + wave * 32 + r32;
-            valid = q < count;
-            const uint32_t qq = valid ? q : count - 1;
-            p = R.full[qq];
-            thr = R.thresh[qq];
-            const uint4 *row = Xb + (size_t)p * CH + h * KS;
+
+    auto tile_step = [&](uint32_t t, const char *cur, const float *cadd_cur, char *nxt,
+                         float *cadd_nxt) __attribute__((always_inline)) {
+        float addn = 0.f;
+        const bool more = t + 1 < t_end;
+        if (more) {
+            issue_tile(t + 1, nxt);
+            if (tid < 64) addn = cadd[(t + 1) * 64 + tid];
+        }
+        uint32_t want = 0;   // bit mb * 16 + e
 #pragma unroll
-            for (int s = 0; s < KS; ++s) b[s] = __builtin_bit_cast(bf16x8, row[s]);
-        }
-        __syncthreads();
-        for (int g = tid; g < 64 * CH; g += 256) {
-            const int r = g / CH, q = g % CH;
-            *(uint4 *)(lds + (r * CH + (q ^ swz<CH>(r))) * 16) = Cb[(size_t)(t * 64 + r) * CH + q];
-        }
-        if (tid < 64) ladd[tid] = cadd[t * 64 + tid];
-        __syncthreads();
-#pragma unroll 1
         for (int mb = 0; mb < 2; ++mb) {
-            floatx16 acc;
+            floatx16 init;
 #pragma unroll
-            for (int e = 0; e < 16; ++e) acc[e] = ladd[mb * 32 + 8 * (e >> 2) + 4 * h + (e & 3)];
-            const int row = mb * 32 + r32;
-            const char *arow = lds + row * (CH * 16);
-            const int sw = swz<CH>(row);
+            for (int e = 0; e < 16; ++e) init[e] = cadd_cur[mb * 32 + 8 * (e >> 2) + 4 * h + (e & 3)];
+            floatx16 acc;
+            const char *tb = cur + mb * (32u * CH * 16u);
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
-                const bf16x8 a = *(const bf16x8 *)(arow + (((h * KS + s) ^ sw) * 16));
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[s], acc, 0, 0, 0);
+                const bf16x8 a = *(const bf16x8 *)(tb + (abase0 ^ (uint32_t)(s * 16)));
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[s], s == 0 ? init : acc, 0, 0, 0);
             }
-            uint32_t want = 0;
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
                 const float v = __uint_as_float(__float_as_uint(acc[e]) & 0xFFFFFFE0u);
                 const uint32_t c = t * 64u + mb * 32u + 8u * (e >> 2) + 4u * h + (e & 3);
-                want |= (valid && v <= thr && c < kreal) ? (1u << e) : 0u;
+                want |= (valid && v <= thr && c < kreal) ? (1u << (mb * 16 + e)) : 0u;
             }
-            uint32_t slot = block_append1((uint32_t)__builtin_popcount(want), &R.cnt[0], sm_app);
-            bool over = false;
+        }
+        uint32_t slot = block_append1((uint32_t)__builtin_popcount(want), &R.cnt[0], sm_app);
+        bool over = false;
+        while (want) {
+            const int bit = __builtin_ctz(want);
+            want &= want - 1u;
+            const int mb = bit >> 4, e = bit & 15;
+            const uint32_t c = t * 64u + mb * 32u + 8u * (e >> 2) + 4u * h + (e & 3);
+            if (slot < R.cap) R.cand[slot] = Cand{p, c};
+            else over = true;
+            ++slot;
+        }
+        if (over) cand_overflow(R, p);
+        if (more && tid < 64) cadd_nxt[tid] = addn;
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+    };
+
+    for (uint32_t pb = blockIdx.x; pb * 128u < count; pb += gridDim.x) {   // uniform
+        const uint32_t q = pb * 128u + wave * 32u + r32;
+        valid = q < count;
+        const uint32_t qq = valid ? q : count - 1;
+        p = R.full[qq];
+        thr = R.thresh[qq];
+        const uint4 *row = Xb + (size_t)p * CH + h * KS;
 #pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                if (want & (1u << e)) {
-                    const uint32_t c = t * 64u + mb * 32u + 8u * (e >> 2) + 4u * h + (e & 3);
-                    if (slot < R.cap) R.cand[slot] = Cand{p, c};
-                    else over = true;
-                    ++slot;
-                }
-            }
-            if (over) cand_overflow(R, p);
+        for (int s = 0; s < KS; ++s) b[s] = __builtin_bit_cast(bf16x8, row[s]);
+        __syncthreads();
+        issue_tile(t_begin, lds0);
+        if (tid < 64) ladd0[tid] = cadd[t_begin * 64 + tid];
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        for (uint32_t t = t_begin; t < t_end; t += 2) {
+            tile_step(t, lds0, ladd0, lds1, ladd1);
+            if (t + 1 >= t_end) break;
+            tile_step(t + 1, lds1, ladd1, lds0, ladd0);
         }
     }
 }
 
 // Exact cost of each candidate (sequential FMA chain over the dims, the oracle's
 // order), min-reduced per point as (cost bits << 32 | center): lowest cost, then
-// lowest index; the result does not depend on the order of the atomics.
-// 256 candidates per block; their two rows staged through LDS in 32-dim slabs.
-__global__ __launch_bounds__(256) void k_exact_cands(const float *__restrict__ X, uint32_t d,
+// lowest index; the result does not depend on the order of the atomics.  One
+// thread per candidate; rows read straight from HBM / L2 with 16-byte loads when
+// vec (d % 4 == 0 and 16-byte aligned rows), so many candidates are in flight.
+__global__ __launch_bounds__(256) void k_exact_cands(const float *__restrict__ X, uint32_t d, int vec,
                                                      const float *__restrict__ C, RefineBufs R) {
-    __shared__ float xs[256][33];
-    __shared__ float cs[256][33];
-    __shared__ Cand cd[256];
     const uint32_t cnt = min(R.cnt[0], R.cap);
-    const int tid = threadIdx.x;
-    const bool vec = (d & 3u) == 0;
-    for (uint32_t base = blockIdx.x * 256u; base < cnt; base += gridDim.x * 256u) {
-        __syncthreads();
-        cd[tid] = base + tid < cnt ? R.cand[base + tid] : Cand{0xFFFFFFFFu, 0u};
-        __syncthreads();
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < cnt; i += gridDim.x * 256u) {
+        const Cand e = R.cand[i];
+        const float *x = X + (uint64_t)e.p * d, *c = C + (uint64_t)e.c * d;
         float s = 0.f;
-        for (uint32_t j0 = 0; j0 < d; j0 += 32) {
-            if (j0) __syncthreads();
-            if (vec) {
-                // 8 float4 per 32-dim row slab
-                for (int q = tid; q < 256 * 8; q += 256) {
-                    const int r = q >> 3, jj = (q & 7) * 4;
-                    const Cand e = cd[r];
-                    float4 xv = make_float4(0.f, 0.f, 0.f, 0.f), cv = xv;
-                    if (e.p != 0xFFFFFFFFu && j0 + jj < d) {
-                        xv = *(const float4 *)(X + (uint64_t)e.p * d + j0 + jj);
-                        cv = *(const float4 *)(C + (uint64_t)e.c * d + j0 + jj);
-                    }
-                    xs[r][jj] = xv.x; xs[r][jj + 1] = xv.y; xs[r][jj + 2] = xv.z; xs[r][jj + 3] = xv.w;
-                    cs[r][jj] = cv.x; cs[r][jj + 1] = cv.y; cs[r][jj + 2] = cv.z; cs[r][jj + 3] = cv.w;
-                }
-            } else {
-                for (int q = tid; q < 256 * 32; q += 256) {
-                    const int r = q >> 5, jj = q & 31;
-                    const Cand e = cd[r];
-                    const bool ok = e.p != 0xFFFFFFFFu && j0 + jj < d;
-                    xs[r][jj] = ok ? X[(uint64_t)e.p * d + j0 + jj] : 0.f;
-                    cs[r][jj] = ok ? C[(uint64_t)e.c * d + j0 + jj] : 0.f;
-                }
+        if (vec) {
+            const float4 *x4 = (const float4 *)x, *c4 = (const float4 *)c;
+            const uint32_t d4 = d >> 2;
+#pragma unroll 8
+            for (uint32_t j = 0; j < d4; ++j) {
+                const float4 a = x4[j], b = c4[j];
+                float t;
+                t = a.x - b.x; s = fmaf(t, t, s);
+                t = a.y - b.y; s = fmaf(t, t, s);
+                t = a.z - b.z; s = fmaf(t, t, s);
+                t = a.w - b.w; s = fmaf(t, t, s);
             }
-            __syncthreads();
-            const int jn = (int)min<uint32_t>(32, d - j0);
-            for (int jj = 0; jj < jn; ++jj) { const float t = xs[tid][jj] - cs[tid][jj]; s = fmaf(t, t, s); }
+        } else {
+            for (uint32_t j = 0; j < d; ++j) { const float t = x[j] - c[j]; s = fmaf(t, t, s); }
         }
-        const Cand e = cd[tid];
-        if (e.p != 0xFFFFFFFFu) {
-            // costs are >= 0 (or NaN), so their bit patterns order like their values
-            const unsigned long long key = ((unsigned long long)__float_as_uint(s) << 32) | e.c;
-            atomicMin(&R.key[e.p], key);
-        }
+        // costs are >= 0 (or NaN), so their bit patterns order like their values
+        const unsigned long long key = ((unsigned long long)__float_as_uint(s) << 32) | e.c;
+        atomicMin(&R.key[e.p], key);
     }
 }
 
@@ -857,7 +923,7 @@ static void launch_bf16(Data &ds, Ctx &c, uint64_t kpad, uint64_t kreal, uint32_
     // statistics: {pair points, listed points} before the collect pass adds candidates
     hipLaunchKernelGGL(k_copy_counts, dim3(1), dim3(1), 0, st, R.cnt, R.cnt + 4);
     // collect pass over the listed points
-    hipLaunchKernelGGL((k_collect_bf16<KS>), dim3(1024), dim3(256), 0, st, ds.Xb.as<const uint4>(),
+    hipLaunchKernelGGL((k_collect_bf16<KS>), dim3(512, kpad >= 128 ? 2 : 1), dim3(256), 0, st, ds.Xb.as<const uint4>(),
                        c.s[S_CB].as<const uint4>(), c.s[S_CADD].as<const float>(), (uint32_t)(kpad / 64),
                        (uint32_t)kreal, R);
 }
@@ -912,6 +978,8 @@ void dense_sqrl2_assign(Data &ds, const float *C, uint64_t k, int precision, uin
     }
     const uint64_t kpad = cdiv(k, 64) * 64;
     const uint64_t n = ds.n;
+    // the E-step reads the BF16 centers through a buffer resource (32-bit byte range)
+    REQUIRE(kpad * dpad * sizeof(uint16_t) < (1ull << 31), "BF16 form: k * d too large (use precision exact)");
     c.s[S_CB].reserve(kpad * dpad * sizeof(uint16_t));
     c.s[S_CADD].reserve(kpad * sizeof(float));
     c.s[S_CNORM].reserve(kpad * sizeof(double));
@@ -942,7 +1010,10 @@ void dense_sqrl2_assign(Data &ds, const float *C, uint64_t k, int precision, uin
     case 256: launch_bf16<16, 2>(ds, c, kpad, k, asn, R, st); break;
     default: launch_bf16<32, 1>(ds, c, kpad, k, asn, R, st); break;
     }
-    hipLaunchKernelGGL(k_exact_cands, dim3(2048), dim3(256), 0, st, ds.X, d, C, R);
+    {
+        const int vec = (d % 4 == 0) && ((uintptr_t)ds.X % 16 == 0) && ((uintptr_t)C % 16 == 0);
+        hipLaunchKernelGGL(k_exact_cands, dim3(2048), dim3(256), 0, st, ds.X, d, vec, C, R);
+    }
     hipLaunchKernelGGL(k_apply_keys, dim3(1024), dim3(256), 0, st, R, asn);
     hipLaunchKernelGGL(k_rescore_full, dim3(256), dim3(256), 0, st, ds.X, d, C, (uint32_t)k, R, asn);
     HIP_CHECK(hipMemcpyAsync(c.refine_host(), R.cnt + 4, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));

@@ -33,18 +33,24 @@ void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st) {
 }
 
 // Points whose assignment is >= k (never produced by the E-step kernels) are
-// left out of every cluster.
-__global__ void k_hist(const uint32_t *__restrict__ asn, uint64_t n, uint32_t nblk, uint32_t k,
-                       uint32_t *__restrict__ hist) {
+// left out of every cluster.  Block b counts its SORT_BLK points in an LDS
+// histogram, then writes column b of hist (cluster-major, hist[c * nblk + b]).
+__global__ __launch_bounds__(256) void k_hist(const uint32_t *__restrict__ asn, uint64_t n, uint32_t nblk,
+                                              uint32_t k, uint32_t *__restrict__ hist) {
+    extern __shared__ uint32_t h[];
     const uint64_t b = blockIdx.x;
+    for (uint32_t c = threadIdx.x; c < k; c += blockDim.x) h[c] = 0;
+    __syncthreads();
     const uint64_t i0 = b * SORT_BLK;
     for (uint32_t q = threadIdx.x; q < SORT_BLK; q += blockDim.x) {
-        uint64_t i = i0 + q;
+        const uint64_t i = i0 + q;
         if (i < n) {
             const uint32_t a = asn[i];
-            if (a < k) atomicAdd(&hist[(uint64_t)a * nblk + b], 1u);
+            if (a < k) atomicAdd(&h[a], 1u);
         }
     }
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < k; c += blockDim.x) hist[(uint64_t)c * nblk + b] = h[c];
 }
 
 // --- exclusive scan of u32 (length L), three phases ------------------------
@@ -131,10 +137,9 @@ __global__ void k_counts(const uint32_t *__restrict__ hist, const uint32_t *__re
 }
 
 // Stable scatter: one wave per histogram block, LDS cursors for all clusters.
-__global__ void k_scatter(const uint32_t *__restrict__ asn, uint64_t n, uint32_t nblk, uint32_t k,
+__global__ void k_scatter(const uint32_t *__restrict__ asn, uint64_t n, uint32_t nblk, uint32_t k, uint32_t kbits,
                           const uint32_t *__restrict__ base, uint32_t *__restrict__ perm) {
     extern __shared__ uint32_t cur[];
-    __shared__ uint32_t keys[64];
     const uint32_t b = blockIdx.x;
     const int lane = threadIdx.x;
     const uint64_t i0 = (uint64_t)b * SORT_BLK;
@@ -150,17 +155,16 @@ __global__ void k_scatter(const uint32_t *__restrict__ asn, uint64_t n, uint32_t
         const uint32_t a = i < n ? asn[i] : 0xFFFFFFFFu;
         const bool valid = a < k;
         const uint32_t key = valid ? a : 0xFFFFFFFFu;
-        keys[lane] = key;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        uint32_t rank = 0, tot = 0;
-        for (int j = 0; j < 64; ++j) {
-            const uint32_t kj = keys[j];
-            const bool eq = kj == key;
-            rank += (eq && j < lane) ? 1u : 0u;
-            tot += eq ? 1u : 0u;
+        // lanes holding the same key: intersect the ballots of each key bit (and of
+        // validity); rank = such lanes below this one, tot = all of them
+        uint64_t same = valid ? __ballot(valid) : ~__ballot(valid);
+        for (uint32_t bit = 0; bit < kbits; ++bit) {
+            const bool on = (key >> bit) & 1u;
+            const uint64_t m = __ballot(on);
+            same &= on ? m : ~m;
         }
+        const uint32_t rank = (uint32_t)__builtin_popcountll(same & ((1ull << lane) - 1ull));
+        const uint32_t tot = (uint32_t)__builtin_popcountll(same);
         uint32_t pos = 0;
         if (valid) { pos = cur[key] + rank; perm[pos] = (uint32_t)i; }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -307,16 +311,18 @@ static SortOut sort_by_cluster(Ctx &c, const uint32_t *asn, uint64_t n, uint64_t
     uint32_t *perm = (uint32_t *)c.s[S_PERM].reserve(n * sizeof(uint32_t));
     uint32_t *cnt = (uint32_t *)c.s[S_CNT].reserve(3 * (k + 1) * sizeof(uint32_t));
     uint32_t *off = cnt + (k + 1), *items = cnt + 2 * (k + 1);
-    fill_u32(hist, 0, L, st);
-    hipLaunchKernelGGL(k_hist, dim3(nblk), dim3(256), 0, st, asn, n, nblk, (uint32_t)k, hist);
+    hipLaunchKernelGGL(k_hist, dim3(nblk), dim3(256), (size_t)k * sizeof(uint32_t), st, asn, n, nblk, (uint32_t)k,
+                       hist);
     scan_exclusive(hist, base, L, c.s[S_MISC0], st);
     hipLaunchKernelGGL(k_counts, dim3((uint32_t)cdiv(k, 256)), dim3(256), 0, st, hist, base, k, nblk, cnt, off,
                        items);
     // items -> exclusive prefix (k+1 entries: the last one becomes the total)
     fill_u32(items + k, 0, 1, st);
     hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, items, k + 1, (uint32_t *)nullptr);
+    uint32_t kbits = 1;   // bits of the largest cluster id
+    while ((1ull << kbits) < k) ++kbits;
     hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(64), (size_t)k * sizeof(uint32_t), st, asn, n, nblk,
-                       (uint32_t)k, base, perm);
+                       (uint32_t)k, kbits, base, perm);
     return SortOut{perm, cnt, off, items};
 }
 

@@ -178,6 +178,17 @@ def main():
         log(json.dumps(fp))
 
     flops = 2.0 * n * k * d
+    # HBM bytes per E-step launch from the profiler's FETCH_SIZE + WRITE_SIZE counters
+    # (scaled by a plain copy of known size), measured on this workload and kept with
+    # the command that produced them; algorithmic bytes for comparison.
+    dpad = 64 if d <= 64 else 128 if d <= 128 else 256 if d <= 256 else 512
+    kpad = (k + 63) // 64 * 64
+    alg_bytes = n * dpad * 2 + n * 4 + n * 4 + kpad * dpad * 2
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_estep_bf16_n{n}_d{d}_k{k}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
     achieved = flops / (bf["estep_ms"] * 1e-3) / 1e12
     mstep_bytes = n * d * 4 + n * 4 * 3
     cpu = None
@@ -248,7 +259,8 @@ def main():
                                f"term + FP32 near-tie re-score + FP64 centroid/cost sums",
                    "n": n, "d": d, "k": k, "seed": args.seed, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_BF16_TFLOPS, "traffic": traffic,
+                     "traffic_algorithmic_bytes": alg_bytes,
                      "kernel": "k_estep_bf16<16,2>", "kernel_ms": bf["estep_ms"],
                      "algorithmic": "2*n*k*d flop per launch"},
         "latency_ms": {"p50": bf["p50_ms"], "p99": bf["p99_ms"]},

@@ -96,6 +96,7 @@ def serving(mcg, reqs, prec, rate, seed, workers):
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, size=len(reqs))
     lat = []
+    stages = []
     with mcg.Server(0, nworkers=workers) as srv:
         t0 = time.perf_counter()
         tickets = []
@@ -111,9 +112,10 @@ def serving(mcg, reqs, prec, rate, seed, workers):
             if r["state"] != mcg.REQ_DONE:
                 raise RuntimeError(f"request failed: {r}")
             lat.append(r["queue_ms"] + r["run_ms"])
+            stages.append(r["stage_ms"])
             srv.release(t)
         wall = time.perf_counter() - t0
-    return np.array(lat), wall
+    return np.array(lat), wall, np.array(stages)
 
 
 def cpu_baseline(X_host, C_host, n_full, nthreads):
@@ -218,14 +220,20 @@ def main():
         svc_ms = float(np.mean(svc))
         rate = 0.7 / (svc_ms * 1e-3)
         log(f"serving: mean service {svc_ms:.2f} ms -> {rate:.1f} req/s")
-        lat_bf, wall_bf = serving(mcg, reqs, mcg.PREC_BF16, rate, args.seed + 11, args.serve_workers)
-        lat_fp, wall_fp = serving(mcg, reqs, mcg.PREC_FP32, rate, args.seed + 11, args.serve_workers)
+        lat_bf, wall_bf, st_bf = serving(mcg, reqs, mcg.PREC_BF16, rate, args.seed + 11, args.serve_workers)
+        lat_fp, wall_fp, _ = serving(mcg, reqs, mcg.PREC_FP32, rate, args.seed + 11, args.serve_workers)
+        worst = int(np.argmax(lat_bf))
         pct = lambda a, q: float(np.percentile(a, q))
         serve = {"requests": len(reqs), "workers": args.serve_workers, "rate_per_s": rate,
                  "mean_service_ms_alone": svc_ms, "offered_load": 0.7,
                  "p50_ms": pct(lat_bf, 50), "p99_ms": pct(lat_bf, 99), "throughput_per_s": len(reqs) / wall_bf,
                  "fp32_p50_ms": pct(lat_fp, 50), "fp32_p99_ms": pct(lat_fp, 99),
                  "fp32_throughput_per_s": len(reqs) / wall_fp,
+                 "mean_stage_ms": dict(zip(["seeding", "sensitivities", "coreset", "lloyd", "assign"],
+                                           [float(x) for x in st_bf.mean(axis=0)])),
+                 "slowest_request": {"n": reqs[worst]["n"], "d": reqs[worst]["d"], "k": reqs[worst]["k"],
+                                     "msr": reqs[worst]["msr"], "latency_ms": float(lat_bf[worst]),
+                                     "stage_ms": [float(x) for x in st_bf[worst]]},
                  "pipeline": "D2 seeding -> LBK sensitivities -> 1% coreset (>= k rows) -> 10 weighted Lloyd "
                              "iterations on the coreset -> assignment of all rows",
                  "mix": "n ~ logU[1e3,5e5], k ~ logU[5,500]; dense d=64/256 SQRL2 or sparse d=2000 2% JSD"}

@@ -228,6 +228,7 @@ typedef struct mcg_result {
     uint64_t numiter;        /* Lloyd iterations on the coreset */
     double queue_ms;         /* submit -> start */
     double run_ms;           /* start -> end */
+    double stage_ms[5];      /* rows + seeding, sensitivities, coreset, Lloyd, final assignment */
     char error[256];         /* message of a failed request */
 } mcg_result;
 

@@ -51,6 +51,7 @@ class Result(C.Structure):
         ("cost", C.c_double), ("coreset_initcost", C.c_double), ("coreset_finalcost", C.c_double),
         ("coreset_m", C.c_uint64), ("numiter", C.c_uint64),
         ("queue_ms", C.c_double), ("run_ms", C.c_double),
+        ("stage_ms", C.c_double * 5),
         ("error", C.c_char * 256),
     ]
 
@@ -434,7 +435,8 @@ class Server:
     def _res(r):
         return dict(state=r.state, status=r.status, cost=r.cost, coreset_initcost=r.coreset_initcost,
                     coreset_finalcost=r.coreset_finalcost, coreset_m=r.coreset_m, numiter=r.numiter,
-                    queue_ms=r.queue_ms, run_ms=r.run_ms, error=r.error.decode(errors="replace"))
+                    queue_ms=r.queue_ms, run_ms=r.run_ms, stage_ms=list(r.stage_ms),
+                    error=r.error.decode(errors="replace"))
 
     def poll(self, ticket):
         r = Result()

@@ -40,22 +40,37 @@ __global__ void k_colsum_partial(const float *__restrict__ X, uint64_t n, uint32
     const uint64_t r0 = (uint64_t)blockIdx.y * rows_per;
     const uint64_t r1 = min<uint64_t>(n, r0 + rows_per);
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < d; j += gridDim.x * blockDim.x) {
-        double s = 0.;
-        for (uint64_t r = r0; r < r1; ++r) s += (double)X[r * d + j];
-        part[(uint64_t)blockIdx.y * d + j] = s;
+        // four rows in flight; the four running sums are added in a fixed order
+        double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
+        uint64_t r = r0;
+        for (; r + 4 <= r1; r += 4) {
+            s0 += (double)X[r * d + j];
+            s1 += (double)X[(r + 1) * d + j];
+            s2 += (double)X[(r + 2) * d + j];
+            s3 += (double)X[(r + 3) * d + j];
+        }
+        for (; r < r1; ++r) s0 += (double)X[r * d + j];
+        part[(uint64_t)blockIdx.y * d + j] = (s0 + s1) + (s2 + s3);
     }
 }
 
+// one block per column: the parts summed by 256 threads, then a fixed tree
 __global__ void k_colsum_final(const double *__restrict__ part, uint32_t nparts, uint32_t d,
                                uint64_t n, float *__restrict__ mu) {
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= d) return;
+    __shared__ double sm[256];
+    const uint32_t j = blockIdx.x;
     double s = 0.;
-    for (uint32_t p = 0; p < nparts; ++p) s += part[(uint64_t)p * d + j];
-    mu[j] = (float)(s / (double)n);
+    for (uint32_t p = threadIdx.x; p < nparts; p += blockDim.x) s += part[(uint64_t)p * d + j];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) mu[j] = (float)(sm[0] / (double)n);
 }
 
-// one wave per row
+// one wave per row; lane l converts columns [4l, 4l + 4) of each 256-column pass
 __global__ void k_make_xb(const float *__restrict__ X, uint64_t n, uint32_t d, uint32_t dpad,
                           const float *__restrict__ mu, uint16_t *__restrict__ Xb,
                           float *__restrict__ xnorm) {
@@ -63,10 +78,17 @@ __global__ void k_make_xb(const float *__restrict__ X, uint64_t n, uint32_t d, u
     const int lane = threadIdx.x & 63;
     if (row >= n) return;
     double s = 0.;
-    for (uint32_t j = lane; j < dpad; j += 64) {
-        float v = j < d ? X[row * d + j] - mu[j] : 0.f;
-        Xb[row * dpad + j] = f2bf_rne(v);
-        s += (double)v * (double)v;
+    for (uint32_t j0 = lane * 4; j0 < dpad; j0 += 256) {
+        float v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t j = j0 + i;
+            v[i] = j < d ? X[row * d + j] - mu[j] : 0.f;
+            s += (double)v[i] * (double)v[i];
+        }
+        const uint2 packed = make_uint2((uint32_t)f2bf_rne(v[0]) | ((uint32_t)f2bf_rne(v[1]) << 16),
+                                        (uint32_t)f2bf_rne(v[2]) | ((uint32_t)f2bf_rne(v[3]) << 16));
+        *(uint2 *)(Xb + row * dpad + j0) = packed;
     }
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
     if (lane == 0) xnorm[row] = (float)(sqrt(s) * (1. + 1e-6));
@@ -102,15 +124,15 @@ void prepare_bf16(Data &ds) {
     const uint32_t d = (uint32_t)ds.d;
     REQUIRE(d <= 512, "BF16 form supports d <= 512");
     ds.dpad = pad_dims(d);
-    const uint32_t rows_per = 512;
+    const uint32_t rows_per = 1024;
     const uint32_t nparts = (uint32_t)cdiv(n, rows_per);
     DBuf part;
     part.reserve((size_t)nparts * d * sizeof(double));
     ds.mu.reserve(d * sizeof(float));
     dim3 g1((uint32_t)cdiv(d, 256), nparts);
     hipLaunchKernelGGL(k_colsum_partial, g1, dim3(256), 0, st, ds.X, n, d, rows_per, part.as<double>());
-    hipLaunchKernelGGL(k_colsum_final, dim3((uint32_t)cdiv(d, 256)), dim3(256), 0, st, part.as<double>(),
-                       nparts, d, n, ds.mu.as<float>());
+    hipLaunchKernelGGL(k_colsum_final, dim3(d), dim3(256), 0, st, part.as<double>(), nparts, d, n,
+                       ds.mu.as<float>());
     ds.Xb.reserve((size_t)n * ds.dpad * sizeof(uint16_t));
     ds.xnorm.reserve(n * sizeof(float));
     hipLaunchKernelGGL(k_make_xb, dim3((uint32_t)cdiv(n, 4)), dim3(256), 0, st, ds.X, n, d, ds.dpad,

@@ -230,6 +230,118 @@ __global__ __launch_bounds__(256) void k_row_costs(const float *__restrict__ X, 
     }
 }
 
+// Cost of one row against one center (column 0 of the center tables), computed by
+// a wave: lanes over the row's stored entries, per-lane FP32 partial sums added by
+// a fixed shuffle tree, then the per-row / per-center terms of k_row_costs.  Every
+// lane returns the value.
+struct Cost1 {
+    const float *X, *val;
+    const uint32_t *idx;
+    const uint64_t *ptr;
+    uint32_t d;
+    float pv, logpv;
+    const float4 *rt;
+    const float *CT;
+    uint32_t kpad;
+    const double *cs, *G;
+    const uint32_t *slot;
+};
+
+template <int MSR>
+__device__ __forceinline__ float row_cost1(const Cost1 &P, uint64_t row, int lane) {
+    uint64_t b, e;
+    if (P.val) { b = P.ptr[row]; e = P.ptr[row + 1]; }
+    else { b = row * P.d; e = b + P.d; }
+    float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (MSR != MCG_SQRL2) r = P.rt[row];
+    const float L = r.x, logL = r.y, Hx = r.z, li = r.w;
+    const float Li = 1.f / L;
+    const float hli = li * __logf(li);
+    float acc = 0.f;
+    for (uint64_t p = b + lane; p < e; p += 64) {
+        const float x = P.val ? P.val[p] : P.X[p];
+        const uint32_t j = P.val ? P.idx[p] : (uint32_t)(p - b);
+        const float ct = P.CT[(uint64_t)j * P.kpad];
+        if (MSR == MCG_SQRL2) acc = fmaf(x, x - 2.f * ct, acc);
+        else if (MSR == MCG_MKL) acc = fmaf(log1pf(x / P.pv), ct, acc);
+        else if (MSR == MCG_REVERSE_MKL) acc = fmaf(x, ct, acc);
+        else {
+            const float xt = (x + P.pv) * Li;
+            const float m1 = xt + ct, m0 = li + ct;
+            acc += 0.5f * ((xt * __logf(xt) - hli) - m1 * __logf(0.5f * m1) + m0 * __logf(0.5f * m0));
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    float v;
+    if (MSR == MCG_SQRL2) {
+        v = (float)(P.cs[0] + (double)acc);
+        v = v < 0.f ? 0.f : v;
+    } else if (MSR == MCG_MKL) {
+        v = post_kl((float)(P.cs[0] - P.cs[1] * (double)(P.logpv - logL) - (double)acc));
+    } else if (MSR == MCG_REVERSE_MKL) {
+        v = post_kl((float)((double)Hx - (double)li * P.cs[0] - (double)acc * (double)Li));
+    } else {
+        v = post_kl((float)(P.G[(uint64_t)P.slot[row] * P.kpad] + (double)acc));
+    }
+    return v;
+}
+
+// every row against one center (k = 1): asn = 0, best = the cost
+template <int MSR>
+__global__ __launch_bounds__(256) void k_row_cost1(Cost1 P, uint64_t n, uint32_t *__restrict__ asn,
+                                                   float *__restrict__ best_out) {
+    const uint64_t row = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const float v = row_cost1<MSR>(P, row, lane);
+    if (lane != 0) return;
+    if (asn) asn[row] = 0;
+    if (best_out) best_out[row] = v;
+}
+
+// D^2 seeding: cost of each earlier center's row cent[a] (a < ci) to the new center
+template <int MSR>
+__global__ __launch_bounds__(256) void k_pp_cc_gen(Cost1 P, const uint32_t *__restrict__ cent, uint32_t ci,
+                                                   float *__restrict__ cc) {
+    const uint32_t a = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (a >= ci) return;
+    const float v = row_cost1<MSR>(P, cent[a], lane);
+    if (lane == 0) cc[a] = v;
+}
+
+// D^2 seeding update, one wave per row (kmeans.h:189-204).  With PRUNE (JSD, whose
+// square root is a metric on the smoothed distributions): a row whose center is
+// far from the new one, JSD(c_a, c_new) > 4 dist (1 + 2^-6) + 2^-17, cannot get
+// closer (triangle inequality for sqrt(JSD); the margins cover the FP32 evaluation
+// of both costs), so its entries are not read and nothing changes.
+template <int MSR, bool PRUNE>
+__global__ __launch_bounds__(256) void k_pp_update_gen(Cost1 P, uint64_t n, const uint64_t *__restrict__ newc_p,
+                                                       uint32_t ci, const float *__restrict__ cc,
+                                                       float *__restrict__ dist, uint32_t *__restrict__ asn,
+                                                       int first, const double *__restrict__ w,
+                                                       unsigned long long *__restrict__ mx) {
+    // waves stride over the rows; one block-wide maximum and one atomic per block
+    const uint64_t nwaves = (uint64_t)gridDim.x * 4;
+    const int lane = threadIdx.x & 63;
+    const uint64_t newc = *newc_p;
+    double a = 0.;
+    for (uint64_t row = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); row < n; row += nwaves) {
+        bool skip = false;
+        float v = 0.f;
+        if (PRUNE && !first && row != newc) {
+            v = dist[row];
+            skip = !(v > 0.f) || cc[asn[row]] > 4.f * v * (1.f + 1.f / 64.f) + 0x1p-17f;
+        }
+        if (!skip) {
+            const float s = row_cost1<MSR>(P, row, lane);
+            if (lane == 0) v = pp_new_dist(row, newc, s, ci, first, dist, asn);
+        }
+        if (lane == 0) a = fmax(a, pp_weighted(v, w, row));
+    }
+    block_max_into(a, mx);
+}
+
 // ---- reference-order per-pair kernel (tiny priors) -----------------------------
 // FP32 (FT=float) following applicator.h:1642-1760 coordinate by coordinate, with
 // pv = max(prior, 7.0069e-42 (lhsum + rhsum)) for each pair.  One wave per row.
@@ -327,9 +439,82 @@ void csr_transforms(Data &ds, double pv, hipStream_t st) {
     for (uint64_t i = 0; i < ds.n; ++i)
         sl[i] = (uint32_t)(std::lower_bound(u.begin(), u.end(), lis[i]) - u.begin());
     ds.li_unique = u;
+    ds.li_dev.reserve(u.size() * sizeof(float));
+    HIP_CHECK(hipMemcpy(ds.li_dev.p, u.data(), u.size() * sizeof(float), hipMemcpyHostToDevice));
     ds.li_slot.reserve(ds.n * 4);
     HIP_CHECK(hipMemcpy(ds.li_slot.p, sl.data(), ds.n * 4, hipMemcpyHostToDevice));
     ds.tf_prior = pv;
+}
+
+// Per-center tables (CT d x kpad, per-center scalars, JSD's G per distinct li) and
+// the per-row terms for k centers C; returns them with the row arrays of ds.
+static Cost1 center_tables(Data &ds, int msr, double pv, const float *C, uint64_t k, hipStream_t st) {
+    Ctx &c = *ds.ctx;
+    const uint32_t d = (uint32_t)ds.d;
+    const uint32_t kpad = (uint32_t)cdiv(k, 64) * 64;
+    Cost1 P{};
+    P.X = ds.csr ? nullptr : ds.X;
+    P.val = ds.csr ? ds.val.as<const float>() : nullptr;
+    P.idx = ds.csr ? ds.idx.as<const uint32_t>() : nullptr;
+    P.ptr = ds.csr ? ds.ptr.as<const uint64_t>() : nullptr;
+    P.d = d;
+    P.pv = (float)pv;
+    P.logpv = (float)log(pv > 0 ? pv : 1.);
+    P.kpad = kpad;
+    float *CT = (float *)c.s[S_CT].reserve((uint64_t)d * kpad * sizeof(float));
+    double *cs = (double *)c.s[S_CPARAM].reserve(2 * (uint64_t)kpad * sizeof(double));
+    hipLaunchKernelGGL(k_center_tables, dim3((uint32_t)k), dim3(256), 0, st, msr, C, (uint32_t)k, kpad, d, pv, CT, cs);
+    P.CT = CT;
+    P.cs = cs;
+    if (msr != MCG_SQRL2) {
+        csr_transforms(ds, pv, st);
+        P.rt = ds.rowinfo.as<const float4>();
+    }
+    if (msr == MCG_JSD) {
+        const uint32_t U = (uint32_t)ds.li_unique.size();
+        REQUIRE(U <= (1u << 20), "JSD: too many distinct row sums");
+        double *Gb = (double *)c.s[S_JSDG].reserve((uint64_t)U * kpad * sizeof(double));
+        hipLaunchKernelGGL(k_jsd_g, dim3((uint32_t)cdiv((uint64_t)U * k, 4)), dim3(256), 0, st, CT, kpad,
+                           (uint32_t)k, d, ds.li_dev.as<const float>(), U, Gb);
+        P.G = Gb;
+        P.slot = ds.li_slot.as<const uint32_t>();
+    }
+    return P;
+}
+
+bool generic_pp_round(Data &ds, int msr, double prior, const float *center_row, const uint32_t *cent, uint32_t ci,
+                      const uint64_t *newc, float *cc, float *dist, uint32_t *asn, int first, const double *w,
+                      unsigned long long *mx, hipStream_t st) {
+    if (msr != MCG_SQRL2 && prior < DIRECT_PRIOR) return false;
+    const double pv = msr == MCG_SQRL2 ? 0. : prior;
+    const Cost1 P = center_tables(ds, msr, pv, center_row, 1, st);
+    const uint64_t n = ds.n;
+    const dim3 grid((uint32_t)std::min<uint64_t>(cdiv(n, 4), 2048)), blk(256);
+    if (msr == MCG_JSD && !first && ci > 0) {
+        hipLaunchKernelGGL(k_pp_cc_gen<MCG_JSD>, dim3((uint32_t)cdiv(ci, 4)), dim3(256), 0, st, P, cent, ci, cc);
+        hipLaunchKernelGGL((k_pp_update_gen<MCG_JSD, true>), grid, blk, 0, st, P, n, newc, ci, (const float *)cc,
+                           dist, asn, first, w, mx);
+        return true;
+    }
+    switch (msr) {
+    case MCG_SQRL2:
+        hipLaunchKernelGGL((k_pp_update_gen<MCG_SQRL2, false>), grid, blk, 0, st, P, n, newc, ci, (const float *)cc,
+                           dist, asn, first, w, mx);
+        break;
+    case MCG_MKL:
+        hipLaunchKernelGGL((k_pp_update_gen<MCG_MKL, false>), grid, blk, 0, st, P, n, newc, ci, (const float *)cc,
+                           dist, asn, first, w, mx);
+        break;
+    case MCG_REVERSE_MKL:
+        hipLaunchKernelGGL((k_pp_update_gen<MCG_REVERSE_MKL, false>), grid, blk, 0, st, P, n, newc, ci,
+                           (const float *)cc, dist, asn, first, w, mx);
+        break;
+    default:
+        hipLaunchKernelGGL((k_pp_update_gen<MCG_JSD, false>), grid, blk, 0, st, P, n, newc, ci, (const float *)cc,
+                           dist, asn, first, w, mx);
+        break;
+    }
+    return true;
 }
 
 void generic_costs(Data &ds, int msr, double prior, const float *C, uint64_t k, float *out_nk, uint32_t *asn,
@@ -352,30 +537,25 @@ void generic_costs(Data &ds, int msr, double prior, const float *C, uint64_t k, 
     }
     const double pv = msr == MCG_SQRL2 ? 0. : prior;
     const uint32_t kpad = (uint32_t)cdiv(k, 64) * 64;
-    float *CT = (float *)c.s[S_CT].reserve((uint64_t)d * kpad * sizeof(float));
-    double *cs = (double *)c.s[S_CPARAM].reserve(2 * (uint64_t)kpad * sizeof(double));
-    hipLaunchKernelGGL(k_center_tables, dim3((uint32_t)k), dim3(256), 0, st, msr, C, (uint32_t)k, kpad, d, pv, CT, cs);
-    const float4 *rt = nullptr;
-    const double *G = nullptr;
-    const uint32_t *slot = nullptr;
-    if (msr != MCG_SQRL2) {
-        csr_transforms(ds, pv, st);
-        rt = ds.rowinfo.as<const float4>();
-    }
-    if (msr == MCG_JSD) {
-        const uint32_t U = (uint32_t)ds.li_unique.size();
-        REQUIRE(U <= (1u << 20), "JSD: too many distinct row sums");
-        float *lis = (float *)c.s[S_LIS].reserve(U * sizeof(float));
-        HIP_CHECK(hipMemcpyAsync(lis, ds.li_unique.data(), U * sizeof(float), hipMemcpyHostToDevice, st));
-        double *Gb = (double *)c.s[S_JSDG].reserve((uint64_t)U * kpad * sizeof(double));
-        hipLaunchKernelGGL(k_jsd_g, dim3((uint32_t)cdiv((uint64_t)U * k, 4)), dim3(256), 0, st, CT, kpad,
-                           (uint32_t)k, d, lis, U, Gb);
-        G = Gb;
-        slot = ds.li_slot.as<const uint32_t>();
-        HIP_CHECK(hipStreamSynchronize(st));  // lis staging
-    }
+    const Cost1 T = center_tables(ds, msr, pv, C, k, st);
+    const float *CT = T.CT;
+    const double *cs = T.cs, *G = T.G;
+    const float4 *rt = T.rt;
+    const uint32_t *slot = T.slot;
     const dim3 grid((uint32_t)cdiv(n, 4)), blk(256);
     const float fpv = (float)pv, lpv = (float)log(pv > 0 ? pv : 1.);
+    if (k == 1 && !out_nk) {
+        const Cost1 P{X, val, idx, ptr, d, fpv, lpv, rt, CT, kpad, cs, G, slot};
+        switch (msr) {
+        case MCG_SQRL2: hipLaunchKernelGGL(k_row_cost1<MCG_SQRL2>, grid, blk, 0, st, P, n, asn, best); break;
+        case MCG_MKL: hipLaunchKernelGGL(k_row_cost1<MCG_MKL>, grid, blk, 0, st, P, n, asn, best); break;
+        case MCG_REVERSE_MKL:
+            hipLaunchKernelGGL(k_row_cost1<MCG_REVERSE_MKL>, grid, blk, 0, st, P, n, asn, best);
+            break;
+        default: hipLaunchKernelGGL(k_row_cost1<MCG_JSD>, grid, blk, 0, st, P, n, asn, best); break;
+        }
+        return;
+    }
     switch (msr) {
     case MCG_SQRL2:
         hipLaunchKernelGGL(k_row_costs<MCG_SQRL2>, grid, blk, 0, st, X, val, idx, ptr, n, d, fpv, lpv, rt, CT, kpad,

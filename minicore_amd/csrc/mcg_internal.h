@@ -83,6 +83,7 @@ struct Data {
     DBuf tf_logx;      // log1p(x / pv) per stored value (MKL)
     DBuf rowinfo;      // per-row float4 {L, log L, Hx, li}
     std::vector<float> li_unique;  // distinct li values (JSD)
+    DBuf li_dev;       // li_unique on the device
     DBuf li_slot;      // per-row index into li_unique
     // column-major copy of the CSR rows (soft M-step), built on first use
     DBuf csc_ptr, csc_row, csc_val;
@@ -186,6 +187,54 @@ void rowsums(Data &ds, hipStream_t st);
 void csr_transforms(Data &ds, double prior, hipStream_t st);
 
 inline uint64_t cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// ---- D^2 seeding helpers shared by the seeding kernels -------------------------
+// Block max of |w_p * dist_p| into *mx (bit patterns of non-negative doubles order
+// like their values): the next draw's fixed-point scale, computed in the pass that
+// writes the distances.  Every thread of the block calls it.
+__device__ __forceinline__ void block_max_into(double a, unsigned long long *mx) {
+    __shared__ double sm[256];
+    sm[threadIdx.x] = a;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(mx, (unsigned long long)__double_as_longlong(sm[0]));
+}
+
+// new distance of point p given its cost s to the new center (kmeans.h:189-204)
+__device__ __forceinline__ float pp_new_dist(uint64_t p, uint64_t newc, float s, uint32_t ci, int first,
+                                             float *__restrict__ dist, uint32_t *__restrict__ asn) {
+    if (first) {
+        const float v = (p == newc) ? 0.f : s;
+        dist[p] = v;
+        asn[p] = 0;
+        return v;
+    }
+    if (p == newc) { dist[p] = 0.f; asn[p] = ci; return 0.f; }
+    const float dv = dist[p];
+    if (dv <= 0.f) return dv;
+    if (s < dv) { dist[p] = s; asn[p] = ci; return s; }
+    return dv;
+}
+
+__device__ __forceinline__ double pp_weighted(float v, const double *__restrict__ w, uint64_t p) {
+    double a = (double)v;
+    if (w) a = w[p] * a;
+    return fabs(a);
+}
+
+// One round of D^2 seeding for the measures without the dense-SQRL2 kernel: the
+// cost of every row to data row *newc (densified into center_row), folded into
+// dist/asn with the weighted maximum into mx.  For JSD (a metric after the square
+// root) rounds after the first skip rows whose center is far from the new one,
+// using the center-to-center costs of the rows cent[0..ci) (scratch cc, k floats).
+// Returns false for priors that need the per-pair kernel (the caller then takes
+// the general cost path).
+bool generic_pp_round(Data &ds, int msr, double prior, const float *center_row, const uint32_t *cent, uint32_t ci,
+                      const uint64_t *newc, float *cc, float *dist, uint32_t *asn, int first, const double *w,
+                      unsigned long long *mx, hipStream_t st);
 
 }  // namespace mcg
 

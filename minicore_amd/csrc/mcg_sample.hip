@@ -115,63 +115,76 @@ __global__ void k_fixed_blocksum(const T *__restrict__ v, const double *__restri
     if (threadIdx.x == 0) bsum[blockIdx.x] = sm[0];
 }
 
-// Single block: choose the index for raw draw r; writes *out.  If the fixed total
-// is 0 the result is 0.
+// Inclusive scan of one u64 per thread over a 1024-thread block (Hillis-Steele in
+// LDS); integer sums, so the association does not matter.
+__device__ __forceinline__ unsigned long long block_scan_u64(unsigned long long x, unsigned long long *sm) {
+    sm[threadIdx.x] = x;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const unsigned long long t = (int)threadIdx.x >= o ? sm[threadIdx.x - o] : 0ull;
+        __syncthreads();
+        sm[threadIdx.x] += t;
+        __syncthreads();
+    }
+    return sm[threadIdx.x];
+}
+
+// Single block of 1024 threads: choose the index for raw draw r; writes *out.  The
+// first index whose fixed-point prefix exceeds u = mulhi(r, total): the block is
+// found from a scan of the block sums, then the element from a scan inside the
+// block (FX_BLK == 1024, one element per thread).  If the total is 0 the result is 0.
 template <class T>
-__global__ void k_fixed_select(const T *__restrict__ v, const double *__restrict__ w, uint64_t n,
-                               const unsigned long long *__restrict__ maxbits, const uint64_t *__restrict__ bsum,
-                               uint32_t nblk, uint64_t r, uint64_t *__restrict__ out) {
-    __shared__ unsigned long long part[1024];
-    __shared__ uint64_t sel_block, sel_u;
+__global__ __launch_bounds__(1024) void k_fixed_select(const T *__restrict__ v, const double *__restrict__ w,
+                                                       uint64_t n, const unsigned long long *__restrict__ maxbits,
+                                                       const uint64_t *__restrict__ bsum, uint32_t nblk, uint64_t r,
+                                                       uint64_t *__restrict__ out) {
+    static_assert(FX_BLK == 1024, "one element per thread in the block scan");
+    __shared__ unsigned long long sm[1024];
+    __shared__ unsigned long long sel_u;
+    __shared__ uint32_t sel_t, sel_b;
     const int s = fixed_shift(__longlong_as_double((long long)*maxbits), n);
-    // total and block prefix: thread t owns blocks [t*per, ...)
+    const uint32_t t = threadIdx.x;
+    // thread t owns block sums [t*per, (t+1)*per)
     const uint32_t per = (nblk + 1023) / 1024;
-    const uint32_t b0 = threadIdx.x * per, b1 = min(nblk, b0 + per);
+    const uint32_t b0 = t * per, b1 = min(nblk, b0 + per);
     unsigned long long acc = 0;
     for (uint32_t b = b0; b < b1; ++b) acc += bsum[b];
-    part[threadIdx.x] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long tot = 0;
-        for (int t = 0; t < 1024; ++t) tot += part[t];
-        if (tot == 0) { sel_block = 0xFFFFFFFFFFFFFFFFull; }
-        else {
-            const uint64_t u = mulhi64(r, tot);
-            unsigned long long run = 0;
-            uint64_t blk = 0;
-            for (int t = 0; t < 1024; ++t) {
-                if (run + part[t] > u) {
-                    const uint32_t bb = t * per, be = min(nblk, bb + per);
-                    for (uint32_t b = bb; b < be; ++b) {
-                        if (run + bsum[b] > u) { blk = b; break; }
-                        run += bsum[b];
-                    }
-                    break;
-                }
-                run += part[t];
-            }
-            sel_block = blk;
-            sel_u = u - run;
-        }
-    }
-    __syncthreads();
-    if (sel_block == 0xFFFFFFFFFFFFFFFFull) {
-        if (threadIdx.x == 0) *out = 0;
+    const unsigned long long incl = block_scan_u64(acc, sm);
+    const unsigned long long tot = sm[1023];
+    if (tot == 0) {
+        if (t == 0) *out = 0;
         return;
     }
-    // inside the block: sequential scan by thread 0 over at most FX_BLK elements
-    if (threadIdx.x == 0) {
-        const uint64_t i0 = sel_block * FX_BLK;
-        unsigned long long run = 0;
-        uint64_t ans = min<uint64_t>(n - 1, i0 + FX_BLK - 1);
-        for (uint64_t i = i0; i < min<uint64_t>(n, i0 + FX_BLK); ++i) {
-            double a = (double)v[i];
-            if (w) a = w[i] * a;
-            run += to_fixed(a, s);
-            if (run > sel_u) { ans = i; break; }
-        }
-        *out = ans;
+    const uint64_t u = mulhi64(r, tot);
+    if (t == 0) {
+        sel_t = 0xFFFFFFFFu;
+        *out = min<uint64_t>(n - 1, (uint64_t)(nblk - 1) * FX_BLK + FX_BLK - 1);   // replaced below
     }
+    __syncthreads();
+    // the unique thread whose range holds u: incl - acc <= u < incl
+    if (incl > u && incl - acc <= u) {
+        unsigned long long run = incl - acc;
+        uint32_t blk = b1 - 1;
+        for (uint32_t b = b0; b < b1; ++b) {
+            if (run + bsum[b] > u) { blk = b; break; }
+            run += bsum[b];
+        }
+        sel_b = blk;
+        sel_u = u - run;
+        sel_t = t;
+    }
+    __syncthreads();
+    // inside the selected block: one element per thread
+    const uint64_t i = (uint64_t)sel_b * FX_BLK + t;
+    unsigned long long f = 0;
+    if (i < n) {
+        double a = (double)v[i];
+        if (w) a = w[i] * a;
+        f = to_fixed(a, s);
+    }
+    const unsigned long long pre = block_scan_u64(f, sm);
+    if (pre > sel_u && pre - f <= sel_u && f > 0) *out = i;
+    // u < total, so some element of the selected block holds it
 }
 
 // Host-side draw from device values (used for the rare restart path).
@@ -206,7 +219,8 @@ void restart_draws(Ctx &c, const float *costs_d, uint64_t n, uint64_t nempty, ui
 __global__ __launch_bounds__(256) void k_pp_update_l2(const float *__restrict__ X, uint64_t n, uint32_t d,
                                                       const uint64_t *__restrict__ newc_p, uint32_t ci,
                                                       float *__restrict__ dist, uint32_t *__restrict__ asn,
-                                                      int first) {
+                                                      int first, const double *__restrict__ w,
+                                                      unsigned long long *__restrict__ mx) {
     __shared__ float xs[256][33];
     __shared__ float cs[32];
     const int tid = threadIdx.x;
@@ -226,30 +240,21 @@ __global__ __launch_bounds__(256) void k_pp_update_l2(const float *__restrict__ 
         const int jn = (int)min<uint32_t>(32, d - j0);
         for (int jj = 0; jj < jn; ++jj) { const float t = xs[tid][jj] - cs[jj]; s = fmaf(t, t, s); }
     }
-    if (p >= n) return;
-    if (first) {
-        dist[p] = (p == newc) ? 0.f : s;
-        asn[p] = 0;
-        return;
-    }
-    if (p == newc) { dist[p] = 0.f; asn[p] = ci; return; }
-    const float dv = dist[p];
-    if (dv <= 0.f) return;
-    if (s < dv) { dist[p] = s; asn[p] = ci; }
+    double a = 0.;
+    if (p < n) a = pp_weighted(pp_new_dist(p, newc, s, ci, first, dist, asn), w, p);
+    block_max_into(a, mx);
 }
 
 // dist update from precomputed costs to the new center (generic measures)
-__global__ void k_pp_update_from(const float *__restrict__ cost, uint64_t n, const uint64_t *__restrict__ newc_p,
-                                 uint32_t ci, float *__restrict__ dist, uint32_t *__restrict__ asn, int first) {
+__global__ __launch_bounds__(256) void k_pp_update_from(const float *__restrict__ cost, uint64_t n,
+                                                        const uint64_t *__restrict__ newc_p, uint32_t ci,
+                                                        float *__restrict__ dist, uint32_t *__restrict__ asn,
+                                                        int first, const double *__restrict__ w,
+                                                        unsigned long long *__restrict__ mx) {
     const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const uint64_t newc = *newc_p;
-    const float s = cost[p];
-    if (first) { dist[p] = (p == newc) ? 0.f : s; asn[p] = 0; return; }
-    if (p == newc) { dist[p] = 0.f; asn[p] = ci; return; }
-    const float dv = dist[p];
-    if (dv <= 0.f) return;
-    if (s < dv) { dist[p] = s; asn[p] = ci; }
+    double a = 0.;
+    if (p < n) a = pp_weighted(pp_new_dist(p, *newc_p, cost[p], ci, first, dist, asn), w, p);
+    block_max_into(a, mx);
 }
 
 // if dist[newc] == 0, take the first index of the maximum distance (kmeans.h:179-186)
@@ -281,6 +286,143 @@ __global__ void k_pp_fallback(const float *__restrict__ dist, uint64_t n, uint64
         __syncthreads();
     }
     if (threadIdx.x == 0) { *newc_p = bi[0]; centers[ci] = (uint32_t)bi[0]; }
+}
+
+// One seeding round's choice (kmeans.h:115-186), one block of 1024 threads: the
+// fixed-point draw r over w * dist (as k_fixed_select); if the drawn point has
+// distance 0, the first index of the maximum distance instead; writes *newc_p and
+// centers[ci], then clears *mx for the next update's maximum.
+__global__ __launch_bounds__(1024) void k_pp_select(const float *__restrict__ dist, const double *__restrict__ w,
+                                                    uint64_t n, unsigned long long *__restrict__ mx,
+                                                    const uint64_t *__restrict__ bsum, uint32_t nblk, uint64_t r,
+                                                    uint64_t *__restrict__ newc_p, uint32_t *__restrict__ centers,
+                                                    uint32_t ci) {
+    __shared__ unsigned long long sm[1024];
+    __shared__ unsigned long long sel_u;
+    __shared__ uint32_t sel_b;
+    __shared__ uint64_t chosen;
+    __shared__ float bv[1024];
+    __shared__ uint64_t bi[1024];
+    const int s = fixed_shift(__longlong_as_double((long long)*mx), n);
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (nblk + 1023) / 1024;
+    const uint32_t b0 = t * per, b1 = min(nblk, b0 + per);
+    unsigned long long acc = 0;
+    for (uint32_t b = b0; b < b1; ++b) acc += bsum[b];
+    const unsigned long long incl = block_scan_u64(acc, sm);
+    const unsigned long long tot = sm[1023];
+    if (t == 0) chosen = 0;
+    __syncthreads();
+    if (tot != 0) {
+        const uint64_t u = mulhi64(r, tot);
+        if (incl > u && incl - acc <= u) {
+            unsigned long long run = incl - acc;
+            uint32_t blk = b1 - 1;
+            for (uint32_t b = b0; b < b1; ++b) {
+                if (run + bsum[b] > u) { blk = b; break; }
+                run += bsum[b];
+            }
+            sel_b = blk;
+            sel_u = u - run;
+        }
+        __syncthreads();
+        const uint64_t i = (uint64_t)sel_b * FX_BLK + t;
+        unsigned long long f = 0;
+        if (i < n) {
+            double a = (double)dist[i];
+            if (w) a = w[i] * a;
+            f = to_fixed(a, s);
+        }
+        const unsigned long long pre = block_scan_u64(f, sm);
+        if (pre > sel_u && pre - f <= sel_u) chosen = i;
+        __syncthreads();
+    }
+    // distance 0 (or all distances 0): first index of the maximum distance
+    if (dist[chosen] == 0.f) {
+        float m = -1.f;
+        uint64_t mi = 0;
+        for (uint64_t i = t; i < n; i += 1024)
+            if (dist[i] > m) { m = dist[i]; mi = i; }
+        bv[t] = m;
+        bi[t] = mi;
+        __syncthreads();
+        for (int o = 512; o > 0; o >>= 1) {
+            if ((int)t < o) {
+                const float ov = bv[t + o];
+                const uint64_t oi = bi[t + o];
+                if (ov > bv[t] || (ov == bv[t] && oi < bi[t])) { bv[t] = ov; bi[t] = oi; }
+            }
+            __syncthreads();
+        }
+        if (t == 0) chosen = bi[0];
+        __syncthreads();
+    }
+    if (t == 0) {
+        *newc_p = chosen;
+        centers[ci] = (uint32_t)chosen;
+        *mx = 0ull;
+    }
+}
+
+// Squared distance (FP64) between the new center's row and each earlier center's
+// row: one wave per earlier center a < ci.
+__global__ __launch_bounds__(256) void k_pp_cc(const float *__restrict__ X, uint32_t d,
+                                               const uint32_t *__restrict__ centers, uint32_t ci,
+                                               const uint64_t *__restrict__ newc_p, double *__restrict__ cc) {
+    const uint32_t a = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (a >= ci) return;
+    const float *x = X + (uint64_t)centers[a] * d, *y = X + *newc_p * d;
+    double s = 0.;
+    for (uint32_t j = lane; j < d; j += 64) {
+        const double t = (double)x[j] - (double)y[j];
+        s += t * t;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) cc[a] = s;
+}
+
+// Dense SQRL2 update after the first round, one thread per point.  A point whose
+// current center a is far from the new one, ||c_a - c_new||^2 > 4 dist (1 + 2^-8),
+// cannot get closer (triangle inequality; the margin covers the FP32 rounding of
+// dist and of the new cost), so its row is not read and nothing changes.  Other
+// points get the exact sequential FMA chain (16-byte loads when vec).
+__global__ __launch_bounds__(256) void k_pp_update_l2_pruned(const float *__restrict__ X, uint64_t n, uint32_t d,
+                                                             int vec, const uint64_t *__restrict__ newc_p,
+                                                             uint32_t ci, const double *__restrict__ cc,
+                                                             float *__restrict__ dist, uint32_t *__restrict__ asn,
+                                                             const double *__restrict__ w,
+                                                             unsigned long long *__restrict__ mx) {
+    const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    double a = 0.;
+    if (p < n) {
+        const uint64_t newc = *newc_p;
+        float v = dist[p];
+        if (p == newc) {
+            v = 0.f;
+            dist[p] = 0.f;
+            asn[p] = ci;
+        } else if (v > 0.f && !(cc[asn[p]] > 4.0 * (double)v * (1.0 + 1.0 / 256.0))) {
+            const float *x = X + p * d, *c = X + newc * d;
+            float s = 0.f;
+            if (vec) {
+                const float4 *x4 = (const float4 *)x, *c4 = (const float4 *)c;
+                for (uint32_t j = 0; j < (d >> 2); ++j) {
+                    const float4 xa = x4[j], cb = c4[j];
+                    float t;
+                    t = xa.x - cb.x; s = fmaf(t, t, s);
+                    t = xa.y - cb.y; s = fmaf(t, t, s);
+                    t = xa.z - cb.z; s = fmaf(t, t, s);
+                    t = xa.w - cb.w; s = fmaf(t, t, s);
+                }
+            } else {
+                for (uint32_t j = 0; j < d; ++j) { const float t = x[j] - c[j]; s = fmaf(t, t, s); }
+            }
+            if (s < v) { v = s; dist[p] = s; asn[p] = ci; }
+        }
+        a = pp_weighted(v, w, p);
+    }
+    block_max_into(a, mx);
 }
 
 __global__ void k_set_u64(uint64_t *p, uint64_t v) { *p = v; }
@@ -318,38 +460,51 @@ void kmeanspp_run(Data &ds, int msr, double prior, uint64_t k, uint64_t seed, co
         cost1 = (float *)c.s[S_STAGE1].reserve(n * sizeof(float));
         asn1 = (uint32_t *)c.s[S_FLAGS].reserve(n * sizeof(uint32_t));
     }
+    // update() writes the distances to the new center and their weighted maximum
+    // (into mx, which must be clear): the fixed-point scale of the next draw
+    // center-to-center costs of the pruned rounds (dense SQRL2: FP64; JSD: FP32)
+    void *ccbuf = c.s[S_STAGE2].reserve(k * sizeof(double));
     auto update = [&](uint32_t ci, int first) {
         if (l2) {
             hipLaunchKernelGGL(k_pp_update_l2, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, ds.X, n, d, newc, ci,
-                               dist_d, asn_d, first);
+                               dist_d, asn_d, first, w, mx);
         } else {
             hipLaunchKernelGGL(k_row_to_dense, dim3(1), dim3(256), 0, st, ds.X,
                                ds.csr ? ds.val.as<const float>() : nullptr,
                                ds.csr ? ds.idx.as<const uint32_t>() : nullptr,
                                ds.csr ? ds.ptr.as<const uint64_t>() : nullptr, d, newc, row);
+            if (generic_pp_round(ds, msr, prior, row, cent, ci, newc, (float *)ccbuf, dist_d, asn_d, first, w, mx,
+                                 st))
+                return;
             generic_costs(ds, msr, prior, row, 1, nullptr, asn1, cost1, st);
             hipLaunchKernelGGL(k_pp_update_from, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, cost1, n, newc, ci,
-                               dist_d, asn_d, first);
+                               dist_d, asn_d, first, w, mx);
         }
     };
     uint64_t j = 0;
     const uint64_t fc = wyrand_at(seed, j++) % n;
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, newc, fc);
+    fill_u32((uint32_t *)mx, 0, 2, st);   // later rounds: k_pp_select clears it
     update(0, 1);
     centers_h[0] = (uint32_t)fc;
     fill_u32(cent, (uint32_t)fc, 1, st);
+    // dense SQRL2 rounds after the first prune by the center-center distances
+    const int vec = (d % 4 == 0) && ((uintptr_t)ds.X % 16 == 0);
+    double *cc = (double *)ccbuf;
     for (uint64_t ci = 1; ci < k; ++ci) {
         const uint64_t r = wyrand_at(seed, j++);
-        fill_u32((uint32_t *)mx, 0, 2, st);
-        hipLaunchKernelGGL(k_absmax<float>, dim3((uint32_t)std::min<uint64_t>(cdiv(n, 256), 1024)), dim3(256), 0, st,
-                           (const float *)dist_d, w, n, mx);
         hipLaunchKernelGGL(k_fixed_blocksum<float>, dim3(nblk), dim3(256), 0, st, (const float *)dist_d, w, n,
                            (const unsigned long long *)mx, bsum);
-        hipLaunchKernelGGL(k_fixed_select<float>, dim3(1), dim3(1024), 0, st, (const float *)dist_d, w, n,
-                           (const unsigned long long *)mx, (const uint64_t *)bsum, nblk, r, newc);
-        hipLaunchKernelGGL(k_pp_fallback, dim3(1), dim3(1024), 0, st, (const float *)dist_d, n, newc, cent,
-                           (uint32_t)ci);
-        update((uint32_t)ci, 0);
+        hipLaunchKernelGGL(k_pp_select, dim3(1), dim3(1024), 0, st, (const float *)dist_d, w, n, mx,
+                           (const uint64_t *)bsum, nblk, r, newc, cent, (uint32_t)ci);
+        if (l2) {
+            hipLaunchKernelGGL(k_pp_cc, dim3((uint32_t)cdiv(ci, 4)), dim3(256), 0, st, ds.X, d,
+                               (const uint32_t *)cent, (uint32_t)ci, newc, cc);
+            hipLaunchKernelGGL(k_pp_update_l2_pruned, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, ds.X, n, d,
+                               vec, newc, (uint32_t)ci, cc, dist_d, asn_d, w, mx);
+        } else {
+            update((uint32_t)ci, 0);
+        }
     }
     HIP_CHECK(hipMemcpyAsync(centers_h, cent, k * 4, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));

@@ -210,6 +210,14 @@ static bool run_request(Req &r, mcg_ctx *ctx) {
     hipStream_t st = c.stream;
     HIP_CHECK(hipSetDevice(c.device));
     auto cancelled = [&] { return r.cancel.load(std::memory_order_relaxed); };
+    // stage boundaries: wall time since the previous boundary (the stages end with a
+    // stream synchronisation, so this is the stage's device time plus its host work)
+    Clock::time_point tb = Clock::now();
+    auto stage_done = [&](int i) {
+        const Clock::time_point t = Clock::now();
+        r.res.stage_ms[i] = std::chrono::duration<double, std::milli>(t - tb).count();
+        tb = t;
+    };
 
     // 1. rows + seeding
     mcg_data *ds_raw = nullptr;
@@ -233,6 +241,7 @@ static bool run_request(Req &r, mcg_ctx *ctx) {
     std::vector<uint32_t> ids(k);
     maybe_fail(r, 1);
     check(mcg_kmeanspp(ds.get(), q.msr, q.prior, k, q.seed, W, ids.data(), A.as<uint32_t>(), Dist.as<float>()));
+    stage_done(0);
     if (cancelled()) return false;
 
     // 2. sensitivities
@@ -242,6 +251,7 @@ static bool run_request(Req &r, mcg_ctx *ctx) {
                        Dist.as<const float>(), n, Costd.as<double>());
     Probs.reserve(n * 4);
     check(mcg_coreset_probs(ctx, q.sens, n, k, Costd.as<const double>(), A.as<const uint32_t>(), W, Probs.as<float>()));
+    stage_done(1);
     if (cancelled()) return false;
 
     // 3. coreset draws (from the seed's stream after the first 2^32 draws, which
@@ -258,6 +268,7 @@ static bool run_request(Req &r, mcg_ctx *ctx) {
     mcg_data *sub_raw = nullptr;
     check(mcg_data_rows(ds.get(), Idx.as<const uint64_t>(), m, &sub_raw));
     std::unique_ptr<mcg_data, int (*)(mcg_data *)> sub(sub_raw, mcg_data_free);
+    stage_done(2);
     if (cancelled()) return false;
 
     // 4. weighted Lloyd on the coreset from the seeded rows
@@ -275,6 +286,7 @@ static bool run_request(Req &r, mcg_ctx *ctx) {
     uint64_t iters = 0;
     check(mcg_lloyd(sub.get(), q.msr, q.prior, Cent.as<float>(), k, Csw.as<const double>(), q.eps, q.maxiter,
                     q.precision, Asub.as<uint32_t>(), nullptr, &ci, &cf, &iters));
+    stage_done(3);
     if (cancelled()) return false;
 
     // 5. all rows to the final centers
@@ -299,6 +311,7 @@ static bool run_request(Req &r, mcg_ctx *ctx) {
     put(q.asn, Afin.p, n * 4);
     put(q.costs, Cfin.p, n * 4);
     HIP_CHECK(hipStreamSynchronize(st));
+    stage_done(4);
     r.res.cost = total;
     r.res.coreset_initcost = ci;
     r.res.coreset_finalcost = cf;

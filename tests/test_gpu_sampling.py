@@ -13,15 +13,45 @@ def say(*a):
 
 
 @pytest.mark.parametrize("weighted", [False, True])
-def test_kmeanspp_dense_bitexact(mcg, ctx, mco, weighted):
-    say(f"kmeans++ dense weighted={weighted}")
-    X, _, _ = gaussian_mixture(20000, 50, 25, 0)
-    w = np.random.default_rng(3).uniform(0.5, 2, 20000) if weighted else None
-    c_ref, a_ref, d_ref = mco.kmeanspp(mco.SQRL2, X, 25, seed=13, w=w)
-    c, a, dd = mcg.Dataset.dense(X, ctx).kmeanspp(mcg.SQRL2, 25, seed=13, weights=w)
+@pytest.mark.parametrize("n,d,k", [(20000, 50, 25), (30000, 64, 200)])
+def test_kmeanspp_dense_bitexact(mcg, ctx, mco, weighted, n, d, k):
+    """Rounds after the first skip points whose center is far from the new one; the
+    result must still equal the oracle, which evaluates every point."""
+    say(f"kmeans++ dense n={n} d={d} k={k} weighted={weighted}")
+    X, _, _ = gaussian_mixture(n, d, 25, 0)
+    w = np.random.default_rng(3).uniform(0.5, 2, n) if weighted else None
+    c_ref, a_ref, d_ref = mco.kmeanspp(mco.SQRL2, X, k, seed=13, w=w)
+    c, a, dd = mcg.Dataset.dense(X, ctx).kmeanspp(mcg.SQRL2, k, seed=13, weights=w)
     assert np.array_equal(c, c_ref)
     assert np.array_equal(a, a_ref)
     assert np.array_equal(dd, d_ref)
+
+
+def test_kmeanspp_duplicate_rows_fallback(mcg, ctx, mco):
+    """5 distinct rows repeated: once all are chosen every distance is 0 and each
+    later round takes the first index of the maximum (kmeans.h:179-186)."""
+    say("kmeans++ duplicate rows")
+    base = np.random.default_rng(4).normal(size=(5, 16)).astype(np.float32)
+    X = base[np.random.default_rng(5).integers(0, 5, 3000)]
+    c_ref, a_ref, d_ref = mco.kmeanspp(mco.SQRL2, X, 9, seed=2)
+    c, a, dd = mcg.Dataset.dense(X, ctx).kmeanspp(mcg.SQRL2, 9, seed=2)
+    assert np.array_equal(c, c_ref)
+    assert np.array_equal(a, a_ref)
+    assert np.array_equal(dd, d_ref)
+
+
+def test_kmeanspp_jsd_pruned_rounds(mcg, ctx, mco):
+    """JSD seeding skips rows whose center is far from the new one (sqrt(JSD) is a
+    metric); every distance must still be the minimum over the chosen centers."""
+    say("kmeans++ csr jsd k=60")
+    n, d, k = 6000, 500, 60
+    data, idx, ptr = poisson_csr(n, d, 0.05, 9, normalize=True)
+    X = mco.CSR(data, idx, ptr, (n, d))
+    c, a, dist = mcg.Dataset.csr(data, idx, ptr, (n, d), ctx).kmeanspp(4, k, prior=1. / d, seed=11)
+    assert len(set(c.tolist())) == k
+    ref = mco.cost_matrix(4, X, X.todense()[c], 1. / d)
+    np.testing.assert_allclose(dist, ref.min(1), rtol=2e-4, atol=2e-6)
+    np.testing.assert_allclose(ref[np.arange(n), a], dist, rtol=2e-4, atol=2e-6)
 
 
 @pytest.mark.parametrize("code", [4, 5])

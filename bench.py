@@ -118,6 +118,119 @@ def serving(mcg, reqs, prec, rate, seed, workers):
     return np.array(lat), wall, np.array(stages)
 
 
+def fp64_check(ds, mcg, X, C0, iters, sample=16384):
+    """Error of the BF16-refined form against FP64, after `iters` iterations from C0:
+    the assignment of a row sample (BF16 E-step against the final centers) against
+    its FP64 argmin, and the sample's total cost (FP64 over the assignment) against
+    the FP64-optimal total.  Differing points are reported with their FP64 top-2 gap."""
+    import numpy as np
+    import torch
+    C = C0.clone()
+    a = torch.empty(X.shape[0], dtype=torch.int32, device="cuda")
+    for _ in range(iters):
+        ds.lloyd_step(mcg.SQRL2, C, a, precision=mcg.PREC_BF16)
+    torch.cuda.synchronize()
+    Cf = C.cpu().numpy()
+    asn, _ = ds.assign(mcg.SQRL2, Cf, precision=mcg.PREC_BF16)
+    rows = np.random.default_rng(0).choice(X.shape[0], sample, replace=False)
+    Xs = X[torch.as_tensor(rows, device="cuda")].cpu().numpy().astype(np.float64)
+    C64 = Cf.astype(np.float64)
+    # exact FP64 distances, chunked
+    best = np.empty(sample)
+    second = np.empty(sample)
+    arg = np.empty(sample, dtype=np.int64)
+    mine = np.empty(sample)
+    cn = (C64 * C64).sum(1)
+    for i in range(0, sample, 2048):
+        xs = Xs[i:i + 2048]
+        dd = (xs * xs).sum(1)[:, None] + cn[None, :] - 2 * xs @ C64.T
+        # refine the two smallest with direct differences (no cancellation)
+        top = np.argpartition(dd, 2, axis=1)[:, :3]
+        ex = ((xs[:, None, :] - C64[top]) ** 2).sum(-1)
+        o = np.argsort(ex, axis=1, kind="stable")
+        b_idx = np.take_along_axis(top, o, 1)
+        exs = np.take_along_axis(ex, o, 1)
+        best[i:i + 2048], second[i:i + 2048], arg[i:i + 2048] = exs[:, 0], exs[:, 1], b_idx[:, 0]
+        a_s = asn[rows[i:i + 2048]].astype(np.int64)
+        mine[i:i + 2048] = ((xs - C64[a_s]) ** 2).sum(-1)
+    diff = arg != asn[rows]
+    gap = (second - best) / np.maximum(best, 1e-300)
+    return {"sample_rows": sample, "iterations": iters,
+            "cost_rel_err_vs_fp64": float(abs(mine.sum() - best.sum()) / best.sum()),
+            "assignments_differing": int(diff.sum()),
+            "differing_with_gap_ge_1e-6": int((diff & (gap >= 1e-6)).sum())}
+
+
+def sparse_config(args, mcg, cfg):
+    """Auxiliary measurements of the sparse workloads (not the headline line):
+    c2: n=1e5, d=2e4, ~400 stored entries per row, rows normalised, prior 1/d,
+        MKL and JSD, k=50, D^2 seeding then 20 hard Lloyd iterations;
+    c4: n=1e6, d=32768 at 1% density, prior 1e-3, k=256, 10 soft EM iterations (T=1)."""
+    import numpy as np
+    from minicore_amd.datagen import poisson_csr_fast, counts_csr_gaps
+    ctx = mcg.Context(0)
+    out = {"config": cfg}
+    if cfg == "c2":
+        n, d, k = 100_000, 20_000, 50
+        log(f"c2: generating n={n} d={d}")
+        data, idx, ptr = poisson_csr_fast(n, d, 0.02, 1, normalize=True)
+        ds = mcg.Dataset.csr(data, idx, ptr, (n, d), ctx)
+        nnz = len(data)
+        alg = nnz * 8 + (n + 1) * 8 + k * d * 4 + n * 8
+        for name, code in (("MKL", mcg.MKL), ("JSD", mcg.JSD)):
+            t0 = time.perf_counter()
+            ids, _, _ = ds.kmeanspp(code, k, prior=1. / d, seed=1)
+            t_seed = time.perf_counter() - t0
+            C0 = np.zeros((k, d), dtype=np.float32)
+            for c, r in enumerate(ids):
+                C0[c, idx[int(ptr[r]):int(ptr[r + 1])]] = data[int(ptr[r]):int(ptr[r + 1])]
+            ds.assign(code, C0, prior=1. / d)   # warm-up (tables, row terms)
+            reps = 10
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                ds.assign(code, C0, prior=1. / d, with_costs=False)
+            t_e = (time.perf_counter() - t0) / reps
+            t0 = time.perf_counter()
+            res = ds.lloyd(code, C0, prior=1. / d, eps=-1., maxiter=19)
+            t_l = time.perf_counter() - t0
+            out[name] = {"seeding_s": t_seed, "estep_ms": 1e3 * t_e, "estep_GBs": alg / t_e / 1e9,
+                         "estep_frac_hbm": alg / t_e / 1e9 / PEAK_HBM_GBS, "lloyd_iters": res["numiter"] + 1,
+                         "lloyd_iters_per_s": (res["numiter"] + 1) / t_l}
+            log(json.dumps({name: out[name]}))
+        out.update({"n": n, "d": d, "k": k, "nnz": nnz, "estep_algorithmic_bytes": alg})
+    else:
+        n, d, k = 1_000_000, 32768, 256
+        log(f"c4: generating n={n} d={d} at 1%")
+        parts = [counts_csr_gaps(100_000, d, 0.01, 3 + i) for i in range(n // 100_000)]
+        data = np.concatenate([p[0] for p in parts])
+        idx = np.concatenate([p[1] for p in parts])
+        ptr = np.zeros(n + 1, dtype=np.uint64)
+        off = 0
+        for i, p in enumerate(parts):
+            ptr[i * 100_000 + 1:(i + 1) * 100_000 + 1] = p[2][1:] + off
+            off += int(p[2][-1])
+        del parts
+        ds = mcg.Dataset.csr(data, idx, ptr, (n, d), ctx)
+        nnz = len(data)
+        rows = np.random.default_rng(4).choice(n, k, replace=False)
+        C0 = np.zeros((k, d), dtype=np.float32)
+        for c, r in enumerate(rows):
+            C0[c, idx[int(ptr[r]):int(ptr[r + 1])]] = data[int(ptr[r]):int(ptr[r + 1])]
+        alg = nnz * 8 + n * k * 4 + k * d * 4
+        ds.cost_matrix(mcg.REVERSE_MKL, C0[:8], prior=1e-3)   # warm-up (row terms)
+        t0 = time.perf_counter()
+        ds.cost_matrix(mcg.REVERSE_MKL, C0, prior=1e-3)
+        t_e = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        res = ds.soft(mcg.REVERSE_MKL, C0, prior=1e-3, temp=1., eps=-1., maxiter=10)
+        t_s = time.perf_counter() - t0
+        out.update({"n": n, "d": d, "k": k, "nnz": nnz, "cost_matrix_ms": 1e3 * t_e,
+                    "cost_matrix_GBs": alg / t_e / 1e9, "estep_algorithmic_bytes": alg,
+                    "soft_em_iters": res["numiter"], "soft_em_s_per_iter": t_s / max(res["numiter"], 1),
+                    "note": "cost matrix timed with its n x k float32 copy back to the host"})
+    return out
+
+
 def cpu_baseline(X_host, C_host, n_full, nthreads):
     from oracle import mco
     ns = X_host.shape[0]
@@ -147,6 +260,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--config", choices=["c3", "c2", "c4"], default="c3",
+                    help="c3: the headline dense Lloyd workload; c2/c4: auxiliary sparse measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path would shard rows "
@@ -157,6 +272,10 @@ def main():
     import torch
     from minicore_amd import _lib as mcg
     from minicore_amd.datagen import gaussian_mixture_device
+
+    if args.config != "c3":
+        print(json.dumps(sparse_config(args, mcg, args.config)), flush=True)
+        return
 
     n, d, k = args.n, args.d, args.k
     log(f"generating n={n} d={d} k={k} on cuda:0")
@@ -173,6 +292,8 @@ def main():
     log("timing BF16 form")
     bf = time_form(ds, ctx, mcg, C0, asn, mcg.PREC_BF16, args.steps, args.warmup)
     log(json.dumps(bf))
+    fp64 = fp64_check(ds, mcg, X, C0, args.steps + args.warmup)
+    log(json.dumps(fp64))
     fp = None
     if not args.no_fp32:
         log("timing plain FP32 form")
@@ -273,6 +394,7 @@ def main():
                      "algorithmic": "2*n*k*d flop per launch"},
         "latency_ms": {"p50": bf["p50_ms"], "p99": bf["p99_ms"]},
         "near_ties_last_step": {"pairs": bf["refine_pairs"], "full": bf["refine_full"]},
+        "error_vs_fp64": fp64,
         "mstep": {"partial_kernel_ms": bf["mstep_partial_ms"],
                   "hbm_GBs": mstep_bytes / (bf["mstep_partial_ms"] * 1e-3) / 1e9},
         "cpu_baseline": cpu,

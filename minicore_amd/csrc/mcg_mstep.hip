@@ -220,21 +220,44 @@ __global__ __launch_bounds__(64) void k_partial(const float *__restrict__ X, uin
             s[i] = 0.;
             cj[i] = cold ? (double)cold[(uint64_t)c * d + j + i] : 0.;
         }
-#pragma unroll 4
-        for (uint32_t q = 0; q < nm; ++q) {
-            float xv[DPL];
-            if (VEC4) {
-                const float4 v = *(const float4 *)(X + (uint64_t)mem[q] * d + j);
-                xv[0] = v.x; xv[1] = v.y; xv[2] = v.z; xv[3] = v.w;
-            } else {
-                xv[0] = X[(uint64_t)mem[q] * d + j];
-            }
-            const double wq = wl[q];
+        auto accumulate = [&](const float (&xv)[DPL], double wq) __attribute__((always_inline)) {
 #pragma unroll
             for (uint32_t i = 0; i < DPL; ++i) {
                 const double x = (double)xv[i];
                 s[i] = w ? __fma_rn(wq, x, s[i]) : __dadd_rn(s[i], x);
                 if (cold) { const double t = x - cj[i]; qsum += w ? wq * (t * t) : t * t; }
+            }
+        };
+        if constexpr (VEC4) {
+            // software pipeline: the next 4 member rows load while the current 4 are
+            // added (in member order, so the sums are unchanged)
+            constexpr uint32_t G = 4;
+            float4 cur[G];
+#pragma unroll
+            for (uint32_t g = 0; g < G; ++g)
+                cur[g] = g < nm ? *(const float4 *)(X + (uint64_t)mem[g] * d + j) : make_float4(0.f, 0.f, 0.f, 0.f);
+            for (uint32_t q0 = 0; q0 < nm; q0 += G) {
+                float4 nxt[G];
+#pragma unroll
+                for (uint32_t g = 0; g < G; ++g) {
+                    const uint32_t q = q0 + G + g;
+                    nxt[g] = q < nm ? *(const float4 *)(X + (uint64_t)mem[q] * d + j) : make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+#pragma unroll
+                for (uint32_t g = 0; g < G; ++g) {
+                    if (q0 + g < nm) {
+                        const float xv[DPL] = {cur[g].x, cur[g].y, cur[g].z, cur[g].w};
+                        accumulate(xv, wl[q0 + g]);
+                    }
+                }
+#pragma unroll
+                for (uint32_t g = 0; g < G; ++g) cur[g] = nxt[g];
+            }
+        } else {
+#pragma unroll 4
+            for (uint32_t q = 0; q < nm; ++q) {
+                const float xv[DPL] = {X[(uint64_t)mem[q] * d + j]};
+                accumulate(xv, wl[q]);
             }
         }
 #pragma unroll

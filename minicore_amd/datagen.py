@@ -62,6 +62,31 @@ def poisson_csr_fast(n, d, density, seed, sigma=1.5, normalize=False):
     return vals.astype(np.float32), cols, indptr
 
 
+def counts_csr_gaps(n, d, density, seed, mu_sigma=2.0, normalize=False):
+    """Sparse counts at the given density with strictly increasing columns drawn as
+    geometric gaps (no sort); values Poisson(mu_j), mu_j ~ LogNormal(0, mu_sigma),
+    zero draws -> 1.  For large shapes (10^6 x 32768 at 1%)."""
+    rng = np.random.default_rng(seed)
+    mu = rng.lognormal(0., mu_sigma, size=d)
+    per = int(density * d * 1.5) + 16
+    gaps = rng.geometric(density, size=(n, per)).astype(np.int64)
+    cols = np.cumsum(gaps, axis=1) - 1 + rng.integers(0, 2, size=(n, 1))
+    keep = cols < d
+    nnz_row = keep.sum(axis=1)
+    nnz_row = np.maximum(nnz_row, 1)
+    cols[:, 0] = np.minimum(cols[:, 0], d - 1)
+    keep[:, 0] = True
+    indices = cols[keep].astype(np.uint32)
+    indptr = np.zeros(n + 1, dtype=np.uint64)
+    indptr[1:] = np.cumsum(keep.sum(axis=1))
+    vals = rng.poisson(mu[indices]).astype(np.float32)
+    vals[vals == 0] = 1.
+    if normalize:
+        rs = np.add.reduceat(vals.astype(np.float64), indptr[:-1].astype(np.int64))
+        vals = (vals / np.repeat(rs, np.diff(indptr).astype(np.int64))).astype(np.float32)
+    return vals, indices, indptr
+
+
 def gaussian_mixture_device(n, d, ncomp, seed, lo=-10., hi=10., scale=(0.5, 2.), device="cuda"):
     """Same construction on the device with torch's generator (for the 10^6-row workload)."""
     import torch

@@ -31,6 +31,8 @@ __device__ __forceinline__ uint16_t f2bf_rne(float f) {
     return (uint16_t)(u >> 16);
 }
 
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
 // ---------------------------------------------------------------------------
 // Dataset preparation: column mean mu (FP64 sums), BF16 copy of X - mu padded
 // to dpad columns, ||x - mu|| per row (rounded up), max over rows.
@@ -70,28 +72,36 @@ __global__ void k_colsum_final(const double *__restrict__ part, uint32_t nparts,
     if (threadIdx.x == 0) mu[j] = (float)(sm[0] / (double)n);
 }
 
-// one wave per row; lane l converts columns [4l, 4l + 4) of each 256-column pass
+// one wave per row; lane l converts columns [4l, 4l + 4) of each 256-column pass.
+// xnorm = ||x - mu||, xdev = ||bf16(x - mu) - (x - mu)|| (each difference is exact
+// in FP32), both rounded up.
 __global__ void k_make_xb(const float *__restrict__ X, uint64_t n, uint32_t d, uint32_t dpad,
                           const float *__restrict__ mu, uint16_t *__restrict__ Xb,
-                          float *__restrict__ xnorm) {
+                          float *__restrict__ xnorm, float *__restrict__ xdev) {
     const uint64_t row = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (row >= n) return;
-    double s = 0.;
+    double s = 0., e = 0.;
     for (uint32_t j0 = lane * 4; j0 < dpad; j0 += 256) {
         float v[4];
+        uint16_t bv[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const uint32_t j = j0 + i;
             v[i] = j < d ? X[row * d + j] - mu[j] : 0.f;
+            bv[i] = f2bf_rne(v[i]);
             s += (double)v[i] * (double)v[i];
+            const double dv = (double)bf2f(bv[i]) - (double)v[i];
+            e += dv * dv;
         }
-        const uint2 packed = make_uint2((uint32_t)f2bf_rne(v[0]) | ((uint32_t)f2bf_rne(v[1]) << 16),
-                                        (uint32_t)f2bf_rne(v[2]) | ((uint32_t)f2bf_rne(v[3]) << 16));
+        const uint2 packed = make_uint2((uint32_t)bv[0] | ((uint32_t)bv[1] << 16), (uint32_t)bv[2] | ((uint32_t)bv[3] << 16));
         *(uint2 *)(Xb + row * dpad + j0) = packed;
     }
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (lane == 0) xnorm[row] = (float)(sqrt(s) * (1. + 1e-6));
+    for (int o = 32; o > 0; o >>= 1) { s += __shfl_xor(s, o); e += __shfl_xor(e, o); }
+    if (lane == 0) {
+        xnorm[row] = (float)(sqrt(s) * (1. + 1e-6));
+        xdev[row] = (float)(sqrt(e) * (1. + 1e-6));
+    }
 }
 
 // max of non-negative floats (their bit patterns order like their values)
@@ -135,8 +145,9 @@ void prepare_bf16(Data &ds) {
                        ds.mu.as<float>());
     ds.Xb.reserve((size_t)n * ds.dpad * sizeof(uint16_t));
     ds.xnorm.reserve(n * sizeof(float));
+    ds.xdev.reserve(n * sizeof(float));
     hipLaunchKernelGGL(k_make_xb, dim3((uint32_t)cdiv(n, 4)), dim3(256), 0, st, ds.X, n, d, ds.dpad,
-                       ds.mu.as<float>(), ds.Xb.as<uint16_t>(), ds.xnorm.as<float>());
+                       ds.mu.as<float>(), ds.Xb.as<uint16_t>(), ds.xnorm.as<float>(), ds.xdev.as<float>());
     DBuf mx;
     mx.reserve(sizeof(float));
     fill_u32(mx.as<uint32_t>(), 0u, 1, st);
@@ -151,49 +162,67 @@ void prepare_bf16(Data &ds) {
 // Center preparation (per E-step).
 // ---------------------------------------------------------------------------
 // one wave per (padded) center: Cb = bf16(-2 (c - mu)), cn = ||c - mu||^2 (FP64)
+// One wave per (padded) center: Cb = bf16(-2 (c - mu)), cn = ||c - mu||^2 (FP64),
+// cinfo = {||c - mu||, ||bf16(c - mu) - (c - mu)||}, both rounded up (bf16(-2v) is
+// exactly -2 bf16(v), so the MFMA operand's error is that of bf16(v)).
 __global__ void k_prep_centers_bf16(const float *__restrict__ C, uint64_t k, uint64_t kpad, uint32_t d,
                                     uint32_t dpad, const float *__restrict__ mu,
-                                    uint16_t *__restrict__ Cb, double *__restrict__ cn) {
+                                    uint16_t *__restrict__ Cb, double *__restrict__ cn, float2 *__restrict__ cinfo) {
     const uint64_t c = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (c >= kpad) return;
-    double s = 0.;
+    double s = 0., e = 0.;
     for (uint32_t j = lane; j < dpad; j += 64) {
-        float v = (c < k && j < d) ? C[c * d + j] - mu[j] : 0.f;
-        Cb[c * dpad + j] = f2bf_rne(-2.f * v);
+        const float v = (c < k && j < d) ? C[c * d + j] - mu[j] : 0.f;
+        const uint16_t bv = f2bf_rne(-2.f * v);
+        Cb[c * dpad + j] = bv;
         s += (double)v * (double)v;
+        const double dv = -0.5 * (double)bf2f(bv) - (double)v;
+        e += dv * dv;
     }
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (lane == 0) cn[c] = s;
+    for (int o = 32; o > 0; o >>= 1) { s += __shfl_xor(s, o); e += __shfl_xor(e, o); }
+    if (lane == 0) {
+        cn[c] = s;
+        cinfo[c] = make_float2((float)(sqrt(s) * (1. + 1e-6)), (float)(sqrt(e) * (1. + 1e-6)));
+    }
 }
 
-// single block: cmax, offset, near-tie coefficients, cadd[c] = off + cn[c]
-__global__ void k_prep_params(const double *__restrict__ cn, uint64_t k, uint64_t kpad, float xmax,
-                              uint32_t d, float *__restrict__ cadd, EParams *__restrict__ prm) {
-    __shared__ double sm[256];
-    double m = 0.;
-    for (uint64_t i = threadIdx.x; i < k; i += blockDim.x) m = fmax(m, cn[i]);
+// single block: cmax, max center rounding error, offset, near-tie coefficients,
+// cadd[c] = off + cn[c]
+__global__ void k_prep_params(const double *__restrict__ cn, const float2 *__restrict__ cinfo, uint64_t k,
+                              uint64_t kpad, float xmax, uint32_t d, float *__restrict__ cadd,
+                              EParams *__restrict__ prm) {
+    __shared__ double sm[256], se[256];
+    double m = 0., me = 0.;
+    for (uint64_t i = threadIdx.x; i < k; i += blockDim.x) {
+        m = fmax(m, cn[i]);
+        me = fmax(me, (double)cinfo[i].y);
+    }
     sm[threadIdx.x] = m;
+    se[threadIdx.x] = me;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+        if ((int)threadIdx.x < s) {
+            sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+            se[threadIdx.x] = fmax(se[threadIdx.x], se[threadIdx.x + s]);
+        }
         __syncthreads();
     }
     const double cmax = sqrt(sm[0]) * (1. + 1e-6);
     const double off = 2.25 * (double)xmax * cmax + 1.;
     // Largest approximate value: off + cmax^2 + 2.1 xmax cmax.
     const double vmax = off + cmax * cmax + 2.1 * (double)xmax * cmax;
-    const double u = 1. / 256.;                       // BF16 unit roundoff
     const double e_abs = 400. * vmax * 0x1p-24        // accumulation, offset, packing
                          + 4. * 0x1p-24 * ((double)xmax + cmax) * ((double)xmax + cmax);
     if (threadIdx.x == 0) {
         EParams p;
         p.off = (float)off;
         p.cmax = (float)cmax;
-        p.tau1 = (float)(4. * (2. * u + u * u) * cmax * (1. + 1e-3));
+        p.tau1 = 0.f;
         p.tau0 = (float)(2. * e_abs);
         p.taur = (float)(1e-4 + 4. * (double)d * 0x1p-24);
-        p.pad[0] = p.pad[1] = p.pad[2] = 0.f;
+        p.dbmax = (float)se[0];
+        p.pad[0] = p.pad[1] = 0.f;
         *prm = p;
     }
     __syncthreads();
@@ -304,8 +333,9 @@ __device__ __forceinline__ void cand_overflow(const RefineBufs &R, uint32_t p) {
 template <int KS, int NB>
 __global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep_bf16(
     const uint4 *__restrict__ Xb, const uint4 *__restrict__ Cb, const float *__restrict__ cadd,
-    const float *__restrict__ xnorm, const EParams *__restrict__ prm, uint32_t n, uint32_t ntiles, uint32_t kreal,
-    uint32_t *__restrict__ asn, RefineBufs R) {
+    const float *__restrict__ xnorm, const float *__restrict__ xdev, const float2 *__restrict__ cinfo,
+    const EParams *__restrict__ prm, uint32_t n, uint32_t ntiles, uint32_t kreal, uint32_t *__restrict__ asn,
+    RefineBufs R) {
     constexpr int CH = KS * 2;                  // 16-byte chunks per row
     constexpr int TILE = 64 * CH * 16;          // bytes per tile (64 rows)
     constexpr int INSTR = CH / 4;               // LDS loads per wave per tile
@@ -511,9 +541,14 @@ __global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep_bf16(
         pc1[nb] = c1;
         pc2[nb] = c2;
         if (mine) {
-            const float xn = xnorm[p];
+            const float xn = xnorm[p], dxn = xdev[p];
             const float dbest = xn * xn + (v1 - P.off);
-            const float tau = P.tau1 * xn + P.tau0 + P.taur * fmaxf(dbest, 0.f);
+            // Cauchy-Schwarz on the rounding errors of the point (dxn) and of the centers:
+            // |x~.c~ - x.c| <= |dx| |c| + |x| |dc| + |dx| |dc|, for the best center and
+            // for any other (cmax, dbmax), doubled for the -2 x.c cross term
+            const float2 ci = cinfo[c1 < kreal ? c1 : 0];
+            const float tcs = 2.f * (dxn * (ci.x + P.cmax) + xn * (ci.y + P.dbmax) + dxn * (ci.y + P.dbmax));
+            const float tau = tcs * (1.f + 1e-3f) + P.tau0 + P.taur * fmaxf(dbest, 0.f);
             const bool ok1 = c1 < kreal;
             asn[p] = ok1 ? c1 : 0u;
             if (!(ok1 && v2 - v1 > tau)) {
@@ -940,7 +975,8 @@ static void launch_bf16(Data &ds, Ctx &c, uint64_t kpad, uint64_t kreal, uint32_
     c.tic(0, ev);
     hipLaunchKernelGGL((k_estep_bf16<KS, NB>), grid, dim3(256), 0, st, ds.Xb.as<const uint4>(),
                        c.s[S_CB].as<const uint4>(), c.s[S_CADD].as<const float>(), ds.xnorm.as<const float>(),
-                       c.s[S_PARAMS].as<const EParams>(), n, (uint32_t)(kpad / 64), (uint32_t)kreal, asn, R);
+                       ds.xdev.as<const float>(), c.s[S_CINFO].as<const float2>(), c.s[S_PARAMS].as<const EParams>(),
+                       n, (uint32_t)(kpad / 64), (uint32_t)kreal, asn, R);
     c.toc(0, ev);
     // statistics: {pair points, listed points} before the collect pass adds candidates
     hipLaunchKernelGGL(k_copy_counts, dim3(1), dim3(1), 0, st, R.cnt, R.cnt + 4);
@@ -1021,10 +1057,13 @@ void dense_sqrl2_assign(Data &ds, const float *C, uint64_t k, int precision, uin
         R.oflag = (uint32_t *)c.s[S_OFLAG].reserve(n * sizeof(uint32_t));
         if (c.s[S_OFLAG].bytes != before) fill_u32(R.oflag, 0u, c.s[S_OFLAG].bytes / 4, st);
     }
+    c.s[S_CINFO].reserve(kpad * sizeof(float2));
     hipLaunchKernelGGL(k_prep_centers_bf16, dim3((uint32_t)cdiv(kpad, 4)), dim3(256), 0, st, C, k, kpad, d, dpad,
-                       ds.mu.as<const float>(), c.s[S_CB].as<uint16_t>(), c.s[S_CNORM].as<double>());
-    hipLaunchKernelGGL(k_prep_params, dim3(1), dim3(256), 0, st, c.s[S_CNORM].as<const double>(), k, kpad,
-                       ds.xmax, d, c.s[S_CADD].as<float>(), c.s[S_PARAMS].as<EParams>());
+                       ds.mu.as<const float>(), c.s[S_CB].as<uint16_t>(), c.s[S_CNORM].as<double>(),
+                       c.s[S_CINFO].as<float2>());
+    hipLaunchKernelGGL(k_prep_params, dim3(1), dim3(256), 0, st, c.s[S_CNORM].as<const double>(),
+                       c.s[S_CINFO].as<const float2>(), k, kpad, ds.xmax, d, c.s[S_CADD].as<float>(),
+                       c.s[S_PARAMS].as<EParams>());
     fill_u32(R.cnt, 0u, 4, st);
     switch (dpad) {
     case 64: launch_bf16<4, 2>(ds, c, kpad, k, asn, R, st); break;

@@ -77,6 +77,7 @@ struct Data {
     bool bf16_ready = false;
     uint32_t dpad = 0;
     DBuf Xb, mu, xnorm;
+    DBuf xdev;         // ||bf16(x - mu) - (x - mu)|| per row (rounded up)
     float xmax = 0.f;
     // per-prior CSR transforms (cached for the last prior seen)
     double tf_prior = -1.;
@@ -118,7 +119,7 @@ enum Slot {
     S_HIST, S_BASE, S_PERM, S_CNT, S_ITEMOFF, S_PART, S_PARTW, S_PARTQ,
     S_CCOST, S_TOTAL, S_CENT_A, S_CENT_B, S_CENT_C, S_COSTS, S_CT, S_CPARAM,
     S_MISC0, S_MISC1, S_MISC2, S_MISC3, S_W, S_STAGE0, S_STAGE1, S_STAGE2,
-    S_LIS, S_TOT, S_JSDG, S_PAIRS, S_THRESH, S_OVER, S_KEYS, S_OFLAG, S_NSLOTS
+    S_LIS, S_TOT, S_JSDG, S_PAIRS, S_THRESH, S_OVER, S_KEYS, S_OFLAG, S_CINFO, S_NSLOTS
 };
 static_assert(S_NSLOTS <= 48, "scratch slots");
 
@@ -127,10 +128,11 @@ static_assert(S_NSLOTS <= 48, "scratch slots");
 struct EParams {
     float off;      // added to every approximate value to keep it positive
     float cmax;     // max_c ||c - mu||
-    float tau1;     // coefficient of ||x - mu|| in the tie threshold
+    float tau1;     // (unused; kept for layout)
     float tau0;     // absolute part of the tie threshold
     float taur;     // relative part (of the approximate best distance)
-    float pad[3];
+    float dbmax;    // max_c ||bf16(c - mu) - (c - mu)||
+    float pad[2];
 };
 
 void prepare_bf16(Data &ds);  // builds Xb, mu, xnorm, xmax (once per dataset)

@@ -86,6 +86,13 @@ int mcg_timing_enable(mcg_ctx *ctx, int on);
 int mcg_timing_reset(mcg_ctx *ctx);
 int mcg_timing_read(mcg_ctx *ctx, int which, double *total_ms, uint64_t *count);
 
+/* Tuning of the BF16-refined dense SQRL2 E-step (on by default): the BF16 MFMA
+ * kernels and the exact kernel give the same assignment bit for bit, and the faster
+ * one for the problem's shape (padded d, padded k, log2 n) is used, timed once per
+ * shape on the context.  Off: always the MFMA kernels.  (No counterpart in the
+ * reference.) */
+int mcg_tuning_enable(mcg_ctx *ctx, int on);
+
 /* Points of the last BF16 E-step on this context that needed an exact re-score:
  * `pairs` chose between their two nearest centers, `full` were re-scored over
  * all centers (no counterpart in the reference; for measurement). */

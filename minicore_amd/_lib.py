@@ -22,7 +22,7 @@ EXPORTS = [
     "mcg_cost_matrix", "mcg_assign_hard", "mcg_centroids_hard", "mcg_lloyd", "mcg_soft_em",
     "mcg_kmeanspp", "mcg_coreset_probs", "mcg_coreset_sample", "mcg_lloyd_step", "mcg_timing_enable",
     "mcg_timing_reset", "mcg_timing_read", "mcg_refine_counts", "mcg_data_rows", "mcg_server_create",
-    "mcg_server_destroy", "mcg_submit", "mcg_poll", "mcg_wait", "mcg_cancel", "mcg_release",
+    "mcg_server_destroy", "mcg_submit", "mcg_poll", "mcg_wait", "mcg_cancel", "mcg_release", "mcg_tuning_enable",
 ]
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = range(5)
@@ -105,6 +105,7 @@ def load():
         "mcg_coreset_sample": (I, [P, U64, P, P, U64, U64, U64, P, P]),
         "mcg_lloyd_step": (I, [P, I, D, P, U64, P, I, P, P]),
         "mcg_timing_enable": (I, [P, I]),
+        "mcg_tuning_enable": (I, [P, I]),
         "mcg_timing_reset": (I, [P]),
         "mcg_timing_read": (I, [P, I, P, P]),
         "mcg_refine_counts": (I, [P, P, P]),
@@ -168,6 +169,10 @@ class Context:
 
     def synchronize(self):
         check(load().mcg_ctx_synchronize(self.h))
+
+    def tuning(self, on=True):
+        """Timing-based choice between bit-equivalent E-step kernels (on by default)."""
+        check(load().mcg_tuning_enable(self.h, 1 if on else 0))
 
     def timing(self, on=True):
         check(load().mcg_timing_enable(self.h, 1 if on else 0))

@@ -827,6 +827,10 @@ int mcg_coreset_sample(mcg_ctx *ctx, uint64_t n, const float *probs, const doubl
     });
 }
 
+int mcg_tuning_enable(mcg_ctx *ctx, int on) {
+    return guard([&] { REQUIRE(ctx, "null ctx"); ctx->c.tuning = on != 0; });
+}
+
 int mcg_timing_enable(mcg_ctx *ctx, int on) {
     return guard([&] { REQUIRE(ctx, "null ctx"); ctx->c.timing = on != 0; });
 }

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 
 namespace mcg {
 
@@ -1011,14 +1012,66 @@ void dense_sqrl2_costs(Data &ds, const float *C, const uint32_t *asn, float *cos
                        (uint32_t)ds.d, C, asn, costs);
 }
 
+static void dense_sqrl2_assign_mfma(Data &ds, const float *C, uint64_t k, int precision, uint32_t *asn,
+                                    hipStream_t st);
+
+// Tuning of the BF16-refined form: it and the exact kernel give the same assignment
+// bit for bit, so the faster one is used.  Clear cases are decided by the work
+// n*k*d; in between, both are timed once per (padded d, padded k, log2 n) on the
+// context (min of two runs each) and the choice is kept.  mcg_tuning_enable(ctx, 0)
+// or MCG_TUNE=0 always takes the MFMA form.
+static int tuned_form(Data &ds, const float *C, uint64_t k, uint32_t *asn, hipStream_t st) {
+    static const bool tune_on = [] { const char *e = getenv("MCG_TUNE"); return !(e && e[0] == '0'); }();
+    if (!tune_on || !ds.ctx->tuning) return MCG_PREC_BF16;
+    const double work = (double)ds.n * (double)k * (double)ds.d;
+    if (work >= 0x1p34) return MCG_PREC_BF16;
+    if (work <= 0x1p21) return MCG_PREC_EXACT;
+    Ctx &c = *ds.ctx;
+    prepare_bf16(ds);
+    uint64_t lg = 0;
+    while ((1ull << lg) < ds.n) ++lg;
+    const uint64_t key = ((uint64_t)ds.dpad << 40) | (cdiv(k, 64) << 8) | lg;
+    for (const auto &kv : c.tuned)
+        if (kv.first == key) return kv.second;
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    auto time_form = [&](int form) {
+        float best = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+            HIP_CHECK(hipEventRecord(e0, st));
+            if (form == MCG_PREC_EXACT) dense_sqrl2_assign_exact(ds, C, k, asn, nullptr, st);
+            else dense_sqrl2_assign_mfma(ds, C, k, MCG_PREC_BF16, asn, st);
+            HIP_CHECK(hipEventRecord(e1, st));
+            HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            best = std::min(best, ms);
+        }
+        return best;
+    };
+    const float t_mfma = time_form(MCG_PREC_BF16), t_exact = time_form(MCG_PREC_EXACT);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const int form = t_exact < t_mfma ? MCG_PREC_EXACT : MCG_PREC_BF16;
+    c.tuned.push_back({key, form});
+    return form;
+}
+
 void dense_sqrl2_assign(Data &ds, const float *C, uint64_t k, int precision, uint32_t *asn,
                         hipStream_t st) {
-    Ctx &c = *ds.ctx;
     REQUIRE(ds.n < (1ull << 31), "dense rows must be < 2^31");
+    if (precision == MCG_PREC_BF16 && ds.d <= 512) precision = tuned_form(ds, C, k, asn, st);
     if (precision == MCG_PREC_EXACT || ds.d > 512 || (precision == MCG_PREC_FP32 && ds.d > 256)) {
         dense_sqrl2_assign_exact(ds, C, k, asn, nullptr, st);
         return;
     }
+    dense_sqrl2_assign_mfma(ds, C, k, precision, asn, st);
+}
+
+static void dense_sqrl2_assign_mfma(Data &ds, const float *C, uint64_t k, int precision, uint32_t *asn,
+                                    hipStream_t st) {
+    Ctx &c = *ds.ctx;
     prepare_bf16(ds);  // mu, dpad (and the BF16 copy) are shared by both MFMA forms
     const uint32_t d = (uint32_t)ds.d, dpad = ds.dpad;
     if (precision == MCG_PREC_FP32) {

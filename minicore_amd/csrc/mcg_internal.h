@@ -107,6 +107,9 @@ struct Ctx {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[2];
     double timed_ms[2] = {0., 0.};
     uint64_t timed_n[2] = {0, 0};
+    // E-step form chosen per problem shape by timing (see dense_sqrl2_assign)
+    bool tuning = true;
+    std::vector<std::pair<uint64_t, int>> tuned;
     void tic(int which, hipEvent_t &a);
     void toc(int which, hipEvent_t a);
     void collect();   // after a stream synchronize: fold pending pairs into the totals

@@ -30,6 +30,9 @@ def mcg():
 
 @pytest.fixture(scope="session")
 def ctx(mcg):
+    """Shared context with the E-step tuning off, so the small parity cases run the
+    BF16 MFMA kernels (tuning would pick the equivalent exact kernel for them)."""
     c = mcg.Context(0)
+    c.tuning(False)
     yield c
     c.close()

@@ -39,6 +39,26 @@ def test_assign_exact_and_bf16_bitexact(mcg, ctx, mco, n, d, k, seed):
     ds.close()
 
 
+@pytest.mark.parametrize("n,d,k,seed", [(500, 20, 8, 11), (3000, 64, 40, 12), (20000, 128, 96, 13)])
+def test_tuned_form_bitexact(mcg, mco, n, d, k, seed):
+    """With the E-step tuning on, whichever kernel the timing picks (exact for the
+    smallest shape, measured for the others) gives the oracle's assignment; the
+    second call reuses the cached choice."""
+    say(f"tuned assign n={n} d={d} k={k}")
+    X, C = mixture(n, d, k, seed)
+    a_ref, c_ref, _ = mco.assign_sqrl2_f32(X, C)
+    tc = mcg.Context(0)
+    try:
+        ds = mcg.Dataset.dense(X, tc)
+        for _ in range(2):
+            a, c = ds.assign(mcg.SQRL2, C, precision=mcg.PREC_BF16)
+            assert np.array_equal(a, a_ref)
+            assert np.array_equal(c, c_ref)
+        ds.close()
+    finally:
+        tc.close()
+
+
 def test_bf16_near_ties_are_rescored(mcg, ctx, mco):
     """Points equidistant (in exact arithmetic) from two centers, or sitting on three
     duplicated centers: both refinement paths run and the result stays exact."""

@@ -86,6 +86,12 @@ int mcg_timing_enable(mcg_ctx *ctx, int on);
 int mcg_timing_reset(mcg_ctx *ctx);
 int mcg_timing_read(mcg_ctx *ctx, int which, double *total_ms, uint64_t *count);
 
+/* Graph replay of mcg_lloyd's iterations (on by default): the iterations after the
+ * first are captured once into a HIP graph and replayed, in batches with one host
+ * synchronisation each.  Off: every iteration is launched kernel by kernel (same
+ * batches, same results).  (No counterpart in the reference.) */
+int mcg_graphs_enable(mcg_ctx *ctx, int on);
+
 /* Tuning of the BF16-refined dense SQRL2 E-step (on by default): the BF16 MFMA
  * kernels and the exact kernel give the same assignment bit for bit, and the faster
  * one for the problem's shape (padded d, padded k, log2 n) is used, timed once per

@@ -23,6 +23,7 @@ EXPORTS = [
     "mcg_kmeanspp", "mcg_coreset_probs", "mcg_coreset_sample", "mcg_lloyd_step", "mcg_timing_enable",
     "mcg_timing_reset", "mcg_timing_read", "mcg_refine_counts", "mcg_data_rows", "mcg_server_create",
     "mcg_server_destroy", "mcg_submit", "mcg_poll", "mcg_wait", "mcg_cancel", "mcg_release", "mcg_tuning_enable",
+    "mcg_graphs_enable",
 ]
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = range(5)
@@ -106,6 +107,7 @@ def load():
         "mcg_lloyd_step": (I, [P, I, D, P, U64, P, I, P, P]),
         "mcg_timing_enable": (I, [P, I]),
         "mcg_tuning_enable": (I, [P, I]),
+        "mcg_graphs_enable": (I, [P, I]),
         "mcg_timing_reset": (I, [P]),
         "mcg_timing_read": (I, [P, I, P, P]),
         "mcg_refine_counts": (I, [P, P, P]),
@@ -173,6 +175,10 @@ class Context:
     def tuning(self, on=True):
         """Timing-based choice between bit-equivalent E-step kernels (on by default)."""
         check(load().mcg_tuning_enable(self.h, 1 if on else 0))
+
+    def graphs(self, on=True):
+        """Graph replay of the Lloyd iterations (on by default; results are the same)."""
+        check(load().mcg_graphs_enable(self.h, 1 if on else 0))
 
     def timing(self, on=True):
         check(load().mcg_timing_enable(self.h, 1 if on else 0))

@@ -4,6 +4,7 @@
 #include "mcg_internal.h"
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -244,21 +245,6 @@ __global__ void k_wsum_d(const double *__restrict__ v, uint64_t n, double *__res
     if (threadIdx.x == 0) *out = sm[0];
 }
 
-__global__ void k_any_nan(const float *__restrict__ p, uint64_t n, uint32_t *__restrict__ flag) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        if (p[i] != p[i]) *flag = 1u;
-}
-
-bool any_nan(const float *p, uint64_t n, hipStream_t st, Ctx &c) {
-    uint32_t *f = (uint32_t *)c.s[S_MISC3].reserve(sizeof(uint32_t));
-    fill_u32(f, 0, 1, st);
-    hipLaunchKernelGGL(k_any_nan, dim3((uint32_t)std::min<uint64_t>(cdiv(n, 256), 1024)), dim3(256), 0, st, p, n, f);
-    uint32_t h = 0;
-    HIP_CHECK(hipMemcpyAsync(&h, f, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipStreamSynchronize(st));
-    return h != 0;
-}
-
 // copy data rows into center slots: C[ids[q]] = row rows[q]
 __global__ void k_rows_to_centers_dense(const float *__restrict__ X, uint32_t d, const uint64_t *__restrict__ rows,
                                         const uint32_t *__restrict__ ids, uint64_t m, float *__restrict__ C) {
@@ -304,6 +290,162 @@ __global__ void k_force(uint32_t *__restrict__ asn, float *__restrict__ cost, co
     const uint64_t r = rows[q];
     if (asn[r] != ids[q]) { asn[r] = ids[q]; if (cost) cost[r] = 0.f; }
 }
+
+__global__ void k_copy_f32(const float *__restrict__ src, float *__restrict__ dst, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+static void copy_f32(const float *src, float *dst, uint64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_copy_f32, dim3((uint32_t)std::min<uint64_t>(cdiv(n, 256), 2048)), dim3(256), 0, st, src, dst,
+                       n);
+}
+
+// ---- device-side control of the Lloyd loop (solve.h:143-170) -------------------
+struct LloydCtl {
+    float cost;            // accepted cost, in float like the reference's FT (solve.h:120-123)
+    float init;            // initial cost
+    uint32_t active;       // the loop goes on
+    uint32_t accept;       // the last candidate was accepted (read by k_lloyd_commit)
+    uint32_t need_host;    // an accepted assignment left a cluster empty: the host restarts it
+    uint32_t nan;          // a candidate center held a NaN (the host throws)
+    uint32_t nanflag;      // set by k_lloyd_nan for the current candidate
+    uint32_t pad;
+    unsigned long long iter;
+};
+static_assert(sizeof(LloydCtl) <= 64, "LloydCtl");
+
+__device__ bool block_any(bool v) {
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    if (v) any = 1;
+    __syncthreads();
+    return any != 0;
+}
+
+__global__ void k_lloyd_nan(const float *__restrict__ p, uint64_t n, LloydCtl *__restrict__ ctl) {
+    bool bad = false;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        bad |= p[i] != p[i];
+    if (bad) ctl->nanflag = 1u;
+}
+
+// after the first E+M pass: cost of the input centers; stop at cost 0 (solve.h:131-137)
+__global__ void k_lloyd_init(LloydCtl *__restrict__ ctl, const double *__restrict__ cost64,
+                             const uint32_t *__restrict__ cnt2, uint64_t k) {
+    bool z = false;
+    for (uint64_t q = threadIdx.x; q < k; q += blockDim.x) z |= cnt2[q] == 0u;
+    const bool empty = block_any(z);
+    if (threadIdx.x) return;
+    const float cost = (float)*cost64;
+    ctl->cost = cost;
+    ctl->init = cost;
+    ctl->iter = 0;
+    ctl->accept = 0;
+    ctl->nan = 0;
+    ctl->nanflag = 0;
+    ctl->need_host = cost != 0.f && empty;
+    ctl->active = cost != 0.f && !empty;
+}
+
+// stop / accept after an iteration's M-step (solve.h:160-170): a cost increase
+// without a restart keeps the old state and stops; otherwise accept, count the
+// iteration and stop on old - new < eps * max(old, new) or iter > maxiter.
+__global__ void k_lloyd_decide(LloydCtl *__restrict__ ctl, const double *__restrict__ cost64,
+                               const uint32_t *__restrict__ cnt2, uint64_t k, double eps, uint64_t maxiter,
+                               int restarted) {
+    bool z = false;
+    for (uint64_t q = threadIdx.x; q < k; q += blockDim.x) z |= cnt2[q] == 0u;
+    const bool empty = block_any(z);
+    if (threadIdx.x) return;
+    const uint32_t nanflag = ctl->nanflag;
+    ctl->nanflag = 0;
+    ctl->accept = 0;
+    if (!ctl->active) return;
+    if (nanflag) {
+        ctl->nan = 1;
+        ctl->active = 0;
+        return;
+    }
+    const float newcost = (float)*cost64;
+    if (newcost > ctl->cost && !restarted) {
+        ctl->active = 0;
+        return;
+    }
+    ctl->accept = 1;
+    ctl->iter += 1;
+    const float old = ctl->cost;
+    ctl->cost = newcost;
+    const float diff = old - newcost;
+    const double hi = (double)newcost < (double)old ? (double)old : (double)newcost;
+    if ((double)diff < eps * hi || ctl->iter > maxiter) ctl->active = 0;
+    else if (empty) { ctl->need_host = 1; ctl->active = 0; }
+}
+
+__global__ void k_lloyd_resume(LloydCtl *__restrict__ ctl) {
+    ctl->need_host = 0;
+    ctl->nanflag = 0;
+    ctl->active = 1;
+}
+
+// mode 0: an accepted candidate becomes the state (C, A, pc, Cm, cnt);
+// mode 1: only the means and counts of the first pass (Cm, cnt)
+__global__ void k_lloyd_commit(const LloydCtl *__restrict__ ctl, int mode, float *__restrict__ C,
+                               const float *__restrict__ Cn, uint64_t kd, uint32_t *__restrict__ A,
+                               const uint32_t *__restrict__ An, uint64_t n, float *__restrict__ pc,
+                               const float *__restrict__ pcn, float *__restrict__ Cm, const float *__restrict__ Cm2,
+                               uint32_t *__restrict__ cnt, const uint32_t *__restrict__ cnt2, uint64_t k) {
+    if (mode == 0 && !ctl->accept) return;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x, t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint64_t i = t0; i < kd; i += stride) Cm[i] = Cm2[i];
+    for (uint64_t i = t0; i < k; i += stride) cnt[i] = cnt2[i];
+    if (mode) return;
+    for (uint64_t i = t0; i < kd; i += stride) C[i] = Cn[i];
+    for (uint64_t i = t0; i < n; i += stride) A[i] = An[i];
+    if (pc)
+        for (uint64_t i = t0; i < n; i += stride) pc[i] = pcn[i];
+}
+
+// One Lloyd iteration captured from the stream and replayed.  Capture is thread
+// local, so other threads' contexts keep working meanwhile; a failed capture or
+// instantiation leaves the caller to launch the iteration itself.
+struct IterGraph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    bool failed = false;
+    ~IterGraph() { reset(); }
+    bool ready() const { return ge != nullptr; }
+    void reset() {
+        if (ge) (void)hipGraphExecDestroy(ge);
+        if (g) (void)hipGraphDestroy(g);
+        ge = nullptr;
+        g = nullptr;
+    }
+    template <class F> void capture(hipStream_t st, F &&body) {
+        HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            body();
+        } catch (...) {
+            hipGraph_t tmp = nullptr;
+            (void)hipStreamEndCapture(st, &tmp);
+            if (tmp) (void)hipGraphDestroy(tmp);
+            throw;
+        }
+        if (hipStreamEndCapture(st, &g) != hipSuccess || !g) {
+            (void)hipGetLastError();
+            g = nullptr;
+            failed = true;
+            return;
+        }
+        if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            reset();
+            failed = true;
+        }
+    }
+    void launch(hipStream_t st) { HIP_CHECK(hipGraphLaunch(ge, st)); }
+};
 
 // ---------------------------------------------------------------------------
 // E-step dispatch: asn (device) and per-point best costs (device, may be null
@@ -577,6 +719,14 @@ int mcg_centroids_hard(mcg_data *ds, const uint32_t *asn, const double *weights,
 
 // perform_hard_clustering (solve.h:98-175).  For dense SQRL2 the M-step pass also
 // measures cost(asn, C) in FP64; elsewhere the E-step's per-point minima are summed.
+//
+// The stop / accept rules run on the device (k_lloyd_decide), so iterations are
+// queued in batches (2, 4, ... up to a size set by the work per iteration) with one
+// host synchronisation per batch; iterations queued after the loop has stopped
+// compute into the candidate buffers and change nothing.  The iterations after the
+// first replay one captured HIP graph.  An accepted assignment that leaves a
+// cluster empty hands the next iteration to the host, which runs the reservoir
+// restart of centroid.h:687-752 and then goes back to device batches.
 int mcg_lloyd(mcg_data *ds, int msr, double prior, float *centers, uint64_t k, const double *weights, double eps,
               uint64_t maxiter, int precision, uint32_t *asn, float *costs, double *initcost, double *finalcost,
               uint64_t *numiter) {
@@ -588,102 +738,128 @@ int mcg_lloyd(mcg_data *ds, int msr, double prior, float *centers, uint64_t k, c
         Ctx &c = *D.ctx;
         hipStream_t st = c.stream;
         HIP_CHECK(hipSetDevice(c.device));
-        const uint64_t n = D.n, d = D.d;
+        const uint64_t n = D.n, d = D.d, kd = k * d;
         const bool l2 = dense_l2(D, msr);
         DBuf sw, sa_out, sk_out;
         const double *W = (const double *)in_dev(c, weights, n * 8, sw);
-        // device state: accepted centers C, candidate Cn, next means Cm; two assignment buffers
-        float *C = (float *)c.s[S_CENT_A].reserve(k * d * 4);
-        float *Cn = (float *)c.s[S_CENT_B].reserve(k * d * 4);
-        float *Cm = (float *)c.s[S_CENT_C].reserve(k * d * 4);
+        // accepted state: centers C, assignment A, per-point costs pc, means of A Cm and
+        // their counts cnt; candidate state: Cn, An, pcn, means Cm2
+        float *C = (float *)c.s[S_CENT_A].reserve(kd * 4);
+        float *Cn = (float *)c.s[S_CENT_B].reserve(kd * 4);
+        float *Cm = (float *)c.s[S_CENT_C].reserve(kd * 4);
+        float *Cm2 = (float *)c.s[S_CENT_D].reserve(kd * 4);
         uint32_t *A = (uint32_t *)c.s[S_ASN_A].reserve(n * 4);
         uint32_t *An = (uint32_t *)c.s[S_ASN_B].reserve(n * 4);
-        float *pc = (float *)c.s[S_COSTS].reserve(2 * n * 4);  // per-point costs for A / An
+        float *pc = (float *)c.s[S_COSTS].reserve(2 * n * 4);
         float *pcn = pc + n;
         double *tot = (double *)c.s[S_TOT].reserve(4 * sizeof(double));
-        HIP_CHECK(hipMemcpyAsync(C, centers, k * d * 4, is_device_ptr(centers) ? hipMemcpyDeviceToDevice
-                                                                               : hipMemcpyHostToDevice, st));
-        std::vector<uint32_t> cnt(k);
-        // Measure cost(asn, Cref) and the means of asn into Cmeans.  Returns FP64 cost.
-        auto mpass = [&](const uint32_t *a, const float *cost_pts, const float *Cref, float *Cmeans) -> double {
-            double cost;
-            HIP_CHECK(hipMemcpyAsync(Cmeans, Cref, k * d * 4, hipMemcpyDeviceToDevice, st));
-            MStepOut mo;
-            if (l2) {
-                mo = dense_mstep(D, a, W, k, Cref, Cmeans, true, st);
-                HIP_CHECK(hipMemcpyAsync(&cost, mo.total, sizeof(double), hipMemcpyDeviceToHost, st));
-            } else {
-                hipLaunchKernelGGL(k_wsum, dim3(1), dim3(1024), 0, st, cost_pts, W, n, tot);
-                mo = D.csr ? csr_mstep(D, a, W, k, Cmeans, st) : dense_mstep(D, a, W, k, Cref, Cmeans, false, st);
-                HIP_CHECK(hipMemcpyAsync(&cost, tot, sizeof(double), hipMemcpyDeviceToHost, st));
-            }
-            HIP_CHECK(hipMemcpyAsync(cnt.data(), mo.counts, k * 4, hipMemcpyDeviceToHost, st));
-            sync(c);
-            return cost;
-        };
-        auto do_estep = [&](const float *Cx, uint32_t *a, float *cp) {
+        char *cbuf = (char *)c.s[S_LCTL].reserve(64 + k * 4);
+        LloydCtl *ctl = (LloydCtl *)cbuf;
+        uint32_t *cnt = (uint32_t *)(cbuf + 64);
+        HIP_CHECK(hipMemcpyAsync(C, centers, kd * 4, is_device_ptr(centers) ? hipMemcpyDeviceToDevice
+                                                                           : hipMemcpyHostToDevice, st));
+        // E-step of Cx into (a, cp), then the FP64 cost and the means of a (empty
+        // clusters keep Cx) into Cm2.  Returns the cost and counts locations.
+        auto measure = [&](const float *Cx, uint32_t *a, float *cp) -> std::pair<const double *, const uint32_t *> {
             estep(D, msr, prior, Cx, k, precision, a, l2 ? nullptr : cp, st);
+            copy_f32(Cx, Cm2, kd, st);
+            if (l2) {
+                const MStepOut mo = dense_mstep(D, a, W, k, Cx, Cm2, true, st);
+                return {mo.total, mo.counts};
+            }
+            hipLaunchKernelGGL(k_wsum, dim3(1), dim3(1024), 0, st, cp, W, n, tot);
+            const MStepOut mo = D.csr ? csr_mstep(D, a, W, k, Cm2, st) : dense_mstep(D, a, W, k, Cx, Cm2, false, st);
+            return {tot, mo.counts};
         };
-        // Centers from assignment a (w.r.t. Cprev) with the empty-cluster restart
-        // of centroid.h:687-752.  Cm already holds the plain means of a.
-        auto next_centers = [&](uint32_t *a, float *cp, const float *Cprev, float *out) -> bool {
-            const uint64_t ne = (uint64_t)std::count(cnt.begin(), cnt.end(), 0u);
-            if (ne == 0) {
-                HIP_CHECK(hipMemcpyAsync(out, Cm, k * d * 4, hipMemcpyDeviceToDevice, st));
-                return false;
-            }
-            if (l2) dense_sqrl2_costs(D, Cprev, a, cp, st);
-            std::vector<uint64_t> rows(ne);
-            std::vector<uint32_t> ids;
-            for (uint64_t q = 0; q < k; ++q) if (!cnt[q]) ids.push_back((uint32_t)q);
-            restart_draws(c, cp, n, ne, rows.data(), st);
-            HIP_CHECK(hipMemcpyAsync(out, Cprev, k * d * 4, hipMemcpyDeviceToDevice, st));
-            copy_rows_to_centers(D, rows.data(), ids.data(), ne, out, st);
-            // full re-assignment against the mixed centers, then force the drawn rows
-            if (l2) dense_sqrl2_assign_exact(D, out, k, a, cp, st);
-            else generic_costs(D, msr, prior, out, k, nullptr, a, cp, st);
-            {
-                char *buf = (char *)c.s[S_MISC2].reserve(ne * 12);
-                HIP_CHECK(hipMemcpyAsync(buf, rows.data(), ne * 8, hipMemcpyHostToDevice, st));
-                HIP_CHECK(hipMemcpyAsync(buf + ne * 8, ids.data(), ne * 4, hipMemcpyHostToDevice, st));
-                hipLaunchKernelGGL(k_force, dim3((uint32_t)cdiv(ne, 256)), dim3(256), 0, st, a, cp,
-                                   (const uint64_t *)buf, (const uint32_t *)(buf + ne * 8), ne);
-            }
-            MStepOut mo = D.csr ? csr_mstep(D, a, W, k, out, st) : dense_mstep(D, a, W, k, out, out, false, st);
-            (void)mo;
+        const uint32_t cgrid = (uint32_t)std::min<uint64_t>(cdiv(2 * kd + 2 * n + k, 256), 2048);
+        auto commit = [&](int mode, const uint32_t *cnt2) {
+            hipLaunchKernelGGL(k_lloyd_commit, dim3(cgrid), dim3(256), 0, st, ctl, mode, C, Cn, kd, A, An, n,
+                               l2 ? nullptr : pc, pcn, Cm, Cm2, cnt, cnt2, k);
+        };
+        // one iteration from the accepted state (no restart); restarted = the host has
+        // already put the candidate centers into Cn
+        auto iteration = [&](int restarted) {
+            if (!restarted) copy_f32(Cm, Cn, kd, st);
+            hipLaunchKernelGGL(k_lloyd_nan, dim3((uint32_t)std::min<uint64_t>(cdiv(kd, 256), 1024)), dim3(256), 0, st,
+                               Cn, kd, ctl);
+            const auto m = measure(Cn, An, pcn);
+            hipLaunchKernelGGL(k_lloyd_decide, dim3(1), dim3(256), 0, st, ctl, m.first, m.second, k, eps, maxiter,
+                               restarted);
+            commit(0, m.second);
+        };
+        // the host part of an iteration that restarts empty clusters (next_centers of
+        // centroid.h:687-752): draws from the accepted costs, the drawn rows as centers,
+        // full re-assignment, the drawn rows forced to their centers, new means in Cn
+        auto restart = [&] {
+            std::vector<uint32_t> hc(k);
+            HIP_CHECK(hipMemcpyAsync(hc.data(), cnt, k * 4, hipMemcpyDeviceToHost, st));
             sync(c);
-            return true;
+            std::vector<uint32_t> ids;
+            for (uint64_t q = 0; q < k; ++q) if (!hc[q]) ids.push_back((uint32_t)q);
+            const uint64_t ne = ids.size();
+            REQUIRE(ne > 0, "internal: restart without an empty cluster");
+            hipLaunchKernelGGL(k_lloyd_resume, dim3(1), dim3(1), 0, st, ctl);
+            if (l2) dense_sqrl2_costs(D, C, A, pc, st);
+            std::vector<uint64_t> rows(ne);
+            restart_draws(c, pc, n, ne, rows.data(), st);
+            copy_f32(C, Cn, kd, st);
+            copy_rows_to_centers(D, rows.data(), ids.data(), ne, Cn, st);
+            if (l2) dense_sqrl2_assign_exact(D, Cn, k, A, pc, st);
+            else generic_costs(D, msr, prior, Cn, k, nullptr, A, pc, st);
+            char *buf = (char *)c.s[S_MISC2].reserve(ne * 12);
+            HIP_CHECK(hipMemcpyAsync(buf, rows.data(), ne * 8, hipMemcpyHostToDevice, st));
+            HIP_CHECK(hipMemcpyAsync(buf + ne * 8, ids.data(), ne * 4, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_force, dim3((uint32_t)cdiv(ne, 256)), dim3(256), 0, st, A, pc,
+                               (const uint64_t *)buf, (const uint32_t *)(buf + ne * 8), ne);
+            if (D.csr) csr_mstep(D, A, W, k, Cn, st);
+            else dense_mstep(D, A, W, k, Cn, Cn, false, st);
+            sync(c);
         };
 
-        do_estep(C, A, pc);
-        double cost64 = mpass(A, pc, C, Cm);
-        float cost = (float)cost64;
-        const double init = cost;
-        uint64_t iter = 0;
-        if (cost == 0.f) {
-            *initcost = 0.;
-            *finalcost = 0.;
-            *numiter = 0;
-        } else {
-            for (;;) {
-                const bool res = next_centers(A, pc, C, Cn);
-                if (any_nan(Cn, k * d, st, c)) MCG_THROW(MCG_ENAN, "Unexpected nan in ctr");
-                do_estep(Cn, An, pcn);
-                const float newcost = (float)mpass(An, pcn, Cn, Cm);
-                if (newcost > cost && !res) break;  // keep C and A (solve.h:160-164)
-                std::swap(C, Cn);
-                std::swap(A, An);
-                std::swap(pc, pcn);
-                ++iter;
-                const float old = cost;
-                cost = newcost;
-                const float diff = old - newcost;
-                if ((double)diff < eps * std::max((double)newcost, (double)old) || iter > maxiter) break;
-            }
-            *initcost = init;
-            *finalcost = cost;
-            *numiter = iter;
+        {
+            const auto m = measure(C, A, pc);
+            hipLaunchKernelGGL(k_lloyd_init, dim3(1), dim3(256), 0, st, ctl, m.first, m.second, k);
+            commit(1, m.second);
         }
+        // batch size limit from the work of one iteration
+        const double work = (double)(D.csr ? D.nnz : n * d) * (double)k;
+        const uint64_t bmax = work <= 0x1p30 ? 16 : work <= 0x1p33 ? 8 : work <= 0x1p35 ? 4 : 1;
+        static const bool graphs_env = [] { const char *e = getenv("MCG_GRAPHS"); return !(e && e[0] == '0'); }();
+        const bool use_graph = graphs_env && c.graphs && !c.timing;
+        IterGraph graph;
+        bool eager_done = false;
+        uint64_t batch = 2;
+        LloydCtl h;
+        for (;;) {
+            HIP_CHECK(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, st));
+            sync(c);
+            if (h.nan) MCG_THROW(MCG_ENAN, "Unexpected nan in ctr");
+            if (h.need_host) {
+                graph.reset();      // the restart may grow scratch buffers the graph points into
+                eager_done = false;
+                restart();
+                iteration(1);
+                continue;
+            }
+            if (!h.active) break;
+            const uint64_t span = maxiter - h.iter;  // h.iter <= maxiter while active
+            const uint64_t left = span == ~0ull ? span : span + 1;
+            const uint64_t nb = std::min<uint64_t>(left, std::min<uint64_t>(batch, bmax));
+            for (uint64_t t = 0; t < nb; ++t) {
+                if (!eager_done || !use_graph) {
+                    iteration(0);
+                    eager_done = true;
+                } else {
+                    if (!graph.ready() && !graph.failed) graph.capture(st, [&] { iteration(0); });
+                    if (graph.ready()) graph.launch(st);
+                    else iteration(0);
+                }
+            }
+            batch *= 2;
+        }
+        *initcost = h.init;
+        *finalcost = h.cost;
+        *numiter = h.iter;
         // outputs
         Out oa = out_dev(c, asn, n * 4, sa_out);
         Out ok = out_dev(c, costs, n * 4, sk_out);
@@ -694,11 +870,9 @@ int mcg_lloyd(mcg_data *ds, int msr, double prior, float *centers, uint64_t k, c
         }
         out_finish(c, oa);
         out_finish(c, ok);
-        HIP_CHECK(hipMemcpyAsync(centers, C, k * d * 4, is_device_ptr(centers) ? hipMemcpyDeviceToDevice
-                                                                               : hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(centers, C, kd * 4, is_device_ptr(centers) ? hipMemcpyDeviceToDevice
+                                                                           : hipMemcpyDeviceToHost, st));
         sync(c);
-        // keep the scratch slots consistent for the next call
-        if (C != c.s[S_CENT_A].as<float>()) { /* buffers are swapped pointers into fixed slots; nothing to do */ }
     });
 }
 
@@ -825,6 +999,10 @@ int mcg_coreset_sample(mcg_ctx *ctx, uint64_t n, const float *probs, const doubl
         out_finish(c, ow);
         sync(c);
     });
+}
+
+int mcg_graphs_enable(mcg_ctx *ctx, int on) {
+    return guard([&] { REQUIRE(ctx, "null ctx"); ctx->c.graphs = on != 0; });
 }
 
 int mcg_tuning_enable(mcg_ctx *ctx, int on) {

@@ -1033,6 +1033,7 @@ static int tuned_form(Data &ds, const float *C, uint64_t k, uint32_t *asn, hipSt
     const uint64_t key = ((uint64_t)ds.dpad << 40) | (cdiv(k, 64) << 8) | lg;
     for (const auto &kv : c.tuned)
         if (kv.first == key) return kv.second;
+    if (stream_capturing(st)) return MCG_PREC_BF16;  // no timing inside a graph capture
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
@@ -1130,7 +1131,9 @@ static void dense_sqrl2_assign_mfma(Data &ds, const float *C, uint64_t k, int pr
     }
     hipLaunchKernelGGL(k_apply_keys, dim3(1024), dim3(256), 0, st, R, asn);
     hipLaunchKernelGGL(k_rescore_full, dim3(256), dim3(256), 0, st, ds.X, d, C, (uint32_t)k, R, asn);
-    HIP_CHECK(hipMemcpyAsync(c.refine_host(), R.cnt + 4, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    // near-tie statistics (not kept for iterations replayed from a graph)
+    if (!stream_capturing(st))
+        HIP_CHECK(hipMemcpyAsync(c.refine_host(), R.cnt + 4, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
 }
 
 }  // namespace mcg

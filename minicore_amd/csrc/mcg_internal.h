@@ -110,6 +110,8 @@ struct Ctx {
     // E-step form chosen per problem shape by timing (see dense_sqrl2_assign)
     bool tuning = true;
     std::vector<std::pair<uint64_t, int>> tuned;
+    // Lloyd iterations after the first replay one captured HIP graph
+    bool graphs = true;
     void tic(int which, hipEvent_t &a);
     void toc(int which, hipEvent_t a);
     void collect();   // after a stream synchronize: fold pending pairs into the totals
@@ -122,7 +124,7 @@ enum Slot {
     S_HIST, S_BASE, S_PERM, S_CNT, S_ITEMOFF, S_PART, S_PARTW, S_PARTQ,
     S_CCOST, S_TOTAL, S_CENT_A, S_CENT_B, S_CENT_C, S_COSTS, S_CT, S_CPARAM,
     S_MISC0, S_MISC1, S_MISC2, S_MISC3, S_W, S_STAGE0, S_STAGE1, S_STAGE2,
-    S_LIS, S_TOT, S_JSDG, S_PAIRS, S_THRESH, S_OVER, S_KEYS, S_OFLAG, S_CINFO, S_NSLOTS
+    S_LIS, S_TOT, S_JSDG, S_PAIRS, S_THRESH, S_OVER, S_KEYS, S_OFLAG, S_CINFO, S_CENT_D, S_LCTL, S_NSLOTS
 };
 static_assert(S_NSLOTS <= 48, "scratch slots");
 
@@ -186,12 +188,17 @@ void restart_draws(Ctx &ctx, const float *costs_d, uint64_t n, uint64_t nempty, 
 void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
 void copy_rows_to_centers(Data &ds, const uint64_t *rows_h, const uint32_t *ids_h, uint64_t m,
                           float *C, hipStream_t st);
-bool any_nan(const float *p, uint64_t n, hipStream_t st, Ctx &ctx);
 void rowsums(Data &ds, hipStream_t st);
 // CSR per-prior transforms
 void csr_transforms(Data &ds, double prior, hipStream_t st);
 
 inline uint64_t cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// true while work launched on st is being captured into a graph
+inline bool stream_capturing(hipStream_t st) {
+    hipStreamCaptureStatus s = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(st, &s) == hipSuccess && s != hipStreamCaptureStatusNone;
+}
 
 // ---- D^2 seeding helpers shared by the seeding kernels -------------------------
 // Block max of |w_p * dist_p| into *mx (bit patterns of non-negative doubles order

@@ -217,3 +217,67 @@ def test_device_resident_input(mcg, ctx, mco):
     a, c = ds.assign(mcg.SQRL2, C)
     a_ref, c_ref, _ = mco.assign_sqrl2_f32(X, C)
     assert np.array_equal(a, a_ref) and np.array_equal(c, c_ref)
+
+
+def _lloyd_runs(mcg, make_ds, msr, C0, prior, w, eps, maxiters, graphs):
+    c = mcg.Context(0)
+    try:
+        c.graphs(graphs)
+        ds = make_ds(c)
+        out = []
+        for t in maxiters:
+            out.append(ds.lloyd(msr, C0, prior=prior, weights=w, eps=eps, maxiter=t))
+            # eager work and synchronisations between the graph replays of two calls
+            ds.assign(msr, C0, prior=prior)
+        ds.close()
+        return out
+    finally:
+        c.close()
+
+
+@pytest.mark.parametrize("kind", ["dense_l2", "dense_l2_weighted", "csr_jsd", "dense_mkl"])
+def test_lloyd_graph_replay_matches_eager(mcg, kind):
+    """The Lloyd loop with its iterations replayed from a captured graph (batches of
+    2, 4, ... per host synchronisation) against the same loop launched kernel by
+    kernel: every output identical for every iteration count, so each replayed
+    iteration reproduces the eager one."""
+    from minicore_amd.datagen import poisson_csr
+    say(f"lloyd graph replay {kind}")
+    w, eps = None, 0.
+    if kind.startswith("dense_l2"):
+        X, C0 = mixture(6000, 40, 12, 21)
+        msr, prior = mcg.SQRL2, 0.
+        if kind.endswith("weighted"):
+            w = np.random.default_rng(3).uniform(0.5, 2., 6000)
+
+        def make_ds(c):
+            return mcg.Dataset.dense(X, c)
+    elif kind == "csr_jsd":
+        n, d = 3000, 500
+        data, idx, ptr = poisson_csr(n, d, 0.03, 9, normalize=True)
+        msr, prior = mcg.JSD, 1. / d
+        dense = np.zeros((n, d), np.float32)
+        for i in range(n):
+            dense[i, idx[ptr[i]:ptr[i + 1]]] = data[ptr[i]:ptr[i + 1]]
+        C0 = dense[np.random.default_rng(4).choice(n, 8, replace=False)].copy()
+
+        def make_ds(c):
+            return mcg.Dataset.csr(data, idx, ptr, (n, d), c)
+    else:
+        rng = np.random.default_rng(5)
+        X = rng.gamma(2., 1., (4000, 30)).astype(np.float32)
+        X /= X.sum(1, keepdims=True)
+        msr, prior = mcg.MKL, 1e-3
+        C0 = X[rng.choice(4000, 7, replace=False)].copy()
+
+        def make_ds(c):
+            return mcg.Dataset.dense(X, c)
+    maxiters = [0, 1, 2, 3, 5, 9, 40]
+    eager = _lloyd_runs(mcg, make_ds, msr, C0, prior, w, eps, maxiters, False)
+    replay = _lloyd_runs(mcg, make_ds, msr, C0, prior, w, eps, maxiters, True)
+    for t, a, b in zip(maxiters, eager, replay):
+        assert a["numiter"] == b["numiter"], t
+        assert a["initcost"] == b["initcost"] and a["finalcost"] == b["finalcost"], t
+        for key in ("centers", "asn", "costs"):
+            assert np.array_equal(a[key], b[key]), (t, key)
+    assert eager[-1]["numiter"] >= 3

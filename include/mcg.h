@@ -86,10 +86,13 @@ int mcg_timing_enable(mcg_ctx *ctx, int on);
 int mcg_timing_reset(mcg_ctx *ctx);
 int mcg_timing_read(mcg_ctx *ctx, int which, double *total_ms, uint64_t *count);
 
-/* Graph replay of mcg_lloyd's iterations (on by default): the iterations after the
+/* Graph replay of mcg_lloyd's iterations (off by default): the iterations after the
  * first are captured once into a HIP graph and replayed, in batches with one host
  * synchronisation each.  Off: every iteration is launched kernel by kernel (same
- * batches, same results).  (No counterpart in the reference.) */
+ * batches, same results).  While a capture is open the HIP runtime fails
+ * synchronous legacy-stream calls (hipMemcpy, hipMemset) made by any other thread of
+ * the process, so enable it only where no other thread makes such calls.  (No
+ * counterpart in the reference.) */
 int mcg_graphs_enable(mcg_ctx *ctx, int on);
 
 /* Tuning of the BF16-refined dense SQRL2 E-step (on by default): the BF16 MFMA

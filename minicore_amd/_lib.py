@@ -177,7 +177,9 @@ class Context:
         check(load().mcg_tuning_enable(self.h, 1 if on else 0))
 
     def graphs(self, on=True):
-        """Graph replay of the Lloyd iterations (on by default; results are the same)."""
+        """Graph replay of the Lloyd iterations (off by default; results are the same).
+        While a capture is open, synchronous legacy-stream HIP calls from other threads
+        of the process fail, so enable it only where no other thread makes them."""
         check(load().mcg_graphs_enable(self.h, 1 if on else 0))
 
     def timing(self, on=True):
@@ -206,6 +208,8 @@ def default_context():
 
 
 def _dtype_code(dt):
+    if str(dt).startswith("torch."):
+        dt = str(dt)[6:]
     dt = np.dtype(dt)
     table = {np.float32: MCG_F32, np.float64: MCG_F64, np.uint8: MCG_U8, np.uint16: MCG_U16,
              np.uint32: MCG_U32, np.uint64: MCG_U64, np.int32: MCG_I32, np.int64: MCG_I64}
@@ -245,11 +249,18 @@ class Dataset:
 
     @classmethod
     def csr(cls, data, indices, indptr, shape, ctx=None):
+        """CSR rows from numpy arrays or contiguous torch tensors (host or device), any
+        accepted dtype; converted and checked on the device."""
         ctx = ctx or default_context()
         h = C.c_void_p()
-        data = np.ascontiguousarray(data)
-        indices = np.ascontiguousarray(indices)
-        indptr = np.ascontiguousarray(indptr)
+
+        def arr(a):
+            if hasattr(a, "data_ptr"):
+                if not a.is_contiguous():
+                    raise ValueError("CSR tensors must be contiguous")
+                return a
+            return np.ascontiguousarray(a)
+        data, indices, indptr = arr(data), arr(indices), arr(indptr)
         n, d = int(shape[0]), int(shape[1])
         check(load().mcg_data_csr(ctx.h, ptr(data), _dtype_code(data.dtype), ptr(indices),
                                   _dtype_code(indices.dtype), ptr(indptr), _dtype_code(indptr.dtype),

@@ -136,33 +136,8 @@ static void sync(Ctx &c) {
 }
 
 // ---------------------------------------------------------------------------
-// element-type conversion on the host
+// element types
 // ---------------------------------------------------------------------------
-template <class T>
-static double rd(const void *p, uint64_t i) { return (double)((const T *)p)[i]; }
-static double read_num(const void *p, int dt, uint64_t i) {
-    switch (dt) {
-    case MCG_F32: return rd<float>(p, i);
-    case MCG_F64: return rd<double>(p, i);
-    case MCG_U8: return rd<uint8_t>(p, i);
-    case MCG_U16: return rd<uint16_t>(p, i);
-    case MCG_U32: return rd<uint32_t>(p, i);
-    case MCG_U64: return rd<uint64_t>(p, i);
-    case MCG_I32: return rd<int32_t>(p, i);
-    case MCG_I64: return rd<int64_t>(p, i);
-    }
-    MCG_THROW(MCG_EINVAL, "unknown dtype");
-}
-static uint64_t read_idx(const void *p, int dt, uint64_t i) {
-    switch (dt) {
-    case MCG_U16: return ((const uint16_t *)p)[i];
-    case MCG_U32: return ((const uint32_t *)p)[i];
-    case MCG_U64: return ((const uint64_t *)p)[i];
-    case MCG_I32: { int32_t v = ((const int32_t *)p)[i]; REQUIRE(v >= 0, "negative index"); return (uint64_t)v; }
-    case MCG_I64: { int64_t v = ((const int64_t *)p)[i]; REQUIRE(v >= 0, "negative index"); return (uint64_t)v; }
-    }
-    MCG_THROW(MCG_EINVAL, "unsupported index dtype");
-}
 static size_t dt_size(int dt) {
     switch (dt) {
     case MCG_F32: case MCG_U32: case MCG_I32: return 4;
@@ -171,6 +146,63 @@ static size_t dt_size(int dt) {
     case MCG_U8: return 1;
     }
     MCG_THROW(MCG_EINVAL, "unknown dtype");
+}
+
+// ---------------------------------------------------------------------------
+// CSR ingest on the device: the caller's dtypes converted to f32 values, u32
+// column indices and u64 offsets, with the checks of the CSR invariants
+// (err[0]: first/last offset, err[1]: decreasing offsets, err[2]: bit 0 column
+// out of range, bit 1 negative index, err[3]: columns not increasing in a row)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double rd_num(const void *p, int dt, uint64_t i) {
+    switch (dt) {
+    case MCG_F32: return ((const float *)p)[i];
+    case MCG_F64: return ((const double *)p)[i];
+    case MCG_U8: return ((const uint8_t *)p)[i];
+    case MCG_U16: return ((const uint16_t *)p)[i];
+    case MCG_U32: return ((const uint32_t *)p)[i];
+    case MCG_U64: return (double)((const uint64_t *)p)[i];
+    case MCG_I32: return ((const int32_t *)p)[i];
+    default: return (double)((const int64_t *)p)[i];
+    }
+}
+
+// index as signed 64-bit (u64 values beyond 2^63 read as negative, i.e. invalid)
+__device__ __forceinline__ int64_t rd_idx(const void *p, int dt, uint64_t i) {
+    switch (dt) {
+    case MCG_U16: return ((const uint16_t *)p)[i];
+    case MCG_U32: return ((const uint32_t *)p)[i];
+    case MCG_U64: return (int64_t)((const uint64_t *)p)[i];
+    case MCG_I32: return ((const int32_t *)p)[i];
+    default: return ((const int64_t *)p)[i];
+    }
+}
+
+__global__ void k_csr_ptr(const void *__restrict__ raw, int dt, uint64_t n, uint64_t nnz, uint64_t *__restrict__ P,
+                          uint8_t *__restrict__ start, uint32_t *__restrict__ err) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const int64_t v = rd_idx(raw, dt, i);
+        P[i] = (uint64_t)v;
+        if ((i == 0 && v != 0) || (i == n && (uint64_t)v != nnz)) err[0] = 1u;
+        if (i < n) {
+            const int64_t nx = rd_idx(raw, dt, i + 1);
+            if (v < 0 || nx < v) err[1] = 1u;
+            else if (nx > v && (uint64_t)nx <= nnz) start[v] = 1;   // first entry of a non-empty row
+        }
+    }
+}
+
+__global__ void k_csr_entries(const void *__restrict__ rv, int vdt, const void *__restrict__ ri, int idt, uint64_t nnz,
+                              uint32_t d, const uint8_t *__restrict__ start, float *__restrict__ val,
+                              uint32_t *__restrict__ idx, uint32_t *__restrict__ err) {
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (uint64_t)gridDim.x * blockDim.x) {
+        const int64_t j = rd_idx(ri, idt, p);
+        if (j < 0) atomicOr(&err[2], 2u);
+        else if ((uint64_t)j >= d) atomicOr(&err[2], 1u);
+        idx[p] = (uint32_t)j;
+        val[p] = (float)rd_num(rv, vdt, p);
+        if (p > 0 && !start[p] && rd_idx(ri, idt, p - 1) >= j) err[3] = 1u;
+    }
 }
 
 __global__ void k_f64_to_f32(const double *__restrict__ in, uint64_t n, uint64_t d, uint64_t ld,
@@ -541,8 +573,20 @@ int mcg_data_dense(mcg_ctx *ctx, const void *x, int dtype, uint64_t n, uint64_t 
         } else {
             D.X = (float *)D.Xown.reserve(n * d * sizeof(float));
             if (dtype == MCG_F32) {
-                HIP_CHECK(hipMemcpy2DAsync(D.X, d * 4, x, ld * 4, d * 4, n,
-                                           dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c.stream));
+                // host rows travel as one contiguous copy (a pitched copy from pageable
+                // memory goes row by row); a pitched source is compacted on the device
+                DBuf tmp;
+                const void *src = x;
+                if (!dev && ld != d) {
+                    tmp.reserve(n * ld * 4);
+                    HIP_CHECK(hipMemcpyAsync(tmp.p, x, n * ld * 4, hipMemcpyHostToDevice, c.stream));
+                    src = tmp.p;
+                }
+                if (!dev && ld == d)
+                    HIP_CHECK(hipMemcpyAsync(D.X, x, n * d * 4, hipMemcpyHostToDevice, c.stream));
+                else
+                    HIP_CHECK(hipMemcpy2DAsync(D.X, d * 4, src, ld * 4, d * 4, n, hipMemcpyDeviceToDevice, c.stream));
+                sync(c);
             } else {
                 DBuf tmp;
                 const void *src = x;
@@ -567,41 +611,22 @@ int mcg_data_csr(mcg_ctx *ctx, const void *data, int data_dtype, const void *ind
         REQUIRE(ctx && out && indptr, "null argument");
         REQUIRE(n > 0 && d > 0, "empty matrix");
         REQUIRE(d < (1ull << 32), "d must fit in uint32");
+        REQUIRE(nnz == 0 || (data && indices), "null argument");
+        REQUIRE(nnz < (1ull << 40), "nnz too large");
+        (void)dt_size(data_dtype);
+        REQUIRE(idx_dtype == MCG_U16 || idx_dtype == MCG_U32 || idx_dtype == MCG_U64 || idx_dtype == MCG_I32 ||
+                    idx_dtype == MCG_I64, "unsupported index dtype");
+        REQUIRE(ptr_dtype == MCG_U16 || ptr_dtype == MCG_U32 || ptr_dtype == MCG_U64 || ptr_dtype == MCG_I32 ||
+                    ptr_dtype == MCG_I64, "unsupported index dtype");
         Ctx &c = ctx->c;
         HIP_CHECK(hipSetDevice(c.device));
-        // bring the three arrays to the host (they are small next to the work)
-        auto host_copy = [&](const void *p, size_t bytes, std::vector<char> &buf) -> const void * {
-            if (!p) return nullptr;
-            if (!is_device_ptr(p)) return p;
-            buf.resize(bytes);
-            HIP_CHECK(hipMemcpy(buf.data(), p, bytes, hipMemcpyDeviceToHost));
-            return buf.data();
-        };
-        std::vector<char> b0, b1, b2;
-        const void *hp = host_copy(indptr, (n + 1) * dt_size(ptr_dtype), b2);
-        const void *hd = host_copy(data, nnz * dt_size(data_dtype), b0);
-        const void *hi = host_copy(indices, nnz * dt_size(idx_dtype), b1);
-        std::vector<uint64_t> P(n + 1);
-        for (uint64_t i = 0; i <= n; ++i) P[i] = read_idx(hp, ptr_dtype, i);
-        REQUIRE(P[0] == 0 && P[n] == nnz, "indptr must start at 0 and end at nnz");
-        for (uint64_t i = 0; i < n; ++i) REQUIRE(P[i] <= P[i + 1], "indptr must be non-decreasing");
-        std::vector<float> V(nnz);
-        std::vector<uint32_t> J(nnz);
-        if (data_dtype == MCG_F32 && nnz) std::memcpy(V.data(), hd, nnz * 4);
-        else for (uint64_t p = 0; p < nnz; ++p) V[p] = (float)read_num(hd, data_dtype, p);
-        if (idx_dtype == MCG_U32 && nnz) {
-            std::memcpy(J.data(), hi, nnz * 4);
-            for (uint64_t p = 0; p < nnz; ++p) REQUIRE(J[p] < d, "column index out of range");
-        } else {
-            for (uint64_t p = 0; p < nnz; ++p) {
-                uint64_t j = read_idx(hi, idx_dtype, p);
-                REQUIRE(j < d, "column index out of range");
-                J[p] = (uint32_t)j;
-            }
-        }
-        for (uint64_t i = 0; i < n; ++i)
-            for (uint64_t p = P[i] + 1; p < P[i + 1]; ++p)
-                REQUIRE(J[p - 1] < J[p], "column indices must be sorted and unique within a row");
+        hipStream_t st = c.stream;
+        // the caller's arrays as they are (device memory borrowed, host memory staged),
+        // converted and checked on the device
+        DBuf s0, s1, s2, flags;
+        const void *rp = in_dev(c, indptr, (n + 1) * dt_size(ptr_dtype), s2);
+        const void *rv = nnz ? in_dev(c, data, nnz * dt_size(data_dtype), s0) : nullptr;
+        const void *ri = nnz ? in_dev(c, indices, nnz * dt_size(idx_dtype), s1) : nullptr;
         std::unique_ptr<mcg_data> ds(new mcg_data);
         Data &D = ds->d;
         D.ctx = &c;
@@ -612,11 +637,23 @@ int mcg_data_csr(mcg_ctx *ctx, const void *data, int data_dtype, const void *ind
         D.val.reserve(nnz * 4);
         D.idx.reserve(nnz * 4);
         D.ptr.reserve((n + 1) * 8);
-        if (nnz) {
-            HIP_CHECK(hipMemcpy(D.val.p, V.data(), nnz * 4, hipMemcpyHostToDevice));
-            HIP_CHECK(hipMemcpy(D.idx.p, J.data(), nnz * 4, hipMemcpyHostToDevice));
-        }
-        HIP_CHECK(hipMemcpy(D.ptr.p, P.data(), (n + 1) * 8, hipMemcpyHostToDevice));
+        const uint64_t words = 4 + cdiv(nnz, 4);   // 4 error words, then one start byte per entry
+        uint32_t *err = (uint32_t *)flags.reserve(words * 4);
+        uint8_t *start = (uint8_t *)(err + 4);
+        fill_u32(err, 0u, words, st);
+        hipLaunchKernelGGL(k_csr_ptr, dim3((uint32_t)std::min<uint64_t>(cdiv(n + 1, 256), 65536)), dim3(256), 0, st, rp,
+                           ptr_dtype, n, nnz, D.ptr.as<uint64_t>(), start, err);
+        if (nnz)
+            hipLaunchKernelGGL(k_csr_entries, dim3((uint32_t)std::min<uint64_t>(cdiv(nnz, 256), 65536)), dim3(256), 0,
+                               st, rv, data_dtype, ri, idx_dtype, nnz, (uint32_t)d, start, D.val.as<float>(),
+                               D.idx.as<uint32_t>(), err);
+        uint32_t h[4];
+        copy_sync(h, err, sizeof(h), hipMemcpyDeviceToHost, st);
+        REQUIRE(!h[0], "indptr must start at 0 and end at nnz");
+        REQUIRE(!h[1], "indptr must be non-decreasing");
+        REQUIRE(!(h[2] & 2u), "negative index");
+        REQUIRE(!(h[2] & 1u), "column index out of range");
+        REQUIRE(!h[3], "column indices must be sorted and unique within a row");
         *out = ds.release();
     });
 }
@@ -692,7 +729,7 @@ int mcg_centroids_hard(mcg_data *ds, const uint32_t *asn, const double *weights,
         {
             std::vector<uint32_t> h(D.n);
             if (is_device_ptr(asn)) {
-                HIP_CHECK(hipMemcpy(h.data(), asn, D.n * 4, hipMemcpyDeviceToHost));
+                copy_sync(h.data(), asn, D.n * 4, hipMemcpyDeviceToHost, c.stream);
             } else {
                 std::memcpy(h.data(), asn, D.n * 4);
             }
@@ -723,10 +760,11 @@ int mcg_centroids_hard(mcg_data *ds, const uint32_t *asn, const double *weights,
 // The stop / accept rules run on the device (k_lloyd_decide), so iterations are
 // queued in batches (2, 4, ... up to a size set by the work per iteration) with one
 // host synchronisation per batch; iterations queued after the loop has stopped
-// compute into the candidate buffers and change nothing.  The iterations after the
-// first replay one captured HIP graph.  An accepted assignment that leaves a
-// cluster empty hands the next iteration to the host, which runs the reservoir
-// restart of centroid.h:687-752 and then goes back to device batches.
+// compute into the candidate buffers and change nothing.  With graphs enabled on
+// the context the iterations after the first replay one captured HIP graph.  An
+// accepted assignment that leaves a cluster empty hands the next iteration to the
+// host, which runs the reservoir restart of centroid.h:687-752 and then goes back to
+// device batches.
 int mcg_lloyd(mcg_data *ds, int msr, double prior, float *centers, uint64_t k, const double *weights, double eps,
               uint64_t maxiter, int precision, uint32_t *asn, float *costs, double *initcost, double *finalcost,
               uint64_t *numiter) {

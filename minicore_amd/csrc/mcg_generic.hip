@@ -120,21 +120,27 @@ __device__ __forceinline__ float jsd_h(float a, float b) {
 }
 
 // G[u][c] = sum_j h(li_u, c~_j), FP64 accumulation.  One wave per (u, c).
-__global__ void k_jsd_g(const float *__restrict__ CT, uint32_t kpad, uint32_t k, uint32_t d,
-                        const float *__restrict__ lis, uint32_t U, double *__restrict__ G) {
-    const uint64_t wid = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (wid >= (uint64_t)U * k) return;
-    const uint32_t u = (uint32_t)(wid / k), c = (uint32_t)(wid % k);
-    const double l = lis[u];
-    double s = 0.;
-    for (uint32_t j = lane; j < d; j += 64) {
-        const double b = CT[(uint64_t)j * kpad + c];
-        const double m = l + b;
-        s += 0.5 * (l * log(2. * l / m) + b * log(2. * b / m));
+__global__ __launch_bounds__(256) void k_jsd_g(const float *__restrict__ CT, uint32_t kpad, uint32_t k, uint32_t d,
+                                               const float *__restrict__ lis, uint32_t U, double *__restrict__ G) {
+    // one block per (u, c), blocks striding over the pairs; lanes over the columns,
+    // then the four wave sums in a fixed order
+    __shared__ double ws[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint64_t pid = blockIdx.x; pid < (uint64_t)U * k; pid += gridDim.x) {
+        const uint32_t u = (uint32_t)(pid / k), c = (uint32_t)(pid % k);
+        const double l = lis[u];
+        double s = 0.;
+        for (uint32_t j = threadIdx.x; j < d; j += 256) {
+            const double b = CT[(uint64_t)j * kpad + c];
+            const double m = l + b;
+            s += 0.5 * (l * log(2. * l / m) + b * log(2. * b / m));
+        }
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (lane == 0) ws[wave] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) G[(uint64_t)u * kpad + c] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
+        __syncthreads();
     }
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (lane == 0) G[(uint64_t)u * kpad + c] = s;
 }
 
 __device__ __forceinline__ float post_kl(float r) {
@@ -269,43 +275,66 @@ struct Cost1 {
     const uint32_t *slot;
 };
 
+// Cost of one row against the single center of P, in two parts: the entry sum over
+// [b, e) by a group of GS lanes (gl = the lane's index in its group; groups are
+// aligned, so the reduction stays inside; four entries per lane are loaded before
+// they are used, and added in entry order), then the per-row and per-center terms
+// (r = the row's {L, log L, H_x, li}, g = JSD's G of the row's li).
+template <int MSR, int GS>
+__device__ __forceinline__ float row_sum1(const Cost1 &P, uint64_t b, uint64_t e, float4 r, int gl) {
+    const float Li = 1.f / r.x, li = r.w;
+    const float hli = li * __logf(li);
+    float acc = 0.f;
+    for (uint64_t p0 = b + gl; p0 < e; p0 += 4 * GS) {
+        float x[4], ct[4];
+        uint32_t j[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint64_t p = p0 + (uint64_t)u * GS;
+            const bool ok = p < e;
+            x[u] = ok ? (P.val ? P.val[p] : P.X[p]) : 0.f;
+            j[u] = ok ? (P.val ? P.idx[p] : (uint32_t)(p - b)) : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) ct[u] = P.CT[(uint64_t)j[u] * P.kpad];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (p0 + (uint64_t)u * GS >= e) break;
+            const float xv = x[u], c = ct[u];
+            if (MSR == MCG_SQRL2) acc = fmaf(xv, xv - 2.f * c, acc);
+            else if (MSR == MCG_MKL) acc = fmaf(log1pf(xv / P.pv), c, acc);
+            else if (MSR == MCG_REVERSE_MKL) acc = fmaf(xv, c, acc);
+            else {
+                const float xt = (xv + P.pv) * Li;
+                const float m1 = xt + c, m0 = li + c;
+                acc += 0.5f * ((xt * __logf(xt) - hli) - m1 * __logf(0.5f * m1) + m0 * __logf(0.5f * m0));
+            }
+        }
+    }
+    for (int o = GS / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    return acc;
+}
+
 template <int MSR>
+__device__ __forceinline__ float finish1(const Cost1 &P, float acc, float4 r, double g) {
+    const float logL = r.y, Hx = r.z, li = r.w, Li = 1.f / r.x;
+    if (MSR == MCG_SQRL2) {
+        const float v = (float)(P.cs[0] + (double)acc);
+        return v < 0.f ? 0.f : v;
+    }
+    if (MSR == MCG_MKL) return post_kl((float)(P.cs[0] - P.cs[1] * (double)(P.logpv - logL) - (double)acc));
+    if (MSR == MCG_REVERSE_MKL) return post_kl((float)((double)Hx - (double)li * P.cs[0] - (double)acc * (double)Li));
+    return post_kl((float)(g + (double)acc));
+}
+
+template <int MSR, int GS = 64>
 __device__ __forceinline__ float row_cost1(const Cost1 &P, uint64_t row, int lane) {
     uint64_t b, e;
     if (P.val) { b = P.ptr[row]; e = P.ptr[row + 1]; }
     else { b = row * P.d; e = b + P.d; }
-    float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (MSR != MCG_SQRL2) r = P.rt[row];
-    const float L = r.x, logL = r.y, Hx = r.z, li = r.w;
-    const float Li = 1.f / L;
-    const float hli = li * __logf(li);
-    float acc = 0.f;
-    for (uint64_t p = b + lane; p < e; p += 64) {
-        const float x = P.val ? P.val[p] : P.X[p];
-        const uint32_t j = P.val ? P.idx[p] : (uint32_t)(p - b);
-        const float ct = P.CT[(uint64_t)j * P.kpad];
-        if (MSR == MCG_SQRL2) acc = fmaf(x, x - 2.f * ct, acc);
-        else if (MSR == MCG_MKL) acc = fmaf(log1pf(x / P.pv), ct, acc);
-        else if (MSR == MCG_REVERSE_MKL) acc = fmaf(x, ct, acc);
-        else {
-            const float xt = (x + P.pv) * Li;
-            const float m1 = xt + ct, m0 = li + ct;
-            acc += 0.5f * ((xt * __logf(xt) - hli) - m1 * __logf(0.5f * m1) + m0 * __logf(0.5f * m0));
-        }
-    }
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    float v;
-    if (MSR == MCG_SQRL2) {
-        v = (float)(P.cs[0] + (double)acc);
-        v = v < 0.f ? 0.f : v;
-    } else if (MSR == MCG_MKL) {
-        v = post_kl((float)(P.cs[0] - P.cs[1] * (double)(P.logpv - logL) - (double)acc));
-    } else if (MSR == MCG_REVERSE_MKL) {
-        v = post_kl((float)((double)Hx - (double)li * P.cs[0] - (double)acc * (double)Li));
-    } else {
-        v = post_kl((float)(P.G[(uint64_t)P.slot[row] * P.kpad] + (double)acc));
-    }
-    return v;
+    const float4 r = MSR != MCG_SQRL2 ? P.rt[row] : make_float4(1.f, 0.f, 0.f, 0.f);
+    const double g = MSR == MCG_JSD ? P.G[(uint64_t)P.slot[row] * P.kpad] : 0.;
+    return finish1<MSR>(P, row_sum1<MSR, GS>(P, b, e, r, lane), r, g);
 }
 
 // every row against one center (k = 1): asn = 0, best = the cost
@@ -332,34 +361,66 @@ __global__ __launch_bounds__(256) void k_pp_cc_gen(Cost1 P, const uint32_t *__re
     if (lane == 0) cc[a] = v;
 }
 
-// D^2 seeding update, one wave per row (kmeans.h:189-204).  With PRUNE (JSD, whose
-// square root is a metric on the smoothed distributions): a row whose center is
-// far from the new one, JSD(c_a, c_new) > 4 dist (1 + 2^-6) + 2^-17, cannot get
-// closer (triangle inequality for sqrt(JSD); the margins cover the FP32 evaluation
-// of both costs), so its entries are not read and nothing changes.
-template <int MSR, bool PRUNE>
+// D^2 seeding update (kmeans.h:189-204).  A wave takes 64 rows at a time: each lane
+// tests its row for pruning and loads its row's extent and per-row terms, then the
+// rows still needed are costed by groups of GS lanes (64 / GS rows at once).  With
+// PRUNE (JSD, whose square root is a metric on the smoothed distributions): a row
+// whose center is far from the new one, JSD(c_a, c_new) > 4 dist (1 + 2^-6) + 2^-17,
+// cannot get closer (triangle inequality for sqrt(JSD); the margins cover the FP32
+// evaluation of both costs), so its entries are not read and nothing changes.
+template <int MSR, bool PRUNE, int GS>
 __global__ __launch_bounds__(256) void k_pp_update_gen(Cost1 P, uint64_t n, const uint64_t *__restrict__ newc_p,
                                                        uint32_t ci, const float *__restrict__ cc,
                                                        float *__restrict__ dist, uint32_t *__restrict__ asn,
                                                        int first, const double *__restrict__ w,
                                                        unsigned long long *__restrict__ mx) {
-    // waves stride over the rows; one block-wide maximum and one atomic per block
+    // waves stride over tiles of 64 rows; one block-wide maximum and one atomic per block
     const uint64_t nwaves = (uint64_t)gridDim.x * 4;
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, g = lane / GS, gl = lane % GS;
     const uint64_t newc = *newc_p;
     double a = 0.;
-    for (uint64_t row = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); row < n; row += nwaves) {
-        bool skip = false;
-        float v = 0.f;
-        if (PRUNE && !first && row != newc) {
-            v = dist[row];
-            skip = !(v > 0.f) || cc[asn[row]] > 4.f * v * (1.f + 1.f / 64.f) + 0x1p-17f;
+    for (uint64_t base = ((uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64; base < n; base += nwaves * 64) {
+        const uint64_t row = base + lane;
+        bool need = row < n;
+        if (PRUNE && need && !first && row != newc) {
+            const float v = dist[row];
+            need = v > 0.f && !(cc[asn[row]] > 4.f * v * (1.f + 1.f / 64.f) + 0x1p-17f);
+            if (!need) a = fmax(a, pp_weighted(v, w, row));   // unchanged
         }
-        if (!skip) {
-            const float s = row_cost1<MSR>(P, row, lane);
-            if (lane == 0) v = pp_new_dist(row, newc, s, ci, first, dist, asn);
+        // the lane's row: entry range, per-row terms, JSD's G
+        uint64_t rb = 0, re = 0;
+        float4 rt = make_float4(1.f, 0.f, 0.f, 0.f);
+        double rg = 0.;
+        if (need) {
+            if (P.val) { rb = P.ptr[row]; re = P.ptr[row + 1]; }
+            else { rb = row * P.d; re = rb + P.d; }
+            if (MSR != MCG_SQRL2) rt = P.rt[row];
+            if (MSR == MCG_JSD) rg = P.G[(uint64_t)P.slot[row] * P.kpad];
         }
-        if (lane == 0) a = fmax(a, pp_weighted(v, w, row));
+        uint64_t mask = __ballot(need);
+        while (mask) {
+            // group g takes the g-th lowest row left in mask
+            uint64_t m = mask;
+            for (int q = 0; q < g; ++q) m &= m - 1;
+            for (int q = 0; q < 64 / GS; ++q) mask &= mask - 1;
+            const bool has = m != 0;
+            const int src = has ? __ffsll((unsigned long long)m) - 1 : lane;
+            // every lane shuffles (uniform), groups without a row get an empty range
+            const uint64_t b = __shfl(rb, src), e0 = __shfl(re, src);
+            const uint64_t e = has ? e0 : b;
+            float4 r;
+            r.x = __shfl(rt.x, src);
+            r.y = __shfl(rt.y, src);
+            r.z = __shfl(rt.z, src);
+            r.w = __shfl(rt.w, src);
+            const double gg = __shfl(rg, src);
+            const float acc = row_sum1<MSR, GS>(P, b, e, r, gl);
+            if (has && gl == 0) {
+                const uint64_t rr = base + (uint64_t)src;
+                a = fmax(a, pp_weighted(pp_new_dist(rr, newc, finish1<MSR>(P, acc, r, gg), ci, first, dist, asn), w,
+                                        rr));
+            }
+        }
     }
     block_max_into(a, mx);
 }
@@ -462,18 +523,20 @@ void csr_transforms(Data &ds, double pv, hipStream_t st) {
         sl[i] = (uint32_t)(std::lower_bound(u.begin(), u.end(), lis[i]) - u.begin());
     ds.li_unique = u;
     ds.li_dev.reserve(u.size() * sizeof(float));
-    HIP_CHECK(hipMemcpy(ds.li_dev.p, u.data(), u.size() * sizeof(float), hipMemcpyHostToDevice));
+    copy_sync(ds.li_dev.p, u.data(), u.size() * sizeof(float), hipMemcpyHostToDevice, st);
     ds.li_slot.reserve(ds.n * 4);
-    HIP_CHECK(hipMemcpy(ds.li_slot.p, sl.data(), ds.n * 4, hipMemcpyHostToDevice));
+    copy_sync(ds.li_slot.p, sl.data(), ds.n * 4, hipMemcpyHostToDevice, st);
     ds.tf_prior = pv;
 }
 
 // Per-center tables (CT d x kpad, per-center scalars, JSD's G per distinct li) and
 // the per-row terms for k centers C; returns them with the row arrays of ds.
-static Cost1 center_tables(Data &ds, int msr, double pv, const float *C, uint64_t k, hipStream_t st) {
+// Center tables with row stride kpad (>= k): a multiple of 64 for the kernels whose
+// lanes run over centers, 1 for the single-center kernels (a dense table of d values).
+static Cost1 center_tables(Data &ds, int msr, double pv, const float *C, uint64_t k, uint32_t kpad,
+                           hipStream_t st) {
     Ctx &c = *ds.ctx;
     const uint32_t d = (uint32_t)ds.d;
-    const uint32_t kpad = (uint32_t)cdiv(k, 64) * 64;
     Cost1 P{};
     P.X = ds.csr ? nullptr : ds.X;
     P.val = ds.csr ? ds.val.as<const float>() : nullptr;
@@ -496,7 +559,7 @@ static Cost1 center_tables(Data &ds, int msr, double pv, const float *C, uint64_
         const uint32_t U = (uint32_t)ds.li_unique.size();
         REQUIRE(U <= (1u << 20), "JSD: too many distinct row sums");
         double *Gb = (double *)c.s[S_JSDG].reserve((uint64_t)U * kpad * sizeof(double));
-        hipLaunchKernelGGL(k_jsd_g, dim3((uint32_t)cdiv((uint64_t)U * k, 4)), dim3(256), 0, st, CT, kpad,
+        hipLaunchKernelGGL(k_jsd_g, dim3((uint32_t)std::min<uint64_t>((uint64_t)U * k, 65536)), dim3(256), 0, st, CT, kpad,
                            (uint32_t)k, d, ds.li_dev.as<const float>(), U, Gb);
         P.G = Gb;
         P.slot = ds.li_slot.as<const uint32_t>();
@@ -504,38 +567,48 @@ static Cost1 center_tables(Data &ds, int msr, double pv, const float *C, uint64_
     return P;
 }
 
+template <int GS>
+static void pp_update_launch(const Cost1 &P, uint64_t n, int msr, const uint64_t *newc, uint32_t ci, const float *cc,
+                             float *dist, uint32_t *asn, int first, const double *w, unsigned long long *mx,
+                             hipStream_t st) {
+    const dim3 grid((uint32_t)std::min<uint64_t>(cdiv(n, 256), 2048)), blk(256);
+    if (msr == MCG_JSD && !first && ci > 0) {
+        hipLaunchKernelGGL((k_pp_update_gen<MCG_JSD, true, GS>), grid, blk, 0, st, P, n, newc, ci, cc, dist, asn,
+                           first, w, mx);
+        return;
+    }
+    switch (msr) {
+    case MCG_SQRL2:
+        hipLaunchKernelGGL((k_pp_update_gen<MCG_SQRL2, false, GS>), grid, blk, 0, st, P, n, newc, ci, cc, dist, asn,
+                           first, w, mx);
+        break;
+    case MCG_MKL:
+        hipLaunchKernelGGL((k_pp_update_gen<MCG_MKL, false, GS>), grid, blk, 0, st, P, n, newc, ci, cc, dist, asn,
+                           first, w, mx);
+        break;
+    case MCG_REVERSE_MKL:
+        hipLaunchKernelGGL((k_pp_update_gen<MCG_REVERSE_MKL, false, GS>), grid, blk, 0, st, P, n, newc, ci, cc, dist,
+                           asn, first, w, mx);
+        break;
+    default:
+        hipLaunchKernelGGL((k_pp_update_gen<MCG_JSD, false, GS>), grid, blk, 0, st, P, n, newc, ci, cc, dist, asn,
+                           first, w, mx);
+        break;
+    }
+}
+
 bool generic_pp_round(Data &ds, int msr, double prior, const float *center_row, const uint32_t *cent, uint32_t ci,
                       const uint64_t *newc, float *cc, float *dist, uint32_t *asn, int first, const double *w,
                       unsigned long long *mx, hipStream_t st) {
     if (msr != MCG_SQRL2 && prior < DIRECT_PRIOR) return false;
     const double pv = msr == MCG_SQRL2 ? 0. : prior;
-    const Cost1 P = center_tables(ds, msr, pv, center_row, 1, st);
+    const Cost1 P = center_tables(ds, msr, pv, center_row, 1, 1, st);
     const uint64_t n = ds.n;
-    const dim3 grid((uint32_t)std::min<uint64_t>(cdiv(n, 4), 2048)), blk(256);
-    if (msr == MCG_JSD && !first && ci > 0) {
+    if (msr == MCG_JSD && !first && ci > 0)
         hipLaunchKernelGGL(k_pp_cc_gen<MCG_JSD>, dim3((uint32_t)cdiv(ci, 4)), dim3(256), 0, st, P, cent, ci, cc);
-        hipLaunchKernelGGL((k_pp_update_gen<MCG_JSD, true>), grid, blk, 0, st, P, n, newc, ci, (const float *)cc,
-                           dist, asn, first, w, mx);
-        return true;
-    }
-    switch (msr) {
-    case MCG_SQRL2:
-        hipLaunchKernelGGL((k_pp_update_gen<MCG_SQRL2, false>), grid, blk, 0, st, P, n, newc, ci, (const float *)cc,
-                           dist, asn, first, w, mx);
-        break;
-    case MCG_MKL:
-        hipLaunchKernelGGL((k_pp_update_gen<MCG_MKL, false>), grid, blk, 0, st, P, n, newc, ci, (const float *)cc,
-                           dist, asn, first, w, mx);
-        break;
-    case MCG_REVERSE_MKL:
-        hipLaunchKernelGGL((k_pp_update_gen<MCG_REVERSE_MKL, false>), grid, blk, 0, st, P, n, newc, ci,
-                           (const float *)cc, dist, asn, first, w, mx);
-        break;
-    default:
-        hipLaunchKernelGGL((k_pp_update_gen<MCG_JSD, false>), grid, blk, 0, st, P, n, newc, ci, (const float *)cc,
-                           dist, asn, first, w, mx);
-        break;
-    }
+    // sparse rows: 8 lanes per row, 8 rows per wave at once; dense rows: a wave per row
+    if (ds.csr) pp_update_launch<8>(P, n, msr, newc, ci, cc, dist, asn, first, w, mx, st);
+    else pp_update_launch<64>(P, n, msr, newc, ci, cc, dist, asn, first, w, mx, st);
     return true;
 }
 
@@ -558,15 +631,16 @@ void generic_costs(Data &ds, int msr, double prior, const float *C, uint64_t k, 
         return;
     }
     const double pv = msr == MCG_SQRL2 ? 0. : prior;
-    const uint32_t kpad = (uint32_t)cdiv(k, 64) * 64;
-    const Cost1 T = center_tables(ds, msr, pv, C, k, st);
+    const bool single = k == 1 && !out_nk;   // k_row_cost1 reads a dense single-center table
+    const uint32_t kpad = single ? 1u : (uint32_t)cdiv(k, 64) * 64;
+    const Cost1 T = center_tables(ds, msr, pv, C, k, kpad, st);
     const float *CT = T.CT;
     const double *cs = T.cs, *G = T.G;
     const float4 *rt = T.rt;
     const uint32_t *slot = T.slot;
     const dim3 grid((uint32_t)cdiv(n, 4)), blk(256);
     const float fpv = (float)pv, lpv = (float)log(pv > 0 ? pv : 1.);
-    if (k == 1 && !out_nk) {
+    if (single) {
         const Cost1 P{X, val, idx, ptr, d, fpv, lpv, rt, CT, kpad, cs, G, slot};
         switch (msr) {
         case MCG_SQRL2: hipLaunchKernelGGL(k_row_cost1<MCG_SQRL2>, grid, blk, 0, st, P, n, asn, best); break;

@@ -110,8 +110,9 @@ struct Ctx {
     // E-step form chosen per problem shape by timing (see dense_sqrl2_assign)
     bool tuning = true;
     std::vector<std::pair<uint64_t, int>> tuned;
-    // Lloyd iterations after the first replay one captured HIP graph
-    bool graphs = true;
+    // Lloyd iterations after the first replay one captured HIP graph (off by default:
+    // while a capture is open, HIP rejects legacy-stream calls from other threads)
+    bool graphs = false;
     void tic(int which, hipEvent_t &a);
     void toc(int which, hipEvent_t a);
     void collect();   // after a stream synchronize: fold pending pairs into the totals
@@ -193,6 +194,14 @@ void rowsums(Data &ds, hipStream_t st);
 void csr_transforms(Data &ds, double prior, hipStream_t st);
 
 inline uint64_t cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// Blocking copy ordered on st (library code never uses the legacy null stream, which
+// would serialise it against every blocking stream of the process).
+inline void copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t st) {
+    if (!bytes) return;
+    HIP_CHECK(hipMemcpyAsync(dst, src, bytes, kind, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+}
 
 // true while work launched on st is being captured into a graph
 inline bool stream_capturing(hipStream_t st) {

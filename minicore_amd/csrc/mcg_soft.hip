@@ -143,14 +143,15 @@ __global__ void k_soft_csc(const uint64_t *__restrict__ cptr, const uint32_t *__
 
 static void build_csc(Data &ds) {
     if (ds.csc_ptr.p) return;
+    hipStream_t st = ds.ctx->stream;
     const uint64_t n = ds.n, d = ds.d, nnz = ds.nnz;
     std::vector<uint64_t> P(n + 1);
     std::vector<uint32_t> J(nnz);
     std::vector<float> V(nnz);
-    HIP_CHECK(hipMemcpy(P.data(), ds.ptr.p, (n + 1) * 8, hipMemcpyDeviceToHost));
+    copy_sync(P.data(), ds.ptr.p, (n + 1) * 8, hipMemcpyDeviceToHost, st);
     if (nnz) {
-        HIP_CHECK(hipMemcpy(J.data(), ds.idx.p, nnz * 4, hipMemcpyDeviceToHost));
-        HIP_CHECK(hipMemcpy(V.data(), ds.val.p, nnz * 4, hipMemcpyDeviceToHost));
+        copy_sync(J.data(), ds.idx.p, nnz * 4, hipMemcpyDeviceToHost, st);
+        copy_sync(V.data(), ds.val.p, nnz * 4, hipMemcpyDeviceToHost, st);
     }
     std::vector<uint64_t> cp(d + 1, 0);
     for (uint64_t p = 0; p < nnz; ++p) cp[J[p] + 1]++;
@@ -167,10 +168,10 @@ static void build_csc(Data &ds) {
     ds.csc_ptr.reserve((d + 1) * 8);
     ds.csc_row.reserve(nnz * 4 + 4);
     ds.csc_val.reserve(nnz * 4 + 4);
-    HIP_CHECK(hipMemcpy(ds.csc_ptr.p, cp.data(), (d + 1) * 8, hipMemcpyHostToDevice));
+    copy_sync(ds.csc_ptr.p, cp.data(), (d + 1) * 8, hipMemcpyHostToDevice, st);
     if (nnz) {
-        HIP_CHECK(hipMemcpy(ds.csc_row.p, rows.data(), nnz * 4, hipMemcpyHostToDevice));
-        HIP_CHECK(hipMemcpy(ds.csc_val.p, cv.data(), nnz * 4, hipMemcpyHostToDevice));
+        copy_sync(ds.csc_row.p, rows.data(), nnz * 4, hipMemcpyHostToDevice, st);
+        copy_sync(ds.csc_val.p, cv.data(), nnz * 4, hipMemcpyHostToDevice, st);
     }
 }
 

@@ -319,12 +319,14 @@ class Dataset:
                                     precision, ptr(asn_dev), C.byref(cost)))
         return cost.value
 
-    def soft(self, msr, centers, prior=0., temp=1., weights=None, eps=1e-8, maxiter=1000):
+    def soft(self, msr, centers, prior=0., temp=1., weights=None, eps=1e-8, maxiter=1000, with_outputs=True):
+        """Soft EM; with_outputs=False leaves the n x k costs and responsibilities on
+        the device (returned as None)."""
         Cc = np.ascontiguousarray(centers, dtype=np.float32).copy()
         k = Cc.shape[0]
         w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
-        costs = np.empty((self.n, k), dtype=np.float32)
-        resp = np.empty((self.n, k), dtype=np.float32)
+        costs = np.empty((self.n, k), dtype=np.float32) if with_outputs else None
+        resp = np.empty((self.n, k), dtype=np.float32) if with_outputs else None
         ic, fc, it = C.c_double(), C.c_double(), C.c_uint64()
         check(load().mcg_soft_em(self.h, msr, prior, temp, ptr(Cc), k, ptr(w), eps, maxiter, ptr(costs),
                                  ptr(resp), C.byref(ic), C.byref(fc), C.byref(it)))

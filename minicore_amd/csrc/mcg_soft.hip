@@ -125,19 +125,52 @@ __global__ void k_soft_den_final(const double *__restrict__ dpart, uint32_t S, u
     den[c] = s;
 }
 
-__global__ void k_soft_csc(const uint64_t *__restrict__ cptr, const uint32_t *__restrict__ crow,
-                           const float *__restrict__ cval, uint32_t d, const float *__restrict__ R, uint32_t k,
-                           const double *__restrict__ den, float *__restrict__ C) {
+// One wave per column j; lane l holds the centers l, l + 64, ... of a group of up to
+// 256 centers, so each stored entry's row of R is read once per group (coalesced
+// 256-byte pieces) and its row index and value once.  Each (center, column) sum
+// runs over the column's entries in order (FP64).
+template <int Q>
+__global__ __launch_bounds__(256) void k_soft_csc(const uint64_t *__restrict__ cptr, const uint32_t *__restrict__ crow,
+                                                  const float *__restrict__ cval, uint32_t d,
+                                                  const float *__restrict__ R, uint32_t k, uint32_t c0,
+                                                  const double *__restrict__ den, float *__restrict__ C) {
     const uint64_t j = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (j >= d) return;
     const uint64_t b = cptr[j], e = cptr[j + 1];
-    for (uint32_t c0 = 0; c0 < k; c0 += 64) {
-        const uint32_t c = c0 + lane;
-        double acc = 0.;
-        if (c < k)
-            for (uint64_t p = b; p < e; ++p) acc += (double)R[(uint64_t)crow[p] * k + c] * (double)cval[p];
-        if (c < k) C[(uint64_t)c * d + j] = (float)(acc / den[c]);
+    double acc[Q];
+    bool ok[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) { acc[q] = 0.; ok[q] = c0 + lane + 64u * q < k; }
+    uint64_t p = b;
+    // two entries per step: both rows' loads are issued before either is added
+    for (; p + 1 < e; p += 2) {
+        const uint32_t i0 = crow[p], i1 = crow[p + 1];
+        const double x0 = (double)cval[p], x1 = (double)cval[p + 1];
+        float r0[Q], r1[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const uint64_t c = c0 + lane + 64u * q;
+            r0[q] = ok[q] ? R[(uint64_t)i0 * k + c] : 0.f;
+            r1[q] = ok[q] ? R[(uint64_t)i1 * k + c] : 0.f;
+        }
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            acc[q] += (double)r0[q] * x0;
+            acc[q] += (double)r1[q] * x1;
+        }
+    }
+    if (p < e) {
+        const uint32_t i0 = crow[p];
+        const double x0 = (double)cval[p];
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            if (ok[q]) acc[q] += (double)R[(uint64_t)i0 * k + c0 + lane + 64u * q] * x0;
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const uint64_t c = c0 + lane + 64u * q;
+        if (ok[q]) C[c * d + j] = (float)(acc[q] / den[c]);
     }
 }
 
@@ -199,8 +232,19 @@ void soft_mstep(Data &ds, const float *R, const double *w, uint64_t k, float *Cn
     double *den = (double *)c.s[S_PARTQ].reserve(k * 8);
     hipLaunchKernelGGL(k_soft_den_part, dim3(S), dim3(256), 0, st, R, n, (uint32_t)k, w, rows_per, dpart);
     hipLaunchKernelGGL(k_soft_den_final, dim3((uint32_t)cdiv(k, 256)), dim3(256), 0, st, dpart, S, (uint32_t)k, den);
-    hipLaunchKernelGGL(k_soft_csc, dim3((uint32_t)cdiv(d, 4)), dim3(256), 0, st, ds.csc_ptr.as<const uint64_t>(),
-                       ds.csc_row.as<const uint32_t>(), ds.csc_val.as<const float>(), d, R, (uint32_t)k, den, Cnew);
+    for (uint32_t c0 = 0; c0 < k; c0 += 256) {
+        const uint32_t q = (uint32_t)std::min<uint64_t>(4, cdiv(k - c0, 64));
+        const dim3 g((uint32_t)cdiv(d, 4)), blk(256);
+        const uint64_t *cp = ds.csc_ptr.as<const uint64_t>();
+        const uint32_t *cr = ds.csc_row.as<const uint32_t>();
+        const float *cv = ds.csc_val.as<const float>();
+        switch (q) {
+        case 1: hipLaunchKernelGGL(k_soft_csc<1>, g, blk, 0, st, cp, cr, cv, d, R, (uint32_t)k, c0, den, Cnew); break;
+        case 2: hipLaunchKernelGGL(k_soft_csc<2>, g, blk, 0, st, cp, cr, cv, d, R, (uint32_t)k, c0, den, Cnew); break;
+        case 3: hipLaunchKernelGGL(k_soft_csc<3>, g, blk, 0, st, cp, cr, cv, d, R, (uint32_t)k, c0, den, Cnew); break;
+        default: hipLaunchKernelGGL(k_soft_csc<4>, g, blk, 0, st, cp, cr, cv, d, R, (uint32_t)k, c0, den, Cnew); break;
+        }
+    }
 }
 
 }  // namespace mcg

@@ -20,6 +20,8 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
+#include <vector>
 
 namespace mcg {
 
@@ -1017,22 +1019,28 @@ static void dense_sqrl2_assign_mfma(Data &ds, const float *C, uint64_t k, int pr
 
 // Tuning of the BF16-refined form: it and the exact kernel give the same assignment
 // bit for bit, so the faster one is used.  Clear cases are decided by the work
-// n*k*d; in between, both are timed once per (padded d, padded k, log2 n) on the
-// context (min of two runs each) and the choice is kept.  mcg_tuning_enable(ctx, 0)
-// or MCG_TUNE=0 always takes the MFMA form.
+// n*k*d (measured: the MFMA form is 1.5x faster at 2^24 and 4x at 2^28.5); in
+// between, both are timed once per device and (padded d, padded k, log2 n), min of
+// two runs each, and the choice is kept for every context of the process.
+// mcg_tuning_enable(ctx, 0) or MCG_TUNE=0 always takes the MFMA form.
+static std::mutex g_tune_mu;
+static std::vector<std::pair<uint64_t, int>> g_tuned;
+
 static int tuned_form(Data &ds, const float *C, uint64_t k, uint32_t *asn, hipStream_t st) {
     static const bool tune_on = [] { const char *e = getenv("MCG_TUNE"); return !(e && e[0] == '0'); }();
     if (!tune_on || !ds.ctx->tuning) return MCG_PREC_BF16;
     const double work = (double)ds.n * (double)k * (double)ds.d;
-    if (work >= 0x1p34) return MCG_PREC_BF16;
+    if (work >= 0x1p26) return MCG_PREC_BF16;
     if (work <= 0x1p21) return MCG_PREC_EXACT;
-    Ctx &c = *ds.ctx;
     prepare_bf16(ds);
     uint64_t lg = 0;
     while ((1ull << lg) < ds.n) ++lg;
-    const uint64_t key = ((uint64_t)ds.dpad << 40) | (cdiv(k, 64) << 8) | lg;
-    for (const auto &kv : c.tuned)
-        if (kv.first == key) return kv.second;
+    const uint64_t key = ((uint64_t)(ds.ctx->device & 0xFF) << 56) | ((uint64_t)ds.dpad << 40) | (cdiv(k, 64) << 8) | lg;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        for (const auto &kv : g_tuned)
+            if (kv.first == key) return kv.second;
+    }
     if (stream_capturing(st)) return MCG_PREC_BF16;  // no timing inside a graph capture
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
@@ -1055,7 +1063,8 @@ static int tuned_form(Data &ds, const float *C, uint64_t k, uint32_t *asn, hipSt
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     const int form = t_exact < t_mfma ? MCG_PREC_EXACT : MCG_PREC_BF16;
-    c.tuned.push_back({key, form});
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned.push_back({key, form});
     return form;
 }
 

@@ -107,9 +107,8 @@ struct Ctx {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[2];
     double timed_ms[2] = {0., 0.};
     uint64_t timed_n[2] = {0, 0};
-    // E-step form chosen per problem shape by timing (see dense_sqrl2_assign)
+    // E-step kernel chosen per problem shape by timing (see dense_sqrl2_assign)
     bool tuning = true;
-    std::vector<std::pair<uint64_t, int>> tuned;
     // Lloyd iterations after the first replay one captured HIP graph (off by default:
     // while a capture is open, HIP rejects legacy-stream calls from other threads)
     bool graphs = false;

@@ -154,6 +154,25 @@ int mcg_lloyd(mcg_data *ds, int msr, double prior, float *centers, uint64_t k,
  * centers and asn must be device pointers. */
 int mcg_lloyd_step(mcg_data *ds, int msr, double prior, float *centers, uint64_t k,
                    const double *weights, int precision, uint32_t *asn, double *cost);
+/* Minibatch hard clustering: perform_hard_minibatch_clustering
+ * (include/minicore/clustering/solve.h:377-578), reached from hcluster(...,
+ * mbsize >= 0) (python/pycluster.h:31-42), for the mean-centroid measures.
+ * Check-ins (iteration 0, every `freq`-th, the last) assign every row, restart
+ * centers with <= reseed_after rows at rows drawn with probability ~ w_i*cost_i,
+ * and keep the centers of the lowest check-in cost; in between, mbsize rows are
+ * drawn (rng() mod n, with or without replacement), sorted, assigned, and each
+ * center with members becomes their weighted mean.  The draws use the wyrand
+ * counter stream of `seed` in the reference's order (seed 0 is the stream of 0,
+ * not std::rand()).  centers (k x d) in: initial, out: best; asn/costs (nullable):
+ * the last check-in's assignment and per-row costs; initcost/finalcost: the first
+ * and the lowest check-in cost, exact fixed-point sums of w_i*cost_i.  Requires
+ * mbsize, maxiter, freq > 0 and, without replacement, mbsize <= n (the reference
+ * would loop forever). */
+int mcg_minibatch(mcg_data *ds, int msr, double prior, float *centers, uint64_t k, const double *weights,
+                  uint64_t mbsize, uint64_t maxiter, uint64_t freq, uint64_t reseed_after, int with_replacement,
+                  uint64_t seed, int precision, uint32_t *asn, float *costs, double *initcost, double *finalcost,
+                  uint64_t *numiter);
+
 /* Soft EM driver: perform_soft_clustering, solve.h:287-374 with
  * set_centroids_full_mean (soft), centroid.h:781-963.  costs and resp are
  * n x k float32 outputs (nullable). */

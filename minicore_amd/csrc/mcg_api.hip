@@ -3,6 +3,7 @@
 // the other translation units.
 #include "mcg_internal.h"
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -313,6 +314,36 @@ void copy_rows_to_centers(Data &ds, const uint64_t *rows_h, const uint32_t *ids_
         hipLaunchKernelGGL(k_rows_to_centers_dense, dim3((uint32_t)std::min<uint64_t>(cdiv(m * ds.d, 256), 4096)),
                            dim3(256), 0, st, ds.X, (uint32_t)ds.d, rows, ids, m, C);
     HIP_CHECK(hipStreamSynchronize(st));  // the host arrays may go away
+}
+
+// minibatch helpers
+__global__ void k_count_asn(const uint32_t *__restrict__ asn, uint64_t n, uint32_t k, uint32_t *__restrict__ cnt) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        if (asn[i] < k) atomicAdd(&cnt[asn[i]], 1u);
+}
+
+// ||x_i - c||^2 as the sequential FMA chain of the exact E-step, one thread per row
+__global__ void k_cost_to_row_l2(const float *__restrict__ X, uint64_t n, uint32_t d, const float *__restrict__ c,
+                                 float *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *x = X + i * d;
+    float s = 0.f;
+    for (uint32_t j = 0; j < d; ++j) { const float t = x[j] - c[j]; s = fmaf(t, t, s); }
+    out[i] = s;
+}
+
+// a row moves to center f when strictly cheaper (solve.h:490-496)
+__global__ void k_take_lower(const float *__restrict__ v, uint64_t n, uint32_t f, float *__restrict__ cost,
+                             uint32_t *__restrict__ asn) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        if (v[i] < cost[i]) { cost[i] = v[i]; asn[i] = f; }
+}
+
+__global__ void k_gather_f64(const double *__restrict__ w, const uint64_t *__restrict__ idx, uint64_t m,
+                             double *__restrict__ out) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m) out[t] = w[idx[t]];
 }
 
 __global__ void k_force(uint32_t *__restrict__ asn, float *__restrict__ cost, const uint64_t *__restrict__ rows,
@@ -911,6 +942,128 @@ int mcg_lloyd(mcg_data *ds, int msr, double prior, float *centers, uint64_t k, c
         HIP_CHECK(hipMemcpyAsync(centers, C, kd * 4, is_device_ptr(centers) ? hipMemcpyDeviceToDevice
                                                                            : hipMemcpyDeviceToHost, st));
         sync(c);
+    });
+}
+
+// perform_hard_minibatch_clustering (solve.h:377-578) for the mean-centroid
+// measures.  One RNG stream (the wyrand counter stream of seed) serves the restart
+// draws and the minibatch draws in the reference's order.  A check-in (iteration 0,
+// every freq-th and the last) assigns every row, restarts each center with <=
+// reseed_after rows at a row drawn over w_i * cost_i, moves rows to a restarted
+// center that is strictly cheaper, and keeps the centers of the lowest check-in
+// cost (exact fixed-point sum); between check-ins mbsize rows are drawn (without
+// replacement: the first mbsize distinct draws), sorted, assigned, and every
+// center with members becomes their weighted mean.  The draws and the check-in
+// bookkeeping run on the host; rows, costs and centers stay on the device.
+int mcg_minibatch(mcg_data *ds, int msr, double prior, float *centers, uint64_t k, const double *weights,
+                  uint64_t mbsize, uint64_t maxiter, uint64_t freq, uint64_t reseed_after, int with_replacement,
+                  uint64_t seed, int precision, uint32_t *asn, float *costs, double *initcost, double *finalcost,
+                  uint64_t *numiter) {
+    return guard([&] {
+        REQUIRE(ds && centers && initcost && finalcost && numiter, "null argument");
+        REQUIRE(k > 0, "k must be positive");
+        REQUIRE(mbsize > 0 && maxiter > 0 && freq > 0, "mbsize, maxiter and the check-in period must be positive");
+        check_msr(msr);
+        Data &D = ds->d;
+        Ctx &c = *D.ctx;
+        hipStream_t st = c.stream;
+        HIP_CHECK(hipSetDevice(c.device));
+        const uint64_t n = D.n, d = D.d, kd = k * d;
+        REQUIRE(with_replacement || mbsize <= n, "mbsize exceeds the number of rows (draws without replacement)");
+        const bool l2 = dense_l2(D, msr);
+        DBuf sw, sa_out, sk_out, bC, bSave, bA, bP, bTmp, bTa, bCnt, bIdx, bWs, bAs, bCs;
+        const double *W = (const double *)in_dev(c, weights, n * 8, sw);
+        std::vector<double> wh;
+        if (weights) {
+            wh.resize(n);
+            if (is_device_ptr(weights)) copy_sync(wh.data(), weights, n * 8, hipMemcpyDeviceToHost, st);
+            else std::memcpy(wh.data(), weights, n * 8);
+        }
+        float *C = (float *)bC.reserve(kd * 4), *Save = (float *)bSave.reserve(kd * 4);
+        uint32_t *A = (uint32_t *)bA.reserve(n * 4), *Ta = (uint32_t *)bTa.reserve(n * 4);
+        float *P = (float *)bP.reserve(n * 4), *Tmp = (float *)bTmp.reserve(n * 4);
+        uint32_t *cnt = (uint32_t *)bCnt.reserve(k * 4);
+        uint64_t *idx_d = (uint64_t *)bIdx.reserve(mbsize * 8);
+        double *Ws = W ? (double *)bWs.reserve(mbsize * 8) : nullptr;
+        uint32_t *As = (uint32_t *)bAs.reserve(mbsize * 4);
+        float *Cs = (float *)bCs.reserve(mbsize * 4);
+        HIP_CHECK(hipMemcpyAsync(C, centers, kd * 4, is_device_ptr(centers) ? hipMemcpyDeviceToDevice
+                                                                           : hipMemcpyHostToDevice, st));
+        copy_f32(C, Save, kd, st);
+        Data sub;
+        std::vector<uint64_t> sel(mbsize);
+        std::vector<uint8_t> seen(with_replacement ? 0 : n);
+        std::vector<uint32_t> hc(k);
+        std::vector<float> hp;
+        uint64_t j = 0, iter = 0;
+        double init = DBL_MAX, best = DBL_MAX;
+        const uint32_t pgrid = (uint32_t)std::min<uint64_t>(cdiv(n, 256), 4096);
+        for (;;) {
+            if (iter % freq == 0 || iter == maxiter - 1) {
+                estep(D, msr, prior, C, k, precision, A, l2 ? nullptr : P, st);
+                if (l2) dense_sqrl2_costs(D, C, A, P, st);
+                fill_u32(cnt, 0u, k, st);
+                hipLaunchKernelGGL(k_count_asn, dim3(pgrid), dim3(256), 0, st, A, n, (uint32_t)k, cnt);
+                copy_sync(hc.data(), cnt, k * 4, hipMemcpyDeviceToHost, st);
+                std::vector<uint32_t> found;
+                for (uint64_t q = 0; q < k; ++q) if (hc[q] <= reseed_after) found.push_back((uint32_t)q);
+                if (!found.empty()) {
+                    hp.resize(n);
+                    copy_sync(hp.data(), P, n * 4, hipMemcpyDeviceToHost, st);
+                    std::vector<double> wv(n);
+                    for (uint64_t i = 0; i < n; ++i) wv[i] = weights ? wh[i] * (double)hp[i] : (double)hp[i];
+                    std::vector<uint64_t> rows(found.size());
+                    for (size_t f = 0; f < found.size(); ++f) rows[f] = draw_fixed(wv, wyrand_at(seed, j++));
+                    copy_rows_to_centers(D, rows.data(), found.data(), found.size(), C, st);
+                    for (uint32_t f : found) {
+                        const float *cf = C + (uint64_t)f * d;
+                        if (l2)
+                            hipLaunchKernelGGL(k_cost_to_row_l2, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, D.X, n,
+                                               (uint32_t)d, cf, Tmp);
+                        else
+                            generic_costs(D, msr, prior, cf, 1, nullptr, Ta, Tmp, st);
+                        hipLaunchKernelGGL(k_take_lower, dim3(pgrid), dim3(256), 0, st, Tmp, n, f, P, A);
+                    }
+                }
+                const double cost = fixed_wsum(c, P, W, n, st);
+                if (iter == 0) init = best = cost;
+                if (cost < best) {
+                    best = cost;
+                    copy_f32(C, Save, kd, st);
+                }
+            }
+            if (++iter == maxiter) break;
+            if (with_replacement) {
+                for (uint64_t t = 0; t < mbsize; ++t) sel[t] = wyrand_at(seed, j++) % n;
+            } else {
+                std::fill(seen.begin(), seen.end(), (uint8_t)0);
+                for (uint64_t t = 0; t < mbsize;) {
+                    const uint64_t r = wyrand_at(seed, j++) % n;
+                    if (!seen[r]) { seen[r] = 1; sel[t++] = r; }
+                }
+            }
+            std::sort(sel.begin(), sel.end());
+            copy_sync(idx_d, sel.data(), mbsize * 8, hipMemcpyHostToDevice, st);
+            rows_into(D, sel.data(), idx_d, mbsize, sub, st);
+            if (Ws)
+                hipLaunchKernelGGL(k_gather_f64, dim3((uint32_t)cdiv(mbsize, 256)), dim3(256), 0, st, W, idx_d, mbsize,
+                                   Ws);
+            estep(sub, msr, prior, C, k, precision, As, l2 ? nullptr : Cs, st);
+            if (sub.csr) csr_mstep(sub, As, Ws, k, C, st);
+            else dense_mstep(sub, As, Ws, k, C, C, false, st);
+        }
+        Out oa = out_dev(c, asn, n * 4, sa_out);
+        Out ok = out_dev(c, costs, n * 4, sk_out);
+        if (oa.dev) HIP_CHECK(hipMemcpyAsync(oa.dev, A, n * 4, hipMemcpyDeviceToDevice, st));
+        if (ok.dev) HIP_CHECK(hipMemcpyAsync(ok.dev, P, n * 4, hipMemcpyDeviceToDevice, st));
+        out_finish(c, oa);
+        out_finish(c, ok);
+        HIP_CHECK(hipMemcpyAsync(centers, Save, kd * 4, is_device_ptr(centers) ? hipMemcpyDeviceToDevice
+                                                                              : hipMemcpyDeviceToHost, st));
+        sync(c);
+        *initcost = init;
+        *finalcost = best;
+        *numiter = iter;
     });
 }
 

@@ -183,6 +183,25 @@ void coreset_sample_from(Ctx &ctx, uint64_t n, const float *probs_d, const doubl
                          uint64_t seed, uint64_t first, uint64_t *idx_d, float *wout_d, hipStream_t st);
 void restart_draws(Ctx &ctx, const float *costs_d, uint64_t n, uint64_t nempty, uint64_t *rows_h,
                    hipStream_t st);
+// Fixed-point inverse-CDF draw over host weights wv for raw draw r.
+uint64_t draw_fixed(const std::vector<double> &wv, uint64_t r);
+// Exact sum of w_i * v_i (v float on the device, w nullable): each term in fixed
+// point with the sampler's shift, integer sums; independent of the order.
+double fixed_wsum(Ctx &ctx, const float *v, const double *w, uint64_t n, hipStream_t st);
+// Rows idx (device idx_d; idx_h the same on the host, or null) of S into D, reusing
+// D's buffers; D's derived data (BF16 copy, row sums, transforms) is dropped.
+void rows_into(const Data &S, const uint64_t *idx_h, const uint64_t *idx_d, uint64_t m, Data &D, hipStream_t st);
+
+// Drop the data derived from a dataset's rows (rebuilt on next use).
+inline void reset_derived(Data &D) {
+    D.bf16_ready = false;
+    D.rowsum.release();
+    D.tf_prior = -1.;
+    D.li_unique.clear();
+    D.csc_ptr.release();
+    D.csc_row.release();
+    D.csc_val.release();
+}
 
 // Small helpers.
 void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
@@ -193,6 +212,19 @@ void rowsums(Data &ds, hipStream_t st);
 void csr_transforms(Data &ds, double prior, hipStream_t st);
 
 inline uint64_t cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// Output j (0-based) of the published wyrand generator seeded with seed, in counter
+// form: state seed + (j+1)*P0, mixed by the 128-bit product with state ^ P1.
+__host__ __device__ inline uint64_t wyrand_at(uint64_t seed, uint64_t j) {
+    const uint64_t s = seed + (j + 1) * 0xa0761d6478bd642fULL;
+    const uint64_t b = s ^ 0xe7037ed1a0b428dbULL;
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul64hi(s, b) ^ (s * b);
+#else
+    const unsigned __int128 p = (unsigned __int128)s * b;
+    return (uint64_t)(p >> 64) ^ (uint64_t)p;
+#endif
+}
 
 // Blocking copy ordered on st (library code never uses the legacy null stream, which
 // would serialise it against every blocking stream of the process).

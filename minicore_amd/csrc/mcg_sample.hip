@@ -13,6 +13,7 @@
 // Sums in the sensitivity formulas are exact fixed-point integer sums for the
 // same reason.
 #include "mcg_internal.h"
+#include <cstring>
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -21,16 +22,6 @@
 
 namespace mcg {
 
-__host__ __device__ inline uint64_t wyrand_at(uint64_t seed, uint64_t j) {
-    const uint64_t s = seed + (j + 1) * 0xa0761d6478bd642fULL;
-    const uint64_t b = s ^ 0xe7037ed1a0b428dbULL;
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __umul64hi(s, b) ^ (s * b);
-#else
-    const unsigned __int128 p = (unsigned __int128)s * b;
-    return (uint64_t)(p >> 64) ^ (uint64_t)p;
-#endif
-}
 
 __host__ __device__ inline uint64_t mulhi64(uint64_t a, uint64_t b) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -203,6 +194,29 @@ static uint64_t draw_host(const std::vector<double> &wv, uint64_t r) {
         if (acc > u) return i;
     }
     return n - 1;
+}
+
+uint64_t draw_fixed(const std::vector<double> &wv, uint64_t r) { return draw_host(wv, r); }
+
+double fixed_wsum(Ctx &c, const float *v, const double *w, uint64_t n, hipStream_t st) {
+    const uint32_t nblk = (uint32_t)cdiv(n, FX_BLK);
+    char *buf = (char *)c.s[S_MISC0].reserve(64 + nblk * 8);
+    unsigned long long *mx = (unsigned long long *)buf;
+    uint64_t *bsum = (uint64_t *)(buf + 64);
+    fill_u32((uint32_t *)mx, 0, 2, st);
+    hipLaunchKernelGGL(k_absmax<float>, dim3((uint32_t)std::min<uint64_t>(cdiv(n, 256), 1024)), dim3(256), 0, st, v, w,
+                       n, mx);
+    hipLaunchKernelGGL(k_fixed_blocksum<float>, dim3(nblk), dim3(256), 0, st, v, w, n,
+                       (const unsigned long long *)mx, bsum);
+    std::vector<uint64_t> h(nblk + 1);
+    HIP_CHECK(hipMemcpyAsync(h.data(), mx, 8, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(h.data() + 1, bsum, nblk * 8, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    double mxv;
+    std::memcpy(&mxv, h.data(), 8);
+    uint64_t T = 0;
+    for (uint32_t b = 0; b < nblk; ++b) T += h[b + 1];
+    return ldexp((double)T, -fixed_shift(mxv, n));
 }
 
 void restart_draws(Ctx &c, const float *costs_d, uint64_t n, uint64_t nempty, uint64_t *rows_h, hipStream_t st) {

@@ -69,8 +69,8 @@ __global__ void k_rows_csr(const float *__restrict__ val, const uint32_t *__rest
     }
 }
 
-static void rows_into(const Data &S, const uint64_t *idx_h, const uint64_t *idx_d, uint64_t m, Data &D,
-                      hipStream_t st) {
+void rows_into(const Data &S, const uint64_t *idx_h, const uint64_t *idx_d, uint64_t m, Data &D, hipStream_t st) {
+    reset_derived(D);
     D.ctx = S.ctx;
     D.csr = S.csr;
     D.n = m;

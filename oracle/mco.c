@@ -477,6 +477,138 @@ int mco_lloyd(int msr, const float *X, const float *val, const uint32_t *idx,
 }
 
 /* ======================================================================
+ * Minibatch hard clustering: perform_hard_minibatch_clustering,
+ * solve.h:377-578, for the mean-centroid measures (set_center,
+ * centroid.h:81-178).  One RNG stream, the wyrand counter stream of `seed`,
+ * serves the restart draws and the minibatch draws in the reference's order.
+ * A check-in (iteration 0, every freq-th, and the last) assigns every row,
+ * restarts each center with <= reseed_after rows (ascending id) at a row
+ * drawn by the fixed-point sampler over w_i*cost_i, moves rows to a
+ * restarted center that is strictly cheaper (restarted centers in order),
+ * and keeps the centers of the lowest check-in cost (at iteration 0 the
+ * saved centers stay the input ones, solve.h:494-499).  Between check-ins,
+ * mbsize rows are drawn (rng() mod n; without replacement the first mbsize
+ * distinct draws), sorted ascending (solve.h:549-551, duplicates kept), each
+ * takes its nearest center, and every center with members becomes their
+ * weighted mean.  The check-in cost is the exact fixed-point sum of
+ * w_i*cost_i (the shift of the sampler), independent of summation order.
+ * ====================================================================== */
+static double point_cost(int msr, const float *X, const float *val, const uint32_t *idx, const uint64_t *indptr,
+                         uint64_t i, uint64_t d, const float *c, double prior) {
+    if (msr == MCO_SQRL2 && !val) return (double)mco_sqrl2_f32(X + i * d, c, d);
+    const double xs = val ? row_sum_csr(val, indptr[i], indptr[i + 1]) : row_sum_dense(X + i * d, d);
+    const double cs = row_sum_dense(c, d);
+    double *xb = (double *)malloc(d * sizeof(double)), *cb = (double *)malloc(d * sizeof(double));
+    const double v = pair_cost(msr, X, val, idx, indptr, i, d, c, prior, xs, cs, xb, cb);
+    free(xb); free(cb);
+    return v;
+}
+
+static double fixed_total(const double *cost, const double *w, uint64_t n) {
+    double mx = 0.;
+    double *t = (double *)malloc((n ? n : 1) * sizeof(double));
+    for (uint64_t i = 0; i < n; ++i) { t[i] = w ? w[i] * cost[i] : cost[i]; if (fabs(t[i]) > mx) mx = fabs(t[i]); }
+    const int s = mco_fixed_shift(mx, n);
+    uint64_t T = 0;
+    for (uint64_t i = 0; i < n; ++i) T += (uint64_t)floor(ldexp(t[i], s));
+    free(t);
+    return ldexp((double)T, -s);
+}
+
+static int cmp_u64(const void *a, const void *b) {
+    const uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    return x < y ? -1 : x > y;
+}
+
+int mco_minibatch(int msr, const float *X, const float *val, const uint32_t *idx, const uint64_t *indptr,
+                  uint64_t n, uint64_t d, float *C, uint64_t k, double prior, const double *w,
+                  uint64_t mbsize, uint64_t maxiter, uint64_t freq, uint64_t reseed_after, int with_rep,
+                  uint64_t seed, uint32_t *asn, double *cost, double *initcost, double *finalcost,
+                  uint64_t *numiter, int nthreads) {
+    if (!mbsize || !maxiter || !freq || (!with_rep && mbsize > n)) return -2;
+    uint64_t j = 0, iter = 0;
+    double init = DBL_MAX, best = DBL_MAX;
+    float *save = (float *)malloc(k * d * sizeof(float));
+    memcpy(save, C, k * d * sizeof(float));
+    uint64_t *sel = (uint64_t *)malloc(mbsize * sizeof(uint64_t));
+    uint32_t *sa = (uint32_t *)malloc(mbsize * sizeof(uint32_t));
+    double *sc = (double *)malloc(mbsize * sizeof(double)), *sw = w ? (double *)malloc(mbsize * sizeof(double)) : NULL;
+    uint8_t *seen = with_rep ? NULL : (uint8_t *)malloc(n);
+    uint64_t *cnt = (uint64_t *)malloc(k * sizeof(uint64_t));
+    float *Xs = NULL, *vs = NULL; uint32_t *is = NULL; uint64_t *ps = NULL;
+    for (;;) {
+        if (iter % freq == 0 || iter == maxiter - 1) {
+            assign_generic(msr, X, val, idx, indptr, n, d, C, k, prior, asn, cost, nthreads);
+            memset(cnt, 0, k * sizeof(uint64_t));
+            for (uint64_t i = 0; i < n; ++i) cnt[asn[i]]++;
+            uint64_t nf = 0;
+            for (uint64_t c = 0; c < k; ++c) if (cnt[c] <= reseed_after) ++nf;
+            if (nf) {
+                uint32_t *found = (uint32_t *)malloc(nf * sizeof(uint32_t));
+                double *wv = (double *)malloc(n * sizeof(double));
+                uint64_t *fw = (uint64_t *)malloc(n * sizeof(uint64_t));
+                for (uint64_t i = 0; i < n; ++i) wv[i] = w ? w[i] * cost[i] : cost[i];
+                mco_fixed_weights(wv, n, fw);
+                uint64_t f = 0;
+                for (uint64_t c = 0; c < k; ++c) if (cnt[c] <= reseed_after) {
+                    const uint64_t r = mco_fixed_draw(fw, n, mco_wyrand_at(seed, j++));
+                    found[f++] = (uint32_t)c;
+                    if (val) {
+                        memset(C + c * d, 0, d * sizeof(float));
+                        for (uint64_t p = indptr[r]; p < indptr[r + 1]; ++p) C[c * d + idx[p]] = val[p];
+                    } else memcpy(C + c * d, X + r * d, d * sizeof(float));
+                }
+                #pragma omp parallel for schedule(dynamic, 64)
+                for (int64_t i = 0; i < (int64_t)n; ++i)
+                    for (uint64_t q = 0; q < nf; ++q) {
+                        const double v = point_cost(msr, X, val, idx, indptr, i, d, C + (uint64_t)found[q] * d, prior);
+                        if (v < cost[i]) { cost[i] = v; asn[i] = found[q]; }
+                    }
+                free(found); free(wv); free(fw);
+            }
+            const double ct = fixed_total(cost, w, n);
+            if (iter == 0) init = best = ct;
+            if (ct < best) { best = ct; memcpy(save, C, k * d * sizeof(float)); }
+        }
+        if (++iter == maxiter) break;
+        if (with_rep) {
+            for (uint64_t t = 0; t < mbsize; ++t) sel[t] = mco_wyrand_at(seed, j++) % n;
+        } else {
+            memset(seen, 0, n);
+            for (uint64_t t = 0; t < mbsize;) {
+                const uint64_t r = mco_wyrand_at(seed, j++) % n;
+                if (!seen[r]) { seen[r] = 1; sel[t++] = r; }
+            }
+        }
+        qsort(sel, mbsize, sizeof(uint64_t), cmp_u64);
+        if (sw) for (uint64_t t = 0; t < mbsize; ++t) sw[t] = w[sel[t]];
+        if (val) {
+            free(vs); free(is); free(ps);
+            ps = (uint64_t *)malloc((mbsize + 1) * sizeof(uint64_t));
+            ps[0] = 0;
+            for (uint64_t t = 0; t < mbsize; ++t) ps[t + 1] = ps[t] + (indptr[sel[t] + 1] - indptr[sel[t]]);
+            vs = (float *)malloc((ps[mbsize] ? ps[mbsize] : 1) * sizeof(float));
+            is = (uint32_t *)malloc((ps[mbsize] ? ps[mbsize] : 1) * sizeof(uint32_t));
+            for (uint64_t t = 0; t < mbsize; ++t) {
+                memcpy(vs + ps[t], val + indptr[sel[t]], (ps[t + 1] - ps[t]) * sizeof(float));
+                memcpy(is + ps[t], idx + indptr[sel[t]], (ps[t + 1] - ps[t]) * sizeof(uint32_t));
+            }
+            assign_generic(msr, NULL, vs, is, ps, mbsize, d, C, k, prior, sa, sc, nthreads);
+            mco_centroids_csr(vs, is, ps, mbsize, d, sa, sw, k, C);
+        } else {
+            if (!Xs) Xs = (float *)malloc(mbsize * d * sizeof(float));
+            for (uint64_t t = 0; t < mbsize; ++t) memcpy(Xs + t * d, X + sel[t] * d, d * sizeof(float));
+            assign_generic(msr, Xs, NULL, NULL, NULL, mbsize, d, C, k, prior, sa, sc, nthreads);
+            mco_centroids_dense(Xs, mbsize, d, sa, sw, k, 256, C);
+        }
+    }
+    memcpy(C, save, k * d * sizeof(float));
+    *initcost = init; *finalcost = best; *numiter = iter;
+    free(save); free(sel); free(sa); free(sc); free(sw); free(seen); free(cnt); free(Xs); free(vs); free(is); free(ps);
+    return 0;
+}
+
+/* ======================================================================
  * Soft clustering.  centroid.h:781-793 (correct_softmax) and 795-963;
  * solve.h:287-374.  softmax is exp(-temp*cost) / row sum in float without
  * max subtraction; a row with any NaN becomes uniform over its argmin ties.

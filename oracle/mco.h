@@ -84,6 +84,16 @@ int mco_lloyd(int msr, const float *X, const float *val, const uint32_t *idx,
               uint32_t *asn, double *cost, double *initcost, double *finalcost,
               uint64_t *numiter, int nthreads);
 
+/* ---- minibatch driver: solve.h:377-578 -------------------------------- */
+/* Dense (val==NULL) or CSR rows; C (k x d) in: initial centers, out: the best
+ * check-in's centers; asn/cost: the last check-in's assignment.  freq is the
+ * check-in period (calc_cost_freq).  Returns -2 for invalid sizes. */
+int mco_minibatch(int msr, const float *X, const float *val, const uint32_t *idx, const uint64_t *indptr,
+                  uint64_t n, uint64_t d, float *C, uint64_t k, double prior, const double *w,
+                  uint64_t mbsize, uint64_t maxiter, uint64_t freq, uint64_t reseed_after, int with_rep,
+                  uint64_t seed, uint32_t *asn, double *cost, double *initcost, double *finalcost,
+                  uint64_t *numiter, int nthreads);
+
 /* ---- soft clustering: solve.h:287-374, centroid.h:781-963 --------------- */
 int mco_soft(int msr, const float *X, const float *val, const uint32_t *idx,
              const uint64_t *indptr, uint64_t n, uint64_t d, float *C, uint64_t k,

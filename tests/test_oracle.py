@@ -190,6 +190,87 @@ def test_lloyd_rules(mco):
     np.testing.assert_allclose(r2["centers"], Cb, rtol=1e-5, atol=1e-5)
 
 
+# ---- minibatch driver (solve.h:377-578) ------------------------------------------------
+def _fixed_total(t):
+    t = np.asarray(t, dtype=np.float64)
+    mx = float(np.abs(t).max())
+    m, e = np.frexp(mx)
+    s = 62 - int(e) - int(len(t)).bit_length()
+    return float(np.ldexp(float(sum(int(v) for v in np.floor(np.ldexp(t, s)))), -s))
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_minibatch_full_batch_is_lloyd(mco, weighted):
+    """mbsize = n without replacement draws every row, so each update is a full
+    Lloyd M-step; with a check-in every iteration and no restarts the best check-in
+    is the last one: the centers and assignment of Lloyd after the same number of
+    updates (mco_lloyd with maxiter T-2 runs T-1 updates)."""
+    from minicore_amd.datagen import gaussian_mixture
+    X, lab, _ = gaussian_mixture(3000, 8, 5, seed=3, lo=-4, hi=4)
+    w = np.random.default_rng(1).uniform(0.5, 2., 3000) if weighted else None
+    C0 = X[[int(np.argmax(lab == c)) for c in range(5)]].copy()
+    T = 6
+    r = mco.minibatch(mco.SQRL2, X, C0, w=w, mbsize=3000, maxiter=T, freq=1, reseed_after=0, with_rep=False, seed=4)
+    L = mco.lloyd(mco.SQRL2, X, C0, w=w, eps=-1., maxiter=T - 2)
+    assert L["numiter"] == T - 1 and r["numiter"] == T
+    assert np.array_equal(r["centers"], L["centers"])
+    assert np.array_equal(r["asn"], L["asn"])
+    assert r["finalcost"] == pytest.approx(L["finalcost"], rel=1e-6)
+    a0, c0, _ = mco.assign_sqrl2_f32(X, C0)
+    assert r["initcost"] == _fixed_total(c0.astype(np.float64) * (w if weighted else 1.))
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_minibatch_restart_draw(mco, weighted):
+    """A center with no rows is restarted at the row drawn over w_i * cost_i with
+    the stream's first output; rows move to it when strictly cheaper; with one
+    iteration the saved centers are still the input ones (solve.h:494-499)."""
+    from minicore_amd.datagen import gaussian_mixture
+    X, lab, _ = gaussian_mixture(2000, 6, 4, seed=5, lo=-4, hi=4)
+    w = np.random.default_rng(2).uniform(0.5, 2., 2000) if weighted else None
+    C0 = np.concatenate([X[[int(np.argmax(lab == c)) for c in range(3)]], np.full((1, 6), 1e3, np.float32)])
+    r = mco.minibatch(mco.SQRL2, X, C0, w=w, mbsize=10, maxiter=1, freq=1, reseed_after=0, seed=9)
+    _, c0, _ = mco.assign_sqrl2_f32(X, C0)
+    wv = c0.astype(np.float64) * (w if weighted else 1.)
+    row = mco.fixed_draw(mco.fixed_weights(wv), mco.wyrand_at(9, 0))
+    C1 = C0.copy()
+    C1[3] = X[row]
+    a1, c1, _ = mco.assign_sqrl2_f32(X, C1)
+    assert np.array_equal(r["asn"], a1) and r["asn"][row] == 3
+    assert np.array_equal(r["costs"], c1.astype(np.float64))
+    assert np.array_equal(r["centers"], C0)
+    assert r["initcost"] == r["finalcost"] == _fixed_total(c1.astype(np.float64) * (w if weighted else 1.))
+
+
+@pytest.mark.parametrize("with_rep", [True, False])
+def test_minibatch_one_update(mco, with_rep):
+    """One minibatch update: mbsize draws rng() mod n (without replacement the first
+    distinct ones), sorted, assigned, and the clusters with members take their mean
+    (ascending row order, duplicates counted); the returned centers are those of the
+    lower of the two check-in costs."""
+    from minicore_amd.datagen import gaussian_mixture
+    n, seed, m = 1500, 21, 300
+    X, lab, _ = gaussian_mixture(n, 5, 4, seed=7, lo=-4, hi=4)
+    C0 = X[[int(np.argmax(lab == c)) for c in range(4)]].copy()
+    r = mco.minibatch(mco.SQRL2, X, C0, mbsize=m, maxiter=2, freq=100, reseed_after=0, with_rep=with_rep, seed=seed)
+    draws, j = [], 0
+    while len(draws) < m:
+        v = mco.wyrand_at(seed, j) % n
+        j += 1
+        if with_rep or v not in draws:
+            draws.append(v)
+    sel = np.sort(np.array(draws, dtype=np.int64))
+    Xs = X[sel]
+    a, _, _ = mco.assign_sqrl2_f32(Xs, C0)
+    C1, _ = mco.centroids(Xs, a, 4, C0)
+    _, c0, _ = mco.assign_sqrl2_f32(X, C0)
+    a1, c1, _ = mco.assign_sqrl2_f32(X, C1)
+    t0, t1 = _fixed_total(c0.astype(np.float64)), _fixed_total(c1.astype(np.float64))
+    assert r["initcost"] == t0 and r["finalcost"] == min(t0, t1)
+    assert np.array_equal(r["centers"], C1 if t1 < t0 else C0)
+    assert np.array_equal(r["asn"], a1)
+
+
 # ---- a5: soft -------------------------------------------------------------------------
 def test_softmax_rows_and_repair(mco):
     rng = np.random.default_rng(2)

@@ -234,6 +234,45 @@ def sparse_config(args, mcg, cfg):
     return out
 
 
+def minibatch_config(args, mcg):
+    """Auxiliary measurement of the minibatch driver (hcluster with mbsize >= 0) on the
+    headline rows: n=1e6, d=256, k=1024, 65,536-row minibatches without replacement,
+    50 iterations, a check-in (full assignment, restarts of centers with <= 5 rows,
+    exact cost) every 10."""
+    import numpy as np
+    import torch
+    from minicore_amd.datagen import gaussian_mixture_device
+    n, d, k = args.n, args.d, args.k
+    log(f"minibatch: generating n={n} d={d} k={k}")
+    X, _, _ = gaussian_mixture_device(n, d, k, args.seed, scale=(0.5, 2.0))
+    g = torch.Generator(device="cpu")
+    g.manual_seed(args.seed)
+    C0 = X[torch.randperm(n, generator=g)[:k].to("cuda")].cpu().numpy()
+    torch.cuda.synchronize()
+    ctx = mcg.Context(0)
+    ds = mcg.Dataset.dense(X, ctx)
+    kw = dict(mbsize=65536, freq=10, reseed_after=5, with_replacement=False, seed=1)
+    ds.minibatch(mcg.SQRL2, C0, maxiter=2, **kw)   # warm-up (BF16 copies, scratch)
+    t0 = time.perf_counter()
+    r = ds.minibatch(mcg.SQRL2, C0, maxiter=50, **kw)
+    t = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ds.minibatch(mcg.SQRL2, C0, maxiter=1, **kw)
+    t1 = time.perf_counter() - t0
+    # updates only between the first and the last check-in
+    kw2 = dict(kw, freq=1000)
+    t0 = time.perf_counter()
+    r2 = ds.minibatch(mcg.SQRL2, C0, maxiter=50, **kw2)
+    t2 = time.perf_counter() - t0
+    return {"config": "minibatch", "n": n, "d": d, "k": k, "mbsize": kw["mbsize"], "iterations": r["numiter"],
+            "checkins": 6, "seconds": t, "ms_per_iteration": 1e3 * t / r["numiter"],
+            "first_checkin_ms": 1e3 * t1, "update_ms": 1e3 * (t2 - 2 * t1) / (r2["numiter"] - 1),
+            "initcost": r["initcost"], "finalcost": r["finalcost"],
+            "note": "first_checkin_ms: one full assignment of the 1e6 rows, restarts of the centers with <= 5 rows "
+                    "and the exact cost; update_ms: one draw + sort + gather + assignment + mean of a "
+                    "65,536-row minibatch (from a run with check-ins only at the first and last iteration)"}
+
+
 def cpu_baseline(X_host, C_host, n_full, nthreads):
     from oracle import mco
     ns = X_host.shape[0]
@@ -263,8 +302,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-workers", type=int, default=4)
-    ap.add_argument("--config", choices=["c3", "c2", "c4"], default="c3",
-                    help="c3: the headline dense Lloyd workload; c2/c4: auxiliary sparse measurements")
+    ap.add_argument("--config", choices=["c3", "c2", "c4", "minibatch"], default="c3",
+                    help="c3: the headline dense Lloyd workload; c2/c4: auxiliary sparse measurements; "
+                         "minibatch: the minibatch driver on the headline rows")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path would shard rows "
@@ -276,6 +316,9 @@ def main():
     from minicore_amd import _lib as mcg
     from minicore_amd.datagen import gaussian_mixture_device
 
+    if args.config == "minibatch":
+        print(json.dumps(minibatch_config(args, mcg)), flush=True)
+        return
     if args.config != "c3":
         print(json.dumps(sparse_config(args, mcg, args.config)), flush=True)
         return

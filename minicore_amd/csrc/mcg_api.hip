@@ -322,22 +322,19 @@ __global__ void k_count_asn(const uint32_t *__restrict__ asn, uint64_t n, uint32
         if (asn[i] < k) atomicAdd(&cnt[asn[i]], 1u);
 }
 
-// ||x_i - c||^2 as the sequential FMA chain of the exact E-step, one thread per row
-__global__ void k_cost_to_row_l2(const float *__restrict__ X, uint64_t n, uint32_t d, const float *__restrict__ c,
-                                 float *__restrict__ out) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float *x = X + i * d;
-    float s = 0.f;
-    for (uint32_t j = 0; j < d; ++j) { const float t = x[j] - c[j]; s = fmaf(t, t, s); }
-    out[i] = s;
-}
-
-// a row moves to center f when strictly cheaper (solve.h:490-496)
-__global__ void k_take_lower(const float *__restrict__ v, uint64_t n, uint32_t f, float *__restrict__ cost,
+// a row moves to its cheapest restarted center fid[a[i]] when that is strictly
+// cheaper than its current cost
+__global__ void k_take_lower(const float *__restrict__ v, const uint32_t *__restrict__ a,
+                             const uint32_t *__restrict__ fid, uint64_t n, float *__restrict__ cost,
                              uint32_t *__restrict__ asn) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        if (v[i] < cost[i]) { cost[i] = v[i]; asn[i] = f; }
+        if (v[i] < cost[i]) { cost[i] = v[i]; asn[i] = fid[a[i]]; }
+}
+
+__global__ void k_gather_rows_f32(const float *__restrict__ C, uint64_t d, const uint32_t *__restrict__ ids,
+                                  uint64_t m, float *__restrict__ out) {
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < m * d; t += (uint64_t)gridDim.x * blockDim.x)
+        out[t] = C[(uint64_t)ids[t / d] * d + t % d];
 }
 
 __global__ void k_gather_f64(const double *__restrict__ w, const uint64_t *__restrict__ idx, uint64_t m,
@@ -971,7 +968,7 @@ int mcg_minibatch(mcg_data *ds, int msr, double prior, float *centers, uint64_t 
         const uint64_t n = D.n, d = D.d, kd = k * d;
         REQUIRE(with_replacement || mbsize <= n, "mbsize exceeds the number of rows (draws without replacement)");
         const bool l2 = dense_l2(D, msr);
-        DBuf sw, sa_out, sk_out, bC, bSave, bA, bP, bTmp, bTa, bCnt, bIdx, bWs, bAs, bCs;
+        DBuf sw, sa_out, sk_out, bC, bSave, bA, bP, bTmp, bTa, bCnt, bIdx, bWs, bAs, bCs, bCf, bFid;
         const double *W = (const double *)in_dev(c, weights, n * 8, sw);
         std::vector<double> wh;
         if (weights) {
@@ -1012,18 +1009,22 @@ int mcg_minibatch(mcg_data *ds, int msr, double prior, float *centers, uint64_t 
                     copy_sync(hp.data(), P, n * 4, hipMemcpyDeviceToHost, st);
                     std::vector<double> wv(n);
                     for (uint64_t i = 0; i < n; ++i) wv[i] = weights ? wh[i] * (double)hp[i] : (double)hp[i];
-                    std::vector<uint64_t> rows(found.size());
-                    for (size_t f = 0; f < found.size(); ++f) rows[f] = draw_fixed(wv, wyrand_at(seed, j++));
+                    std::vector<uint64_t> rows(found.size()), rr(found.size());
+                    for (size_t f = 0; f < found.size(); ++f) rr[f] = wyrand_at(seed, j++);
+                    draw_fixed_many(wv, rr.data(), found.size(), rows.data());
                     copy_rows_to_centers(D, rows.data(), found.data(), found.size(), C, st);
-                    for (uint32_t f : found) {
-                        const float *cf = C + (uint64_t)f * d;
-                        if (l2)
-                            hipLaunchKernelGGL(k_cost_to_row_l2, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, D.X, n,
-                                               (uint32_t)d, cf, Tmp);
-                        else
-                            generic_costs(D, msr, prior, cf, 1, nullptr, Ta, Tmp, st);
-                        hipLaunchKernelGGL(k_take_lower, dim3(pgrid), dim3(256), 0, st, Tmp, n, f, P, A);
-                    }
+                    // rows move to the cheapest restarted center (first in id order on
+                    // ties) when it is strictly cheaper: the sequential scan of
+                    // solve.h:490-496 over the restarted centers, as one assignment
+                    const uint64_t nf = found.size();
+                    float *Cf = (float *)bCf.reserve(nf * d * 4);
+                    uint32_t *fid = (uint32_t *)bFid.reserve(nf * 4);
+                    copy_sync(fid, found.data(), nf * 4, hipMemcpyHostToDevice, st);
+                    hipLaunchKernelGGL(k_gather_rows_f32, dim3((uint32_t)std::min<uint64_t>(cdiv(nf * d, 256), 4096)),
+                                       dim3(256), 0, st, C, d, fid, nf, Cf);
+                    estep(D, msr, prior, Cf, nf, precision, Ta, l2 ? nullptr : Tmp, st);
+                    if (l2) dense_sqrl2_costs(D, Cf, Ta, Tmp, st);
+                    hipLaunchKernelGGL(k_take_lower, dim3(pgrid), dim3(256), 0, st, Tmp, Ta, fid, n, P, A);
                 }
                 const double cost = fixed_wsum(c, P, W, n, st);
                 if (iter == 0) init = best = cost;

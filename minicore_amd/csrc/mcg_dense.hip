@@ -139,8 +139,9 @@ void prepare_bf16(Data &ds) {
     ds.dpad = pad_dims(d);
     const uint32_t rows_per = 1024;
     const uint32_t nparts = (uint32_t)cdiv(n, rows_per);
-    DBuf part;
-    part.reserve((size_t)nparts * d * sizeof(double));
+    // scratch of the context (a freed buffer would synchronise the device)
+    DBuf &part = c.s[S_PREP];
+    part.reserve((size_t)nparts * d * sizeof(double) + 256);
     ds.mu.reserve(d * sizeof(float));
     dim3 g1((uint32_t)cdiv(d, 256), nparts);
     hipLaunchKernelGGL(k_colsum_partial, g1, dim3(256), 0, st, ds.X, n, d, rows_per, part.as<double>());
@@ -151,12 +152,11 @@ void prepare_bf16(Data &ds) {
     ds.xdev.reserve(n * sizeof(float));
     hipLaunchKernelGGL(k_make_xb, dim3((uint32_t)cdiv(n, 4)), dim3(256), 0, st, ds.X, n, d, ds.dpad,
                        ds.mu.as<float>(), ds.Xb.as<uint16_t>(), ds.xnorm.as<float>(), ds.xdev.as<float>());
-    DBuf mx;
-    mx.reserve(sizeof(float));
-    fill_u32(mx.as<uint32_t>(), 0u, 1, st);
+    unsigned int *mx = (unsigned int *)((char *)part.p + (size_t)nparts * d * sizeof(double));
+    fill_u32(mx, 0u, 1, st);
     hipLaunchKernelGGL(k_max_f32, dim3((uint32_t)std::min<uint64_t>(cdiv(n, 256), 2048)), dim3(256), 0, st,
-                       ds.xnorm.as<const float>(), n, mx.as<unsigned int>());
-    HIP_CHECK(hipMemcpyAsync(&ds.xmax, mx.p, sizeof(float), hipMemcpyDeviceToHost, st));
+                       ds.xnorm.as<const float>(), n, mx);
+    HIP_CHECK(hipMemcpyAsync(&ds.xmax, mx, sizeof(float), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
     ds.bf16_ready = true;
 }

@@ -124,7 +124,7 @@ enum Slot {
     S_HIST, S_BASE, S_PERM, S_CNT, S_ITEMOFF, S_PART, S_PARTW, S_PARTQ,
     S_CCOST, S_TOTAL, S_CENT_A, S_CENT_B, S_CENT_C, S_COSTS, S_CT, S_CPARAM,
     S_MISC0, S_MISC1, S_MISC2, S_MISC3, S_W, S_STAGE0, S_STAGE1, S_STAGE2,
-    S_LIS, S_TOT, S_JSDG, S_PAIRS, S_THRESH, S_OVER, S_KEYS, S_OFLAG, S_CINFO, S_CENT_D, S_LCTL, S_NSLOTS
+    S_LIS, S_TOT, S_JSDG, S_PAIRS, S_THRESH, S_OVER, S_KEYS, S_OFLAG, S_CINFO, S_CENT_D, S_LCTL, S_PREP, S_ROWLEN, S_NSLOTS
 };
 static_assert(S_NSLOTS <= 48, "scratch slots");
 
@@ -183,8 +183,9 @@ void coreset_sample_from(Ctx &ctx, uint64_t n, const float *probs_d, const doubl
                          uint64_t seed, uint64_t first, uint64_t *idx_d, float *wout_d, hipStream_t st);
 void restart_draws(Ctx &ctx, const float *costs_d, uint64_t n, uint64_t nempty, uint64_t *rows_h,
                    hipStream_t st);
-// Fixed-point inverse-CDF draw over host weights wv for raw draw r.
-uint64_t draw_fixed(const std::vector<double> &wv, uint64_t r);
+// Fixed-point inverse-CDF draws over host weights wv for raw draws r[0..m): the
+// weights are converted once, each draw is a search of their prefix sums.
+void draw_fixed_many(const std::vector<double> &wv, const uint64_t *r, uint64_t m, uint64_t *out);
 // Exact sum of w_i * v_i (v float on the device, w nullable): each term in fixed
 // point with the sampler's shift, integer sums; independent of the order.
 double fixed_wsum(Ctx &ctx, const float *v, const double *w, uint64_t n, hipStream_t st);

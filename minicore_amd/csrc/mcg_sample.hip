@@ -178,25 +178,24 @@ __global__ __launch_bounds__(1024) void k_fixed_select(const T *__restrict__ v, 
     // u < total, so some element of the selected block holds it
 }
 
-// Host-side draw from device values (used for the rare restart path).
-static uint64_t draw_host(const std::vector<double> &wv, uint64_t r) {
+// Host-side draws from device values (the empty-cluster and check-in restarts).
+void draw_fixed_many(const std::vector<double> &wv, const uint64_t *r, uint64_t m, uint64_t *out) {
+    // the fixed-point weights and their prefix once, then one search per draw: the
+    // first index whose inclusive prefix exceeds u
     const uint64_t n = wv.size();
     double mx = 0.;
     for (double x : wv) if (x > mx) mx = x;
     const int s = fixed_shift(mx, n);
+    std::vector<uint64_t> pre(n);
     uint64_t T = 0;
-    for (double x : wv) T += to_fixed(x, s);
-    if (T == 0) return 0;
-    const uint64_t u = mulhi64(r, T);
-    uint64_t acc = 0;
-    for (uint64_t i = 0; i < n; ++i) {
-        acc += to_fixed(wv[i], s);
-        if (acc > u) return i;
+    for (uint64_t i = 0; i < n; ++i) { T += to_fixed(wv[i], s); pre[i] = T; }
+    for (uint64_t q = 0; q < m; ++q) {
+        if (T == 0) { out[q] = 0; continue; }
+        const uint64_t u = mulhi64(r[q], T);
+        out[q] = (uint64_t)(std::upper_bound(pre.begin(), pre.end(), u) - pre.begin());
+        if (out[q] >= n) out[q] = n - 1;
     }
-    return n - 1;
 }
-
-uint64_t draw_fixed(const std::vector<double> &wv, uint64_t r) { return draw_host(wv, r); }
 
 double fixed_wsum(Ctx &c, const float *v, const double *w, uint64_t n, hipStream_t st) {
     const uint32_t nblk = (uint32_t)cdiv(n, FX_BLK);
@@ -224,7 +223,9 @@ void restart_draws(Ctx &c, const float *costs_d, uint64_t n, uint64_t nempty, ui
     HIP_CHECK(hipMemcpyAsync(cf.data(), costs_d, n * 4, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
     std::vector<double> wv(cf.begin(), cf.end());
-    for (uint64_t e = 0; e < nempty; ++e) rows_h[e] = draw_host(wv, wyrand_at(n, e));
+    std::vector<uint64_t> r(nempty);
+    for (uint64_t e = 0; e < nempty; ++e) r[e] = wyrand_at(n, e);
+    draw_fixed_many(wv, r.data(), nempty, rows_h);
     (void)c;
 }
 

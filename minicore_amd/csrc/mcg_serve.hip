@@ -83,7 +83,7 @@ void rows_into(const Data &S, const uint64_t *idx_h, const uint64_t *idx_d, uint
         HIP_CHECK(hipStreamSynchronize(st));
         return;
     }
-    DBuf len;
+    DBuf &len = S.ctx->s[S_ROWLEN];
     len.reserve(m * 8);
     hipLaunchKernelGGL(k_rows_len, dim3((uint32_t)cdiv(m, 256)), dim3(256), 0, st, S.ptr.as<const uint64_t>(), idx_d,
                        m, len.as<uint64_t>());

@@ -173,6 +173,21 @@ int mcg_minibatch(mcg_data *ds, int msr, double prior, float *centers, uint64_t 
                   uint64_t seed, int precision, uint32_t *asn, float *costs, double *initcost, double *finalcost,
                   uint64_t *numiter);
 
+/* Coreset-minibatch hard clustering: hmb_coreset_clustering
+ * (include/minicore/clustering/solve.h:598-789), reached from hcluster(...,
+ * mbsize >= 0, cs=True) (python/pycluster.h:37-39).  ceil(maxiter / freq) + 1
+ * check-ins as in mcg_minibatch (the saved centers change only when a later
+ * check-in is strictly cheaper); after each but the last, LBK sensitivities from
+ * the check-in (coreset.h:487-533) and freq coresets of min(mbsize, n) i.i.d.
+ * draws (seeded by the stream's next output, sorted), each clustered from the
+ * current centers by perform_hard_clustering with eps 1e-3 and maxiter
+ * 1 + 2*(round < 2) on the coreset rows with weights w_i * w_i / (m p_i)
+ * (solve.h:771-772).  Outputs as mcg_minibatch. */
+int mcg_cs_minibatch(mcg_data *ds, int msr, double prior, float *centers, uint64_t k, const double *weights,
+                     uint64_t mbsize, uint64_t maxiter, uint64_t freq, uint64_t reseed_after, uint64_t seed,
+                     int precision, uint32_t *asn, float *costs, double *initcost, double *finalcost,
+                     uint64_t *numiter);
+
 /* Soft EM driver: perform_soft_clustering, solve.h:287-374 with
  * set_centroids_full_mean (soft), centroid.h:781-963.  costs and resp are
  * n x k float32 outputs (nullable). */

@@ -23,7 +23,7 @@ EXPORTS = [
     "mcg_kmeanspp", "mcg_coreset_probs", "mcg_coreset_sample", "mcg_lloyd_step", "mcg_timing_enable",
     "mcg_timing_reset", "mcg_timing_read", "mcg_refine_counts", "mcg_data_rows", "mcg_server_create",
     "mcg_server_destroy", "mcg_submit", "mcg_poll", "mcg_wait", "mcg_cancel", "mcg_release", "mcg_tuning_enable",
-    "mcg_graphs_enable", "mcg_minibatch",
+    "mcg_graphs_enable", "mcg_minibatch", "mcg_cs_minibatch",
 ]
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = range(5)
@@ -109,6 +109,7 @@ def load():
         "mcg_tuning_enable": (I, [P, I]),
         "mcg_graphs_enable": (I, [P, I]),
         "mcg_minibatch": (I, [P, I, D, P, U64, P, U64, U64, U64, U64, I, U64, I, P, P, P, P, P]),
+        "mcg_cs_minibatch": (I, [P, I, D, P, U64, P, U64, U64, U64, U64, U64, I, P, P, P, P, P]),
         "mcg_timing_reset": (I, [P]),
         "mcg_timing_read": (I, [P, I, P, P]),
         "mcg_refine_counts": (I, [P, P, P]),
@@ -325,6 +326,19 @@ class Dataset:
         check(load().mcg_minibatch(self.h, msr, prior, ptr(Cc), k, ptr(w), mbsize, maxiter, freq, reseed_after,
                                    1 if with_replacement else 0, seed, precision, ptr(asn), ptr(costs),
                                    C.byref(ic), C.byref(fc), C.byref(it)))
+        return dict(centers=Cc, asn=asn, costs=costs, initcost=ic.value, finalcost=fc.value, numiter=it.value)
+
+    def cs_minibatch(self, msr, centers, prior=0., weights=None, mbsize=1000, maxiter=100, freq=10, reseed_after=1,
+                     seed=0, precision=PREC_BF16):
+        """hmb_coreset_clustering: best centers, last check-in asn/costs."""
+        Cc = np.ascontiguousarray(centers, dtype=np.float32).copy()
+        k = Cc.shape[0]
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        asn = np.empty(self.n, dtype=np.uint32)
+        costs = np.empty(self.n, dtype=np.float32)
+        ic, fc, it = C.c_double(), C.c_double(), C.c_uint64()
+        check(load().mcg_cs_minibatch(self.h, msr, prior, ptr(Cc), k, ptr(w), mbsize, maxiter, freq, reseed_after,
+                                      seed, precision, ptr(asn), ptr(costs), C.byref(ic), C.byref(fc), C.byref(it)))
         return dict(centers=Cc, asn=asn, costs=costs, initcost=ic.value, finalcost=fc.value, numiter=it.value)
 
     def lloyd_step(self, msr, centers_dev, asn_dev, prior=0., weights=None, precision=PREC_BF16):

@@ -337,6 +337,11 @@ __global__ void k_gather_rows_f32(const float *__restrict__ C, uint64_t d, const
         out[t] = C[(uint64_t)ids[t / d] * d + t % d];
 }
 
+__global__ void k_f32_to_f64_copy(const float *__restrict__ a, uint64_t n, double *__restrict__ b) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        b[i] = (double)a[i];
+}
+
 __global__ void k_gather_f64(const double *__restrict__ w, const uint64_t *__restrict__ idx, uint64_t m,
                              double *__restrict__ out) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -942,16 +947,68 @@ int mcg_lloyd(mcg_data *ds, int msr, double prior, float *centers, uint64_t k, c
     });
 }
 
+// The check-in of the minibatch drivers (solve.h:445-509, 644-710): every row
+// assigned (A, P), each center with <= reseed_after rows restarted at a row drawn
+// over w_i * cost_i with the next outputs of the driver's stream (one fixed-point
+// prefix for all of them), rows moved to the cheapest restarted center when that is
+// strictly cheaper (the reference's sequential scan over the restarted centers,
+// as one assignment), and the exact fixed-point total of w_i * cost_i returned.
+struct CheckIn {
+    Data &D;
+    int msr;
+    double prior;
+    uint64_t k;
+    int precision;
+    const double *W;                 // device weights or null
+    const std::vector<double> &wh;   // host weights (empty without)
+    uint64_t reseed_after;
+    uint32_t *A, *Ta, *cnt, *fid;
+    float *P, *Tmp, *Cf;
+    std::vector<uint32_t> hc;
+    std::vector<float> hp;
+
+    double run(float *C, uint64_t seed, uint64_t &j, hipStream_t st) {
+        Ctx &c = *D.ctx;
+        const uint64_t n = D.n, d = D.d;
+        const bool l2 = dense_l2(D, msr);
+        estep(D, msr, prior, C, k, precision, A, l2 ? nullptr : P, st);
+        if (l2) dense_sqrl2_costs(D, C, A, P, st);
+        fill_u32(cnt, 0u, k, st);
+        const uint32_t pgrid = (uint32_t)std::min<uint64_t>(cdiv(n, 256), 4096);
+        hipLaunchKernelGGL(k_count_asn, dim3(pgrid), dim3(256), 0, st, A, n, (uint32_t)k, cnt);
+        hc.resize(k);
+        copy_sync(hc.data(), cnt, k * 4, hipMemcpyDeviceToHost, st);
+        std::vector<uint32_t> found;
+        for (uint64_t q = 0; q < k; ++q) if (hc[q] <= reseed_after) found.push_back((uint32_t)q);
+        if (!found.empty()) {
+            const uint64_t nf = found.size();
+            hp.resize(n);
+            copy_sync(hp.data(), P, n * 4, hipMemcpyDeviceToHost, st);
+            std::vector<double> wv(n);
+            for (uint64_t i = 0; i < n; ++i) wv[i] = wh.empty() ? (double)hp[i] : wh[i] * (double)hp[i];
+            std::vector<uint64_t> rows(nf), rr(nf);
+            for (uint64_t f = 0; f < nf; ++f) rr[f] = wyrand_at(seed, j++);
+            draw_fixed_many(wv, rr.data(), nf, rows.data());
+            copy_rows_to_centers(D, rows.data(), found.data(), nf, C, st);
+            copy_sync(fid, found.data(), nf * 4, hipMemcpyHostToDevice, st);
+            hipLaunchKernelGGL(k_gather_rows_f32, dim3((uint32_t)std::min<uint64_t>(cdiv(nf * d, 256), 4096)),
+                               dim3(256), 0, st, C, d, fid, nf, Cf);
+            estep(D, msr, prior, Cf, nf, precision, Ta, l2 ? nullptr : Tmp, st);
+            if (l2) dense_sqrl2_costs(D, Cf, Ta, Tmp, st);
+            hipLaunchKernelGGL(k_take_lower, dim3(pgrid), dim3(256), 0, st, Tmp, Ta, fid, n, P, A);
+        }
+        return fixed_wsum(c, P, W, n, st);
+    }
+};
+
 // perform_hard_minibatch_clustering (solve.h:377-578) for the mean-centroid
 // measures.  One RNG stream (the wyrand counter stream of seed) serves the restart
-// draws and the minibatch draws in the reference's order.  A check-in (iteration 0,
-// every freq-th and the last) assigns every row, restarts each center with <=
-// reseed_after rows at a row drawn over w_i * cost_i, moves rows to a restarted
-// center that is strictly cheaper, and keeps the centers of the lowest check-in
-// cost (exact fixed-point sum); between check-ins mbsize rows are drawn (without
-// replacement: the first mbsize distinct draws), sorted, assigned, and every
-// center with members becomes their weighted mean.  The draws and the check-in
-// bookkeeping run on the host; rows, costs and centers stay on the device.
+// draws and the minibatch draws in the reference's order.  Check-ins at iteration
+// 0, every freq-th and the last keep the centers of the lowest check-in cost;
+// between check-ins mbsize rows are drawn (without replacement: the first mbsize
+// distinct draws), sorted, assigned, and every center with members becomes their
+// weighted mean.  The draws and the check-in bookkeeping run on the host; rows,
+// costs and centers stay on the device.
 int mcg_minibatch(mcg_data *ds, int msr, double prior, float *centers, uint64_t k, const double *weights,
                   uint64_t mbsize, uint64_t maxiter, uint64_t freq, uint64_t reseed_after, int with_replacement,
                   uint64_t seed, int precision, uint32_t *asn, float *costs, double *initcost, double *finalcost,
@@ -977,9 +1034,9 @@ int mcg_minibatch(mcg_data *ds, int msr, double prior, float *centers, uint64_t 
             else std::memcpy(wh.data(), weights, n * 8);
         }
         float *C = (float *)bC.reserve(kd * 4), *Save = (float *)bSave.reserve(kd * 4);
-        uint32_t *A = (uint32_t *)bA.reserve(n * 4), *Ta = (uint32_t *)bTa.reserve(n * 4);
-        float *P = (float *)bP.reserve(n * 4), *Tmp = (float *)bTmp.reserve(n * 4);
-        uint32_t *cnt = (uint32_t *)bCnt.reserve(k * 4);
+        CheckIn ck{D, msr, prior, k, precision, W, wh, reseed_after, (uint32_t *)bA.reserve(n * 4),
+                   (uint32_t *)bTa.reserve(n * 4), (uint32_t *)bCnt.reserve(k * 4), (uint32_t *)bFid.reserve(k * 4),
+                   (float *)bP.reserve(n * 4), (float *)bTmp.reserve(n * 4), (float *)bCf.reserve(kd * 4), {}, {}};
         uint64_t *idx_d = (uint64_t *)bIdx.reserve(mbsize * 8);
         double *Ws = W ? (double *)bWs.reserve(mbsize * 8) : nullptr;
         uint32_t *As = (uint32_t *)bAs.reserve(mbsize * 4);
@@ -990,43 +1047,11 @@ int mcg_minibatch(mcg_data *ds, int msr, double prior, float *centers, uint64_t 
         Data sub;
         std::vector<uint64_t> sel(mbsize);
         std::vector<uint8_t> seen(with_replacement ? 0 : n);
-        std::vector<uint32_t> hc(k);
-        std::vector<float> hp;
         uint64_t j = 0, iter = 0;
         double init = DBL_MAX, best = DBL_MAX;
-        const uint32_t pgrid = (uint32_t)std::min<uint64_t>(cdiv(n, 256), 4096);
         for (;;) {
             if (iter % freq == 0 || iter == maxiter - 1) {
-                estep(D, msr, prior, C, k, precision, A, l2 ? nullptr : P, st);
-                if (l2) dense_sqrl2_costs(D, C, A, P, st);
-                fill_u32(cnt, 0u, k, st);
-                hipLaunchKernelGGL(k_count_asn, dim3(pgrid), dim3(256), 0, st, A, n, (uint32_t)k, cnt);
-                copy_sync(hc.data(), cnt, k * 4, hipMemcpyDeviceToHost, st);
-                std::vector<uint32_t> found;
-                for (uint64_t q = 0; q < k; ++q) if (hc[q] <= reseed_after) found.push_back((uint32_t)q);
-                if (!found.empty()) {
-                    hp.resize(n);
-                    copy_sync(hp.data(), P, n * 4, hipMemcpyDeviceToHost, st);
-                    std::vector<double> wv(n);
-                    for (uint64_t i = 0; i < n; ++i) wv[i] = weights ? wh[i] * (double)hp[i] : (double)hp[i];
-                    std::vector<uint64_t> rows(found.size()), rr(found.size());
-                    for (size_t f = 0; f < found.size(); ++f) rr[f] = wyrand_at(seed, j++);
-                    draw_fixed_many(wv, rr.data(), found.size(), rows.data());
-                    copy_rows_to_centers(D, rows.data(), found.data(), found.size(), C, st);
-                    // rows move to the cheapest restarted center (first in id order on
-                    // ties) when it is strictly cheaper: the sequential scan of
-                    // solve.h:490-496 over the restarted centers, as one assignment
-                    const uint64_t nf = found.size();
-                    float *Cf = (float *)bCf.reserve(nf * d * 4);
-                    uint32_t *fid = (uint32_t *)bFid.reserve(nf * 4);
-                    copy_sync(fid, found.data(), nf * 4, hipMemcpyHostToDevice, st);
-                    hipLaunchKernelGGL(k_gather_rows_f32, dim3((uint32_t)std::min<uint64_t>(cdiv(nf * d, 256), 4096)),
-                                       dim3(256), 0, st, C, d, fid, nf, Cf);
-                    estep(D, msr, prior, Cf, nf, precision, Ta, l2 ? nullptr : Tmp, st);
-                    if (l2) dense_sqrl2_costs(D, Cf, Ta, Tmp, st);
-                    hipLaunchKernelGGL(k_take_lower, dim3(pgrid), dim3(256), 0, st, Tmp, Ta, fid, n, P, A);
-                }
-                const double cost = fixed_wsum(c, P, W, n, st);
+                const double cost = ck.run(C, seed, j, st);
                 if (iter == 0) init = best = cost;
                 if (cost < best) {
                     best = cost;
@@ -1055,8 +1080,108 @@ int mcg_minibatch(mcg_data *ds, int msr, double prior, float *centers, uint64_t 
         }
         Out oa = out_dev(c, asn, n * 4, sa_out);
         Out ok = out_dev(c, costs, n * 4, sk_out);
-        if (oa.dev) HIP_CHECK(hipMemcpyAsync(oa.dev, A, n * 4, hipMemcpyDeviceToDevice, st));
-        if (ok.dev) HIP_CHECK(hipMemcpyAsync(ok.dev, P, n * 4, hipMemcpyDeviceToDevice, st));
+        if (oa.dev) HIP_CHECK(hipMemcpyAsync(oa.dev, ck.A, n * 4, hipMemcpyDeviceToDevice, st));
+        if (ok.dev) HIP_CHECK(hipMemcpyAsync(ok.dev, ck.P, n * 4, hipMemcpyDeviceToDevice, st));
+        out_finish(c, oa);
+        out_finish(c, ok);
+        HIP_CHECK(hipMemcpyAsync(centers, Save, kd * 4, is_device_ptr(centers) ? hipMemcpyDeviceToDevice
+                                                                              : hipMemcpyDeviceToHost, st));
+        sync(c);
+        *initcost = init;
+        *finalcost = best;
+        *numiter = iter;
+    });
+}
+
+// hmb_coreset_clustering (solve.h:598-789).  ceil(maxiter / freq) + 1 check-ins
+// (the saved centers change only when a later check-in is strictly cheaper); after
+// each but the last, LBK sensitivities from the check-in's costs and assignment
+// (make_sampler without an alias sampler), then freq coresets: m = min(mbsize, n)
+// i.i.d. draws seeded by the stream's next output (sample_k with replacement,
+// sorted), weights w_i * w_i / (m p_i) (solve.h:771-772 multiplies the row weights
+// into coreset weights that already hold them), and perform_hard_clustering on the
+// coreset rows from the current centers, eps 1e-3, maxiter 1 + 2 (round < 2).
+int mcg_cs_minibatch(mcg_data *ds, int msr, double prior, float *centers, uint64_t k, const double *weights,
+                     uint64_t mbsize, uint64_t maxiter, uint64_t freq, uint64_t reseed_after, uint64_t seed,
+                     int precision, uint32_t *asn, float *costs, double *initcost, double *finalcost,
+                     uint64_t *numiter) {
+    return guard([&] {
+        REQUIRE(ds && centers && initcost && finalcost && numiter, "null argument");
+        REQUIRE(k > 0, "k must be positive");
+        REQUIRE(mbsize > 0 && freq > 0, "mbsize and the check-in period must be positive");
+        check_msr(msr);
+        Data &D = ds->d;
+        Ctx &c = *D.ctx;
+        hipStream_t st = c.stream;
+        HIP_CHECK(hipSetDevice(c.device));
+        const uint64_t n = D.n, d = D.d, kd = k * d;
+        const uint64_t rounds = (freq - 1 + maxiter) / freq, m = std::min(mbsize, n);
+        DBuf sw, sa_out, sk_out, bC, bSave, bA, bP, bTmp, bTa, bCnt, bCf, bFid, bCost, bProbs, bIdx, bCw, bIdxS,
+            bCsw, bAs;
+        const double *W = (const double *)in_dev(c, weights, n * 8, sw);
+        std::vector<double> wh;
+        if (weights) {
+            wh.resize(n);
+            if (is_device_ptr(weights)) copy_sync(wh.data(), weights, n * 8, hipMemcpyDeviceToHost, st);
+            else std::memcpy(wh.data(), weights, n * 8);
+        }
+        float *C = (float *)bC.reserve(kd * 4), *Save = (float *)bSave.reserve(kd * 4);
+        CheckIn ck{D, msr, prior, k, precision, W, wh, reseed_after, (uint32_t *)bA.reserve(n * 4),
+                   (uint32_t *)bTa.reserve(n * 4), (uint32_t *)bCnt.reserve(k * 4), (uint32_t *)bFid.reserve(k * 4),
+                   (float *)bP.reserve(n * 4), (float *)bTmp.reserve(n * 4), (float *)bCf.reserve(kd * 4), {}, {}};
+        double *cost64 = (double *)bCost.reserve(n * 8);
+        float *probs = (float *)bProbs.reserve(n * 4);
+        uint64_t *idx_d = (uint64_t *)bIdx.reserve(m * 8), *idx_s = (uint64_t *)bIdxS.reserve(m * 8);
+        float *cw = (float *)bCw.reserve(m * 4);
+        double *csw = (double *)bCsw.reserve(m * 8);
+        uint32_t *As = (uint32_t *)bAs.reserve(m * 4);
+        HIP_CHECK(hipMemcpyAsync(C, centers, kd * 4, is_device_ptr(centers) ? hipMemcpyDeviceToDevice
+                                                                           : hipMemcpyHostToDevice, st));
+        copy_f32(C, Save, kd, st);
+        mcg_data sub;
+        std::vector<uint64_t> hi(m);
+        std::vector<float> hw(m);
+        std::vector<std::pair<uint64_t, double>> pr(m);
+        std::vector<uint64_t> si(m);
+        std::vector<double> sw_h(m);
+        uint64_t j = 0, iter = 0;
+        double init = DBL_MAX, best = DBL_MAX;
+        const uint32_t pgrid = (uint32_t)std::min<uint64_t>(cdiv(n, 256), 4096);
+        for (;;) {
+            const double cost = ck.run(C, seed, j, st);
+            if (iter == 0) init = best = cost;
+            else if (cost < best) {
+                best = cost;
+                copy_f32(C, Save, kd, st);
+            }
+            if (++iter > rounds) break;
+            hipLaunchKernelGGL(k_f32_to_f64_copy, dim3(pgrid), dim3(256), 0, st, ck.P, n, cost64);
+            coreset_probs_run(c, MCG_SENS_LBK, n, k, cost64, ck.A, W, probs, st);
+            for (uint64_t jj = 0; jj < freq; ++jj) {
+                const uint64_t r = wyrand_at(seed, j++);
+                coreset_sample_from(c, n, probs, W, m, r, 0, idx_d, cw, st);
+                copy_sync(hi.data(), idx_d, m * 8, hipMemcpyDeviceToHost, st);
+                copy_sync(hw.data(), cw, m * 4, hipMemcpyDeviceToHost, st);
+                for (uint64_t t = 0; t < m; ++t) pr[t] = {hi[t], (wh.empty() ? 1. : wh[hi[t]]) * (double)hw[t]};
+                std::stable_sort(pr.begin(), pr.end(),
+                                 [](const std::pair<uint64_t, double> &a, const std::pair<uint64_t, double> &b) {
+                                     return a.first < b.first;
+                                 });
+                for (uint64_t t = 0; t < m; ++t) { si[t] = pr[t].first; sw_h[t] = pr[t].second; }
+                copy_sync(idx_s, si.data(), m * 8, hipMemcpyHostToDevice, st);
+                copy_sync(csw, sw_h.data(), m * 8, hipMemcpyHostToDevice, st);
+                rows_into(D, si.data(), idx_s, m, sub.d, st);
+                double ic = 0., fc = 0.;
+                uint64_t it = 0;
+                const int rc = mcg_lloyd(&sub, msr, prior, C, k, csw, 1e-3, 1 + 2 * (iter < 2), precision, As,
+                                         nullptr, &ic, &fc, &it);
+                if (rc != MCG_OK) MCG_THROW(rc, g_err);
+            }
+        }
+        Out oa = out_dev(c, asn, n * 4, sa_out);
+        Out ok = out_dev(c, costs, n * 4, sk_out);
+        if (oa.dev) HIP_CHECK(hipMemcpyAsync(oa.dev, ck.A, n * 4, hipMemcpyDeviceToDevice, st));
+        if (ok.dev) HIP_CHECK(hipMemcpyAsync(ok.dev, ck.P, n * 4, hipMemcpyDeviceToDevice, st));
         out_finish(c, oa);
         out_finish(c, ok);
         HIP_CHECK(hipMemcpyAsync(centers, Save, kd * 4, is_device_ptr(centers) ? hipMemcpyDeviceToDevice

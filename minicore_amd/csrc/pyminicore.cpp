@@ -279,20 +279,18 @@ py::object centers_csr_tuple(const std::vector<float> &C, uint64_t k, uint64_t d
     return py::make_tuple(py::make_tuple(data, idx, indptr), shape);
 }
 
-void no_coreset_minibatch(bool cs) {
-    if (cs) throw py::value_error("coreset-minibatch clustering (cs=True) is not on the GPU path");
-}
-
 // minibatch arguments of hcluster (python/pycluster.h:31-42)
 struct MiniArgs {
     py::ssize_t mbsize, ncheckins, reseed_count;
     bool with_rep;
     uint64_t seed;
+    bool cs;
 };
 
 // ---- hcluster -------------------------------------------------------------------------
-// mbsize < 0: perform_hard_clustering; otherwise perform_hard_minibatch_clustering
-// with the check-in period ceil(maxiter / ncheckins) (ncheckins < 0 means 10).
+// mbsize < 0: perform_hard_clustering; otherwise perform_hard_minibatch_clustering,
+// or hmb_coreset_clustering with cs, with the check-in period ceil(maxiter /
+// ncheckins) (ncheckins < 0 means 10).
 py::dict run_hard(mcg_data *h, uint64_t n, uint64_t d, std::vector<float> &C, uint64_t k, double prior,
                   int msr, py::object w, double eps, uint64_t maxiter, const MiniArgs &mb) {
     py::array_t<uint32_t> asn(n);
@@ -306,9 +304,15 @@ py::dict run_hard(mcg_data *h, uint64_t n, uint64_t d, std::vector<float> &C, ui
         const uint64_t nck = mb.ncheckins < 0 ? 10 : (uint64_t)mb.ncheckins;
         if (nck == 0) throw std::invalid_argument("ncheckins must be positive");
         const uint64_t freq = (maxiter + nck - 1) / nck;
-        check(mcg_minibatch(h, msr, prior, C.data(), k, wptr(w), (uint64_t)mb.mbsize, maxiter, freq ? freq : 1,
-                            mb.reseed_count < 0 ? 0 : (uint64_t)mb.reseed_count, mb.with_rep ? 1 : 0, mb.seed,
-                            g_sqrl2_precision, asn.mutable_data(), costs.mutable_data(), &ic, &fc, &it));
+        const uint64_t reseed = mb.reseed_count < 0 ? 0 : (uint64_t)mb.reseed_count;
+        if (mb.cs)
+            check(mcg_cs_minibatch(h, msr, prior, C.data(), k, wptr(w), (uint64_t)mb.mbsize, maxiter, freq ? freq : 1,
+                                   reseed, mb.seed, g_sqrl2_precision, asn.mutable_data(), costs.mutable_data(), &ic,
+                                   &fc, &it));
+        else
+            check(mcg_minibatch(h, msr, prior, C.data(), k, wptr(w), (uint64_t)mb.mbsize, maxiter, freq ? freq : 1,
+                                reseed, mb.with_rep ? 1 : 0, mb.seed, g_sqrl2_precision, asn.mutable_data(),
+                                costs.mutable_data(), &ic, &fc, &it));
     }
     return py::dict("initcost"_a = ic, "finalcost"_a = fc, "numiter"_a = it, "costs"_a = costs, "asn"_a = asn);
 }
@@ -316,7 +320,6 @@ py::dict run_hard(mcg_data *h, uint64_t n, uint64_t d, std::vector<float> &C, ui
 py::object hcluster_dense(py::array dataset, py::object centers, double prior, py::object msr, py::object weights,
                           double eps, uint64_t maxiter, uint64_t seed, py::ssize_t mbsize, py::ssize_t ncheckins,
                           py::ssize_t reseed_count, bool with_rep, bool cs) {
-    no_coreset_minibatch(cs);
     const int m = measure_code(msr);
     Dense D = dense_of(dataset);
     uint64_t k = 0;
@@ -328,7 +331,7 @@ py::object hcluster_dense(py::array dataset, py::object centers, double prior, p
     if (k == 0) throw std::invalid_argument("no centers");
     auto w = weights_f64(weights, D.n);
     py::dict r = run_hard(D.dev->h, D.n, D.d, C, k, prior, m, w, eps, maxiter,
-                          MiniArgs{mbsize, ncheckins, reseed_count, with_rep, seed});
+                          MiniArgs{mbsize, ncheckins, reseed_count, with_rep, seed, cs});
     if (D.f64) {
         py::array_t<double> ctr({(py::ssize_t)k, (py::ssize_t)D.d});
         for (uint64_t q = 0; q < k * D.d; ++q) ctr.mutable_data()[q] = C[q];
@@ -344,14 +347,13 @@ py::object hcluster_dense(py::array dataset, py::object centers, double prior, p
 py::object hcluster_csr(PyCSparseMatrix &smw, py::object centers, double prior, py::object msr, py::object weights,
                         double eps, uint64_t maxiter, uint64_t seed, py::ssize_t mbsize, py::ssize_t ncheckins,
                         py::ssize_t reseed_count, bool with_rep, bool cs) {
-    no_coreset_minibatch(cs);
     const int m = measure_code(msr);
     uint64_t k = 0;
     auto C = center_block(centers, smw.nc, [&](const std::vector<int64_t> &ids) { return smw.dense_rows(ids); }, &k);
     if (k == 0) throw std::invalid_argument("no centers");
     auto w = weights_f64(weights, smw.nr);
     py::dict r = run_hard(smw.device(), smw.nr, smw.nc, C, k, prior, m, w, eps, maxiter,
-                          MiniArgs{mbsize, ncheckins, reseed_count, with_rep, seed});
+                          MiniArgs{mbsize, ncheckins, reseed_count, with_rep, seed, cs});
     r["centers"] = centers_csr_tuple(C, k, smw.nc);
     return r;
 }

@@ -609,6 +609,117 @@ int mco_minibatch(int msr, const float *X, const float *val, const uint32_t *idx
 }
 
 /* ======================================================================
+ * Coreset-minibatch hard clustering: hmb_coreset_clustering, solve.h:598-789.
+ * Outer rounds: ceil(maxiter / freq) + 1 check-ins (the check-in of the
+ * minibatch driver above; the saved centers change only when a later
+ * check-in is strictly cheaper).  After each check-in but the last: LBK
+ * sensitivities from the check-in's costs and assignment (coreset.h:487-533,
+ * make_sampler without an alias sampler), then freq times: a coreset of
+ * m = min(mbsize, n) i.i.d. draws with the stream's next output as the draw
+ * seed (reservoir_simd::sample_k WITH_REPLACEMENT, sorted ascending), weights
+ * w_i/(m p_i) times w_i again (solve.h:771-772 multiplies the row weights into
+ * the coreset weights, which already hold them), and perform_hard_clustering
+ * on the coreset rows from the current centers with eps 1e-3 and maxiter
+ * 1 + 2*(round < 2) (solve.h:768).
+ * ====================================================================== */
+static int cmp_pair_u64(const void *a, const void *b) {
+    const uint64_t x = ((const uint64_t *)a)[0], y = ((const uint64_t *)b)[0];
+    return x < y ? -1 : x > y;
+}
+
+int mco_cs_minibatch(int msr, const float *X, const float *val, const uint32_t *idx, const uint64_t *indptr,
+                     uint64_t n, uint64_t d, float *C, uint64_t k, double prior, const double *w,
+                     uint64_t mbsize, uint64_t maxiter, uint64_t freq, uint64_t reseed_after,
+                     uint64_t seed, uint32_t *asn, double *cost, double *initcost, double *finalcost,
+                     uint64_t *numiter, int nthreads) {
+    if (!mbsize || !freq) return -2;
+    const uint64_t rounds = (freq - 1 + maxiter) / freq, m = mbsize < n ? mbsize : n;
+    uint64_t j = 0, iter = 0;
+    double init = DBL_MAX, best = DBL_MAX;
+    int rc = 0;
+    float *save = (float *)malloc(k * d * sizeof(float));
+    memcpy(save, C, k * d * sizeof(float));
+    uint64_t *cnt = (uint64_t *)malloc(k * sizeof(uint64_t));
+    float *probs = (float *)malloc(n * sizeof(float));
+    uint64_t *ind = (uint64_t *)malloc(m * sizeof(uint64_t)), *pairs = (uint64_t *)malloc(2 * m * sizeof(uint64_t));
+    float *wt = (float *)malloc(m * sizeof(float));
+    double *csw = (double *)malloc(m * sizeof(double)), *sc = (double *)malloc(m * sizeof(double));
+    uint32_t *sa = (uint32_t *)malloc(m * sizeof(uint32_t));
+    float *Xs = val ? NULL : (float *)malloc(m * d * sizeof(float));
+    for (;;) {
+        assign_generic(msr, X, val, idx, indptr, n, d, C, k, prior, asn, cost, nthreads);
+        memset(cnt, 0, k * sizeof(uint64_t));
+        for (uint64_t i = 0; i < n; ++i) cnt[asn[i]]++;
+        uint64_t nf = 0;
+        for (uint64_t c = 0; c < k; ++c) if (cnt[c] <= reseed_after) ++nf;
+        if (nf) {
+            uint32_t *found = (uint32_t *)malloc(nf * sizeof(uint32_t));
+            double *wv = (double *)malloc(n * sizeof(double));
+            uint64_t *fw = (uint64_t *)malloc(n * sizeof(uint64_t));
+            for (uint64_t i = 0; i < n; ++i) wv[i] = w ? w[i] * cost[i] : cost[i];
+            mco_fixed_weights(wv, n, fw);
+            uint64_t f = 0;
+            for (uint64_t c = 0; c < k; ++c) if (cnt[c] <= reseed_after) {
+                const uint64_t r = mco_fixed_draw(fw, n, mco_wyrand_at(seed, j++));
+                found[f++] = (uint32_t)c;
+                if (val) {
+                    memset(C + c * d, 0, d * sizeof(float));
+                    for (uint64_t p = indptr[r]; p < indptr[r + 1]; ++p) C[c * d + idx[p]] = val[p];
+                } else memcpy(C + c * d, X + r * d, d * sizeof(float));
+            }
+            #pragma omp parallel for schedule(dynamic, 64)
+            for (int64_t i = 0; i < (int64_t)n; ++i)
+                for (uint64_t q = 0; q < nf; ++q) {
+                    const double v = point_cost(msr, X, val, idx, indptr, i, d, C + (uint64_t)found[q] * d, prior);
+                    if (v < cost[i]) { cost[i] = v; asn[i] = found[q]; }
+                }
+            free(found); free(wv); free(fw);
+        }
+        const double ct = fixed_total(cost, w, n);
+        if (iter == 0) init = best = ct;
+        else if (ct < best) { best = ct; memcpy(save, C, k * d * sizeof(float)); }
+        if (++iter > rounds) break;
+        if (mco_coreset_probs(4, n, k, cost, asn, w, probs) != 0) { rc = -3; break; }
+        for (uint64_t jj = 0; jj < freq && !rc; ++jj) {
+            const uint64_t r = mco_wyrand_at(seed, j++);
+            mco_coreset_sample(n, probs, w, m, r, 0, ind, wt);
+            for (uint64_t t = 0; t < m; ++t) { pairs[2 * t] = ind[t]; pairs[2 * t + 1] = t; }
+            qsort(pairs, m, 2 * sizeof(uint64_t), cmp_pair_u64);
+            for (uint64_t t = 0; t < m; ++t) {
+                const uint64_t i = pairs[2 * t];
+                csw[t] = (w ? w[i] : 1.) * (double)wt[pairs[2 * t + 1]];
+                ind[t] = i;
+            }
+            double ic, fc; uint64_t it;
+            const uint64_t sub = 1 + 2 * (iter < 2);
+            if (val) {
+                uint64_t *ps = (uint64_t *)malloc((m + 1) * sizeof(uint64_t));
+                ps[0] = 0;
+                for (uint64_t t = 0; t < m; ++t) ps[t + 1] = ps[t] + (indptr[ind[t] + 1] - indptr[ind[t]]);
+                float *vs = (float *)malloc((ps[m] ? ps[m] : 1) * sizeof(float));
+                uint32_t *is = (uint32_t *)malloc((ps[m] ? ps[m] : 1) * sizeof(uint32_t));
+                for (uint64_t t = 0; t < m; ++t) {
+                    memcpy(vs + ps[t], val + indptr[ind[t]], (ps[t + 1] - ps[t]) * sizeof(float));
+                    memcpy(is + ps[t], idx + indptr[ind[t]], (ps[t + 1] - ps[t]) * sizeof(uint32_t));
+                }
+                if (mco_lloyd(msr, NULL, vs, is, ps, m, d, C, k, prior, csw, 1e-3, sub, sa, sc, &ic, &fc, &it,
+                              nthreads)) rc = -1;
+                free(ps); free(vs); free(is);
+            } else {
+                for (uint64_t t = 0; t < m; ++t) memcpy(Xs + t * d, X + ind[t] * d, d * sizeof(float));
+                if (mco_lloyd(msr, Xs, NULL, NULL, NULL, m, d, C, k, prior, csw, 1e-3, sub, sa, sc, &ic, &fc, &it,
+                              nthreads)) rc = -1;
+            }
+        }
+        if (rc) break;
+    }
+    memcpy(C, save, k * d * sizeof(float));
+    *initcost = init; *finalcost = best; *numiter = iter;
+    free(save); free(cnt); free(probs); free(ind); free(pairs); free(wt); free(csw); free(sc); free(sa); free(Xs);
+    return rc;
+}
+
+/* ======================================================================
  * Soft clustering.  centroid.h:781-793 (correct_softmax) and 795-963;
  * solve.h:287-374.  softmax is exp(-temp*cost) / row sum in float without
  * max subtraction; a row with any NaN becomes uniform over its argmin ties.

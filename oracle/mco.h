@@ -94,6 +94,14 @@ int mco_minibatch(int msr, const float *X, const float *val, const uint32_t *idx
                   uint64_t seed, uint32_t *asn, double *cost, double *initcost, double *finalcost,
                   uint64_t *numiter, int nthreads);
 
+/* Coreset-minibatch driver, solve.h:598-789 (see mco.c).  Returns -1 on a NaN
+ * center in a coreset Lloyd run, -2 for invalid sizes. */
+int mco_cs_minibatch(int msr, const float *X, const float *val, const uint32_t *idx, const uint64_t *indptr,
+                     uint64_t n, uint64_t d, float *C, uint64_t k, double prior, const double *w,
+                     uint64_t mbsize, uint64_t maxiter, uint64_t freq, uint64_t reseed_after,
+                     uint64_t seed, uint32_t *asn, double *cost, double *initcost, double *finalcost,
+                     uint64_t *numiter, int nthreads);
+
 /* ---- soft clustering: solve.h:287-374, centroid.h:781-963 --------------- */
 int mco_soft(int msr, const float *X, const float *val, const uint32_t *idx,
              const uint64_t *indptr, uint64_t n, uint64_t d, float *C, uint64_t k,

@@ -57,6 +57,8 @@ def _setup(L):
         "mco_lloyd": (I, [I, P, P, P, P, U64, U64, P, U64, D, P, D, U64, P, P, P, P, P, I]),
         "mco_minibatch": (I, [I, P, P, P, P, U64, U64, P, U64, D, P, U64, U64, U64, U64, I, U64, P, P, P, P, P,
                               I]),
+        "mco_cs_minibatch": (I, [I, P, P, P, P, U64, U64, P, U64, D, P, U64, U64, U64, U64, U64, P, P, P, P, P,
+                                 I]),
         "mco_soft": (I, [I, P, P, P, P, U64, U64, P, U64, D, D, P, D, U64, P, P, P, P, P]),
         "mco_softmax_rows": (None, [P, U64, U64, D, P]),
         "mco_kmeanspp": (U64, [I, P, P, P, P, U64, U64, U64, D, U64, P, P, P, P, I]),
@@ -218,6 +220,24 @@ def minibatch(msr, X, C0, prior=0., w=None, mbsize=100, maxiter=20, freq=5, rese
                              nthreads)
     if rc != 0:
         raise ValueError("oracle minibatch: invalid sizes")
+    return dict(centers=C, asn=asn, costs=cost, initcost=float(ic[0]), finalcost=float(fc[0]), numiter=int(it[0]))
+
+
+def cs_minibatch(msr, X, C0, prior=0., w=None, mbsize=100, maxiter=20, freq=5, reseed_after=0, seed=0,
+                 nthreads=0):
+    """hmb_coreset_clustering (solve.h:598-789) as restated in mco.c."""
+    Xd, val, idx, ptr, n, d = _mat(X)
+    C = _f32(C0).copy()
+    k = C.shape[0]
+    w = _f64(w)
+    asn = np.empty(n, dtype=np.uint32)
+    cost = np.empty(n, dtype=np.float64)
+    ic, fc = np.zeros(1), np.zeros(1)
+    it = np.zeros(1, dtype=np.uint64)
+    rc = lib().mco_cs_minibatch(msr, _p(Xd), _p(val), _p(idx), _p(ptr), n, d, _p(C), k, prior, _p(w), mbsize,
+                                maxiter, freq, reseed_after, seed, _p(asn), _p(cost), _p(ic), _p(fc), _p(it), nthreads)
+    if rc != 0:
+        raise ValueError(f"oracle coreset minibatch failed ({rc})")
     return dict(centers=C, asn=asn, costs=cost, initcost=float(ic[0]), finalcost=float(fc[0]), numiter=int(it[0]))
 
 

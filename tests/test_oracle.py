@@ -271,6 +271,35 @@ def test_minibatch_one_update(mco, with_rep):
     assert np.array_equal(r["asn"], a1)
 
 
+@pytest.mark.parametrize("weighted", [False, True])
+def test_cs_minibatch_one_round(mco, weighted):
+    """One outer round of the coreset-minibatch driver is: check-in; LBK
+    probabilities from its costs and assignment; a coreset of m draws seeded by the
+    stream's first output, sorted, weighted w_i * w_i/(m p_i); Lloyd on it from the
+    current centers (eps 1e-3, maxiter 3 in the first rounds); check-in; the
+    cheaper check-in's centers are returned."""
+    from minicore_amd.datagen import gaussian_mixture
+    n, m, seed = 4000, 500, 13
+    X, lab, _ = gaussian_mixture(n, 6, 5, seed=2, lo=-4, hi=4)
+    w = np.random.default_rng(4).uniform(0.5, 2., n) if weighted else None
+    C0 = X[[int(np.argmax(lab == c)) for c in range(5)]].copy()
+    r = mco.cs_minibatch(mco.SQRL2, X, C0, w=w, mbsize=m, maxiter=1, freq=1, reseed_after=0, seed=seed)
+    assert r["numiter"] == 2
+    a0, c0, _ = mco.assign_sqrl2_f32(X, C0)
+    probs = mco.coreset_probs(4, 5, c0.astype(np.float64), a0, w)
+    wt, ind = mco.coreset_sample(probs, m, mco.wyrand_at(seed, 0), w)
+    order = np.argsort(ind, kind="stable")
+    ind, wt = ind[order], wt[order]
+    csw = (w[ind] if weighted else 1.) * wt.astype(np.float64)
+    L = mco.lloyd(mco.SQRL2, X[ind.astype(np.int64)], C0, w=csw, eps=1e-3, maxiter=3)
+    a1, c1, _ = mco.assign_sqrl2_f32(X, L["centers"])
+    mult = w if weighted else 1.
+    t0, t1 = _fixed_total(c0.astype(np.float64) * mult), _fixed_total(c1.astype(np.float64) * mult)
+    assert r["initcost"] == t0 and r["finalcost"] == min(t0, t1)
+    assert np.array_equal(r["centers"], L["centers"] if t1 < t0 else C0)
+    assert np.array_equal(r["asn"], a1)
+
+
 # ---- a5: soft -------------------------------------------------------------------------
 def test_softmax_rows_and_repair(mco):
     rng = np.random.default_rng(2)

@@ -264,7 +264,15 @@ def minibatch_config(args, mcg):
     t0 = time.perf_counter()
     r2 = ds.minibatch(mcg.SQRL2, C0, maxiter=50, **kw2)
     t2 = time.perf_counter() - t0
+    # the coreset-minibatch driver: 100 iterations in 10 check-in rounds of 10
+    # coresets of 65,536 rows (Lloyd with maxiter 3 / 1 on each)
+    t0 = time.perf_counter()
+    rc = ds.cs_minibatch(mcg.SQRL2, C0, mbsize=65536, maxiter=100, freq=10, reseed_after=5, seed=1)
+    tc = time.perf_counter() - t0
+    cs = {"rounds": rc["numiter"], "coresets": 100, "seconds": tc, "ms_per_coreset_round": 1e3 * tc / 100,
+          "initcost": rc["initcost"], "finalcost": rc["finalcost"]}
     return {"config": "minibatch", "n": n, "d": d, "k": k, "mbsize": kw["mbsize"], "iterations": r["numiter"],
+            "coreset_minibatch": cs,
             "checkins": 6, "seconds": t, "ms_per_iteration": 1e3 * t / r["numiter"],
             "first_checkin_ms": 1e3 * t1, "update_ms": 1e3 * (t2 - 2 * t1) / (r2["numiter"] - 1),
             "initcost": r["initcost"], "finalcost": r["finalcost"],

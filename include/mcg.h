@@ -54,6 +54,30 @@ extern "C" {
 #define MCG_JSD          4
 #define MCG_MKL          5
 #define MCG_REVERSE_MKL 11
+/* further measures (applicator.h:1673-1760 dense, 1778-1891 sparse), evaluated pair by
+ * pair.  Hard clustering uses the weighted-mean centroid for those mapped to a mean
+ * policy by msr2pol (centroid.h:182-207); L1, L2 and TVD (median policies) and the
+ * similarity / probability-cosine forms (no policy) are cost-only here: mcg_cost_matrix,
+ * mcg_assign_hard, mcg_kmeanspp. */
+#define MCG_L1                            0
+#define MCG_L2                            1
+#define MCG_JSM                           3
+#define MCG_HELLINGER                     6
+#define MCG_BHATTACHARYYA_METRIC          7
+#define MCG_BHATTACHARYYA_DISTANCE        8
+#define MCG_TVD                           9
+#define MCG_LLR                          10
+#define MCG_UWLLR                        12
+#define MCG_IS                           13
+#define MCG_REVERSE_IS                   14
+#define MCG_COSINE_DISTANCE              15
+#define MCG_PROBABILITY_COSINE_DISTANCE  16
+#define MCG_COSINE_SIMILARITY            17
+#define MCG_PROBABILITY_COSINE_SIMILARITY 18
+#define MCG_SIS                          23
+#define MCG_RSIS                         24
+#define MCG_SRLRT                        25
+#define MCG_SRULRT                       26
 
 /* cross-term precision for dense SQRL2 */
 #define MCG_PREC_FP32  0   /* FP32 MFMA cross term, FP32 compare ("plain FP32 form") */

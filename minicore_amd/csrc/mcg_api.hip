@@ -529,8 +529,14 @@ static void estep(Data &ds, int msr, double prior, const float *C, uint64_t k, i
 }
 
 static void check_msr(int msr) {
-    REQUIRE(msr == MCG_SQRL2 || msr == MCG_JSD || msr == MCG_MKL || msr == MCG_REVERSE_MKL,
-            "measure must be SQRL2 (2), JSD (4), MKL (5) or REVERSE_MKL (11)");
+    REQUIRE(msr_supported(msr), "unsupported measure code (supported: 0-15, 16-18, 23-26; see distance.h)");
+}
+
+// hard clustering needs a mean-centroid measure (centroid.h:182-207)
+static void check_mean_msr(int msr) {
+    check_msr(msr);
+    REQUIRE(msr_mean_centroid(msr), "this measure's centroid policy (L1/TVD/geometric median, or none) is not "
+                                    "on the GPU path; its costs are (mcg_cost_matrix, mcg_assign_hard)");
 }
 
 }  // namespace mcg
@@ -804,7 +810,7 @@ int mcg_lloyd(mcg_data *ds, int msr, double prior, float *centers, uint64_t k, c
     return guard([&] {
         REQUIRE(ds && centers && asn, "null argument");
         REQUIRE(k > 0, "k must be positive");
-        check_msr(msr);
+        check_mean_msr(msr);
         Data &D = ds->d;
         Ctx &c = *D.ctx;
         hipStream_t st = c.stream;
@@ -1017,7 +1023,7 @@ int mcg_minibatch(mcg_data *ds, int msr, double prior, float *centers, uint64_t 
         REQUIRE(ds && centers && initcost && finalcost && numiter, "null argument");
         REQUIRE(k > 0, "k must be positive");
         REQUIRE(mbsize > 0 && maxiter > 0 && freq > 0, "mbsize, maxiter and the check-in period must be positive");
-        check_msr(msr);
+        check_mean_msr(msr);
         Data &D = ds->d;
         Ctx &c = *D.ctx;
         hipStream_t st = c.stream;
@@ -1109,7 +1115,7 @@ int mcg_cs_minibatch(mcg_data *ds, int msr, double prior, float *centers, uint64
         REQUIRE(ds && centers && initcost && finalcost && numiter, "null argument");
         REQUIRE(k > 0, "k must be positive");
         REQUIRE(mbsize > 0 && freq > 0, "mbsize and the check-in period must be positive");
-        check_msr(msr);
+        check_mean_msr(msr);
         Data &D = ds->d;
         Ctx &c = *D.ctx;
         hipStream_t st = c.stream;
@@ -1200,7 +1206,7 @@ int mcg_soft_em(mcg_data *ds, int msr, double prior, double temp, float *centers
     return guard([&] {
         REQUIRE(ds && centers, "null argument");
         REQUIRE(k > 0, "k must be positive");
-        check_msr(msr);
+        check_mean_msr(msr);
         Data &D = ds->d;
         Ctx &c = *D.ctx;
         hipStream_t st = c.stream;
@@ -1353,7 +1359,7 @@ int mcg_lloyd_step(mcg_data *ds, int msr, double prior, float *centers, uint64_t
     return guard([&] {
         REQUIRE(ds && centers && asn, "null argument");
         REQUIRE(k > 0, "k must be positive");
-        check_msr(msr);
+        check_mean_msr(msr);
         REQUIRE(is_device_ptr(centers) && is_device_ptr(asn), "mcg_lloyd_step needs device centers and asn");
         Data &D = ds->d;
         Ctx &c = *D.ctx;

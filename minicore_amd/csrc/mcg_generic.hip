@@ -490,6 +490,119 @@ __global__ __launch_bounds__(256) void k_pairs_direct(int msr, const float *__re
     }
 }
 
+// The further measures, pair by pair in FP32 (FT = float) over every coordinate,
+// following applicator.h:1642-1760 with a = smoothed center, b = smoothed row:
+// HELLINGER sqrt(sum (sqrt a - sqrt b)^2) / sqrt 2; BHATTACHARYYA from sum sqrt(a b)
+// (1 if within 1e-8 of 1; metric sqrt(1 - s), distance -log s); TVD sum |a - b| / 2;
+// L1 sum |c - x|; L2 sqrt(sum (c - x)^2); cosine (c + pv, x + pv) or, for the
+// probability forms, (a, b), clamped to [0, 1], distances acos / pi; the LLR family
+// sum lambda a log(a/m) + (1-lambda) b log(b/m) with lambda = lhsum / (lhsum + rhsum)
+// and m = lambda a + (1-lambda) b (LLR and SRLRT times lhsum + rhsum; SRLRT, SRULRT
+// square roots); IS sum a/b - log(a/b) - 1 (the -1 per coordinate, not as -d at the
+// end, which would cancel in FP32), REVERSE_IS swapped; SIS and REVERSE_SIS
+// sum log((a/b + b/a + 2) / 4) / 2; JSM sqrt(JSD).  One wave per row, lanes over
+// centers; a CSR row is walked along the dense coordinates.
+__global__ __launch_bounds__(256) void k_pairs_more(int msr, const float *__restrict__ X, const float *__restrict__ val,
+                                                    const uint32_t *__restrict__ idx, const uint64_t *__restrict__ ptr,
+                                                    uint64_t n, uint32_t d, const double *__restrict__ rsum,
+                                                    const float *__restrict__ C, uint32_t k,
+                                                    const double *__restrict__ csum, float prior,
+                                                    float *__restrict__ out, uint32_t *__restrict__ asn,
+                                                    float *__restrict__ best_out) {
+    const uint64_t row = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    float best = 0.f;
+    uint32_t bi = 0xFFFFFFFFu;
+    const float xs = (float)rsum[row];
+    for (uint32_t c0 = 0; c0 < k; c0 += 64) {
+        const uint32_t c = c0 + lane;
+        const uint32_t cc = c < k ? c : k - 1;
+        const float cs = (float)csum[cc];
+        const float psum0 = prior * (float)d;
+        float lhsum = cs + psum0, rhsum = xs + psum0;
+        const float pv = fmaxf(prior, 7.0069e-42f * (lhsum + rhsum));
+        const float psum = pv * (float)d;
+        lhsum = cs + psum;
+        rhsum = xs + psum;
+        const float lhrsi = 1.f / lhsum, rhrsi = 1.f / rhsum;
+        float lhinc = (pv != 0.f && lhsum != 0.f) ? pv * lhrsi : 0.f;
+        float rhinc = (pv != 0.f && rhsum != 0.f) ? pv * rhrsi : 0.f;
+        if (lhinc != lhinc) lhinc = 0.f;
+        if (rhinc != rhinc) rhinc = 0.f;
+        const float lam = lhsum / (lhsum + rhsum), lam1 = 1.f - lam;
+        float s = 0.f, s2 = 0.f, s3 = 0.f;
+        uint64_t p = val ? ptr[row] : 0, pe = val ? ptr[row + 1] : 0;
+        const float *cr = C + (uint64_t)cc * d;
+        for (uint32_t j = 0; j < d; ++j) {
+            float x;
+            if (val) { x = (p < pe && idx[p] == j) ? val[p++] : 0.f; }
+            else x = X[row * d + j];
+            const float cj = cr[j];
+            const float a = cj * lhrsi + lhinc, b = x * rhrsi + rhinc;
+            switch (msr) {
+            case MCG_L1: s += fabsf(cj - x); break;
+            case MCG_L2: { const float t = cj - x; s = fmaf(t, t, s); } break;
+            case MCG_HELLINGER: { const float t = sqrtf(a) - sqrtf(b); s = fmaf(t, t, s); } break;
+            case MCG_BHATTACHARYYA_METRIC: case MCG_BHATTACHARYYA_DISTANCE: s += sqrtf(a * b); break;
+            case MCG_TVD: s += fabsf(a - b); break;
+            case MCG_COSINE_DISTANCE: case MCG_COSINE_SIMILARITY: {
+                const float u = cj + pv, v = x + pv;
+                s = fmaf(u, v, s); s2 = fmaf(u, u, s2); s3 = fmaf(v, v, s3);
+            } break;
+            case MCG_PROBABILITY_COSINE_DISTANCE: case MCG_PROBABILITY_COSINE_SIMILARITY:
+                s = fmaf(a, b, s); s2 = fmaf(a, a, s2); s3 = fmaf(b, b, s3); break;
+            case MCG_JSM: { const float m = a + b; s += 0.5f * (a * logf(2.f * a / m) + b * logf(2.f * b / m)); } break;
+            case MCG_LLR: case MCG_UWLLR: case MCG_SRLRT: case MCG_SRULRT: {
+                const float m = lam * a + lam1 * b;
+                s += (a != 0.f ? lam * a * logf(a / m) : 0.f) + (b != 0.f ? lam1 * b * logf(b / m) : 0.f);
+            } break;
+            case MCG_IS: { const float r = a / b; s += (r - logf(r)) - 1.f; } break;
+            case MCG_REVERSE_IS: { const float r = b / a; s += (r - logf(r)) - 1.f; } break;
+            default: s += 0.5f * logf((a / b + b / a + 2.f) * 0.25f); break;   // SIS, RSIS
+            }
+        }
+        float v;
+        switch (msr) {
+        case MCG_L1: v = s; break;
+        case MCG_L2: v = sqrtf(s); break;
+        case MCG_HELLINGER: v = sqrtf(s) * 0.70710678f; break;
+        case MCG_BHATTACHARYYA_METRIC: case MCG_BHATTACHARYYA_DISTANCE:
+            if (1.f - s < 1e-8f) s = 1.f;
+            v = msr == MCG_BHATTACHARYYA_METRIC ? sqrtf(fmaxf(1.f - s, 0.f)) : -logf(s);
+            break;
+        case MCG_TVD: v = 0.5f * s; break;
+        case MCG_COSINE_DISTANCE: case MCG_COSINE_SIMILARITY:
+        case MCG_PROBABILITY_COSINE_DISTANCE: case MCG_PROBABILITY_COSINE_SIMILARITY: {
+            float r = s / sqrtf(s2 * s3);
+            r = r > 1.f ? 1.f : (r < 0.f ? 0.f : r);
+            if (msr == MCG_COSINE_DISTANCE || msr == MCG_PROBABILITY_COSINE_DISTANCE) r = acosf(r) * 0.31830988618f;
+            v = r;
+        } break;
+        default: {   // KL family tail (applicator.h:1750-1760)
+            float r = s;
+            if (msr == MCG_LLR || msr == MCG_SRLRT) r *= (lhsum + rhsum);
+            r = r < 0.f ? 0.f : r;
+            if (msr == MCG_SRULRT || msr == MCG_SRLRT || msr == MCG_JSM) r = sqrtf(r);
+            v = r == INFINITY ? FLT_MAX : r;
+        } break;
+        }
+        if (c < k) {
+            if (out) out[row * k + c] = v;
+            if (bi == 0xFFFFFFFFu || v < best) { best = v; bi = c; }
+        }
+    }
+    if (asn) {
+        for (int o = 32; o > 0; o >>= 1) {
+            const float ob = __shfl_xor(best, o);
+            const uint32_t oi = __shfl_xor(bi, o);
+            const bool take = (oi != 0xFFFFFFFFu) && (bi == 0xFFFFFFFFu || ob < best || (ob == best && oi < bi));
+            if (take) { best = ob; bi = oi; }
+        }
+        if (lane == 0) { asn[row] = bi; if (best_out) best_out[row] = best; }
+    }
+}
+
 __global__ void k_center_sums(const float *__restrict__ C, uint32_t k, uint32_t d, double *__restrict__ out) {
     const uint64_t c = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -600,7 +713,7 @@ static void pp_update_launch(const Cost1 &P, uint64_t n, int msr, const uint64_t
 bool generic_pp_round(Data &ds, int msr, double prior, const float *center_row, const uint32_t *cent, uint32_t ci,
                       const uint64_t *newc, float *cc, float *dist, uint32_t *asn, int first, const double *w,
                       unsigned long long *mx, hipStream_t st) {
-    if (msr != MCG_SQRL2 && prior < DIRECT_PRIOR) return false;
+    if (!msr_decomposed(msr) || (msr != MCG_SQRL2 && prior < DIRECT_PRIOR)) return false;
     const double pv = msr == MCG_SQRL2 ? 0. : prior;
     const Cost1 P = center_tables(ds, msr, pv, center_row, 1, 1, st);
     const uint64_t n = ds.n;
@@ -622,6 +735,14 @@ void generic_costs(Data &ds, int msr, double prior, const float *C, uint64_t k, 
     const uint32_t *idx = ds.csr ? ds.idx.as<const uint32_t>() : nullptr;
     const uint64_t *ptr = ds.csr ? ds.ptr.as<const uint64_t>() : nullptr;
     REQUIRE(k < (1ull << 31), "k too large");
+    if (!msr_decomposed(msr)) {
+        rowsums(ds, st);
+        double *csum = (double *)c.s[S_CPARAM].reserve(k * sizeof(double));
+        hipLaunchKernelGGL(k_center_sums, dim3((uint32_t)cdiv(k, 4)), dim3(256), 0, st, C, (uint32_t)k, d, csum);
+        hipLaunchKernelGGL(k_pairs_more, dim3((uint32_t)cdiv(n, 4)), dim3(256), 0, st, msr, X, val, idx, ptr, n, d,
+                           ds.rowsum.as<const double>(), C, (uint32_t)k, csum, (float)prior, out_nk, asn, best);
+        return;
+    }
     if (msr != MCG_SQRL2 && prior < DIRECT_PRIOR) {
         rowsums(ds, st);
         double *csum = (double *)c.s[S_CPARAM].reserve(k * sizeof(double));

@@ -162,6 +162,33 @@ MStepOut dense_mstep(Data &ds, const uint32_t *asn, const double *w, uint64_t k,
 MStepOut csr_mstep(Data &ds, const uint32_t *asn, const double *w, uint64_t k, float *Cnew,
                    hipStream_t st);
 
+// Measures with a kernel here: the four with decomposed kernels, and the further ones
+// evaluated pair by pair.
+inline bool msr_decomposed(int msr) {
+    return msr == MCG_SQRL2 || msr == MCG_JSD || msr == MCG_MKL || msr == MCG_REVERSE_MKL;
+}
+inline bool msr_supported(int msr) {
+    switch (msr) {
+    case MCG_L1: case MCG_L2: case MCG_SQRL2: case MCG_JSM: case MCG_JSD: case MCG_MKL: case MCG_HELLINGER:
+    case MCG_BHATTACHARYYA_METRIC: case MCG_BHATTACHARYYA_DISTANCE: case MCG_TVD: case MCG_LLR:
+    case MCG_REVERSE_MKL: case MCG_UWLLR: case MCG_IS: case MCG_REVERSE_IS: case MCG_COSINE_DISTANCE:
+    case MCG_PROBABILITY_COSINE_DISTANCE: case MCG_COSINE_SIMILARITY: case MCG_PROBABILITY_COSINE_SIMILARITY:
+    case MCG_SIS: case MCG_RSIS: case MCG_SRLRT: case MCG_SRULRT:
+        return true;
+    }
+    return false;
+}
+// msr2pol (centroid.h:182-207): the measures whose centers are weighted means
+inline bool msr_mean_centroid(int msr) {
+    switch (msr) {
+    case MCG_UWLLR: case MCG_LLR: case MCG_MKL: case MCG_JSD: case MCG_SQRL2: case MCG_REVERSE_MKL: case MCG_IS:
+    case MCG_REVERSE_IS: case MCG_SIS: case MCG_RSIS: case MCG_SRULRT: case MCG_SRLRT: case MCG_JSM:
+    case MCG_COSINE_DISTANCE: case MCG_BHATTACHARYYA_METRIC: case MCG_BHATTACHARYYA_DISTANCE: case MCG_HELLINGER:
+        return true;
+    }
+    return false;
+}
+
 // Generic-measure kernels (dense or CSR rows vs dense centers), FP32.
 void generic_costs(Data &ds, int msr, double prior, const float *C, uint64_t k, float *out_nk,
                    uint32_t *asn, float *best, hipStream_t st);

@@ -502,8 +502,7 @@ int mcg_submit(mcg_server *s, const mcg_request *req, uint64_t *ticket) {
         const mcg_request &q = *req;
         REQUIRE(q.data && q.n > 0 && q.d > 0 && q.k > 0, "request needs rows and k > 0");
         REQUIRE(q.centers && q.asn, "request needs centers and asn outputs");
-        REQUIRE(q.msr == MCG_SQRL2 || q.msr == MCG_JSD || q.msr == MCG_MKL || q.msr == MCG_REVERSE_MKL,
-                "unsupported measure");
+        REQUIRE(msr_supported(q.msr) && msr_mean_centroid(q.msr), "unsupported measure (needs a mean centroid)");
         REQUIRE(q.coreset_frac > 0. && q.coreset_frac <= 1., "coreset_frac must be in (0, 1]");
         REQUIRE(q.sens == MCG_SENS_BFL || q.sens == MCG_SENS_FL || q.sens == MCG_SENS_VX || q.sens == MCG_SENS_LBK,
                 "unsupported sensitivity method");

@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <set>
 #include <memory>
 #include <random>
 #include <stdexcept>
@@ -59,16 +60,24 @@ int measure_code(py::object msr) {
     if (py::isinstance<py::str>(msr)) {
         std::string s = msr.cast<std::string>();
         std::transform(s.begin(), s.end(), s.begin(), ::toupper);
+        // prob2str names (distance.h:273-304) and the enum aliases (distance.h:52-62)
         static const std::map<std::string, int> names = {
-            {"SQRL2", 2}, {"JSD", 4}, {"PSD", 4}, {"MKL", 5}, {"REVERSE_MKL", 11}, {"RMKL", 11}};
+            {"L1", 0}, {"L2", 1}, {"SQRL2", 2}, {"JSM", 3}, {"PSM", 3}, {"JSD", 4}, {"PSD", 4}, {"MKL", 5},
+            {"HELLINGER", 6}, {"BHATTACHARYYA_METRIC", 7}, {"BHATTACHARYYA_DISTANCE", 8},
+            {"TOTAL_VARIATION_DISTANCE", 9}, {"TVD", 9}, {"LLR", 10}, {"WLLR", 10}, {"REVERSE_MKL", 11},
+            {"RMKL", 11}, {"UWLLR", 12}, {"ITAKURA_SAITO", 13}, {"IS", 13}, {"REVERSE_ITAKURA_SAITO", 14},
+            {"COSINE_DISTANCE", 15}, {"PROBABILITY_COSINE_DISTANCE", 16}, {"COSINE_SIMILARITY", 17},
+            {"PROBABILITY_COSINE_SIMILARITY", 18}, {"SYMMETRIC_ITAKURA_SAITO", 23}, {"SIS", 23},
+            {"RSYMMETRIC_ITAKURA_SAITO", 24}, {"REVERSE_SYMMETRIC_ITAKURA_SAITO", 24}, {"RSIS", 24}, {"SRLRT", 25},
+            {"SRULRT", 26}};
         auto it = names.find(s);
-        if (it == names.end()) throw std::invalid_argument("measure " + s + " has no GPU kernel (supported: SQRL2, JSD, MKL, REVERSE_MKL)");
+        if (it == names.end()) throw std::invalid_argument("measure " + s + " has no GPU kernel");
         code = it->second;
     } else {
         code = msr.cast<int>();
     }
-    if (code != 2 && code != 4 && code != 5 && code != 11)
-        throw std::invalid_argument("measure " + std::to_string(code) + " has no GPU kernel (supported: 2, 4, 5, 11)");
+    static const std::set<int> ok = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 23, 24, 25, 26};
+    if (!ok.count(code)) throw std::invalid_argument("measure " + std::to_string(code) + " has no GPU kernel");
     return code;
 }
 

@@ -81,7 +81,7 @@ uint64_t mco_fixed_draw(const uint64_t *fw, uint64_t n, uint64_t r) {
  * first = data row x ("ctr" in the reference's parameter list), second =
  * center c ("mr"); lhsum/lhinc belong to the center, rhsum/rhinc to the row.
  * ====================================================================== */
-typedef struct { double pv, lhrsi, rhrsi, lhinc, rhinc; } smooth_t;
+typedef struct { double pv, lhrsi, rhrsi, lhinc, rhinc, lhsum, rhsum; } smooth_t;
 
 /* applicator.h:1642-1656 */
 static smooth_t smoothing(double prior, uint64_t d, double xsum, double csum) {
@@ -92,6 +92,7 @@ static smooth_t smoothing(double prior, uint64_t d, double xsum, double csum) {
     s.pv = prior > smallest ? prior : smallest;
     prior_sum = s.pv * (double)d;
     lhsum = csum + prior_sum; rhsum = xsum + prior_sum;
+    s.lhsum = lhsum; s.rhsum = rhsum;
     s.lhrsi = 1. / lhsum; s.rhrsi = 1. / rhsum;
     s.lhinc = (s.pv != 0. && lhsum != 0.) ? s.pv * s.lhrsi : 0.;
     s.rhinc = (s.pv != 0. && rhsum != 0.) ? s.pv * s.rhrsi : 0.;
@@ -122,8 +123,80 @@ static double post_kl(double r) {
     return r;
 }
 
+/* ---- the further measures, applicator.h:1673-1760 (dense rows: every coordinate).
+ * a = smoothed center value (lhrsi, lhinc), b = smoothed row value (rhrsi, rhinc).
+ * Per coordinate: HELLINGER (sqrt a - sqrt b)^2, BHATTACHARYYA sqrt(a b), TVD |a - b|/2,
+ * L1 |c - x|, LLR family (lambda = lhsum / (lhsum + rhsum), m = lambda a + (1-lambda) b)
+ * lambda a log(a/m) + (1-lambda) b log(b/m), IS a/b - log(a/b) - 1 (REVERSE_IS with a, b
+ * swapped), SIS log((a/b + b/a + 2)/4) / 2; cosine: dot / (norm norm) of (c + pv, x + pv)
+ * or of (a, b) for the probability forms. */
+static double llr_term(double a, double b, double lam) {
+    const double m = lam * a + (1. - lam) * b;
+    return (a != 0. ? lam * a * log(a / m) : 0.) + (b != 0. ? (1. - lam) * b * log(b / m) : 0.);
+}
+
+/* applicator.h:1736-1760: the KL-family tail (LLR scaling, clamp, square roots, infinities) */
+static double kl_tail(int msr, double klc, const smooth_t *sm) {
+    double ret = klc;
+    if (msr == MCO_LLR || msr == MCO_SRLRT) ret *= (sm->lhsum + sm->rhsum);
+    ret = ret < 0. ? 0. : ret;
+    if (msr == MCO_SRULRT || msr == MCO_SRLRT || msr == MCO_JSM) ret = sqrt(ret);
+    if (ret == INFINITY) ret = DBL_MAX;
+    return ret;
+}
+
+static double msr_more_dense(int msr, const double *x, const double *c, uint64_t d, double prior, double xsum,
+                             double csum) {
+    const smooth_t sm = smoothing(prior, d, xsum, csum);
+    const double lam = sm.lhsum / (sm.lhsum + sm.rhsum);
+    double s = 0., s2 = 0., s3 = 0.;
+    for (uint64_t j = 0; j < d; ++j) {
+        const double a = c[j] * sm.lhrsi + sm.lhinc, b = x[j] * sm.rhrsi + sm.rhinc;
+        switch (msr) {
+        case MCO_L1: s += fabs(c[j] - x[j]); break;
+        case MCO_L2: { const double t = c[j] - x[j]; s += t * t; } break;
+        case MCO_HELLINGER: { const double t = sqrt(a) - sqrt(b); s += t * t; } break;
+        case MCO_BHATTACHARYYA_METRIC: case MCO_BHATTACHARYYA_DISTANCE: s += sqrt(a * b); break;
+        case MCO_TVD: s += fabs(a - b); break;
+        case MCO_COSINE_DISTANCE: case MCO_COSINE_SIMILARITY: {
+            const double u = c[j] + sm.pv, v = x[j] + sm.pv;
+            s += u * v; s2 += u * u; s3 += v * v;
+        } break;
+        case MCO_PROBABILITY_COSINE_DISTANCE: case MCO_PROBABILITY_COSINE_SIMILARITY:
+            s += a * b; s2 += a * a; s3 += b * b; break;
+        case MCO_JSD: case MCO_JSM: s += coord_term(MCO_JSD, a, b); break;
+        case MCO_LLR: case MCO_UWLLR: case MCO_SRLRT: case MCO_SRULRT: s += llr_term(a, b, lam); break;
+        case MCO_IS: s += a / b - log(a / b); break;
+        case MCO_REVERSE_IS: s += b / a - log(b / a); break;
+        case MCO_SIS: case MCO_RSIS: s += .5 * log((a / b + b / a + 2.) * .25); break;
+        default: return NAN;
+        }
+    }
+    switch (msr) {
+    case MCO_L1: return s;
+    case MCO_L2: return sqrt(s);
+    case MCO_HELLINGER: return sqrt(s) * 0.70710678118654752440;
+    case MCO_BHATTACHARYYA_METRIC: case MCO_BHATTACHARYYA_DISTANCE: {
+        if (1. - s < 1e-8) s = 1.;
+        return msr == MCO_BHATTACHARYYA_METRIC ? sqrt(1. - s > 0. ? 1. - s : 0.) : -log(s);
+    }
+    case MCO_TVD: return .5 * s;
+    case MCO_COSINE_DISTANCE: case MCO_COSINE_SIMILARITY:
+    case MCO_PROBABILITY_COSINE_DISTANCE: case MCO_PROBABILITY_COSINE_SIMILARITY: {
+        double r = s / sqrt(s2 * s3);
+        r = r > 1. ? 1. : (r < 0. ? 0. : r);
+        if (msr == MCO_COSINE_DISTANCE || msr == MCO_PROBABILITY_COSINE_DISTANCE) r = acos(r) * 0.31830988618379067153;
+        return r;
+    }
+    case MCO_IS: case MCO_REVERSE_IS: return kl_tail(msr, s - (double)d, &sm);
+    default: return kl_tail(msr, s, &sm);
+    }
+}
+
 double mco_msr_dense(int msr, const double *x, const double *c, uint64_t d,
                      double prior, double xsum, double csum) {
+    if (msr != MCO_SQRL2 && msr != MCO_MKL && msr != MCO_REVERSE_MKL && msr != MCO_JSD)
+        return msr_more_dense(msr, x, c, d, prior, xsum, csum);
     if (msr == MCO_SQRL2) {                       /* applicator.h:1680-1685 */
         double s = 0.;
         for (uint64_t j = 0; j < d; ++j) { double t = c[j] - x[j]; s += t * t; }
@@ -152,12 +225,88 @@ uint64_t mco_shared_zeros(const uint32_t *a, uint64_t an, const uint32_t *b, uin
     return d - u;
 }
 
+/* applicator.h:1778-1891 for the further measures: union terms plus the closed-form
+ * shared-zero terms the reference adds (the cosine forms use the union only, and
+ * REVERSE_SIS swaps the vectors but not the increments, as the reference does). */
+static double msr_more_sparse(int msr, const double *xv, const uint32_t *xi, uint64_t xn, const double *cv,
+                              const uint32_t *ci, uint64_t cn, uint64_t d, double prior, double xsum, double csum,
+                              uint64_t sharednz) {
+    const smooth_t sm = smoothing(prior, d, xsum, csum);
+    const double lam = sm.lhsum / (sm.lhsum + sm.rhsum), li = sm.lhinc, ri = sm.rhinc;
+    double s = 0., s2 = 0., s3 = 0.;
+    uint64_t i = 0, j = 0;
+    while (i < xn || j < cn) {
+        double x, c;
+        if (j >= cn || (i < xn && xi[i] < ci[j])) { x = xv[i++]; c = 0.; }
+        else if (i >= xn || ci[j] < xi[i])        { x = 0.; c = cv[j++]; }
+        else                                      { x = xv[i++]; c = cv[j++]; }
+        const double a = c * sm.lhrsi + li, b = x * sm.rhrsi + ri;
+        switch (msr) {
+        case MCO_L1: s += fabs((c + sm.pv) - (x + sm.pv)); break;
+        case MCO_L2: { const double t = c - x; s += t * t; } break;
+        case MCO_HELLINGER: { const double t = sqrt(a) - sqrt(b); s += t * t; } break;
+        case MCO_BHATTACHARYYA_METRIC: case MCO_BHATTACHARYYA_DISTANCE: s += sqrt(a * b); break;
+        case MCO_TVD: s += fabs(a - b); break;
+        case MCO_COSINE_DISTANCE: case MCO_COSINE_SIMILARITY: {
+            const double u = c + sm.pv, v = x + sm.pv;
+            s += u * v; s2 += u * u; s3 += v * v;
+        } break;
+        case MCO_PROBABILITY_COSINE_DISTANCE: case MCO_PROBABILITY_COSINE_SIMILARITY:
+            s += a * b; s2 += a * a; s3 += b * b; break;
+        case MCO_JSM: s += coord_term(MCO_JSD, a, b); break;
+        case MCO_LLR: case MCO_UWLLR: case MCO_SRLRT: case MCO_SRULRT: s += llr_term(a, b, lam); break;
+        case MCO_IS: s += a / b - log(a / b); break;
+        case MCO_REVERSE_IS: s += b / a - log(b / a); break;
+        case MCO_SIS: s += .5 * log((a / b + b / a + 2.) * .25); break;
+        case MCO_RSIS: {
+            const double a2 = x * sm.rhrsi + li, b2 = c * sm.lhrsi + ri;
+            s += .5 * log((a2 / b2 + b2 / a2 + 2.) * .25);
+        } break;
+        default: return NAN;
+        }
+    }
+    const double z = (double)sharednz;
+    switch (msr) {
+    case MCO_L1: return s;
+    case MCO_L2: return sqrt(s);
+    case MCO_HELLINGER: { const double t = sqrt(li) - sqrt(ri); return sqrt(s + t * t * z) * 0.70710678118654752440; }
+    case MCO_BHATTACHARYYA_METRIC: case MCO_BHATTACHARYYA_DISTANCE: {
+        s += sqrt(li * ri) * z;
+        if (1. - s < 1e-8) s = 1.;
+        return msr == MCO_BHATTACHARYYA_METRIC ? sqrt(1. - s > 0. ? 1. - s : 0.) : -log(s);
+    }
+    case MCO_TVD: return .5 * (s + fabs(li - ri) * z);
+    case MCO_COSINE_DISTANCE: case MCO_COSINE_SIMILARITY:
+    case MCO_PROBABILITY_COSINE_DISTANCE: case MCO_PROBABILITY_COSINE_SIMILARITY: {
+        double r = s / sqrt(s2 * s3);
+        r = r > 1. ? 1. : (r < 0. ? 0. : r);
+        if (msr == MCO_COSINE_DISTANCE || msr == MCO_PROBABILITY_COSINE_DISTANCE) r = acos(r) * 0.31830988618379067153;
+        return r;
+    }
+    case MCO_JSM: {
+        const double lhl = li != 0. ? log(li) : 0., rhl = ri != 0. ? log(ri) : 0.;
+        const double shl = log((li + ri) * .5), shincl = (li + ri) > 0. ? (li + ri) * shl : 0.;
+        return kl_tail(msr, s + ((lhl * li + rhl * ri - shincl) * z) * .5, &sm);
+    }
+    case MCO_LLR: case MCO_UWLLR: case MCO_SRLRT: case MCO_SRULRT: {
+        const double em = lam * li + (1. - lam) * ri;
+        const double ec = (li != 0. ? lam * li * log(li / em) : 0.) + (ri != 0. ? (1. - lam) * ri * log(ri / em) : 0.);
+        return kl_tail(msr, s + ec * z, &sm);
+    }
+    case MCO_IS: return kl_tail(msr, s + z * (li / ri - log(li / ri)) - (double)d, &sm);
+    case MCO_REVERSE_IS: return kl_tail(msr, s + z * (ri / li - log(ri / li)) - (double)d, &sm);
+    default: return kl_tail(msr, s + z * .5 * log((li / ri + ri / li + 2.) * .25), &sm);
+    }
+}
+
 /* applicator.h:1762-1897 sparse-sparse branch. */
 double mco_msr_sparse(int msr, const double *xv, const uint32_t *xi, uint64_t xn,
                       const double *cv, const uint32_t *ci, uint64_t cn, uint64_t d,
                       double prior, double xsum, double csum, uint64_t *sharednz_out) {
     uint64_t sharednz = mco_shared_zeros(xi, xn, ci, cn, d);
     if (sharednz_out) *sharednz_out = sharednz;
+    if (msr != MCO_SQRL2 && msr != MCO_MKL && msr != MCO_REVERSE_MKL && msr != MCO_JSD)
+        return msr_more_sparse(msr, xv, xi, xn, cv, ci, cn, d, prior, xsum, csum, sharednz);
     uint64_t i = 0, j = 0;
     if (msr == MCO_SQRL2) {                        /* :1797-1806 */
         double s = 0.;
@@ -201,6 +350,14 @@ float mco_sqrl2_f32(const float *x, const float *c, uint64_t d) {
 /* Cost of CSR row i against a dense center, walking all d coordinates. */
 static double csr_row_cost(int msr, const float *val, const uint32_t *idx, uint64_t b, uint64_t e,
                            uint64_t d, const float *c, double prior, double xsum, double csum) {
+    if (msr != MCO_SQRL2 && msr != MCO_MKL && msr != MCO_REVERSE_MKL && msr != MCO_JSD) {
+        double *xb = (double *)calloc(d, sizeof(double)), *cb = (double *)malloc(d * sizeof(double));
+        for (uint64_t p = b; p < e; ++p) xb[idx[p]] = val[p];
+        for (uint64_t j = 0; j < d; ++j) cb[j] = c[j];
+        const double v = msr_more_dense(msr, xb, cb, d, prior, xsum, csum);
+        free(xb); free(cb);
+        return v;
+    }
     if (msr == MCO_SQRL2) {
         double s = 0.; uint64_t p = b;
         for (uint64_t j = 0; j < d; ++j) {

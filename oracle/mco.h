@@ -24,6 +24,14 @@ extern "C" {
 
 /* Measure codes: include/minicore/dist/distance.h:24-64 */
 enum { MCO_SQRL2 = 2, MCO_JSD = 4, MCO_MKL = 5, MCO_REVERSE_MKL = 11 };
+/* further measures (distance.h:24-64): applicator.h:1673-1749 (dense), 1778-1891 (sparse) */
+enum {
+    MCO_L1 = 0, MCO_L2 = 1, MCO_JSM = 3, MCO_HELLINGER = 6, MCO_BHATTACHARYYA_METRIC = 7,
+    MCO_BHATTACHARYYA_DISTANCE = 8, MCO_TVD = 9, MCO_LLR = 10, MCO_UWLLR = 12, MCO_IS = 13,
+    MCO_REVERSE_IS = 14, MCO_COSINE_DISTANCE = 15, MCO_PROBABILITY_COSINE_DISTANCE = 16,
+    MCO_COSINE_SIMILARITY = 17, MCO_PROBABILITY_COSINE_SIMILARITY = 18, MCO_SIS = 23, MCO_RSIS = 24,
+    MCO_SRLRT = 25, MCO_SRULRT = 26
+};
 
 /* ---- random numbers: published wyrand, counter form ------------------- */
 uint64_t mco_wyrand_at(uint64_t seed, uint64_t j);      /* j-th output (0-based) */

@@ -15,6 +15,12 @@ _LIB = os.path.join(_HERE, "_build", "libmco.so")
 _lib = None
 
 SQRL2, JSD, MKL, REVERSE_MKL = 2, 4, 5, 11
+(L1, L2, JSM, HELLINGER, BHATTACHARYYA_METRIC, BHATTACHARYYA_DISTANCE, TVD, LLR, UWLLR, IS, REVERSE_IS,
+ COSINE_DISTANCE, PROBABILITY_COSINE_DISTANCE, COSINE_SIMILARITY, PROBABILITY_COSINE_SIMILARITY, SIS, RSIS, SRLRT,
+ SRULRT) = 0, 1, 3, 6, 7, 8, 9, 10, 12, 13, 14, 15, 16, 17, 18, 23, 24, 25, 26
+MORE_MEASURES = (L1, L2, JSM, HELLINGER, BHATTACHARYYA_METRIC, BHATTACHARYYA_DISTANCE, TVD, LLR, UWLLR, IS,
+                 REVERSE_IS, COSINE_DISTANCE, PROBABILITY_COSINE_DISTANCE, COSINE_SIMILARITY,
+                 PROBABILITY_COSINE_SIMILARITY, SIS, RSIS, SRLRT, SRULRT)
 
 
 def build():

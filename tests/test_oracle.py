@@ -91,6 +91,88 @@ def test_sparse_equals_dense_formula(mco):
             assert b == pytest.approx(a, rel=1e-9, abs=1e-12)
 
 
+# ---- further measures (applicator.h:1673-1760, 1778-1891) ---------------------------
+def _smoothed_pair(mco, x, c, prior):
+    """(smoothed center a, smoothed row b, lhsum, rhsum) as applicator.h:1642-1656."""
+    d = len(x)
+    lhsum, rhsum = c.sum() + prior * d, x.sum() + prior * d
+    pv = max(prior, 7.0069e-42 * (lhsum + rhsum))
+    lhsum, rhsum = c.sum() + pv * d, x.sum() + pv * d
+    return (c + pv) / lhsum, (x + pv) / rhsum, lhsum, rhsum
+
+
+def test_more_measures_sparse_equals_dense(mco):
+    """The sparse branch (union terms plus the reference's closed-form shared-zero
+    terms) equals the dense formula over all coordinates.  Not for REVERSE_SIS, whose
+    sparse branch swaps the vectors but not the increments, or the cosine forms at a
+    real prior, whose sparse branch ignores the shared zeros."""
+    rng = np.random.default_rng(17)
+    for _ in range(25):
+        d = int(rng.integers(5, 200))
+        x = rng.poisson(0.4, d).astype(float)
+        c = rng.poisson(0.6, d).astype(float)
+        x[rng.integers(d)] += 1
+        c[rng.integers(d)] += 1
+        xi, ci = np.nonzero(x)[0], np.nonzero(c)[0]
+        for code in mco.MORE_MEASURES:
+            if code == mco.RSIS:
+                continue
+            p = 0. if code in (mco.COSINE_DISTANCE, mco.COSINE_SIMILARITY, mco.PROBABILITY_COSINE_DISTANCE,
+                               mco.PROBABILITY_COSINE_SIMILARITY) else float(rng.choice([1e-3, 0.1, 1.]))
+            a = mco.msr_dense(code, x, c, p)
+            b, _ = mco.msr_sparse(code, x[xi], xi, c[ci], ci, d, p)
+            assert b == pytest.approx(a, rel=1e-9, abs=1e-10), code
+
+
+def test_more_measures_self_and_scipy(mco):
+    from scipy.spatial.distance import cityblock, cosine, euclidean
+    rng = np.random.default_rng(3)
+    x = rng.gamma(1., 1., 40)
+    c = rng.gamma(1., 1., 40)
+    for code in mco.MORE_MEASURES:
+        v = mco.msr_dense(code, x, x, 0.1)
+        if code in (mco.COSINE_SIMILARITY, mco.PROBABILITY_COSINE_SIMILARITY):
+            assert v == pytest.approx(1., abs=1e-12)
+        else:
+            assert abs(v) < 2e-7, code      # acos near 1 loses half the digits
+    p = 0.05
+    a, b, lhs, rhs = _smoothed_pair(mco, x, c, p)
+    assert mco.msr_dense(mco.L1, x, c, p) == pytest.approx(cityblock(x, c), rel=1e-12)
+    assert mco.msr_dense(mco.L2, x, c, p) == pytest.approx(euclidean(x, c), rel=1e-12)
+    assert mco.msr_dense(mco.TVD, x, c, p) == pytest.approx(0.5 * cityblock(a, b), rel=1e-12)
+    assert mco.msr_dense(mco.JSM, x, c, p) == pytest.approx(jensenshannon(a, b), rel=1e-9)
+    assert mco.msr_dense(mco.PROBABILITY_COSINE_SIMILARITY, x, c, p) == pytest.approx(1 - cosine(a, b), rel=1e-12)
+    pv = max(p, 7.0069e-42 * (lhs + rhs))
+    cs = 1 - cosine(c + pv, x + pv)
+    assert mco.msr_dense(mco.COSINE_SIMILARITY, x, c, p) == pytest.approx(cs, rel=1e-12)
+    assert mco.msr_dense(mco.COSINE_DISTANCE, x, c, p) == pytest.approx(np.arccos(cs) / np.pi, rel=1e-12)
+
+
+def test_more_measures_identities(mco):
+    """Itakura-Saito is the Bregman divergence of -log t (center first); SIS is the
+    mean IS to the midpoint; UWLLR with equal sums is JSD; LLR scales UWLLR by the
+    total mass; Hellinger equals the Bhattacharyya metric on distributions."""
+    rng = np.random.default_rng(5)
+    x = rng.gamma(2., 1., 30)
+    c = rng.gamma(2., 1., 30)
+    p = 0.2
+    a, b, lhs, rhs = _smoothed_pair(mco, x, c, p)
+    breg = lambda u, v: float(np.sum(-np.log(u) + np.log(v) + (u - v) / v))   # phi(u) - phi(v) - phi'(v)(u - v)
+    assert mco.msr_dense(mco.IS, x, c, p) == pytest.approx(breg(a, b), rel=1e-10)
+    assert mco.msr_dense(mco.REVERSE_IS, x, c, p) == pytest.approx(breg(b, a), rel=1e-10)
+    m = 0.5 * (a + b)
+    assert mco.msr_dense(mco.SIS, x, c, p) == pytest.approx(0.5 * (breg(a, m) + breg(b, m)), rel=1e-10)
+    u = mco.msr_dense(mco.UWLLR, x, c, p)
+    assert mco.msr_dense(mco.LLR, x, c, p) == pytest.approx(u * (lhs + rhs), rel=1e-12)
+    assert mco.msr_dense(mco.SRULRT, x, c, p) == pytest.approx(np.sqrt(u), rel=1e-12)
+    xe = x * (c.sum() / x.sum())                        # equal sums: lambda = 1/2
+    assert mco.msr_dense(mco.UWLLR, xe, c, p) == pytest.approx(mco.msr_dense(mco.JSD, xe, c, p), rel=1e-10)
+    h = mco.msr_dense(mco.HELLINGER, x, c, p)
+    assert h == pytest.approx(mco.msr_dense(mco.BHATTACHARYYA_METRIC, x, c, p), rel=1e-9)
+    bc = float(np.sum(np.sqrt(a * b)))
+    assert mco.msr_dense(mco.BHATTACHARYYA_DISTANCE, x, c, p) == pytest.approx(-np.log(bc), rel=1e-10)
+
+
 def test_sqrl2_f32_is_sequential_fma(mco):
     rng = np.random.default_rng(1)
     x = rng.standard_normal(77).astype(np.float32)

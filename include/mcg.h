@@ -225,6 +225,19 @@ int mcg_soft_em(mcg_data *ds, int msr, double prior, double temp, float *centers
  * centers u32[k] (row indices), asn u32[n], costs f32[n].  weights nullable. */
 int mcg_kmeanspp(mcg_data *ds, int msr, double prior, uint64_t k, uint64_t seed,
                  const double *weights, uint32_t *centers, uint32_t *asn, float *costs);
+
+/* D^2 seeding followed by `rounds` rounds of local search++ (localsearchpp_rounds,
+ * include/minicore/optim/lsearchpp.h:21-121, reached from kmeans.h:209-212 and the
+ * kmeanspp bindings' lspp argument).  Each round draws a row with the seeding
+ * stream's next output (over the distances; with weights over w_i times the last
+ * round's new costs, as lsearchpp.h:53-56 does), and swaps it in for the center whose
+ * removal charge (rows' second-best minus best) is below the credit of the rows it
+ * would serve better; charges and credit are exact fixed-point sums.  The n x k cost
+ * matrix stays on the device.  Costs are row-to-center-row in the seeding's
+ * orientation (the reference's matrix uses the opposite argument order, which
+ * matters only for asymmetric measures). */
+int mcg_kmeanspp_ls(mcg_data *ds, int msr, double prior, uint64_t k, uint64_t seed, const double *weights,
+                    uint64_t rounds, uint32_t *centers, uint32_t *asn, float *costs);
 /* Sensitivities: CoresetSampler::make_sampler, coreset.h:386-578.
  * costs f64[n], asn u32[n], weights f64[n] or NULL; probs f32[n]. */
 int mcg_coreset_probs(mcg_ctx *ctx, int sens, uint64_t n, uint64_t ncenters,

@@ -525,6 +525,170 @@ void kmeanspp_run(Data &ds, int msr, double prior, uint64_t k, uint64_t seed, co
     HIP_CHECK(hipStreamSynchronize(st));
 }
 
+// ---- local search++ (lsearchpp.h:21-121) ---------------------------------------
+// M[i][y] = cost of row i to center row ctrs[y] (the seeding's orientation), n x k
+// floats; dense SQRL2 by the sequential FMA chain, other cases by the cost kernels.
+
+// ||x_i - x_r||^2 by the sequential FMA chain, r = *row_p (one thread per row)
+__global__ void k_cost_to_row_l2(const float *__restrict__ X, uint64_t n, uint32_t d,
+                                 const uint64_t *__restrict__ row_p, float *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *x = X + i * d, *c = X + *row_p * d;
+    float s = 0.f;
+    for (uint32_t j = 0; j < d; ++j) { const float t = x[j] - c[j]; s = fmaf(t, t, s); }
+    out[i] = s;
+}
+
+// column y of M
+__global__ void k_set_column(float *__restrict__ M, uint64_t n, uint32_t k, uint32_t y, const float *__restrict__ v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        M[i * k + y] = v[i];
+}
+
+// row minima (first index on ties)
+__global__ void k_row_min(const float *__restrict__ M, uint64_t n, uint32_t k, float *__restrict__ dist,
+                          uint32_t *__restrict__ asn) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const float *r = M + i * k;
+        float m = r[0];
+        uint32_t b = 0;
+        for (uint32_t y = 1; y < k; ++y) if (r[y] < m) { m = r[y]; b = y; }
+        dist[i] = m;
+        if (asn) asn[i] = b;
+    }
+}
+
+// per row: the charge (second best minus best, to the best center) of a row the
+// drawn row would serve worse, else the credit (old minus new); their maximum
+__global__ void k_ls_terms(const float *__restrict__ M, uint64_t n, uint32_t k, const float *__restrict__ nc,
+                           const float *__restrict__ dist, float *__restrict__ t, float *__restrict__ g,
+                           uint32_t *__restrict__ top, unsigned long long *__restrict__ mx) {
+    double a = 0.;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        float ti = 0.f, gi = 0.f;
+        uint32_t j1 = 0;
+        if (nc[i] > dist[i]) {
+            const float *r = M + i * k;
+            float v1 = r[0], v2 = r[1];
+            uint32_t j2 = 1;
+            if (v2 < v1) { const float tv = v1; v1 = v2; v2 = tv; j1 = 1; j2 = 0; }
+            for (uint32_t y = 2; y < k; ++y) {
+                const float v = r[y];
+                if (v < v2) {
+                    if (v < v1) { v2 = v1; j2 = j1; v1 = v; j1 = y; }
+                    else { v2 = v; j2 = y; }
+                }
+            }
+            (void)j2;
+            if (v1 != v2) ti = v2 - v1;
+        } else {
+            gi = dist[i] - nc[i];
+        }
+        t[i] = ti;
+        g[i] = gi;
+        top[i] = j1;
+        a = fmax(a, fmax((double)ti, (double)gi));
+    }
+    block_max_into(a, mx);
+}
+
+__global__ void k_ls_sums(const float *__restrict__ t, const float *__restrict__ g, const uint32_t *__restrict__ top,
+                          uint64_t n, const unsigned long long *__restrict__ mx, unsigned long long *__restrict__ cf,
+                          unsigned long long *__restrict__ gain) {
+    const int s = fixed_shift(__longlong_as_double((long long)*mx), n);
+    unsigned long long ga = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (t[i] > 0.f) atomicAdd(&cf[top[i]], (unsigned long long)to_fixed((double)t[i], s));
+        ga += to_fixed((double)g[i], s);
+    }
+    for (int o = 32; o > 0; o >>= 1) ga += __shfl_xor(ga, o);
+    if ((threadIdx.x & 63) == 0 && ga) atomicAdd(gain, ga);
+}
+
+void localsearch_run(Data &ds, int msr, double prior, uint64_t k, uint64_t seed, const double *w, uint64_t rounds,
+                     uint32_t *centers_h, uint32_t *asn_d, float *dist_d, hipStream_t st) {
+    if (rounds == 0 || k < 2) return;
+    Ctx &c = *ds.ctx;
+    const uint64_t n = ds.n;
+    const uint32_t d = (uint32_t)ds.d;
+    const bool l2 = !ds.csr && msr == MCG_SQRL2;
+    DBuf bM, bNc, bT, bG, bTop, bC, bRow, bTa, bMisc, bIds;
+    float *M = (float *)bM.reserve(n * k * 4), *nc = (float *)bNc.reserve(n * 4);
+    float *t = (float *)bT.reserve(n * 4), *g = (float *)bG.reserve(n * 4);
+    uint32_t *top = (uint32_t *)bTop.reserve(n * 4), *ta = (uint32_t *)bTa.reserve(n * 4);
+    float *C = (float *)bC.reserve(k * d * 4), *row = (float *)bRow.reserve(d * 4);
+    const uint32_t nblk = (uint32_t)cdiv(n, FX_BLK);
+    char *misc = (char *)bMisc.reserve(64 + nblk * 8 + k * 8);
+    uint64_t *sel_d = (uint64_t *)misc;
+    unsigned long long *mx = (unsigned long long *)(misc + 8), *gain = (unsigned long long *)(misc + 16);
+    uint64_t *bsum = (uint64_t *)(misc + 64);
+    unsigned long long *cf = (unsigned long long *)(misc + 64 + nblk * 8);
+    const uint32_t pgrid = (uint32_t)std::min<uint64_t>(cdiv(n, 256), 2048);
+    fill_u32((uint32_t *)nc, 0u, n, st);
+    // M from the seeded centers
+    {
+        std::vector<uint64_t> rows(centers_h, centers_h + k);
+        std::vector<uint32_t> ids(k);
+        for (uint64_t y = 0; y < k; ++y) ids[y] = (uint32_t)y;
+        copy_rows_to_centers(ds, rows.data(), ids.data(), k, C, st);
+    }
+    for (uint64_t y = 0; y < k; ++y) {
+        if (l2) {
+            hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, sel_d, (uint64_t)centers_h[y]);
+            hipLaunchKernelGGL(k_cost_to_row_l2, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, ds.X, n, d, sel_d, t);
+        } else {
+            generic_costs(ds, msr, prior, C + y * d, 1, nullptr, ta, t, st);
+        }
+        hipLaunchKernelGGL(k_set_column, dim3(pgrid), dim3(256), 0, st, M, n, (uint32_t)k, (uint32_t)y, t);
+    }
+    hipLaunchKernelGGL(k_row_min, dim3(pgrid), dim3(256), 0, st, M, n, (uint32_t)k, dist_d, (uint32_t *)nullptr);
+    uint64_t j = k;   // the seeding used the stream's outputs 0 .. k-1
+    std::vector<unsigned long long> hcf(k + 1);
+    for (uint64_t rd = 0; rd < rounds; ++rd) {
+        // the draw: over the distances, or over w times the last round's new costs
+        // (lsearchpp.h:53-56)
+        const float *v = w ? nc : dist_d;
+        fill_u32((uint32_t *)mx, 0u, 2, st);
+        hipLaunchKernelGGL(k_absmax<float>, dim3((uint32_t)std::min<uint64_t>(cdiv(n, 256), 1024)), dim3(256), 0, st,
+                           v, w, n, mx);
+        hipLaunchKernelGGL(k_fixed_blocksum<float>, dim3(nblk), dim3(256), 0, st, v, w, n,
+                           (const unsigned long long *)mx, bsum);
+        hipLaunchKernelGGL(k_fixed_select<float>, dim3(1), dim3(1024), 0, st, v, w, n, (const unsigned long long *)mx,
+                           (const uint64_t *)bsum, nblk, wyrand_at(seed, j++), sel_d);
+        if (l2) {
+            hipLaunchKernelGGL(k_cost_to_row_l2, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, ds.X, n, d, sel_d,
+                               nc);
+        } else {
+            hipLaunchKernelGGL(k_row_to_dense, dim3(1), dim3(256), 0, st, ds.X,
+                               ds.csr ? ds.val.as<const float>() : nullptr,
+                               ds.csr ? ds.idx.as<const uint32_t>() : nullptr,
+                               ds.csr ? ds.ptr.as<const uint64_t>() : nullptr, d, (const uint64_t *)sel_d, row);
+            generic_costs(ds, msr, prior, row, 1, nullptr, ta, nc, st);
+        }
+        fill_u32((uint32_t *)mx, 0u, 2 + 2, st);   // mx and gain
+        fill_u32((uint32_t *)cf, 0u, 2 * k, st);
+        hipLaunchKernelGGL(k_ls_terms, dim3(pgrid), dim3(256), 0, st, M, n, (uint32_t)k, nc, dist_d, t, g, top, mx);
+        hipLaunchKernelGGL(k_ls_sums, dim3(pgrid), dim3(256), 0, st, t, g, top, n, (const unsigned long long *)mx, cf,
+                           gain);
+        uint64_t sel = 0;
+        HIP_CHECK(hipMemcpyAsync(hcf.data(), cf, k * 8, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(&hcf[k], gain, 8, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(&sel, sel_d, 8, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        uint64_t am = 0;
+        for (uint64_t y = 1; y < k; ++y) if (hcf[y] < hcf[am]) am = y;
+        if (hcf[am] < hcf[k]) {   // charge below credit: swap (lsearchpp.h:104-114)
+            centers_h[am] = (uint32_t)sel;
+            hipLaunchKernelGGL(k_set_column, dim3(pgrid), dim3(256), 0, st, M, n, (uint32_t)k, (uint32_t)am, nc);
+            hipLaunchKernelGGL(k_row_min, dim3(pgrid), dim3(256), 0, st, M, n, (uint32_t)k, dist_d,
+                               (uint32_t *)nullptr);
+        }
+    }
+    hipLaunchKernelGGL(k_row_min, dim3(pgrid), dim3(256), 0, st, M, n, (uint32_t)k, dist_d, asn_d);
+    HIP_CHECK(hipStreamSynchronize(st));
+}
+
 // ---- coreset sensitivities ----------------------------------------------------
 struct CsSums {
     unsigned long long maxw, maxc;           // |w|, |w*cost| maxima (bit patterns)

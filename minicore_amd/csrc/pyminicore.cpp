@@ -505,19 +505,24 @@ struct PPResult {
     double total = 0.;
 };
 
-PPResult run_pp(mcg_data *h, uint64_t n, int m, uint64_t k, double prior, uint64_t seed, const double *w) {
+// D^2 seeding, then lspp rounds of local search++ (kmeans.h:209-212)
+PPResult run_pp(mcg_data *h, uint64_t n, int m, uint64_t k, double prior, uint64_t seed, const double *w,
+                uint64_t lspp) {
     PPResult r;
     const uint64_t kk = std::min(k, n);
     r.centers.resize(kk);
     r.asn.resize(n);
     r.costs.resize(n);
-    check(mcg_kmeanspp(h, m, prior, k, seed, w, r.centers.data(), r.asn.data(), r.costs.data()));
+    if (lspp)
+        check(mcg_kmeanspp_ls(h, m, prior, k, seed, w, lspp, r.centers.data(), r.asn.data(), r.costs.data()));
+    else
+        check(mcg_kmeanspp(h, m, prior, k, seed, w, r.centers.data(), r.asn.data(), r.costs.data()));
     for (float c : r.costs) r.total += c;
     return r;
 }
 
 void check_pp_args(py::ssize_t lspp, bool expskips, py::ssize_t n_local_trials) {
-    if (lspp > 0) throw py::value_error("local search++ (lspp > 0) is not on the GPU path");
+    if (lspp < 0) throw py::value_error("lspp must be >= 0");
     if (n_local_trials > 1) throw py::value_error("n_local_trials > 1 is not on the GPU path");
     (void)expskips;  // sampling method choice; the GPU path has one exact sampler
 }
@@ -533,10 +538,11 @@ py::object kmeanspp_dense(py::array arr, int k, py::object msr, py::object prior
     if (pr < 0.) pr = 1. / (double)D.d;
     uint64_t s = pp_seed(seed.cast<uint64_t>());
     auto w = weights_f64(weights, D.n);
-    PPResult best = run_pp(D.dev->h, D.n, m, (uint64_t)k, pr, s, wptr(w));
+    PPResult best = run_pp(D.dev->h, D.n, m, (uint64_t)k, pr, s, wptr(w), (uint64_t)lspp);
     const unsigned nt = ntimes.cast<unsigned>();
     for (unsigned t = 0; t < nt; ++t) {  // extra runs keep the cheapest (smw.h:385-393)
-        PPResult r = run_pp(D.dev->h, D.n, m, (uint64_t)k, pr, s + 0x9e3779b97f4a7c15ULL * (t + 1), wptr(w));
+        PPResult r = run_pp(D.dev->h, D.n, m, (uint64_t)k, pr, s + 0x9e3779b97f4a7c15ULL * (t + 1), wptr(w),
+                            (uint64_t)lspp);
         if (r.total < best.total) best = std::move(r);
     }
     py::array_t<uint32_t> ctr(best.centers.size());
@@ -555,7 +561,7 @@ py::object kmeanspp_csr(PyCSparseMatrix &smw, py::object msr, py::int_ k, double
     if (prior < 0.) prior = 1. / (double)smw.nc;
     auto w = weights_f64(weights, smw.nr);
     const uint64_t kk = k.cast<uint64_t>();
-    PPResult r = run_pp(smw.device(), smw.nr, m, kk, prior, pp_seed(seed), wptr(w));
+    PPResult r = run_pp(smw.device(), smw.nr, m, kk, prior, pp_seed(seed), wptr(w), (uint64_t)lspp);
     py::array_t<uint32_t> ctr(r.centers.size());
     std::memcpy(ctr.mutable_data(), r.centers.data(), r.centers.size() * 4);
     py::array_t<double> costs(smw.nr);

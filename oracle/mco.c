@@ -1029,6 +1029,110 @@ uint64_t mco_kmeanspp(int msr, const float *X, const float *val, const uint32_t 
 }
 
 /* ======================================================================
+ * Local search++ after D^2 seeding: localsearchpp_rounds, lsearchpp.h:21-121
+ * (kmeans.h:209-212).  M[i][y] = cost of row i to center row ctrs[y] (the
+ * seeding's orientation; the reference's matrix uses the opposite argument
+ * order, which differs only for asymmetric measures).  Each round draws a row
+ * (fixed-point, over the distances, or over w_i times the previous round's
+ * new costs when weighted, lsearchpp.h:53-56) with the seeding stream's next
+ * output, computes every row's cost to it, charges each row that would be
+ * worse off to its best center (second best minus best, FP32 differences)
+ * and credits the others (old minus new), and swaps the cheapest center out
+ * when its charge is below the credit.  Charges and credit are exact
+ * fixed-point sums (the reference adds floats under OpenMP atomics).  At the
+ * end each row takes its first cheapest center.
+ * ====================================================================== */
+static double row_pair_cost(int msr, const float *X, const float *val, const uint32_t *idx, const uint64_t *indptr,
+                            uint64_t d, uint64_t i, uint64_t c, const double *rsum, double prior) {
+    if (msr == MCO_SQRL2 && !val) return (double)mco_sqrl2_f32(X + i * d, X + c * d, d);
+    return pair_rows(msr, X, val, idx, indptr, d, i, c, rsum, prior);
+}
+
+uint64_t mco_kmeanspp_ls(int msr, const float *X, const float *val, const uint32_t *idx,
+                         const uint64_t *indptr, uint64_t n, uint64_t d, uint64_t k,
+                         double prior, uint64_t seed, const double *w, uint64_t rounds,
+                         uint32_t *centers, uint32_t *asn, float *distf, int nthreads) {
+    k = mco_kmeanspp(msr, X, val, idx, indptr, n, d, k, prior, seed, w, centers, asn, distf, nthreads);
+    if (rounds == 0 || k < 2) return k;
+    double *rsum = (double *)malloc(n * sizeof(double));
+    for (uint64_t i = 0; i < n; ++i)
+        rsum[i] = val ? row_sum_csr(val, indptr[i], indptr[i + 1]) : row_sum_dense(X + i * d, d);
+    float *M = (float *)malloc(n * k * sizeof(float)), *dist = (float *)malloc(n * sizeof(float));
+    float *nc = (float *)calloc(n, sizeof(float)), *t = (float *)malloc(n * sizeof(float));
+    float *g = (float *)malloc(n * sizeof(float));
+    uint32_t *top = (uint32_t *)malloc(n * sizeof(uint32_t));
+    double *wv = (double *)malloc(n * sizeof(double));
+    uint64_t *fw = (uint64_t *)malloc(n * sizeof(uint64_t)), *cf = (uint64_t *)malloc(k * sizeof(uint64_t));
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i)
+        for (uint64_t y = 0; y < k; ++y)
+            M[i * k + y] = (float)row_pair_cost(msr, X, val, idx, indptr, d, i, centers[y], rsum, prior);
+    for (uint64_t i = 0; i < n; ++i) {
+        float m = M[i * k];
+        for (uint64_t y = 1; y < k; ++y) if (M[i * k + y] < m) m = M[i * k + y];
+        dist[i] = m;
+    }
+    uint64_t j = k;   /* the seeding used outputs 0..k-1 */
+    for (uint64_t rd = 0; rd < rounds; ++rd) {
+        for (uint64_t i = 0; i < n; ++i) wv[i] = w ? w[i] * (double)nc[i] : (double)dist[i];
+        mco_fixed_weights(wv, n, fw);
+        const uint64_t sel = mco_fixed_draw(fw, n, mco_wyrand_at(seed, j++));
+        #pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < (int64_t)n; ++i)
+            nc[i] = (float)row_pair_cost(msr, X, val, idx, indptr, d, i, sel, rsum, prior);
+        double mx = 0.;
+        for (uint64_t i = 0; i < n; ++i) {
+            t[i] = 0.f; g[i] = 0.f; top[i] = 0;
+            if (nc[i] > dist[i]) {
+                const float *r = M + i * k;
+                float v1 = r[0], v2 = r[1];
+                uint32_t j1 = 0, j2 = 1;
+                if (v2 < v1) { float tv = v1; v1 = v2; v2 = tv; uint32_t tj = j1; j1 = j2; j2 = tj; }
+                for (uint64_t y = 2; y < k; ++y) {
+                    const float v = r[y];
+                    if (v < v2) {
+                        if (v < v1) { v2 = v1; j2 = j1; v1 = v; j1 = (uint32_t)y; }
+                        else { v2 = v; j2 = (uint32_t)y; }
+                    }
+                }
+                (void)j2;
+                if (v1 != v2) { t[i] = v2 - v1; top[i] = j1; }
+            } else {
+                g[i] = dist[i] - nc[i];
+            }
+            if ((double)t[i] > mx) mx = t[i];
+            if ((double)g[i] > mx) mx = g[i];
+        }
+        const int s = mco_fixed_shift(mx, n);
+        memset(cf, 0, k * sizeof(uint64_t));
+        uint64_t gain = 0;
+        for (uint64_t i = 0; i < n; ++i) {
+            if (t[i] > 0.f) cf[top[i]] += (uint64_t)floor(ldexp((double)t[i], s));
+            if (g[i] > 0.f) gain += (uint64_t)floor(ldexp((double)g[i], s));
+        }
+        uint64_t am = 0;
+        for (uint64_t y = 1; y < k; ++y) if (cf[y] < cf[am]) am = y;
+        if (cf[am] < gain) {       /* delta = charge - credit < 0 */
+            centers[am] = (uint32_t)sel;
+            for (uint64_t i = 0; i < n; ++i) M[i * k + am] = nc[i];
+            for (uint64_t i = 0; i < n; ++i) {
+                float m = M[i * k];
+                for (uint64_t y = 1; y < k; ++y) if (M[i * k + y] < m) m = M[i * k + y];
+                dist[i] = m;
+            }
+        }
+    }
+    for (uint64_t i = 0; i < n; ++i) {
+        uint32_t b = 0;
+        for (uint64_t y = 1; y < k; ++y) if (M[i * k + y] < M[i * k + b]) b = (uint32_t)y;
+        asn[i] = b;
+        distf[i] = dist[i];
+    }
+    free(rsum); free(M); free(dist); free(nc); free(t); free(g); free(top); free(wv); free(fw); free(cf);
+    return k;
+}
+
+/* ======================================================================
  * Coreset sensitivities, coreset.h:423-578.  Sums over points are exact:
  * each term is mapped to fixed point with one shift for the whole vector
  * (from its max) and added as integers, so the result does not depend on

@@ -126,6 +126,12 @@ uint64_t mco_kmeanspp(int msr, const float *X, const float *val, const uint32_t 
                       double prior, uint64_t seed, const double *w,
                       uint32_t *centers, uint32_t *asn, float *dist, int nthreads);
 
+/* D^2 seeding followed by `rounds` rounds of local search++ (lsearchpp.h:21-121). */
+uint64_t mco_kmeanspp_ls(int msr, const float *X, const float *val, const uint32_t *idx,
+                         const uint64_t *indptr, uint64_t n, uint64_t d, uint64_t k,
+                         double prior, uint64_t seed, const double *w, uint64_t rounds,
+                         uint32_t *centers, uint32_t *asn, float *dist, int nthreads);
+
 /* ---- coreset sensitivities and sampling: coreset.h:340-699 ------------ */
 /* sens: 0 BFL, 1 FL, 3 VX, 4 LBK.  probs out in float (CoresetSampler<float,...>). */
 int  mco_coreset_probs(int sens, uint64_t n, uint64_t ncenters, const double *costs,

@@ -68,6 +68,7 @@ def _setup(L):
         "mco_soft": (I, [I, P, P, P, P, U64, U64, P, U64, D, D, P, D, U64, P, P, P, P, P]),
         "mco_softmax_rows": (None, [P, U64, U64, D, P]),
         "mco_kmeanspp": (U64, [I, P, P, P, P, U64, U64, U64, D, U64, P, P, P, P, I]),
+        "mco_kmeanspp_ls": (U64, [I, P, P, P, P, U64, U64, U64, D, U64, P, U64, P, P, P, I]),
         "mco_coreset_probs": (I, [I, U64, U64, P, P, P, P]),
         "mco_coreset_sample": (None, [U64, P, P, U64, U64, U64, P, P]),
     }
@@ -295,6 +296,17 @@ def kmeanspp(msr, X, k, prior=0., seed=0, w=None, nthreads=0):
     dist = np.zeros(n, dtype=np.float32)
     kk = lib().mco_kmeanspp(msr, _p(Xd), _p(val), _p(idx), _p(ptr), n, d, k, prior, seed, _p(w),
                             _p(centers), _p(asn), _p(dist), nthreads)
+    return centers[:kk], asn, dist
+
+
+def kmeanspp_ls(msr, X, k, rounds, prior=0., seed=0, w=None, nthreads=0):
+    Xd, val, idx, ptr, n, d = _mat(X)
+    w = _f64(w)
+    centers = np.zeros(k, dtype=np.uint32)
+    asn = np.zeros(n, dtype=np.uint32)
+    dist = np.zeros(n, dtype=np.float32)
+    kk = lib().mco_kmeanspp_ls(msr, _p(Xd), _p(val), _p(idx), _p(ptr), n, d, k, prior, seed, _p(w), rounds,
+                               _p(centers), _p(asn), _p(dist), nthreads)
     return centers[:kk], asn, dist
 
 

@@ -68,6 +68,32 @@ def test_kmeanspp_csr_consistent(mcg, ctx, mco, code):
     np.testing.assert_allclose(dist[m], ref.min(1)[m], rtol=2e-4, atol=1e-6)
 
 
+@pytest.mark.parametrize("weighted", [False, True])
+def test_kmeanspp_localsearch_bitexact(mcg, ctx, mco, weighted):
+    say(f"kmeans++ with local search++ weighted={weighted}")
+    X, _, _ = gaussian_mixture(20000, 32, 40, 2)
+    w = np.random.default_rng(2).uniform(0.5, 2, 20000) if weighted else None
+    c_ref, a_ref, d_ref = mco.kmeanspp_ls(mco.SQRL2, X, 40, 30, seed=7, w=w)
+    c, a, dd = mcg.Dataset.dense(X, ctx).kmeanspp(mcg.SQRL2, 40, seed=7, weights=w, lspp=30)
+    assert np.array_equal(c, c_ref)
+    assert np.array_equal(a, a_ref)
+    assert np.array_equal(dd, d_ref)
+
+
+def test_kmeanspp_localsearch_csr_consistent(mcg, ctx, mco):
+    say("kmeans++ with local search++ csr jsd")
+    n, d = 3000, 300
+    data, idx, ptr = poisson_csr(n, d, 0.04, 5, normalize=True)
+    X = mco.CSR(data, idx, ptr, (n, d))
+    c, a, dist = mcg.Dataset.csr(data, idx, ptr, (n, d), ctx).kmeanspp(4, 12, prior=1. / d, seed=9, lspp=10)
+    assert len(set(c.tolist())) == 12
+    ref = mco.cost_matrix(4, X, X.todense()[c], 1. / d)
+    np.testing.assert_allclose(dist, ref.min(1), rtol=2e-4, atol=2e-6)
+    Ks = np.sort(ref, 1)
+    clear = Ks[:, 1] - Ks[:, 0] > 1e-4 * Ks[:, 1]
+    assert np.array_equal(a[clear], ref.argmin(1)[clear])
+
+
 @pytest.mark.parametrize("sens", [0, 1, 3, 4])
 @pytest.mark.parametrize("weighted", [False, True])
 def test_coreset_probs_and_sample_bitexact(mcg, ctx, mco, sens, weighted):

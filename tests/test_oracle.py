@@ -433,6 +433,25 @@ def test_kmeanspp_second_center_distribution(mco):
 
 
 # ---- fixed-point sampler ------------------------------------------------------------------
+@pytest.mark.parametrize("weighted", [False, True])
+def test_localsearch_rounds_invariants(mco, weighted):
+    """After local search++ the distances and assignment are the row minima over the
+    returned centers (first index on ties), the centers are distinct rows, and on a
+    mixture the rounds lower the seeding cost."""
+    from minicore_amd.datagen import gaussian_mixture
+    X, _, _ = gaussian_mixture(2500, 8, 10, seed=6)
+    w = np.random.default_rng(1).uniform(0.5, 2., 2500) if weighted else None
+    c0, _, d0 = mco.kmeanspp(mco.SQRL2, X, 10, seed=3, w=w)
+    c, a, d = mco.kmeanspp_ls(mco.SQRL2, X, 10, 25, seed=3, w=w)
+    c_same, a_same, d_same = mco.kmeanspp_ls(mco.SQRL2, X, 10, 0, seed=3, w=w)
+    assert np.array_equal(c_same, c0) and np.array_equal(d_same, d0)
+    assert len(set(c.tolist())) == 10
+    ab, cb, _ = mco.assign_sqrl2_f32(X, X[c.astype(np.int64)])
+    assert np.array_equal(a, ab) and np.array_equal(d, cb)
+    if not weighted:   # the weighted draw reuses the last round's new costs (lsearchpp.h:53-56)
+        assert float(d.astype(np.float64).sum()) < float(d0.astype(np.float64).sum())
+
+
 def test_fixed_draw_semantics(mco):
     w = np.array([0., 3., 0., 1., 0.])
     fw = mco.fixed_weights(w)

@@ -38,6 +38,7 @@ extern "C" {
 #define MCG_EDEVICE      -3   /* HIP runtime / device error */
 #define MCG_EUNSUPPORTED -4   /* measure / dtype / shape without a GPU kernel */
 #define MCG_ENAN         -5   /* NaN center (centroid.h:771-775 raises runtime_error) */
+#define MCG_EIO          -6   /* file open / read / write failed (reference: std::runtime_error) */
 
 /* element types */
 #define MCG_F32 0
@@ -254,6 +255,29 @@ int mcg_coreset_sample(mcg_ctx *ctx, uint64_t n, const float *probs, const doubl
  * selection rows(mat, coreset.indices_) / set_center(row(smat, i), ...) of
  * hmb_coreset_clustering, include/minicore/clustering/solve.h:748-762. */
 int mcg_data_rows(const mcg_data *ds, const uint64_t *idx, uint64_t m, mcg_data **out);
+
+/* ---- on-disk formats around the path (host only, no device work) --------- */
+/* CoresetSampler::write(gzFile), include/minicore/coreset/coreset.h:282-293: a
+ * gzip stream of u64 n, u64 seed, f32 probs[n], u32 flag (137 when weights
+ * follow, 0 otherwise) and f32 weights[n].  weights may be NULL.  The sampler of
+ * the FL method is refused, as the reference refuses it (coreset.h:283). */
+int mcg_sampler_write(const char *path, uint64_t n, uint64_t seed, const float *probs, const float *weights);
+/* CoresetSampler::read(gzFile), coreset.h:306-322 (gzip or uncompressed input).
+ * Fills *n, *seed and *has_weights (any nonzero flag means weights follow, as in
+ * the reference's release build); probs and weights are written only when cap >= n
+ * (call first with cap = 0 to learn n).  A truncated file is an MCG_EIO error. */
+int mcg_sampler_read(const char *path, uint64_t *n, uint64_t *seed, int *has_weights, float *probs,
+                     float *weights, uint64_t cap);
+/* IndexCoreset<IT, FT>::write(gzFile), coreset.h:113-123: a gzip stream of u64 m,
+ * IT indices[m], FT weights[m].  idx_dtype MCG_U32 / MCG_U64 / MCG_U16 / MCG_U8,
+ * w_dtype MCG_F32 / MCG_F64: the IT and FT the file was made with. */
+int mcg_index_coreset_write(const char *path, uint64_t m, const void *indices, int idx_dtype,
+                            const void *weights, int w_dtype);
+/* IndexCoreset<IT, FT>::read(gzFile), coreset.h:94-106: *m filled; indices and
+ * weights written when cap >= m.  Weights are read as m * sizeof(FT) bytes, the
+ * layout write produces (the reference's read sizes them by IT). */
+int mcg_index_coreset_read(const char *path, uint64_t *m, void *indices, int idx_dtype, void *weights,
+                           int w_dtype, uint64_t cap);
 
 /* ---- serving ------------------------------------------------------------ */
 /* A server runs clustering requests from many callers on one GPU: worker

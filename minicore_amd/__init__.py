@@ -51,8 +51,10 @@ def CSparseMatrix(sparray):
     return _ext().CSparseMatrix(sparray)
 
 
-def CoresetSampler():
-    return _ext().CoresetSampler()
+def CoresetSampler(*args, **kwargs):
+    """CoresetSampler() or CoresetSampler(path): the latter reads a stream written by
+    CoresetSampler.write (coreset.h:260-264)."""
+    return _ext().CoresetSampler(*args, **kwargs)
 
 
 def set_sqrl2_precision(p):

@@ -23,7 +23,8 @@ EXPORTS = [
     "mcg_kmeanspp", "mcg_coreset_probs", "mcg_coreset_sample", "mcg_lloyd_step", "mcg_timing_enable",
     "mcg_timing_reset", "mcg_timing_read", "mcg_refine_counts", "mcg_data_rows", "mcg_server_create",
     "mcg_server_destroy", "mcg_submit", "mcg_poll", "mcg_wait", "mcg_cancel", "mcg_release", "mcg_tuning_enable",
-    "mcg_graphs_enable", "mcg_minibatch", "mcg_cs_minibatch", "mcg_kmeanspp_ls",
+    "mcg_graphs_enable", "mcg_minibatch", "mcg_cs_minibatch", "mcg_kmeanspp_ls", "mcg_sampler_write",
+    "mcg_sampler_read", "mcg_index_coreset_write", "mcg_index_coreset_read",
 ]
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = range(5)
@@ -111,6 +112,10 @@ def load():
         "mcg_minibatch": (I, [P, I, D, P, U64, P, U64, U64, U64, U64, I, U64, I, P, P, P, P, P]),
         "mcg_cs_minibatch": (I, [P, I, D, P, U64, P, U64, U64, U64, U64, U64, I, P, P, P, P, P]),
         "mcg_kmeanspp_ls": (I, [P, I, D, U64, U64, P, U64, P, P, P]),
+        "mcg_sampler_write": (I, [C.c_char_p, U64, U64, P, P]),
+        "mcg_sampler_read": (I, [C.c_char_p, P, P, P, P, P, U64]),
+        "mcg_index_coreset_write": (I, [C.c_char_p, U64, P, I, P, I]),
+        "mcg_index_coreset_read": (I, [C.c_char_p, P, P, I, P, I, U64]),
         "mcg_timing_reset": (I, [P]),
         "mcg_timing_read": (I, [P, I, P, P]),
         "mcg_refine_counts": (I, [P, P, P]),
@@ -402,6 +407,53 @@ def coreset_sample(probs, m, seed, weights=None, first_draw=0, ctx=None):
     wt = np.empty(max(m, 1), dtype=np.float32)
     check(load().mcg_coreset_sample(ctx.h, len(probs), ptr(probs), ptr(w), m, seed, first_draw, ptr(idx), ptr(wt)))
     return wt[:m], idx[:m]
+
+
+def _path(path):
+    return os.fsencode(os.fspath(path))
+
+
+def sampler_write(path, probs, seed, weights=None):
+    """CoresetSampler::write (coreset.h:282-293): gzip stream of n, seed, f32 probs,
+    the weights flag (137 / 0) and f32 weights."""
+    probs = np.ascontiguousarray(probs, dtype=np.float32)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+    if w is not None and len(w) != len(probs):
+        raise ValueError("weights must match probs")
+    check(load().mcg_sampler_write(_path(path), len(probs), int(seed), ptr(probs), ptr(w)))
+
+
+def sampler_read(path):
+    """CoresetSampler::read (coreset.h:306-322): returns (probs f32, seed, weights f32 or None)."""
+    L = load()
+    n, seed, hw = C.c_uint64(0), C.c_uint64(0), C.c_int(0)
+    check(L.mcg_sampler_read(_path(path), C.byref(n), C.byref(seed), C.byref(hw), None, None, 0))
+    probs = np.empty(max(n.value, 1), dtype=np.float32)
+    w = np.empty(max(n.value, 1), dtype=np.float32) if hw.value else None
+    check(L.mcg_sampler_read(_path(path), C.byref(n), C.byref(seed), C.byref(hw), ptr(probs), ptr(w), n.value))
+    return probs[:n.value], seed.value, None if w is None else w[:n.value]
+
+
+def index_coreset_write(path, indices, weights, idx_dtype=np.uint32, w_dtype=np.float32):
+    """IndexCoreset<IT, FT>::write (coreset.h:113-123): gzip stream of m, IT indices, FT weights."""
+    idx = np.ascontiguousarray(indices, dtype=idx_dtype)
+    w = np.ascontiguousarray(weights, dtype=w_dtype)
+    if len(idx) != len(w):
+        raise ValueError("indices and weights must have the same length")
+    check(load().mcg_index_coreset_write(_path(path), len(idx), ptr(idx), _dtype_code(idx.dtype), ptr(w),
+                                         _dtype_code(w.dtype)))
+
+
+def index_coreset_read(path, idx_dtype=np.uint32, w_dtype=np.float32):
+    """IndexCoreset<IT, FT>::read (coreset.h:94-106): returns (indices IT, weights FT)."""
+    L = load()
+    m = C.c_uint64(0)
+    ic, wc = _dtype_code(idx_dtype), _dtype_code(w_dtype)
+    check(L.mcg_index_coreset_read(_path(path), C.byref(m), None, ic, None, wc, 0))
+    idx = np.empty(max(m.value, 1), dtype=idx_dtype)
+    w = np.empty(max(m.value, 1), dtype=w_dtype)
+    check(L.mcg_index_coreset_read(_path(path), C.byref(m), ptr(idx), ic, ptr(w), wc, m.value))
+    return idx[:m.value], w[:m.value]
 
 
 class Server:

@@ -21,7 +21,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MCG_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["mcg_api.hip", "mcg_dense.hip", "mcg_mstep.hip", "mcg_generic.hip", "mcg_sample.hip", "mcg_serve.hip",
-           "mcg_soft.hip"]
+           "mcg_soft.hip", "mcg_format.hip"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
@@ -53,7 +53,7 @@ def build_lib(jobs=None):
             if out.strip():
                 print(out)
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lz"])
     # A kernel template whose body the host pass cannot instantiate loses its launch
     # stub without a compile error; the library would then fail to load.
     undef = [l.split()[-1] for l in _run(["nm", "-u", "-C", tmp]).splitlines() if "mcg::" in l]

@@ -15,23 +15,6 @@ namespace mcg {
 static thread_local std::string g_err;
 void set_error(const std::string &m) { g_err = m; }
 
-template <class F>
-static int guard(F &&f) {
-    try {
-        f();
-        return MCG_OK;
-    } catch (const Error &e) {
-        set_error(e.msg);
-        return e.code;
-    } catch (const std::bad_alloc &) {
-        set_error("host allocation failed");
-        return MCG_ENOMEM;
-    } catch (const std::exception &e) {
-        set_error(e.what());
-        return MCG_EDEVICE;
-    }
-}
-
 bool is_device_ptr(const void *p) {
     if (!p) return false;
     hipPointerAttribute_t a;

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstdio>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 #include "../../include/mcg.h"
@@ -26,6 +28,24 @@ struct Error {
                       std::string(#expr) + ": " + hipGetErrorString(_e));                    \
     } while (0)
 #define REQUIRE(cond, msg) do { if (!(cond)) MCG_THROW(MCG_EINVAL, (msg)); } while (0)
+
+// Runs f, turning exceptions into a status code and the thread's error message.
+template <class F>
+int guard(F &&f) {
+    try {
+        f();
+        return MCG_OK;
+    } catch (const Error &e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed");
+        return MCG_ENOMEM;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return MCG_EDEVICE;
+    }
+}
 
 // ---- device buffers -------------------------------------------------------
 struct DBuf {

@@ -573,11 +573,15 @@ py::object kmeanspp_csr(PyCSparseMatrix &smw, py::object msr, py::int_ k, double
 struct CoresetSampler {
     std::vector<float> probs;
     std::vector<double> weights;  // empty if unweighted
-    uint64_t np = 0, b = 0;
+    uint64_t np = 0, b = 0, seed = 137;
     int sens = 0;
     bool ready = false;
 
-    void make_sampler(size_t ncenters, py::array costs, py::array assignments, py::object w, uint64_t, int sens_) {
+    CoresetSampler() = default;
+    explicit CoresetSampler(const std::string &path) { read(path); }
+
+    void make_sampler(size_t ncenters, py::array costs, py::array assignments, py::object w, uint64_t seed_,
+                      int sens_) {
         if (costs.ndim() != 1) throw std::invalid_argument("costs must be 1-dimensional");
         if (costs.dtype().kind() != 'f') throw std::invalid_argument("Costs is not double or float");
         auto c = py::array_t<double, py::array::c_style | py::array::forcecast>::ensure(costs);
@@ -593,10 +597,34 @@ struct CoresetSampler {
             weights.assign(wd.data(), wd.data() + wd.size());
         }
         sens = sens_;
+        seed = seed_;
         b = ncenters;
         probs.resize(np);
         check(mcg_coreset_probs(ctx(), sens, np, ncenters, c.data(), a.data(), weights.empty() ? nullptr : weights.data(),
                                 probs.data()));
+        ready = true;
+    }
+
+    // CoresetSampler::write / read (coreset.h:267-338): the gzip stream of n, seed,
+    // probs, the weights flag and the weights, all f32 as in CoresetSampler<float, ...>
+    void write(const std::string &path) const {
+        if (!ready) throw std::invalid_argument("Can't write without created sampler. Call make_sampler");
+        if (sens == MCG_SENS_FL) throw std::runtime_error("Not implemented: serialization for FL coreset samplers");
+        std::vector<float> wf(weights.begin(), weights.end());
+        check(mcg_sampler_write(path.c_str(), np, seed, probs.data(), weights.empty() ? nullptr : wf.data()));
+    }
+    void read(const std::string &path) {
+        uint64_t n = 0, sd = 0;
+        int hw = 0;
+        check(mcg_sampler_read(path.c_str(), &n, &sd, &hw, nullptr, nullptr, 0));
+        std::vector<float> p(n), wf(hw ? n : 0);
+        check(mcg_sampler_read(path.c_str(), &n, &sd, &hw, p.data(), hw ? wf.data() : nullptr, n));
+        probs.swap(p);
+        weights.assign(wf.begin(), wf.end());
+        np = n;
+        seed = sd;
+        sens = -1;  // not stored in the stream; the sampler draws like the non-FL methods
+        b = 0;
         ready = true;
     }
 
@@ -698,6 +726,9 @@ PYBIND11_MODULE(_pyminicore, m) {
 
     py::class_<CoresetSampler>(m, "CoresetSampler")
         .def(py::init<>())
+        .def(py::init<const std::string &>(), py::arg("path"))
+        .def("write", &CoresetSampler::write, py::arg("path"))
+        .def("read", &CoresetSampler::read, py::arg("path"))
         .def("make_sampler", &CoresetSampler::make_sampler, py::arg("ncenters"), py::arg("costs"),
              py::arg("assignments"), py::arg("weights") = py::none(), py::arg("seed") = 13, py::arg("sens") = 0)
         .def("get_probs", &CoresetSampler::get_probs)

@@ -57,9 +57,9 @@ extern "C" {
 #define MCG_REVERSE_MKL 11
 /* further measures (applicator.h:1673-1760 dense, 1778-1891 sparse), evaluated pair by
  * pair.  Hard clustering uses the weighted-mean centroid for those mapped to a mean
- * policy by msr2pol (centroid.h:182-207); L1, L2 and TVD (median policies) and the
- * similarity / probability-cosine forms (no policy) are cost-only here: mcg_cost_matrix,
- * mcg_assign_hard, mcg_kmeanspp. */
+ * policy by msr2pol (centroid.h:182-207); L1, TVD and L2 cluster with their median
+ * policies (mcg_lloyd); the similarity / probability-cosine forms (no policy) are
+ * cost-only: mcg_cost_matrix, mcg_assign_hard, mcg_kmeanspp. */
 #define MCG_L1                            0
 #define MCG_L2                            1
 #define MCG_JSM                           3
@@ -165,7 +165,12 @@ int mcg_centroids_hard(mcg_data *ds, const uint32_t *asn, const double *weights,
                        float *centers, uint64_t *n_empty);
 /* Lloyd driver: perform_hard_clustering, solve.h:98-175, with the empty-cluster
  * restart of centroid.h:687-752.  centers (k x d) in/out; asn/costs outputs
- * (costs nullable).  Returns the reference's (initcost, finalcost, numiter). */
+ * (costs nullable).  Returns the reference's (initcost, finalcost, numiter).
+ * L1, TVD and L2 use their median policies (set_centroids_l1 / _tvd / _l2,
+ * centroid.h:434-647, with the restart of centroid.h:442-480): coordinate-wise
+ * medians (dense and CSR; weighted only dense, as in the reference) and the
+ * dense geometric median (Weiszfeld, blaze_adaptor.h:614-690).  Weighted median
+ * centers need positive weights; CSR medians need non-negative values. */
 int mcg_lloyd(mcg_data *ds, int msr, double prior, float *centers, uint64_t k,
               const double *weights, double eps, uint64_t maxiter, int precision,
               uint32_t *asn, float *costs, double *initcost, double *finalcost,

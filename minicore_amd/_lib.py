@@ -5,6 +5,7 @@ fallback; a missing or unloadable libmcg.so raises immediately.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -165,9 +166,14 @@ class Context:
         check(load().mcg_ctx_create(device, C.byref(h)))
         self.h = h
         self.device = device
+        self._datasets = weakref.WeakSet()
 
     def close(self):
+        """Frees the datasets still open on this context, then the context (the C ABI
+        requires datasets to go first)."""
         if self.h:
+            for ds in list(self._datasets):
+                ds.close()
             load().mcg_ctx_destroy(self.h)
             self.h = None
 
@@ -234,6 +240,7 @@ class Dataset:
         self.ctx = ctx
         self.h = h
         self._keep = keep
+        ctx._datasets.add(self)
         n, d, nnz = C.c_uint64(), C.c_uint64(), C.c_uint64()
         check(load().mcg_data_shape(h, C.byref(n), C.byref(d), C.byref(nnz)))
         self.n, self.d, self.nnz = n.value, d.value, nnz.value

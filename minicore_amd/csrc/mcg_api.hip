@@ -787,12 +787,94 @@ int mcg_centroids_hard(mcg_data *ds, const uint32_t *asn, const double *weights,
 // accepted assignment that leaves a cluster empty hands the next iteration to the
 // host, which runs the reservoir restart of centroid.h:687-752 and then goes back to
 // device batches.
+// perform_hard_clustering (solve.h:98-175) for the median policies of L1, TVD and
+// L2 (centroid.h:434-647): host-driven, one synchronisation per iteration.  The
+// M-step works on copies of the accepted assignment and costs (the empty-cluster
+// restart moves rows in them); set_centroids_hard reports no restart for these
+// policies, so a rising cost always reverts (solve.h:160-164).
+static void lloyd_median(Data &D, int msr, double prior, float *centers, uint64_t k, const double *weights,
+                         double eps, uint64_t maxiter, uint32_t *asn, float *costs, double *initcost,
+                         double *finalcost, uint64_t *numiter) {
+    Ctx &c = *D.ctx;
+    hipStream_t st = c.stream;
+    const uint64_t n = D.n, d = D.d, kd = k * d;
+    const int pol = median_policy(msr);
+    if (D.csr) {
+        if (weights)
+            MCG_THROW(MCG_EUNSUPPORTED, pol == 2 ? "Not implemented: weighted TVD median for CSparseMatrix"
+                                                 : "Not implemented: weighted L1 median for CSparseMatrix");
+        if (pol == 3)
+            MCG_THROW(MCG_EUNSUPPORTED, "geometric median of CSR rows: the reference indexes its per-member costs "
+                                        "by row id (csc.h:1170-1177), which has no defined result");
+        REQUIRE(!csr_has_negative(D, c.stream), "CSR L1/TVD median centers need non-negative values");
+    }
+    DBuf sw, sa_out, sk_out;
+    const double *W = (const double *)in_dev(c, weights, n * 8, sw);
+    if (weights && pol != 3) {
+        std::vector<double> hw(n);
+        copy_sync(hw.data(), W, n * 8, hipMemcpyDeviceToHost, st);
+        for (uint64_t i = 0; i < n; ++i) REQUIRE(hw[i] > 0., "weighted median centers need positive weights");
+    }
+    float *C = (float *)c.s[S_CENT_A].reserve(kd * 4);
+    float *Cc = (float *)c.s[S_CENT_B].reserve(kd * 4);
+    uint32_t *A = (uint32_t *)c.s[S_ASN_A].reserve(n * 4);
+    uint32_t *An = (uint32_t *)c.s[S_ASN_B].reserve(n * 4);
+    float *pc = (float *)c.s[S_COSTS].reserve(2 * n * 4);
+    float *pcn = pc + n;
+    HIP_CHECK(hipMemcpyAsync(C, centers, kd * 4, is_device_ptr(centers) ? hipMemcpyDeviceToDevice
+                                                                       : hipMemcpyHostToDevice, st));
+    generic_costs(D, msr, prior, C, k, nullptr, A, pc, st);
+    float cost = (float)fixed_wsum(c, pc, W, n, st);
+    uint64_t iter = 0;
+    *initcost = cost;
+    if (cost == 0.f) {
+        *initcost = 0.;
+        *finalcost = 0.;
+    } else {
+        copy_f32(C, Cc, kd, st);
+        for (;;) {
+            HIP_CHECK(hipMemcpyAsync(An, A, n * 4, hipMemcpyDeviceToDevice, st));
+            copy_f32(pc, pcn, n, st);
+            median_mstep(D, msr, W, k, An, pcn, Cc, st);
+            generic_costs(D, msr, prior, Cc, k, nullptr, An, pcn, st);
+            const float newcost = (float)fixed_wsum(c, pcn, W, n, st);
+            if (newcost > cost) break;   // the accepted state stays (the reference re-assigns to it)
+            copy_f32(Cc, C, kd, st);
+            std::swap(A, An);
+            std::swap(pc, pcn);
+            ++iter;
+            const float old = cost;
+            cost = newcost;
+            const float diff = old - newcost;
+            if ((double)diff < eps * std::max((double)newcost, (double)old) || iter > maxiter) break;
+        }
+        *finalcost = cost;
+    }
+    *numiter = iter;
+    Out oa = out_dev(c, asn, n * 4, sa_out);
+    Out ok = out_dev(c, costs, n * 4, sk_out);
+    HIP_CHECK(hipMemcpyAsync(oa.dev, A, n * 4, hipMemcpyDeviceToDevice, st));
+    if (ok.dev) HIP_CHECK(hipMemcpyAsync(ok.dev, pc, n * 4, hipMemcpyDeviceToDevice, st));
+    out_finish(c, oa);
+    out_finish(c, ok);
+    HIP_CHECK(hipMemcpyAsync(centers, C, kd * 4, is_device_ptr(centers) ? hipMemcpyDeviceToDevice
+                                                                       : hipMemcpyDeviceToHost, st));
+    sync(c);
+}
+
 int mcg_lloyd(mcg_data *ds, int msr, double prior, float *centers, uint64_t k, const double *weights, double eps,
               uint64_t maxiter, int precision, uint32_t *asn, float *costs, double *initcost, double *finalcost,
               uint64_t *numiter) {
     return guard([&] {
         REQUIRE(ds && centers && asn, "null argument");
         REQUIRE(k > 0, "k must be positive");
+        check_msr(msr);
+        if (median_policy(msr)) {
+            HIP_CHECK(hipSetDevice(ds->d.ctx->device));
+            lloyd_median(ds->d, msr, prior, centers, k, weights, eps, maxiter, asn, costs, initcost, finalcost,
+                         numiter);
+            return;
+        }
         check_mean_msr(msr);
         Data &D = ds->d;
         Ctx &c = *D.ctx;

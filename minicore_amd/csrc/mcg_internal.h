@@ -144,7 +144,7 @@ enum Slot {
     S_HIST, S_BASE, S_PERM, S_CNT, S_ITEMOFF, S_PART, S_PARTW, S_PARTQ,
     S_CCOST, S_TOTAL, S_CENT_A, S_CENT_B, S_CENT_C, S_COSTS, S_CT, S_CPARAM,
     S_MISC0, S_MISC1, S_MISC2, S_MISC3, S_W, S_STAGE0, S_STAGE1, S_STAGE2,
-    S_LIS, S_TOT, S_JSDG, S_PAIRS, S_THRESH, S_OVER, S_KEYS, S_OFLAG, S_CINFO, S_CENT_D, S_LCTL, S_PREP, S_ROWLEN, S_NSLOTS
+    S_LIS, S_TOT, S_JSDG, S_PAIRS, S_THRESH, S_OVER, S_KEYS, S_OFLAG, S_CINFO, S_CENT_D, S_LCTL, S_PREP, S_ROWLEN, S_MED0, S_MED1, S_MED2, S_NSLOTS
 };
 static_assert(S_NSLOTS <= 48, "scratch slots");
 
@@ -179,6 +179,23 @@ struct MStepOut {
 };
 MStepOut dense_mstep(Data &ds, const uint32_t *asn, const double *w, uint64_t k, const float *Cold,
                      float *Cnew, bool want_cost, hipStream_t st);
+// Stable counting sort of point ids by assignment (ids >= k left out): perm holds
+// each cluster's members contiguous and ascending, at off[c], counts[c] of them;
+// itemoff the prefix of 256-member chunks.  k <= 16384.
+struct SortOut {
+    uint32_t *perm, *counts, *off, *itemoff;
+};
+SortOut sort_by_cluster(Ctx &c, const uint32_t *asn, uint64_t n, uint64_t k, hipStream_t st);
+// Exclusive prefix sum of L u32 values (tmp: scratch for block totals).
+void scan_exclusive(const uint32_t *in, uint32_t *out, uint64_t L, DBuf &tmp, hipStream_t st);
+// Median centroid policies (L1, TVD, L2 -> geometric median), centroid.h:434-647:
+// the M-step from the accepted assignment and costs (Ar, pr: working copies that
+// the empty-cluster restart updates), previous centers in C, new centers into C.
+inline int median_policy(int msr) { return msr == MCG_L1 ? 1 : msr == MCG_TVD ? 2 : msr == MCG_L2 ? 3 : 0; }
+void median_mstep(Data &ds, int msr, const double *w, uint64_t k, uint32_t *Ar, float *pr, float *C,
+                  hipStream_t st);
+// true if a stored CSR value is negative (the CSR median rules assume non-negative data)
+bool csr_has_negative(Data &ds, hipStream_t st);
 MStepOut csr_mstep(Data &ds, const uint32_t *asn, const double *w, uint64_t k, float *Cnew,
                    hipStream_t st);
 

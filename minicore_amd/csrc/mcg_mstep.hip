@@ -114,7 +114,7 @@ __global__ void k_scan_apply(const uint32_t *__restrict__ in, uint64_t L, const 
     }
 }
 
-static void scan_exclusive(const uint32_t *in, uint32_t *out, uint64_t L, DBuf &tmp, hipStream_t st) {
+void scan_exclusive(const uint32_t *in, uint32_t *out, uint64_t L, DBuf &tmp, hipStream_t st) {
     const uint64_t per_blk = 256ull * SCAN_PER;
     const uint64_t nb = cdiv(L, per_blk);
     tmp.reserve((nb + 1) * sizeof(uint32_t));
@@ -320,11 +320,7 @@ __global__ void k_total(const double *__restrict__ ccost, const double *__restri
     if (threadIdx.x == 0) { total[0] = a[0]; total[1] = b[0]; }
 }
 
-struct SortOut {
-    uint32_t *perm, *counts, *off, *itemoff;
-};
-
-static SortOut sort_by_cluster(Ctx &c, const uint32_t *asn, uint64_t n, uint64_t k, hipStream_t st) {
+SortOut sort_by_cluster(Ctx &c, const uint32_t *asn, uint64_t n, uint64_t k, hipStream_t st) {
     REQUIRE(k <= MAX_SORT_K, "k must be <= 16384 for the GPU M-step");
     REQUIRE(n < (1ull << 32) - 1, "n must fit in uint32");
     const uint32_t nblk = (uint32_t)cdiv(n, SORT_BLK);

@@ -1227,3 +1227,267 @@ void mco_coreset_sample(uint64_t n, const float *probs, const double *w, uint64_
     }
     free(pd); free(fw); free(pre);
 }
+
+/* ======================================================================
+ * Median centroid policies: set_centroids_l1 / set_centroids_tvd /
+ * set_centroids_l2 (centroid.h:434-647) with l1_median / weighted_median /
+ * tvd_median (optim/kmedian.h:124-350, util/csc.h:829-950) and the dense
+ * geometric median (util/blaze_adaptor.h:614-690), under the hard Lloyd loop
+ * of solve.h:98-175 (set_centroids_hard never reports a restart for these
+ * policies, solve.h:207-215, so a rising cost always reverts).
+ *
+ * Empty clusters (centroid.h:442-480): each, in ascending id, takes the row
+ * drawn over the current per-point costs (fixed-point sampler, wyrand stream
+ * of n, draws 0, 1, ... per M-step), and every row strictly closer to it by
+ * raw L1 (L1 and TVD policies) or L2 (geometric median) moves there; repeated
+ * until no cluster is empty.  Then per cluster, members in ascending row id:
+ *  - one member: the center is that row (raw, also under TVD);
+ *  - L1, unweighted: per column the median of the members' values, even
+ *    counts 0.5f * (v[h-1] + v[h]) in float (kmedian.h:124-150);
+ *  - L1, weighted: weighted_median never adds to its weight total
+ *    (kmedian.h:275-285), so every column takes the value of the heaviest
+ *    member (ties: the smallest value);
+ *  - TVD: the same on x_ij * (1 / rowsum_i) in double (kmedian.h:347-350),
+ *    rounded to float; weighted, the member of rank j carries w[j], the
+ *    weight of ROW j (the full weight vector indexed by rank, kmedian.h:340);
+ *  - CSR L1 / TVD (unweighted only; csc.h throws for weights): the rules of
+ *    csc.h:852-884 / 924-950 over the stored values with (m - nel) implicit
+ *    zeros, for non-negative data (see csr_median_value);
+ *  - geometric median (dense): Weiszfeld from the current center; per
+ *    round d_i = max(float(||x_i - c||), floor) (floor 1e-30f, weighted
+ *    1e-40f), cost = float(sum d_i) (weighted: sum w_i d_i); stop when the
+ *    previous cost minus this one is <= 0 (the center stays the one just
+ *    evaluated); else q_i = 1/d_i (weighted w_i/d_i), the center becomes
+ *    sum_i (q_i / Q) x_i (weighted q_i * (1/Q)).  All in FP64 here, rounded
+ *    to float per center; member sums are 256 strided sequential partials
+ *    added as a binary tree, center sums 4 strided partials added
+ *    ((p0 + p1) + (p2 + p3)), the order the GPU library uses.  The reference
+ *    works in float, where w/1e-40f overflows (parity unpinned there).
+ * ====================================================================== */
+static int median_policy(int msr) {
+    return msr == MCO_L1 ? 1 : msr == MCO_TVD ? 2 : msr == MCO_L2 ? 3 : 0;
+}
+
+static int cmp_f32(const void *a, const void *b) {
+    const float x = *(const float *)a, y = *(const float *)b;
+    return x < y ? -1 : x > y;
+}
+static int cmp_f64(const void *a, const void *b) {
+    const double x = *(const double *)a, y = *(const double *)b;
+    return x < y ? -1 : x > y;
+}
+
+static double tree_sum256(const double *v, uint64_t m) {
+    double p[256];
+    for (int t = 0; t < 256; ++t) {
+        double s = 0.;
+        for (uint64_t q = (uint64_t)t; q < m; q += 256) s += v[q];
+        p[t] = s;
+    }
+    for (int o = 128; o > 0; o >>= 1)
+        for (int t = 0; t < o; ++t) p[t] += p[t + o];
+    return p[0];
+}
+
+/* Center value of one CSR column from its nel stored values s (sorted
+ * ascending, non-negative) among m members: full = (m - nel) zeros then s.
+ * csc.h:852-884 (L1) / 924-950 (TVD): with nel > m/2 and all stored values
+ * positive, full[h] (odd) or (full[h] + full[h+1]) / 2 (even); with an explicit
+ * zero stored, odd full[h], even L1 (full[h] + full[h+1]) / 2, TVD
+ * (full[h-1] + full[h]) / 2; with odd m and nel == m/2 and all positive,
+ * s[0] / 2; otherwise 0.  Where the reference reads one past the column's
+ * values (m == 2 with the even rule h+1), the index is clamped to the last. */
+static float csr_median_value(const float *s, uint64_t nel, uint64_t m, int tvd) {
+    const uint64_t h = m / 2, z = m - nel;
+    const int odd = (int)(m & 1);
+    #define FULL(r) ((r) < z ? 0.f : s[(r) - z])
+    if (nel > h) {
+        if (odd) return FULL(h);
+        const uint64_t r1 = h + 1 < m ? h + 1 : m - 1;
+        if (s[0] > 0.f || !tvd) return (float)(.5 * (double)(FULL(h) + FULL(r1)));
+        return (float)(.5 * (double)(FULL(h - 1) + FULL(h)));
+    }
+    if (odd && nel == h && s[0] > 0.f) return (float)(.5 * (double)s[0]);
+    return 0.f;
+    #undef FULL
+}
+
+static void geo_median(const float *X, uint64_t d, const uint64_t *mem, uint64_t m, const double *w, float *cur,
+                       double eps, double *cq, double *qq) {
+    const double floor_ = w ? (double)1e-40f : (double)1e-30f;
+    float prev = FLT_MAX;
+    double *acc = (double *)malloc(4 * d * sizeof(double));
+    for (uint64_t it = 0; it < 100000; ++it) {
+        for (uint64_t q = 0; q < m; ++q) {
+            const float *x = X + mem[q] * d;
+            double s = 0.;
+            for (uint64_t j = 0; j < d; ++j) { const double t = (double)x[j] - (double)cur[j]; s = fma(t, t, s); }
+            double c = (double)(float)sqrt(s);
+            if (isnan(c)) c = floor_;
+            cq[q] = c > floor_ ? c : floor_;
+            qq[q] = w ? w[mem[q]] * cq[q] : cq[q];
+        }
+        const float curf = (float)tree_sum256(qq, m);
+        const float dist = prev - curf;
+        if ((double)dist <= eps) break;
+        for (uint64_t q = 0; q < m; ++q) qq[q] = w ? w[mem[q]] / cq[q] : 1. / cq[q];
+        const double S = tree_sum256(qq, m), inv = 1. / S;
+        memset(acc, 0, 4 * d * sizeof(double));
+        for (uint64_t q = 0; q < m; ++q) {
+            const double coef = w ? qq[q] * inv : qq[q] / S;
+            const float *x = X + mem[q] * d;
+            double *a = acc + (q & 3) * d;
+            for (uint64_t j = 0; j < d; ++j) a[j] = fma(coef, (double)x[j], a[j]);
+        }
+        for (uint64_t j = 0; j < d; ++j) cur[j] = (float)((acc[j] + acc[d + j]) + (acc[2 * d + j] + acc[3 * d + j]));
+        prev = curf;
+    }
+    free(acc);
+}
+
+/* The M-step of one median policy from (asn, cost), which the empty-cluster
+ * restart updates in place.  C holds the previous centers (the geometric
+ * median starts from them; restarted clusters from their drawn row).
+ * Returns 0, or -2 for an unsupported case (weighted CSR, CSR geometric median). */
+int mco_median_centers(int msr, const float *X, const float *val, const uint32_t *idx, const uint64_t *indptr,
+                       uint64_t n, uint64_t d, const double *w, uint64_t k, uint32_t *asn, double *cost, float *C,
+                       double eps) {
+    const int pol = median_policy(msr);
+    if (!pol) return -2;
+    if (val && (w || pol == 3)) return -2;
+    uint64_t *cnt = (uint64_t *)calloc(k + 1, sizeof(uint64_t));
+    uint64_t draw = 0;
+    for (;;) {
+        memset(cnt, 0, (k + 1) * sizeof(uint64_t));
+        for (uint64_t i = 0; i < n; ++i) cnt[asn[i]]++;
+        int any = 0;
+        for (uint64_t c = 0; c < k; ++c) {
+            if (cnt[c]) continue;
+            any = 1;
+            uint64_t *fw = (uint64_t *)malloc((n ? n : 1) * sizeof(uint64_t));
+            mco_fixed_weights(cost, n, fw);
+            const uint64_t r = mco_fixed_draw(fw, n, mco_wyrand_at(n, draw++));
+            free(fw);
+            if (val) {
+                memset(C + c * d, 0, d * sizeof(float));
+                for (uint64_t p = indptr[r]; p < indptr[r + 1]; ++p) C[c * d + idx[p]] = val[p];
+            } else memcpy(C + c * d, X + r * d, d * sizeof(float));
+            const int rm = pol == 3 ? MCO_L2 : MCO_L1;
+            for (uint64_t i = 0; i < n; ++i) {
+                const double v = point_cost(rm, X, val, idx, indptr, i, d, C + c * d, 0.);
+                if (v < cost[i]) { asn[i] = (uint32_t)c; cost[i] = v; }
+            }
+        }
+        if (!any) break;
+    }
+    /* members per cluster, ascending row id */
+    uint64_t *off = (uint64_t *)calloc(k + 1, sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; ++i) off[asn[i] + 1]++;
+    for (uint64_t c = 0; c < k; ++c) off[c + 1] += off[c];
+    uint64_t *mem = (uint64_t *)malloc((n ? n : 1) * sizeof(uint64_t)), *cur = (uint64_t *)malloc(k * sizeof(uint64_t));
+    memcpy(cur, off, k * sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; ++i) mem[cur[asn[i]]++] = i;
+    double *rs = (double *)malloc((n ? n : 1) * sizeof(double));
+    for (uint64_t i = 0; i < n; ++i) rs[i] = val ? row_sum_csr(val, indptr[i], indptr[i + 1]) : row_sum_dense(X + i * d, d);
+    float *vf = (float *)malloc((n ? n : 1) * sizeof(float));
+    double *vd = (double *)malloc((n ? n : 1) * sizeof(double)), *qq = (double *)malloc((n ? n : 1) * sizeof(double));
+    for (uint64_t c = 0; c < k; ++c) {
+        const uint64_t *mm = mem + off[c], m = off[c + 1] - off[c];
+        float *ctr = C + c * d;
+        if (m == 1) {
+            const uint64_t r = mm[0];
+            if (val) {
+                memset(ctr, 0, d * sizeof(float));
+                for (uint64_t p = indptr[r]; p < indptr[r + 1]; ++p) ctr[idx[p]] = val[p];
+            } else memcpy(ctr, X + r * d, d * sizeof(float));
+            continue;
+        }
+        if (pol == 3) { geo_median(X, d, mm, m, w, ctr, eps, vd, qq); continue; }
+        if (val) {
+            /* per column, the stored values of the members */
+            for (uint64_t j = 0; j < d; ++j) {
+                uint64_t nel = 0;
+                for (uint64_t q = 0; q < m; ++q) {
+                    const uint64_t r = mm[q];
+                    for (uint64_t p = indptr[r]; p < indptr[r + 1]; ++p) {
+                        if (idx[p] != j) continue;
+                        vf[nel++] = pol == 2 ? val[p] * (float)(1. / rs[r]) : val[p];
+                    }
+                }
+                if (!nel) { ctr[j] = 0.f; continue; }
+                qsort(vf, nel, sizeof(float), cmp_f32);
+                ctr[j] = csr_median_value(vf, nel, m, pol == 2);
+            }
+            continue;
+        }
+        if (w) {
+            /* heaviest member (L1: by row weight; TVD: by w[rank]), smallest value */
+            double wmax = -INFINITY;
+            for (uint64_t q = 0; q < m; ++q) { const double wq = pol == 2 ? w[q] : w[mm[q]]; if (wq > wmax) wmax = wq; }
+            for (uint64_t j = 0; j < d; ++j) {
+                double best = INFINITY;
+                for (uint64_t q = 0; q < m; ++q) {
+                    const double wq = pol == 2 ? w[q] : w[mm[q]];
+                    if (wq != wmax) continue;
+                    const float x = X[mm[q] * d + j];
+                    const double v = pol == 2 ? (double)x * (1. / rs[mm[q]]) : (double)x;
+                    if (v < best) best = v;
+                }
+                ctr[j] = (float)best;
+            }
+            continue;
+        }
+        const uint64_t h = m / 2;
+        for (uint64_t j = 0; j < d; ++j) {
+            if (pol == 1) {
+                for (uint64_t q = 0; q < m; ++q) vf[q] = X[mm[q] * d + j];
+                qsort(vf, m, sizeof(float), cmp_f32);
+                ctr[j] = (m & 1) ? vf[h] : 0.5f * (vf[h - 1] + vf[h]);
+            } else {
+                for (uint64_t q = 0; q < m; ++q) vd[q] = (double)X[mm[q] * d + j] * (1. / rs[mm[q]]);
+                qsort(vd, m, sizeof(double), cmp_f64);
+                ctr[j] = (float)((m & 1) ? vd[h] : 0.5 * (vd[h - 1] + vd[h]));
+            }
+        }
+    }
+    free(cnt); free(off); free(mem); free(cur); free(rs); free(vf); free(vd); free(qq);
+    return 0;
+}
+
+/* perform_hard_clustering (solve.h:98-175) for L1, TVD and L2; costs in the
+ * control flow rounded to float, as in mco_lloyd. */
+int mco_lloyd_median(int msr, const float *X, const float *val, const uint32_t *idx, const uint64_t *indptr,
+                     uint64_t n, uint64_t d, float *C, uint64_t k, double prior, const double *w, double eps,
+                     uint64_t maxiter, uint32_t *asn, double *cost, double *initcost, double *finalcost,
+                     uint64_t *numiter) {
+    if (!median_policy(msr)) return -2;
+    if (val && (w || msr == MCO_L2)) return -2;
+    assign_generic(msr, X, val, idx, indptr, n, d, C, k, prior, asn, cost, 0);
+    float fcost = (float)total_cost(cost, w, n);
+    *initcost = fcost;
+    if (fcost == 0.f) { *initcost = 0.; *finalcost = 0.; *numiter = 0; return 0; }
+    float *Cc = (float *)malloc(k * d * sizeof(float));
+    memcpy(Cc, C, k * d * sizeof(float));
+    uint64_t iter = 0;
+    int rc = 0;
+    for (;;) {
+        rc = mco_median_centers(msr, X, val, idx, indptr, n, d, w, k, asn, cost, Cc, 0.);
+        if (rc) break;
+        assign_generic(msr, X, val, idx, indptr, n, d, Cc, k, prior, asn, cost, 0);
+        const float newcost = (float)total_cost(cost, w, n);
+        if (newcost > fcost) {
+            assign_generic(msr, X, val, idx, indptr, n, d, C, k, prior, asn, cost, 0);
+            break;
+        }
+        memcpy(C, Cc, k * d * sizeof(float));
+        ++iter;
+        const float old = fcost;
+        fcost = newcost;
+        const float diff = old - newcost;
+        if ((double)diff < eps * ((double)newcost > (double)old ? (double)newcost : (double)old) || iter > maxiter) break;
+    }
+    free(Cc);
+    *finalcost = fcost;
+    *numiter = iter;
+    return rc;
+}

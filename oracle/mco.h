@@ -131,6 +131,14 @@ uint64_t mco_kmeanspp_ls(int msr, const float *X, const float *val, const uint32
                          const uint64_t *indptr, uint64_t n, uint64_t d, uint64_t k,
                          double prior, uint64_t seed, const double *w, uint64_t rounds,
                          uint32_t *centers, uint32_t *asn, float *dist, int nthreads);
+/* median centroid policies (L1, TVD, L2): centroid.h:434-647 */
+int mco_median_centers(int msr, const float *X, const float *val, const uint32_t *idx, const uint64_t *indptr,
+                       uint64_t n, uint64_t d, const double *w, uint64_t k, uint32_t *asn, double *cost, float *C,
+                       double eps);
+int mco_lloyd_median(int msr, const float *X, const float *val, const uint32_t *idx, const uint64_t *indptr,
+                     uint64_t n, uint64_t d, float *C, uint64_t k, double prior, const double *w, double eps,
+                     uint64_t maxiter, uint32_t *asn, double *cost, double *initcost, double *finalcost,
+                     uint64_t *numiter);
 
 /* ---- coreset sensitivities and sampling: coreset.h:340-699 ------------ */
 /* sens: 0 BFL, 1 FL, 3 VX, 4 LBK.  probs out in float (CoresetSampler<float,...>). */

@@ -69,6 +69,8 @@ def _setup(L):
         "mco_softmax_rows": (None, [P, U64, U64, D, P]),
         "mco_kmeanspp": (U64, [I, P, P, P, P, U64, U64, U64, D, U64, P, P, P, P, I]),
         "mco_kmeanspp_ls": (U64, [I, P, P, P, P, U64, U64, U64, D, U64, P, U64, P, P, P, I]),
+        "mco_median_centers": (I, [I, P, P, P, P, U64, U64, P, U64, P, P, P, D]),
+        "mco_lloyd_median": (I, [I, P, P, P, P, U64, U64, P, U64, D, P, D, U64, P, P, P, P, P]),
         "mco_coreset_probs": (I, [I, U64, U64, P, P, P, P]),
         "mco_coreset_sample": (None, [U64, P, P, U64, U64, U64, P, P]),
     }
@@ -261,6 +263,39 @@ def lloyd(msr, X, C0, prior=0., w=None, eps=1e-10, maxiter=100, nthreads=0):
                          maxiter, _p(asn), _p(cost), _p(ic), _p(fc), _p(it), nthreads)
     if rc != 0:
         raise RuntimeError("oracle lloyd failed (NaN center)")
+    return dict(centers=C, asn=asn, costs=cost, initcost=float(ic[0]), finalcost=float(fc[0]),
+                numiter=int(it[0]))
+
+
+def median_centers(msr, X, asn, cost, C0, w=None):
+    """One median-policy M-step (L1, TVD, L2) from (asn, cost); returns
+    (centers, asn, cost) after the empty-cluster restarts."""
+    Xd, val, idx, ptr, n, d = _mat(X)
+    C = _f32(C0).copy()
+    a = np.ascontiguousarray(asn, dtype=np.uint32).copy()
+    c = np.ascontiguousarray(cost, dtype=np.float64).copy()
+    w = _f64(w)
+    rc = lib().mco_median_centers(msr, _p(Xd), _p(val), _p(idx), _p(ptr), n, d, _p(w), C.shape[0], _p(a), _p(c),
+                                  _p(C), 0.)
+    if rc != 0:
+        raise ValueError("unsupported median case")
+    return C, a, c
+
+
+def lloyd_median(msr, X, C0, prior=0., w=None, eps=1e-10, maxiter=100):
+    """perform_hard_clustering for the median policies (L1, TVD, L2)."""
+    Xd, val, idx, ptr, n, d = _mat(X)
+    C = _f32(C0).copy()
+    k = C.shape[0]
+    w = _f64(w)
+    asn = np.empty(n, dtype=np.uint32)
+    cost = np.empty(n, dtype=np.float64)
+    ic, fc = np.zeros(1), np.zeros(1)
+    it = np.zeros(1, dtype=np.uint64)
+    rc = lib().mco_lloyd_median(msr, _p(Xd), _p(val), _p(idx), _p(ptr), n, d, _p(C), k, prior, _p(w), eps,
+                                maxiter, _p(asn), _p(cost), _p(ic), _p(fc), _p(it))
+    if rc != 0:
+        raise ValueError("unsupported median case")
     return dict(centers=C, asn=asn, costs=cost, initcost=float(ic[0]), finalcost=float(fc[0]),
                 numiter=int(it[0]))
 

@@ -500,3 +500,157 @@ def test_coreset_sample_unbiased(mco):
         tot.append(wt.astype(np.float64).sum())
     # E[sum of weights] = n
     assert abs(np.mean(tot) - n) < 0.05 * n
+
+
+# ---- median centroid policies (centroid.h:434-647) -------------------------------------
+def _median_case(seed=21, n=601, d=9, k=5):
+    rng = np.random.default_rng(seed)
+    X = (rng.normal(size=(n, d)) * 3 + rng.integers(-5, 5, size=(1, d))).astype(np.float32)
+    asn = rng.integers(0, k, n).astype(np.uint32)
+    return X, asn, np.ones(n), np.zeros((k, d), np.float32)
+
+
+def test_l1_median_unweighted_is_numpy_median(mco):
+    X, asn, cost, C0 = _median_case()
+    asn[asn == 4] = 3
+    asn[:7] = 4                             # an odd cluster; the random ones give even sizes too
+    asn[7:9] = 0
+    C, a, _ = mco.median_centers(mco.L1, X, asn, cost, C0)
+    for q in range(5):
+        ref = np.median(X[a == q], axis=0).astype(np.float32)   # float32: (a+b)/2 in float
+        assert np.array_equal(C[q], ref), q
+    assert {int((a == q).sum()) % 2 for q in range(5)} == {0, 1}
+
+
+def test_l1_median_weighted_is_heaviest_member(mco):
+    # weighted_median's weight total stays 0 (kmedian.h:275-285): the heaviest member's values
+    X, asn, cost, C0 = _median_case(seed=22)
+    rng = np.random.default_rng(3)
+    w = rng.random(len(X)) + 0.1
+    C, a, _ = mco.median_centers(mco.L1, X, asn, cost, C0, w=w)
+    for q in range(5):
+        mem = np.nonzero(a == q)[0]
+        assert np.array_equal(C[q], X[mem[np.argmax(w[mem])]])
+    # tied heaviest weights: per column the smallest of the tied rows
+    w2 = np.ones(len(X))
+    C2, a2, _ = mco.median_centers(mco.L1, X, asn, cost, C0, w=w2)
+    for q in range(5):
+        assert np.array_equal(C2[q], X[a2 == q].min(0))
+
+
+def test_tvd_median_unweighted_and_weighted(mco):
+    X, asn, cost, C0 = _median_case(seed=23)
+    X = np.abs(X) + 0.5
+    C, a, _ = mco.median_centers(mco.TVD, X, asn, cost, C0)
+    P = X.astype(np.float64) * (1. / X.astype(np.float64).sum(1, keepdims=True))
+    for q in range(5):
+        assert np.array_equal(C[q], np.median(P[a == q], axis=0).astype(np.float32))
+    # weighted: the member of rank j carries w[j] (the full vector indexed by rank, kmedian.h:340)
+    w = np.random.default_rng(4).random(len(X)) + 0.1
+    Cw, aw, _ = mco.median_centers(mco.TVD, X, asn, cost, C0, w=w)
+    for q in range(5):
+        mem = np.nonzero(aw == q)[0]
+        j = int(np.argmax(w[:len(mem)]))
+        assert np.array_equal(Cw[q], P[mem[j]].astype(np.float32))
+
+
+def test_single_member_center_is_raw_row(mco):
+    X, asn, cost, C0 = _median_case(seed=24)
+    X = np.abs(X) + 1
+    asn[asn == 2] = 1
+    asn[10] = 2
+    for msr in (mco.L1, mco.TVD, mco.L2):
+        C, a, _ = mco.median_centers(msr, X, asn, cost, C0)
+        assert np.array_equal(C[2], X[10]), msr
+
+
+def test_geometric_median_optimal(mco):
+    from scipy.optimize import minimize
+    X, asn, cost, C0 = _median_case(seed=25, n=300, d=4, k=3)
+    C0 = np.stack([X[asn == q].mean(0) + 0.3 for q in range(3)]).astype(np.float32)
+    w = np.random.default_rng(5).random(len(X)) + 0.2
+    for ww in (None, w):
+        C, a, _ = mco.median_centers(mco.L2, X, asn, cost, C0, w=ww)
+        for q in range(3):
+            P = X[a == q].astype(np.float64)
+            wq = np.ones(len(P)) if ww is None else ww[a == q]
+            f = lambda c: float((wq * np.sqrt(((P - c) ** 2).sum(1))).sum())
+            opt = minimize(f, P.mean(0), method="Nelder-Mead", options=dict(xatol=1e-9, fatol=1e-12, maxiter=20000))
+            assert f(C[q].astype(np.float64)) <= opt.fun * (1 + 1e-6), q
+            np.testing.assert_allclose(C[q], opt.x, atol=2e-3)
+    # symmetric points: the center of symmetry
+    S = np.array([[1, 0], [-1, 0], [0, 1], [0, -1], [2, 2], [-2, -2], [2, -2], [-2, 2]], np.float32) + 5
+    Cs, _, _ = mco.median_centers(mco.L2, S, np.zeros(8, np.uint32), np.ones(8), np.array([[5.3, 4.8]], np.float32))
+    # (the stop rule compares float costs, so the iterate stops within ~sqrt(ulp) of the optimum)
+    np.testing.assert_allclose(Cs[0], [5, 5], atol=1e-3)
+
+
+def test_median_restart_fills_empty_clusters(mco):
+    X, asn, cost, C0 = _median_case(seed=26)
+    asn[asn == 3] = 0
+    cost = np.random.default_rng(6).random(len(X)) + 0.5
+    for msr in (mco.L1, mco.TVD, mco.L2):
+        Xp = np.abs(X) + 1 if msr == mco.TVD else X
+        C, a, c = mco.median_centers(msr, Xp, asn, cost, C0)
+        assert set(np.unique(a)) == set(range(5))
+        # the first draw (stream of n, draw 0) over the costs becomes center 3
+        fw = mco.fixed_weights(cost)
+        r = mco.fixed_draw(fw, int(mco.wyrand_at(len(X), 0)))
+        assert a[r] == 3
+
+
+def _csr(mco, X):
+    import scipy.sparse as sp
+    return mco.CSR.from_scipy(sp.csr_matrix(X))
+
+
+def test_csr_l1_median_rules(mco):
+    rng = np.random.default_rng(27)
+    n, d = 40, 6
+    X = rng.integers(1, 9, size=(n, d)).astype(np.float32)
+    X[rng.random((n, d)) < 0.2] = 0
+    for m in (5, 6, 7, 2):
+        asn = np.full(n, 1, np.uint32)
+        asn[:m] = 0
+        C, a, _ = mco.median_centers(mco.L1, _csr(mco, X), asn, np.ones(n), np.zeros((2, d), np.float32))
+        F = np.sort(X[:m], axis=0)
+        h = m // 2
+        for j in range(d):
+            nel = int((X[:m, j] != 0).sum())
+            if nel > h:
+                want = F[h, j] if m & 1 else np.float32(0.5) * (F[h, j] + F[min(h + 1, m - 1), j])
+            elif m & 1 and nel == h:
+                want = np.float32(0.5) * X[:m, j][X[:m, j] > 0].min()
+            else:
+                want = 0
+            assert C[0, j] == want, (m, j)
+
+
+def test_csr_tvd_even_rule_and_dense_odd(mco):
+    rng = np.random.default_rng(28)
+    n, d = 30, 5
+    X = rng.integers(1, 9, size=(n, d)).astype(np.float32)
+    asn = np.zeros(n, np.uint32)
+    asn[15:] = 1
+    C, a, _ = mco.median_centers(mco.TVD, _csr(mco, X), asn, np.ones(n), np.zeros((2, d), np.float32))
+    inv = (1. / X.astype(np.float64).sum(1)).astype(np.float32)
+    P = X * inv[:, None]
+    for q, m in ((0, 15), (1, 15)):
+        assert np.array_equal(C[q], np.sort(P[a == q], axis=0)[m // 2])
+    # no implicit zeros and odd counts: CSR == dense TVD median up to the float vs double value
+    Cd, _, _ = mco.median_centers(mco.TVD, X, asn, np.ones(n), np.zeros((2, d), np.float32))
+    np.testing.assert_allclose(C, Cd, rtol=1e-6)
+
+
+def test_lloyd_median_rules(mco):
+    from minicore_amd.datagen import gaussian_mixture
+    X, lab, means = gaussian_mixture(1500, 6, 5, seed=12, lo=-3, hi=3)
+    C0 = X[[0, 1, 2, 3, 4]].copy()
+    for msr in (mco.L1, mco.L2):
+        r = mco.lloyd_median(msr, X, C0, eps=1e-10, maxiter=50)
+        assert r["finalcost"] <= r["initcost"]
+        D = mco.cost_matrix(msr, X, r["centers"], 0.)
+        assert np.array_equal(D.argmin(1), r["asn"])
+        if msr == mco.L1:   # converged: the centers are the medians of their assignment
+            C2, _, _ = mco.median_centers(msr, X, r["asn"], r["costs"], r["centers"])
+            assert np.array_equal(C2, r["centers"])

@@ -281,6 +281,81 @@ def minibatch_config(args, mcg):
                     "65,536-row minibatch (from a run with check-ins only at the first and last iteration)"}
 
 
+def median_config(args, mcg):
+    """Auxiliary measurement of the median centroid policies (hcluster with msr L1, TVD
+    or L2; centroid.h:434-647) on the headline rows: per policy the E-step alone
+    (tiled L1 / L2 kernel, pair-by-pair TVD) and the Lloyd iteration (E-step + median
+    M-step) from the difference of runs with maxiter 0 and maxiter 4 (eps 0).  TVD
+    runs on exp(x / 8) of the same rows (it needs positive values)."""
+    import numpy as np
+    import torch
+    from minicore_amd.datagen import gaussian_mixture_device
+    n, d, k = args.n, args.d, args.k
+    log(f"median: generating n={n} d={d} k={k}")
+    X, _, _ = gaussian_mixture_device(n, d, k, args.seed, scale=(0.5, 2.0))
+    g = torch.Generator(device="cpu")
+    g.manual_seed(args.seed)
+    rows = torch.randperm(n, generator=g)[:k].to("cuda")
+    C0 = X[rows].cpu().numpy()
+    ctx = mcg.Context(0)
+    L = mcg.load()
+    asn = torch.empty(n, dtype=torch.int32, device="cuda")
+    cst = torch.empty(n, dtype=torch.float32, device="cuda")
+    res = {}
+    M = 4
+    for name, msr, Xm, Cm in (("L1", 0, X, C0), ("TVD", 9, torch.exp(X / 8.), np.exp(C0 / 8.)), ("L2", 1, X, C0)):
+        Cm = np.ascontiguousarray(Cm, dtype=np.float32)
+        ds = mcg.Dataset.dense(Xm, ctx)
+
+        def estep():
+            mcg.check(L.mcg_assign_hard(ds.h, msr, 0., mcg.ptr(Cm), k, mcg.PREC_BF16, mcg.ptr(asn), mcg.ptr(cst)))
+
+        estep()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            estep()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        e_ms = 1e3 * float(np.median(ts))
+        ds.lloyd(msr, Cm, eps=0., maxiter=0)   # warm-up (scratch)
+        t0 = time.perf_counter()
+        r0 = ds.lloyd(msr, Cm, eps=0., maxiter=0)
+        t_0 = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        rm = ds.lloyd(msr, Cm, eps=0., maxiter=M)
+        t_m = time.perf_counter() - t0
+        it = rm["numiter"] - r0["numiter"]
+        ms_iter = 1e3 * (t_m - t_0) / it if it > 0 else None
+        tflops = 2. * n * k * d / (e_ms * 1e-3) / 1e12
+        res[name] = {"estep_ms": e_ms, "estep_tflops": tflops, "estep_frac_fp32_peak": tflops / PEAK_FP32_TFLOPS,
+                     "ms_per_iteration": ms_iter,
+                     "mstep_ms": None if ms_iter is None else ms_iter - e_ms,
+                     "iterations": rm["numiter"], "initcost": rm["initcost"], "finalcost": rm["finalcost"],
+                     "estep_kernel": "k_pairs_more (pair by pair)" if msr == 9 else
+                                     f"k_tile_lp<{'true' if msr == 1 else 'false'}>"}
+        log(name, json.dumps(res[name]))
+        ds.close()
+    cpu = None
+    if not args.no_cpu:
+        from oracle import mco
+        ns = min(args.cpu_sample // 8, n)
+        Xs = X[:ns].cpu().numpy()
+        t0 = time.perf_counter()
+        a, c, _ = mco.assign_f64(0, Xs, C0, 0.)
+        mco.median_centers(0, Xs, a, c, C0)
+        t = time.perf_counter() - t0
+        cpu = dict(value=1.0 / (t * n / ns), unit="iters/s", cores=int(os.environ.get("OMP_NUM_THREADS", "0")) or
+                   (os.cpu_count() or 1), kind="port",
+                   sample=f"one L1 E+M iteration of the oracle (FP64 per-pair loop, sorted medians) on {ns} of "
+                          f"the {n} rows (k and d unchanged), {t:.2f} s, scaled by n/{ns}")
+        log(json.dumps(cpu))
+    return {"config": "median", "n": n, "d": d, "k": k, "policies": res, "cpu_baseline": cpu,
+            "note": "E-step FLOP count 2*n*k*d (difference and absolute-value add, or FMA, per pair and "
+                    "dimension) against the FP32 vector peak; mstep_ms = ms_per_iteration - estep_ms"}
+
+
 def cpu_baseline(X_host, C_host, n_full, nthreads):
     from oracle import mco
     ns = X_host.shape[0]
@@ -310,9 +385,10 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-workers", type=int, default=4)
-    ap.add_argument("--config", choices=["c3", "c2", "c4", "minibatch"], default="c3",
+    ap.add_argument("--config", choices=["c3", "c2", "c4", "minibatch", "median"], default="c3",
                     help="c3: the headline dense Lloyd workload; c2/c4: auxiliary sparse measurements; "
-                         "minibatch: the minibatch driver on the headline rows")
+                         "minibatch: the minibatch driver on the headline rows; median: the L1 / TVD / L2 "
+                         "median centroid policies on the headline rows")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path would shard rows "
@@ -326,6 +402,9 @@ def main():
 
     if args.config == "minibatch":
         print(json.dumps(minibatch_config(args, mcg)), flush=True)
+        return
+    if args.config == "median":
+        print(json.dumps(median_config(args, mcg)), flush=True)
         return
     if args.config != "c3":
         print(json.dumps(sparse_config(args, mcg, args.config)), flush=True)

@@ -539,7 +539,7 @@ __global__ __launch_bounds__(256) void k_pairs_more(int msr, const float *__rest
             if (val) { x = (p < pe && idx[p] == j) ? val[p++] : 0.f; }
             else x = X[row * d + j];
             const float cj = cr[j];
-            const float a = cj * lhrsi + lhinc, b = x * rhrsi + rhinc;
+            const float a = fmaf(cj, lhrsi, lhinc), b = fmaf(x, rhrsi, rhinc);
             switch (msr) {
             case MCG_L1: s += fabsf(cj - x); break;
             case MCG_L2: { const float t = cj - x; s = fmaf(t, t, s); } break;
@@ -600,6 +600,258 @@ __global__ __launch_bounds__(256) void k_pairs_more(int msr, const float *__rest
             if (take) { best = ob; bi = oi; }
         }
         if (lane == 0) { asn[row] = bi; if (best_out) best_out[row] = best; }
+    }
+}
+
+// Dense L1 / L2 costs by 64-row x 64-center tiles (the median policies' E-step):
+// 256 threads, thread (tr, tc) owns rows r0 + 4tr + {0..3} and centers c0 + 4tc +
+// {0..3}; 32-dim chunks of both staged in LDS dim-major, read as float4.  Each
+// pair's sum runs over j in order, as in k_pairs_more, so the values are the same;
+// VALU-bound at 2 operations per pair and dimension.  Argmin: a thread scans its
+// centers in increasing order (strict <), then the 16 threads of a row combine
+// (value, index) with the lower index on ties.
+template <bool L2>
+__global__ __launch_bounds__(256) void k_tile_lp(const float *__restrict__ X, uint64_t n, uint32_t d,
+                                                 const float *__restrict__ C, uint32_t k, float *__restrict__ out,
+                                                 uint32_t *__restrict__ asn, float *__restrict__ best_out) {
+    constexpr uint32_t TJ = 32, TR = 64, LD = TR + 4;
+    __shared__ float xs[TJ * LD], cs[TJ * LD];
+    __shared__ float bvs[16 * TR];
+    __shared__ uint32_t bis[16 * TR];
+    const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+    const uint64_t r0 = (uint64_t)blockIdx.x * TR;
+    float best[4];
+    uint32_t bidx[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) { best[a] = 0.f; bidx[a] = 0xFFFFFFFFu; }
+    const uint32_t sr = tid >> 2, sj = (tid & 3) * 8;   // staging: row / center sr, dims sj..sj+7
+    const uint64_t xrow = min<uint64_t>(r0 + sr, n - 1);
+    for (uint32_t c0 = 0; c0 < k; c0 += TR) {
+        const uint32_t crow = min<uint32_t>(c0 + sr, k - 1);
+        float acc[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = 0.f;
+        for (uint32_t j0 = 0; j0 < d; j0 += TJ) {
+#pragma unroll
+            for (uint32_t q = 0; q < 8; ++q) {
+                const uint32_t j = j0 + sj + q;
+                xs[(sj + q) * LD + sr] = j < d ? X[xrow * d + j] : 0.f;
+                cs[(sj + q) * LD + sr] = j < d ? C[(uint64_t)crow * d + j] : 0.f;
+            }
+            __syncthreads();
+            const uint32_t jn = min(TJ, d - j0);
+            for (uint32_t j = 0; j < jn; ++j) {
+                const float4 xv = *(const float4 *)&xs[j * LD + tr * 4];
+                const float4 cv = *(const float4 *)&cs[j * LD + tc * 4];
+                const float xa[4] = {xv.x, xv.y, xv.z, xv.w}, cb[4] = {cv.x, cv.y, cv.z, cv.w};
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        if constexpr (L2) {
+                            const float t = cb[b] - xa[a];
+                            acc[a][b] = fmaf(t, t, acc[a][b]);
+                        } else {
+                            acc[a][b] += fabsf(cb[b] - xa[a]);
+                        }
+                    }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const uint64_t row = r0 + tr * 4 + a;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const uint32_t c = c0 + tc * 4 + b;
+                if (c >= k || row >= n) continue;
+                const float v = L2 ? sqrtf(acc[a][b]) : acc[a][b];
+                if (out) out[row * k + c] = v;
+                if (bidx[a] == 0xFFFFFFFFu || v < best[a]) { best[a] = v; bidx[a] = c; }
+            }
+        }
+    }
+    if (!asn) return;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        bvs[tc * TR + tr * 4 + a] = best[a];
+        bis[tc * TR + tr * 4 + a] = bidx[a];
+    }
+    __syncthreads();
+    if (tid < (int)TR) {
+        float b = bvs[tid];
+        uint32_t ix = bis[tid];
+        for (int t = 1; t < 16; ++t) {
+            const float ob = bvs[t * TR + tid];
+            const uint32_t oi = bis[t * TR + tid];
+            if (oi != 0xFFFFFFFFu && (ix == 0xFFFFFFFFu || ob < b || (ob == b && oi < ix))) { b = ob; ix = oi; }
+        }
+        const uint64_t row = r0 + tid;
+        if (row < n) {
+            asn[row] = ix;
+            if (best_out) best_out[row] = b;
+        }
+    }
+}
+
+// Dense L1 / L2 costs against one center (the empty-cluster restarts): one thread
+// per row, 32-dim slices of the block's 256 rows staged in LDS with coalesced loads;
+// the per-pair order of k_tile_lp, so the values are the same.
+template <bool L2>
+__global__ __launch_bounds__(256) void k_lp_single(const float *__restrict__ X, uint64_t n, uint32_t d,
+                                                   const float *__restrict__ C, float *__restrict__ out,
+                                                   uint32_t *__restrict__ asn, float *__restrict__ best_out) {
+    __shared__ float xs[256][33];
+    __shared__ float cs[32];
+    const int tid = threadIdx.x;
+    const uint64_t p0 = (uint64_t)blockIdx.x * 256, p = p0 + tid;
+    float acc = 0.f;
+    for (uint32_t j0 = 0; j0 < d; j0 += 32) {
+        __syncthreads();
+        for (int q = tid; q < 256 * 32; q += 256) {
+            const int r = q >> 5, jj = q & 31;
+            const uint64_t pr = p0 + r;
+            xs[r][jj] = (pr < n && j0 + jj < d) ? X[pr * d + j0 + jj] : 0.f;
+        }
+        if (tid < 32) cs[tid] = j0 + tid < d ? C[j0 + tid] : 0.f;
+        __syncthreads();
+        const int jn = (int)min<uint32_t>(32, d - j0);
+        for (int jj = 0; jj < jn; ++jj) {
+            if constexpr (L2) {
+                const float t = cs[jj] - xs[tid][jj];
+                acc = fmaf(t, t, acc);
+            } else {
+                acc += fabsf(cs[jj] - xs[tid][jj]);
+            }
+        }
+    }
+    if (p >= n) return;
+    const float v = L2 ? sqrtf(acc) : acc;
+    if (out) out[p] = v;
+    if (asn) asn[p] = 0u;
+    if (best_out) best_out[p] = v;
+}
+
+// The smoothing of one (row, center) pair as k_pairs_more computes it
+// (applicator.h:1642-1656 in FP32): a = c * lhrsi + lhinc, b = x * rhrsi + rhinc.
+struct PairSmooth {
+    float lhrsi, lhinc, rhrsi, rhinc;
+};
+__device__ __forceinline__ PairSmooth pair_smooth(float cs, float xs, float prior, uint32_t d) {
+    const float psum0 = prior * (float)d;
+    float lhsum = cs + psum0, rhsum = xs + psum0;
+    const float pv = fmaxf(prior, 7.0069e-42f * (lhsum + rhsum));
+    const float psum = pv * (float)d;
+    lhsum = cs + psum;
+    rhsum = xs + psum;
+    PairSmooth r;
+    r.lhrsi = 1.f / lhsum;
+    r.rhrsi = 1.f / rhsum;
+    r.lhinc = (pv != 0.f && lhsum != 0.f) ? pv * r.lhrsi : 0.f;
+    r.rhinc = (pv != 0.f && rhsum != 0.f) ? pv * r.rhrsi : 0.f;
+    if (r.lhinc != r.lhinc) r.lhinc = 0.f;
+    if (r.rhinc != r.rhinc) r.rhinc = 0.f;
+    return r;
+}
+
+// Dense TVD costs by 64 x 64 tiles (the layout of k_tile_lp): the smoothing of each
+// of a thread's 16 pairs is set up per center tile; per dimension two FMAs, a
+// difference and an absolute-value add, in k_pairs_more's order and form.
+__global__ __launch_bounds__(256) void k_tile_tvd(const float *__restrict__ X, uint64_t n, uint32_t d,
+                                                  const double *__restrict__ rsum, const float *__restrict__ C,
+                                                  uint32_t k, const double *__restrict__ csum, float prior,
+                                                  float *__restrict__ out, uint32_t *__restrict__ asn,
+                                                  float *__restrict__ best_out) {
+    constexpr uint32_t TJ = 32, TR = 64, LD = TR + 4;
+    __shared__ float xs[TJ * LD], cs[TJ * LD];
+    __shared__ float bvs[16 * TR];
+    __shared__ uint32_t bis[16 * TR];
+    const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+    const uint64_t r0 = (uint64_t)blockIdx.x * TR;
+    float best[4];
+    uint32_t bidx[4];
+    float xsum[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        best[a] = 0.f;
+        bidx[a] = 0xFFFFFFFFu;
+        xsum[a] = (float)rsum[min<uint64_t>(r0 + tr * 4 + a, n - 1)];
+    }
+    const uint32_t sr = tid >> 2, sj = (tid & 3) * 8;
+    const uint64_t xrow = min<uint64_t>(r0 + sr, n - 1);
+    for (uint32_t c0 = 0; c0 < k; c0 += TR) {
+        const uint32_t crow = min<uint32_t>(c0 + sr, k - 1);
+        PairSmooth sm[4][4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const float csb = (float)csum[min<uint32_t>(c0 + tc * 4 + b, k - 1)];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) sm[a][b] = pair_smooth(csb, xsum[a], prior, d);
+        }
+        float acc[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = 0.f;
+        for (uint32_t j0 = 0; j0 < d; j0 += TJ) {
+#pragma unroll
+            for (uint32_t q = 0; q < 8; ++q) {
+                const uint32_t j = j0 + sj + q;
+                xs[(sj + q) * LD + sr] = j < d ? X[xrow * d + j] : 0.f;
+                cs[(sj + q) * LD + sr] = j < d ? C[(uint64_t)crow * d + j] : 0.f;
+            }
+            __syncthreads();
+            const uint32_t jn = min(TJ, d - j0);
+            for (uint32_t j = 0; j < jn; ++j) {
+                const float4 xv = *(const float4 *)&xs[j * LD + tr * 4];
+                const float4 cv = *(const float4 *)&cs[j * LD + tc * 4];
+                const float xa[4] = {xv.x, xv.y, xv.z, xv.w}, cb[4] = {cv.x, cv.y, cv.z, cv.w};
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        const float av = fmaf(cb[b], sm[a][b].lhrsi, sm[a][b].lhinc);
+                        const float bv = fmaf(xa[a], sm[a][b].rhrsi, sm[a][b].rhinc);
+                        acc[a][b] += fabsf(av - bv);
+                    }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const uint64_t row = r0 + tr * 4 + a;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const uint32_t c = c0 + tc * 4 + b;
+                if (c >= k || row >= n) continue;
+                const float v = 0.5f * acc[a][b];
+                if (out) out[row * k + c] = v;
+                if (bidx[a] == 0xFFFFFFFFu || v < best[a]) { best[a] = v; bidx[a] = c; }
+            }
+        }
+    }
+    if (!asn) return;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        bvs[tc * TR + tr * 4 + a] = best[a];
+        bis[tc * TR + tr * 4 + a] = bidx[a];
+    }
+    __syncthreads();
+    if (tid < (int)TR) {
+        float b = bvs[tid];
+        uint32_t ix = bis[tid];
+        for (int t = 1; t < 16; ++t) {
+            const float ob = bvs[t * TR + tid];
+            const uint32_t oi = bis[t * TR + tid];
+            if (oi != 0xFFFFFFFFu && (ix == 0xFFFFFFFFu || ob < b || (ob == b && oi < ix))) { b = ob; ix = oi; }
+        }
+        const uint64_t row = r0 + tid;
+        if (row < n) {
+            asn[row] = ix;
+            if (best_out) best_out[row] = b;
+        }
     }
 }
 
@@ -735,10 +987,33 @@ void generic_costs(Data &ds, int msr, double prior, const float *C, uint64_t k, 
     const uint32_t *idx = ds.csr ? ds.idx.as<const uint32_t>() : nullptr;
     const uint64_t *ptr = ds.csr ? ds.ptr.as<const uint64_t>() : nullptr;
     REQUIRE(k < (1ull << 31), "k too large");
+    if (!ds.csr && (msr == MCG_L1 || msr == MCG_L2) && k == 1) {
+        if (msr == MCG_L2)
+            hipLaunchKernelGGL(k_lp_single<true>, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, X, n, d, C, out_nk,
+                               asn, best);
+        else
+            hipLaunchKernelGGL(k_lp_single<false>, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, X, n, d, C, out_nk,
+                               asn, best);
+        return;
+    }
+    if (!ds.csr && (msr == MCG_L1 || msr == MCG_L2)) {
+        if (msr == MCG_L2)
+            hipLaunchKernelGGL(k_tile_lp<true>, dim3((uint32_t)cdiv(n, 64)), dim3(256), 0, st, X, n, d, C, (uint32_t)k,
+                               out_nk, asn, best);
+        else
+            hipLaunchKernelGGL(k_tile_lp<false>, dim3((uint32_t)cdiv(n, 64)), dim3(256), 0, st, X, n, d, C,
+                               (uint32_t)k, out_nk, asn, best);
+        return;
+    }
     if (!msr_decomposed(msr)) {
         rowsums(ds, st);
         double *csum = (double *)c.s[S_CPARAM].reserve(k * sizeof(double));
         hipLaunchKernelGGL(k_center_sums, dim3((uint32_t)cdiv(k, 4)), dim3(256), 0, st, C, (uint32_t)k, d, csum);
+        if (!ds.csr && msr == MCG_TVD) {
+            hipLaunchKernelGGL(k_tile_tvd, dim3((uint32_t)cdiv(n, 64)), dim3(256), 0, st, X, n, d,
+                               ds.rowsum.as<const double>(), C, (uint32_t)k, csum, (float)prior, out_nk, asn, best);
+            return;
+        }
         hipLaunchKernelGGL(k_pairs_more, dim3((uint32_t)cdiv(n, 4)), dim3(256), 0, st, msr, X, val, idx, ptr, n, d,
                            ds.rowsum.as<const double>(), C, (uint32_t)k, csum, (float)prior, out_nk, asn, best);
         return;

@@ -114,7 +114,7 @@ struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     // named scratch buffers
-    DBuf s[48];
+    DBuf s[56];
     DBuf pinned_dummy;
     void *host_pinned = nullptr;
     size_t host_pinned_bytes = 0;
@@ -144,9 +144,9 @@ enum Slot {
     S_HIST, S_BASE, S_PERM, S_CNT, S_ITEMOFF, S_PART, S_PARTW, S_PARTQ,
     S_CCOST, S_TOTAL, S_CENT_A, S_CENT_B, S_CENT_C, S_COSTS, S_CT, S_CPARAM,
     S_MISC0, S_MISC1, S_MISC2, S_MISC3, S_W, S_STAGE0, S_STAGE1, S_STAGE2,
-    S_LIS, S_TOT, S_JSDG, S_PAIRS, S_THRESH, S_OVER, S_KEYS, S_OFLAG, S_CINFO, S_CENT_D, S_LCTL, S_PREP, S_ROWLEN, S_MED0, S_MED1, S_MED2, S_NSLOTS
+    S_LIS, S_TOT, S_JSDG, S_PAIRS, S_THRESH, S_OVER, S_KEYS, S_OFLAG, S_CINFO, S_CENT_D, S_LCTL, S_PREP, S_ROWLEN, S_MED0, S_MED1, S_MED2, S_MED3, S_NSLOTS
 };
-static_assert(S_NSLOTS <= 48, "scratch slots");
+static_assert(S_NSLOTS <= 56, "scratch slots");
 
 // ---- kernels (launchers) ----------------------------------------------------
 // E-step parameters for the BF16 near-tie test, written on the device.
@@ -250,6 +250,8 @@ void coreset_sample_from(Ctx &ctx, uint64_t n, const float *probs_d, const doubl
                          uint64_t seed, uint64_t first, uint64_t *idx_d, float *wout_d, hipStream_t st);
 void restart_draws(Ctx &ctx, const float *costs_d, uint64_t n, uint64_t nempty, uint64_t *rows_h,
                    hipStream_t st);
+// Draw j of the wyrand stream of n over device costs (the sampler of restart_draws).
+uint64_t restart_draw_device(Ctx &ctx, const float *costs_d, uint64_t n, uint64_t j, hipStream_t st);
 // Fixed-point inverse-CDF draws over host weights wv for raw draws r[0..m): the
 // weights are converted once, each draw is a search of their prefix sums.
 void draw_fixed_many(const std::vector<double> &wv, const uint64_t *r, uint64_t m, uint64_t *out);

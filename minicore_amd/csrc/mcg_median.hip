@@ -47,13 +47,26 @@ __device__ __forceinline__ double okey_value(uint64_t k) {
 template <bool TVD> struct MedT { using K = uint32_t; using V = float; };
 template <> struct MedT<true> { using K = uint64_t; using V = double; };
 
-// member value: x (L1) or x * (1 / rowsum) in FP64 (TVD, kmedian.h:349)
+// member value at sorted position p (rows gathered in cluster order): x (L1) or
+// x * (1 / rowsum) in FP64 (TVD, kmedian.h:349)
 template <bool TVD>
-__device__ __forceinline__ typename MedT<TVD>::V med_value(const float *__restrict__ X, uint32_t d, uint32_t r,
-                                                           uint32_t j, const double *__restrict__ inv) {
-    const float x = X[(uint64_t)r * d + j];
-    if constexpr (TVD) return __dmul_rn((double)x, inv[r]);
+__device__ __forceinline__ typename MedT<TVD>::V med_value(const float *__restrict__ Xp, uint32_t d, uint32_t p,
+                                                           uint32_t j, const double *__restrict__ invp) {
+    const float x = Xp[(uint64_t)p * d + j];
+    if constexpr (TVD) return __dmul_rn((double)x, invp[p]);
     else return x;
+}
+
+// rows in cluster order: Xp[p] = X[perm[p]], one wave per row
+__global__ __launch_bounds__(256) void k_gather_perm(const float *__restrict__ X, uint32_t d,
+                                                     const uint32_t *__restrict__ perm, uint64_t n,
+                                                     float *__restrict__ Xp) {
+    const int lane = threadIdx.x & 63;
+    for (uint64_t p = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); p < n; p += (uint64_t)gridDim.x * 4) {
+        const float *src = X + (uint64_t)perm[p] * d;
+        float *dst = Xp + p * d;
+        for (uint32_t j = lane; j < d; j += 64) dst[j] = src[j];
+    }
 }
 
 __device__ __forceinline__ void wave_sync() {
@@ -63,29 +76,30 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // Per-column median of the members of cluster blockIdx.y (m >= 2): rank h for odd
-// m; ranks h-1 and h averaged for even m (0.5 * (a + b) in the value type).
-template <bool TVD>
-__global__ __launch_bounds__(256) void k_med_select(const float *__restrict__ X, uint32_t d,
-                                                    const uint32_t *__restrict__ perm,
+// m; ranks h-1 and h averaged for even m (0.5 * (a + b) in the value type).  COLS
+// columns per block (one per thread of a column group), 256 / COLS member groups.
+template <bool TVD, uint32_t COLS>
+__global__ __launch_bounds__(256) void k_med_select(const float *__restrict__ Xp, uint32_t d,
                                                     const uint32_t *__restrict__ counts,
                                                     const uint32_t *__restrict__ off,
-                                                    const double *__restrict__ inv, float *__restrict__ C) {
+                                                    const double *__restrict__ invp, float *__restrict__ C) {
     using K = typename MedT<TVD>::K;
     using V = typename MedT<TVD>::V;
     constexpr int BITS = 8 * (int)sizeof(K);
-    __shared__ uint32_t hist[256 * MED_COLS];
-    __shared__ K pref[MED_COLS], key2[MED_COLS];
-    __shared__ uint32_t rnk[MED_COLS], eqc[MED_COLS], nd2[MED_COLS];
+    __shared__ uint32_t hist[256 * COLS];
+    __shared__ K pref[COLS], key2[COLS];
+    __shared__ uint32_t rnk[COLS], eqc[COLS], nd2[COLS];
     __shared__ int need2;
     const uint32_t c = blockIdx.y;
     const uint32_t m = counts[c];
     if (m < 2) return;
-    const int tid = threadIdx.x, lane = tid & 63, g = tid >> 6;
-    const uint32_t j = blockIdx.x * MED_COLS + lane;
+    constexpr uint32_t GROUPS = 256 / COLS;
+    const int tid = threadIdx.x, lane = tid % (int)COLS, g = tid / (int)COLS;
+    const uint32_t j = blockIdx.x * COLS + lane;
     const bool valid = j < d;
-    const uint32_t *mem = perm + off[c];
+    const uint32_t b0 = off[c];
     const uint32_t h = m >> 1;
-    if (tid < (int)MED_COLS) {
+    if (tid < (int)COLS) {
         pref[tid] = 0;
         rnk[tid] = (m & 1) ? h : h - 1;
         nd2[tid] = 0;
@@ -93,22 +107,32 @@ __global__ __launch_bounds__(256) void k_med_select(const float *__restrict__ X,
     if (tid == 0) need2 = 0;
     for (int p = 0; p < BITS / 8; ++p) {
         const int shift = BITS - 8 * (p + 1);
-        for (uint32_t q = tid; q < 256 * MED_COLS; q += 256) hist[q] = 0;
+        for (uint32_t q = tid; q < 256 * COLS; q += 256) hist[q] = 0;
         __syncthreads();
         if (valid) {
             const K pr = pref[lane];
             const K hi = p == 0 ? (K)0 : (K)(~(K)0 << (shift + 8));
-            for (uint32_t q = g; q < m; q += MED_GROUPS) {
-                const K key = okey(med_value<TVD>(X, d, mem[q], j, inv));
-                if ((key & hi) == pr) atomicAdd(&hist[(uint32_t)((key >> shift) & 255u) * MED_COLS + lane], 1u);
+            // 8 members per step: their loads are independent and in flight together
+            constexpr uint32_t U = 8;
+            for (uint32_t q0 = g; q0 < m; q0 += U * GROUPS) {
+                K key[U];
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u) {
+                    const uint32_t q = q0 + u * GROUPS;
+                    key[u] = q < m ? okey(med_value<TVD>(Xp, d, b0 + q, j, invp)) : (K)0;
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u)
+                    if (q0 + u * GROUPS < m && (key[u] & hi) == pr)
+                        atomicAdd(&hist[(uint32_t)((key[u] >> shift) & 255u) * COLS + lane], 1u);
             }
         }
         __syncthreads();
-        if (tid < (int)MED_COLS && valid) {
+        if (tid < (int)COLS && valid) {
             const uint32_t r = rnk[tid];
             uint32_t cum = 0;
             for (uint32_t b = 0; b < 256; ++b) {
-                const uint32_t hb = hist[b * MED_COLS + tid];
+                const uint32_t hb = hist[b * COLS + tid];
                 if (cum + hb > r) {
                     pref[tid] |= (K)b << shift;
                     rnk[tid] = r - cum;
@@ -122,7 +146,7 @@ __global__ __launch_bounds__(256) void k_med_select(const float *__restrict__ X,
     }
     if (!(m & 1)) {
         // rank h: the same key while equal keys remain, else the smallest larger key
-        if (tid < (int)MED_COLS && valid) {
+        if (tid < (int)COLS && valid) {
             if (rnk[tid] + 1 < eqc[tid]) key2[tid] = pref[tid];
             else { key2[tid] = ~(K)0; nd2[tid] = 1; need2 = 1; }
         }
@@ -131,8 +155,8 @@ __global__ __launch_bounds__(256) void k_med_select(const float *__restrict__ X,
             if (valid && nd2[lane]) {
                 const K k1 = pref[lane];
                 K best = ~(K)0;
-                for (uint32_t q = g; q < m; q += MED_GROUPS) {
-                    const K key = okey(med_value<TVD>(X, d, mem[q], j, inv));
+                for (uint32_t q = g; q < m; q += GROUPS) {
+                    const K key = okey(med_value<TVD>(Xp, d, b0 + q, j, invp));
                     if (key > k1 && key < best) best = key;
                 }
                 atomicMin(&key2[lane], best);
@@ -140,7 +164,7 @@ __global__ __launch_bounds__(256) void k_med_select(const float *__restrict__ X,
             __syncthreads();
         }
     }
-    if (tid < (int)MED_COLS && valid) {
+    if (tid < (int)COLS && valid) {
         const V v1 = okey_value(pref[tid]);
         V out = v1;
         if (!(m & 1)) out = (V)0.5 * (v1 + okey_value(key2[tid]));
@@ -151,11 +175,11 @@ __global__ __launch_bounds__(256) void k_med_select(const float *__restrict__ X,
 // Weighted: per column the smallest value among the members of largest weight
 // (L1: w[row]; TVD: w[rank], the reference's indexing).
 template <bool TVD>
-__global__ __launch_bounds__(256) void k_med_heaviest(const float *__restrict__ X, uint32_t d,
+__global__ __launch_bounds__(256) void k_med_heaviest(const float *__restrict__ Xp, uint32_t d,
                                                       const uint32_t *__restrict__ perm,
                                                       const uint32_t *__restrict__ counts,
                                                       const uint32_t *__restrict__ off,
-                                                      const double *__restrict__ inv, const double *__restrict__ w,
+                                                      const double *__restrict__ invp, const double *__restrict__ w,
                                                       float *__restrict__ C) {
     using K = typename MedT<TVD>::K;
     __shared__ double wl[256];
@@ -181,7 +205,7 @@ __global__ __launch_bounds__(256) void k_med_heaviest(const float *__restrict__ 
         for (uint32_t q = g; q < m; q += MED_GROUPS) {
             const double wq = TVD ? w[q] : w[mem[q]];
             if (wq == wmax) {
-                const K key = okey(med_value<TVD>(X, d, mem[q], j, inv));
+                const K key = okey(med_value<TVD>(Xp, d, off[c] + q, j, invp));
                 b = key < b ? key : b;
             }
         }
@@ -215,14 +239,26 @@ __global__ void k_med_single_csr(const float *__restrict__ val, const uint32_t *
     for (uint64_t p = ptr[r] + threadIdx.x; p < ptr[r + 1]; p += blockDim.x) ctr[idx[p]] = val[p];
 }
 
-// 1 / (row sum), the sum sequential in FP64 (one thread per row)
-__global__ void k_inv_rowsum_seq(const float *__restrict__ X, uint64_t n, uint32_t d, double *__restrict__ inv) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float *x = X + i * d;
+// 1 / (row sum), the sum sequential in FP64: one thread per row, 32-dim slices of
+// the block's 256 rows staged in LDS with coalesced loads
+__global__ __launch_bounds__(256) void k_inv_rowsum_seq(const float *__restrict__ X, uint64_t n, uint32_t d,
+                                                        double *__restrict__ inv) {
+    __shared__ float xs[256][33];
+    const int tid = threadIdx.x;
+    const uint64_t p0 = (uint64_t)blockIdx.x * 256;
     double s = 0.;
-    for (uint32_t j = 0; j < d; ++j) s = __dadd_rn(s, (double)x[j]);
-    inv[i] = __ddiv_rn(1., s);
+    for (uint32_t j0 = 0; j0 < d; j0 += 32) {
+        __syncthreads();
+        for (int q = tid; q < 256 * 32; q += 256) {
+            const int r = q >> 5, jj = q & 31;
+            const uint64_t pr = p0 + r;
+            xs[r][jj] = (pr < n && j0 + jj < d) ? X[pr * d + j0 + jj] : 0.f;
+        }
+        __syncthreads();
+        const int jn = (int)min<uint32_t>(32, d - j0);
+        for (int jj = 0; jj < jn; ++jj) s = __dadd_rn(s, (double)xs[tid][jj]);
+    }
+    if (p0 + tid < n) inv[p0 + tid] = __ddiv_rn(1., s);
 }
 
 __global__ void k_inv_rowsum_csr_seq(const float *__restrict__ val, const uint64_t *__restrict__ ptr, uint64_t n,
@@ -370,21 +406,37 @@ __global__ void k_geo_init(const uint32_t *__restrict__ counts, uint64_t k, floa
 
 // per member (sorted position p): distance to its cluster's current center, FP64
 // sequential FMA, rounded to float, floored; tq = w * dist (or dist)
-__global__ void k_geo_dist(const float *__restrict__ X, uint32_t d, const uint32_t *__restrict__ perm, uint64_t n,
-                           const uint32_t *__restrict__ A, const float *__restrict__ C,
-                           const uint32_t *__restrict__ active, const double *__restrict__ w, double floor_,
-                           double *__restrict__ cq, double *__restrict__ tq) {
-    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const uint32_t r = perm[p];
-    const uint32_t c = A[r];
-    if (!active[c]) return;
-    const float *x = X + (uint64_t)r * d, *cc = C + (uint64_t)c * d;
+__global__ __launch_bounds__(256) void k_geo_dist(const float *__restrict__ Xp, uint32_t d,
+                                                  const uint32_t *__restrict__ perm, uint64_t n,
+                                                  const uint32_t *__restrict__ A, const float *__restrict__ C,
+                                                  const uint32_t *__restrict__ active, const double *__restrict__ w,
+                                                  double floor_, double *__restrict__ cq, double *__restrict__ tq) {
+    __shared__ float xs[256][33];
+    const int tid = threadIdx.x;
+    const uint64_t p0 = (uint64_t)blockIdx.x * 256, p = p0 + tid;
+    const uint32_t r = p < n ? perm[p] : 0u;
+    const uint32_t c = p < n ? A[r] : 0u;
+    const bool on = p < n && active[c];
+    const float *cc = C + (uint64_t)c * d;
     double s = 0.;
-    for (uint32_t j = 0; j < d; ++j) {
-        const double t = __dsub_rn((double)x[j], (double)cc[j]);
-        s = __fma_rn(t, t, s);
+    // 32-dim slices of the block's 256 gathered rows, staged with coalesced loads
+    for (uint32_t j0 = 0; j0 < d; j0 += 32) {
+        __syncthreads();
+        for (int q = tid; q < 256 * 32; q += 256) {
+            const int rr = q >> 5, jj = q & 31;
+            const uint64_t pr = p0 + rr;
+            xs[rr][jj] = (pr < n && j0 + jj < d) ? Xp[pr * d + j0 + jj] : 0.f;
+        }
+        __syncthreads();
+        if (on) {
+            const int jn = (int)min<uint32_t>(32, d - j0);
+            for (int jj = 0; jj < jn; ++jj) {
+                const double t = __dsub_rn((double)xs[tid][jj], (double)cc[j0 + jj]);
+                s = __fma_rn(t, t, s);
+            }
+        }
     }
+    if (!on) return;
     double v = (double)(float)__dsqrt_rn(s);
     if (v != v) v = floor_;
     v = v > floor_ ? v : floor_;
@@ -409,7 +461,7 @@ __device__ double block_tree_sum(double s, double *sm) {
 __global__ __launch_bounds__(256) void k_geo_sums(const uint32_t *__restrict__ perm,
                                                   const uint32_t *__restrict__ counts,
                                                   const uint32_t *__restrict__ off, const double *__restrict__ cq,
-                                                  const double *__restrict__ tq, const double *__restrict__ w,
+                                                  double *__restrict__ tq, const double *__restrict__ w,
                                                   double eps, float *__restrict__ prev, uint32_t *__restrict__ active,
                                                   double *__restrict__ Q, uint32_t *__restrict__ nanflag) {
     __shared__ double sm[256];
@@ -437,16 +489,21 @@ __global__ __launch_bounds__(256) void k_geo_sums(const uint32_t *__restrict__ p
     }
     const double tot = block_tree_sum(s, sm);
     if (threadIdx.x == 0) Q[c] = tot;
+    // the members' coefficients of the new center, into tq (its costs are summed)
+    const double inv = __ddiv_rn(1., tot);
+    for (uint32_t q = threadIdx.x; q < m; q += 256) {
+        const double cv = cq[b + q];
+        tq[b + q] = w ? __dmul_rn(__ddiv_rn(w[perm[b + q]], cv), inv) : __ddiv_rn(__ddiv_rn(1., cv), tot);
+    }
 }
 
 // new center of each cluster still active: sum_i coef_i x_i, members split over 4
 // groups (member q -> group q mod 4), groups added ((g0 + g1) + (g2 + g3))
-__global__ __launch_bounds__(256) void k_geo_update(const float *__restrict__ X, uint32_t d,
+__global__ __launch_bounds__(256) void k_geo_update(const float *__restrict__ Xp, uint32_t d,
                                                     const uint32_t *__restrict__ perm,
                                                     const uint32_t *__restrict__ counts,
-                                                    const uint32_t *__restrict__ off, const double *__restrict__ cq,
-                                                    const double *__restrict__ w, const uint32_t *__restrict__ active,
-                                                    const double *__restrict__ Q, float *__restrict__ C) {
+                                                    const uint32_t *__restrict__ off, const double *__restrict__ coef,
+                                                    const uint32_t *__restrict__ active, float *__restrict__ C) {
     __shared__ double part[MED_GROUPS][MED_COLS];
     const uint32_t c = blockIdx.y;
     if (!active[c]) return;
@@ -454,15 +511,9 @@ __global__ __launch_bounds__(256) void k_geo_update(const float *__restrict__ X,
     const uint32_t j = blockIdx.x * MED_COLS + lane;
     const uint32_t m = counts[c], b = off[c];
     double a = 0.;
-    if (j < d) {
-        const double Qc = Q[c], inv = __ddiv_rn(1., Qc);
-        for (uint32_t q = g; q < m; q += MED_GROUPS) {
-            const uint32_t r = perm[b + q];
-            const double cv = cq[b + q];
-            const double coef = w ? __dmul_rn(__ddiv_rn(w[r], cv), inv) : __ddiv_rn(__ddiv_rn(1., cv), Qc);
-            a = __fma_rn(coef, (double)X[(uint64_t)r * d + j], a);
-        }
-    }
+    if (j < d)
+        for (uint32_t q = g; q < m; q += MED_GROUPS)
+            a = __fma_rn(coef[b + q], (double)Xp[(uint64_t)(b + q) * d + j], a);
     part[g][lane] = a;
     __syncthreads();
     if (g == 0 && j < d)
@@ -483,8 +534,8 @@ __global__ void k_count_active(const uint32_t *__restrict__ active, uint64_t k, 
     if (threadIdx.x == 0) *out = sm[0];
 }
 
-static void geo_median(Data &D, const double *w, uint64_t k, const SortOut &so, const uint32_t *A, float *C,
-                       hipStream_t st) {
+static void geo_median(Data &D, const float *Xp, const double *w, uint64_t k, const SortOut &so, const uint32_t *A,
+                       float *C, hipStream_t st) {
     Ctx &c = *D.ctx;
     const uint64_t n = D.n;
     const uint32_t d = (uint32_t)D.d;
@@ -505,12 +556,11 @@ static void geo_median(Data &D, const double *w, uint64_t k, const SortOut &so, 
     while (rounds < GEO_MAX_ROUNDS) {
         const uint64_t nb = std::min<uint64_t>(batch, GEO_MAX_ROUNDS - rounds);
         for (uint64_t t = 0; t < nb; ++t) {
-            hipLaunchKernelGGL(k_geo_dist, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, D.X, d, so.perm, n, A, C,
+            hipLaunchKernelGGL(k_geo_dist, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, Xp, d, so.perm, n, A, C,
                                active, w, floor_, cq, tq);
             hipLaunchKernelGGL(k_geo_sums, dim3((uint32_t)k), dim3(256), 0, st, so.perm, so.counts, so.off, cq, tq, w,
                                0., prev, active, Q, flags);
-            hipLaunchKernelGGL(k_geo_update, gu, dim3(256), 0, st, D.X, d, so.perm, so.counts, so.off, cq, w, active,
-                               Q, C);
+            hipLaunchKernelGGL(k_geo_update, gu, dim3(256), 0, st, Xp, d, so.perm, so.counts, so.off, tq, active, C);
         }
         rounds += nb;
         hipLaunchKernelGGL(k_count_active, dim3(1), dim3(256), 0, st, active, k, flags + 1);
@@ -563,8 +613,6 @@ void median_mstep(Data &D, int msr, const double *w, uint64_t k, uint32_t *Ar, f
     // to it by raw L1 (L2 for the geometric median) moves there; until none is empty
     float *tmp = (float *)c.s[S_MED2].reserve(n * sizeof(float));
     std::vector<uint32_t> hc(k);
-    std::vector<float> hp;
-    std::vector<double> wv;
     uint64_t draw = 0;
     SortOut so;
     for (;;) {
@@ -573,14 +621,8 @@ void median_mstep(Data &D, int msr, const double *w, uint64_t k, uint32_t *Ar, f
         std::vector<uint32_t> empty;
         for (uint64_t q = 0; q < k; ++q) if (!hc[q]) empty.push_back((uint32_t)q);
         if (empty.empty()) break;
-        hp.resize(n);
-        wv.resize(n);
         for (const uint32_t id : empty) {
-            copy_sync(hp.data(), pr, n * 4, hipMemcpyDeviceToHost, st);
-            for (uint64_t i = 0; i < n; ++i) wv[i] = (double)hp[i];
-            const uint64_t raw = wyrand_at(n, draw++);
-            uint64_t row = 0;
-            draw_fixed_many(wv, &raw, 1, &row);
+            const uint64_t row = restart_draw_device(c, pr, n, draw++, st);
             copy_rows_to_centers(D, &row, &id, 1, C, st);
             generic_costs(D, pol == 3 ? MCG_L2 : MCG_L1, 0., C + (uint64_t)id * d, 1, tmp, nullptr, nullptr, st);
             hipLaunchKernelGGL(k_take_lower1, dim3((uint32_t)std::min<uint64_t>(cdiv(n, 256), 4096)), dim3(256), 0,
@@ -593,25 +635,34 @@ void median_mstep(Data &D, int msr, const double *w, uint64_t k, uint32_t *Ar, f
     }
     const dim3 g1((uint32_t)cdiv(d, 256), (uint32_t)k), gm((uint32_t)cdiv(d, MED_COLS), (uint32_t)k);
     hipLaunchKernelGGL(k_med_single, g1, dim3(256), 0, st, D.X, d, so.perm, so.counts, so.off, C);
+    // the rows in cluster order, read contiguously by every pass below
+    float *Xp = (float *)c.s[S_MED3].reserve(n * d * sizeof(float));
+    hipLaunchKernelGGL(k_gather_perm, dim3((uint32_t)std::min<uint64_t>(cdiv(n, 4), 65536)), dim3(256), 0, st, D.X,
+                       d, so.perm, n, Xp);
     if (pol == 3) {
-        geo_median(D, w, k, so, Ar, C, st);
+        geo_median(D, Xp, w, k, so, Ar, C, st);
         return;
     }
-    double *inv = nullptr;
+    double *invp = nullptr;
     if (pol == 2) {
-        inv = (double *)c.s[S_MED0].reserve(n * sizeof(double));
-        hipLaunchKernelGGL(k_inv_rowsum_seq, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, D.X, n, d, inv);
+        invp = (double *)c.s[S_MED0].reserve(n * sizeof(double));
+        hipLaunchKernelGGL(k_inv_rowsum_seq, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, Xp, n, d, invp);
     }
     if (w) {
         if (pol == 2)
-            hipLaunchKernelGGL(k_med_heaviest<true>, gm, dim3(256), 0, st, D.X, d, so.perm, so.counts, so.off, inv, w, C);
+            hipLaunchKernelGGL(k_med_heaviest<true>, gm, dim3(256), 0, st, Xp, d, so.perm, so.counts, so.off, invp, w,
+                               C);
         else
-            hipLaunchKernelGGL(k_med_heaviest<false>, gm, dim3(256), 0, st, D.X, d, so.perm, so.counts, so.off, inv, w, C);
+            hipLaunchKernelGGL(k_med_heaviest<false>, gm, dim3(256), 0, st, Xp, d, so.perm, so.counts, so.off, invp,
+                               w, C);
     } else {
+        // 16 columns and 16 member groups per block: 16 KB of histograms, so many
+        // blocks share a CU and hide the member loads' latency
+        const dim3 g16((uint32_t)cdiv(d, 16), (uint32_t)k);
         if (pol == 2)
-            hipLaunchKernelGGL(k_med_select<true>, gm, dim3(256), 0, st, D.X, d, so.perm, so.counts, so.off, inv, C);
+            hipLaunchKernelGGL((k_med_select<true, 16>), g16, dim3(256), 0, st, Xp, d, so.counts, so.off, invp, C);
         else
-            hipLaunchKernelGGL(k_med_select<false>, gm, dim3(256), 0, st, D.X, d, so.perm, so.counts, so.off, inv, C);
+            hipLaunchKernelGGL((k_med_select<false, 16>), g16, dim3(256), 0, st, Xp, d, so.counts, so.off, invp, C);
     }
 }
 

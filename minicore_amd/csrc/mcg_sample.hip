@@ -883,4 +883,31 @@ void coreset_sample_from(Ctx &c, uint64_t n, const float *probs, const double *w
     HIP_CHECK(hipStreamSynchronize(st));
 }
 
+// One restart draw on the device: draw j of the wyrand stream of n over the costs,
+// the fixed-point sampler of draw_fixed_many (all-zero costs give row 0, as there).
+uint64_t restart_draw_device(Ctx &c, const float *costs_d, uint64_t n, uint64_t j, hipStream_t st) {
+    const uint32_t nblk = (uint32_t)cdiv(n, FX_BLK);
+    char *buf = (char *)c.s[S_MISC0].reserve(64 + nblk * 8 + 16);
+    unsigned long long *mx = (unsigned long long *)buf;
+    uint64_t *bsum = (uint64_t *)(buf + 64);
+    uint64_t *idx = (uint64_t *)(buf + 64 + nblk * 8);
+    float *wout = (float *)(idx + 1);
+    fill_u32((uint32_t *)mx, 0, 2, st);
+    hipLaunchKernelGGL(k_absmax<float>, dim3((uint32_t)std::min<uint64_t>(cdiv(n, 256), 1024)), dim3(256), 0, st,
+                       costs_d, (const double *)nullptr, n, mx);
+    hipLaunchKernelGGL(k_fixed_blocksum<float>, dim3(nblk), dim3(256), 0, st, costs_d, (const double *)nullptr, n,
+                       (const unsigned long long *)mx, bsum);
+    std::vector<uint64_t> h(nblk);
+    copy_sync(h.data(), bsum, nblk * 8, hipMemcpyDeviceToHost, st);
+    uint64_t T = 0;
+    for (uint32_t b = 0; b < nblk; ++b) { T += h[b]; h[b] = T; }
+    if (T == 0) return 0;
+    HIP_CHECK(hipMemcpyAsync(bsum, h.data(), nblk * 8, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_draws, dim3(1), dim3(64), 0, st, costs_d, (const double *)nullptr, n,
+                       (const unsigned long long *)mx, (const uint64_t *)bsum, nblk, T, (uint64_t)1, n, j, idx, wout);
+    uint64_t row = 0;
+    copy_sync(&row, idx, 8, hipMemcpyDeviceToHost, st);
+    return row;
+}
+
 }  // namespace mcg

@@ -145,3 +145,40 @@ def test_hcluster_l1_binding(mcg, mco):
     ref = mco.lloyd_median(L1, X, C0, eps=1e-10, maxiter=20)
     assert np.array_equal(np.asarray(r["asn"]), ref["asn"])
     assert np.array_equal(np.asarray(r["centers"]), ref["centers"])
+
+
+@pytest.mark.parametrize("shape", [(1000, 37, 70), (64, 32, 64), (130, 300, 5), (77, 1, 1), (700, 45, 1)])
+def test_tiled_kernels_equal_pair_kernel(mcg, ctx, shape):
+    """Dense L1 / L2 / TVD costs come from the tiled kernels, CSR ones from the
+    pair-by-pair kernel; both sum each pair over the dimensions in order with the same
+    smoothing, so a dense matrix and its CSR form give identical costs and assignments
+    (ragged n, k and d)."""
+    import scipy.sparse as sp
+    n, d, k = shape
+    say(f"tiled vs pair n={n} d={d} k={k}")
+    rng = np.random.default_rng(n + d + k)
+    X = rng.normal(size=(n, d)).astype(np.float32)
+    X[rng.random((n, d)) < 0.3] = 0.
+    C = rng.normal(size=(k, d)).astype(np.float32)
+    S = sp.csr_matrix(X)
+    dd = mcg.Dataset.dense(X, ctx)
+    dc = mcg.Dataset.csr(S.data.astype(np.float32), S.indices.astype(np.uint32), S.indptr.astype(np.uint64),
+                         (n, d), ctx)
+    A = np.abs(X)
+    SA = sp.csr_matrix(A)
+    dp = mcg.Dataset.dense(A, ctx)
+    dq = mcg.Dataset.csr(SA.data.astype(np.float32), SA.indices.astype(np.uint32), SA.indptr.astype(np.uint64),
+                         (n, d), ctx)
+    Cp = np.abs(C) + 0.1       # TVD with positive centers
+    for msr in (L1, L2, TVD):
+        Cm = Cp if msr == TVD else C
+        a = (dd if msr != TVD else dp).cost_matrix(msr, Cm, prior=0.01 if msr == TVD else 0.)
+        b = (dc if msr != TVD else dq).cost_matrix(msr, Cm, prior=0.01 if msr == TVD else 0.)
+        assert np.array_equal(a, b), msr
+        pr = 0.01 if msr == TVD else 0.
+        ad, cd = (dd if msr != TVD else dp).assign(msr, Cm, prior=pr, with_costs=True)
+        ac, cc = (dc if msr != TVD else dq).assign(msr, Cm, prior=pr, with_costs=True)
+        assert np.array_equal(ad, ac) and np.array_equal(cd, cc)
+        assert np.array_equal(ad, a.argmin(1))
+    for x in (dd, dc, dp, dq):
+        x.close()

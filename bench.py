@@ -284,7 +284,7 @@ def minibatch_config(args, mcg):
 def median_config(args, mcg):
     """Auxiliary measurement of the median centroid policies (hcluster with msr L1, TVD
     or L2; centroid.h:434-647) on the headline rows: per policy the E-step alone
-    (tiled L1 / L2 kernel, pair-by-pair TVD) and the Lloyd iteration (E-step + median
+    (tiled L1 / L2 / TVD kernels) and the Lloyd iteration (E-step + median
     M-step) from the difference of runs with maxiter 0 and maxiter 4 (eps 0).  TVD
     runs on exp(x / 8) of the same rows (it needs positive values)."""
     import numpy as np
@@ -333,8 +333,7 @@ def median_config(args, mcg):
                      "ms_per_iteration": ms_iter,
                      "mstep_ms": None if ms_iter is None else ms_iter - e_ms,
                      "iterations": rm["numiter"], "initcost": rm["initcost"], "finalcost": rm["finalcost"],
-                     "estep_kernel": "k_pairs_more (pair by pair)" if msr == 9 else
-                                     f"k_tile_lp<{'true' if msr == 1 else 'false'}>"}
+                     "estep_kernel": "k_tile_tvd" if msr == 9 else f"k_tile_lp<{'true' if msr == 1 else 'false'}>"}
         log(name, json.dumps(res[name]))
         ds.close()
     cpu = None

@@ -186,12 +186,15 @@ int mcg_lloyd_step(mcg_data *ds, int msr, double prior, float *centers, uint64_t
                    const double *weights, int precision, uint32_t *asn, double *cost);
 /* Minibatch hard clustering: perform_hard_minibatch_clustering
  * (include/minicore/clustering/solve.h:377-578), reached from hcluster(...,
- * mbsize >= 0) (python/pycluster.h:31-42), for the mean-centroid measures.
+ * mbsize >= 0) (python/pycluster.h:31-42).
  * Check-ins (iteration 0, every `freq`-th, the last) assign every row, restart
  * centers with <= reseed_after rows at rows drawn with probability ~ w_i*cost_i,
  * and keep the centers of the lowest check-in cost; in between, mbsize rows are
  * drawn (rng() mod n, with or without replacement), sorted, assigned, and each
- * center with members becomes their weighted mean.  The draws use the wyrand
+ * center with members becomes their weighted mean, or for L1 / TVD / L2 their
+ * median (solve.h:555-572: weighted L1 and TVD index the weight vector by member
+ * rank, so weighted runs need mbsize <= n; the geometric median uses the members'
+ * weights).  The draws use the wyrand
  * counter stream of `seed` in the reference's order (seed 0 is the stream of 0,
  * not std::rand()).  centers (k x d) in: initial, out: best; asn/costs (nullable):
  * the last check-in's assignment and per-row costs; initcost/finalcost: the first

@@ -522,6 +522,26 @@ static void check_mean_msr(int msr) {
                                     "on the GPU path; its costs are (mcg_cost_matrix, mcg_assign_hard)");
 }
 
+// The hard drivers' measures: a mean policy, or a median policy with what the
+// reference supports for it (no weighted or geometric median on CSR rows, and
+// non-negative CSR values for the CSR median rules).
+static void check_hard_msr(Data &D, int msr, const double *weights) {
+    check_msr(msr);
+    if (!median_policy(msr)) {
+        check_mean_msr(msr);
+        return;
+    }
+    if (!D.csr) return;
+    if (weights)
+        MCG_THROW(MCG_EUNSUPPORTED, msr == MCG_TVD ? "Not implemented: weighted TVD median for CSparseMatrix"
+                                                   : "Not implemented: weighted L1 median for CSparseMatrix");
+    if (msr == MCG_L2)
+        MCG_THROW(MCG_EUNSUPPORTED, "geometric median of CSR rows: the reference indexes its per-member costs by "
+                                    "row id (csc.h:1170-1177), which has no defined result");
+    HIP_CHECK(hipSetDevice(D.ctx->device));
+    REQUIRE(!csr_has_negative(D, D.ctx->stream), "CSR L1/TVD median centers need non-negative values");
+}
+
 }  // namespace mcg
 
 using namespace mcg;
@@ -1088,13 +1108,16 @@ int mcg_minibatch(mcg_data *ds, int msr, double prior, float *centers, uint64_t 
         REQUIRE(ds && centers && initcost && finalcost && numiter, "null argument");
         REQUIRE(k > 0, "k must be positive");
         REQUIRE(mbsize > 0 && maxiter > 0 && freq > 0, "mbsize, maxiter and the check-in period must be positive");
-        check_mean_msr(msr);
         Data &D = ds->d;
+        check_hard_msr(D, msr, weights);
         Ctx &c = *D.ctx;
         hipStream_t st = c.stream;
         HIP_CHECK(hipSetDevice(c.device));
         const uint64_t n = D.n, d = D.d, kd = k * d;
         REQUIRE(with_replacement || mbsize <= n, "mbsize exceeds the number of rows (draws without replacement)");
+        REQUIRE(!(weights && (msr == MCG_L1 || msr == MCG_TVD)) || mbsize <= n,
+                "weighted L1 / TVD minibatch updates index the weights by member rank (kmedian.h:340): mbsize "
+                "must not exceed the number of rows");
         const bool l2 = dense_l2(D, msr);
         DBuf sw, sa_out, sk_out, bC, bSave, bA, bP, bTmp, bTa, bCnt, bIdx, bWs, bAs, bCs, bCf, bFid;
         const double *W = (const double *)in_dev(c, weights, n * 8, sw);
@@ -1146,7 +1169,8 @@ int mcg_minibatch(mcg_data *ds, int msr, double prior, float *centers, uint64_t 
                 hipLaunchKernelGGL(k_gather_f64, dim3((uint32_t)cdiv(mbsize, 256)), dim3(256), 0, st, W, idx_d, mbsize,
                                    Ws);
             estep(sub, msr, prior, C, k, precision, As, l2 ? nullptr : Cs, st);
-            if (sub.csr) csr_mstep(sub, As, Ws, k, C, st);
+            if (median_policy(msr)) median_update(sub, msr, Ws, W, k, As, C, st);
+            else if (sub.csr) csr_mstep(sub, As, Ws, k, C, st);
             else dense_mstep(sub, As, Ws, k, C, C, false, st);
         }
         Out oa = out_dev(c, asn, n * 4, sa_out);
@@ -1180,7 +1204,7 @@ int mcg_cs_minibatch(mcg_data *ds, int msr, double prior, float *centers, uint64
         REQUIRE(ds && centers && initcost && finalcost && numiter, "null argument");
         REQUIRE(k > 0, "k must be positive");
         REQUIRE(mbsize > 0 && freq > 0, "mbsize and the check-in period must be positive");
-        check_mean_msr(msr);
+        check_hard_msr(ds->d, msr, weights);
         Data &D = ds->d;
         Ctx &c = *D.ctx;
         hipStream_t st = c.stream;

@@ -194,6 +194,10 @@ void scan_exclusive(const uint32_t *in, uint32_t *out, uint64_t L, DBuf &tmp, hi
 inline int median_policy(int msr) { return msr == MCG_L1 ? 1 : msr == MCG_TVD ? 2 : msr == MCG_L2 ? 3 : 0; }
 void median_mstep(Data &ds, int msr, const double *w, uint64_t k, uint32_t *Ar, float *pr, float *C,
                   hipStream_t st);
+// The minibatch update's median centers (solve.h:555-572): wm per-row weights of ds
+// (the geometric median), wr the weight vector indexed by member rank (L1, TVD).
+void median_update(Data &ds, int msr, const double *wm, const double *wr, uint64_t k, const uint32_t *asn,
+                   float *C, hipStream_t st);
 // true if a stored CSR value is negative (the CSR median rules assume non-negative data)
 bool csr_has_negative(Data &ds, hipStream_t st);
 MStepOut csr_mstep(Data &ds, const uint32_t *asn, const double *w, uint64_t k, float *Cnew,

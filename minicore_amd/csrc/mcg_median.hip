@@ -82,7 +82,8 @@ template <bool TVD, uint32_t COLS>
 __global__ __launch_bounds__(256) void k_med_select(const float *__restrict__ Xp, uint32_t d,
                                                     const uint32_t *__restrict__ counts,
                                                     const uint32_t *__restrict__ off,
-                                                    const double *__restrict__ invp, float *__restrict__ C) {
+                                                    const double *__restrict__ invp, uint32_t min_m,
+                                                    float *__restrict__ C) {
     using K = typename MedT<TVD>::K;
     using V = typename MedT<TVD>::V;
     constexpr int BITS = 8 * (int)sizeof(K);
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(256) void k_med_select(const float *__restrict__ Xp
     __shared__ int need2;
     const uint32_t c = blockIdx.y;
     const uint32_t m = counts[c];
-    if (m < 2) return;
+    if (m < min_m) return;
     constexpr uint32_t GROUPS = 256 / COLS;
     const int tid = threadIdx.x, lane = tid % (int)COLS, g = tid / (int)COLS;
     const uint32_t j = blockIdx.x * COLS + lane;
@@ -172,27 +173,29 @@ __global__ __launch_bounds__(256) void k_med_select(const float *__restrict__ Xp
     }
 }
 
-// Weighted: per column the smallest value among the members of largest weight
-// (L1: w[row]; TVD: w[rank], the reference's indexing).
+// Weighted: per column the smallest value among the members of largest weight:
+// wm[row] (set_centroids_l1), or wr[rank] where the reference indexes the weight
+// vector by member rank (TVD; L1 in the minibatch update).
 template <bool TVD>
 __global__ __launch_bounds__(256) void k_med_heaviest(const float *__restrict__ Xp, uint32_t d,
                                                       const uint32_t *__restrict__ perm,
                                                       const uint32_t *__restrict__ counts,
                                                       const uint32_t *__restrict__ off,
-                                                      const double *__restrict__ invp, const double *__restrict__ w,
+                                                      const double *__restrict__ invp, const double *__restrict__ wm,
+                                                      const double *__restrict__ wr, int byrank, uint32_t min_m,
                                                       float *__restrict__ C) {
     using K = typename MedT<TVD>::K;
     __shared__ double wl[256];
     __shared__ K best[MED_COLS];
     const uint32_t c = blockIdx.y;
     const uint32_t m = counts[c];
-    if (m < 2) return;
+    if (m < min_m) return;
     const int tid = threadIdx.x, lane = tid & 63, g = tid >> 6;
     const uint32_t j = blockIdx.x * MED_COLS + lane;
     const uint32_t *mem = perm + off[c];
-    double wm = -DBL_MAX;
-    for (uint32_t q = tid; q < m; q += 256) wm = fmax(wm, TVD ? w[q] : w[mem[q]]);
-    wl[tid] = wm;
+    double wmx = -DBL_MAX;
+    for (uint32_t q = tid; q < m; q += 256) wmx = fmax(wmx, byrank ? wr[q] : wm[mem[q]]);
+    wl[tid] = wmx;
     if (tid < (int)MED_COLS) best[tid] = ~(K)0;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
@@ -203,7 +206,7 @@ __global__ __launch_bounds__(256) void k_med_heaviest(const float *__restrict__ 
     if (j < d) {
         K b = ~(K)0;
         for (uint32_t q = g; q < m; q += MED_GROUPS) {
-            const double wq = TVD ? w[q] : w[mem[q]];
+            const double wq = byrank ? wr[q] : wm[mem[q]];
             if (wq == wmax) {
                 const K key = okey(med_value<TVD>(Xp, d, off[c] + q, j, invp));
                 b = key < b ? key : b;
@@ -364,13 +367,13 @@ __device__ float wave_select(const float *__restrict__ v, uint32_t nel, uint32_t
 __global__ __launch_bounds__(256) void k_seg_median(const float *__restrict__ vals,
                                                     const uint32_t *__restrict__ segoff,
                                                     const uint32_t *__restrict__ counts, uint32_t d, uint64_t nseg,
-                                                    int tvd, float *__restrict__ C) {
+                                                    int tvd, uint32_t min_m, float *__restrict__ C) {
     __shared__ uint32_t hist[4][256];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (uint64_t s = (uint64_t)blockIdx.x * 4 + wv; s < nseg; s += (uint64_t)gridDim.x * 4) {
         const uint32_t c = (uint32_t)(s / d), j = (uint32_t)(s % d);
         const uint32_t m = counts[c];
-        if (m < 2) continue;
+        if (m < min_m) continue;
         const uint32_t b = segoff[s], nel = segoff[s + 1] - b;
         float out = 0.f;
         if (nel) {
@@ -571,8 +574,8 @@ static void geo_median(Data &D, const float *Xp, const double *w, uint64_t k, co
     }
 }
 
-static void csr_median(Data &D, int tvd, uint64_t k, const SortOut &so, const uint32_t *A, float *C,
-                       hipStream_t st) {
+static void csr_median(Data &D, int tvd, uint64_t k, const SortOut &so, const uint32_t *A, bool single_raw,
+                       float *C, hipStream_t st) {
     Ctx &c = *D.ctx;
     const uint64_t n = D.n, nnz = D.nnz;
     const uint32_t d = (uint32_t)D.d;
@@ -594,11 +597,68 @@ static void csr_median(Data &D, int tvd, uint64_t k, const SortOut &so, const ui
     HIP_CHECK(hipMemcpyAsync(cursor, segoff, (nseg + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(k_seg_fill, dim3((uint32_t)cdiv(n, 4)), dim3(256), 0, st, val, idx, ptr, n, d, A, (uint32_t)k,
                        inv, cursor, vals);
-    hipLaunchKernelGGL(k_med_single_csr, dim3((uint32_t)k), dim3(256), 0, st, val, idx, ptr, d, so.perm, so.counts,
-                       so.off, C);
+    if (single_raw)
+        hipLaunchKernelGGL(k_med_single_csr, dim3((uint32_t)k), dim3(256), 0, st, val, idx, ptr, d, so.perm,
+                           so.counts, so.off, C);
     const uint32_t grid = (uint32_t)std::min<uint64_t>(cdiv(nseg, 4), 65536);
-    hipLaunchKernelGGL(k_seg_median, dim3(grid), dim3(256), 0, st, vals, segoff, so.counts, d, nseg, tvd, C);
+    hipLaunchKernelGGL(k_seg_median, dim3(grid), dim3(256), 0, st, vals, segoff, so.counts, d, nseg, tvd,
+                       single_raw ? 2u : 1u, C);
 }
+
+// Centers from a sorted assignment.  wm: per-row weights of D; wr: the vector the
+// reference indexes by member rank (TVD always, L1 when rank_l1).  single_raw: a
+// one-member cluster takes its raw row (set_centroids_*); else the rule decides.
+static void median_centers_core(Data &D, int pol, const double *wm, const double *wr, uint64_t k,
+                                const SortOut &so, const uint32_t *A, bool single_raw, bool rank_l1, float *C,
+                                hipStream_t st) {
+    Ctx &c = *D.ctx;
+    const uint64_t n = D.n;
+    const uint32_t d = (uint32_t)D.d;
+    const uint32_t min_m = single_raw ? 2u : 1u;
+    const double *w = wm;
+    if (D.csr) {
+        csr_median(D, pol == 2, k, so, A, single_raw, C, st);
+        return;
+    }
+    const dim3 g1((uint32_t)cdiv(d, 256), (uint32_t)k), gm((uint32_t)cdiv(d, MED_COLS), (uint32_t)k);
+    // one member: its raw row (set_centroids_*; the geometric median of one row is
+    // that row in both drivers)
+    if (single_raw || pol == 3)
+        hipLaunchKernelGGL(k_med_single, g1, dim3(256), 0, st, D.X, d, so.perm, so.counts, so.off, C);
+    // the rows in cluster order, read contiguously by every pass below
+    float *Xp = (float *)c.s[S_MED3].reserve(n * d * sizeof(float));
+    hipLaunchKernelGGL(k_gather_perm, dim3((uint32_t)std::min<uint64_t>(cdiv(n, 4), 65536)), dim3(256), 0, st, D.X,
+                       d, so.perm, n, Xp);
+    if (pol == 3) {
+        geo_median(D, Xp, wm, k, so, A, C, st);
+        return;
+    }
+    double *invp = nullptr;
+    if (pol == 2) {
+        invp = (double *)c.s[S_MED0].reserve(n * sizeof(double));
+        hipLaunchKernelGGL(k_inv_rowsum_seq, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, Xp, n, d, invp);
+    }
+    if (w) {
+        const int byrank = pol == 2 || rank_l1;
+        if (pol == 2)
+            hipLaunchKernelGGL(k_med_heaviest<true>, gm, dim3(256), 0, st, Xp, d, so.perm, so.counts, so.off, invp, wm,
+                               wr, byrank, min_m, C);
+        else
+            hipLaunchKernelGGL(k_med_heaviest<false>, gm, dim3(256), 0, st, Xp, d, so.perm, so.counts, so.off, invp,
+                               wm, wr, byrank, min_m, C);
+    } else {
+        // 16 columns and 16 member groups per block: 16 KB of histograms, so many
+        // blocks share a CU and hide the member loads' latency
+        const dim3 g16((uint32_t)cdiv(d, 16), (uint32_t)k);
+        if (pol == 2)
+            hipLaunchKernelGGL((k_med_select<true, 16>), g16, dim3(256), 0, st, Xp, d, so.counts, so.off, invp, min_m,
+                               C);
+        else
+            hipLaunchKernelGGL((k_med_select<false, 16>), g16, dim3(256), 0, st, Xp, d, so.counts, so.off, invp, min_m,
+                               C);
+    }
+}
+
 
 void median_mstep(Data &D, int msr, const double *w, uint64_t k, uint32_t *Ar, float *pr, float *C,
                   hipStream_t st) {
@@ -629,41 +689,20 @@ void median_mstep(Data &D, int msr, const double *w, uint64_t k, uint32_t *Ar, f
                                st, tmp, id, n, pr, Ar);
         }
     }
-    if (D.csr) {
-        csr_median(D, pol == 2, k, so, Ar, C, st);
-        return;
-    }
-    const dim3 g1((uint32_t)cdiv(d, 256), (uint32_t)k), gm((uint32_t)cdiv(d, MED_COLS), (uint32_t)k);
-    hipLaunchKernelGGL(k_med_single, g1, dim3(256), 0, st, D.X, d, so.perm, so.counts, so.off, C);
-    // the rows in cluster order, read contiguously by every pass below
-    float *Xp = (float *)c.s[S_MED3].reserve(n * d * sizeof(float));
-    hipLaunchKernelGGL(k_gather_perm, dim3((uint32_t)std::min<uint64_t>(cdiv(n, 4), 65536)), dim3(256), 0, st, D.X,
-                       d, so.perm, n, Xp);
-    if (pol == 3) {
-        geo_median(D, Xp, w, k, so, Ar, C, st);
-        return;
-    }
-    double *invp = nullptr;
-    if (pol == 2) {
-        invp = (double *)c.s[S_MED0].reserve(n * sizeof(double));
-        hipLaunchKernelGGL(k_inv_rowsum_seq, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, Xp, n, d, invp);
-    }
-    if (w) {
-        if (pol == 2)
-            hipLaunchKernelGGL(k_med_heaviest<true>, gm, dim3(256), 0, st, Xp, d, so.perm, so.counts, so.off, invp, w,
-                               C);
-        else
-            hipLaunchKernelGGL(k_med_heaviest<false>, gm, dim3(256), 0, st, Xp, d, so.perm, so.counts, so.off, invp,
-                               w, C);
-    } else {
-        // 16 columns and 16 member groups per block: 16 KB of histograms, so many
-        // blocks share a CU and hide the member loads' latency
-        const dim3 g16((uint32_t)cdiv(d, 16), (uint32_t)k);
-        if (pol == 2)
-            hipLaunchKernelGGL((k_med_select<true, 16>), g16, dim3(256), 0, st, Xp, d, so.counts, so.off, invp, C);
-        else
-            hipLaunchKernelGGL((k_med_select<false, 16>), g16, dim3(256), 0, st, Xp, d, so.counts, so.off, invp, C);
-    }
+    median_centers_core(D, pol, w, w, k, so, Ar, true, false, C, st);
+}
+
+// The minibatch update's centers (solve.h:555-572): no restart (clusters without
+// members keep their center), a one-member cluster by the rule (TVD normalises it),
+// weighted L1 / TVD by the weight vector indexed by member rank (wr, the full
+// vector), the geometric median by the members' weights (wm, per row of D).
+void median_update(Data &D, int msr, const double *wm, const double *wr, uint64_t k, const uint32_t *A, float *C,
+                   hipStream_t st) {
+    const int pol = median_policy(msr);
+    REQUIRE(pol, "internal: not a median policy");
+    REQUIRE(D.n < (1ull << 32), "n must fit in uint32 for median centers");
+    SortOut so = sort_by_cluster(*D.ctx, A, D.n, k, st);
+    median_centers_core(D, pol, wm, wr, k, so, A, false, true, C, st);
 }
 
 // non-negative CSR values (the median rules above assume them); true if any is negative

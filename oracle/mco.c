@@ -600,11 +600,16 @@ static int centroids_with_restart(int msr, const float *X, const float *val, con
 
 /* perform_hard_clustering, solve.h:98-175.  FT is float for float data, so
  * every cost in the control flow is rounded to float (solve.h:120-123). */
+static int median_policy(int msr);
+
 int mco_lloyd(int msr, const float *X, const float *val, const uint32_t *idx,
               const uint64_t *indptr, uint64_t n, uint64_t d, float *C, uint64_t k,
               double prior, const double *w, double eps, uint64_t maxiter,
               uint32_t *asn, double *cost, double *initcost, double *finalcost,
               uint64_t *numiter, int nthreads) {
+    if (median_policy(msr))   /* L1, TVD, L2: the median policies (solve.h:207-215) */
+        return mco_lloyd_median(msr, X, val, idx, indptr, n, d, C, k, prior, w, eps, maxiter, asn, cost, initcost,
+                                finalcost, numiter);
     assign_generic(msr, X, val, idx, indptr, n, d, C, k, prior, asn, cost, nthreads);
     float fcost = (float)total_cost(cost, w, n);
     *initcost = fcost;
@@ -636,7 +641,9 @@ int mco_lloyd(int msr, const float *X, const float *val, const uint32_t *idx,
 /* ======================================================================
  * Minibatch hard clustering: perform_hard_minibatch_clustering,
  * solve.h:377-578, for the mean-centroid measures (set_center,
- * centroid.h:81-178).  One RNG stream, the wyrand counter stream of `seed`,
+ * centroid.h:81-178) and the median policies (solve.h:555-572: l1_median with
+ * the weight vector indexed by member rank, tvd_median likewise, the geometric
+ * median with the members' weights; clusters without members keep their center).  One RNG stream, the wyrand counter stream of `seed`,
  * serves the restart draws and the minibatch draws in the reference's order.
  * A check-in (iteration 0, every freq-th, and the last) assigns every row,
  * restarts each center with <= reseed_after rows (ascending id) at a row
@@ -650,6 +657,10 @@ int mco_lloyd(int msr, const float *X, const float *val, const uint32_t *idx,
  * weighted mean.  The check-in cost is the exact fixed-point sum of
  * w_i*cost_i (the shift of the sampler), independent of summation order.
  * ====================================================================== */
+static void median_core(int pol, const float *X, const float *val, const uint32_t *idx, const uint64_t *indptr,
+                        uint64_t n, uint64_t d, const double *wm, const double *wr, uint64_t k, const uint32_t *asn,
+                        float *C, double eps, int single_raw, int rank_l1);
+
 static double point_cost(int msr, const float *X, const float *val, const uint32_t *idx, const uint64_t *indptr,
                          uint64_t i, uint64_t d, const float *c, double prior) {
     if (msr == MCO_SQRL2 && !val) return (double)mco_sqrl2_f32(X + i * d, c, d);
@@ -683,6 +694,11 @@ int mco_minibatch(int msr, const float *X, const float *val, const uint32_t *idx
                   uint64_t seed, uint32_t *asn, double *cost, double *initcost, double *finalcost,
                   uint64_t *numiter, int nthreads) {
     if (!mbsize || !maxiter || !freq || (!with_rep && mbsize > n)) return -2;
+    {   /* median updates (solve.h:555-572): CSR only unweighted L1 / TVD; weights by rank need mbsize <= n */
+        const int pol = median_policy(msr);
+        if (pol && val && (w || pol == 3)) return -2;
+        if (pol && w && pol != 3 && mbsize > n) return -2;
+    }
     uint64_t j = 0, iter = 0;
     double init = DBL_MAX, best = DBL_MAX;
     float *save = (float *)malloc(k * d * sizeof(float));
@@ -751,12 +767,16 @@ int mco_minibatch(int msr, const float *X, const float *val, const uint32_t *idx
                 memcpy(is + ps[t], idx + indptr[sel[t]], (ps[t + 1] - ps[t]) * sizeof(uint32_t));
             }
             assign_generic(msr, NULL, vs, is, ps, mbsize, d, C, k, prior, sa, sc, nthreads);
-            mco_centroids_csr(vs, is, ps, mbsize, d, sa, sw, k, C);
+            if (median_policy(msr)) median_core(median_policy(msr), NULL, vs, is, ps, mbsize, d, NULL, NULL, k, sa, C,
+                                                0., 0, 1);
+            else mco_centroids_csr(vs, is, ps, mbsize, d, sa, sw, k, C);
         } else {
             if (!Xs) Xs = (float *)malloc(mbsize * d * sizeof(float));
             for (uint64_t t = 0; t < mbsize; ++t) memcpy(Xs + t * d, X + sel[t] * d, d * sizeof(float));
             assign_generic(msr, Xs, NULL, NULL, NULL, mbsize, d, C, k, prior, sa, sc, nthreads);
-            mco_centroids_dense(Xs, mbsize, d, sa, sw, k, 256, C);
+            if (median_policy(msr)) median_core(median_policy(msr), Xs, NULL, NULL, NULL, mbsize, d, sw, w, k, sa, C,
+                                                0., 0, 1);
+            else mco_centroids_dense(Xs, mbsize, d, sa, sw, k, 256, C);
         }
     }
     memcpy(C, save, k * d * sizeof(float));
@@ -1345,6 +1365,91 @@ static void geo_median(const float *X, uint64_t d, const uint64_t *mem, uint64_t
     free(acc);
 }
 
+/* The centers of one median policy for the assignment asn (empty clusters keep
+ * theirs).  wm: per-row weights (the weighted L1 of set_centroids_l1 and the
+ * geometric median); wr: the weight vector the reference indexes by member rank
+ * (TVD always; L1 too when rank_l1, the minibatch update's l1_median, kmedian.h:340).
+ * single_raw: a one-member cluster takes its raw row (set_centroids_*), else the
+ * rule for one member (the minibatch update; TVD then normalises it). */
+static void median_core(int pol, const float *X, const float *val, const uint32_t *idx, const uint64_t *indptr,
+                        uint64_t n, uint64_t d, const double *wm, const double *wr, uint64_t k, const uint32_t *asn,
+                        float *C, double eps, int single_raw, int rank_l1) {
+    /* members per cluster, ascending row id */
+    uint64_t *off = (uint64_t *)calloc(k + 1, sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; ++i) off[asn[i] + 1]++;
+    for (uint64_t c = 0; c < k; ++c) off[c + 1] += off[c];
+    uint64_t *mem = (uint64_t *)malloc((n ? n : 1) * sizeof(uint64_t)), *cur = (uint64_t *)malloc(k * sizeof(uint64_t));
+    memcpy(cur, off, k * sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; ++i) mem[cur[asn[i]]++] = i;
+    double *rs = (double *)malloc((n ? n : 1) * sizeof(double));
+    for (uint64_t i = 0; i < n; ++i) rs[i] = val ? row_sum_csr(val, indptr[i], indptr[i + 1]) : row_sum_dense(X + i * d, d);
+    float *vf = (float *)malloc((n ? n : 1) * sizeof(float));
+    double *vd = (double *)malloc((n ? n : 1) * sizeof(double)), *qq = (double *)malloc((n ? n : 1) * sizeof(double));
+    for (uint64_t c = 0; c < k; ++c) {
+        const uint64_t *mm = mem + off[c], m = off[c + 1] - off[c];
+        float *ctr = C + c * d;
+        if (m == 0) continue;
+        if (m == 1 && (single_raw || pol == 3)) {   /* geomedian of one row: the row */
+            const uint64_t r = mm[0];
+            if (val) {
+                memset(ctr, 0, d * sizeof(float));
+                for (uint64_t p = indptr[r]; p < indptr[r + 1]; ++p) ctr[idx[p]] = val[p];
+            } else memcpy(ctr, X + r * d, d * sizeof(float));
+            continue;
+        }
+        if (pol == 3) { geo_median(X, d, mm, m, wm, ctr, eps, vd, qq); continue; }
+        if (val) {
+            /* per column, the stored values of the members */
+            for (uint64_t j = 0; j < d; ++j) {
+                uint64_t nel = 0;
+                for (uint64_t q = 0; q < m; ++q) {
+                    const uint64_t r = mm[q];
+                    for (uint64_t p = indptr[r]; p < indptr[r + 1]; ++p) {
+                        if (idx[p] != j) continue;
+                        vf[nel++] = pol == 2 ? val[p] * (float)(1. / rs[r]) : val[p];
+                    }
+                }
+                if (!nel) { ctr[j] = 0.f; continue; }
+                qsort(vf, nel, sizeof(float), cmp_f32);
+                ctr[j] = csr_median_value(vf, nel, m, pol == 2);
+            }
+            continue;
+        }
+        if (wm) {
+            /* heaviest member (by row weight, or by wr[rank]), smallest value */
+            const int byrank = pol == 2 || rank_l1;
+            double wmax = -INFINITY;
+            for (uint64_t q = 0; q < m; ++q) { const double wq = byrank ? wr[q] : wm[mm[q]]; if (wq > wmax) wmax = wq; }
+            for (uint64_t j = 0; j < d; ++j) {
+                double best = INFINITY;
+                for (uint64_t q = 0; q < m; ++q) {
+                    const double wq = byrank ? wr[q] : wm[mm[q]];
+                    if (wq != wmax) continue;
+                    const float x = X[mm[q] * d + j];
+                    const double v = pol == 2 ? (double)x * (1. / rs[mm[q]]) : (double)x;
+                    if (v < best) best = v;
+                }
+                ctr[j] = (float)best;
+            }
+            continue;
+        }
+        const uint64_t h = m / 2;
+        for (uint64_t j = 0; j < d; ++j) {
+            if (pol == 1) {
+                for (uint64_t q = 0; q < m; ++q) vf[q] = X[mm[q] * d + j];
+                qsort(vf, m, sizeof(float), cmp_f32);
+                ctr[j] = (m & 1) ? vf[h] : 0.5f * (vf[h - 1] + vf[h]);
+            } else {
+                for (uint64_t q = 0; q < m; ++q) vd[q] = (double)X[mm[q] * d + j] * (1. / rs[mm[q]]);
+                qsort(vd, m, sizeof(double), cmp_f64);
+                ctr[j] = (float)((m & 1) ? vd[h] : 0.5 * (vd[h - 1] + vd[h]));
+            }
+        }
+    }
+    free(off); free(mem); free(cur); free(rs); free(vf); free(vd); free(qq);
+}
+
+
 /* The M-step of one median policy from (asn, cost), which the empty-cluster
  * restart updates in place.  C holds the previous centers (the geometric
  * median starts from them; restarted clusters from their drawn row).
@@ -1380,77 +1485,8 @@ int mco_median_centers(int msr, const float *X, const float *val, const uint32_t
         }
         if (!any) break;
     }
-    /* members per cluster, ascending row id */
-    uint64_t *off = (uint64_t *)calloc(k + 1, sizeof(uint64_t));
-    for (uint64_t i = 0; i < n; ++i) off[asn[i] + 1]++;
-    for (uint64_t c = 0; c < k; ++c) off[c + 1] += off[c];
-    uint64_t *mem = (uint64_t *)malloc((n ? n : 1) * sizeof(uint64_t)), *cur = (uint64_t *)malloc(k * sizeof(uint64_t));
-    memcpy(cur, off, k * sizeof(uint64_t));
-    for (uint64_t i = 0; i < n; ++i) mem[cur[asn[i]]++] = i;
-    double *rs = (double *)malloc((n ? n : 1) * sizeof(double));
-    for (uint64_t i = 0; i < n; ++i) rs[i] = val ? row_sum_csr(val, indptr[i], indptr[i + 1]) : row_sum_dense(X + i * d, d);
-    float *vf = (float *)malloc((n ? n : 1) * sizeof(float));
-    double *vd = (double *)malloc((n ? n : 1) * sizeof(double)), *qq = (double *)malloc((n ? n : 1) * sizeof(double));
-    for (uint64_t c = 0; c < k; ++c) {
-        const uint64_t *mm = mem + off[c], m = off[c + 1] - off[c];
-        float *ctr = C + c * d;
-        if (m == 1) {
-            const uint64_t r = mm[0];
-            if (val) {
-                memset(ctr, 0, d * sizeof(float));
-                for (uint64_t p = indptr[r]; p < indptr[r + 1]; ++p) ctr[idx[p]] = val[p];
-            } else memcpy(ctr, X + r * d, d * sizeof(float));
-            continue;
-        }
-        if (pol == 3) { geo_median(X, d, mm, m, w, ctr, eps, vd, qq); continue; }
-        if (val) {
-            /* per column, the stored values of the members */
-            for (uint64_t j = 0; j < d; ++j) {
-                uint64_t nel = 0;
-                for (uint64_t q = 0; q < m; ++q) {
-                    const uint64_t r = mm[q];
-                    for (uint64_t p = indptr[r]; p < indptr[r + 1]; ++p) {
-                        if (idx[p] != j) continue;
-                        vf[nel++] = pol == 2 ? val[p] * (float)(1. / rs[r]) : val[p];
-                    }
-                }
-                if (!nel) { ctr[j] = 0.f; continue; }
-                qsort(vf, nel, sizeof(float), cmp_f32);
-                ctr[j] = csr_median_value(vf, nel, m, pol == 2);
-            }
-            continue;
-        }
-        if (w) {
-            /* heaviest member (L1: by row weight; TVD: by w[rank]), smallest value */
-            double wmax = -INFINITY;
-            for (uint64_t q = 0; q < m; ++q) { const double wq = pol == 2 ? w[q] : w[mm[q]]; if (wq > wmax) wmax = wq; }
-            for (uint64_t j = 0; j < d; ++j) {
-                double best = INFINITY;
-                for (uint64_t q = 0; q < m; ++q) {
-                    const double wq = pol == 2 ? w[q] : w[mm[q]];
-                    if (wq != wmax) continue;
-                    const float x = X[mm[q] * d + j];
-                    const double v = pol == 2 ? (double)x * (1. / rs[mm[q]]) : (double)x;
-                    if (v < best) best = v;
-                }
-                ctr[j] = (float)best;
-            }
-            continue;
-        }
-        const uint64_t h = m / 2;
-        for (uint64_t j = 0; j < d; ++j) {
-            if (pol == 1) {
-                for (uint64_t q = 0; q < m; ++q) vf[q] = X[mm[q] * d + j];
-                qsort(vf, m, sizeof(float), cmp_f32);
-                ctr[j] = (m & 1) ? vf[h] : 0.5f * (vf[h - 1] + vf[h]);
-            } else {
-                for (uint64_t q = 0; q < m; ++q) vd[q] = (double)X[mm[q] * d + j] * (1. / rs[mm[q]]);
-                qsort(vd, m, sizeof(double), cmp_f64);
-                ctr[j] = (float)((m & 1) ? vd[h] : 0.5 * (vd[h - 1] + vd[h]));
-            }
-        }
-    }
-    free(cnt); free(off); free(mem); free(cur); free(rs); free(vf); free(vd); free(qq);
+    free(cnt);
+    median_core(pol, X, val, idx, indptr, n, d, w, w, k, asn, C, eps, 1, 0);
     return 0;
 }
 

@@ -126,3 +126,111 @@ def test_minibatch_invalid_sizes(mcg, ctx):
         with pytest.raises(mcg.MCGError):
             ds.minibatch(mcg.SQRL2, X[:3], **kw)
     ds.close()
+
+
+def _median_compare(got, ref, tol=1e-5):
+    """Median-policy runs: FP32 pair costs on the GPU against FP64 in the oracle, so
+    the check-in totals agree to rounding; draws, restarts and the best-centers
+    choice follow them, and then centers and assignment are identical."""
+    assert got["numiter"] == ref["numiter"]
+    assert got["initcost"] == pytest.approx(ref["initcost"], rel=tol)
+    assert got["finalcost"] == pytest.approx(ref["finalcost"], rel=tol)
+    assert np.array_equal(got["asn"], ref["asn"])
+    assert np.array_equal(got["centers"], ref["centers"])
+
+
+@pytest.mark.parametrize("msr,weighted,with_rep", [(0, False, False), (0, True, False), (9, False, True),
+                                                   (9, True, False), (1, False, False), (1, True, True)])
+def test_minibatch_median_dense(mcg, ctx, mco, msr, weighted, with_rep):
+    """The median updates of solve.h:555-572 (weighted L1 / TVD by member rank, the
+    geometric median by member weights) inside the minibatch driver."""
+    say(f"minibatch median dense msr={msr} weighted={weighted} with_rep={with_rep}")
+    n, d, k = 6000, 16, 10
+    X, lab, _ = gaussian_mixture(n, d, k, 4, lo=-6, hi=6)
+    if msr == 9:
+        X = np.exp(X / 4.).astype(np.float32)
+    w = np.random.default_rng(6).uniform(0.5, 2., n) if weighted else None
+    C0 = _start(X, lab, k)
+    # an empty cluster, restarted at the first check-in (TVD compares distributions,
+    # so that center is made a near point mass)
+    if msr == 9:
+        C0[4] = 1e-3
+        C0[4, 0] = 1e3
+    else:
+        C0[4] = 1e3
+    kw = dict(mbsize=900, maxiter=13, freq=4, reseed_after=2, seed=21)
+    ref = mco.minibatch(msr, X, C0, w=w, with_rep=with_rep, **kw)
+    ds = mcg.Dataset.dense(X, ctx)
+    got = ds.minibatch(msr, C0, weights=w, with_replacement=with_rep, **kw)
+    ds.close()
+    _median_compare(got, ref)
+
+
+def test_minibatch_median_csr_l1(mcg, ctx, mco):
+    say("minibatch median csr L1")
+    n, d, k = 3000, 60, 6
+    data, idx, ptr = poisson_csr(n, d, 0.4, 14, normalize=False)
+    X = mco.CSR(data, idx, ptr, (n, d))
+    C0 = X.todense()[np.random.default_rng(3).choice(n, k, replace=False)].copy()
+    kw = dict(mbsize=700, maxiter=9, freq=3, reseed_after=1, seed=8)
+    ref = mco.minibatch(0, X, C0, **kw)
+    ds = mcg.Dataset.csr(data, idx, ptr, (n, d), ctx)
+    got = ds.minibatch(0, C0, **kw)
+    ds.close()
+    agree = np.mean(got["asn"] == ref["asn"])
+    say("  agree", agree)
+    if agree == 1.0:
+        _median_compare(got, ref, tol=1e-4)
+    else:   # count data: exact L1 ties between centers may resolve the other way in FP32
+        assert agree > 0.99
+        assert got["finalcost"] == pytest.approx(ref["finalcost"], rel=1e-3)
+
+
+@pytest.mark.parametrize("msr", [0, 1])
+def test_cs_minibatch_median_dense(mcg, ctx, mco, msr):
+    """Coreset minibatch with a median policy.  The weighted Lloyd on one coreset (the
+    oracle's draws from the oracle's LBK probabilities) is bit-identical to the
+    oracle's, and so is the whole driver (its LBK probabilities come from FP32 costs
+    here and FP64 ones in the oracle; on this data no draw depends on the difference)."""
+    say(f"cs minibatch median msr={msr}")
+    n, d, k = 5000, 12, 8
+    X, lab, _ = gaussian_mixture(n, d, k, 5, lo=-6, hi=6)
+    C0 = _start(X, lab, k)
+    a, c, _ = mco.assign_f64(msr, X, C0, 0.)
+    probs = mco.coreset_probs(4, k, c, a)
+    wt, ind = mco.coreset_sample(probs, 800, mco.wyrand_at(9, 0))
+    o = np.argsort(ind, kind="stable")
+    ind, csw = ind[o], wt[o].astype(np.float64)
+    for mi in (0, 3):
+        ref = mco.lloyd_median(msr, X[ind], C0, w=csw, eps=1e-3, maxiter=mi)
+        ds = mcg.Dataset.dense(X[ind], ctx)
+        got = ds.lloyd(msr, C0, weights=csw, eps=1e-3, maxiter=mi)
+        ds.close()
+        _median_compare(got, ref)
+    Xs, labs, _ = gaussian_mixture(n, d, k, 6, lo=-40, hi=40)
+    C1 = _start(Xs, labs, k) + 0.5
+    kw = dict(mbsize=800, maxiter=8, freq=4, reseed_after=1, seed=9)
+    ref = mco.cs_minibatch(msr, Xs, C1, **kw)
+    ds = mcg.Dataset.dense(Xs, ctx)
+    got = ds.cs_minibatch(msr, C1, **kw)
+    ds.close()
+    _median_compare(got, ref)
+
+
+def test_minibatch_median_unsupported(mcg, ctx):
+    n, d = 400, 10
+    data, idx, ptr = poisson_csr(n, d, 0.4, 15, normalize=False)
+    ds = mcg.Dataset.csr(data, idx, ptr, (n, d), ctx)
+    C0 = np.ones((3, d), np.float32)
+    with pytest.raises(mcg.MCGError) as e:
+        ds.minibatch(0, C0, weights=np.ones(n), mbsize=100, maxiter=3, freq=1)
+    assert e.value.code == -4
+    with pytest.raises(mcg.MCGError) as e:
+        ds.minibatch(1, C0, mbsize=100, maxiter=3, freq=1)
+    assert e.value.code == -4
+    ds.close()
+    dd = mcg.Dataset.dense(np.random.default_rng(2).random((n, d)).astype(np.float32), ctx)
+    with pytest.raises(mcg.MCGError) as e:   # rank-indexed weights need mbsize <= n
+        dd.minibatch(0, C0, weights=np.ones(n), mbsize=n + 10, maxiter=3, freq=1, with_replacement=True)
+    assert e.value.code == -1
+    dd.close()

@@ -302,6 +302,24 @@ def test_minibatch_full_batch_is_lloyd(mco, weighted):
     assert r["initcost"] == _fixed_total(c0.astype(np.float64) * (w if weighted else 1.))
 
 
+@pytest.mark.parametrize("msr,weighted", [(0, False), (1, False), (1, True)])
+def test_minibatch_full_batch_is_median_lloyd(mco, msr, weighted):
+    """The median updates of the minibatch driver (solve.h:555-572): with every row
+    drawn, no restarts and a check-in every iteration they are the median M-steps of
+    Lloyd (L1 unweighted never raises the cost; the geometric median keeps the
+    members' weights in both drivers)."""
+    from minicore_amd.datagen import gaussian_mixture
+    X, lab, _ = gaussian_mixture(2400, 6, 5, seed=5, lo=-4, hi=4)
+    w = np.random.default_rng(2).uniform(0.5, 2., 2400) if weighted else None
+    C0 = (X[[int(np.argmax(lab == c)) for c in range(5)]] + 0.25).astype(np.float32)
+    T = 5
+    r = mco.minibatch(msr, X, C0, w=w, mbsize=2400, maxiter=T, freq=1, reseed_after=0, with_rep=False, seed=4)
+    L = mco.lloyd_median(msr, X, C0, w=w, eps=-1., maxiter=T - 2)
+    assert L["numiter"] == T - 1 and r["numiter"] == T
+    assert np.array_equal(r["centers"], L["centers"])
+    assert np.array_equal(r["asn"], L["asn"])
+
+
 @pytest.mark.parametrize("weighted", [False, True])
 def test_minibatch_restart_draw(mco, weighted):
     """A center with no rows is restarted at the row drawn over w_i * cost_i with

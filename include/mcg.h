@@ -152,6 +152,11 @@ int mcg_data_shape(const mcg_data *ds, uint64_t *n, uint64_t *d, uint64_t *nnz);
  * (cmp) and pycluster_soft.h:38-40 (soft costs). */
 int mcg_cost_matrix(mcg_data *ds, int msr, double prior, const float *centers, uint64_t k,
                     float *out);
+/* Condensed pairwise costs among the rows: pcmp, python/pycmp.cpp:249-338.
+ * out[n*i - i*(i+1)/2 + (j - i - 1)] = msr(first = row i, second = row j) for all
+ * i < j; n*(n-1)/2 floats, host or device.  Each value equals the cost matrix's
+ * (mcg_cost_matrix of the rows against the rows as centers). */
+int mcg_pairwise(mcg_data *ds, int msr, double prior, float *out);
 /* Hard E-step: asn[i] = argmin_c msr(row i, center c) (lowest index on ties),
  * costs[i] = that minimum (nullable).  Replaces assign_points_hard,
  * include/minicore/clustering/solve.h:234-285.  precision applies to dense SQRL2. */

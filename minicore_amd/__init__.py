@@ -18,7 +18,7 @@ except ImportError as e:  # pragma: no cover - exercised when the extension is n
     _pyminicore = None
     _IMPORT_ERROR = e
 
-__all__ = ["hcluster", "scluster", "kmeanspp", "cluster", "cmp", "CSparseMatrix", "CoresetSampler",
+__all__ = ["hcluster", "scluster", "kmeanspp", "cluster", "cmp", "pcmp", "CSparseMatrix", "CoresetSampler",
            "cluster_from_centers", "set_sqrl2_precision"]
 
 
@@ -94,6 +94,11 @@ def cmp(matrix, data, *, msr=2, prior=0., **kw):
     m = _as_matrix(matrix)
     d = _as_matrix(data)
     return _ext().cmp(m, d, msr=msr, prior=prior, **kw)
+
+
+def pcmp(matrix, *, msr=2, prior=0., use_float=-1):
+    """Condensed pairwise costs msr(row i, row j), i < j (python/pycmp.cpp:249-338)."""
+    return _ext().pcmp(_as_matrix(matrix), msr=msr, prior=prior, use_float=use_float)
 
 
 def cluster(data, *, msr, k, prior=0., seed=0, nmkc=0, ntimes=1, lspp=0, use_exponential_skips=False,

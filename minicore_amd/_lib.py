@@ -25,7 +25,7 @@ EXPORTS = [
     "mcg_timing_reset", "mcg_timing_read", "mcg_refine_counts", "mcg_data_rows", "mcg_server_create",
     "mcg_server_destroy", "mcg_submit", "mcg_poll", "mcg_wait", "mcg_cancel", "mcg_release", "mcg_tuning_enable",
     "mcg_graphs_enable", "mcg_minibatch", "mcg_cs_minibatch", "mcg_kmeanspp_ls", "mcg_sampler_write",
-    "mcg_sampler_read", "mcg_index_coreset_write", "mcg_index_coreset_read",
+    "mcg_sampler_read", "mcg_index_coreset_write", "mcg_index_coreset_read", "mcg_pairwise",
 ]
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = range(5)
@@ -99,6 +99,7 @@ def load():
         "mcg_data_free": (I, [P]),
         "mcg_data_shape": (I, [P, P, P, P]),
         "mcg_cost_matrix": (I, [P, I, D, P, U64, P]),
+        "mcg_pairwise": (I, [P, I, D, P]),
         "mcg_assign_hard": (I, [P, I, D, P, U64, I, P, P]),
         "mcg_centroids_hard": (I, [P, P, P, U64, P, P]),
         "mcg_lloyd": (I, [P, I, D, P, U64, P, D, U64, I, P, P, P, P, P]),
@@ -299,6 +300,14 @@ class Dataset:
         out = np.empty((self.n, centers.shape[0]), dtype=np.float32)
         check(load().mcg_cost_matrix(self.h, msr, prior, ptr(centers), centers.shape[0], ptr(out)))
         return out
+
+    def pairwise(self, msr, prior=0.):
+        """Condensed upper triangle of msr(row i, row j), i < j (pcmp)."""
+        m = self.n * (self.n - 1) // 2 if self.n > 1 else 0
+        out = np.empty(max(m, 1), dtype=np.float32)
+        if m:
+            check(load().mcg_pairwise(self.h, msr, prior, ptr(out)))
+        return out[:m]
 
     def assign(self, msr, centers, prior=0., precision=PREC_BF16, with_costs=True):
         centers = np.ascontiguousarray(centers, dtype=np.float32)

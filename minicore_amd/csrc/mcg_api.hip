@@ -734,6 +734,64 @@ int mcg_cost_matrix(mcg_data *ds, int msr, double prior, const float *centers, u
     });
 }
 
+// condensed index of (i, j), i < j: n*i - i*(i+1)/2 + (j - i - 1) (pycmp.cpp:261)
+__global__ void k_condense(const float *__restrict__ blk, uint64_t R, uint64_t K, uint64_t i0, uint64_t j0, uint64_t n,
+                           float *__restrict__ out) {
+    const uint64_t total = R * K;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t a = e / K, b = e - a * K;
+        const uint64_t i = i0 + a, j = j0 + b;
+        if (j > i) out[n * i - i * (i + 1) / 2 + (j - i - 1)] = blk[e];
+    }
+}
+
+// pcmp (python/pycmp.cpp:249-338): out[(i, j)] = msr(first = row i, second = row j) for
+// i < j in condensed order, by blocks of rows against blocks of rows as centers
+// (blocks below the diagonal skipped), each block through the cost-matrix kernels, so
+// every value equals the cost matrix's.
+int mcg_pairwise(mcg_data *ds, int msr, double prior, float *out) {
+    return guard([&] {
+        REQUIRE(ds && out, "null argument");
+        check_msr(msr);
+        Data &D = ds->d;
+        Ctx &c = *D.ctx;
+        hipStream_t st = c.stream;
+        HIP_CHECK(hipSetDevice(c.device));
+        const uint64_t n = D.n, d = D.d, npairs = n * (n - (n ? 1 : 0)) / 2;
+        if (npairs == 0) return;
+        DBuf so, bblk, bctr, bidx, bidxc;
+        Out o = out_dev(c, out, npairs * 4, so);
+        const uint64_t B = 4096;
+        float *blk = (float *)bblk.reserve(B * B * 4);
+        float *ctr = D.csr ? (float *)bctr.reserve(B * d * 4) : nullptr;
+        uint64_t *idx_d = (uint64_t *)bidx.reserve(B * 8);
+        std::vector<uint64_t> ids(B);
+        std::vector<uint32_t> cids(B);
+        mcg_data sub;
+        for (uint64_t i0 = 0; i0 + 1 < n; i0 += B) {
+            const uint64_t R = std::min(B, n - i0);
+            for (uint64_t t = 0; t < R; ++t) ids[t] = i0 + t;
+            copy_sync(idx_d, ids.data(), R * 8, hipMemcpyHostToDevice, st);
+            rows_into(D, ids.data(), idx_d, R, sub.d, st);
+            for (uint64_t j0 = i0; j0 < n; j0 += B) {
+                const uint64_t K = std::min(B, n - j0);
+                const float *C = D.X + j0 * d;
+                if (D.csr) {
+                    std::vector<uint64_t> rows(K);
+                    for (uint64_t t = 0; t < K; ++t) { rows[t] = j0 + t; cids[t] = (uint32_t)t; }
+                    copy_rows_to_centers(D, rows.data(), cids.data(), K, ctr, st);
+                    C = ctr;
+                }
+                generic_costs(sub.d, msr, prior, C, K, blk, nullptr, nullptr, st);
+                hipLaunchKernelGGL(k_condense, dim3((uint32_t)std::min<uint64_t>(cdiv(R * K, 256), 65536)), dim3(256),
+                                   0, st, blk, R, K, i0, j0, n, (float *)o.dev);
+            }
+        }
+        out_finish(c, o);
+        sync(c);
+    });
+}
+
 int mcg_assign_hard(mcg_data *ds, int msr, double prior, const float *centers, uint64_t k, int precision,
                     uint32_t *asn, float *costs) {
     return guard([&] {

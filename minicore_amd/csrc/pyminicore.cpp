@@ -489,6 +489,30 @@ py::object cmp_arrays(py::array lhs, py::array rhs, py::object msr, double prior
     return out;
 }
 
+// ---- pcmp (pycmp.cpp:249-338): condensed upper triangle, out[(i, j)] = msr(first = row i,
+// second = row j) ----------------------------------------------------------------------------
+py::array pairwise(mcg_data *h, uint64_t n, int msr, double prior) {
+    const py::ssize_t np = (py::ssize_t)(n > 1 ? n * (n - 1) / 2 : 0);
+    py::array_t<float> out(np);
+    if (np) check(mcg_pairwise(h, msr, prior, out.mutable_data()));
+    return out;
+}
+
+py::array pcmp_csr(PyCSparseMatrix &lhs, py::object msr, py::object betaprior, py::ssize_t) {
+    return pairwise(lhs.device(), lhs.nr, measure_code(msr), betaprior.cast<double>());
+}
+
+// dense: float32 results, returned as float64 when use_float is 0 or the input is
+// wider than 4 bytes and use_float < 0 (the reference then computes in double)
+py::array pcmp_dense(py::array mat, py::object msr, py::object betaprior, py::ssize_t use_float) {
+    if (mat.ndim() != 2) throw std::invalid_argument("pcmp expects a 2-d numpy matrix");
+    const bool luf = use_float < 0 ? mat.itemsize() <= 4 : use_float != 0;
+    Dense D = dense_of(mat);
+    py::array out = pairwise(D.dev->h, D.n, measure_code(msr), betaprior.cast<double>());
+    if (!luf) out = py::cast<py::array>(out.attr("astype")("float64"));
+    return out;
+}
+
 // ---- kmeanspp (smw.h:179-410) --------------------------------------------------------------
 py::object asn_array(const std::vector<uint32_t> &a, uint64_t k) {
     const uint64_t n = a.size();
@@ -716,6 +740,11 @@ PYBIND11_MODULE(_pyminicore, m) {
     m.def("cmp", &cmp_csr_csr, py::arg("matrix"), py::arg("data"), py::arg("msr") = 2, py::arg("prior") = 0.);
     m.def("cmp", &cmp_arrays, py::arg("lhs"), py::arg("rhs"), py::arg("msr") = 2, py::arg("prior") = 0.,
           py::arg("reverse") = true, py::arg("use_double") = -1);
+
+    m.def("pcmp", &pcmp_csr, py::arg("matrix"), py::arg("msr") = 2, py::arg("prior") = 0.,
+          py::arg("use_float") = py::ssize_t(-1));
+    m.def("pcmp", &pcmp_dense, py::arg("matrix"), py::arg("msr") = 2, py::arg("prior") = 0.,
+          py::arg("use_float") = py::ssize_t(-1));
 
     m.def("kmeanspp", &kmeanspp_dense, py::arg("matrix"), py::arg("k"), py::arg("msr") = py::int_(2),
           py::arg("prior") = 0., py::arg("seed") = 0, py::arg("ntimes") = 0, py::arg("lspp") = 0,

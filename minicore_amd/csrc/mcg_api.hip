@@ -1353,7 +1353,13 @@ int mcg_soft_em(mcg_data *ds, int msr, double prior, double temp, float *centers
     return guard([&] {
         REQUIRE(ds && centers, "null argument");
         REQUIRE(k > 0, "k must be positive");
-        check_mean_msr(msr);
+        check_msr(msr);
+        // set_centroids_full_mean (soft, centroid.h:795-963): every measure takes the
+        // responsibility-weighted mean except L1 and L2, whose weighted median /
+        // geometric median over all rows are not on this path
+        if (msr == MCG_L1 || msr == MCG_L2)
+            MCG_THROW(MCG_EUNSUPPORTED, "soft clustering with L1 / L2 (weighted median centers over all rows, "
+                                        "centroid.h:815-835) is not on the GPU path");
         Data &D = ds->d;
         Ctx &c = *D.ctx;
         hipStream_t st = c.stream;

@@ -392,7 +392,7 @@ py::dict run_soft(mcg_data *h, uint64_t n, uint64_t d, std::vector<float> &C, ui
 py::object scluster_dense(py::array mat, py::object centers, py::object msr, double prior, double temp,
                           uint64_t maxiter, py::ssize_t mbsize, py::ssize_t, std::string savepref, py::object weights) {
     if (prior < 0.) prior = 0.;  // pycluster_softdense.cpp:71
-    if (mbsize > 0) throw py::value_error("minibatch soft clustering is not on the GPU path");
+    if (mbsize > 0) throw std::runtime_error("Not yet completed: minibatch soft clustering");   // solve.h:319-321
     const int m = measure_code(msr);
     Dense D = dense_of(mat);
     uint64_t k = 0;
@@ -412,7 +412,7 @@ py::object scluster_dense(py::array mat, py::object centers, py::object msr, dou
 
 py::object scluster_csr(PyCSparseMatrix &smw, py::object centers, py::object msr, double prior, double temp,
                         uint64_t maxiter, py::ssize_t mbsize, py::ssize_t, std::string savepref, py::object weights) {
-    if (mbsize > 0) throw py::value_error("minibatch soft clustering is not on the GPU path");
+    if (mbsize > 0) throw std::runtime_error("Not yet completed: minibatch soft clustering");   // solve.h:319-321
     const int m = measure_code(msr);
     uint64_t k = 0;
     auto C = center_block(centers, smw.nc, [&](const std::vector<int64_t> &ids) { return smw.dense_rows(ids); }, &k);

@@ -131,7 +131,7 @@ def test_csr_centroids(mcg, ctx, mco):
 
 
 @pytest.mark.parametrize("sparse", [False, True])
-@pytest.mark.parametrize("name,code", [("SQRL2", 2), ("REVERSE_MKL", 11), ("JSD", 4)])
+@pytest.mark.parametrize("name,code", [("SQRL2", 2), ("REVERSE_MKL", 11), ("JSD", 4), ("TVD", 9), ("HELLINGER", 6)])
 def test_soft_em(mcg, ctx, mco, name, code, sparse):
     say(f"soft {name} sparse={sparse}")
     if sparse:
@@ -149,7 +149,7 @@ def test_soft_em(mcg, ctx, mco, name, code, sparse):
         prior = 0.5
     # a temperature at the cost scale keeps the responsibilities soft (no underflow to argmin-uniform)
     K0 = mco.cost_matrix(code, X, C0, prior)
-    temp = float(2. / np.median(K0)) if code == 2 else 1.
+    temp = float(2. / np.median(K0)) if code in (2, 9, 6) else 1.
     # eps < 0: both run exactly maxiter iterations
     ref = mco.soft(code, X, C0, prior=prior, temp=temp, eps=-1., maxiter=6)
     got = ds.soft(code, C0, prior=prior, temp=temp, eps=-1., maxiter=6)
@@ -158,3 +158,15 @@ def test_soft_em(mcg, ctx, mco, name, code, sparse):
     assert got["finalcost"] == pytest.approx(ref["finalcost"], rel=2e-3)
     np.testing.assert_allclose(got["asn"], ref["asn"], atol=2e-3)
     np.testing.assert_allclose(got["centers"], ref["centers"], rtol=2e-3, atol=1e-4)
+
+
+def test_soft_em_median_measures_refused(mcg, ctx):
+    """Soft clustering with L1 / L2 takes weighted medians over all rows
+    (centroid.h:815-835): not on the GPU path, refused with MCG_EUNSUPPORTED."""
+    X = np.random.default_rng(3).random((100, 5)).astype(np.float32)
+    ds = mcg.Dataset.dense(X, ctx)
+    for code in (0, 1):
+        with pytest.raises(mcg.MCGError) as e:
+            ds.soft(code, X[:3], maxiter=2)
+        assert e.value.code == -4
+    ds.close()

@@ -79,6 +79,11 @@ extern "C" {
 #define MCG_RSIS                         24
 #define MCG_SRLRT                        25
 #define MCG_SRULRT                       26
+/* Poisson Bregman divergence (I-divergence), bnj::poisson_bregman, dist/distance.h:576-585:
+ * sum_j xh log(xh / ch) - xh + ch with xh = x + prior, ch = c + prior (first = row,
+ * second = center).  The reference gives it no measure code; 64 is this library's,
+ * outside the reference's range.  Weighted-mean centers (a Bregman divergence). */
+#define MCG_POISSON                      64
 
 /* cross-term precision for dense SQRL2 */
 #define MCG_PREC_FP32  0   /* FP32 MFMA cross term, FP32 compare ("plain FP32 form") */

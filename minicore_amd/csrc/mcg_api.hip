@@ -512,7 +512,8 @@ static void estep(Data &ds, int msr, double prior, const float *C, uint64_t k, i
 }
 
 static void check_msr(int msr) {
-    REQUIRE(msr_supported(msr), "unsupported measure code (supported: 0-15, 16-18, 23-26; see distance.h)");
+    REQUIRE(msr_supported(msr), "unsupported measure code (supported: 0-18, 23-26 of distance.h, and 64 for the "
+                                "Poisson I-divergence)");
 }
 
 // hard clustering needs a mean-centroid measure (centroid.h:182-207)

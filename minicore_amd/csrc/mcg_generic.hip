@@ -502,6 +502,16 @@ __global__ __launch_bounds__(256) void k_pairs_direct(int msr, const float *__re
 // end, which would cancel in FP32), REVERSE_IS swapped; SIS and REVERSE_SIS
 // sum log((a/b + b/a + 2) / 4) / 2; JSM sqrt(JSD).  One wave per row, lanes over
 // centers; a CSR row is walked along the dense coordinates.
+// One coordinate of the I-divergence: xh log(xh / ch) - xh + ch, written as
+// ch (t log1p(u) - u) with t = xh / ch = 1 + u, which keeps the relative error small
+// when xh and ch are close; xh = 0 gives ch, ch = 0 < xh gives +inf.
+__device__ __forceinline__ float poisson_term(float xh, float ch) {
+    if (xh == 0.f) return ch;
+    if (ch == 0.f) return INFINITY;
+    const float u = (xh - ch) / ch;
+    return ch * ((1.f + u) * log1pf(u) - u);
+}
+
 __global__ __launch_bounds__(256) void k_pairs_more(int msr, const float *__restrict__ X, const float *__restrict__ val,
                                                     const uint32_t *__restrict__ idx, const uint64_t *__restrict__ ptr,
                                                     uint64_t n, uint32_t d, const double *__restrict__ rsum,
@@ -559,11 +569,13 @@ __global__ __launch_bounds__(256) void k_pairs_more(int msr, const float *__rest
             } break;
             case MCG_IS: { const float r = a / b; s += (r - logf(r)) - 1.f; } break;
             case MCG_REVERSE_IS: { const float r = b / a; s += (r - logf(r)) - 1.f; } break;
+            case MCG_POISSON: s += poisson_term(x + prior, cj + prior); break;
             default: s += 0.5f * logf((a / b + b / a + 2.f) * 0.25f); break;   // SIS, RSIS
             }
         }
         float v;
         switch (msr) {
+        case MCG_POISSON: v = s < 0.f ? 0.f : (s == INFINITY ? FLT_MAX : s); break;
         case MCG_L1: v = s; break;
         case MCG_L2: v = sqrtf(s); break;
         case MCG_HELLINGER: v = sqrtf(s) * 0.70710678f; break;

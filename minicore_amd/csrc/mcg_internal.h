@@ -214,7 +214,7 @@ inline bool msr_supported(int msr) {
     case MCG_BHATTACHARYYA_METRIC: case MCG_BHATTACHARYYA_DISTANCE: case MCG_TVD: case MCG_LLR:
     case MCG_REVERSE_MKL: case MCG_UWLLR: case MCG_IS: case MCG_REVERSE_IS: case MCG_COSINE_DISTANCE:
     case MCG_PROBABILITY_COSINE_DISTANCE: case MCG_COSINE_SIMILARITY: case MCG_PROBABILITY_COSINE_SIMILARITY:
-    case MCG_SIS: case MCG_RSIS: case MCG_SRLRT: case MCG_SRULRT:
+    case MCG_SIS: case MCG_RSIS: case MCG_SRLRT: case MCG_SRULRT: case MCG_POISSON:
         return true;
     }
     return false;
@@ -225,6 +225,7 @@ inline bool msr_mean_centroid(int msr) {
     case MCG_UWLLR: case MCG_LLR: case MCG_MKL: case MCG_JSD: case MCG_SQRL2: case MCG_REVERSE_MKL: case MCG_IS:
     case MCG_REVERSE_IS: case MCG_SIS: case MCG_RSIS: case MCG_SRULRT: case MCG_SRLRT: case MCG_JSM:
     case MCG_COSINE_DISTANCE: case MCG_BHATTACHARYYA_METRIC: case MCG_BHATTACHARYYA_DISTANCE: case MCG_HELLINGER:
+    case MCG_POISSON:
         return true;
     }
     return false;

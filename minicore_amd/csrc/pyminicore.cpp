@@ -69,14 +69,15 @@ int measure_code(py::object msr) {
             {"COSINE_DISTANCE", 15}, {"PROBABILITY_COSINE_DISTANCE", 16}, {"COSINE_SIMILARITY", 17},
             {"PROBABILITY_COSINE_SIMILARITY", 18}, {"SYMMETRIC_ITAKURA_SAITO", 23}, {"SIS", 23},
             {"RSYMMETRIC_ITAKURA_SAITO", 24}, {"REVERSE_SYMMETRIC_ITAKURA_SAITO", 24}, {"RSIS", 24}, {"SRLRT", 25},
-            {"SRULRT", 26}};
+            {"SRULRT", 26}, {"POISSON", 64}, {"POISSON_BREGMAN", 64}, {"IDIV", 64}};
         auto it = names.find(s);
         if (it == names.end()) throw std::invalid_argument("measure " + s + " has no GPU kernel");
         code = it->second;
     } else {
         code = msr.cast<int>();
     }
-    static const std::set<int> ok = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 23, 24, 25, 26};
+    static const std::set<int> ok = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 23, 24, 25, 26,
+                                     64};
     if (!ok.count(code)) throw std::invalid_argument("measure " + std::to_string(code) + " has no GPU kernel");
     return code;
 }

@@ -145,8 +145,26 @@ static double kl_tail(int msr, double klc, const smooth_t *sm) {
     return ret;
 }
 
+/* One coordinate of the I-divergence, bnj::poisson_bregman (distance.h:576-585):
+ * xh log(xh/ch) - xh + ch, xh = x + prior, ch = c + prior (0 for xh = 0 gives ch). */
+static double poisson_coord(double x, double c, double prior) {
+    const double xh = x + prior, ch = c + prior;
+    if (xh == 0.) return ch;
+    if (ch == 0.) return INFINITY;
+    return xh * log(xh / ch) - xh + ch;
+}
+static double poisson_post(double s) {
+    if (s < 0.) s = 0.;
+    return s > FLT_MAX ? FLT_MAX : s;
+}
+
 static double msr_more_dense(int msr, const double *x, const double *c, uint64_t d, double prior, double xsum,
                              double csum) {
+    if (msr == MCO_POISSON) {
+        double s = 0.;
+        for (uint64_t j = 0; j < d; ++j) s += poisson_coord(x[j], c[j], prior);
+        return poisson_post(s);
+    }
     const smooth_t sm = smoothing(prior, d, xsum, csum);
     const double lam = sm.lhsum / (sm.lhsum + sm.rhsum);
     double s = 0., s2 = 0., s3 = 0.;
@@ -231,6 +249,16 @@ uint64_t mco_shared_zeros(const uint32_t *a, uint64_t an, const uint32_t *b, uin
 static double msr_more_sparse(int msr, const double *xv, const uint32_t *xi, uint64_t xn, const double *cv,
                               const uint32_t *ci, uint64_t cn, uint64_t d, double prior, double xsum, double csum,
                               uint64_t sharednz) {
+    if (msr == MCO_POISSON) {   /* shared zeros contribute prior*log(1) - prior + prior = 0 */
+        double s = 0.;
+        uint64_t i = 0, j = 0;
+        while (i < xn || j < cn) {
+            if (j >= cn || (i < xn && xi[i] < ci[j])) s += poisson_coord(xv[i++], 0., prior);
+            else if (i >= xn || ci[j] < xi[i])        s += poisson_coord(0., cv[j++], prior);
+            else { s += poisson_coord(xv[i], cv[j], prior); ++i; ++j; }
+        }
+        return poisson_post(s);
+    }
     const smooth_t sm = smoothing(prior, d, xsum, csum);
     const double lam = sm.lhsum / (sm.lhsum + sm.rhsum), li = sm.lhinc, ri = sm.rhinc;
     double s = 0., s2 = 0., s3 = 0.;

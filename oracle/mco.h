@@ -30,7 +30,8 @@ enum {
     MCO_BHATTACHARYYA_DISTANCE = 8, MCO_TVD = 9, MCO_LLR = 10, MCO_UWLLR = 12, MCO_IS = 13,
     MCO_REVERSE_IS = 14, MCO_COSINE_DISTANCE = 15, MCO_PROBABILITY_COSINE_DISTANCE = 16,
     MCO_COSINE_SIMILARITY = 17, MCO_PROBABILITY_COSINE_SIMILARITY = 18, MCO_SIS = 23, MCO_RSIS = 24,
-    MCO_SRLRT = 25, MCO_SRULRT = 26
+    MCO_SRLRT = 25, MCO_SRULRT = 26,
+    MCO_POISSON = 64   /* I-divergence (poisson_bregman, distance.h:576-585); no reference code */
 };
 
 /* ---- random numbers: published wyrand, counter form ------------------- */

@@ -18,6 +18,7 @@ SQRL2, JSD, MKL, REVERSE_MKL = 2, 4, 5, 11
 (L1, L2, JSM, HELLINGER, BHATTACHARYYA_METRIC, BHATTACHARYYA_DISTANCE, TVD, LLR, UWLLR, IS, REVERSE_IS,
  COSINE_DISTANCE, PROBABILITY_COSINE_DISTANCE, COSINE_SIMILARITY, PROBABILITY_COSINE_SIMILARITY, SIS, RSIS, SRLRT,
  SRULRT) = 0, 1, 3, 6, 7, 8, 9, 10, 12, 13, 14, 15, 16, 17, 18, 23, 24, 25, 26
+POISSON = 64   # I-divergence (poisson_bregman); no code in the reference
 MORE_MEASURES = (L1, L2, JSM, HELLINGER, BHATTACHARYYA_METRIC, BHATTACHARYYA_DISTANCE, TVD, LLR, UWLLR, IS,
                  REVERSE_IS, COSINE_DISTANCE, PROBABILITY_COSINE_DISTANCE, COSINE_SIMILARITY,
                  PROBABILITY_COSINE_SIMILARITY, SIS, RSIS, SRLRT, SRULRT)

@@ -9,7 +9,7 @@ from minicore_amd.datagen import poisson_csr
 
 pytestmark = pytest.mark.gpu
 
-MORE = [0, 1, 3, 6, 7, 8, 9, 10, 12, 13, 14, 15, 16, 17, 18, 23, 24, 25, 26]
+MORE = [0, 1, 3, 6, 7, 8, 9, 10, 12, 13, 14, 15, 16, 17, 18, 23, 24, 25, 26, 64]
 
 
 def say(*a):
@@ -49,7 +49,7 @@ def test_cost_matrix_csr(mcg, ctx, mco, code):
     _check(got, ref, code)
 
 
-@pytest.mark.parametrize("code", [6, 13, 10, 23, 15])
+@pytest.mark.parametrize("code", [6, 13, 10, 23, 15, 64])
 def test_assign_and_lloyd_mean_measures(mcg, ctx, mco, code):
     say(f"assign + lloyd msr={code}")
     rng = np.random.default_rng(code)

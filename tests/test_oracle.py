@@ -173,6 +173,35 @@ def test_more_measures_identities(mco):
     assert mco.msr_dense(mco.BHATTACHARYYA_DISTANCE, x, c, p) == pytest.approx(-np.log(bc), rel=1e-10)
 
 
+def test_poisson_divergence(mco):
+    """The I-divergence (poisson_bregman, distance.h:576-585): the sum of scipy's
+    elementwise kl_div on the prior-shifted vectors; sparse == dense; 0 on itself; and,
+    a Bregman divergence, the mean is the best center (cost rises along any
+    perturbation of it)."""
+    from scipy.special import kl_div
+    rng = np.random.default_rng(7)
+    d = 40
+    x = rng.poisson(2., d).astype(float)
+    c = rng.poisson(2., d).astype(float)
+    for prior in (0.5, 1e-3):
+        ref = float(kl_div(x + prior, c + prior).sum())
+        assert mco.msr_dense(mco.POISSON, x, c, prior) == pytest.approx(ref, rel=1e-12)
+        xi, ci = np.nonzero(x)[0], np.nonzero(c)[0]
+        v, _ = mco.msr_sparse(mco.POISSON, x[xi], xi, c[ci], ci, d, prior)
+        assert v == pytest.approx(ref, rel=1e-12)
+        assert mco.msr_dense(mco.POISSON, x, x, prior) == 0.
+    # zero prior: a coordinate with x > 0 = c is infinite (clamped to FLT_MAX)
+    assert mco.msr_dense(mco.POISSON, np.array([1., 0.]), np.array([0., 1.]), 0.) == pytest.approx(3.4028234663852886e38)
+    X = rng.gamma(2., 1., (200, 6))
+    m = X.mean(0)
+    f = lambda cc: sum(mco.msr_dense(mco.POISSON, r, cc, 0.1) for r in X)
+    base = f(m)
+    for j in range(6):
+        for eps in (1e-3, -1e-3):
+            e = np.zeros(6); e[j] = eps
+            assert f(m + e) > base
+
+
 def test_sqrl2_f32_is_sequential_fma(mco):
     rng = np.random.default_rng(1)
     x = rng.standard_normal(77).astype(np.float32)

@@ -240,7 +240,8 @@ int mcg_soft_em(mcg_data *ds, int msr, double prior, double temp, float *centers
 
 /* ---- seeding and coresets ------------------------------------------------ */
 /* D^2 seeding: coresets::kmeanspp, include/minicore/optim/kmeans.h:49-215,
- * as bound by python/smw.h:179-410 (n_local_trials = 1, lspp = 0).
+ * as bound by python/smw.h:179-410 (n_local_trials = 1, lspp = 0; mcg_kmeanspp_ls and
+ * mcg_kmeanspp_local take the other two).
  * centers u32[k] (row indices), asn u32[n], costs f32[n].  weights nullable. */
 int mcg_kmeanspp(mcg_data *ds, int msr, double prior, uint64_t k, uint64_t seed,
                  const double *weights, uint32_t *centers, uint32_t *asn, float *costs);
@@ -257,6 +258,20 @@ int mcg_kmeanspp(mcg_data *ds, int msr, double prior, uint64_t k, uint64_t seed,
  * matters only for asymmetric measures). */
 int mcg_kmeanspp_ls(mcg_data *ds, int msr, double prior, uint64_t k, uint64_t seed, const double *weights,
                     uint64_t rounds, uint32_t *centers, uint32_t *asn, float *costs);
+
+/* D^2 seeding with `trials` local trials per center (coresets::kmeanspp with
+ * n_local_samples = trials, kmeans.h:79-92, 114-177; the bindings' n_local_trials,
+ * python/smw.h:181-249), then `rounds` rounds of local search++ as in
+ * mcg_kmeanspp_ls.  trials = 1 is mcg_kmeanspp (then mcg_kmeanspp_ls).  With
+ * trials > 1: the first center is the cheapest of `trials` uniform rows; each later
+ * round draws `trials` rows from w * dist (outputs 0 .. trials-1 of the wyrand
+ * stream seeded by the round's draw) and keeps the one whose update gives the lowest
+ * exact total (the first on ties).  As in the reference, candidates after the first
+ * start from the distances and assignment saved right after the first center, and
+ * the winner's are kept.  The device holds `trials` copies of the distances and
+ * assignment. */
+int mcg_kmeanspp_local(mcg_data *ds, int msr, double prior, uint64_t k, uint64_t seed, const double *weights,
+                       uint64_t trials, uint64_t rounds, uint32_t *centers, uint32_t *asn, float *costs);
 /* Sensitivities: CoresetSampler::make_sampler, coreset.h:386-578.
  * costs f64[n], asn u32[n], weights f64[n] or NULL; probs f32[n]. */
 int mcg_coreset_probs(mcg_ctx *ctx, int sens, uint64_t n, uint64_t ncenters,

@@ -24,7 +24,7 @@ EXPORTS = [
     "mcg_kmeanspp", "mcg_coreset_probs", "mcg_coreset_sample", "mcg_lloyd_step", "mcg_timing_enable",
     "mcg_timing_reset", "mcg_timing_read", "mcg_refine_counts", "mcg_data_rows", "mcg_server_create",
     "mcg_server_destroy", "mcg_submit", "mcg_poll", "mcg_wait", "mcg_cancel", "mcg_release", "mcg_tuning_enable",
-    "mcg_graphs_enable", "mcg_minibatch", "mcg_cs_minibatch", "mcg_kmeanspp_ls", "mcg_sampler_write",
+    "mcg_graphs_enable", "mcg_minibatch", "mcg_cs_minibatch", "mcg_kmeanspp_ls", "mcg_kmeanspp_local", "mcg_sampler_write",
     "mcg_sampler_read", "mcg_index_coreset_write", "mcg_index_coreset_read", "mcg_pairwise",
 ]
 
@@ -114,6 +114,7 @@ def load():
         "mcg_minibatch": (I, [P, I, D, P, U64, P, U64, U64, U64, U64, I, U64, I, P, P, P, P, P]),
         "mcg_cs_minibatch": (I, [P, I, D, P, U64, P, U64, U64, U64, U64, U64, I, P, P, P, P, P]),
         "mcg_kmeanspp_ls": (I, [P, I, D, U64, U64, P, U64, P, P, P]),
+        "mcg_kmeanspp_local": (I, [P, I, D, U64, U64, P, U64, U64, P, P, P]),
         "mcg_sampler_write": (I, [C.c_char_p, U64, U64, P, P]),
         "mcg_sampler_read": (I, [C.c_char_p, P, P, P, P, P, U64]),
         "mcg_index_coreset_write": (I, [C.c_char_p, U64, P, I, P, I]),
@@ -390,14 +391,18 @@ class Dataset:
         check(load().mcg_data_rows(self.h, ptr(idx), len(idx), C.byref(h)))
         return Dataset(self.ctx, h)
 
-    def kmeanspp(self, msr, k, prior=0., seed=0, weights=None, lspp=0):
-        """D^2 seeding (then lspp rounds of local search++): centers, asn, costs."""
+    def kmeanspp(self, msr, k, prior=0., seed=0, weights=None, lspp=0, n_local_trials=1):
+        """D^2 seeding (n_local_trials candidates per center, then lspp rounds of local
+        search++): centers, asn, costs."""
         w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
         kk = min(k, self.n)
         centers = np.empty(kk, dtype=np.uint32)
         asn = np.empty(self.n, dtype=np.uint32)
         costs = np.empty(self.n, dtype=np.float32)
-        if lspp:
+        if n_local_trials > 1:
+            check(load().mcg_kmeanspp_local(self.h, msr, prior, k, seed, ptr(w), n_local_trials, lspp, ptr(centers),
+                                            ptr(asn), ptr(costs)))
+        elif lspp:
             check(load().mcg_kmeanspp_ls(self.h, msr, prior, k, seed, ptr(w), lspp, ptr(centers), ptr(asn),
                                          ptr(costs)))
         else:

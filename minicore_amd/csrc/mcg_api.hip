@@ -1456,7 +1456,46 @@ int mcg_kmeanspp_ls(mcg_data *ds, int msr, double prior, uint64_t k, uint64_t se
         float *Dd = ok.dev ? (float *)ok.dev : (float *)c.s[S_COSTS].reserve(n * 4);
         std::vector<uint32_t> ch(k);
         kmeanspp_run(D, msr, prior, k, seed, W, ch.data(), A, Dd, st);
-        localsearch_run(D, msr, prior, k, seed, W, rounds, ch.data(), A, Dd, st);
+        localsearch_run(D, msr, prior, k, seed, W, rounds, k, ch.data(), A, Dd, st);
+        if (is_device_ptr(centers)) HIP_CHECK(hipMemcpyAsync(centers, ch.data(), k * 4, hipMemcpyHostToDevice, st));
+        else std::memcpy(centers, ch.data(), k * 4);
+        out_finish(c, oa);
+        out_finish(c, ok);
+        sync(c);
+    });
+}
+
+int mcg_kmeanspp_local(mcg_data *ds, int msr, double prior, uint64_t k, uint64_t seed, const double *weights,
+                       uint64_t trials, uint64_t rounds, uint32_t *centers, uint32_t *asn, float *costs) {
+    return guard([&] {
+        REQUIRE(ds && centers, "null argument");
+        REQUIRE(k > 0, "k must be positive");
+        REQUIRE(trials >= 1, "n_local_trials must be positive");
+        check_msr(msr);
+        Data &D = ds->d;
+        Ctx &c = *D.ctx;
+        hipStream_t st = c.stream;
+        HIP_CHECK(hipSetDevice(c.device));
+        const uint64_t n = D.n;
+        if (k > n) k = n;
+        REQUIRE(trials == 1 || (double)n * (double)(trials + 3) * 8. < 64e9,
+                "local trials keep n_local_trials copies of the distances and assignment: too large");
+        REQUIRE(rounds == 0 || (double)n * (double)k * 4. < 64e9, "local search++ keeps an n x k cost matrix: too large");
+        DBuf sw, sa, sk;
+        const double *W = (const double *)in_dev(c, weights, n * 8, sw);
+        Out oa = out_dev(c, asn, n * 4, sa);
+        Out ok = out_dev(c, costs, n * 4, sk);
+        uint32_t *A = oa.dev ? (uint32_t *)oa.dev : (uint32_t *)c.s[S_ASN_A].reserve(n * 4);
+        float *Dd = ok.dev ? (float *)ok.dev : (float *)c.s[S_COSTS].reserve(n * 4);
+        std::vector<uint32_t> ch(k);
+        uint64_t j0 = k;   // stream outputs the seeding used
+        if (trials > 1) {
+            kmeanspp_local_run(D, msr, prior, k, seed, W, trials, ch.data(), A, Dd, st);
+            j0 = trials + k - 1;
+        } else {
+            kmeanspp_run(D, msr, prior, k, seed, W, ch.data(), A, Dd, st);
+        }
+        localsearch_run(D, msr, prior, k, seed, W, rounds, j0, ch.data(), A, Dd, st);
         if (is_device_ptr(centers)) HIP_CHECK(hipMemcpyAsync(centers, ch.data(), k * 4, hipMemcpyHostToDevice, st));
         else std::memcpy(centers, ch.data(), k * 4);
         out_finish(c, oa);

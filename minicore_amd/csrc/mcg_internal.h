@@ -244,9 +244,13 @@ void soft_mstep(Data &ds, const float *resp, const double *w, uint64_t k, float 
 // Sampling.
 void kmeanspp_run(Data &ds, int msr, double prior, uint64_t k, uint64_t seed, const double *w,
                   uint32_t *centers_h, uint32_t *asn_d, float *dist_d, hipStream_t st);
-// Local search++ rounds after kmeanspp_run (centers_h, asn_d, dist_d in/out).
+// D^2 seeding with L > 1 local trials per center (kmeans.h:79-92, 114-177).
+void kmeanspp_local_run(Data &ds, int msr, double prior, uint64_t k, uint64_t seed, const double *w, uint64_t L,
+                        uint32_t *centers_h, uint32_t *asn_d, float *dist_d, hipStream_t st);
+// Local search++ rounds after the seeding (centers_h, asn_d, dist_d in/out); the
+// draws continue the seeding's stream at output j0.
 void localsearch_run(Data &ds, int msr, double prior, uint64_t k, uint64_t seed, const double *w, uint64_t rounds,
-                     uint32_t *centers_h, uint32_t *asn_d, float *dist_d, hipStream_t st);
+                     uint64_t j0, uint32_t *centers_h, uint32_t *asn_d, float *dist_d, hipStream_t st);
 void coreset_probs_run(Ctx &ctx, int sens, uint64_t n, uint64_t ncenters, const double *costs_d,
                        const uint32_t *asn_d, const double *w_d, float *probs_d, hipStream_t st);
 void coreset_sample_run(Ctx &ctx, uint64_t n, const float *probs_d, const double *w_d, uint64_t m,

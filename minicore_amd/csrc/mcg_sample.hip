@@ -525,6 +525,160 @@ void kmeanspp_run(Data &ds, int msr, double prior, uint64_t k, uint64_t seed, co
     HIP_CHECK(hipStreamSynchronize(st));
 }
 
+// ---- D^2 seeding with local trials (kmeans.h:79-92, 114-177) ----------------------
+// Costs of every row to row *row_p for dense SQRL2: the sequential FMA chain of
+// k_pp_update_l2 (rows staged through LDS in 32-column slices), written to out.
+__global__ __launch_bounds__(256) void k_cost_row_l2(const float *__restrict__ X, uint64_t n, uint32_t d,
+                                                     const uint64_t *__restrict__ row_p, float *__restrict__ out) {
+    __shared__ float xs[256][33];
+    __shared__ float cs[32];
+    const int tid = threadIdx.x;
+    const uint64_t p0 = (uint64_t)blockIdx.x * 256, p = p0 + tid;
+    const float *cr = X + *row_p * d;
+    float s = 0.f;
+    for (uint32_t j0 = 0; j0 < d; j0 += 32) {
+        __syncthreads();
+        for (int q = tid; q < 256 * 32; q += 256) {
+            const int r = q >> 5, jj = q & 31;
+            const uint64_t pr = p0 + r;
+            xs[r][jj] = (pr < n && j0 + jj < d) ? X[pr * d + j0 + jj] : 0.f;
+        }
+        if (tid < 32) cs[tid] = j0 + tid < d ? cr[j0 + tid] : 0.f;
+        __syncthreads();
+        const int jn = (int)min<uint32_t>(32, d - j0);
+        for (int jj = 0; jj < jn; ++jj) { const float t = xs[tid][jj] - cs[jj]; s = fmaf(t, t, s); }
+    }
+    if (p < n) out[p] = s;
+}
+
+// First-center candidate: out_i = cost_i, 0 for the candidate's own row (setdists).
+__global__ void k_local_first(const float *__restrict__ cost, uint64_t n, const uint64_t *__restrict__ row_p,
+                              float *__restrict__ out) {
+    const uint64_t r = *row_p;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = i == r ? 0.f : cost[i];
+}
+
+// One candidate of a round: out_i = cost_i, asn_out_i = ci where cost_i < base_i,
+// else base (every row, the candidate's own included).  base and out may alias.
+__global__ void k_local_merge(const float *cost, const float *base, const uint32_t *base_a, uint64_t n, uint32_t ci,
+                              float *out, uint32_t *out_a) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const float v = cost[i], b = base[i];
+        const uint32_t a = base_a[i];
+        if (v < b) { out[i] = v; out_a[i] = ci; }
+        else { out[i] = b; out_a[i] = a; }
+    }
+}
+
+// The new center: its assignment is its round, its distance 0.
+__global__ void k_local_settle(uint64_t newc, uint32_t ci, float *__restrict__ dist, uint32_t *__restrict__ asn) {
+    dist[newc] = 0.f;
+    asn[newc] = ci;
+}
+
+void kmeanspp_local_run(Data &ds, int msr, double prior, uint64_t k, uint64_t seed, const double *w, uint64_t L,
+                        uint32_t *centers_h, uint32_t *asn_d, float *dist_d, hipStream_t st) {
+    const uint64_t n = ds.n;
+    const uint32_t d = (uint32_t)ds.d;
+    const bool l2 = !ds.csr && msr == MCG_SQRL2;
+    const uint32_t nblk = (uint32_t)cdiv(n, FX_BLK);
+    // candidate t > 0 of a round keeps its distances and assignment in slot t - 1
+    DBuf bCand, bOd, bOa, bTd, bTa, bRow, bScr, bMisc;
+    float *cand = (float *)bCand.reserve(n * 4), *od = (float *)bOd.reserve(n * 4);
+    uint32_t *oa = (uint32_t *)bOa.reserve(n * 4), *scr = (uint32_t *)bScr.reserve(n * 4);
+    float *td = (float *)bTd.reserve((L - 1) * n * 4);
+    uint32_t *ta = (uint32_t *)bTa.reserve((L - 1) * n * 4);
+    float *row = (float *)bRow.reserve(d * 4);
+    // smp[L] | mx of the draw | mx[L] of the candidates | bsum of the draw | bsum[L][nblk]
+    uint64_t *smp = (uint64_t *)bMisc.reserve((2 * L + 2 + (L + 1) * (uint64_t)nblk) * 8);
+    unsigned long long *mxs = (unsigned long long *)(smp + L), *mxc = mxs + 2;
+    uint64_t *bss = (uint64_t *)(mxc + L), *bsc = bss + nblk;
+    const uint32_t pgrid = (uint32_t)std::min<uint64_t>(cdiv(n, 256), 2048);
+    const uint32_t agrid = (uint32_t)std::min<uint64_t>(cdiv(n, 256), 1024);
+    auto costs_to = [&](uint64_t t) {   // cand = cost of every row to row smp[t]
+        if (l2) {
+            hipLaunchKernelGGL(k_cost_row_l2, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, ds.X, n, d,
+                               (const uint64_t *)(smp + t), cand);
+        } else {
+            hipLaunchKernelGGL(k_row_to_dense, dim3(1), dim3(256), 0, st, ds.X,
+                               ds.csr ? ds.val.as<const float>() : nullptr,
+                               ds.csr ? ds.idx.as<const uint32_t>() : nullptr,
+                               ds.csr ? ds.ptr.as<const uint64_t>() : nullptr, d, (const uint64_t *)(smp + t), row);
+            generic_costs(ds, msr, prior, row, 1, nullptr, scr, cand, st);
+        }
+    };
+    auto sum_into = [&](uint64_t t, const float *v) {   // fixed-point total of w * v into slot t
+        hipLaunchKernelGGL(k_absmax<float>, dim3(agrid), dim3(256), 0, st, v, w, n, mxc + t);
+        hipLaunchKernelGGL(k_fixed_blocksum<float>, dim3(nblk), dim3(256), 0, st, v, w, n,
+                           (const unsigned long long *)(mxc + t), bsc + t * nblk);
+    };
+    std::vector<uint64_t> h(2 * L + 2 + (L + 1) * (uint64_t)nblk);
+    // exact totals of the L slots and the drawn rows; returns the first strict minimum
+    auto pick = [&]() {
+        copy_sync(h.data(), smp, h.size() * 8, hipMemcpyDeviceToHost, st);
+        uint64_t best = 0;
+        double bv = 0.;
+        for (uint64_t t = 0; t < L; ++t) {
+            double mv;
+            std::memcpy(&mv, &h[L + 2 + t], 8);
+            const uint64_t *bs = h.data() + 2 * L + 2 + nblk + t * nblk;
+            uint64_t T = 0;
+            for (uint32_t b = 0; b < nblk; ++b) T += bs[b];
+            const double v = ldexp((double)T, -fixed_shift(mv, n));
+            if (t == 0 || v < bv) { bv = v; best = t; }
+        }
+        return best;
+    };
+    uint64_t j = 0;
+    // first center: the cheapest of L uniform rows
+    fill_u32((uint32_t *)mxc, 0u, 2 * L, st);
+    for (uint64_t t = 0; t < L; ++t) {
+        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, smp + t, wyrand_at(seed, j++) % n);
+        costs_to(t);
+        float *out = t ? td + (t - 1) * n : dist_d;
+        hipLaunchKernelGGL(k_local_first, dim3(pgrid), dim3(256), 0, st, (const float *)cand, n,
+                           (const uint64_t *)(smp + t), out);
+        sum_into(t, out);
+    }
+    uint64_t b = pick();
+    centers_h[0] = (uint32_t)h[b];
+    if (b) HIP_CHECK(hipMemcpyAsync(dist_d, td + (b - 1) * n, n * 4, hipMemcpyDeviceToDevice, st));
+    fill_u32(asn_d, 0u, n, st);
+    fill_u32(oa, 0u, n, st);
+    HIP_CHECK(hipMemcpyAsync(od, dist_d, n * 4, hipMemcpyDeviceToDevice, st));   // never refreshed (kmeans.h:104)
+    for (uint64_t ci = 1; ci < k; ++ci) {
+        const uint64_t rngv = wyrand_at(seed, j++);
+        // L draws over w * dist with outputs 0 .. L-1 of the stream seeded by rngv
+        fill_u32((uint32_t *)mxs, 0u, 4 + 2 * L, st);   // the draw's mx (and pad), the candidates'
+        hipLaunchKernelGGL(k_absmax<float>, dim3(agrid), dim3(256), 0, st, (const float *)dist_d, w, n, mxs);
+        hipLaunchKernelGGL(k_fixed_blocksum<float>, dim3(nblk), dim3(256), 0, st, (const float *)dist_d, w, n,
+                           (const unsigned long long *)mxs, bss);
+        for (uint64_t t = 0; t < L; ++t)
+            hipLaunchKernelGGL(k_fixed_select<float>, dim3(1), dim3(1024), 0, st, (const float *)dist_d, w, n,
+                               (const unsigned long long *)mxs, (const uint64_t *)bss, nblk, wyrand_at(rngv, t),
+                               smp + t);
+        for (uint64_t t = 0; t < L; ++t) {
+            costs_to(t);
+            float *out = t ? td + (t - 1) * n : dist_d;
+            uint32_t *out_a = t ? ta + (t - 1) * n : asn_d;
+            hipLaunchKernelGGL(k_local_merge, dim3(pgrid), dim3(256), 0, st, (const float *)cand,
+                               (const float *)(t ? od : dist_d), (const uint32_t *)(t ? oa : asn_d), n,
+                               (uint32_t)ci, out, out_a);
+            sum_into(t, out);
+        }
+        b = pick();
+        const uint64_t newc = h[b];
+        centers_h[ci] = (uint32_t)newc;
+        if (b) {
+            HIP_CHECK(hipMemcpyAsync(dist_d, td + (b - 1) * n, n * 4, hipMemcpyDeviceToDevice, st));
+            HIP_CHECK(hipMemcpyAsync(asn_d, ta + (b - 1) * n, n * 4, hipMemcpyDeviceToDevice, st));
+        }
+        hipLaunchKernelGGL(k_local_settle, dim3(1), dim3(1), 0, st, newc, (uint32_t)ci, dist_d, asn_d);
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
+}
+
 // ---- local search++ (lsearchpp.h:21-121) ---------------------------------------
 // M[i][y] = cost of row i to center row ctrs[y] (the seeding's orientation), n x k
 // floats; dense SQRL2 by the sequential FMA chain, other cases by the cost kernels.
@@ -607,7 +761,7 @@ __global__ void k_ls_sums(const float *__restrict__ t, const float *__restrict__
 }
 
 void localsearch_run(Data &ds, int msr, double prior, uint64_t k, uint64_t seed, const double *w, uint64_t rounds,
-                     uint32_t *centers_h, uint32_t *asn_d, float *dist_d, hipStream_t st) {
+                     uint64_t j0, uint32_t *centers_h, uint32_t *asn_d, float *dist_d, hipStream_t st) {
     if (rounds == 0 || k < 2) return;
     Ctx &c = *ds.ctx;
     const uint64_t n = ds.n;
@@ -643,7 +797,7 @@ void localsearch_run(Data &ds, int msr, double prior, uint64_t k, uint64_t seed,
         hipLaunchKernelGGL(k_set_column, dim3(pgrid), dim3(256), 0, st, M, n, (uint32_t)k, (uint32_t)y, t);
     }
     hipLaunchKernelGGL(k_row_min, dim3(pgrid), dim3(256), 0, st, M, n, (uint32_t)k, dist_d, (uint32_t *)nullptr);
-    uint64_t j = k;   // the seeding used the stream's outputs 0 .. k-1
+    uint64_t j = j0;   // the seeding used the stream's outputs 0 .. j0-1
     std::vector<unsigned long long> hcf(k + 1);
     for (uint64_t rd = 0; rd < rounds; ++rd) {
         // the draw: over the distances, or over w times the last round's new costs

@@ -532,13 +532,16 @@ struct PPResult {
 
 // D^2 seeding, then lspp rounds of local search++ (kmeans.h:209-212)
 PPResult run_pp(mcg_data *h, uint64_t n, int m, uint64_t k, double prior, uint64_t seed, const double *w,
-                uint64_t lspp) {
+                uint64_t lspp, uint64_t trials) {
     PPResult r;
     const uint64_t kk = std::min(k, n);
     r.centers.resize(kk);
     r.asn.resize(n);
     r.costs.resize(n);
-    if (lspp)
+    if (trials > 1)
+        check(mcg_kmeanspp_local(h, m, prior, k, seed, w, trials, lspp, r.centers.data(), r.asn.data(),
+                                 r.costs.data()));
+    else if (lspp)
         check(mcg_kmeanspp_ls(h, m, prior, k, seed, w, lspp, r.centers.data(), r.asn.data(), r.costs.data()));
     else
         check(mcg_kmeanspp(h, m, prior, k, seed, w, r.centers.data(), r.asn.data(), r.costs.data()));
@@ -546,28 +549,29 @@ PPResult run_pp(mcg_data *h, uint64_t n, int m, uint64_t k, double prior, uint64
     return r;
 }
 
-void check_pp_args(py::ssize_t lspp, bool expskips, py::ssize_t n_local_trials) {
+// n_local_trials <= 1 is the single-sample path (kmeans.h:114-186 tests n_local_samples <= 1)
+uint64_t check_pp_args(py::ssize_t lspp, bool expskips, py::ssize_t n_local_trials) {
     if (lspp < 0) throw py::value_error("lspp must be >= 0");
-    if (n_local_trials > 1) throw py::value_error("n_local_trials > 1 is not on the GPU path");
     (void)expskips;  // sampling method choice; the GPU path has one exact sampler
+    return n_local_trials > 1 ? (uint64_t)n_local_trials : 1;
 }
 
 uint64_t pp_seed(uint64_t seed) { return seed ? seed : std::mt19937_64(std::rand())(); }
 
 py::object kmeanspp_dense(py::array arr, int k, py::object msr, py::object prior, py::object seed, py::object ntimes,
                           py::ssize_t lspp, py::object weights, bool expskips, py::ssize_t n_local_trials) {
-    check_pp_args(lspp, expskips, n_local_trials);
+    const uint64_t trials = check_pp_args(lspp, expskips, n_local_trials);
     const int m = measure_code(msr);
     Dense D = dense_of(arr);
     double pr = prior.cast<double>();
     if (pr < 0.) pr = 1. / (double)D.d;
     uint64_t s = pp_seed(seed.cast<uint64_t>());
     auto w = weights_f64(weights, D.n);
-    PPResult best = run_pp(D.dev->h, D.n, m, (uint64_t)k, pr, s, wptr(w), (uint64_t)lspp);
+    PPResult best = run_pp(D.dev->h, D.n, m, (uint64_t)k, pr, s, wptr(w), (uint64_t)lspp, trials);
     const unsigned nt = ntimes.cast<unsigned>();
     for (unsigned t = 0; t < nt; ++t) {  // extra runs keep the cheapest (smw.h:385-393)
         PPResult r = run_pp(D.dev->h, D.n, m, (uint64_t)k, pr, s + 0x9e3779b97f4a7c15ULL * (t + 1), wptr(w),
-                            (uint64_t)lspp);
+                            (uint64_t)lspp, trials);
         if (r.total < best.total) best = std::move(r);
     }
     py::array_t<uint32_t> ctr(best.centers.size());
@@ -581,12 +585,12 @@ py::object kmeanspp_csr(PyCSparseMatrix &smw, py::object msr, py::int_ k, double
                         py::ssize_t lspp, bool expskips, py::ssize_t n_local_trials, py::object weights) {
     // Extra runs (ntimes) never replace the first in the reference (smw.h:248-256 compares the
     // first run's cost with itself), so the first run is the result.
-    check_pp_args(lspp, expskips, n_local_trials);
+    const uint64_t trials = check_pp_args(lspp, expskips, n_local_trials);
     const int m = measure_code(msr);
     if (prior < 0.) prior = 1. / (double)smw.nc;
     auto w = weights_f64(weights, smw.nr);
     const uint64_t kk = k.cast<uint64_t>();
-    PPResult r = run_pp(smw.device(), smw.nr, m, kk, prior, pp_seed(seed), wptr(w), (uint64_t)lspp);
+    PPResult r = run_pp(smw.device(), smw.nr, m, kk, prior, pp_seed(seed), wptr(w), (uint64_t)lspp, trials);
     py::array_t<uint32_t> ctr(r.centers.size());
     std::memcpy(ctr.mutable_data(), r.centers.data(), r.centers.size() * 4);
     py::array_t<double> costs(smw.nr);

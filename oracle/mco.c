@@ -706,7 +706,7 @@ static double fixed_total(const double *cost, const double *w, uint64_t n) {
     for (uint64_t i = 0; i < n; ++i) { t[i] = w ? w[i] * cost[i] : cost[i]; if (fabs(t[i]) > mx) mx = fabs(t[i]); }
     const int s = mco_fixed_shift(mx, n);
     uint64_t T = 0;
-    for (uint64_t i = 0; i < n; ++i) T += (uint64_t)floor(ldexp(t[i], s));
+    for (uint64_t i = 0; i < n; ++i) if (t[i] > 0.) T += (uint64_t)floor(ldexp(t[i], s));
     free(t);
     return ldexp((double)T, -s);
 }
@@ -1096,12 +1096,11 @@ static double row_pair_cost(int msr, const float *X, const float *val, const uin
     return pair_rows(msr, X, val, idx, indptr, d, i, c, rsum, prior);
 }
 
-uint64_t mco_kmeanspp_ls(int msr, const float *X, const float *val, const uint32_t *idx,
-                         const uint64_t *indptr, uint64_t n, uint64_t d, uint64_t k,
-                         double prior, uint64_t seed, const double *w, uint64_t rounds,
-                         uint32_t *centers, uint32_t *asn, float *distf, int nthreads) {
-    k = mco_kmeanspp(msr, X, val, idx, indptr, n, d, k, prior, seed, w, centers, asn, distf, nthreads);
-    if (rounds == 0 || k < 2) return k;
+/* rounds of local search++ on seeded centers; the draws continue the seeding's
+ * stream at output j0 */
+static void ls_rounds(int msr, const float *X, const float *val, const uint32_t *idx, const uint64_t *indptr,
+                      uint64_t n, uint64_t d, uint64_t k, double prior, uint64_t seed, const double *w,
+                      uint64_t rounds, uint64_t j0, uint32_t *centers, uint32_t *asn, float *distf) {
     double *rsum = (double *)malloc(n * sizeof(double));
     for (uint64_t i = 0; i < n; ++i)
         rsum[i] = val ? row_sum_csr(val, indptr[i], indptr[i + 1]) : row_sum_dense(X + i * d, d);
@@ -1120,7 +1119,7 @@ uint64_t mco_kmeanspp_ls(int msr, const float *X, const float *val, const uint32
         for (uint64_t y = 1; y < k; ++y) if (M[i * k + y] < m) m = M[i * k + y];
         dist[i] = m;
     }
-    uint64_t j = k;   /* the seeding used outputs 0..k-1 */
+    uint64_t j = j0;
     for (uint64_t rd = 0; rd < rounds; ++rd) {
         for (uint64_t i = 0; i < n; ++i) wv[i] = w ? w[i] * (double)nc[i] : (double)dist[i];
         mco_fixed_weights(wv, n, fw);
@@ -1177,6 +1176,15 @@ uint64_t mco_kmeanspp_ls(int msr, const float *X, const float *val, const uint32
         distf[i] = dist[i];
     }
     free(rsum); free(M); free(dist); free(nc); free(t); free(g); free(top); free(wv); free(fw); free(cf);
+}
+
+uint64_t mco_kmeanspp_ls(int msr, const float *X, const float *val, const uint32_t *idx,
+                         const uint64_t *indptr, uint64_t n, uint64_t d, uint64_t k,
+                         double prior, uint64_t seed, const double *w, uint64_t rounds,
+                         uint32_t *centers, uint32_t *asn, float *distf, int nthreads) {
+    k = mco_kmeanspp(msr, X, val, idx, indptr, n, d, k, prior, seed, w, centers, asn, distf, nthreads);
+    /* the seeding used the stream's outputs 0..k-1 */
+    if (rounds > 0 && k >= 2) ls_rounds(msr, X, val, idx, indptr, n, d, k, prior, seed, w, rounds, k, centers, asn, distf);
     return k;
 }
 
@@ -1554,4 +1562,85 @@ int mco_lloyd_median(int msr, const float *X, const float *val, const uint32_t *
     *finalcost = fcost;
     *numiter = iter;
     return rc;
+}
+
+/* ======================================================================
+ * k-means++ with n_local_samples = L > 1 (kmeans.h:79-92, 114-177):
+ *  - the first center is the cheapest (exact fixed-point total of w_i * d_i) of
+ *    L rows rng() mod n, the first on ties;
+ *  - the distances and assignment right after it are kept as the reference's
+ *    odists / oasn, which it never refreshes (kmeans.h:104);
+ *  - each later round takes one raw draw r = rng() and L samples from the
+ *    fixed-point sampler over w_i * d_i with the outputs 0..L-1 of the stream
+ *    seeded by r (reservoir_simd::sample_k is un-vendored: parity unpinned; with
+ *    replacement here); candidate 0 updates the current distances in place,
+ *    candidates t > 0 start from odists / oasn; the lowest total wins (the first
+ *    on ties), and the winner's distances and assignment are kept;
+ *  - the new center's assignment is its round and its distance 0.
+ * Costs are the seeding's (pair_rows, row first) for every row, the new center's
+ * own row included.  `rounds` rounds of local search++ follow (kmeans.h:209-212).
+ * ====================================================================== */
+uint64_t mco_kmeanspp_local(int msr, const float *X, const float *val, const uint32_t *idx,
+                            const uint64_t *indptr, uint64_t n, uint64_t d, uint64_t k, double prior,
+                            uint64_t seed, const double *w, uint64_t L, uint64_t rounds, uint32_t *centers,
+                            uint32_t *asn, float *distf) {
+    if (n < k) k = n;
+    if (k == 0 || L < 1) return 0;
+    double *rsum = (double *)malloc(n * sizeof(double));
+    for (uint64_t i = 0; i < n; ++i)
+        rsum[i] = val ? row_sum_csr(val, indptr[i], indptr[i + 1]) : row_sum_dense(X + i * d, d);
+    double *dist = (double *)malloc(n * sizeof(double)), *od = (double *)malloc(n * sizeof(double));
+    double *tmp = (double *)malloc(n * sizeof(double)), *rv = (double *)malloc(n * sizeof(double));
+    uint32_t *oa = (uint32_t *)malloc(n * sizeof(uint32_t)), *ta = (uint32_t *)malloc(n * sizeof(uint32_t));
+    uint64_t *fw = (uint64_t *)malloc(n * sizeof(uint64_t)), *smp = (uint64_t *)malloc(L * sizeof(uint64_t));
+    uint64_t j = 0;
+    /* first center: the cheapest of L uniform rows (setdists: its own distance 0) */
+    uint64_t fc = mco_wyrand_at(seed, j++) % n;
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i)
+        dist[i] = ((uint64_t)i == fc) ? 0. : pair_rows(msr, X, val, idx, indptr, d, i, fc, rsum, prior);
+    double fcost = fixed_total(dist, w, n);
+    for (uint64_t t = 1; t < L; ++t) {
+        const uint64_t ofc = mco_wyrand_at(seed, j++) % n;
+        #pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < (int64_t)n; ++i)
+            tmp[i] = ((uint64_t)i == ofc) ? 0. : pair_rows(msr, X, val, idx, indptr, d, i, ofc, rsum, prior);
+        const double ocost = fixed_total(tmp, w, n);
+        if (ocost < fcost) { double *s = dist; dist = tmp; tmp = s; fcost = ocost; fc = ofc; }
+    }
+    centers[0] = (uint32_t)fc;
+    for (uint64_t i = 0; i < n; ++i) { asn[i] = 0; od[i] = dist[i]; oa[i] = 0; }
+    for (uint64_t ci = 1; ci < k; ++ci) {
+        const uint64_t rngv = mco_wyrand_at(seed, j++);
+        for (uint64_t i = 0; i < n; ++i) rv[i] = w ? w[i] * dist[i] : dist[i];
+        mco_fixed_weights(rv, n, fw);
+        for (uint64_t t = 0; t < L; ++t) smp[t] = mco_fixed_draw(fw, n, mco_wyrand_at(rngv, t));
+        double dsum = -1.;
+        uint64_t newc = smp[0];
+        for (uint64_t t = 0; t < L; ++t) {
+            double *tg = dist;
+            uint32_t *tga = asn;
+            if (t > 0) { memcpy(tmp, od, n * sizeof(double)); memcpy(ta, oa, n * sizeof(uint32_t)); tg = tmp; tga = ta; }
+            const uint64_t nextc = smp[t];
+            #pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < (int64_t)n; ++i) {
+                const double v = pair_rows(msr, X, val, idx, indptr, d, i, nextc, rsum, prior);
+                if (v < tg[i]) { tg[i] = v; tga[i] = (uint32_t)ci; }
+            }
+            const double nsum = fixed_total(tg, w, n);
+            if (dsum < 0.) { dsum = nsum; newc = nextc; }
+            else if (nsum < dsum) {
+                memcpy(dist, tmp, n * sizeof(double)); memcpy(asn, ta, n * sizeof(uint32_t));
+                dsum = nsum; newc = nextc;
+            }
+        }
+        centers[ci] = (uint32_t)newc;
+        asn[newc] = (uint32_t)ci;
+        dist[newc] = 0.;
+    }
+    for (uint64_t i = 0; i < n; ++i) distf[i] = (float)dist[i];
+    free(rsum); free(dist); free(od); free(tmp); free(rv); free(oa); free(ta); free(fw); free(smp);
+    /* the seeding used the stream's outputs 0..L+k-2 */
+    if (rounds > 0 && k >= 2) ls_rounds(msr, X, val, idx, indptr, n, d, k, prior, seed, w, rounds, L + k - 1, centers, asn, distf);
+    return k;
 }

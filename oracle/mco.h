@@ -132,6 +132,11 @@ uint64_t mco_kmeanspp_ls(int msr, const float *X, const float *val, const uint32
                          const uint64_t *indptr, uint64_t n, uint64_t d, uint64_t k,
                          double prior, uint64_t seed, const double *w, uint64_t rounds,
                          uint32_t *centers, uint32_t *asn, float *dist, int nthreads);
+/* k-means++ with n_local_samples > 1: kmeans.h:79-92, 114-177 */
+uint64_t mco_kmeanspp_local(int msr, const float *X, const float *val, const uint32_t *idx,
+                            const uint64_t *indptr, uint64_t n, uint64_t d, uint64_t k, double prior,
+                            uint64_t seed, const double *w, uint64_t L, uint64_t rounds, uint32_t *centers,
+                            uint32_t *asn, float *distf);
 /* median centroid policies (L1, TVD, L2): centroid.h:434-647 */
 int mco_median_centers(int msr, const float *X, const float *val, const uint32_t *idx, const uint64_t *indptr,
                        uint64_t n, uint64_t d, const double *w, uint64_t k, uint32_t *asn, double *cost, float *C,

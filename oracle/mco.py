@@ -72,6 +72,7 @@ def _setup(L):
         "mco_kmeanspp_ls": (U64, [I, P, P, P, P, U64, U64, U64, D, U64, P, U64, P, P, P, I]),
         "mco_median_centers": (I, [I, P, P, P, P, U64, U64, P, U64, P, P, P, D]),
         "mco_lloyd_median": (I, [I, P, P, P, P, U64, U64, P, U64, D, P, D, U64, P, P, P, P, P]),
+        "mco_kmeanspp_local": (U64, [I, P, P, P, P, U64, U64, U64, D, U64, P, U64, U64, P, P, P]),
         "mco_coreset_probs": (I, [I, U64, U64, P, P, P, P]),
         "mco_coreset_sample": (None, [U64, P, P, U64, U64, U64, P, P]),
     }
@@ -343,6 +344,19 @@ def kmeanspp_ls(msr, X, k, rounds, prior=0., seed=0, w=None, nthreads=0):
     dist = np.zeros(n, dtype=np.float32)
     kk = lib().mco_kmeanspp_ls(msr, _p(Xd), _p(val), _p(idx), _p(ptr), n, d, k, prior, seed, _p(w), rounds,
                                _p(centers), _p(asn), _p(dist), nthreads)
+    return centers[:kk], asn, dist
+
+
+def kmeanspp_local(msr, X, k, trials, prior=0., seed=0, w=None, lspp=0):
+    """k-means++ with n_local_trials = trials > 1 (kmeans.h:79-177), then lspp rounds of
+    local search++."""
+    Xd, val, idx, ptr, n, d = _mat(X)
+    w = _f64(w)
+    centers = np.empty(k, dtype=np.uint32)
+    asn = np.empty(n, dtype=np.uint32)
+    dist = np.empty(n, dtype=np.float32)
+    kk = lib().mco_kmeanspp_local(msr, _p(Xd), _p(val), _p(idx), _p(ptr), n, d, k, prior, seed, _p(w), trials,
+                                  lspp, _p(centers), _p(asn), _p(dist))
     return centers[:kk], asn, dist
 
 

@@ -40,6 +40,49 @@ def test_kmeanspp_duplicate_rows_fallback(mcg, ctx, mco):
     assert np.array_equal(dd, d_ref)
 
 
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("trials,lspp", [(3, 0), (2, 0), (5, 12)])
+def test_kmeanspp_local_trials_bitexact(mcg, ctx, mco, weighted, trials, lspp):
+    """n_local_trials > 1 (kmeans.h:79-177): candidates, stale copies, exact totals and
+    the continued stream of local search++ all equal the oracle."""
+    say(f"kmeans++ local trials={trials} lspp={lspp} weighted={weighted}")
+    n, d, k = 12000, 40, 30
+    X, _, _ = gaussian_mixture(n, d, 30, 4)
+    w = np.random.default_rng(8).uniform(0.5, 2, n) if weighted else None
+    c_ref, a_ref, d_ref = mco.kmeanspp_local(mco.SQRL2, X, k, trials, seed=17, w=w, lspp=lspp)
+    c, a, dd = mcg.Dataset.dense(X, ctx).kmeanspp(mcg.SQRL2, k, seed=17, weights=w, lspp=lspp,
+                                                  n_local_trials=trials)
+    assert np.array_equal(c, c_ref)
+    assert np.array_equal(a, a_ref)
+    assert np.array_equal(dd, d_ref)
+
+
+def test_kmeanspp_local_trials_csr(mcg, ctx, mco):
+    """CSR JSD with local trials: the first center is one of the uniform candidates and
+    each distance is the cost to the assigned center."""
+    say("kmeans++ local trials csr jsd")
+    n, d, k, L = 3000, 300, 10, 3
+    data, idx, ptr = poisson_csr(n, d, 0.04, 6, normalize=True)
+    X = mco.CSR(data, idx, ptr, (n, d))
+    c, a, dist = mcg.Dataset.csr(data, idx, ptr, (n, d), ctx).kmeanspp(4, k, prior=1. / d, seed=5,
+                                                                        n_local_trials=L)
+    assert c[0] in [mco.wyrand_at(5, t) % n for t in range(L)]
+    ref = mco.cost_matrix(4, X, X.todense()[c], 1. / d)
+    np.testing.assert_allclose(dist, ref[np.arange(n), a], rtol=2e-4, atol=2e-6)
+    cr, ar, dr = mco.kmeanspp_local(4, X, k, L, prior=1. / d, seed=5)
+    assert c[0] == cr[0]
+
+
+def test_kmeanspp_local_trials_binding(mco):
+    """pyminicore's kmeanspp(..., n_local_trials=4) runs the local-trial seeding."""
+    import minicore_amd as mc
+    say("api kmeanspp n_local_trials=4")
+    X, _, _ = gaussian_mixture(8000, 24, 20, 9)
+    ids, asn, costs = mc.kmeanspp(X, 20, seed=31, ntimes=0, n_local_trials=4)
+    c_ref, a_ref, d_ref = mco.kmeanspp_local(mco.SQRL2, X, 20, 4, seed=31)
+    assert np.array_equal(ids, c_ref) and np.array_equal(asn, a_ref) and np.array_equal(costs, d_ref)
+
+
 def test_kmeanspp_jsd_pruned_rounds(mcg, ctx, mco):
     """JSD seeding skips rows whose center is far from the new one (sqrt(JSD) is a
     metric); every distance must still be the minimum over the chosen centers."""

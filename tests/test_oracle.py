@@ -479,6 +479,86 @@ def test_kmeanspp_second_center_distribution(mco):
     assert stat < chi2.ppf(0.999, m.sum() - 1)
 
 
+def _local_trials_loops(mco, X, k, L, seed, w):
+    """kmeans.h:79-177 with n_local_samples = L, as plain loops over a tiny input: the
+    cheapest of L uniform first centers, odists / oasn taken once after it, then per
+    round L draws (outputs 0..L-1 of the stream seeded by the round's raw draw) of
+    which candidate 0 updates the distances in place and the others start from the
+    stale copies; the cheapest total wins, the first on ties.  Totals are correctly
+    rounded sums (math.fsum)."""
+    import math
+    n = len(X)
+    cost = lambda i, c: float(mco.sqrl2_f32(X[i], X[c]))
+    tot = lambda dv: math.fsum(dv[i] * (w[i] if w is not None else 1.) for i in range(n))
+    j = 0
+    best = None
+    for t in range(L):
+        fc = mco.wyrand_at(seed, j) % n
+        j += 1
+        dv = [0. if i == fc else cost(i, fc) for i in range(n)]
+        if best is None or tot(dv) < best[0]:
+            best = (tot(dv), fc, dv)
+    centers, dist, asn = [best[1]], list(best[2]), [0] * n
+    od, oa = list(dist), list(asn)
+    for ci in range(1, k):
+        rngv = mco.wyrand_at(seed, j)
+        j += 1
+        fw = mco.fixed_weights(np.array([dist[i] * (w[i] if w is not None else 1.) for i in range(n)]))
+        smp = [mco.fixed_draw(fw, mco.wyrand_at(rngv, t)) for t in range(L)]
+        dsum, newc = None, smp[0]
+        for t in range(L):
+            dv, av = (dist, asn) if t == 0 else (list(od), list(oa))
+            for i in range(n):
+                v = cost(i, smp[t])
+                if v < dv[i]:
+                    dv[i], av[i] = v, ci
+            s = tot(dv)
+            if dsum is None:
+                dsum, newc = s, smp[t]
+            elif s < dsum:
+                dist, asn, dsum, newc = dv, av, s, smp[t]
+        centers.append(newc)
+        asn[newc] = ci
+        dist[newc] = 0.
+    return np.array(centers), np.array(asn), np.array(dist, np.float32)
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("L", [2, 3, 5])
+def test_kmeanspp_local_trials_loops(mco, weighted, L):
+    """n_local_trials > 1 against the plain-loop statement on tiny inputs; several
+    seeds so that later candidates win some rounds (the stale-copy rule matters)."""
+    rng = np.random.default_rng(L)
+    for seed in range(1, 7):
+        X = rng.standard_normal((36, 3)).astype(np.float32)
+        w = rng.uniform(0.2, 3., 36) if weighted else None
+        c, a, d = mco.kmeanspp_local(mco.SQRL2, X, 6, L, seed=seed, w=w)
+        c2, a2, d2 = _local_trials_loops(mco, X, 6, L, seed, w)
+        assert np.array_equal(c, c2) and np.array_equal(a, a2) and np.array_equal(d, d2)
+
+
+def test_kmeanspp_local_trials_invariants(mco):
+    """The first center is the cheapest of the L uniform rows; every distance is the
+    cost to the assigned center; L candidates never lose to one (the cost after the
+    first center is at most any candidate's); local search++ then leaves row minima."""
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((300, 4)).astype(np.float32)
+    n, L = 300, 4
+    c, a, d = mco.kmeanspp_local(mco.SQRL2, X, 8, L, seed=21)
+    cands = [mco.wyrand_at(21, t) % n for t in range(L)]
+    totals = [float(mco.assign_sqrl2_f32(X, X[[r]])[1].astype(np.float64).sum()) for r in cands]
+    assert c[0] == cands[int(np.argmin(totals))]
+    own = np.array([mco.sqrl2_f32(X[i], X[c[a[i]]]) for i in range(n)], np.float32)
+    assert np.array_equal(d, own)
+    c1, a1, d1 = mco.kmeanspp_local(mco.SQRL2, X, 1, L, seed=21)
+    assert c1[0] == c[0] and not a1.any()
+    cl, al, dl = mco.kmeanspp_local(mco.SQRL2, X, 8, L, seed=21, lspp=15)
+    c0, a0, d0 = mco.kmeanspp_local(mco.SQRL2, X, 8, L, seed=21, lspp=0)
+    assert np.array_equal(c0, c) and np.array_equal(d0, d)
+    ab, cb, _ = mco.assign_sqrl2_f32(X, X[cl.astype(np.int64)])
+    assert np.array_equal(al, ab) and np.array_equal(dl, cb)
+
+
 # ---- fixed-point sampler ------------------------------------------------------------------
 @pytest.mark.parametrize("weighted", [False, True])
 def test_localsearch_rounds_invariants(mco, weighted):

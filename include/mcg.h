@@ -233,7 +233,11 @@ int mcg_cs_minibatch(mcg_data *ds, int msr, double prior, float *centers, uint64
 
 /* Soft EM driver: perform_soft_clustering, solve.h:287-374 with
  * set_centroids_full_mean (soft), centroid.h:781-963.  costs and resp are
- * n x k float32 outputs (nullable). */
+ * n x k float32 outputs (nullable).  Centers are responsibility-weighted means;
+ * for L1 (dense rows) the weighted_median rule over all rows and for L2 the
+ * Weiszfeld geometric median of all rows, with weights w_i * r_ic
+ * (centroid.h:815-838; CSR L1 -> MCG_EUNSUPPORTED as the reference throws; an L2
+ * center whose weights are all zero -> MCG_ENAN). */
 int mcg_soft_em(mcg_data *ds, int msr, double prior, double temp, float *centers, uint64_t k,
                 const double *weights, double eps, uint64_t maxiter, float *costs, float *resp,
                 double *initcost, double *finalcost, uint64_t *numiter);

@@ -1356,12 +1356,11 @@ int mcg_soft_em(mcg_data *ds, int msr, double prior, double temp, float *centers
         REQUIRE(k > 0, "k must be positive");
         check_msr(msr);
         // set_centroids_full_mean (soft, centroid.h:795-963): every measure takes the
-        // responsibility-weighted mean except L1 and L2, whose weighted median /
-        // geometric median over all rows are not on this path
-        if (msr == MCG_L1 || msr == MCG_L2)
-            MCG_THROW(MCG_EUNSUPPORTED, "soft clustering with L1 / L2 (weighted median centers over all rows, "
-                                        "centroid.h:815-835) is not on the GPU path");
+        // responsibility-weighted mean except L1 and L2, which take the weighted median
+        // and the weighted geometric median over all rows (centroid.h:815-838)
+        const bool med = msr == MCG_L1 || msr == MCG_L2;
         Data &D = ds->d;
+        if (msr == MCG_L1 && D.csr) MCG_THROW(MCG_EUNSUPPORTED, "Not implemented: weighted L1 median for CSparseMatrix");
         Ctx &c = *D.ctx;
         hipStream_t st = c.stream;
         HIP_CHECK(hipSetDevice(c.device));
@@ -1384,7 +1383,8 @@ int mcg_soft_em(mcg_data *ds, int msr, double prior, double temp, float *centers
             const double oldcost = cost;
             softmax_rows(K, n, k, (float)temp, R, W, rowdot, st);
             hipLaunchKernelGGL(k_wsum_d, dim3(1), dim3(1024), 0, st, rowdot, n, tot);
-            soft_mstep(D, R, W, k, Cc, st);
+            if (med) soft_median_mstep(D, msr, R, W, k, Cc, K, st);   // K is rewritten below
+            else soft_mstep(D, R, W, k, Cc, st);
             generic_costs(D, msr, prior, Cc, k, K, nullptr, nullptr, st);
             HIP_CHECK(hipMemcpyAsync(&cost, tot, sizeof(double), hipMemcpyDeviceToHost, st));
             sync(c);

@@ -240,6 +240,10 @@ void softmax_rows(const float *costs, uint64_t n, uint64_t k, float temp, float 
                   const double *w, double *rowdot, hipStream_t st);
 void soft_mstep(Data &ds, const float *resp, const double *w, uint64_t k, float *Cnew,
                 hipStream_t st);
+// Soft M-step for L1 (dense) and L2: weighted median / Weiszfeld geometric median of
+// all rows per center, from the centers in C (updated in place); K (n x k) is scratch.
+void soft_median_mstep(Data &ds, int msr, const float *R, const double *w, uint64_t k, float *C, float *K,
+                       hipStream_t st);
 
 // Sampling.
 void kmeanspp_run(Data &ds, int msr, double prior, uint64_t k, uint64_t seed, const double *w,

@@ -9,6 +9,7 @@
 // All sums are FP64 in a fixed order (row splits added in order).
 #include "mcg_internal.h"
 #include <algorithm>
+#include <cstring>
 #include <cfloat>
 #include <cmath>
 
@@ -244,6 +245,264 @@ void soft_mstep(Data &ds, const float *R, const double *w, uint64_t k, float *Cn
         case 3: hipLaunchKernelGGL(k_soft_csc<3>, g, blk, 0, st, cp, cr, cv, d, R, (uint32_t)k, c0, den, Cnew); break;
         default: hipLaunchKernelGGL(k_soft_csc<4>, g, blk, 0, st, cp, cr, cv, d, R, (uint32_t)k, c0, den, Cnew); break;
         }
+    }
+}
+
+// ---- soft L1 / L2 centers (set_centroids_full_mean, centroid.h:815-838) ----------
+// Per center the weights are wc_i = w_i * r_ic (float) over ALL rows.
+//  - L1 (dense rows; kmedian.h:266-301): weighted_median never adds up its weights,
+//    so each coordinate is the value of the row of largest weight, the smallest
+//    value among rows tied at that weight.  All-zero weights take its lower_bound
+//    branch with mid = 0: the first value v >= 0 in sorted order and the next, u,
+//    give v == 0 ? (v + u) / 2 : u (u = v when v is last; no value >= 0 gives 0,
+//    where the reference reads past the column).
+//  - L2 (blaze_adaptor.h:615-656 dense, csc.h:1131-1203 CSR): Weiszfeld from the
+//    current center for all centers at once; a center stops when its cost is not
+//    below the previous one (dense: sum wc_i * dist_i rounded to float, distances
+//    floored at 1e-30f; CSR: the unweighted sum of distances, floored at 1e-80, in
+//    double), after at most 99999 updates.  The update sum_i q_i x_i / sum_i q_i,
+//    q_i = wc_i / dist_i, runs as the soft M-step with q_i / max_c q as the
+//    responsibilities (FP64 sums in a fixed order).
+
+__device__ __forceinline__ uint32_t fkey(float v) {   // order-preserving float -> u32
+    const uint32_t u = __float_as_uint(v);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float fkey_value(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+__device__ __forceinline__ float soft_wc(const double *w, const float *R, uint64_t i, uint32_t k, uint32_t c) {
+    return (w ? (float)w[i] : 1.f) * R[i * k + c];
+}
+
+// largest key of wc_ic per center (block: rows [r0, r1), threads over centers)
+__global__ __launch_bounds__(256) void k_soft_wmax(const float *__restrict__ R, const double *__restrict__ w,
+                                                   uint64_t n, uint32_t k, uint64_t rows_per,
+                                                   uint32_t *__restrict__ mx) {
+    const uint64_t r0 = (uint64_t)blockIdx.x * rows_per, r1 = min<uint64_t>(n, r0 + rows_per);
+    for (uint32_t c = threadIdx.x; c < k; c += blockDim.x) {
+        uint32_t m = 0;
+        for (uint64_t i = r0; i < r1; ++i) m = max(m, fkey(soft_wc(w, R, i, k, c)));
+        atomicMax(mx + c, m);
+    }
+}
+
+// rows at the largest weight: their number and the first of them, per center
+__global__ __launch_bounds__(256) void k_soft_ties(const float *__restrict__ R, const double *__restrict__ w,
+                                                   uint64_t n, uint32_t k, uint64_t rows_per,
+                                                   const uint32_t *__restrict__ mx, uint32_t *__restrict__ cnt,
+                                                   uint32_t *__restrict__ first) {
+    const uint64_t r0 = (uint64_t)blockIdx.x * rows_per, r1 = min<uint64_t>(n, r0 + rows_per);
+    for (uint32_t c = threadIdx.x; c < k; c += blockDim.x) {
+        const uint32_t m = mx[c];
+        uint32_t t = 0, f = 0xFFFFFFFFu;
+        for (uint64_t i = r0; i < r1; ++i)
+            if (fkey(soft_wc(w, R, i, k, c)) == m) { if (!t) f = (uint32_t)i; ++t; }
+        if (t) { atomicAdd(cnt + c, t); atomicMin(first + c, f); }
+    }
+}
+
+// columnwise minimum (as keys) over the rows tied at the largest weight, for the
+// centers in list[] (grid.x over them, grid.y over row chunks, threads over columns)
+__global__ __launch_bounds__(256) void k_soft_tie_min(const float *__restrict__ X, uint64_t n, uint32_t d,
+                                                      const float *__restrict__ R, const double *__restrict__ w,
+                                                      uint32_t k, const uint32_t *__restrict__ mx,
+                                                      const uint32_t *__restrict__ list, uint64_t rows_per,
+                                                      uint32_t *__restrict__ ck) {
+    const uint32_t c = list[blockIdx.x];
+    const uint64_t r0 = (uint64_t)blockIdx.y * rows_per, r1 = min<uint64_t>(n, r0 + rows_per);
+    const uint32_t m = mx[c];
+    for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) {
+        uint32_t v = 0xFFFFFFFFu;
+        for (uint64_t i = r0; i < r1; ++i)
+            if (fkey(soft_wc(w, R, i, k, c)) == m) v = min(v, fkey(X[i * d + j]));
+        if (v != 0xFFFFFFFFu) atomicMin(ck + (uint64_t)c * d + j, v);
+    }
+}
+
+__global__ void k_soft_tie_store(const uint32_t *__restrict__ ck, const uint32_t *__restrict__ list, uint32_t nl,
+                                 uint32_t d, float *__restrict__ C) {
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (uint64_t)nl * d) return;
+    const uint32_t c = list[e / d], j = (uint32_t)(e % d);
+    C[(uint64_t)c * d + j] = fkey_value(ck[(uint64_t)c * d + j]);
+}
+
+// the all-zero-weight rule per column (one thread per column), written to the
+// centers in list[]
+__global__ void k_soft_zero_rule(const float *__restrict__ X, uint64_t n, uint32_t d,
+                                 const uint32_t *__restrict__ list, uint32_t nl, float *__restrict__ C) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d) return;
+    float v1 = FLT_MAX, v2 = FLT_MAX;
+    uint64_t cnt = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        const float x = X[i * d + j];
+        if (!(x >= 0.f)) continue;
+        ++cnt;
+        if (x < v1) { v2 = v1; v1 = x; } else if (x < v2) v2 = x;
+    }
+    float r = 0.f;
+    if (cnt) {
+        if (cnt == 1) v2 = v1;
+        r = v1 == 0.f ? (float)(.5 * ((double)v1 + (double)v2)) : v2;
+    }
+    for (uint32_t q = 0; q < nl; ++q) C[(uint64_t)list[q] * d + j] = r;
+}
+
+// Weiszfeld terms per center over a chunk of rows: the stopping cost (weighted or
+// not) into part[chunk][c], the largest q = wc / dist into qmax[c] (bit pattern)
+__global__ __launch_bounds__(256) void k_geo_soft_terms(const float *__restrict__ K, const float *__restrict__ R,
+                                                        const double *__restrict__ w, uint64_t n, uint32_t k,
+                                                        double floor_d, int weighted, uint64_t rows_per,
+                                                        double *__restrict__ part,
+                                                        unsigned long long *__restrict__ qmax) {
+    const uint64_t r0 = (uint64_t)blockIdx.x * rows_per, r1 = min<uint64_t>(n, r0 + rows_per);
+    for (uint32_t c = threadIdx.x; c < k; c += blockDim.x) {
+        double cs = 0., qm = 0.;
+        for (uint64_t i = r0; i < r1; ++i) {
+            const float dd = K[i * k + c];
+            const double v = (dd != dd || (double)dd < floor_d) ? floor_d : (double)dd;
+            const float wc = soft_wc(w, R, i, k, c);
+            cs += weighted ? (double)wc * v : v;
+            qm = fmax(qm, (double)wc / v);
+        }
+        part[(uint64_t)blockIdx.x * k + c] = cs;
+        atomicMax(qmax + c, (unsigned long long)__double_as_longlong(qm));
+    }
+}
+
+__global__ void k_sum_parts(const double *__restrict__ part, uint32_t S, uint32_t k, double *__restrict__ out) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= k) return;
+    double s = 0.;
+    for (uint32_t b = 0; b < S; ++b) s += part[(uint64_t)b * k + c];
+    out[c] = s;
+}
+
+// K (distances) -> Q = (wc / dist) / qmax_c for the centers still moving, else 0
+__global__ void k_geo_soft_q(float *__restrict__ K, const float *__restrict__ R, const double *__restrict__ w,
+                             uint64_t n, uint32_t k, double floor_d, const unsigned long long *__restrict__ qmax,
+                             const uint32_t *__restrict__ alive) {
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * k;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = e / k;
+        const uint32_t c = (uint32_t)(e % k);
+        float q = 0.f;
+        if (alive[c]) {
+            const float dd = K[e];
+            const double v = (dd != dd || (double)dd < floor_d) ? floor_d : (double)dd;
+            q = (float)(((double)soft_wc(w, R, i, k, c) / v) / __longlong_as_double((long long)qmax[c]));
+        }
+        K[e] = q;
+    }
+}
+
+__global__ void k_copy_alive(const float *__restrict__ Cn, const uint32_t *__restrict__ alive, uint32_t k, uint32_t d,
+                             float *__restrict__ C) {
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (uint64_t)k * d) return;
+    if (alive[e / d]) C[e] = Cn[e];
+}
+
+void soft_median_mstep(Data &ds, int msr, const float *R, const double *w, uint64_t k, float *Cc, float *K,
+                       hipStream_t st) {
+    Ctx &c = *ds.ctx;
+    const uint64_t n = ds.n;
+    const uint32_t d = (uint32_t)ds.d, kk = (uint32_t)k;
+    const uint32_t S = (uint32_t)std::min<uint64_t>(cdiv(n, 256), 1024);
+    const uint64_t rows_per = cdiv(n, S);
+    const uint32_t kg = (uint32_t)cdiv(k, 256);
+    if (msr == MCG_L1) {
+        if (ds.csr) MCG_THROW(MCG_EUNSUPPORTED, "Not implemented: weighted L1 median for CSparseMatrix");
+        uint32_t *mx = (uint32_t *)c.s[S_MED0].reserve(k * 16);
+        uint32_t *cnt = mx + k, *first = cnt + k, *list = first + k;
+        fill_u32(mx, 0u, k, st);
+        fill_u32(cnt, 0u, k, st);
+        fill_u32(first, 0xFFFFFFFFu, k, st);
+        hipLaunchKernelGGL(k_soft_wmax, dim3(S), dim3(256), 0, st, R, w, n, kk, rows_per, mx);
+        hipLaunchKernelGGL(k_soft_ties, dim3(S), dim3(256), 0, st, R, w, n, kk, rows_per, (const uint32_t *)mx, cnt,
+                           first);
+        std::vector<uint32_t> h(3 * k);
+        copy_sync(h.data(), mx, 3 * k * 4, hipMemcpyDeviceToHost, st);
+        std::vector<uint64_t> rows;
+        std::vector<uint32_t> ids, tied, zero;
+        for (uint32_t q = 0; q < kk; ++q) {
+            const uint32_t key = h[q];
+            const bool pos = (key & 0x80000000u) && (key & 0x7fffffffu);   // largest weight > 0
+            if (!pos) zero.push_back(q);
+            else if (h[k + q] == 1) { rows.push_back(h[2 * k + q]); ids.push_back(q); }
+            else tied.push_back(q);
+        }
+        if (!rows.empty()) copy_rows_to_centers(ds, rows.data(), ids.data(), rows.size(), Cc, st);
+        if (!tied.empty()) {
+            uint32_t *ck = (uint32_t *)c.s[S_MED1].reserve(k * d * 4);
+            HIP_CHECK(hipMemcpyAsync(list, tied.data(), tied.size() * 4, hipMemcpyHostToDevice, st));
+            for (uint32_t q : tied) fill_u32(ck + (uint64_t)q * d, 0xFFFFFFFFu, d, st);
+            hipLaunchKernelGGL(k_soft_tie_min, dim3((uint32_t)tied.size(), S), dim3(256), 0, st, ds.X, n, d, R, w, kk,
+                               (const uint32_t *)mx, (const uint32_t *)list, rows_per, ck);
+            hipLaunchKernelGGL(k_soft_tie_store, dim3((uint32_t)cdiv(tied.size() * d, 256)), dim3(256), 0, st,
+                               (const uint32_t *)ck, (const uint32_t *)list, (uint32_t)tied.size(), d, Cc);
+        }
+        if (!zero.empty()) {
+            uint32_t *zl = (uint32_t *)c.s[S_MED2].reserve(k * 4);
+            HIP_CHECK(hipMemcpyAsync(zl, zero.data(), zero.size() * 4, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_soft_zero_rule, dim3((uint32_t)cdiv(d, 64)), dim3(64), 0, st, ds.X, n, d,
+                               (const uint32_t *)zl, (uint32_t)zero.size(), Cc);
+        }
+        HIP_CHECK(hipStreamSynchronize(st));   // the host lists above live until here
+        return;
+    }
+    // L2: Weiszfeld for all centers at once
+    if (!ds.csr && n == 1) {   // one row: that row (blaze_adaptor.h:618)
+        std::vector<uint64_t> rows(k, 0);
+        std::vector<uint32_t> ids(k);
+        for (uint32_t q = 0; q < kk; ++q) ids[q] = q;
+        copy_rows_to_centers(ds, rows.data(), ids.data(), k, Cc, st);
+        HIP_CHECK(hipStreamSynchronize(st));
+        return;
+    }
+    const bool csr = ds.csr;
+    const double floor_d = csr ? 1e-80 : (double)1e-30f;
+    char *m0 = (char *)c.s[S_MED0].reserve(k * 8 * 3 + (uint64_t)S * k * 8);
+    double *cost = (double *)m0;
+    unsigned long long *qmax = (unsigned long long *)(m0 + k * 8);
+    uint32_t *alive_d = (uint32_t *)(m0 + k * 16);
+    double *part = (double *)(m0 + k * 24);
+    float *Cn = (float *)c.s[S_CENT_C].reserve(k * d * 4);
+    std::vector<double> prev(k, csr ? DBL_MAX : (double)FLT_MAX);
+    std::vector<uint32_t> alive(k, 1u);
+    std::vector<uint64_t> hb(2 * k);
+    const uint32_t eg = (uint32_t)std::min<uint64_t>(cdiv(n * k, 256), 4096);
+    for (uint64_t it = 0;;) {
+        generic_costs(ds, MCG_L2, 0., Cc, k, K, nullptr, nullptr, st);
+        fill_u32((uint32_t *)qmax, 0u, 2 * k, st);
+        hipLaunchKernelGGL(k_geo_soft_terms, dim3(S), dim3(256), 0, st, (const float *)K, R, w, n, kk, floor_d,
+                           csr ? 0 : 1, rows_per, part, qmax);
+        hipLaunchKernelGGL(k_sum_parts, dim3(kg), dim3(256), 0, st, (const double *)part, S, kk, cost);
+        copy_sync(hb.data(), cost, 2 * k * 8, hipMemcpyDeviceToHost, st);
+        bool any = false;
+        for (uint32_t q = 0; q < kk; ++q) {
+            if (!alive[q]) continue;
+            double cur;
+            std::memcpy(&cur, &hb[q], 8);
+            if (!csr) cur = (double)(float)cur;
+            if (!(cur < prev[q])) { alive[q] = 0; continue; }
+            prev[q] = cur;
+            any = true;
+            double qm;
+            std::memcpy(&qm, &hb[k + q], 8);
+            if (!(qm > 0.))
+                MCG_THROW(MCG_ENAN, "Optimization failed: nan (geometric median of a center whose weights are all zero)");
+        }
+        if (!any || ++it == 100000) break;
+        HIP_CHECK(hipMemcpyAsync(alive_d, alive.data(), k * 4, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_geo_soft_q, dim3(eg), dim3(256), 0, st, K, R, w, n, kk, floor_d,
+                           (const unsigned long long *)qmax, (const uint32_t *)alive_d);
+        soft_mstep(ds, K, nullptr, k, Cn, st);
+        hipLaunchKernelGGL(k_copy_alive, dim3((uint32_t)cdiv(k * d, 256)), dim3(256), 0, st, (const float *)Cn,
+                           (const uint32_t *)alive_d, kk, d, Cc);
+        HIP_CHECK(hipStreamSynchronize(st));   // alive (host) is rewritten next round
     }
 }
 

@@ -956,6 +956,116 @@ static void soft_costs(int msr, const float *X, const float *val, const uint32_t
     free(tmp);
 }
 
+/* ======================================================================
+ * Soft M-step for L1 and L2 (set_centroids_full_mean, centroid.h:815-838): per
+ * center, weights wc_i = w_i * r_ic (float, unweighted: r_ic) over ALL rows.
+ *  - L1, dense rows (kmedian.h:266-301, weighted_median, which never adds up its
+ *    weights): each coordinate takes the value of the row of largest weight, the
+ *    smallest value among rows of equal largest weight.  A center whose weights
+ *    are all zero falls to the rule's lower_bound branch with mid = 0: in sorted
+ *    order the first value v >= 0 and the next one w give v == 0 ? (v + w) / 2 : w
+ *    (w = v when v is the last; no value >= 0 gives 0, where the reference reads
+ *    past the column).  CSR rows: the reference throws (csc.h:834); -1 here.
+ *  - L2, dense rows (blaze_adaptor.h:615-656): Weiszfeld from the current center,
+ *    distances floored at 1e-30 (NaN -> 1e-30); the cost sum_i wc_i * dist_i is
+ *    rounded to float; stop when it is not below the previous one; else the center
+ *    becomes sum_i (wc_i / dist_i) x_i / sum_i (wc_i / dist_i).  One row: that row.
+ *  - L2, CSR rows (csc.h:1131-1203 with no row list): the same update in double
+ *    with the floor 1e-80, but the stopping cost is the UNWEIGHTED sum of the
+ *    distances (csc.h:1151), compared in double; at most 99999 updates (both).
+ *    A center whose weights are all zero gets NaN there (the reference throws): -2.
+ * Weighted sums here are in double and rounded to float once (parity is to a
+ * tolerance: the reference's float sums have no fixed order).
+ * ====================================================================== */
+static double soft_row_l2(const float *X, const float *val, const uint32_t *idx, const uint64_t *indptr,
+                          uint64_t d, uint64_t i, const float *c) {
+    double s = 0.;
+    if (!val) {
+        for (uint64_t j = 0; j < d; ++j) { const double t = (double)X[i * d + j] - (double)c[j]; s += t * t; }
+    } else {
+        double cc = 0.;
+        for (uint64_t j = 0; j < d; ++j) cc += (double)c[j] * (double)c[j];
+        s = cc;
+        for (uint64_t p = indptr[i]; p < indptr[i + 1]; ++p) {
+            const double x = val[p], cj = c[idx[p]];
+            s += (x - cj) * (x - cj) - cj * cj;
+        }
+        if (s < 0.) s = 0.;
+    }
+    return sqrt(s);
+}
+
+static int soft_median_step(int msr, const float *X, const float *val, const uint32_t *idx, const uint64_t *indptr,
+                            uint64_t n, uint64_t d, const float *resp, const double *w, uint64_t k, float *Cc) {
+    if (msr == MCO_L1 && val) return -1;
+    float *wc = (float *)malloc(n * sizeof(float));
+    double *dist = (double *)malloc(n * sizeof(double)), *acc = (double *)malloc(d * sizeof(double));
+    for (uint64_t c = 0; c < k; ++c) {
+        float *ctr = Cc + c * d;
+        for (uint64_t i = 0; i < n; ++i) wc[i] = (w ? (float)w[i] : 1.f) * resp[i * k + c];
+        if (msr == MCO_L1) {
+            float mw = -FLT_MAX;
+            for (uint64_t i = 0; i < n; ++i) if (wc[i] > mw) mw = wc[i];
+            if (mw > 0.f) {
+                for (uint64_t j = 0; j < d; ++j) {
+                    float v = FLT_MAX;
+                    for (uint64_t i = 0; i < n; ++i) if (wc[i] == mw && X[i * d + j] < v) v = X[i * d + j];
+                    ctr[j] = v;
+                }
+            } else {
+                for (uint64_t j = 0; j < d; ++j) {
+                    float v1 = FLT_MAX, v2 = FLT_MAX;   /* two smallest values >= 0, duplicates counted */
+                    int cnt = 0;
+                    for (uint64_t i = 0; i < n; ++i) {
+                        const float x = X[i * d + j];
+                        if (!(x >= 0.f)) continue;
+                        ++cnt;
+                        if (x < v1) { v2 = v1; v1 = x; } else if (x < v2) v2 = x;
+                    }
+                    if (cnt == 0) ctr[j] = 0.f;
+                    else {
+                        if (cnt == 1) v2 = v1;
+                        ctr[j] = v1 == 0.f ? (float)(.5 * ((double)v1 + (double)v2)) : v2;
+                    }
+                }
+            }
+            continue;
+        }
+        /* L2: Weiszfeld */
+        if (!val && n == 1) { memcpy(ctr, X, d * sizeof(float)); continue; }
+        const double floor_d = val ? 1e-80 : (double)1e-30f;
+        double prev = val ? DBL_MAX : (double)FLT_MAX;
+        for (int it = 0;;) {
+            double cur = 0.;
+            for (uint64_t i = 0; i < n; ++i) {
+                double v = soft_row_l2(X, val, idx, indptr, d, i, ctr);
+                if (!val) v = (double)(float)v;
+                if (isnan(v) || v < floor_d) v = floor_d;
+                dist[i] = v;
+                cur += val ? v : (double)wc[i] * v;
+            }
+            if (!val) cur = (double)(float)cur;
+            if (cur >= prev || ++it == 100000) break;
+            prev = cur;
+            double qs = 0.;
+            for (uint64_t j = 0; j < d; ++j) acc[j] = 0.;
+            for (uint64_t i = 0; i < n; ++i) {
+                const double q = (double)wc[i] / dist[i];
+                qs += q;
+                if (val) {
+                    for (uint64_t p = indptr[i]; p < indptr[i + 1]; ++p) acc[idx[p]] += q * (double)val[p];
+                } else {
+                    for (uint64_t j = 0; j < d; ++j) acc[j] += q * (double)X[i * d + j];
+                }
+            }
+            if (!(qs > 0.)) { free(wc); free(dist); free(acc); return -2; }   /* all weights zero: NaN */
+            for (uint64_t j = 0; j < d; ++j) ctr[j] = (float)(acc[j] / qs);
+        }
+    }
+    free(wc); free(dist); free(acc);
+    return 0;
+}
+
 int mco_soft(int msr, const float *X, const float *val, const uint32_t *idx,
              const uint64_t *indptr, uint64_t n, uint64_t d, float *C, uint64_t k,
              double prior, double temp, const double *w, double eps, uint64_t maxiter,
@@ -976,6 +1086,14 @@ int mco_soft(int msr, const float *X, const float *val, const uint32_t *idx,
             ret += dt * (w ? w[i] : 1.);
         }
         cost = ret;
+        if (msr == MCO_L1 || msr == MCO_L2) {
+            const int rc = soft_median_step(msr, X, val, idx, indptr, n, d, resp, w, k, Cc);
+            if (rc) {
+                free(num); free(den); free(Cc);
+                return rc;
+            }
+            goto costs;
+        }
         /* M-step: dense numerator carries w (centroid.h:840-873); CSR omits it (:925-939). */
         memset(num, 0, k * d * sizeof(double)); memset(den, 0, k * sizeof(double));
         for (uint64_t i = 0; i < n; ++i) {
@@ -993,6 +1111,7 @@ int mco_soft(int msr, const float *X, const float *val, const uint32_t *idx,
         }
         for (uint64_t c = 0; c < k; ++c)
             for (uint64_t j = 0; j < d; ++j) Cc[c * d + j] = (float)(num[c * d + j] / den[c]);
+    costs:
         soft_costs(msr, X, val, idx, indptr, n, d, Cc, k, prior, costs);
         if (init < 0.) init = cost;
         if (oldcost >= cost) memcpy(C, Cc, k * d * sizeof(float));

@@ -311,8 +311,12 @@ def soft(msr, X, C0, prior=0., temp=1., w=None, eps=1e-8, maxiter=1000):
     resp = np.empty((n, k), dtype=np.float32)
     ic, fc = np.zeros(1), np.zeros(1)
     it = np.zeros(1, dtype=np.uint64)
-    lib().mco_soft(msr, _p(Xd), _p(val), _p(idx), _p(ptr), n, d, _p(C), k, prior, temp, _p(w), eps,
-                   maxiter, _p(costs), _p(resp), _p(ic), _p(fc), _p(it))
+    rc = lib().mco_soft(msr, _p(Xd), _p(val), _p(idx), _p(ptr), n, d, _p(C), k, prior, temp, _p(w), eps,
+                        maxiter, _p(costs), _p(resp), _p(ic), _p(fc), _p(it))
+    if rc == -1:
+        raise NotImplementedError("weighted L1 median for CSR rows (csc.h:834)")
+    if rc:
+        raise FloatingPointError("geometric median of a center whose weights are all zero")
     return dict(centers=C, costs=costs, asn=resp, initcost=float(ic[0]), finalcost=float(fc[0]),
                 numiter=int(it[0]))
 

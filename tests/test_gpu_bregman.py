@@ -160,13 +160,69 @@ def test_soft_em(mcg, ctx, mco, name, code, sparse):
     np.testing.assert_allclose(got["centers"], ref["centers"], rtol=2e-3, atol=1e-4)
 
 
-def test_soft_em_median_measures_refused(mcg, ctx):
-    """Soft clustering with L1 / L2 takes weighted medians over all rows
-    (centroid.h:815-835): not on the GPU path, refused with MCG_EUNSUPPORTED."""
-    X = np.random.default_rng(3).random((100, 5)).astype(np.float32)
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("code", [0, 1])
+def test_soft_em_median_measures(mcg, ctx, mco, code, weighted):
+    """Soft L1 / L2 (centroid.h:815-838): the weighted median (heaviest row) and the
+    Weiszfeld geometric median of all rows under w_i * r_ic, against the oracle."""
+    say(f"soft median code={code} weighted={weighted}")
+    rng = np.random.default_rng(21)
+    X = rng.normal(size=(900, 12)).astype(np.float32)
+    X[:300] += 2.5
+    X[300:600] -= 2.5
+    w = rng.uniform(.5, 2., 900) if weighted else None
+    C0 = X[[1, 400, 800]] + np.float32(.2)
+    K0 = mco.cost_matrix(code, X, C0, 0.)
+    temp = float(3. / np.median(K0))
+    ref = mco.soft(code, X, C0, temp=temp, eps=-1., maxiter=4, w=w)
+    got = mcg.Dataset.dense(X, ctx).soft(code, C0, temp=temp, eps=-1., maxiter=4, weights=w)
+    assert got["numiter"] == ref["numiter"] == 4
+    assert got["finalcost"] == pytest.approx(ref["finalcost"], rel=1e-4)
+    np.testing.assert_allclose(got["centers"], ref["centers"], rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(got["asn"], ref["asn"], atol=1e-4)
+    if code == 0:   # every center is a data row
+        assert all((X == c).all(1).any() for c in got["centers"])
+
+
+def test_soft_em_l1_ties_and_zero_weights(mcg, ctx, mco):
+    """k = 1: every row ties at weight 1, so the L1 center is the columnwise minimum;
+    all-zero weights take the lower_bound rule; the L2 form with all-zero weights
+    fails as the reference's NaN check does (MCG_ENAN)."""
+    say("soft median ties / zero weights")
+    rng = np.random.default_rng(4)
+    X = rng.normal(size=(500, 7)).astype(np.float32)
+    X[9, 2] = 0.
     ds = mcg.Dataset.dense(X, ctx)
-    for code in (0, 1):
-        with pytest.raises(mcg.MCGError) as e:
-            ds.soft(code, X[:3], maxiter=2)
-        assert e.value.code == -4
+    got = ds.soft(0, X[:1] + 1, maxiter=1)
+    np.testing.assert_array_equal(got["centers"][0], X.min(0))
+    z = np.zeros(500)
+    got = ds.soft(0, X[:2], maxiter=3, weights=z)
+    ref = mco.soft(0, X, X[:2], maxiter=3, w=z)
+    np.testing.assert_array_equal(got["centers"], ref["centers"])
+    with pytest.raises(mcg.MCGError) as e:
+        ds.soft(1, X[:2] + 1, maxiter=2, weights=z)
+    assert e.value.code == -5
+    ds.close()
+
+
+def test_soft_em_median_csr(mcg, ctx, mco):
+    """CSR rows: L2 runs the double-precision Weiszfeld that stops on the unweighted
+    sum of distances (csc.h:1131-1203); L1 is refused as the reference throws."""
+    say("soft median csr")
+    n, d, k = 700, 60, 3
+    data, idx, ptr = csr_data(n, d, 0.2, 12, False)
+    X = mco.CSR(data, idx, ptr, (n, d))
+    Xd = X.todense()
+    C0 = Xd[[0, 300, 600]] + np.float32(.3)
+    w = np.random.default_rng(5).uniform(.5, 2., n)
+    K0 = mco.cost_matrix(1, X, C0, 0.)
+    temp = float(3. / np.median(K0))
+    ds = mcg.Dataset.csr(data, idx, ptr, (n, d), ctx)
+    ref = mco.soft(1, X, C0, temp=temp, eps=-1., maxiter=3, w=w)
+    got = ds.soft(1, C0, temp=temp, eps=-1., maxiter=3, weights=w)
+    assert got["finalcost"] == pytest.approx(ref["finalcost"], rel=1e-4)
+    np.testing.assert_allclose(got["centers"], ref["centers"], rtol=1e-3, atol=2e-4)
+    with pytest.raises(mcg.MCGError) as e:
+        ds.soft(0, C0, maxiter=2)
+    assert e.value.code == -4
     ds.close()

@@ -450,6 +450,124 @@ def test_soft_decreases_and_centers(mco):
     np.testing.assert_allclose(r["asn"].sum(1), 1., rtol=1e-5)
 
 
+def _first_resp(mco, msr, X, C0, temp):
+    """Responsibilities of the first soft E-step (as the oracle forms them)."""
+    return mco.softmax_rows(mco.cost_matrix(msr, X, C0, 0.).astype(np.float32), temp)
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_soft_l1_heaviest_row(mco, weighted):
+    """Soft L1 centers (weighted_median over all rows, kmedian.h:266-301): each
+    center is the row of largest w_i * r_ic; with k = 1 every row ties and the center
+    is the columnwise minimum (of the rows of largest weight, when weighted)."""
+    rng = np.random.default_rng(3)
+    X = rng.normal(size=(300, 6)).astype(np.float32)
+    w = rng.uniform(.5, 2., 300) if weighted else None
+    C0 = X[[3, 50, 120]]
+    r = mco.soft(mco.L1, X, C0, temp=.7, maxiter=1, w=w)
+    R = _first_resp(mco, mco.L1, X, C0, .7)
+    Wt = R * (w.astype(np.float32)[:, None] if weighted else np.float32(1.))
+    for c in range(3):
+        assert (Wt[:, c] == Wt[:, c].max()).sum() == 1
+        np.testing.assert_array_equal(r["centers"][c], X[Wt[:, c].argmax()])
+    w1 = np.ones(300)
+    w1[[7, 9, 200]] = 3.
+    r1 = mco.soft(mco.L1, X, X[:1], maxiter=1, w=w1 if weighted else None)
+    expect = X[[7, 9, 200]].min(0) if weighted else X.min(0)
+    np.testing.assert_array_equal(r1["centers"][0], expect)
+
+
+def test_soft_l1_zero_weight_rule(mco):
+    """All-zero weights take weighted_median's lower_bound branch with mid = 0: the
+    first value >= 0 in sorted order, v, and the next, u, give v == 0 ? (v+u)/2 : u."""
+    rng = np.random.default_rng(5)
+    X = rng.normal(size=(40, 5)).astype(np.float32)
+    X[3, 1] = 0.
+    X[:, 4] = -np.abs(X[:, 4])
+    X[11, 4] = 2.5                       # a single value >= 0 in column 4
+    r = mco.soft(mco.L1, X, X[:2], maxiter=3, w=np.zeros(40))
+    for j in range(5):
+        v = np.sort(X[:, j][X[:, j] >= 0])
+        u = v[1] if len(v) > 1 else v[0]
+        want = np.float32(.5 * (float(v[0]) + float(u))) if v[0] == 0 else u
+        assert r["centers"][0, j] == want and r["centers"][1, j] == want
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_soft_l2_weiszfeld_optimal(mco, weighted):
+    """Soft L2 centers are the geometric medians of all rows under w_i * r_ic
+    (blaze_adaptor.h:615-656): each center's weighted cost is within 1e-6 of scipy's
+    Nelder-Mead optimum of the same objective."""
+    from scipy.optimize import minimize
+    rng = np.random.default_rng(9)
+    X = rng.normal(size=(250, 3)).astype(np.float32)
+    X[:120] += 3.
+    w = rng.uniform(.5, 2., 250) if weighted else None
+    C0 = X[[0, 200]] + np.float32(.1)
+    r = mco.soft(mco.L2, X, C0, temp=2., maxiter=1, w=w)
+    R = _first_resp(mco, mco.L2, X, C0, 2.)
+    Xd = X.astype(np.float64)
+    for c in range(2):
+        wc = (R[:, c] * (w.astype(np.float32) if weighted else np.float32(1.))).astype(np.float64)
+        f = lambda y: float((wc * np.linalg.norm(Xd - y, axis=1)).sum())
+        best = minimize(f, Xd.mean(0), method="Nelder-Mead", options=dict(xatol=1e-10, fatol=1e-12, maxiter=20000))
+        assert f(r["centers"][c].astype(np.float64)) <= best.fun * (1 + 1e-6)
+    # started on a data row, the floored distance (1e-30) gives that row all the
+    # weight: the center stays on it
+    r0 = mco.soft(mco.L2, X, X[[0, 200]], temp=2., maxiter=1, w=w)
+    np.testing.assert_array_equal(r0["centers"], X[[0, 200]])
+
+
+def test_soft_l2_csr_unweighted_stop(mco):
+    """CSR rows (csc.h:1131-1203): the update uses the weights but stops when the
+    UNWEIGHTED sum of distances no longer falls.  With k = 1 (all weights 1) the
+    center is the plain geometric median; with skewed weights the iterate is the
+    plain-loop Weiszfeld in double that stops on the unweighted sum, which here stops
+    earlier than the weighted objective would."""
+    rng = np.random.default_rng(2)
+    n, d = 120, 4
+    Xd = np.abs(rng.normal(size=(n, d))).astype(np.float32)
+    Xd[Xd < .4] = 0.
+    X = mco.CSR.from_dense(Xd) if hasattr(mco.CSR, "from_dense") else None
+    if X is None:
+        ptr = np.zeros(n + 1, np.uint64)
+        idx, val = [], []
+        for i in range(n):
+            nz = np.nonzero(Xd[i])[0]
+            idx += nz.tolist(); val += Xd[i, nz].tolist()
+            ptr[i + 1] = len(idx)
+        X = mco.CSR(np.array(val, np.float32), np.array(idx, np.uint32), ptr, (n, d))
+    c0 = Xd[:1] + np.float32(.37)
+    rd = mco.soft(mco.L2, Xd, c0, maxiter=1)
+    rc = mco.soft(mco.L2, X, c0, maxiter=1)
+    # the float stop of the dense form ends earlier than the double stop of the CSR form
+    np.testing.assert_allclose(rc["centers"], rd["centers"], atol=3e-4)
+    f = lambda y: float(np.linalg.norm(Xd.astype(np.float64) - y.astype(np.float64), axis=1).sum())
+    assert f(rc["centers"][0]) <= f(rd["centers"][0]) * (1 + 1e-9)
+    # skewed weights: a plain Weiszfeld loop in double with the unweighted stop
+    w = np.where(np.arange(n) < 10, 50., .1)
+    r = mco.soft(mco.L2, X, c0, maxiter=1, w=w)
+    wc = w.astype(np.float32).astype(np.float64)
+    Xx = Xd.astype(np.float64)
+
+    def loop(stop_weighted):
+        c = c0[0].copy()
+        prev = np.inf
+        for _ in range(100000):
+            dist = np.maximum(np.sqrt(((Xx - c.astype(np.float64)) ** 2).sum(1)), 1e-80)
+            cur = float((wc * dist).sum() if stop_weighted else dist.sum())
+            if cur >= prev:
+                break
+            prev = cur
+            q = wc / dist
+            c = ((q[:, None] * Xx).sum(0) / q.sum()).astype(np.float32)
+        return c
+    np.testing.assert_allclose(r["centers"][0], loop(False), rtol=1e-5, atol=1e-6)
+    assert np.abs(loop(True) - loop(False)).max() > 1e-3
+    with pytest.raises(NotImplementedError):
+        mco.soft(mco.L1, X, Xd[:2], maxiter=1)
+
+
 # ---- a6: D^2 seeding --------------------------------------------------------------------
 def test_kmeanspp_first_center_and_assignments(mco):
     rng = np.random.default_rng(8)

@@ -150,7 +150,7 @@ __device__ __forceinline__ float post_kl(float r) {
 }
 
 // One wave per row; lanes over centers in chunks of 64; row entries in batches
-// of 64 broadcast with shuffles.
+// of 64 broadcast from their lanes (scalar reads; LDS permutes for JSD).
 template <int MSR>
 __global__ __launch_bounds__(256) void k_row_costs(const float *__restrict__ X, const float *__restrict__ val,
                                                    const uint32_t *__restrict__ idx, const uint64_t *__restrict__ ptr,
@@ -190,20 +190,22 @@ __global__ __launch_bounds__(256) void k_row_costs(const float *__restrict__ X, 
                 else { const float xt = (x + pv) * Li; tq = xt; tq2 = xt * __logf(xt); }
             }
             const int cnt = (int)min<uint64_t>(64, e - p0);
-            // 8 entries at a time: their center-table gathers are issued together,
+            // BT entries at a time: their center-table gathers are issued together,
             // then accumulated in entry order (the same sums as one at a time)
+            constexpr int BT = MSR == MCG_JSD ? 8 : 16;
             int t0 = 0;
-            for (; t0 + 8 <= cnt; t0 += 8) {
-                float xv[8], ct[8], xl[8];
+            for (; t0 + BT <= cnt; t0 += BT) {
+                float xv[BT], ct[BT], xl[BT];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const uint32_t j = __shfl(jq, t0 + u);
-                    xv[u] = __shfl(tq, t0 + u);
+                for (int u = 0; u < BT; ++u) {
+                    // JSD is VALU-bound: its broadcasts stay on the LDS pipe
+                    const uint32_t j = MSR == MCG_JSD ? (uint32_t)__shfl((int)jq, t0 + u) : bcast_u32(jq, t0 + u);
+                    xv[u] = MSR == MCG_JSD ? __shfl(tq, t0 + u) : bcast_f32(tq, t0 + u);
                     if (MSR == MCG_JSD) xl[u] = __shfl(tq2, t0 + u);
                     ct[u] = CT[(uint64_t)j * kpad + cc];
                 }
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
+                for (int u = 0; u < BT; ++u) {
                     if (MSR == MCG_SQRL2) acc = fmaf(xv[u], xv[u] - 2.f * ct[u], acc);
                     else if (MSR == MCG_MKL || MSR == MCG_REVERSE_MKL) acc = fmaf(xv[u], ct[u], acc);
                     else {
@@ -213,8 +215,8 @@ __global__ __launch_bounds__(256) void k_row_costs(const float *__restrict__ X, 
                 }
             }
             for (int t = t0; t < cnt; ++t) {
-                const uint32_t j = __shfl(jq, t);
-                const float xv = __shfl(tq, t);
+                const uint32_t j = MSR == MCG_JSD ? (uint32_t)__shfl((int)jq, t) : bcast_u32(jq, t);
+                const float xv = MSR == MCG_JSD ? __shfl(tq, t) : bcast_f32(tq, t);
                 const float ct = CT[(uint64_t)j * kpad + cc];
                 if (MSR == MCG_SQRL2) acc = fmaf(xv, xv - 2.f * ct, acc);
                 else if (MSR == MCG_MKL || MSR == MCG_REVERSE_MKL) acc = fmaf(xv, ct, acc);

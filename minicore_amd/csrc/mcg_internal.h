@@ -327,6 +327,22 @@ inline bool stream_capturing(hipStream_t st) {
 // Block max of |w_p * dist_p| into *mx (bit patterns of non-negative doubles order
 // like their values): the next draw's fixed-point scale, computed in the pass that
 // writes the distances.  Every thread of the block calls it.
+// Lane l's value to the whole wave; l must be the same on every lane (scalar
+// register reads, not LDS permutes).
+__device__ __forceinline__ uint32_t bcast_u32(uint32_t v, int l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+__device__ __forceinline__ float bcast_f32(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ uint64_t bcast_u64(uint64_t v, int l) {
+    const uint32_t lo = bcast_u32((uint32_t)v, l), hi = bcast_u32((uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ double bcast_f64(double v, int l) {
+    return __longlong_as_double((long long)bcast_u64((uint64_t)__double_as_longlong(v), l));
+}
+
 __device__ __forceinline__ void block_max_into(double a, unsigned long long *mx) {
     __shared__ double sm[256];
     sm[threadIdx.x] = a;

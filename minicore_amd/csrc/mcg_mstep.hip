@@ -410,8 +410,8 @@ __global__ __launch_bounds__(256) void k_csr_mean(const float *__restrict__ val,
         }
         const uint32_t nrow = min(64u, nm - g);
         for (uint32_t t = 0; t < nrow; ++t) {
-            const uint64_t b = __shfl(rb, t), e = __shfl(re, t);
-            const double wt = __shfl(wq, t);
+            const uint64_t b = bcast_u64(rb, (int)t), e = bcast_u64(re, (int)t);
+            const double wt = bcast_f64(wq, (int)t);
             for (uint64_t p = b + lane; p < e; p += 64) {
                 const uint32_t j = idx[p];
                 if (j >= col1) break;

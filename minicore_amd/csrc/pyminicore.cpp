@@ -415,6 +415,8 @@ py::object scluster_csr(PyCSparseMatrix &smw, py::object centers, py::object msr
                         uint64_t maxiter, py::ssize_t mbsize, py::ssize_t, std::string savepref, py::object weights) {
     if (mbsize > 0) throw std::runtime_error("Not yet completed: minibatch soft clustering");   // solve.h:319-321
     const int m = measure_code(msr);
+    // csc.h:834 throws NotImplementedError, a std::runtime_error
+    if (m == MCG_L1) throw std::runtime_error("Not implemented: weighted L1 median for CSparseMatrix");
     uint64_t k = 0;
     auto C = center_block(centers, smw.nc, [&](const std::vector<int64_t> &ids) { return smw.dense_rows(ids); }, &k);
     if (k == 0) throw std::invalid_argument("no centers");

@@ -54,6 +54,32 @@ def test_scluster_returns(mco):
     assert len(r["centers"]) == 2
 
 
+def test_scluster_median_measures(mco):
+    """scluster with L1 / L2: centers from the weighted median and the Weiszfeld
+    geometric median of all rows, as the oracle; on CSR rows L1 raises as the
+    reference's weighted L1 median does."""
+    say("api scluster L1 / L2")
+    rng = np.random.default_rng(6)
+    X = rng.gamma(4., 1., (600, 10)).astype(np.float32)   # continuous: no tied weights
+    X[:200] += 6.
+    C0 = X[[1, 300, 500]] + np.float32(.5)
+    for code in (0, 1):
+        r = mc.scluster(X, C0, msr=code, temp=.3, maxiter=4)
+        (cd, ci, cp), shape = r["centers"]
+        C = sp.csr_matrix((cd, ci, cp), shape=tuple(shape)).toarray()
+        ref = mco.soft(code, X, C0, temp=.3, maxiter=4, eps=1e-8)
+        assert r["numiter"] == ref["numiter"]
+        assert r["finalcost"] == pytest.approx(ref["finalcost"], rel=1e-4)
+        np.testing.assert_allclose(C, ref["centers"], rtol=1e-4, atol=1e-4)
+    # centers given as row indices: Weiszfeld started on a data row stays on it
+    r = mc.scluster(X, [1, 300, 500], msr=1, temp=.3, maxiter=4)
+    (cd, ci, cp), shape = r["centers"]
+    C = sp.csr_matrix((cd, ci, cp), shape=tuple(shape)).toarray()
+    np.testing.assert_allclose(C, X[[1, 300, 500]], rtol=0, atol=1e-20)
+    with pytest.raises(RuntimeError):
+        mc.scluster(sp.csr_matrix(X), [1, 300, 500], msr=0, maxiter=2)
+
+
 def test_cmp_orientation(mco):
     say("api cmp")
     rng = np.random.default_rng(2)

@@ -8,6 +8,8 @@ Python package (minicore/__init__.py:23-261).
 
 There is no CPU fallback: importing the wrappers needs the built extension.
 """
+import os
+
 import numpy as np
 
 try:
@@ -19,7 +21,23 @@ except ImportError as e:  # pragma: no cover - exercised when the extension is n
     _IMPORT_ERROR = e
 
 __all__ = ["hcluster", "scluster", "kmeanspp", "cluster", "cmp", "pcmp", "CSparseMatrix", "CoresetSampler",
-           "cluster_from_centers", "set_sqrl2_precision"]
+           "cluster_from_centers", "set_sqrl2_precision", "set_num_threads", "get_num_threads"]
+
+_num_threads = [os.cpu_count() or 1]
+
+
+def set_num_threads(n):
+    """pyminicore.set_num_threads (OpenMP threads of the CPU kernels).  The clustering
+    here runs on the GPU, so the value is only kept and returned by get_num_threads."""
+    n = int(n)
+    if n < 1:
+        raise ValueError("number of threads must be positive")
+    _num_threads[0] = n
+
+
+def get_num_threads():
+    """pyminicore.get_num_threads: the value last given to set_num_threads."""
+    return _num_threads[0]
 
 
 def _ext():

@@ -37,3 +37,14 @@ def test_wrappers_need_extension(monkeypatch):
     monkeypatch.setattr(minicore_amd, "_IMPORT_ERROR", ImportError("x"), raising=False)
     with pytest.raises(RuntimeError):
         minicore_amd.hcluster([[1.0]], [0])
+
+
+def test_thread_count_functions():
+    """set_num_threads / get_num_threads (pyminicore) keep and return the count."""
+    import minicore_amd as mc
+    old = mc.get_num_threads()
+    mc.set_num_threads(5)
+    assert mc.get_num_threads() == 5
+    with pytest.raises(ValueError):
+        mc.set_num_threads(0)
+    mc.set_num_threads(old)

@@ -21,7 +21,8 @@ except ImportError as e:  # pragma: no cover - exercised when the extension is n
     _IMPORT_ERROR = e
 
 __all__ = ["hcluster", "scluster", "kmeanspp", "cluster", "cmp", "pcmp", "CSparseMatrix", "CoresetSampler",
-           "cluster_from_centers", "set_sqrl2_precision", "set_num_threads", "get_num_threads"]
+           "cluster_from_centers", "set_sqrl2_precision", "set_num_threads", "get_num_threads", "get_counthist",
+           "hiercluster_kmeanspp"]
 
 _num_threads = [os.cpu_count() or 1]
 
@@ -131,3 +132,6 @@ def cluster(data, *, msr, k, prior=0., seed=0, nmkc=0, ntimes=1, lspp=0, use_exp
                         savepref=outpref)
     return hcluster(data, centers=ids, prior=prior, msr=msr, weights=weights, eps=clustereps, maxiter=maxiter,
                     with_rep=with_rep, cs=cs)
+
+
+from .recursive_cluster import get_counthist, hiercluster_kmeanspp  # noqa: E402

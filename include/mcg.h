@@ -93,6 +93,7 @@ extern "C" {
 /* coreset sensitivity methods: include/minicore/coreset/coreset.h:34-48 */
 #define MCG_SENS_BFL 0
 #define MCG_SENS_FL  1
+#define MCG_SENS_LFKF 2   /* GMM coreset (make_gmm_sampler, coreset.h:341-381) with alpha_est = 0 */
 #define MCG_SENS_VX  3
 #define MCG_SENS_LBK 4
 

@@ -135,3 +135,5 @@ def cluster(data, *, msr, k, prior=0., seed=0, nmkc=0, ntimes=1, lspp=0, use_exp
 
 
 from .recursive_cluster import get_counthist, hiercluster_kmeanspp  # noqa: E402
+from . import constants  # noqa: E402
+from .constants import CSR as csr_tuple, KMCRSV  # noqa: E402

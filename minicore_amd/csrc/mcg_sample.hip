@@ -917,6 +917,10 @@ __global__ void k_cs_sens(int sens, const double *__restrict__ cost, const uint3
             const double Wc = from_fixed((int64_t)ws[a], sw), Sc = from_fixed((int64_t)cs[a], sc);
             const double adj = alpha2 * Sc / (Wc * mean) + 4. * W / Wc;
             v = tcinv * cost[i] + adj;
+        } else if (sens == MCG_SENS_LFKF) {
+            // 2 T / W_a (coreset.h:374-375 with alpha_est = 0); the common 2 T cancels
+            // in the normalisation
+            v = 1. / from_fixed((int64_t)ws[a], sw);
         } else if (sens == MCG_SENS_VX) {
             v = wc / TC + 1. / (double)cnt[a];
         } else if (sens == MCG_SENS_BFL) {
@@ -963,8 +967,9 @@ __global__ void k_cs_norm(const double *__restrict__ p, uint64_t n, const CsSums
 
 void coreset_probs_run(Ctx &c, int sens, uint64_t n, uint64_t ncenters, const double *cost, const uint32_t *asn,
                        const double *w, float *probs, hipStream_t st) {
-    REQUIRE(sens == MCG_SENS_LBK || sens == MCG_SENS_VX || sens == MCG_SENS_BFL || sens == MCG_SENS_FL,
-            "sensitivity method must be BFL (0), FL (1), VX (3) or LBK (4)");
+    REQUIRE(sens == MCG_SENS_LBK || sens == MCG_SENS_VX || sens == MCG_SENS_BFL || sens == MCG_SENS_FL ||
+                sens == MCG_SENS_LFKF,
+            "sensitivity method must be BFL (0), FL (1), LFKF (2), VX (3) or LBK (4)");
     const uint64_t bytes = sizeof(CsSums) + 3 * ncenters * 8;
     char *buf = (char *)c.s[S_MISC0].reserve(bytes);
     HIP_CHECK(hipMemsetAsync(buf, 0, bytes, st));

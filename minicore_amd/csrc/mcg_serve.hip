@@ -504,7 +504,8 @@ int mcg_submit(mcg_server *s, const mcg_request *req, uint64_t *ticket) {
         REQUIRE(q.centers && q.asn, "request needs centers and asn outputs");
         REQUIRE(msr_supported(q.msr) && msr_mean_centroid(q.msr), "unsupported measure (needs a mean centroid)");
         REQUIRE(q.coreset_frac > 0. && q.coreset_frac <= 1., "coreset_frac must be in (0, 1]");
-        REQUIRE(q.sens == MCG_SENS_BFL || q.sens == MCG_SENS_FL || q.sens == MCG_SENS_VX || q.sens == MCG_SENS_LBK,
+        REQUIRE(q.sens == MCG_SENS_BFL || q.sens == MCG_SENS_FL || q.sens == MCG_SENS_VX || q.sens == MCG_SENS_LBK ||
+                    q.sens == MCG_SENS_LFKF,
                 "unsupported sensitivity method");
         REQUIRE(!q.indices || q.indptr, "CSR request needs indptr");
         auto r = std::make_shared<Req>();

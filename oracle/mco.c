@@ -1357,6 +1357,8 @@ int mco_coreset_probs(int sens, uint64_t n, uint64_t ncenters, const double *cos
         }
         for (uint64_t i = 0; i < n; ++i) p[i] = tcinv * costs[i] + adj[asn[i]];
         free(adj);
+    } else if (sens == 2) {                             /* LFKF, :341-381, alpha_est = 0: 2T/W_a */
+        for (uint64_t i = 0; i < n; ++i) p[i] = 1. / fx_back(ws[asn[i]], fw);
     } else if (sens == 3) {                             /* VX, :423-455 */
         for (uint64_t i = 0; i < n; ++i) p[i] = wc[i] / TC + 1. / (double)cnt[asn[i]];
     } else if (sens == 0) {                             /* BFL, :534-578 */

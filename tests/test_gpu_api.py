@@ -132,3 +132,21 @@ def test_hiercluster_kmeanspp():
             assert (asn == key).sum() <= 6
     walk(out, X, 0)
     assert out["children"]
+
+
+def test_reference_coreset_example(mco):
+    """The reference's python/examples/coreset_example.py, unchanged but for the
+    module name: seeding, an LBK sampler from mc.constants.SENSDICT, 100 draws."""
+    say("api coreset example")
+    dat = np.random.default_rng(0).random(10000).reshape(1000, 10)
+    out, asn, costs = mc.kmeanspp(dat, k=25, msr=2)
+    cs = mc.CoresetSampler()
+    sensid = mc.constants.SENSDICT["LBK"]
+    cs.make_sampler(25, costs=costs, assignments=asn, sens=sensid)
+    weights, ids = cs.sample(100)
+    assert len(ids) == len(weights) == 100 and (np.asarray(weights) > 0).all()
+    cs2 = mc.CoresetSampler()
+    cs2.make_sampler(25, costs=costs, assignments=asn, sens=mc.constants.SENSDICT["LFKF"])
+    p = cs2.get_probs()
+    cnt = np.bincount(np.asarray(asn, dtype=np.int64), minlength=25)
+    np.testing.assert_allclose(p, 1. / (cnt[np.asarray(asn, dtype=np.int64)] * (cnt > 0).sum()), rtol=1e-5)

@@ -137,7 +137,7 @@ def test_kmeanspp_localsearch_csr_consistent(mcg, ctx, mco):
     assert np.array_equal(a[clear], ref.argmin(1)[clear])
 
 
-@pytest.mark.parametrize("sens", [0, 1, 3, 4])
+@pytest.mark.parametrize("sens", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("weighted", [False, True])
 def test_coreset_probs_and_sample_bitexact(mcg, ctx, mco, sens, weighted):
     say(f"coreset sens={sens} weighted={weighted}")

@@ -709,6 +709,27 @@ def test_fixed_draw_semantics(mco):
 
 
 # ---- a7: coresets -----------------------------------------------------------------------
+@pytest.mark.parametrize("weighted", [False, True])
+def test_coreset_probs_lfkf(mco, weighted):
+    """LFKF (make_gmm_sampler, coreset.h:341-381, alpha_est = 0): p_i = 2T/W_a
+    normalised, i.e. (1/W_a) / sum_j 1/W_{a_j}; unweighted 1 / (n_a * nonempty)."""
+    rng = np.random.default_rng(12)
+    n, k = 3000, 9
+    asn = rng.integers(0, k, n).astype(np.uint32)
+    asn[asn == 4] = 5                       # an empty cluster
+    costs = rng.exponential(1., n)
+    w = rng.uniform(.5, 2., n) if weighted else None
+    p = mco.coreset_probs(2, k, costs, asn, w)
+    wf = w.astype(np.float32).astype(np.float64) if weighted else np.ones(n)
+    Wa = np.bincount(asn, weights=wf, minlength=k)
+    want = (1. / Wa[asn]) / (1. / Wa[asn]).sum()
+    np.testing.assert_allclose(p, want, rtol=1e-6)
+    if not weighted:
+        cnt = np.bincount(asn, minlength=k)
+        np.testing.assert_allclose(p, 1. / (cnt[asn] * (cnt > 0).sum()), rtol=1e-6)
+    assert abs(float(p.astype(np.float64).sum()) - 1.) < 1e-5
+
+
 @pytest.mark.parametrize("sens", [0, 3, 4])
 def test_coreset_probs_invariants(mco, sens):
     rng = np.random.default_rng(sens)

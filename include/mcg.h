@@ -180,8 +180,10 @@ int mcg_centroids_hard(mcg_data *ds, const uint32_t *asn, const double *weights,
  * L1, TVD and L2 use their median policies (set_centroids_l1 / _tvd / _l2,
  * centroid.h:434-647, with the restart of centroid.h:442-480): coordinate-wise
  * medians (dense and CSR; weighted only dense, as in the reference) and the
- * dense geometric median (Weiszfeld, blaze_adaptor.h:614-690).  Weighted median
- * centers need positive weights; CSR medians need non-negative values. */
+ * geometric median (Weiszfeld: dense rows as blaze_adaptor.h:614-690; unweighted
+ * CSR rows as csc.h:1131-1203 in FP64, with the member costs indexed by position
+ * where the reference indexes them by row id).  Weighted median centers need
+ * positive weights; CSR L1 / TVD medians need non-negative values. */
 int mcg_lloyd(mcg_data *ds, int msr, double prior, float *centers, uint64_t k,
               const double *weights, double eps, uint64_t maxiter, int precision,
               uint32_t *asn, float *costs, double *initcost, double *finalcost,

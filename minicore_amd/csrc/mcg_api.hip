@@ -880,12 +880,12 @@ static void lloyd_median(Data &D, int msr, double prior, float *centers, uint64_
     const int pol = median_policy(msr);
     if (D.csr) {
         if (weights)
-            MCG_THROW(MCG_EUNSUPPORTED, pol == 2 ? "Not implemented: weighted TVD median for CSparseMatrix"
-                                                 : "Not implemented: weighted L1 median for CSparseMatrix");
-        if (pol == 3)
-            MCG_THROW(MCG_EUNSUPPORTED, "geometric median of CSR rows: the reference indexes its per-member costs "
-                                        "by row id (csc.h:1170-1177), which has no defined result");
-        REQUIRE(!csr_has_negative(D, c.stream), "CSR L1/TVD median centers need non-negative values");
+            MCG_THROW(MCG_EUNSUPPORTED, pol == 2   ? "Not implemented: weighted TVD median for CSparseMatrix"
+                                        : pol == 3 ? "Not implemented: weighted geometric median for CSparseMatrix"
+                                                   : "Not implemented: weighted L1 median for CSparseMatrix");
+        // the geometric median of CSR rows follows csc.h:1131-1203 with its member
+        // costs indexed by position (the reference indexes them by row id)
+        if (pol != 3) REQUIRE(!csr_has_negative(D, c.stream), "CSR L1/TVD median centers need non-negative values");
     }
     DBuf sw, sa_out, sk_out;
     const double *W = (const double *)in_dev(c, weights, n * 8, sw);

@@ -400,20 +400,35 @@ __global__ __launch_bounds__(256) void k_seg_median(const float *__restrict__ va
 
 // ---- geometric median (Weiszfeld) ----------------------------------------------------
 __global__ void k_geo_init(const uint32_t *__restrict__ counts, uint64_t k, float *__restrict__ prev,
-                           uint32_t *__restrict__ active) {
+                           double *__restrict__ prevd, uint32_t *__restrict__ active) {
     const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= k) return;
     prev[c] = FLT_MAX;
+    prevd[c] = DBL_MAX;
     active[c] = counts[c] >= 2 ? 1u : 0u;
 }
 
+// CSR rows in cluster order as dense rows (Xp cleared beforehand): one wave per row
+__global__ __launch_bounds__(256) void k_scatter_perm_csr(const float *__restrict__ val,
+                                                          const uint32_t *__restrict__ idx,
+                                                          const uint64_t *__restrict__ ptr, uint32_t d,
+                                                          const uint32_t *__restrict__ perm, uint64_t n,
+                                                          float *__restrict__ Xp) {
+    const uint64_t p = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (p >= n) return;
+    const uint32_t r = perm[p];
+    for (uint64_t q = ptr[r] + (threadIdx.x & 63); q < ptr[r + 1]; q += 64) Xp[p * d + idx[q]] = val[q];
+}
+
 // per member (sorted position p): distance to its cluster's current center, FP64
-// sequential FMA, rounded to float, floored; tq = w * dist (or dist)
+// sequential FMA, rounded to float (kept in FP64 when `exact`), floored;
+// tq = w * dist (or dist)
 __global__ __launch_bounds__(256) void k_geo_dist(const float *__restrict__ Xp, uint32_t d,
                                                   const uint32_t *__restrict__ perm, uint64_t n,
                                                   const uint32_t *__restrict__ A, const float *__restrict__ C,
                                                   const uint32_t *__restrict__ active, const double *__restrict__ w,
-                                                  double floor_, double *__restrict__ cq, double *__restrict__ tq) {
+                                                  double floor_, int exact, double *__restrict__ cq,
+                                                  double *__restrict__ tq) {
     __shared__ float xs[256][33];
     const int tid = threadIdx.x;
     const uint64_t p0 = (uint64_t)blockIdx.x * 256, p = p0 + tid;
@@ -440,7 +455,7 @@ __global__ __launch_bounds__(256) void k_geo_dist(const float *__restrict__ Xp, 
         }
     }
     if (!on) return;
-    double v = (double)(float)__dsqrt_rn(s);
+    double v = exact ? __dsqrt_rn(s) : (double)(float)__dsqrt_rn(s);
     if (v != v) v = floor_;
     v = v > floor_ ? v : floor_;
     cq[p] = v;
@@ -460,12 +475,15 @@ __device__ double block_tree_sum(double s, double *sm) {
     return r;
 }
 
-// per cluster: the round's cost, the stop rule (blaze_adaptor.h:638-646), Q = sum q_i
+// per cluster: the round's cost, the stop rule (dense: blaze_adaptor.h:638-646, the
+// cost in float; CSR rule: csc.h:1150-1158, the cost in FP64, stopping when it is
+// within eps of the previous one or above it), Q = sum q_i
 __global__ __launch_bounds__(256) void k_geo_sums(const uint32_t *__restrict__ perm,
                                                   const uint32_t *__restrict__ counts,
                                                   const uint32_t *__restrict__ off, const double *__restrict__ cq,
                                                   double *__restrict__ tq, const double *__restrict__ w,
-                                                  double eps, float *__restrict__ prev, uint32_t *__restrict__ active,
+                                                  double eps, int csr_rule, float *__restrict__ prev,
+                                                  double *__restrict__ prevd, uint32_t *__restrict__ active,
                                                   double *__restrict__ Q, uint32_t *__restrict__ nanflag) {
     __shared__ double sm[256];
     __shared__ int stop;
@@ -474,14 +492,21 @@ __global__ __launch_bounds__(256) void k_geo_sums(const uint32_t *__restrict__ p
     const uint32_t m = counts[c], b = off[c];
     double s = 0.;
     for (uint32_t q = threadIdx.x; q < m; q += 256) s = __dadd_rn(s, tq[b + q]);
-    const float curf = (float)block_tree_sum(s, sm);
+    const double cur = block_tree_sum(s, sm);
+    const float curf = (float)cur;
     if (threadIdx.x == 0) {
-        const float dist = prev[c] - curf;
         stop = 0;
-        if ((double)dist <= eps) stop = 1;
-        else if (dist != dist) { stop = 1; *nanflag = 1u; }
+        if (csr_rule) {
+            const double dist = fabs(prevd[c] - cur);
+            if (dist <= eps || cur > prevd[c]) stop = 1;
+            else if (dist != dist) { stop = 1; *nanflag = 1u; }
+        } else {
+            const float dist = prev[c] - curf;
+            if ((double)dist <= eps) stop = 1;
+            else if (dist != dist) { stop = 1; *nanflag = 1u; }
+        }
         if (stop) active[c] = 0u;
-        else prev[c] = curf;
+        else { prev[c] = curf; prevd[c] = cur; }
     }
     __syncthreads();
     if (stop) return;
@@ -544,15 +569,18 @@ static void geo_median(Data &D, const float *Xp, const double *w, uint64_t k, co
     const uint32_t d = (uint32_t)D.d;
     double *cq = (double *)c.s[S_MED0].reserve(2 * n * sizeof(double));
     double *tq = cq + n;
-    char *buf = (char *)c.s[S_MED1].reserve(64 + k * 16);
+    char *buf = (char *)c.s[S_MED1].reserve(64 + k * 24);
     uint32_t *flags = (uint32_t *)buf;   // [0] nan, [1] active count
     float *prev = (float *)(buf + 64);
     uint32_t *active = (uint32_t *)(buf + 64 + k * 4);
     double *Q = (double *)(buf + 64 + k * 8);
+    double *prevd = (double *)(buf + 64 + k * 16);
     fill_u32(flags, 0u, 2, st);
-    hipLaunchKernelGGL(k_geo_init, dim3((uint32_t)cdiv(k, 256)), dim3(256), 0, st, so.counts, k, prev, active);
-    // weighted: max(d, 1e-40) in the weighted overload, 1e-30 in the unweighted one
-    const double floor_ = w ? (double)1e-40f : (double)1e-30f;
+    hipLaunchKernelGGL(k_geo_init, dim3((uint32_t)cdiv(k, 256)), dim3(256), 0, st, so.counts, k, prev, prevd, active);
+    // dense rows: max(d, 1e-40) in the weighted overload, 1e-30 in the unweighted one
+    // (blaze_adaptor.h:622,664); CSR rows: 1e-80 in FP64 (csc.h:1148)
+    const bool csr_rule = D.csr;
+    const double floor_ = csr_rule ? 1e-80 : w ? (double)1e-40f : (double)1e-30f;
     const dim3 gu((uint32_t)cdiv(d, MED_COLS), (uint32_t)k);
     uint64_t rounds = 0, batch = 4;
     uint32_t hf[2];
@@ -560,9 +588,9 @@ static void geo_median(Data &D, const float *Xp, const double *w, uint64_t k, co
         const uint64_t nb = std::min<uint64_t>(batch, GEO_MAX_ROUNDS - rounds);
         for (uint64_t t = 0; t < nb; ++t) {
             hipLaunchKernelGGL(k_geo_dist, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, Xp, d, so.perm, n, A, C,
-                               active, w, floor_, cq, tq);
+                               active, w, floor_, csr_rule ? 1 : 0, cq, tq);
             hipLaunchKernelGGL(k_geo_sums, dim3((uint32_t)k), dim3(256), 0, st, so.perm, so.counts, so.off, cq, tq, w,
-                               0., prev, active, Q, flags);
+                               0., csr_rule ? 1 : 0, prev, prevd, active, Q, flags);
             hipLaunchKernelGGL(k_geo_update, gu, dim3(256), 0, st, Xp, d, so.perm, so.counts, so.off, tq, active, C);
         }
         rounds += nb;
@@ -616,6 +644,21 @@ static void median_centers_core(Data &D, int pol, const double *wm, const double
     const uint32_t d = (uint32_t)D.d;
     const uint32_t min_m = single_raw ? 2u : 1u;
     const double *w = wm;
+    if (D.csr && pol == 3) {
+        // the geometric median of CSR rows (unweighted; csc.h:1131-1203 with the
+        // member costs indexed by position): the members as dense rows in cluster
+        // order, then the Weiszfeld rounds under the CSR rule
+        REQUIRE(!wm, "Not implemented: weighted geometric median for CSparseMatrix");
+        REQUIRE((double)n * (double)d * 4. < 64e9, "the CSR geometric median densifies the rows: n x d too large");
+        hipLaunchKernelGGL(k_med_single_csr, dim3((uint32_t)k), dim3(256), 0, st, D.val.as<const float>(),
+                           D.idx.as<const uint32_t>(), D.ptr.as<const uint64_t>(), d, so.perm, so.counts, so.off, C);
+        float *Xp = (float *)c.s[S_MED3].reserve(n * d * sizeof(float));
+        fill_u32((uint32_t *)Xp, 0u, n * d, st);
+        hipLaunchKernelGGL(k_scatter_perm_csr, dim3((uint32_t)cdiv(n, 4)), dim3(256), 0, st, D.val.as<const float>(),
+                           D.idx.as<const uint32_t>(), D.ptr.as<const uint64_t>(), d, so.perm, n, Xp);
+        geo_median(D, Xp, nullptr, k, so, A, C, st);
+        return;
+    }
     if (D.csr) {
         csr_median(D, pol == 2, k, so, A, single_raw, C, st);
         return;

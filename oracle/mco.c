@@ -1489,24 +1489,35 @@ static float csr_median_value(const float *s, uint64_t nel, uint64_t m, int tvd)
     #undef FULL
 }
 
+/* csr_rule: the geometric median of CSR rows (csc.h:1131-1203, unweighted, with the
+ * member costs indexed by position): distances kept in FP64, floored at 1e-80, the
+ * cost summed in FP64, stopping when it is within eps of the previous one or above it. */
 static void geo_median(const float *X, uint64_t d, const uint64_t *mem, uint64_t m, const double *w, float *cur,
-                       double eps, double *cq, double *qq) {
-    const double floor_ = w ? (double)1e-40f : (double)1e-30f;
+                       double eps, double *cq, double *qq, int csr_rule) {
+    const double floor_ = csr_rule ? 1e-80 : w ? (double)1e-40f : (double)1e-30f;
     float prev = FLT_MAX;
+    double prevd = DBL_MAX;
     double *acc = (double *)malloc(4 * d * sizeof(double));
     for (uint64_t it = 0; it < 100000; ++it) {
         for (uint64_t q = 0; q < m; ++q) {
             const float *x = X + mem[q] * d;
             double s = 0.;
             for (uint64_t j = 0; j < d; ++j) { const double t = (double)x[j] - (double)cur[j]; s = fma(t, t, s); }
-            double c = (double)(float)sqrt(s);
+            double c = csr_rule ? sqrt(s) : (double)(float)sqrt(s);
             if (isnan(c)) c = floor_;
             cq[q] = c > floor_ ? c : floor_;
             qq[q] = w ? w[mem[q]] * cq[q] : cq[q];
         }
-        const float curf = (float)tree_sum256(qq, m);
-        const float dist = prev - curf;
-        if ((double)dist <= eps) break;
+        const double curd = tree_sum256(qq, m);
+        const float curf = (float)curd;
+        if (csr_rule) {
+            const double dist = fabs(prevd - curd);
+            if (dist <= eps || curd > prevd || isnan(dist)) break;
+        } else {
+            const float dist = prev - curf;
+            if ((double)dist <= eps || isnan(dist)) break;
+        }
+        prevd = curd;
         for (uint64_t q = 0; q < m; ++q) qq[q] = w ? w[mem[q]] / cq[q] : 1. / cq[q];
         const double S = tree_sum256(qq, m), inv = 1. / S;
         memset(acc, 0, 4 * d * sizeof(double));
@@ -1542,6 +1553,12 @@ static void median_core(int pol, const float *X, const float *val, const uint32_
     for (uint64_t i = 0; i < n; ++i) rs[i] = val ? row_sum_csr(val, indptr[i], indptr[i + 1]) : row_sum_dense(X + i * d, d);
     float *vf = (float *)malloc((n ? n : 1) * sizeof(float));
     double *vd = (double *)malloc((n ? n : 1) * sizeof(double)), *qq = (double *)malloc((n ? n : 1) * sizeof(double));
+    float *Xdense = NULL;   /* CSR rows as dense rows, for the geometric median */
+    if (val && pol == 3) {
+        Xdense = (float *)calloc(n * d + 1, sizeof(float));
+        for (uint64_t i = 0; i < n; ++i)
+            for (uint64_t p = indptr[i]; p < indptr[i + 1]; ++p) Xdense[i * d + idx[p]] = val[p];
+    }
     for (uint64_t c = 0; c < k; ++c) {
         const uint64_t *mm = mem + off[c], m = off[c + 1] - off[c];
         float *ctr = C + c * d;
@@ -1554,7 +1571,7 @@ static void median_core(int pol, const float *X, const float *val, const uint32_
             } else memcpy(ctr, X + r * d, d * sizeof(float));
             continue;
         }
-        if (pol == 3) { geo_median(X, d, mm, m, wm, ctr, eps, vd, qq); continue; }
+        if (pol == 3) { geo_median(val ? Xdense : X, d, mm, m, wm, ctr, eps, vd, qq, val != NULL); continue; }
         if (val) {
             /* per column, the stored values of the members */
             for (uint64_t j = 0; j < d; ++j) {
@@ -1603,7 +1620,7 @@ static void median_core(int pol, const float *X, const float *val, const uint32_
             }
         }
     }
-    free(off); free(mem); free(cur); free(rs); free(vf); free(vd); free(qq);
+    free(off); free(mem); free(cur); free(rs); free(vf); free(vd); free(qq); free(Xdense);
 }
 
 
@@ -1616,7 +1633,7 @@ int mco_median_centers(int msr, const float *X, const float *val, const uint32_t
                        double eps) {
     const int pol = median_policy(msr);
     if (!pol) return -2;
-    if (val && (w || pol == 3)) return -2;
+    if (val && w) return -2;
     uint64_t *cnt = (uint64_t *)calloc(k + 1, sizeof(uint64_t));
     uint64_t draw = 0;
     for (;;) {
@@ -1654,7 +1671,7 @@ int mco_lloyd_median(int msr, const float *X, const float *val, const uint32_t *
                      uint64_t maxiter, uint32_t *asn, double *cost, double *initcost, double *finalcost,
                      uint64_t *numiter) {
     if (!median_policy(msr)) return -2;
-    if (val && (w || msr == MCO_L2)) return -2;
+    if (val && w) return -2;
     assign_generic(msr, X, val, idx, indptr, n, d, C, k, prior, asn, cost, 0);
     float fcost = (float)total_cost(cost, w, n);
     *initcost = fcost;

@@ -97,6 +97,30 @@ def test_median_centers_single_step_csr_bitexact(mcg, ctx, mco):
         ds.close()
 
 
+def test_geometric_median_csr(mcg, ctx, mco):
+    """The geometric median of CSR rows (csc.h:1131-1203 with position-indexed member
+    costs: FP64 distances, 1e-80 floor, stop when the FP64 cost is not below the last):
+    one M-step from the GPU's own assignment equals the oracle's centers; a full run
+    matches the oracle's assignment and cost."""
+    say("geometric median csr")
+    n, d, k = 1500, 48, 5
+    data, idx, ptr = poisson_csr(n, d, 0.3, 21, normalize=False)
+    X = mco.CSR(data, idx, ptr, (n, d))
+    C0 = X.todense()[:k] + np.float32(.25)
+    ds = mcg.Dataset.csr(data, idx, ptr, (n, d), ctx)
+    a0, c0 = ds.assign(L2, C0, with_costs=True)
+    Cm, _, _ = mco.median_centers(L2, X, a0, c0.astype(np.float64), C0)
+    got = ds.lloyd(L2, C0, eps=-1., maxiter=0)
+    assert np.array_equal(got["centers"], Cm if got["numiter"] == 1 else C0)
+    got = ds.lloyd(L2, C0, eps=1e-10, maxiter=8)
+    ref = mco.lloyd_median(L2, X, C0, eps=1e-10, maxiter=8)
+    assert got["numiter"] == ref["numiter"]
+    np.testing.assert_allclose(got["centers"], ref["centers"], rtol=1e-4, atol=1e-5)
+    assert np.mean(got["asn"] == ref["asn"]) > 0.999
+    assert got["finalcost"] == pytest.approx(ref["finalcost"], rel=1e-5)
+    ds.close()
+
+
 def test_median_restart_empty_cluster(mcg, ctx, mco):
     say("median restart")
     X, C0 = _data(L1, n=2000, d=8, k=5, seed=9)
@@ -119,8 +143,8 @@ def test_median_errors(mcg, ctx):
         ds.lloyd(L1, C0, weights=np.ones(n), maxiter=2)
     assert e.value.code == -4 and "weighted L1 median" in str(e.value)
     with pytest.raises(mcg.MCGError) as e:
-        ds.lloyd(L2, C0, maxiter=2)
-    assert e.value.code == -4
+        ds.lloyd(L2, C0, weights=np.ones(n), maxiter=2)
+    assert e.value.code == -4 and "weighted geometric median" in str(e.value)
     neg = data.copy()
     neg[3] = -1.
     dn = mcg.Dataset.csr(neg, idx, ptr, (n, d), ctx)

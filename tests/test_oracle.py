@@ -908,6 +908,34 @@ def test_csr_tvd_even_rule_and_dense_odd(mco):
     np.testing.assert_allclose(C, Cd, rtol=1e-6)
 
 
+def test_csr_geometric_median_optimal(mco):
+    """CSR geometric median (csc.h:1131-1203, position-indexed member costs, FP64,
+    1e-80 floor): each center's cost within 1e-7 of scipy's Nelder-Mead optimum of the
+    same rows, and at least as low as the dense rule's (float costs) on the same rows."""
+    from scipy.optimize import minimize
+    rng = np.random.default_rng(17)
+    n, d = 90, 4
+    Xd = np.abs(rng.normal(size=(n, d))).astype(np.float32)
+    Xd[Xd < .3] = 0.
+    ptr = np.zeros(n + 1, np.uint64)
+    idx, val = [], []
+    for i in range(n):
+        nz = np.nonzero(Xd[i])[0]
+        idx += nz.tolist(); val += Xd[i, nz].tolist()
+        ptr[i + 1] = len(idx)
+    X = mco.CSR(np.array(val, np.float32), np.array(idx, np.uint32), ptr, (n, d))
+    asn = (np.arange(n) % 3).astype(np.uint32)
+    C0 = Xd[:3] + np.float32(.2)
+    C, _, _ = mco.median_centers(mco.L2, X, asn.copy(), np.ones(n), C0.copy())
+    Cd, _, _ = mco.median_centers(mco.L2, Xd, asn.copy(), np.ones(n), C0.copy())
+    for c in range(3):
+        P = Xd[asn == c].astype(np.float64)
+        f = lambda y: float(np.linalg.norm(P - y, axis=1).sum())
+        best = minimize(f, P.mean(0), method="Nelder-Mead", options=dict(xatol=1e-11, fatol=1e-13, maxiter=40000))
+        assert f(C[c].astype(np.float64)) <= best.fun * (1 + 1e-7)
+        assert f(C[c].astype(np.float64)) <= f(Cd[c].astype(np.float64)) * (1 + 1e-9)
+
+
 def test_lloyd_median_rules(mco):
     from minicore_amd.datagen import gaussian_mixture
     X, lab, means = gaussian_mixture(1500, 6, 5, seed=12, lo=-3, hi=3)

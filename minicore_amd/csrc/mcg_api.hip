@@ -534,11 +534,10 @@ static void check_hard_msr(Data &D, int msr, const double *weights) {
     }
     if (!D.csr) return;
     if (weights)
-        MCG_THROW(MCG_EUNSUPPORTED, msr == MCG_TVD ? "Not implemented: weighted TVD median for CSparseMatrix"
-                                                   : "Not implemented: weighted L1 median for CSparseMatrix");
-    if (msr == MCG_L2)
-        MCG_THROW(MCG_EUNSUPPORTED, "geometric median of CSR rows: the reference indexes its per-member costs by "
-                                    "row id (csc.h:1170-1177), which has no defined result");
+        MCG_THROW(MCG_EUNSUPPORTED, msr == MCG_TVD  ? "Not implemented: weighted TVD median for CSparseMatrix"
+                                    : msr == MCG_L2 ? "Not implemented: weighted geometric median for CSparseMatrix"
+                                                    : "Not implemented: weighted L1 median for CSparseMatrix");
+    if (msr == MCG_L2) return;   // the CSR geometric median takes any values
     HIP_CHECK(hipSetDevice(D.ctx->device));
     REQUIRE(!csr_has_negative(D, D.ctx->stream), "CSR L1/TVD median centers need non-negative values");
 }

@@ -722,9 +722,9 @@ int mco_minibatch(int msr, const float *X, const float *val, const uint32_t *idx
                   uint64_t seed, uint32_t *asn, double *cost, double *initcost, double *finalcost,
                   uint64_t *numiter, int nthreads) {
     if (!mbsize || !maxiter || !freq || (!with_rep && mbsize > n)) return -2;
-    {   /* median updates (solve.h:555-572): CSR only unweighted L1 / TVD; weights by rank need mbsize <= n */
+    {   /* median updates (solve.h:555-572): CSR only unweighted; weights by rank need mbsize <= n */
         const int pol = median_policy(msr);
-        if (pol && val && (w || pol == 3)) return -2;
+        if (pol && val && w) return -2;
         if (pol && w && pol != 3 && mbsize > n) return -2;
     }
     uint64_t j = 0, iter = 0;

@@ -166,16 +166,21 @@ def test_minibatch_median_dense(mcg, ctx, mco, msr, weighted, with_rep):
     _median_compare(got, ref)
 
 
-def test_minibatch_median_csr_l1(mcg, ctx, mco):
-    say("minibatch median csr L1")
+@pytest.mark.parametrize("msr", [0, 1])
+def test_minibatch_median_csr_l1(mcg, ctx, mco, msr):
+    """CSR rows: the L1 median and the geometric median (FP64 Weiszfeld under the CSR
+    rule) inside the minibatch driver."""
+    say(f"minibatch median csr msr={msr}")
     n, d, k = 3000, 60, 6
     data, idx, ptr = poisson_csr(n, d, 0.4, 14, normalize=False)
     X = mco.CSR(data, idx, ptr, (n, d))
     C0 = X.todense()[np.random.default_rng(3).choice(n, k, replace=False)].copy()
+    if msr == 1:
+        C0 += np.float32(.25)
     kw = dict(mbsize=700, maxiter=9, freq=3, reseed_after=1, seed=8)
-    ref = mco.minibatch(0, X, C0, **kw)
+    ref = mco.minibatch(msr, X, C0, **kw)
     ds = mcg.Dataset.csr(data, idx, ptr, (n, d), ctx)
-    got = ds.minibatch(0, C0, **kw)
+    got = ds.minibatch(msr, C0, **kw)
     ds.close()
     agree = np.mean(got["asn"] == ref["asn"])
     say("  agree", agree)
@@ -226,7 +231,7 @@ def test_minibatch_median_unsupported(mcg, ctx):
         ds.minibatch(0, C0, weights=np.ones(n), mbsize=100, maxiter=3, freq=1)
     assert e.value.code == -4
     with pytest.raises(mcg.MCGError) as e:
-        ds.minibatch(1, C0, mbsize=100, maxiter=3, freq=1)
+        ds.minibatch(1, C0, weights=np.ones(n), mbsize=100, maxiter=3, freq=1)
     assert e.value.code == -4
     ds.close()
     dd = mcg.Dataset.dense(np.random.default_rng(2).random((n, d)).astype(np.float32), ctx)

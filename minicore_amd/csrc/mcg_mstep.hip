@@ -379,7 +379,7 @@ MStepOut dense_mstep(Data &ds, const uint32_t *asn, const double *w, uint64_t k,
 // distinct columns, so lanes never collide), and the 4 accumulators are added
 // in a fixed order.
 // ---------------------------------------------------------------------------
-constexpr uint32_t CSR_CW = 2048;
+constexpr uint32_t CSR_CW = 1024;
 
 __global__ __launch_bounds__(256) void k_csr_mean(const float *__restrict__ val, const uint32_t *__restrict__ idx,
                                                   const uint64_t *__restrict__ ptr, uint32_t d,
@@ -409,18 +409,41 @@ __global__ __launch_bounds__(256) void k_csr_mean(const float *__restrict__ val,
             if (w) wq = w[r];
         }
         const uint32_t nrow = min(64u, nm - g);
-        for (uint32_t t = 0; t < nrow; ++t) {
-            const uint64_t b = bcast_u64(rb, (int)t), e = bcast_u64(re, (int)t);
-            const double wt = bcast_f64(wq, (int)t);
-            for (uint64_t p = b + lane; p < e; p += 64) {
-                const uint32_t j = idx[p];
-                if (j >= col1) break;
-                double *a = &acc[wave][j - col0];
-                *a = __dadd_rn(*a, __dmul_rn(wt, (double)val[p]));
+        // rows in batches of RB: the first 64 window entries of every row of a batch
+        // are loaded before any is added; then the rows are added in order, each in
+        // full (a row's entries past the first 64 before the next row)
+        constexpr uint32_t RB = 4;
+        for (uint32_t t0 = 0; t0 < nrow; t0 += RB) {
+            uint64_t bb[RB], ee[RB];
+            double ww[RB];
+            uint32_t jv[RB];
+            float vv[RB];
+#pragma unroll
+            for (uint32_t u = 0; u < RB; ++u) {
+                const uint32_t t = t0 + u < nrow ? t0 + u : t0;
+                bb[u] = bcast_u64(rb, (int)t);
+                ee[u] = t0 + u < nrow ? bcast_u64(re, (int)t) : bb[u];
+                ww[u] = bcast_f64(wq, (int)t);
+                const uint64_t p = bb[u] + lane;
+                jv[u] = p < ee[u] ? idx[p] : 0xFFFFFFFFu;
+                vv[u] = p < ee[u] ? val[p] : 0.f;
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+#pragma unroll
+            for (uint32_t u = 0; u < RB; ++u) {
+                if (jv[u] < col1) {
+                    double *a = &acc[wave][jv[u] - col0];
+                    *a = __dadd_rn(*a, __dmul_rn(ww[u], (double)vv[u]));
+                    for (uint64_t p = bb[u] + 64 + lane; p < ee[u]; p += 64) {
+                        const uint32_t j = idx[p];
+                        if (j >= col1) break;
+                        double *a2 = &acc[wave][j - col0];
+                        *a2 = __dadd_rn(*a2, __dmul_rn(ww[u], (double)val[p]));
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            }
         }
     }
     __syncthreads();

@@ -374,10 +374,10 @@ MStepOut dense_mstep(Data &ds, const uint32_t *asn, const double *w, uint64_t k,
 
 // ---------------------------------------------------------------------------
 // CSR M-step.  After the same sort, one block per (cluster, column window of
-// CSR_CW columns); 4 waves each own an LDS FP64 accumulator and take members
-// q = wave, wave+4, ...; a wave walks its rows in order (row entries have
-// distinct columns, so lanes never collide), and the 4 accumulators are added
-// in a fixed order.
+// CSR_CW columns, chunk of the cluster's members); 4 waves each own an LDS FP64
+// accumulator and take members in groups of 64 in turn; a wave walks its rows in
+// order (row entries have distinct columns, so lanes never collide), the 4
+// accumulators are added in a fixed order, and the chunks in chunk order.
 // ---------------------------------------------------------------------------
 constexpr uint32_t CSR_CW = 1024;
 
@@ -386,15 +386,18 @@ __global__ __launch_bounds__(256) void k_csr_mean(const float *__restrict__ val,
                                                   const uint32_t *__restrict__ perm,
                                                   const uint32_t *__restrict__ counts,
                                                   const uint32_t *__restrict__ off, const double *__restrict__ w,
-                                                  float *__restrict__ cnew, double *__restrict__ cw) {
+                                                  uint32_t k, double *__restrict__ part) {
     __shared__ double acc[4][CSR_CW];
-    const uint32_t c = blockIdx.y;
+    const uint32_t c = blockIdx.y, z = blockIdx.z, Z = gridDim.z;
     const uint32_t col0 = blockIdx.x * CSR_CW;
     const uint32_t col1 = min(d, col0 + CSR_CW);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (uint32_t j = tid; j < 4 * CSR_CW; j += 256) (&acc[0][0])[j] = 0.;
     __syncthreads();
-    const uint32_t m0 = off[c], nm = counts[c];
+    // member chunk z of the cluster's members (contiguous, in sorted order)
+    const uint32_t cm = counts[c], chunk = (cm + Z - 1) / Z;
+    const uint32_t q0 = min(cm, z * chunk), q1 = min(cm, q0 + chunk);
+    const uint32_t m0 = off[c] + q0, nm = q1 - q0;
     for (uint32_t g = wave * 64; g < nm; g += 256) {
         // lanes find the first entry >= col0 of 64 member rows at once
         const uint32_t q = g + lane;
@@ -447,25 +450,46 @@ __global__ __launch_bounds__(256) void k_csr_mean(const float *__restrict__ val,
         }
     }
     __syncthreads();
-    if (nm == 0) return;
-    const double wt = cw[c];
+    double *pz = part + ((uint64_t)z * k + c) * d;
     for (uint32_t j = col0 + tid; j < col1; j += 256) {
         const uint32_t jj = j - col0;
-        double s = __dadd_rn(__dadd_rn(acc[0][jj], acc[1][jj]), __dadd_rn(acc[2][jj], acc[3][jj]));
-        cnew[(uint64_t)c * d + j] = (float)__ddiv_rn(s, wt);
+        pz[j] = __dadd_rn(__dadd_rn(acc[0][jj], acc[1][jj]), __dadd_rn(acc[2][jj], acc[3][jj]));
     }
 }
 
-// per-cluster weight (sequential over members) -> cw[c]
-__global__ void k_cluster_weight(const uint32_t *__restrict__ perm, const uint32_t *__restrict__ counts,
-                                 const uint32_t *__restrict__ off, const double *__restrict__ w, uint64_t k,
-                                 double *__restrict__ cw) {
-    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= k) return;
-    double s = 0.;
+// the chunks' partial sums added in chunk order, divided by the cluster's weight
+// (empty clusters are left as they are)
+__global__ void k_csr_mean_final(const double *__restrict__ part, uint32_t Z, uint32_t k, uint32_t d,
+                                 const uint32_t *__restrict__ counts, const double *__restrict__ cw,
+                                 float *__restrict__ cnew) {
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (uint64_t)k * d) return;
+    const uint32_t c = (uint32_t)(e / d);
+    if (counts[c] == 0) return;
+    double s = part[e];
+    for (uint32_t z = 1; z < Z; ++z) s = __dadd_rn(s, part[(uint64_t)z * k * d + e]);
+    cnew[e] = (float)__ddiv_rn(s, cw[c]);
+}
+
+// per-cluster weight -> cw[c]: one block per cluster, 256 strided partial sums
+// added as a binary tree (a fixed order)
+__global__ __launch_bounds__(256) void k_cluster_weight(const uint32_t *__restrict__ perm,
+                                                        const uint32_t *__restrict__ counts,
+                                                        const uint32_t *__restrict__ off,
+                                                        const double *__restrict__ w, uint64_t k,
+                                                        double *__restrict__ cw) {
+    __shared__ double sm[256];
+    const uint32_t c = blockIdx.x;
     const uint32_t m0 = off[c], nm = counts[c];
-    for (uint32_t q = 0; q < nm; ++q) s = __dadd_rn(s, w ? w[perm[m0 + q]] : 1.);
-    cw[c] = s;
+    double s = 0.;
+    for (uint32_t q = threadIdx.x; q < nm; q += 256) s = __dadd_rn(s, w ? w[perm[m0 + q]] : 1.);
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sm[threadIdx.x] = __dadd_rn(sm[threadIdx.x], sm[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) cw[c] = sm[0];
 }
 
 MStepOut csr_mstep(Data &ds, const uint32_t *asn, const double *w, uint64_t k, float *Cnew, hipStream_t st) {
@@ -473,11 +497,18 @@ MStepOut csr_mstep(Data &ds, const uint32_t *asn, const double *w, uint64_t k, f
     SortOut so = sort_by_cluster(c, asn, ds.n, k, st);
     double *ccost = (double *)c.s[S_CCOST].reserve(2 * k * sizeof(double));
     double *total = (double *)c.s[S_TOTAL].reserve(4 * sizeof(double));
-    hipLaunchKernelGGL(k_cluster_weight, dim3((uint32_t)cdiv(k, 256)), dim3(256), 0, st, so.perm, so.counts, so.off,
-                       w, k, ccost + k);
-    dim3 g((uint32_t)cdiv(ds.d, CSR_CW), (uint32_t)k);
+    hipLaunchKernelGGL(k_cluster_weight, dim3((uint32_t)k), dim3(256), 0, st, so.perm, so.counts, so.off, w, k,
+                       ccost + k);
+    // up to 8 member chunks per cluster, so a large cluster is spread over blocks
+    // (partial sums: Z x k x d FP64)
+    const uint64_t kd = k * ds.d;
+    const uint32_t Z = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(8, (uint64_t)(2e9 / 8.) / kd));
+    double *part = (double *)c.s[S_PART].reserve((uint64_t)Z * kd * sizeof(double));
+    dim3 g((uint32_t)cdiv(ds.d, CSR_CW), (uint32_t)k, Z);
     hipLaunchKernelGGL(k_csr_mean, g, dim3(256), 0, st, ds.val.as<const float>(), ds.idx.as<const uint32_t>(),
-                       ds.ptr.as<const uint64_t>(), (uint32_t)ds.d, so.perm, so.counts, so.off, w, Cnew, ccost + k);
+                       ds.ptr.as<const uint64_t>(), (uint32_t)ds.d, so.perm, so.counts, so.off, w, (uint32_t)k, part);
+    hipLaunchKernelGGL(k_csr_mean_final, dim3((uint32_t)cdiv(kd, 256)), dim3(256), 0, st, (const double *)part, Z,
+                       (uint32_t)k, (uint32_t)ds.d, so.counts, (const double *)(ccost + k), Cnew);
     return MStepOut{so.counts, total};
 }
 

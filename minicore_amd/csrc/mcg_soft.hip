@@ -134,11 +134,13 @@ template <int Q>
 __global__ __launch_bounds__(256) void k_soft_csc(const uint64_t *__restrict__ cptr, const uint32_t *__restrict__ crow,
                                                   const float *__restrict__ cval, uint32_t d,
                                                   const float *__restrict__ R, uint32_t k, uint32_t c0,
-                                                  const double *__restrict__ den, float *__restrict__ C) {
+                                                  double *__restrict__ part) {
     const uint64_t j = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (j >= d) return;
-    const uint64_t b = cptr[j], e = cptr[j + 1];
+    // entry chunk z of the column (contiguous, in order)
+    const uint64_t cb = cptr[j], ce = cptr[j + 1], Z = gridDim.y, chunk = (ce - cb + Z - 1) / Z;
+    const uint64_t b = min(ce, cb + blockIdx.y * chunk), e = min(ce, b + chunk);
     double acc[Q];
     bool ok[Q];
 #pragma unroll
@@ -171,8 +173,18 @@ __global__ __launch_bounds__(256) void k_soft_csc(const uint64_t *__restrict__ c
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const uint64_t c = c0 + lane + 64u * q;
-        if (ok[q]) C[c * d + j] = (float)(acc[q] / den[c]);
+        if (ok[q]) part[((uint64_t)blockIdx.y * k + c) * d + j] = acc[q];
     }
+}
+
+// the columns' chunk partials added in chunk order, over the center's denominator
+__global__ void k_soft_csc_final(const double *__restrict__ part, uint32_t Z, uint32_t k, uint32_t d,
+                                 const double *__restrict__ den, float *__restrict__ C) {
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (uint64_t)k * d) return;
+    double s = part[e];
+    for (uint32_t z = 1; z < Z; ++z) s += part[(uint64_t)z * k * d + e];
+    C[e] = (float)(s / den[e / d]);
 }
 
 static void build_csc(Data &ds) {
@@ -233,19 +245,26 @@ void soft_mstep(Data &ds, const float *R, const double *w, uint64_t k, float *Cn
     double *den = (double *)c.s[S_PARTQ].reserve(k * 8);
     hipLaunchKernelGGL(k_soft_den_part, dim3(S), dim3(256), 0, st, R, n, (uint32_t)k, w, rows_per, dpart);
     hipLaunchKernelGGL(k_soft_den_final, dim3((uint32_t)cdiv(k, 256)), dim3(256), 0, st, dpart, S, (uint32_t)k, den);
+    // up to 8 entry chunks per column, so the columns with many entries are spread
+    // over waves (partial sums: Z x k x d FP64)
+    const uint64_t kd = k * (uint64_t)d;
+    const uint32_t Z = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(8, (uint64_t)(2e9 / 8.) / kd));
+    double *part = (double *)c.s[S_PART].reserve((uint64_t)Z * kd * sizeof(double));
     for (uint32_t c0 = 0; c0 < k; c0 += 256) {
         const uint32_t q = (uint32_t)std::min<uint64_t>(4, cdiv(k - c0, 64));
-        const dim3 g((uint32_t)cdiv(d, 4)), blk(256);
+        const dim3 g((uint32_t)cdiv(d, 4), Z), blk(256);
         const uint64_t *cp = ds.csc_ptr.as<const uint64_t>();
         const uint32_t *cr = ds.csc_row.as<const uint32_t>();
         const float *cv = ds.csc_val.as<const float>();
         switch (q) {
-        case 1: hipLaunchKernelGGL(k_soft_csc<1>, g, blk, 0, st, cp, cr, cv, d, R, (uint32_t)k, c0, den, Cnew); break;
-        case 2: hipLaunchKernelGGL(k_soft_csc<2>, g, blk, 0, st, cp, cr, cv, d, R, (uint32_t)k, c0, den, Cnew); break;
-        case 3: hipLaunchKernelGGL(k_soft_csc<3>, g, blk, 0, st, cp, cr, cv, d, R, (uint32_t)k, c0, den, Cnew); break;
-        default: hipLaunchKernelGGL(k_soft_csc<4>, g, blk, 0, st, cp, cr, cv, d, R, (uint32_t)k, c0, den, Cnew); break;
+        case 1: hipLaunchKernelGGL(k_soft_csc<1>, g, blk, 0, st, cp, cr, cv, d, R, (uint32_t)k, c0, part); break;
+        case 2: hipLaunchKernelGGL(k_soft_csc<2>, g, blk, 0, st, cp, cr, cv, d, R, (uint32_t)k, c0, part); break;
+        case 3: hipLaunchKernelGGL(k_soft_csc<3>, g, blk, 0, st, cp, cr, cv, d, R, (uint32_t)k, c0, part); break;
+        default: hipLaunchKernelGGL(k_soft_csc<4>, g, blk, 0, st, cp, cr, cv, d, R, (uint32_t)k, c0, part); break;
         }
     }
+    hipLaunchKernelGGL(k_soft_csc_final, dim3((uint32_t)cdiv(kd, 256)), dim3(256), 0, st, (const double *)part, Z,
+                       (uint32_t)k, d, (const double *)den, Cnew);
 }
 
 // ---- soft L1 / L2 centers (set_centroids_full_mean, centroid.h:815-838) ----------

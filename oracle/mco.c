@@ -1307,6 +1307,14 @@ uint64_t mco_kmeanspp_ls(int msr, const float *X, const float *val, const uint32
     return k;
 }
 
+/* local search++ rounds from given centers (k >= 2); the draws use outputs j0, j0+1, ...
+ * of the stream of seed */
+void mco_localsearch(int msr, const float *X, const float *val, const uint32_t *idx, const uint64_t *indptr,
+                     uint64_t n, uint64_t d, uint64_t k, double prior, uint64_t seed, const double *w,
+                     uint64_t rounds, uint64_t j0, uint32_t *centers, uint32_t *asn, float *distf) {
+    ls_rounds(msr, X, val, idx, indptr, n, d, k, prior, seed, w, rounds, j0, centers, asn, distf);
+}
+
 /* ======================================================================
  * Coreset sensitivities, coreset.h:423-578.  Sums over points are exact:
  * each term is mapped to fixed point with one shift for the whole vector

@@ -132,6 +132,11 @@ uint64_t mco_kmeanspp_ls(int msr, const float *X, const float *val, const uint32
                          const uint64_t *indptr, uint64_t n, uint64_t d, uint64_t k,
                          double prior, uint64_t seed, const double *w, uint64_t rounds,
                          uint32_t *centers, uint32_t *asn, float *dist, int nthreads);
+/* local search++ rounds (lsearchpp.h:21-121) from the given centers (k >= 2, distinct
+ * rows); draws from outputs j0, j0+1, ... of the stream of seed */
+void mco_localsearch(int msr, const float *X, const float *val, const uint32_t *idx, const uint64_t *indptr,
+                     uint64_t n, uint64_t d, uint64_t k, double prior, uint64_t seed, const double *w,
+                     uint64_t rounds, uint64_t j0, uint32_t *centers, uint32_t *asn, float *distf);
 /* k-means++ with n_local_samples > 1: kmeans.h:79-92, 114-177 */
 uint64_t mco_kmeanspp_local(int msr, const float *X, const float *val, const uint32_t *idx,
                             const uint64_t *indptr, uint64_t n, uint64_t d, uint64_t k, double prior,

@@ -70,6 +70,7 @@ def _setup(L):
         "mco_softmax_rows": (None, [P, U64, U64, D, P]),
         "mco_kmeanspp": (U64, [I, P, P, P, P, U64, U64, U64, D, U64, P, P, P, P, I]),
         "mco_kmeanspp_ls": (U64, [I, P, P, P, P, U64, U64, U64, D, U64, P, U64, P, P, P, I]),
+        "mco_localsearch": (None, [I, P, P, P, P, U64, U64, U64, D, U64, P, U64, U64, P, P, P]),
         "mco_median_centers": (I, [I, P, P, P, P, U64, U64, P, U64, P, P, P, D]),
         "mco_lloyd_median": (I, [I, P, P, P, P, U64, U64, P, U64, D, P, D, U64, P, P, P, P, P]),
         "mco_kmeanspp_local": (U64, [I, P, P, P, P, U64, U64, U64, D, U64, P, U64, U64, P, P, P]),
@@ -349,6 +350,19 @@ def kmeanspp_ls(msr, X, k, rounds, prior=0., seed=0, w=None, nthreads=0):
     kk = lib().mco_kmeanspp_ls(msr, _p(Xd), _p(val), _p(idx), _p(ptr), n, d, k, prior, seed, _p(w), rounds,
                                _p(centers), _p(asn), _p(dist), nthreads)
     return centers[:kk], asn, dist
+
+
+def localsearch(msr, X, centers, rounds, prior=0., seed=0, w=None, j0=0):
+    """local search++ rounds from the given center rows (draws j0, j0+1, ... of the
+    stream of seed); returns (centers, asn, dist)."""
+    Xd, val, idx, ptr, n, d = _mat(X)
+    w = _f64(w)
+    centers = np.array(centers, dtype=np.uint32)
+    asn = np.zeros(n, dtype=np.uint32)
+    dist = np.zeros(n, dtype=np.float32)
+    lib().mco_localsearch(msr, _p(Xd), _p(val), _p(idx), _p(ptr), n, d, len(centers), prior, seed, _p(w), rounds,
+                          j0, _p(centers), _p(asn), _p(dist))
+    return centers, asn, dist
 
 
 def kmeanspp_local(msr, X, k, trials, prior=0., seed=0, w=None, lspp=0):

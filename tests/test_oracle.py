@@ -752,6 +752,111 @@ def test_coreset_probs_invariants(mco, sens):
     np.testing.assert_allclose(p2, 1. / 900, rtol=1e-6)
 
 
+def test_coreset_probs_lbk_hand_cases(mco):
+    """LBK (coreset.h:487-533) on 3 points in 2 clusters, worked out by hand from the
+    reference's steps with a = 16 ln 2 + 32:
+    unweighted, costs (0, 2, 4), clusters (0, 0, 1): mean cost 2, cluster terms
+    a + 6 and 4a + 12, so sens = (a + 6, 2a + 6, 6a + 12) over 9a + 24;
+    weighted w = (1, 3, 2), costs (1, 0, 2): W = 6, mean cost 5/6, cluster terms
+    0.6a + 6 and 4.8a + 12, sens = (1.8a + 6, 0.6a + 6, 7.2a + 12) (the point term
+    uses the raw cost).  A wrong a, or a dropped 4W/W_c or 2a S_c/(W_c mean) term,
+    changes these ratios."""
+    a = 16. * np.log(2.) + 32.
+    asn = np.array([0, 0, 1], np.uint32)
+    p = mco.coreset_probs(4, 2, [0., 2., 4.], asn)
+    s = np.array([a + 6, 2 * a + 6, 6 * a + 12])
+    np.testing.assert_allclose(p, s / s.sum(), rtol=2e-7)
+    p = mco.coreset_probs(4, 2, [1., 0., 2.], asn, w=[1., 3., 2.])
+    s = np.array([1.8 * a + 6, .6 * a + 6, 7.2 * a + 12])
+    np.testing.assert_allclose(p, s / s.sum(), rtol=2e-7)
+    # alpha comes from the number of centers: the same points labelled with k = 3
+    p3 = mco.coreset_probs(4, 3, [0., 2., 4.], asn)
+    a3 = 16. * np.log(3.) + 32.
+    s3 = np.array([a3 + 6, 2 * a3 + 6, 6 * a3 + 12 + 0.])
+    # cluster 2 is empty and assigned to nobody, so only a changes
+    np.testing.assert_allclose(p3, s3 / s3.sum(), rtol=2e-7)
+
+
+def test_coreset_probs_vx_bfl_hand_cases(mco):
+    """VX (coreset.h:423-455): s_i = w_i c_i / sum(w c) + 1/count_a.  BFL
+    (coreset.h:534-578): p_i = (w_i c_i / sum(w c) + w_i / (W_a count_a)) / 2,
+    both normalised.  Worked by hand:
+    unweighted costs (1, 1, 2), clusters (0, 0, 1): VX s = (.75, .75, 1.5) -> (.25, .25, .5);
+    BFL p = (.25, .25, .75) -> (.2, .2, .6);
+    weighted w = (2, 1, 1), costs (1, 2, 1), clusters (0, 1, 1): VX s = (1.4, .9, .7) / 3
+    (counts, not weights, in the second term); BFL p = (.7, .325, .225) / 1.25."""
+    asn = np.array([0, 0, 1], np.uint32)
+    np.testing.assert_allclose(mco.coreset_probs(3, 2, [1., 1., 2.], asn), [.25, .25, .5], rtol=2e-7)
+    np.testing.assert_allclose(mco.coreset_probs(0, 2, [1., 1., 2.], asn), [.2, .2, .6], rtol=2e-7)
+    asn = np.array([0, 1, 1], np.uint32)
+    w = [2., 1., 1.]
+    np.testing.assert_allclose(mco.coreset_probs(3, 2, [1., 2., 1.], asn, w=w), np.array([1.4, .9, .7]) / 3.,
+                               rtol=2e-7)
+    np.testing.assert_allclose(mco.coreset_probs(0, 2, [1., 2., 1.], asn, w=w),
+                               np.array([.7, .325, .225]) / 1.25, rtol=2e-7)
+
+
+def test_coreset_probs_fl_hand_cases(mco):
+    """FL (coreset.h:456-486), not normalised: unweighted ceil(n c_i / sum c) + 1, so
+    costs (1, 1, 1, 5) (n c / sum = .5, .5, .5, 2.5) give (2, 2, 2, 4) (floor or round
+    would not); weighted w_i c_i / sum(w c): w = (2, 1, 1), costs (1, 2, 1) give
+    (.4, .4, .2)."""
+    p = mco.coreset_probs(1, 2, [1., 1., 1., 5.], np.array([0, 0, 1, 1], np.uint32))
+    assert p.tolist() == [2., 2., 2., 4.]
+    p = mco.coreset_probs(1, 2, [1., 2., 1.], np.array([0, 1, 1], np.uint32), w=[2., 1., 1.])
+    np.testing.assert_allclose(p, [.4, .4, .2], rtol=2e-7)
+
+
+# One round of local search++ (lsearchpp.h:60-110) on 1-d points, worked out by hand:
+# rows whose cost to the drawn row exceeds their distance charge (second best - best)
+# to their best center (nothing when the two are equal); the others credit
+# (distance - new cost); the center with the smallest charge (the first on ties) is
+# swapped for the drawn row when its charge is below the credit.
+LS_CASES = [
+    # X = 0, 1, 10, 11, 20 seeded at rows (0, 1): every possible draw replaces slot 0
+    # (charges (1, 1), credit 441 / 460 / 380 for draws 2 / 3 / 4).
+    ([0, 1, 10, 11, 20], [0, 1], {2: [2, 1], 3: [3, 1], 4: [4, 1]}),
+    # X = 0..5, 50, 51 seeded at rows (4, 0):
+    # draw 6 or 7: charges (48, 24), credit 4324 -> slot 1 replaced;
+    # draw 1: charges (824, 16), credit 4 -> kept (charging new cost - distance instead
+    #   would give slot 1 a charge of 1 and swap);
+    # draws 2, 3: charges (816, 16) / (816, 24), credit 4 -> kept;
+    # draw 5: charges (24, 24), credit 185 -> the first of the tied centers, slot 0.
+    ([0, 1, 2, 3, 4, 5, 50, 51], [4, 0], {6: [4, 6], 7: [4, 7], 1: [4, 0], 2: [4, 0], 3: [4, 0], 5: [5, 0]}),
+    # X = 0, 2, 4 seeded at rows (0, 1): the only draw is row 2, charges (4, 4) equal
+    # the credit 4, and a zero gain is no swap.
+    ([0, 2, 4], [0, 1], {2: [0, 1]}),
+    # X = 0, 1, -1, -2 seeded at rows (0, 1): draw 3 leaves row 2's cost unchanged
+    # (1 = 1), so it credits 0 and is not charged: charges (1, 1), credit 4 -> slot 0
+    # (charging it, 3 to slot 0, would swap slot 1 instead); draw 2: charges (1, 1),
+    # credit 4 -> slot 0.
+    ([0, 1, -1, -2], [0, 1], {3: [3, 1], 2: [2, 1]}),
+]
+
+
+@pytest.mark.parametrize("case", range(len(LS_CASES)))
+def test_localsearch_one_round_hand_cases(mco, case):
+    xs, start, table = LS_CASES[case]
+    X = np.array(xs, np.float32)[:, None]
+    n = len(xs)
+    d0 = np.min((X - X[start].T) ** 2, axis=1).astype(np.float64)
+    fw = mco.fixed_weights(d0)
+    seen = set()
+    seed = 0
+    while seen != set(table) and seed < 200000:
+        seed += 1
+        sel = int(mco.fixed_draw(fw, mco.wyrand_at(seed, 0)))
+        if sel in seen:
+            continue
+        assert sel in table, sel
+        seen.add(sel)
+        c, a, d = mco.localsearch(mco.SQRL2, X, start, 1, seed=seed, j0=0)
+        assert c.tolist() == table[sel], (sel, c.tolist())
+        M = (X - X[c].T) ** 2
+        assert np.array_equal(a, np.argmin(M, axis=1)) and np.array_equal(d, M.min(axis=1))
+    assert seen == set(table)
+
+
 def test_coreset_sample_unbiased(mco):
     rng = np.random.default_rng(21)
     n = 500

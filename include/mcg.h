@@ -284,6 +284,17 @@ int mcg_kmeanspp_local(mcg_data *ds, int msr, double prior, uint64_t k, uint64_t
 int mcg_coreset_probs(mcg_ctx *ctx, int sens, uint64_t n, uint64_t ncenters,
                       const double *costs, const uint32_t *asn, const double *weights,
                       float *probs);
+/* mcg_coreset_probs without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  costs, asn, weights
+ * (nullable) and probs must be device buffers; results are bit-identical to
+ * mcg_coreset_probs.  The kernels' scratch is one fixed device array per GPU
+ * (33.5 MiB, held from the first mcg_ctx_create on that GPU until the process
+ * ends), so n <= 4194304 and ncenters <= 65536 (MCG_EINVAL beyond), and calls
+ * that overlap on one GPU (from any context or thread) race on it. */
+int mcg_coreset_probs_noalloc(mcg_ctx *ctx, int sens, uint64_t n, uint64_t ncenters,
+                              const double *costs, const uint32_t *asn, const double *weights,
+                              float *probs);
 /* m independent draws from probs: CoresetSampler::sample, coreset.h:625-651.
  * Draw t uses the (first_draw + t)-th output of the seed's random stream.
  * indices u64[m], out_weights f32[m] = w_i / (m * p_i). */

@@ -26,6 +26,7 @@ EXPORTS = [
     "mcg_server_destroy", "mcg_submit", "mcg_poll", "mcg_wait", "mcg_cancel", "mcg_release", "mcg_tuning_enable",
     "mcg_graphs_enable", "mcg_minibatch", "mcg_cs_minibatch", "mcg_kmeanspp_ls", "mcg_kmeanspp_local", "mcg_sampler_write",
     "mcg_sampler_read", "mcg_index_coreset_write", "mcg_index_coreset_read", "mcg_pairwise",
+    "mcg_coreset_probs_noalloc",
 ]
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = range(5)
@@ -106,6 +107,7 @@ def load():
         "mcg_soft_em": (I, [P, I, D, D, P, U64, P, D, U64, P, P, P, P, P]),
         "mcg_kmeanspp": (I, [P, I, D, U64, U64, P, P, P, P]),
         "mcg_coreset_probs": (I, [P, I, U64, U64, P, P, P, P]),
+        "mcg_coreset_probs_noalloc": (I, [P, I, U64, U64, P, P, P, P]),
         "mcg_coreset_sample": (I, [P, U64, P, P, U64, U64, U64, P, P]),
         "mcg_lloyd_step": (I, [P, I, D, P, U64, P, I, P, P]),
         "mcg_timing_enable": (I, [P, I]),

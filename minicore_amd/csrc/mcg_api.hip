@@ -570,6 +570,8 @@ int mcg_ctx_create(int device, mcg_ctx **out) {
         HIP_CHECK(hipGetDeviceCount(&n));
         REQUIRE(device >= 0 && device < n, "invalid device ordinal");
         HIP_CHECK(hipSetDevice(device));
+        // the fixed scratch of the allocation-free entry points, once per device
+        noalloc_setup(device);
         std::unique_ptr<mcg_ctx> c(new mcg_ctx);
         c->c.device = device;
         HIP_CHECK(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));

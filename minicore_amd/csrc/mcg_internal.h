@@ -257,6 +257,14 @@ void localsearch_run(Data &ds, int msr, double prior, uint64_t k, uint64_t seed,
                      uint64_t j0, uint32_t *centers_h, uint32_t *asn_d, float *dist_d, hipStream_t st);
 void coreset_probs_run(Ctx &ctx, int sens, uint64_t n, uint64_t ncenters, const double *costs_d,
                        const uint32_t *asn_d, const double *w_d, float *probs_d, hipStream_t st);
+// The same on a caller-provided device scratch of coreset_scratch_bytes(n, ncenters)
+// bytes (256-byte aligned); launches only, no allocation.
+uint64_t coreset_scratch_bytes(uint64_t n, uint64_t ncenters);
+void coreset_probs_core(int sens, uint64_t n, uint64_t ncenters, const double *costs_d, const uint32_t *asn_d,
+                        const double *w_d, float *probs_d, void *scratch, hipStream_t st);
+// Set-up of the fixed scratch of the allocation-free entry points for the current
+// device (called by mcg_ctx_create with the context's device current).
+void noalloc_setup(int device);
 void coreset_sample_run(Ctx &ctx, uint64_t n, const float *probs_d, const double *w_d, uint64_t m,
                         uint64_t seed, uint64_t *idx_d, float *wout_d, hipStream_t st);
 void coreset_sample_from(Ctx &ctx, uint64_t n, const float *probs_d, const double *w_d, uint64_t m,

@@ -965,18 +965,30 @@ __global__ void k_cs_norm(const double *__restrict__ p, uint64_t n, const CsSums
         probs[i] = (float)(p[i] / P);
 }
 
+uint64_t coreset_scratch_bytes(uint64_t n, uint64_t ncenters) {
+    return cdiv(sizeof(CsSums) + 3 * ncenters * 8, 256) * 256 + n * 8;
+}
+
 void coreset_probs_run(Ctx &c, int sens, uint64_t n, uint64_t ncenters, const double *cost, const uint32_t *asn,
                        const double *w, float *probs, hipStream_t st) {
+    void *scratch = c.s[S_MISC0].reserve(coreset_scratch_bytes(n, ncenters));
+    coreset_probs_core(sens, n, ncenters, cost, asn, w, probs, scratch, st);
+}
+
+void coreset_probs_core(int sens, uint64_t n, uint64_t ncenters, const double *cost, const uint32_t *asn,
+                        const double *w, float *probs, void *scratch, hipStream_t st) {
     REQUIRE(sens == MCG_SENS_LBK || sens == MCG_SENS_VX || sens == MCG_SENS_BFL || sens == MCG_SENS_FL ||
                 sens == MCG_SENS_LFKF,
             "sensitivity method must be BFL (0), FL (1), LFKF (2), VX (3) or LBK (4)");
-    const uint64_t bytes = sizeof(CsSums) + 3 * ncenters * 8;
-    char *buf = (char *)c.s[S_MISC0].reserve(bytes);
-    HIP_CHECK(hipMemsetAsync(buf, 0, bytes, st));
+    // scratch: the sums (cleared by a fill kernel, so that the work can be captured into
+    // a graph), then the unnormalised sensitivities at the next 256-byte boundary
+    const uint64_t head = cdiv(sizeof(CsSums) + 3 * ncenters * 8, 256) * 256;
+    char *buf = (char *)scratch;
+    fill_u32((uint32_t *)buf, 0u, (sizeof(CsSums) + 3 * ncenters * 8) / 4, st);
     CsSums *S = (CsSums *)buf;
     unsigned long long *ws = (unsigned long long *)(buf + sizeof(CsSums));
     unsigned long long *cs = ws + ncenters, *cnt = cs + ncenters;
-    double *p = (double *)c.s[S_STAGE0].reserve(n * 8);
+    double *p = (double *)(buf + head);
     const uint32_t g = (uint32_t)std::min<uint64_t>(cdiv(n, 256), 2048);
     hipLaunchKernelGGL(k_cs_max, dim3(g), dim3(256), 0, st, cost, w, n, S);
     hipLaunchKernelGGL(k_cs_sums, dim3(g), dim3(256), 0, st, cost, asn, w, n, S, ws, cs, cnt);

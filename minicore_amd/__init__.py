@@ -21,8 +21,7 @@ except ImportError as e:  # pragma: no cover - exercised when the extension is n
     _IMPORT_ERROR = e
 
 __all__ = ["hcluster", "scluster", "kmeanspp", "cluster", "cmp", "pcmp", "CSparseMatrix", "CoresetSampler",
-           "cluster_from_centers", "set_sqrl2_precision", "set_num_threads", "get_num_threads", "get_counthist",
-           "hiercluster_kmeanspp"]
+           "cluster_from_centers", "set_sqrl2_precision", "set_num_threads", "get_num_threads"]
 
 _num_threads = [os.cpu_count() or 1]
 
@@ -134,6 +133,5 @@ def cluster(data, *, msr, k, prior=0., seed=0, nmkc=0, ntimes=1, lspp=0, use_exp
                     with_rep=with_rep, cs=cs)
 
 
-from .recursive_cluster import get_counthist, hiercluster_kmeanspp  # noqa: E402
 from . import constants  # noqa: E402
 from .constants import CSR as csr_tuple, KMCRSV  # noqa: E402

@@ -339,6 +339,21 @@ __global__ void k_force(uint32_t *__restrict__ asn, float *__restrict__ cost, co
     if (asn[r] != ids[q]) { asn[r] = ids[q]; if (cost) cost[r] = 0.f; }
 }
 
+// out[0] = *cost, out[1] = number of clusters without members
+__global__ void k_step_summary(const double *__restrict__ cost, const uint32_t *__restrict__ counts, uint64_t k,
+                               double *__restrict__ out) {
+    __shared__ uint32_t sm[256];
+    uint32_t z = 0;
+    for (uint64_t q = threadIdx.x; q < k; q += blockDim.x) z += counts[q] == 0;
+    sm[threadIdx.x] = z;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { out[0] = *cost; out[1] = (double)sm[0]; }
+}
+
 __global__ void k_copy_f32(const float *__restrict__ src, float *__restrict__ dst, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
@@ -1578,6 +1593,12 @@ int mcg_timing_read(mcg_ctx *ctx, int which, double *total_ms, uint64_t *count) 
 }
 
 // lloyd_iteration (kmeans.h:359-479): one E-step and one M-step on device state.
+// An empty cluster takes a row drawn with probability ~ w_j * cost_j (the cost of
+// row j to its new mean for dense SQRL2, kmeans.h:433-440; the E-step minimum for
+// the other measures), that row moves to it, and the means are formed again, until
+// no cluster is empty (kmeans.h:431-454).  Draw e of one call is output e of the
+// wyrand stream of seed 0 (lloyd_iteration's default seed) through the fixed-point
+// sampler, the drawn row's weight set to zero for the later draws.
 int mcg_lloyd_step(mcg_data *ds, int msr, double prior, float *centers, uint64_t k, const double *weights,
                    int precision, uint32_t *asn, double *cost) {
     return guard([&] {
@@ -1589,24 +1610,65 @@ int mcg_lloyd_step(mcg_data *ds, int msr, double prior, float *centers, uint64_t
         Ctx &c = *D.ctx;
         hipStream_t st = c.stream;
         HIP_CHECK(hipSetDevice(c.device));
+        const uint64_t n = D.n;
         DBuf sw;
-        const double *W = (const double *)in_dev(c, weights, D.n * 8, sw);
-        double h = 0.;
-        if (dense_l2(D, msr)) {
+        const double *W = (const double *)in_dev(c, weights, n * 8, sw);
+        double *res = (double *)c.s[S_MISC3].reserve(2 * sizeof(double));
+        float *pc = nullptr;
+        MStepOut mo;
+        const bool l2 = dense_l2(D, msr);
+        if (l2) {
             dense_sqrl2_assign(D, centers, k, precision, asn, st);
-            MStepOut mo = dense_mstep(D, asn, W, k, centers, centers, true, st);
-            HIP_CHECK(hipMemcpyAsync(&h, mo.total, sizeof(double), hipMemcpyDeviceToHost, st));
+            mo = dense_mstep(D, asn, W, k, centers, centers, true, st);
         } else {
-            float *pc = (float *)c.s[S_COSTS].reserve(D.n * 4);
+            pc = (float *)c.s[S_COSTS].reserve(n * 4);
             double *tot = (double *)c.s[S_TOT].reserve(4 * sizeof(double));
             generic_costs(D, msr, prior, centers, k, nullptr, asn, pc, st);
-            hipLaunchKernelGGL(k_wsum, dim3(1), dim3(1024), 0, st, pc, W, D.n, tot);
-            if (D.csr) csr_mstep(D, asn, W, k, centers, st);
-            else dense_mstep(D, asn, W, k, centers, centers, false, st);
-            HIP_CHECK(hipMemcpyAsync(&h, tot, sizeof(double), hipMemcpyDeviceToHost, st));
+            hipLaunchKernelGGL(k_wsum, dim3(1), dim3(1024), 0, st, pc, W, n, tot);
+            mo = D.csr ? csr_mstep(D, asn, W, k, centers, st) : dense_mstep(D, asn, W, k, centers, centers, false, st);
+            mo.total = tot;
         }
+        hipLaunchKernelGGL(k_step_summary, dim3(1), dim3(256), 0, st, mo.total, mo.counts, k, res);
+        double h[2] = {0., 0.};
+        HIP_CHECK(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, st));
         sync(c);
-        if (cost) *cost = h;
+        if (cost) *cost = h[0];
+        if (h[1] == 0.) return;
+        // reseed empty clusters (rare): host draws, device moves and means
+        std::vector<double> wh;
+        if (weights) {
+            wh.resize(n);
+            copy_sync(wh.data(), W, n * 8, hipMemcpyDeviceToHost, st);
+        }
+        if (l2) pc = (float *)c.s[S_COSTS].reserve(n * 4);
+        std::vector<uint32_t> hc(k);
+        std::vector<float> hp(n);
+        uint64_t j = 0;
+        for (uint64_t round = 0; round <= k; ++round) {
+            copy_sync(hc.data(), mo.counts, k * 4, hipMemcpyDeviceToHost, st);
+            std::vector<uint32_t> ids;
+            for (uint64_t q = 0; q < k; ++q) if (!hc[q]) ids.push_back((uint32_t)q);
+            if (ids.empty()) return;
+            REQUIRE(round < k, "internal: empty clusters after k reseeds");
+            if (l2) dense_sqrl2_costs(D, centers, asn, pc, st);
+            copy_sync(hp.data(), pc, n * 4, hipMemcpyDeviceToHost, st);
+            std::vector<double> wv(n);
+            for (uint64_t i = 0; i < n; ++i) wv[i] = wh.empty() ? (double)hp[i] : wh[i] * (double)hp[i];
+            std::vector<uint64_t> rows(ids.size());
+            for (size_t e = 0; e < ids.size(); ++e) {
+                const uint64_t r = wyrand_at(0, j++);
+                draw_fixed_many(wv, &r, 1, &rows[e]);
+                wv[rows[e]] = 0.;
+            }
+            copy_rows_to_centers(D, rows.data(), ids.data(), ids.size(), centers, st);
+            char *buf = (char *)c.s[S_MISC2].reserve(ids.size() * 12);
+            HIP_CHECK(hipMemcpyAsync(buf, rows.data(), ids.size() * 8, hipMemcpyHostToDevice, st));
+            HIP_CHECK(hipMemcpyAsync(buf + ids.size() * 8, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_force, dim3((uint32_t)cdiv(ids.size(), 256)), dim3(256), 0, st, asn, pc,
+                               (const uint64_t *)buf, (const uint32_t *)(buf + ids.size() * 8), (uint64_t)ids.size());
+            mo = D.csr ? csr_mstep(D, asn, W, k, centers, st) : dense_mstep(D, asn, W, k, centers, centers, false, st);
+            sync(c);
+        }
     });
 }
 

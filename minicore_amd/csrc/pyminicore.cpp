@@ -39,7 +39,6 @@ void check(int rc) {
     if (rc == MCG_OK) return;
     std::string msg = mcg_last_error();
     if (rc == MCG_EINVAL) throw std::invalid_argument(msg);
-    if (rc == MCG_EUNSUPPORTED) throw py::value_error("unsupported on the GPU path: " + msg);
     throw std::runtime_error("libmcg error " + std::to_string(rc) + ": " + msg);
 }
 

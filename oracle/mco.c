@@ -542,6 +542,52 @@ uint64_t mco_centroids_dense(const float *X, uint64_t n, uint64_t d, const uint3
     return empty;
 }
 
+/* One pass of lloyd_iteration (kmeans.h:359-479) on dense rows, squared L2, in the
+ * order E-step then M-step: assign every row to its nearest center (FP32 FMA chain,
+ * lowest index on ties), *cost = sum w_i ||x_i - c_asn(i)||^2 against the input
+ * centers (FP64), then the weighted means (chunk 256 association).  An empty
+ * cluster (ascending id) takes a row drawn with probability ~ w_j * cost_j, cost_j
+ * the FP32 cost of row j to its new mean (kmeans.h:433-440), by the fixed-point
+ * sampler with output e of the wyrand stream of 0 for the e-th draw of the call;
+ * the drawn row's weight is zero for the later draws, it moves to the cluster and
+ * becomes its center, and the means are formed again (from scratch) until no
+ * cluster is empty (kmeans.h:431-454).  Returns the number of reseeds. */
+uint64_t mco_lloyd_step(const float *X, uint64_t n, uint64_t d, float *C, uint64_t k, const double *w,
+                        uint32_t *asn, double *cost) {
+    float *pc = (float *)malloc(n * sizeof(float));
+    mco_assign_sqrl2_f32(X, n, d, C, k, asn, pc, NULL, 0);
+    double tot = 0.;
+    for (uint64_t i = 0; i < n; ++i) {
+        const float *x = X + i * d, *c = C + (uint64_t)asn[i] * d;
+        double s = 0.;
+        for (uint64_t j = 0; j < d; ++j) { double t = (double)x[j] - (double)c[j]; s += t * t; }
+        tot += (w ? w[i] : 1.) * s;
+    }
+    *cost = tot;
+    uint64_t draws = 0;
+    double *wv = (double *)malloc(n * sizeof(double));
+    uint64_t *fw = (uint64_t *)malloc(n * sizeof(uint64_t));
+    uint64_t *cnt = (uint64_t *)malloc(k * sizeof(uint64_t));
+    for (uint64_t round = 0; round <= k; ++round) {
+        uint64_t empty = mco_centroids_dense(X, n, d, asn, w, k, 256, C);
+        if (!empty) break;
+        memset(cnt, 0, k * sizeof(uint64_t));
+        for (uint64_t i = 0; i < n; ++i) cnt[asn[i]]++;
+        for (uint64_t i = 0; i < n; ++i)
+            wv[i] = (w ? w[i] : 1.) * (double)mco_sqrl2_f32(X + i * d, C + (uint64_t)asn[i] * d, d);
+        for (uint64_t q = 0; q < k; ++q) {
+            if (cnt[q]) continue;
+            mco_fixed_weights(wv, n, fw);
+            const uint64_t r = mco_fixed_draw(fw, n, mco_wyrand_at(0, draws++));
+            wv[r] = 0.;
+            asn[r] = (uint32_t)q;
+            memcpy(C + q * d, X + r * d, d * sizeof(float));
+        }
+    }
+    free(pc); free(wv); free(fw); free(cnt);
+    return draws;
+}
+
 uint64_t mco_centroids_csr(const float *val, const uint32_t *idx, const uint64_t *indptr,
                            uint64_t n, uint64_t d, const uint32_t *asn, const double *w,
                            uint64_t k, float *C) {

@@ -80,6 +80,10 @@ void mco_assign_f64(int msr, const float *X, const float *val, const uint32_t *i
  * the number of empty clusters. */
 uint64_t mco_centroids_dense(const float *X, uint64_t n, uint64_t d, const uint32_t *asn,
                              const double *w, uint64_t k, uint64_t chunk, float *C_inout);
+/* one pass of lloyd_iteration (kmeans.h:359-479), dense squared L2, E then M with
+ * the empty-cluster reseed; returns the number of reseed draws */
+uint64_t mco_lloyd_step(const float *X, uint64_t n, uint64_t d, float *C, uint64_t k, const double *w,
+                        uint32_t *asn, double *cost);
 uint64_t mco_centroids_csr(const float *val, const uint32_t *idx, const uint64_t *indptr,
                            uint64_t n, uint64_t d, const uint32_t *asn, const double *w,
                            uint64_t k, float *C_inout);

@@ -70,6 +70,7 @@ def _setup(L):
         "mco_softmax_rows": (None, [P, U64, U64, D, P]),
         "mco_kmeanspp": (U64, [I, P, P, P, P, U64, U64, U64, D, U64, P, P, P, P, I]),
         "mco_kmeanspp_ls": (U64, [I, P, P, P, P, U64, U64, U64, D, U64, P, U64, P, P, P, I]),
+        "mco_lloyd_step": (U64, [P, U64, U64, P, U64, P, P, P]),
         "mco_localsearch": (None, [I, P, P, P, P, U64, U64, U64, D, U64, P, U64, U64, P, P, P]),
         "mco_median_centers": (I, [I, P, P, P, P, U64, U64, P, U64, P, P, P, D]),
         "mco_lloyd_median": (I, [I, P, P, P, P, U64, U64, P, U64, D, P, D, U64, P, P, P, P, P]),
@@ -251,6 +252,19 @@ def cs_minibatch(msr, X, C0, prior=0., w=None, mbsize=100, maxiter=20, freq=5, r
     if rc != 0:
         raise ValueError(f"oracle coreset minibatch failed ({rc})")
     return dict(centers=C, asn=asn, costs=cost, initcost=float(ic[0]), finalcost=float(fc[0]), numiter=int(it[0]))
+
+
+def lloyd_step(X, C, w=None):
+    """One lloyd_iteration pass (dense squared L2): returns (new centers, asn, cost of
+    the assignment against C, number of reseed draws)."""
+    X = _f32(X)
+    n, d = X.shape
+    C = np.array(C, dtype=np.float32, order="C")
+    w = _f64(w)
+    asn = np.empty(n, dtype=np.uint32)
+    cost = np.zeros(1)
+    draws = lib().mco_lloyd_step(_p(X), n, d, _p(C), C.shape[0], _p(w), _p(asn), _p(cost))
+    return C, asn, float(cost[0]), int(draws)
 
 
 def lloyd(msr, X, C0, prior=0., w=None, eps=1e-10, maxiter=100, nthreads=0):

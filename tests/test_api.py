@@ -49,23 +49,3 @@ def test_thread_count_functions():
         mc.set_num_threads(0)
     mc.set_num_threads(old)
 
-
-def test_get_counthist():
-    """get_counthist (pycounthist.cpp:72-155): value -> ascending indices, index dtype
-    by length (pyhelpers.h:220-225)."""
-    import numpy as np
-    import minicore_amd as mc
-    a = np.array([3, 1, 3, 2, 1, 3], dtype=np.uint32)
-    h = mc.get_counthist(a)
-    assert set(h) == {1, 2, 3}
-    assert h[3].tolist() == [0, 2, 5] and h[1].tolist() == [1, 4] and h[2].tolist() == [3]
-    assert h[3].dtype == np.uint8
-    b = np.random.default_rng(0).integers(0, 7, 70000)
-    hb = mc.get_counthist(b)
-    assert all(v.dtype == np.uint32 for v in hb.values())
-    for key, idx in hb.items():
-        assert np.array_equal(idx, np.nonzero(b == key)[0])
-    f = mc.get_counthist(np.array([.5, .25, .5], dtype=np.float32))
-    assert f[.5].tolist() == [0, 2] and isinstance(next(iter(f)), float)
-    rows = mc.get_counthist(np.array([[1, 2, 1], [0, 0, 5]]))
-    assert rows[0][1].tolist() == [0, 2] and rows[1][0].tolist() == [0, 1]

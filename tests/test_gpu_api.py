@@ -110,30 +110,6 @@ def test_coreset_sampler_class(mco):
     assert len(np.unique(iu)) == 300 and np.all(np.diff(iu.astype(np.int64)) > 0)
 
 
-def test_hiercluster_kmeanspp():
-    """hiercluster_kmeanspp (recursive_cluster.py:7-32) with its defaults (JSD seeding
-    with 2 local trials, 3 Lloyd iterations): every cluster with more than radix rows
-    has a child built from exactly its rows, the others none."""
-    say("api hiercluster_kmeanspp")
-    rng = np.random.default_rng(7)
-    X = (rng.random((1500, 12)) + .05).astype(np.float32)
-    out = mc.hiercluster_kmeanspp(X, radix=6)
-
-    def walk(node, data, depth):
-        assert node["hierdepth"] == depth and node["centers"].shape == (6, 12)
-        asn = np.asarray(node["asn"])
-        assert len(asn) == len(data) and "costs:" in node
-        for key, child in node["children"].items():
-            idx = np.nonzero(asn == key)[0]
-            assert len(idx) > 6
-            np.testing.assert_array_equal(child["dataset"], data[idx])
-            walk(child, child["dataset"], depth + 1)
-        for key in set(asn.tolist()) - set(node["children"]):
-            assert (asn == key).sum() <= 6
-    walk(out, X, 0)
-    assert out["children"]
-
-
 def test_reference_coreset_example(mco):
     """The reference's python/examples/coreset_example.py, unchanged but for the
     module name: seeding, an LBK sampler from mc.constants.SENSDICT, 100 draws."""

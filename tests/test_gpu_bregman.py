@@ -4,6 +4,7 @@ import os
 
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 from minicore_amd.datagen import poisson_csr
 
@@ -128,6 +129,48 @@ def test_csr_centroids(mcg, ctx, mco):
     got, _ = mcg.Dataset.csr(data2.astype(np.float32), idx2, (np.arange(11) * 5).astype(np.uint64), (10, 10),
                              ctx).centroids(np.zeros(10, np.uint32), 1, np.zeros((1, 10)))
     assert list(got[0, ::2].astype(int)) == [47, 49, 51, 53, 55]
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_csr_centroids_skewed(mcg, ctx, mco, weighted):
+    """One cluster of 6,000 of 8,000 rows: it spans many 256-row groups of its member
+    chunks (the chunk loop, the four per-wave accumulators and the rows loaded ahead),
+    the others are small; centers within 1e-6 of the oracle's FP64 means."""
+    say(f"csr centroids skewed weighted={weighted}")
+    n, d, k = 8000, 3000, 7
+    data, idx, ptr = csr_data(n, d, 0.02, 19, False)
+    rng = np.random.default_rng(3)
+    asn = np.where(rng.random(n) < 0.75, 2, rng.integers(0, k, n)).astype(np.uint32)
+    w = rng.uniform(0.5, 2, n) if weighted else None
+    prev = np.zeros((k, d), np.float32)
+    ref, _ = mco.centroids(mco.CSR(data, idx, ptr, (n, d)), asn, k, prev, w)
+    got, _ = mcg.Dataset.csr(data, idx, ptr, (n, d), ctx).centroids(asn, k, prev, w)
+    assert (asn == 2).sum() > 5000
+    np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-9)
+
+
+@pytest.mark.parametrize("k", [130, 300])
+def test_soft_em_csr_many_centers(mcg, ctx, mco, k):
+    """Soft CSR EM with k = 130 (two-and-a-bit center quads per lane) and k = 300
+    (two 256-center groups), with a few columns stored in every row so that each of
+    their entry chunks holds entries: costs, responsibilities and centers against the
+    oracle."""
+    say(f"soft csr k={k}")
+    n, d = 1200, 400
+    data, idx, ptr = csr_data(n, d, 0.03, 23, False)
+    A = sp.csr_matrix((data, idx.astype(np.int64), ptr.astype(np.int64)), shape=(n, d)).tolil()
+    A[:, [5, 77, 301]] = np.random.default_rng(2).integers(1, 9, (n, 3)).astype(np.float32)
+    A = A.tocsr()
+    A.sort_indices()
+    data, idx, ptr = A.data.astype(np.float32), A.indices.astype(np.uint32), A.indptr.astype(np.uint64)
+    X = mco.CSR(data, idx, ptr, (n, d))
+    C0 = X.todense()[np.random.default_rng(k).choice(n, k, replace=False)] + 0.25
+    ref = mco.soft(11, X, C0, prior=1e-3, temp=1., eps=-1., maxiter=3)
+    got = mcg.Dataset.csr(data, idx, ptr, (n, d), ctx).soft(11, C0, prior=1e-3, temp=1., eps=-1., maxiter=3)
+    assert got["numiter"] == ref["numiter"] == 3
+    assert got["finalcost"] == pytest.approx(ref["finalcost"], rel=2e-4)
+    np.testing.assert_allclose(got["asn"], ref["asn"], atol=2e-4)
+    np.testing.assert_allclose(got["centers"], ref["centers"], rtol=2e-4, atol=1e-5)
 
 
 @pytest.mark.parametrize("sparse", [False, True])

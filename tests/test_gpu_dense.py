@@ -198,6 +198,32 @@ def test_lloyd_empty_cluster_restart(mcg, ctx, mco):
     assert got["finalcost"] == pytest.approx(ref["finalcost"], rel=1e-6)
 
 
+@pytest.mark.parametrize("weighted", [False, True])
+def test_lloyd_step_reseed(mcg, ctx, mco, weighted):
+    """mcg_lloyd_step (lloyd_iteration, kmeans.h:359-479) against the oracle's pass:
+    two centers far from the data lose all rows in the first step and are reseeded
+    at drawn rows (kmeans.h:431-454); assignment and centers bit for bit, cost within
+    1e-12, over three steps."""
+    import torch
+    say(f"lloyd_step reseed weighted={weighted}")
+    X, C = mixture(6000, 40, 24, 31)
+    C[-2:] = 1e3
+    w = np.random.default_rng(4).uniform(0.5, 2., 6000) if weighted else None
+    ds = mcg.Dataset.dense(X, ctx)
+    Ct = torch.from_numpy(C).cuda()
+    at = torch.empty(6000, dtype=torch.int32, device="cuda")
+    Cr = C.copy()
+    for step in range(3):
+        cost = ds.lloyd_step(mcg.SQRL2, Ct, at, weights=w)
+        Cr, ar, cr, draws = mco.lloyd_step(X, Cr, w)
+        if step == 0:
+            assert draws == 2
+        assert np.array_equal(at.cpu().numpy().astype(np.uint32), ar), step
+        assert np.array_equal(Ct.cpu().numpy(), Cr), step
+        assert cost == pytest.approx(cr, rel=1e-12), step
+    ds.close()
+
+
 def test_cost_matrix_sqrl2(mcg, ctx, mco):
     say("cost matrix sqrl2")
     X, C = mixture(1000, 40, 30, 7)

@@ -857,6 +857,42 @@ def test_localsearch_one_round_hand_cases(mco, case):
     assert seen == set(table)
 
 
+def test_lloyd_step_means_and_reseed(mco):
+    """One lloyd_iteration pass (kmeans.h:359-479): without empty clusters the new
+    centers are the weighted means of the brute-force assignment and the cost is the
+    FP64 sum against the input centers.  With an empty cluster, worked by hand on
+    x = 0, 1, 10, 11 and centers .5, 10.5, 100 (cluster 2 empty): every row costs .25
+    to its new mean, the row r that the sampler draws (uniform here) moves to
+    cluster 2 and becomes its center, and its partner is left alone in its cluster;
+    weighted w = 1, 3, 1, 1: the first mean is .75 and the draw weights are
+    (.5625, .1875, .25, .25)."""
+    rng = np.random.default_rng(8)
+    X = rng.standard_normal((900, 7)).astype(np.float32)
+    C0 = X[:12].copy()
+    for w in (None, rng.uniform(.5, 2., 900)):
+        C, a, cost, draws = mco.lloyd_step(X, C0, w)
+        a_ref = np.argmin(((X[:, None, :].astype(np.float64) - C0[None].astype(np.float64)) ** 2).sum(-1), axis=1)
+        assert draws == 0 and np.array_equal(a, a_ref)
+        ww = np.ones(900) if w is None else w
+        for c in range(12):
+            m = a_ref == c
+            np.testing.assert_allclose(C[c], (ww[m, None] * X[m]).sum(0) / ww[m].sum(), rtol=1e-6, atol=1e-7)
+        want = (ww * ((X.astype(np.float64) - C0[a_ref].astype(np.float64)) ** 2).sum(1)).sum()
+        assert cost == pytest.approx(want, rel=1e-12)
+    X = np.array([[0.], [1.], [10.], [11.]], np.float32)
+    C0 = np.array([[.5], [10.5], [100.]], np.float32)
+    for w, dw, c0, want_cost in ((None, [.25] * 4, .5, 1.), ([1., 3., 1., 1.], [.5625, .1875, .25, .25], .75, 1.5)):
+        r = int(mco.fixed_draw(mco.fixed_weights(dw), mco.wyrand_at(0, 0)))
+        C, a, cost, draws = mco.lloyd_step(X, C0, w)
+        a_want = np.array([0, 0, 1, 1])
+        a_want[r] = 2
+        C_want = np.array([c0, 10.5, 100.])
+        C_want[2] = X[r, 0]
+        C_want[a_want[r ^ 1]] = X[r ^ 1, 0]
+        assert draws == 1 and a.tolist() == a_want.tolist() and C[:, 0].tolist() == C_want.tolist()
+        assert cost == want_cost
+
+
 def test_coreset_sample_unbiased(mco):
     rng = np.random.default_rng(21)
     n = 500

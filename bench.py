@@ -196,6 +196,21 @@ def sparse_config(args, mcg, cfg):
             out[name] = {"seeding_s": t_seed, "estep_ms": 1e3 * t_e, "estep_GBs": alg / t_e / 1e9,
                          "estep_frac_hbm": alg / t_e / 1e9 / PEAK_HBM_GBS, "lloyd_iters": res["numiter"] + 1,
                          "lloyd_iters_per_s": (res["numiter"] + 1) / t_l}
+            if not args.no_cpu:
+                from oracle import mco
+                ns = 2000
+                Xs = mco.CSR(data[:int(ptr[ns])], idx[:int(ptr[ns])], ptr[:ns + 1].copy(), (ns, d))
+                info = cpu_info()
+
+                def it():
+                    a, _, _ = mco.assign_f64(code, Xs, C0, 1. / d)
+                    mco.centroids(Xs, a, k, C0)
+                t, _ = _median_time(it, 5)
+                out[name]["cpu_baseline"] = dict(
+                    value=1.0 / (t * n / ns), unit="iters/s", cores=info["threads"], kind="port",
+                    sample=f"{name} E+M iterations of the oracle (FP64 per-pair loop over the row's and the "
+                           f"center's coordinates, OpenMP over rows) on {ns} of the {n} rows: median {t:.2f} s of "
+                           f"5 after one warm-up, scaled by n/{ns}", **info)
             log(json.dumps({name: out[name]}))
         out.update({"n": n, "d": d, "k": k, "nnz": nnz, "estep_algorithmic_bytes": alg})
     else:
@@ -226,6 +241,20 @@ def sparse_config(args, mcg, cfg):
         t0 = time.perf_counter()
         res = ds.soft(mcg.REVERSE_MKL, C0, prior=1e-3, temp=1., eps=-1., maxiter=10, with_outputs=False)
         t_s = time.perf_counter() - t0
+        if not args.no_cpu:
+            from oracle import mco
+            ns = 100
+            Xs = mco.CSR(data[:int(ptr[ns])], idx[:int(ptr[ns])], ptr[:ns + 1].copy(), (ns, d))
+            info = cpu_info()
+            t1, _ = _median_time(lambda: mco.soft(11, Xs, C0, prior=1e-3, temp=1., eps=-1., maxiter=1), 3)
+            t2, _ = _median_time(lambda: mco.soft(11, Xs, C0, prior=1e-3, temp=1., eps=-1., maxiter=2), 3)
+            per = max(t2 - t1, 1e-9) * n / ns
+            out["cpu_baseline"] = dict(
+                value=per, unit="s/iter", cores=info["threads"], kind="port",
+                sample=f"soft REVERSE_MKL EM of the oracle (FP64 per-pair loop, OpenMP over rows) on {ns} of the "
+                       f"{n} rows: one iteration = median of 2 iterations ({t2:.2f} s) minus median of 1 "
+                       f"({t1:.2f} s), 3 runs each after a warm-up, scaled by n/{ns}", **info)
+            log(json.dumps(out["cpu_baseline"]))
         out.update({"n": n, "d": d, "k": k, "nnz": nnz, "cost_matrix_ms": 1e3 * t_e,
                     "cost_matrix_GBs": alg / t_e / 1e9, "estep_algorithmic_bytes": alg,
                     "soft_em_iters": res["numiter"], "soft_em_s_per_iter": t_s / max(res["numiter"], 1),
@@ -341,32 +370,67 @@ def median_config(args, mcg):
         from oracle import mco
         ns = min(args.cpu_sample // 8, n)
         Xs = X[:ns].cpu().numpy()
-        t0 = time.perf_counter()
-        a, c, _ = mco.assign_f64(0, Xs, C0, 0.)
-        mco.median_centers(0, Xs, a, c, C0)
-        t = time.perf_counter() - t0
-        cpu = dict(value=1.0 / (t * n / ns), unit="iters/s", cores=int(os.environ.get("OMP_NUM_THREADS", "0")) or
-                   (os.cpu_count() or 1), kind="port",
-                   sample=f"one L1 E+M iteration of the oracle (FP64 per-pair loop, sorted medians) on {ns} of "
-                          f"the {n} rows (k and d unchanged), {t:.2f} s, scaled by n/{ns}")
+        info = cpu_info()
+
+        def it():
+            a, c, _ = mco.assign_f64(0, Xs, C0, 0.)
+            mco.median_centers(0, Xs, a, c, C0)
+        t, _ = _median_time(it, 3)
+        cpu = dict(value=1.0 / (t * n / ns), unit="iters/s", cores=info["threads"], kind="port",
+                   sample=f"L1 E+M iterations of the oracle (FP64 per-pair loop, sorted medians) on {ns} of the {n} "
+                          f"rows (k and d unchanged): median {t:.2f} s of 3 after one warm-up, scaled by n/{ns}",
+                   **info)
         log(json.dumps(cpu))
     return {"config": "median", "n": n, "d": d, "k": k, "policies": res, "cpu_baseline": cpu,
             "note": "E-step FLOP count 2*n*k*d (difference and absolute-value add, or FMA, per pair and "
                     "dimension) against the FP32 vector peak; mstep_ms = ms_per_iteration - estep_ms"}
 
 
-def cpu_baseline(X_host, C_host, n_full, nthreads):
+def cpu_info():
+    """Host CPU model, logical CPUs of the machine, and the threads the CPU legs use:
+    OMP_NUM_THREADS when set (the CPU share a one-GPU job is given), else every CPU
+    this process may run on."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or avail
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "threads": threads}
+
+
+def _median_time(fn, reps):
+    import numpy as np
+    fn()    # warm-up (thread pool, page faults)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts
+
+
+def cpu_baseline(X_host, C_host, n_full, reps=5):
+    """One E+M iteration of the oracle's per-pair FP32 loop (OpenMP over rows) on a
+    row sample: median of `reps` timed iterations after one warm-up, scaled by n/sample."""
     from oracle import mco
+    info = cpu_info()
     ns = X_host.shape[0]
-    mco.assign_sqrl2_f32(X_host[:256], C_host, nthreads)  # warm-up (thread pool)
-    t0 = time.perf_counter()
-    a, _, _ = mco.assign_sqrl2_f32(X_host, C_host, nthreads)
-    mco.centroids(X_host, a, C_host.shape[0], C_host)
-    t = time.perf_counter() - t0
+
+    def it():
+        a, _, _ = mco.assign_sqrl2_f32(X_host, C_host, info["threads"])
+        mco.centroids(X_host, a, C_host.shape[0], C_host)
+    t, ts = _median_time(it, reps)
     per_iter_full = t * n_full / ns
-    return dict(value=1.0 / per_iter_full, unit="iters/s", cores=nthreads, kind="port",
-                sample=f"one E+M iteration of the oracle's per-pair FP32 loop on {ns} of the {n_full} rows "
-                       f"(k and d unchanged), {t:.2f} s, scaled by n/{ns}")
+    return dict(value=1.0 / per_iter_full, unit="iters/s", cores=info["threads"], kind="port",
+                sample=f"E+M iterations of the oracle's per-pair FP32 loop on {ns} of the {n_full} rows (k and d "
+                       f"unchanged): median {t:.2f} s of {reps} after one warm-up, scaled by n/{ns}",
+                **info)
 
 
 def main():
@@ -449,9 +513,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         ns = min(args.cpu_sample, n)
-        nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-        log(f"cpu baseline on {ns} rows, {nthreads} threads")
-        cpu = cpu_baseline(X[:ns].cpu().numpy(), C0.cpu().numpy(), n, nthreads)
+        log(f"cpu baseline on {ns} rows, {cpu_info()}")
+        cpu = cpu_baseline(X[:ns].cpu().numpy(), C0.cpu().numpy(), n)
         log(json.dumps(cpu))
 
     serve = None

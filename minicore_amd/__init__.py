@@ -21,6 +21,7 @@ except ImportError as e:  # pragma: no cover - exercised when the extension is n
     _IMPORT_ERROR = e
 
 __all__ = ["hcluster", "scluster", "kmeanspp", "cluster", "cmp", "pcmp", "CSparseMatrix", "CoresetSampler",
+           "SparseMatrixWrapper", "SumOpts",
            "cluster_from_centers", "set_sqrl2_precision", "set_num_threads", "get_num_threads"]
 
 _num_threads = [os.cpu_count() or 1]
@@ -67,6 +68,18 @@ def _as_matrix(matrix):
 
 def CSparseMatrix(sparray):
     return _ext().CSparseMatrix(sparray)
+
+
+def SparseMatrixWrapper(*args, **kwargs):
+    """SparseMatrixWrapper(scipy CSR matrix, skip_empty=False, use_float=True): an owned copy of
+    the rows (python/smw.h, pysmw.cpp:31-265), accepted by hcluster, scluster, cmp and kmeanspp."""
+    return _ext().SparseMatrixWrapper(*args, **kwargs)
+
+
+def SumOpts(*args, **kwargs):
+    """SumOpts(measure, k=10, prior=0., sm='BFL', outlier_fraction=0., max_rounds=100, soft=False):
+    the options object of kmeanspp(matrix, opts) (clustering/mtx2cs.h:13-80, pysmw.cpp:293-366)."""
+    return _ext().SumOpts(*args, **kwargs)
 
 
 def CoresetSampler(*args, **kwargs):

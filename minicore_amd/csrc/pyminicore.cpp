@@ -353,7 +353,8 @@ py::object hcluster_dense(py::array dataset, py::object centers, double prior, p
     return r;
 }
 
-py::object hcluster_csr(PyCSparseMatrix &smw, py::object centers, double prior, py::object msr, py::object weights,
+template <class M>
+py::object hcluster_csr(M &smw, py::object centers, double prior, py::object msr, py::object weights,
                         double eps, uint64_t maxiter, uint64_t seed, py::ssize_t mbsize, py::ssize_t ncheckins,
                         py::ssize_t reseed_count, bool with_rep, bool cs) {
     const int m = measure_code(msr);
@@ -410,7 +411,8 @@ py::object scluster_dense(py::array mat, py::object centers, py::object msr, dou
     return run_soft(D.dev->h, D.n, D.d, C, k, prior, m, temp, maxiter, savepref, weights_f64(weights, D.n));
 }
 
-py::object scluster_csr(PyCSparseMatrix &smw, py::object centers, py::object msr, double prior, double temp,
+template <class M>
+py::object scluster_csr(M &smw, py::object centers, py::object msr, double prior, double temp,
                         uint64_t maxiter, py::ssize_t mbsize, py::ssize_t, std::string savepref, py::object weights) {
     if (mbsize > 0) throw std::runtime_error("Not yet completed: minibatch soft clustering");   // solve.h:319-321
     const int m = measure_code(msr);
@@ -435,7 +437,11 @@ py::array transpose_copy(py::array_t<float> a) {
     return py::cast<py::array>(py::module_::import("numpy").attr("ascontiguousarray")(a.attr("T")));
 }
 
-py::object cmp_csr_array(PyCSparseMatrix &lhs, py::array arr, py::object msr, py::object betaprior, py::object reverse) {
+// rows2d_reverse: the 2-d array form honours `reverse` (the SparseMatrixWrapper binding,
+// pycmp.cpp:84-96); the CSparseMatrix binding ignores it there (pycmp.cpp:191-197)
+template <class M>
+py::object cmp_csr_array(M &lhs, py::array arr, py::object msr, py::object betaprior, py::object reverse,
+                         bool rows2d_reverse = false) {
     const int m = measure_code(msr);
     const double prior = betaprior.cast<double>();
     const bool rev = reverse.cast<bool>();
@@ -454,6 +460,10 @@ py::object cmp_csr_array(PyCSparseMatrix &lhs, py::array arr, py::object msr, py
     }
     if (a.ndim() != 2) throw std::invalid_argument("NumPy array expected to have 1 or two dimensions.");
     if ((uint64_t)a.shape(1) != lhs.nc) throw std::invalid_argument("Array must be of the same dimensionality as the matrix");
+    if (rev && rows2d_reverse) {   // out[x][y] = msr(first = matrix row x, second = data row y)
+        std::vector<float> C(a.data(), a.data() + a.size());
+        return cost_block(lhs.device(), lhs.nr, C, (uint64_t)a.shape(0), m, prior);
+    }
     // out[x][y] = msr(first = data row y, second = matrix row x)   (pycmp.cpp:193-197)
     Dense D = dense_of(a);
     return transpose_copy(cost_block(D.dev->h, D.n, lhs.dense_all(), lhs.nr, m, prior));
@@ -582,7 +592,8 @@ py::object kmeanspp_dense(py::array arr, int k, py::object msr, py::object prior
     return py::make_tuple(ctr, asn_array(best.asn, (uint64_t)k), costs);
 }
 
-py::object kmeanspp_csr(PyCSparseMatrix &smw, py::object msr, py::int_ k, double prior, uint64_t seed, unsigned,
+template <class M>
+py::object kmeanspp_csr(M &smw, py::object msr, py::int_ k, double prior, uint64_t seed, unsigned,
                         py::ssize_t lspp, bool expskips, py::ssize_t n_local_trials, py::object weights) {
     // Extra runs (ntimes) never replace the first in the reference (smw.h:248-256 compares the
     // first run's cost with itself), so the first run is the result.
@@ -706,6 +717,221 @@ struct CoresetSampler {
     }
 };
 
+// ---- SparseMatrixWrapper (python/smw.h:10-120, pysmw.cpp:31-265): an owned CSR copy -------
+// Rows keep their entries sorted by column (csc2sparse, csc.h:1221-1261); skip_empty drops
+// rows without entries; values are held as float32 (use_float) or float64.  The device copy
+// is built on first use (float64 values are converted to float32 on the device: the GPU path
+// computes in FP32).  blaze archives (tofile / fromfile / a path argument) need blaze, which
+// is not part of this build: they raise.
+struct SparseMatrixWrapper {
+    bool f32 = true;
+    std::vector<float> vf;
+    std::vector<double> vd;
+    std::vector<uint32_t> idx;
+    std::vector<uint64_t> ptr{0};
+    uint64_t nr = 0, nc = 0;
+    DataPtr dev;
+
+    SparseMatrixWrapper() = default;
+    SparseMatrixWrapper(py::object sp, bool skip_empty, bool use_float) : f32(use_float) {
+        if (py::isinstance<py::str>(sp) || !py::hasattr(sp, "indices"))
+            throw std::runtime_error("SparseMatrixWrapper from a blaze archive needs blaze, which this build does not "
+                                     "include; construct it from a scipy.sparse CSR matrix");
+        auto ip = py::array_t<int64_t, py::array::c_style | py::array::forcecast>::ensure(sp.attr("indptr"));
+        auto ix = py::array_t<int64_t, py::array::c_style | py::array::forcecast>::ensure(sp.attr("indices"));
+        auto dv = py::array_t<double, py::array::c_style | py::array::forcecast>::ensure(sp.attr("data"));
+        auto shape = py::cast<py::sequence>(sp.attr("shape"));
+        const uint64_t rows = shape[0].cast<uint64_t>();
+        nc = shape[1].cast<uint64_t>();
+        if (!ip || !ix || !dv || (uint64_t)ip.size() != rows + 1)
+            throw std::invalid_argument("expected a CSR matrix with indptr of rows + 1 entries");
+        const int64_t *P = ip.data(), *J = ix.data();
+        const double *V = dv.data();
+        std::vector<std::pair<uint32_t, double>> row;
+        ptr.assign(1, 0);
+        for (uint64_t i = 0; i < rows; ++i) {
+            if (P[i] < 0 || P[i + 1] < P[i] || P[i + 1] > (int64_t)dv.size())
+                throw std::invalid_argument("indptr out of range or decreasing");
+            if (skip_empty && P[i + 1] == P[i]) continue;
+            row.clear();
+            for (int64_t q = P[i]; q < P[i + 1]; ++q) {
+                if (J[q] < 0 || (uint64_t)J[q] >= nc) throw std::invalid_argument("column index out of range");
+                row.emplace_back((uint32_t)J[q], V[q]);
+            }
+            std::stable_sort(row.begin(), row.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+            for (auto &e : row) {
+                idx.push_back(e.first);
+                if (f32) vf.push_back((float)e.second); else vd.push_back(e.second);
+            }
+            ptr.push_back(idx.size());
+        }
+        nr = ptr.size() - 1;
+    }
+    uint64_t nnz() const { return idx.size(); }
+    double value(uint64_t q) const { return f32 ? (double)vf[q] : vd[q]; }
+    mcg_data *device() {
+        if (!dev) {
+            if (nr == 0) throw std::invalid_argument("empty matrix");
+            auto h = std::make_shared<DataHandle>();
+            const void *data = f32 ? (const void *)vf.data() : (const void *)vd.data();
+            static const float zero = 0.f;
+            check(mcg_data_csr(ctx(), nnz() ? data : &zero, f32 ? MCG_F32 : MCG_F64, nnz() ? idx.data() : (const uint32_t *)&zero,
+                               MCG_U32, ptr.data(), MCG_U64, nr, nc, nnz(), &h->h));
+            dev = h;
+        }
+        return dev->h;
+    }
+    std::vector<float> dense_rows(const std::vector<int64_t> &rows) const {
+        std::vector<float> out(rows.size() * nc, 0.f);
+        for (size_t q = 0; q < rows.size(); ++q) {
+            const int64_t r = rows[q];
+            if (r < 0 || (uint64_t)r >= nr) throw std::invalid_argument("center index out of range");
+            for (uint64_t e = ptr[r]; e < ptr[r + 1]; ++e) out[q * nc + idx[e]] = (float)value(e);
+        }
+        return out;
+    }
+    std::vector<float> dense_all() const {
+        std::vector<int64_t> rows(nr);
+        for (uint64_t i = 0; i < nr; ++i) rows[i] = (int64_t)i;
+        return dense_rows(rows);
+    }
+    SparseMatrixWrapper select(const std::vector<int64_t> *rows, const std::vector<int64_t> *cols) const {
+        SparseMatrixWrapper o;
+        o.f32 = f32;
+        o.nc = cols ? cols->size() : nc;
+        std::vector<int64_t> cmap;
+        if (cols) {
+            cmap.assign(nc, -1);
+            for (size_t q = 0; q < cols->size(); ++q) {
+                const int64_t c = (*cols)[q];
+                if (c < 0 || (uint64_t)c >= nc) throw std::invalid_argument("column index out of range");
+                cmap[c] = (int64_t)q;
+            }
+        }
+        const uint64_t m = rows ? rows->size() : nr;
+        std::vector<std::pair<uint32_t, double>> row;
+        for (uint64_t q = 0; q < m; ++q) {
+            const int64_t r = rows ? (*rows)[q] : (int64_t)q;
+            if (r < 0 || (uint64_t)r >= nr) throw std::invalid_argument("row index out of range");
+            row.clear();
+            for (uint64_t e = ptr[r]; e < ptr[r + 1]; ++e) {
+                if (!cols) row.emplace_back(idx[e], value(e));
+                else if (cmap[idx[e]] >= 0) row.emplace_back((uint32_t)cmap[idx[e]], value(e));
+            }
+            std::stable_sort(row.begin(), row.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+            for (auto &pr : row) {
+                o.idx.push_back(pr.first);
+                if (f32) o.vf.push_back((float)pr.second); else o.vd.push_back(pr.second);
+            }
+            o.ptr.push_back(o.idx.size());
+        }
+        o.nr = m;
+        return o;
+    }
+    void transpose() {
+        SparseMatrixWrapper o;
+        o.f32 = f32;
+        o.nr = nc;
+        o.nc = nr;
+        std::vector<uint64_t> cnt(nc + 1, 0);
+        for (auto j : idx) cnt[j + 1]++;
+        for (uint64_t j = 0; j < nc; ++j) cnt[j + 1] += cnt[j];
+        o.ptr = cnt;
+        o.idx.resize(nnz());
+        if (f32) o.vf.resize(nnz()); else o.vd.resize(nnz());
+        for (uint64_t i = 0; i < nr; ++i)
+            for (uint64_t e = ptr[i]; e < ptr[i + 1]; ++e) {
+                const uint64_t q = cnt[idx[e]]++;
+                o.idx[q] = (uint32_t)i;
+                if (f32) o.vf[q] = vf[e]; else o.vd[q] = vd[e];
+            }
+        *this = std::move(o);
+    }
+    std::string describe() const {
+        return "Matrix of " + std::to_string(nr) + "/" + std::to_string(nc) + " elements of " +
+               (f32 ? "float32" : "double") + ", " + std::to_string(nnz()) + " nonzeros";
+    }
+};
+
+std::vector<int64_t> int_list(py::object o) {
+    auto a = py::array_t<int64_t, py::array::c_style | py::array::forcecast>::ensure(o);
+    if (!a) throw std::invalid_argument("Wrong dtype");
+    return std::vector<int64_t>(a.data(), a.data() + a.size());
+}
+
+// out[l][r] = msr(first = rhs row r, second = lhs row l), or with reverse msr(first = lhs
+// row l, second = rhs row r) (pycmp.cpp:100-146); float32 unless use_float is 0 (or -1 with
+// both matrices double), then returned as float64 (the GPU computes in FP32)
+py::object cmp_smw_smw(SparseMatrixWrapper &lhs, SparseMatrixWrapper &rhs, py::object msr, py::object betaprior,
+                       bool reverse, int use_float) {
+    const int m = measure_code(msr);
+    if (use_float < 0) use_float = lhs.f32 || rhs.f32;
+    if (lhs.nc != rhs.nc) throw std::invalid_argument("mismatched # columns");
+    const double prior = betaprior.cast<double>();
+    py::array out = reverse ? py::array(cost_block(lhs.device(), lhs.nr, rhs.dense_all(), rhs.nr, m, prior))
+                            : transpose_copy(cost_block(rhs.device(), rhs.nr, lhs.dense_all(), lhs.nr, m, prior));
+    if (!use_float) out = py::cast<py::array>(out.attr("astype")("float64"));
+    return out;
+}
+
+// ---- SumOpts (clustering/mtx2cs.h:13-80, pysmw.cpp:293-366): the seeding options the kmeanspp
+// overloads read (measure, k, prior value, extra sample tries, lspp, local trials, seed) --------
+struct SumOpts {
+    int dis = MCG_JSD;
+    int prior_kind = 0;           // NONE, DIRICHLET, GAMMA_BETA, FEATURE_SPECIFIC_PRIOR
+    double gamma = 1.;
+    unsigned k = 10;
+    uint64_t seed = 0;
+    unsigned extra_sample_tries = 1;
+    unsigned lloyd_max_rounds = 1000;
+    int lspp = 0;
+    int sm = MCG_SENS_BFL;
+    bool soft = false, discrete_metric_search = false, use_exponential_skips = false;
+    double outlier_fraction = 0.;
+    size_t n_local_trials = 1;
+
+    static int sm_of(const std::string &s) {
+        static const std::map<std::string, int> names = {{"BFL", MCG_SENS_BFL}, {"FL", MCG_SENS_FL},
+                                                         {"LFKF", MCG_SENS_LFKF}, {"VX", MCG_SENS_VX},
+                                                         {"LBK", MCG_SENS_LBK}};
+        auto it = names.find(s);
+        return it == names.end() ? MCG_SENS_BFL : it->second;   // str2sm falls back to BFL
+    }
+    SumOpts(py::object measure, py::ssize_t k_, double prior_value, py::object sm_, double outlier, size_t rounds,
+            bool soft_)
+        : dis(measure_code(measure)), prior_kind(prior_value == 0. ? 0 : prior_value == 1. ? 1 : 2), gamma(prior_value),
+          k((unsigned)k_), lloyd_max_rounds((unsigned)rounds), soft(soft_), outlier_fraction(outlier) {
+        sm = py::isinstance<py::str>(sm_) ? sm_of(sm_.cast<std::string>()) : sm_.cast<int>();
+    }
+    static const char *sm_name(int v) {
+        switch (v) {
+        case MCG_SENS_BFL: return "BFL";
+        case MCG_SENS_VX: return "VX";
+        case MCG_SENS_LFKF: return "LFKF";
+        case MCG_SENS_LBK: return "LBK";
+        case MCG_SENS_FL: return "FL";
+        }
+        return "UNKNOWN";
+    }
+    std::string to_string() const {
+        char buf[512];
+        std::snprintf(buf, sizeof(buf),
+                      "K: %u. Construction method: %d. Seed:%zu. gamma: %g. prior: %d. extrasampletries: %u. dis: %d. "
+                      "%s clustering.", k, sm, (size_t)seed, gamma, prior_kind, extra_sample_tries, dis,
+                      soft ? "soft" : "hard");
+        return buf;
+    }
+};
+
+// kmeanspp(matrix, opts, weights) (pysmw.cpp:374-381, pycsparse.cpp:91-99): the option
+// fields passed to the keyword form, with ntimes = max(extra_sample_tries - 1, 0)
+template <class M>
+py::object kmeanspp_opts(M &smw, const SumOpts &so, py::object weights) {
+    return kmeanspp_csr(smw, py::int_(so.dis), py::int_(so.k), so.gamma, so.seed,
+                        (unsigned)std::max<int>((int)so.extra_sample_tries - 1, 0), so.lspp, so.use_exponential_skips,
+                        (py::ssize_t)so.n_local_trials, weights);
+}
+
 }  // namespace
 
 PYBIND11_MODULE(_pyminicore, m) {
@@ -724,7 +950,151 @@ PYBIND11_MODULE(_pyminicore, m) {
                                    (uint64_t)data.size());
         }, py::arg("data"), py::arg("indices"), py::arg("indptr"), py::arg("shape"));
 
-    m.def("hcluster", &hcluster_csr, py::arg("smw"), py::arg("centers"), py::arg("prior") = 0., py::arg("msr") = 2,
+    py::class_<SparseMatrixWrapper>(m, "SparseMatrixWrapper")
+        .def(py::init<>())
+        .def(py::init<py::object, bool, bool>(), py::arg("sparray"), py::arg("skip_empty") = false,
+             py::arg("use_float") = true)
+        .def("is_float", [](const SparseMatrixWrapper &x) { return x.f32; })
+        .def("is_double", [](const SparseMatrixWrapper &x) { return !x.f32; })
+        .def("rows", [](const SparseMatrixWrapper &x) { return x.nr; })
+        .def("columns", [](const SparseMatrixWrapper &x) { return x.nc; })
+        .def("nonzeros", [](const SparseMatrixWrapper &x) { return x.nnz(); })
+        .def("__str__", &SparseMatrixWrapper::describe)
+        .def("__repr__", &SparseMatrixWrapper::describe)
+        .def("tocsr", [](const SparseMatrixWrapper &x) {   // pysmw.cpp:40-62
+            py::array_t<py::ssize_t> indptr(x.nr + 1), shape(2);
+            py::array_t<int32_t> indices(x.nnz());
+            py::array_t<double> data(x.nnz());
+            for (uint64_t i = 0; i <= x.nr; ++i) indptr.mutable_data()[i] = (py::ssize_t)x.ptr[i];
+            for (uint64_t q = 0; q < x.nnz(); ++q) {
+                indices.mutable_data()[q] = (int32_t)x.idx[q];
+                data.mutable_data()[q] = x.value(q);
+            }
+            shape.mutable_data()[0] = (py::ssize_t)x.nr;
+            shape.mutable_data()[1] = (py::ssize_t)x.nc;
+            return py::make_tuple(py::make_tuple(data, indices, indptr), shape);
+        })
+        .def("rowsel", [](const SparseMatrixWrapper &x, py::array idx) -> py::object {   // pysmw.cpp:87-123
+            if (idx.dtype().kind() == 'f') throw std::invalid_argument("Unexpected type");
+            auto rows = int_list(idx);
+            auto v = x.dense_rows(rows);
+            if (x.f32) {
+                py::array_t<float> out({(py::ssize_t)rows.size(), (py::ssize_t)x.nc});
+                std::memcpy(out.mutable_data(), v.data(), v.size() * 4);
+                return out;
+            }
+            py::array_t<double> out({(py::ssize_t)rows.size(), (py::ssize_t)x.nc});
+            std::fill(out.mutable_data(), out.mutable_data() + out.size(), 0.);
+            for (size_t q = 0; q < rows.size(); ++q)
+                for (uint64_t e = x.ptr[rows[q]]; e < x.ptr[rows[q] + 1]; ++e)
+                    out.mutable_data()[q * x.nc + x.idx[e]] = x.vd[e];
+            return out;
+        }, py::arg("idx"))
+        .def("submatrix", [](const SparseMatrixWrapper &x, py::object rowsel, py::object columnsel) {   // :127-169
+            std::vector<int64_t> r, c;
+            if (!rowsel.is_none()) r = int_list(rowsel);
+            if (!columnsel.is_none()) c = int_list(columnsel);
+            return x.select(rowsel.is_none() ? nullptr : &r, columnsel.is_none() ? nullptr : &c);
+        }, py::arg("rowsel") = py::none(), py::arg("columnsel") = py::none())
+        .def("transpose_", [](SparseMatrixWrapper &x) { x.transpose(); })
+        .def("sum", [](const SparseMatrixWrapper &x, int byrow, bool usefloat) -> py::object {   // :204-237
+            if (byrow < -1 || byrow > 1) throw std::invalid_argument("byrow must be -1 (total sum), 0 (by column) or by row (1)");
+            std::vector<double> acc(byrow == 1 ? x.nr : byrow == 0 ? x.nc : 1, 0.);
+            for (uint64_t i = 0; i < x.nr; ++i)
+                for (uint64_t e = x.ptr[i]; e < x.ptr[i + 1]; ++e)
+                    acc[byrow == 1 ? i : byrow == 0 ? x.idx[e] : 0] += x.value(e);
+            if (byrow == -1) return py::float_(acc[0]);
+            if (usefloat) {
+                py::array_t<float> out(acc.size());
+                for (size_t q = 0; q < acc.size(); ++q) out.mutable_data()[q] = (float)acc[q];
+                return out;
+            }
+            py::array_t<double> out(acc.size());
+            std::memcpy(out.mutable_data(), acc.data(), acc.size() * 8);
+            return out;
+        }, py::arg("kind") = -1, py::arg("usefloat") = true)
+        .def("count_nnz", [](const SparseMatrixWrapper &x, int byrow) -> py::object {   // :238-265
+            if (byrow <= 0) return py::int_(x.nnz());
+            py::array_t<uint64_t> out(x.nr);
+            for (uint64_t i = 0; i < x.nr; ++i) out.mutable_data()[i] = x.ptr[i + 1] - x.ptr[i];
+            const char *dt = x.nc <= 0xFFu ? "uint8" : x.nc <= 0xFFFFu ? "uint16" : x.nc <= 0xFFFFFFFFu ? "uint32" : "uint64";
+            return out.attr("astype")(dt);
+        }, py::arg("byrow") = 0)
+        .def("tofile", [](const SparseMatrixWrapper &, std::string) {
+            throw std::runtime_error("SparseMatrixWrapper.tofile writes a blaze archive; blaze is not part of this build");
+        }, py::arg("path"))
+        .def("fromfile", [](SparseMatrixWrapper &, std::string) {
+            throw std::runtime_error("SparseMatrixWrapper.fromfile reads a blaze archive; blaze is not part of this build");
+        }, py::arg("path"));
+
+    py::class_<SumOpts>(m, "SumOpts")
+        .def(py::init<py::object, py::ssize_t, double, py::object, double, size_t, bool>(), py::arg("measure") = 0,
+             py::arg("k") = 10, py::arg("prior") = 0., py::arg("sm") = "BFL", py::arg("outlier_fraction") = 0.,
+             py::arg("max_rounds") = 100, py::arg("soft") = false)
+        .def("__str__", &SumOpts::to_string)
+        .def("__repr__", &SumOpts::to_string)
+        .def_readwrite("lspp", &SumOpts::lspp)
+        .def_readwrite("gamma", &SumOpts::gamma)
+        .def_readwrite("k", &SumOpts::k)
+        .def_readwrite("search_max_rounds", &SumOpts::lloyd_max_rounds)
+        .def_readwrite("extra_sample_rounds", &SumOpts::extra_sample_tries)
+        .def_readwrite("soft", &SumOpts::soft)
+        .def_readwrite("outlier_fraction", &SumOpts::outlier_fraction)
+        .def_readwrite("discrete_metric_search", &SumOpts::discrete_metric_search)
+        .def_readwrite("use_exponential_skips", &SumOpts::use_exponential_skips)
+        .def_property("cs", [](const SumOpts &o) { return std::string(SumOpts::sm_name(o.sm)); },
+                      [](SumOpts &o, py::object v) {
+                          if (py::isinstance<py::str>(v)) o.sm = SumOpts::sm_of(v.cast<std::string>());
+                          else if (py::isinstance<py::int_>(v)) o.sm = v.cast<int>();
+                          else throw std::invalid_argument("value must be str or int");
+                      })
+        .def_property("prior", [](const SumOpts &o) -> std::string {
+            switch (o.prior_kind) {
+            case 0: return "NONE";
+            case 1: return "DIRICHLET";
+            case 2: return "GAMMA";
+            case 3: return "FSP";
+            }
+            throw std::invalid_argument("Invalid prior");
+        }, [](SumOpts &o, py::object v) {
+            if (v.is_none()) { o.prior_kind = 0; return; }
+            if (py::isinstance<py::str>(v)) {
+                static const std::map<std::string, int> names = {{"NONE", 0}, {"DIRICHLET", 1}, {"GAMMA", 2},
+                                                                 {"GAMMA_BETA", 2}, {"FSP", 3},
+                                                                 {"FEATURE_SPECIFIC_PRIOR", 3}};
+                auto it = names.find(v.cast<std::string>());
+                if (it == names.end())
+                    throw std::out_of_range("Prior must be NONE, FSP, FEATURE_SPECIFIC_PRIOR, GAMMA, or DIRICHLET");
+                o.prior_kind = it->second;
+            } else {
+                const auto x = v.cast<py::ssize_t>();
+                if (x > 3) throw std::out_of_range("x must be <= 3 if an integer, to represent various priors");
+                o.prior_kind = (int)x;
+            }
+        });
+
+    // SparseMatrixWrapper overloads first: a py::array parameter would also accept the object
+    m.def("hcluster", &hcluster_csr<SparseMatrixWrapper>, py::arg("smw"), py::arg("centers"), py::arg("prior") = 0.,
+          py::arg("msr") = 2, py::arg("weights") = py::none(), py::arg("eps") = 1e-10, py::arg("maxiter") = 100,
+          py::arg("seed") = 0, py::arg("mbsize") = py::ssize_t(-1), py::arg("ncheckins") = py::ssize_t(-1),
+          py::arg("reseed_count") = py::ssize_t(5), py::arg("with_rep") = false, py::arg("cs") = false);
+    m.def("scluster", &scluster_csr<SparseMatrixWrapper>, py::arg("smw"), py::arg("centers"), py::arg("msr") = 2,
+          py::arg("prior") = 0., py::arg("temp") = 1., py::arg("maxiter") = 1000, py::arg("mbsize") = py::ssize_t(-1),
+          py::arg("mbn") = py::ssize_t(-1), py::arg("savepref") = "", py::arg("weights") = py::none());
+    m.def("cmp", [](SparseMatrixWrapper &lhs, py::array arr, py::object msr, py::object prior, py::object reverse) {
+        return cmp_csr_array(lhs, arr, msr, prior, reverse, true);
+    }, py::arg("matrix"), py::arg("data"), py::arg("msr") = 2, py::arg("prior") = 0., py::arg("reverse") = false);
+    m.def("cmp", &cmp_smw_smw, py::arg("matrix"), py::arg("data"), py::arg("msr") = 2, py::arg("prior") = 0.,
+          py::arg("reverse") = false, py::arg("use_float") = -1);
+    m.def("kmeanspp", &kmeanspp_csr<SparseMatrixWrapper>, py::arg("smw"), py::arg("msr"), py::arg("k"),
+          py::arg("prior") = 0., py::arg("seed") = 0, py::arg("ntimes") = 1, py::arg("lspp") = 0,
+          py::arg("expskips") = false, py::arg("n_local_trials") = 1, py::arg("weights") = py::none());
+    m.def("kmeanspp", &kmeanspp_opts<SparseMatrixWrapper>, py::arg("smw"), py::arg("opts"),
+          py::arg("weights") = py::none());
+    m.def("kmeanspp", &kmeanspp_opts<PyCSparseMatrix>, py::arg("smw"), py::arg("opts"),
+          py::arg("weights") = py::none());
+
+    m.def("hcluster", &hcluster_csr<PyCSparseMatrix>, py::arg("smw"), py::arg("centers"), py::arg("prior") = 0., py::arg("msr") = 2,
           py::arg("weights") = py::none(), py::arg("eps") = 1e-10, py::arg("maxiter") = 100, py::arg("seed") = 0,
           py::arg("mbsize") = py::ssize_t(-1), py::arg("ncheckins") = py::ssize_t(-1),
           py::arg("reseed_count") = py::ssize_t(5), py::arg("with_rep") = false, py::arg("cs") = false);
@@ -734,14 +1104,16 @@ PYBIND11_MODULE(_pyminicore, m) {
           py::arg("reseed_count") = py::ssize_t(5), py::arg("with_rep") = false, py::arg("cs") = false,
           "Clusters the rows using the centers provided; Lloyd / EM until convergence");
 
-    m.def("scluster", &scluster_csr, py::arg("smw"), py::arg("centers"), py::arg("msr") = 2, py::arg("prior") = 0.,
+    m.def("scluster", &scluster_csr<PyCSparseMatrix>, py::arg("smw"), py::arg("centers"), py::arg("msr") = 2, py::arg("prior") = 0.,
           py::arg("temp") = 1., py::arg("maxiter") = 1000, py::arg("mbsize") = py::ssize_t(-1),
           py::arg("mbn") = py::ssize_t(-1), py::arg("savepref") = "", py::arg("weights") = py::none());
     m.def("scluster", &scluster_dense, py::arg("matrix"), py::arg("centers"), py::arg("msr") = 2, py::arg("prior") = 0.,
           py::arg("temp") = 1., py::arg("maxiter") = 1000, py::arg("mbsize") = py::ssize_t(-1),
           py::arg("mbn") = py::ssize_t(-1), py::arg("savepref") = "", py::arg("weights") = py::none());
 
-    m.def("cmp", &cmp_csr_array, py::arg("matrix"), py::arg("data"), py::arg("msr") = 2, py::arg("prior") = 0.,
+    m.def("cmp", [](PyCSparseMatrix &lhs, py::array arr, py::object msr, py::object prior, py::object reverse) {
+        return cmp_csr_array(lhs, arr, msr, prior, reverse);
+    }, py::arg("matrix"), py::arg("data"), py::arg("msr") = 2, py::arg("prior") = 0.,
           py::arg("reverse") = false);
     m.def("cmp", &cmp_csr_csr, py::arg("matrix"), py::arg("data"), py::arg("msr") = 2, py::arg("prior") = 0.);
     m.def("cmp", &cmp_arrays, py::arg("lhs"), py::arg("rhs"), py::arg("msr") = 2, py::arg("prior") = 0.,
@@ -755,7 +1127,7 @@ PYBIND11_MODULE(_pyminicore, m) {
     m.def("kmeanspp", &kmeanspp_dense, py::arg("matrix"), py::arg("k"), py::arg("msr") = py::int_(2),
           py::arg("prior") = 0., py::arg("seed") = 0, py::arg("ntimes") = 0, py::arg("lspp") = 0,
           py::arg("weights") = py::none(), py::arg("expskips") = false, py::arg("n_local_trials") = 1);
-    m.def("kmeanspp", &kmeanspp_csr, py::arg("smw"), py::arg("msr"), py::arg("k"), py::arg("prior") = 0.,
+    m.def("kmeanspp", &kmeanspp_csr<PyCSparseMatrix>, py::arg("smw"), py::arg("msr"), py::arg("k"), py::arg("prior") = 0.,
           py::arg("seed") = 0, py::arg("ntimes") = 1, py::arg("lspp") = 0, py::arg("expskips") = false,
           py::arg("n_local_trials") = 1, py::arg("weights") = py::none());
 

@@ -94,6 +94,67 @@ def test_cmp_orientation(mco):
     np.testing.assert_allclose(out2, ref, rtol=2e-4)
 
 
+def test_sparse_matrix_wrapper_overloads(mco):
+    """The SparseMatrixWrapper overloads (pycluster.cpp:6-26, pycluster_soft.cpp:4-28,
+    pycmp.cpp:16-146, pysmw.cpp:368-381) and kmeanspp(matrix, SumOpts) for both sparse
+    types (pysmw.cpp:374-381, pycsparse.cpp:91-99): each against the oracle, and the
+    wrapper's results equal the CSparseMatrix path on the same rows."""
+    say("api SparseMatrixWrapper")
+    n, d = 1200, 250
+    data, idx, ptr = poisson_csr(n, d, 0.05, 7, normalize=True)
+    m = sp.csr_matrix((data, idx.astype(np.int32), ptr.astype(np.int64)), shape=(n, d))
+    X = mco.CSR(data, idx, ptr, (n, d))
+    w = mc.SparseMatrixWrapper(m)
+    prior = 1. / d
+    # kmeanspp keyword form: the same seeding as the CSparseMatrix path
+    ids, asn, costs = mc.kmeanspp(w, 6, msr=5, prior=prior, seed=9)
+    ids2, asn2, costs2 = mc.kmeanspp(m, 6, msr=5, prior=prior, seed=9)
+    assert np.array_equal(ids, ids2) and np.array_equal(asn, asn2) and np.array_equal(costs, costs2)
+    assert asn.dtype == np.uint8 and costs.dtype == np.float64 and len(set(ids.tolist())) == 6
+    K = mco.cost_matrix(mco.MKL, X, X.todense()[ids.astype(np.int64)], prior)
+    np.testing.assert_allclose(costs, K[np.arange(n), asn], rtol=2e-4, atol=1e-7)
+    # kmeanspp(matrix, SumOpts): k, measure and prior from the options (random seed: 0)
+    so = mc.SumOpts("MKL", k=5, prior=prior)
+    for mat in (w, mc.CSparseMatrix(m)):
+        i3, a3, c3 = mc._ext().kmeanspp(mat, so)
+        assert len(i3) == 5 and len(set(i3.tolist())) == 5 and a3.dtype == np.uint8
+        K3 = mco.cost_matrix(mco.MKL, X, X.todense()[i3.astype(np.int64)], prior)
+        np.testing.assert_allclose(c3, K3[np.arange(n), a3], rtol=2e-4, atol=1e-7)
+    # hcluster / scluster: equal to the CSparseMatrix path
+    r = mc.hcluster(w, ids, msr=5, prior=prior, maxiter=10)
+    r2 = mc.hcluster(m, ids, msr=5, prior=prior, maxiter=10)
+    assert r["numiter"] == r2["numiter"] and np.array_equal(r["asn"], r2["asn"]) and r["finalcost"] == r2["finalcost"]
+    (cd, ci, cp), shape = r["centers"]
+    C = sp.csr_matrix((cd, ci, cp), shape=tuple(shape)).toarray()
+    a_ref, _, gap = mco.assign_f64(mco.MKL, X, C, prior)
+    assert np.all(gap[a_ref != r["asn"]] < 1e-4)
+    s1 = mc.scluster(w, ids, msr=11, prior=prior, maxiter=3)
+    s2 = mc.scluster(m, ids, msr=11, prior=prior, maxiter=3)
+    assert np.array_equal(s1["asn"], s2["asn"]) and np.array_equal(s1["costs"], s2["costs"])
+    ref = mco.soft(11, X, X.todense()[ids.astype(np.int64)], prior=prior, maxiter=3)
+    np.testing.assert_allclose(s1["costs"], ref["costs"], rtol=2e-4)
+    # cmp(smw, array): out[x, y] = msr(first = array row y, second = matrix row x); with
+    # reverse msr(first = matrix row x, second = array row y); 1-d arrays likewise
+    B = np.random.default_rng(2).random((4, d)).astype(np.float32)
+    out = mc._ext().cmp(w, B, msr=5, prior=prior)
+    np.testing.assert_allclose(out, mco.cost_matrix(mco.MKL, B, X.todense(), prior).T, rtol=2e-4)
+    outr = mc._ext().cmp(w, B, msr=5, prior=prior, reverse=True)
+    np.testing.assert_allclose(outr, mco.cost_matrix(mco.MKL, X, B, prior), rtol=2e-4)
+    out1 = mc._ext().cmp(w, B[1], msr=5, prior=prior, reverse=True)
+    np.testing.assert_allclose(out1, outr[:, 1], rtol=1e-6)
+    # cmp(smw, smw): out[l, r] = msr(first = rhs row r, second = lhs row l); use_float=0 gives float64
+    w2 = mc.SparseMatrixWrapper(m[:30])
+    o = mc._ext().cmp(w2, w, msr=4, prior=prior)
+    assert o.shape == (30, n) and o.dtype == np.float32
+    np.testing.assert_allclose(o, mco.cost_matrix(mco.JSD, X, X.todense()[:30], prior).T, rtol=2e-4, atol=1e-7)
+    orv = mc._ext().cmp(w2, w, msr=5, prior=prior, reverse=True, use_float=0)
+    assert orv.dtype == np.float64
+    Xs = mco.CSR(data[:int(ptr[30])], idx[:int(ptr[30])], ptr[:31].copy(), (30, d))
+    # self-distances are 0 in FP64 and a few 1e-7 after the FP32 cancellation of the
+    # decomposed terms (costs here are ~1.6)
+    np.testing.assert_allclose(orv, mco.cost_matrix(mco.MKL, Xs, X.todense(), prior), rtol=2e-4, atol=2e-6)
+
+
 def test_coreset_sampler_class(mco):
     say("api coreset sampler")
     rng = np.random.default_rng(3)

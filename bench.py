@@ -236,11 +236,29 @@ def sparse_config(args, mcg, cfg):
         t0 = time.perf_counter()
         ds.cost_matrix(mcg.REVERSE_MKL, C0, prior=1e-3)
         t_e = time.perf_counter() - t0
-        # first call builds the column-major copy used by the soft M-step
+        # first call builds the column-major copy and the tiles used by the soft M-step
         ds.soft(mcg.REVERSE_MKL, C0, prior=1e-3, temp=1., eps=-1., maxiter=1, with_outputs=False)
+        ctx.synchronize()
+        ctx.timing(True)
         t0 = time.perf_counter()
         res = ds.soft(mcg.REVERSE_MKL, C0, prior=1e-3, temp=1., eps=-1., maxiter=10, with_outputs=False)
         t_s = time.perf_counter() - t0
+        e_ms, e_n = ctx.timing_read(0)
+        m_ms, m_n = ctx.timing_read(1)
+        ctx.timing(False)
+        e_ms, m_ms = e_ms / max(e_n, 1), m_ms / max(m_n, 1)
+        # algorithmic bytes per launch: the stored entries (8 B each), the n x k costs /
+        # responsibilities (FP32) and the k x d table / FP64 numerator
+        e_alg = nnz * 8 + n * k * 4 + k * d * 4
+        m_alg = nnz * 8 + n * k * 4 + k * d * 8
+        dom_ms, dom_alg, dom = (m_ms, m_alg, "k_tile_mstep") if m_ms >= e_ms else (e_ms, e_alg, "k_tile_estep")
+        ach = dom_alg / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.
+        out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                           "frac": ach / PEAK_HBM_GBS, "traffic": None, "kernel": dom,
+                           "algorithmic_bytes": dom_alg, "kernel_ms": dom_ms,
+                           "vector_tflops": 2. * nnz * k / (dom_ms * 1e-3) / 1e12 if dom_ms > 0 else 0.}
+        out.update({"estep_kernel_ms": e_ms, "estep_launches": e_n, "mstep_kernel_ms": m_ms,
+                    "mstep_launches": m_n})
         if not args.no_cpu:
             from oracle import mco
             ns = 100

@@ -1050,6 +1050,10 @@ void generic_costs(Data &ds, int msr, double prior, const float *C, uint64_t k, 
     const uint32_t *slot = T.slot;
     const dim3 grid((uint32_t)cdiv(n, 4)), blk(256);
     const float fpv = (float)pv, lpv = (float)log(pv > 0 ? pv : 1.);
+    if (!single && tiled_costs_ok(ds, msr, k) && tiled_costs_ready(ds, k, st)) {
+        tiled_costs(ds, msr, CT, kpad, k, fpv, lpv, rt, cs, G, slot, out_nk, asn, best, st);
+        return;
+    }
     if (single) {
         const Cost1 P{X, val, idx, ptr, d, fpv, lpv, rt, CT, kpad, cs, G, slot};
         switch (msr) {

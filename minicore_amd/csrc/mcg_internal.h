@@ -80,6 +80,22 @@ struct DevView {
     DBuf staged;
 };
 
+// ---- tiled sparse products (mcg_tiles.hip) ------------------------------------
+// A sparse matrix reorganised into tiles of tg accumulator indices (twr per wave, 16
+// waves) x ts operand rows: per tile the per-index end offsets inside the wave's list
+// (u8), the 16 waves' base offsets (u16) and the tile's first entry (u32); entries
+// (byte offset of the operand row in the stage buffer, value bits).  cpl: centers per
+// lane of the kernel the tiles were built for (0: none built).
+struct Tiles {
+    bool ready = false;
+    int cpl = 0;
+    uint32_t twr = 0, tg = 0, ts = 0;
+    uint64_t nacc = 0, nop = 0;
+    uint32_t ngroups = 0, nstages = 0;
+    uint32_t maxseg = 0, maxwave = 0;   // largest stage segment of a block / of a wave
+    DBuf cnt, wb, off, ent;
+};
+
 // ---- dataset ----------------------------------------------------------------
 struct Data {
     Ctx *ctx = nullptr;
@@ -108,6 +124,8 @@ struct Data {
     DBuf li_slot;      // per-row index into li_unique
     // column-major copy of the CSR rows (soft M-step), built on first use
     DBuf csc_ptr, csc_row, csc_val;
+    // tiles of the rows (E-step) and of the columns (soft M-step), built on first use
+    Tiles rtiles, ctiles;
 };
 
 struct Ctx {
@@ -231,6 +249,18 @@ inline bool msr_mean_centroid(int msr) {
     return false;
 }
 
+// Tiled sparse products (CSR rows): the decomposed costs' cross terms against the
+// center table CT (d x kpad) and the soft M-step numerator over the column tiles.
+bool tiled_costs_ok(const Data &ds, int msr, uint64_t k);
+bool tiled_costs_ready(Data &ds, uint64_t k, hipStream_t st);
+void tiled_costs(Data &ds, int msr, const float *CT, uint32_t kpad, uint64_t k, float pv, float logpv,
+                 const float4 *rt, const double *cs, const double *G, const uint32_t *slot, float *out_nk,
+                 uint32_t *asn, float *best, hipStream_t st);
+bool tiled_mstep_ok(const Data &ds, uint64_t k);
+bool tiled_mstep_ready(Data &ds, uint64_t k, const uint64_t *cptr, const uint32_t *crow, const float *cval,
+                       hipStream_t st);
+void tiled_soft_numer(Data &ds, const float *R, uint64_t k, const double *den, float *Cnew, hipStream_t st);
+
 // Generic-measure kernels (dense or CSR rows vs dense centers), FP32.
 void generic_costs(Data &ds, int msr, double prior, const float *C, uint64_t k, float *out_nk,
                    uint32_t *asn, float *best, hipStream_t st);
@@ -286,6 +316,10 @@ void rows_into(const Data &S, const uint64_t *idx_h, const uint64_t *idx_d, uint
 // Drop the data derived from a dataset's rows (rebuilt on next use).
 inline void reset_derived(Data &D) {
     D.bf16_ready = false;
+    D.rtiles.ready = false;
+    D.rtiles.cpl = 0;
+    D.ctiles.ready = false;
+    D.ctiles.cpl = 0;
     D.rowsum.release();
     D.tf_prior = -1.;
     D.li_unique.clear();

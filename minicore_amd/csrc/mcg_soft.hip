@@ -245,6 +245,11 @@ void soft_mstep(Data &ds, const float *R, const double *w, uint64_t k, float *Cn
     double *den = (double *)c.s[S_PARTQ].reserve(k * 8);
     hipLaunchKernelGGL(k_soft_den_part, dim3(S), dim3(256), 0, st, R, n, (uint32_t)k, w, rows_per, dpart);
     hipLaunchKernelGGL(k_soft_den_final, dim3((uint32_t)cdiv(k, 256)), dim3(256), 0, st, dpart, S, (uint32_t)k, den);
+    if (tiled_mstep_ok(ds, k) && tiled_mstep_ready(ds, k, ds.csc_ptr.as<const uint64_t>(),
+                                                   ds.csc_row.as<const uint32_t>(), ds.csc_val.as<const float>(), st)) {
+        tiled_soft_numer(ds, R, k, den, Cnew, st);
+        return;
+    }
     // up to 8 entry chunks per column, so the columns with many entries are spread
     // over waves (partial sums: Z x k x d FP64)
     const uint64_t kd = k * (uint64_t)d;

@@ -149,11 +149,12 @@ def test_csr_centroids_skewed(mcg, ctx, mco, weighted):
     np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-9)
 
 
-@pytest.mark.parametrize("k", [130, 300])
+@pytest.mark.parametrize("k", [100, 130, 300])
 def test_soft_em_csr_many_centers(mcg, ctx, mco, k):
-    """Soft CSR EM with k = 130 (two-and-a-bit center quads per lane) and k = 300
-    (two 256-center groups), with a few columns stored in every row so that each of
-    their entry chunks holds entries: costs, responsibilities and centers against the
+    """Soft CSR EM with k = 100 (two centers per lane in the tiled E-step), k = 130
+    (two-and-a-bit center quads per lane) and k = 300 (two 256-center slices), with a
+    few columns stored in every row, so that stages overflow and the tiles are rebuilt
+    with fewer operand rows per stage: costs, responsibilities and centers against the
     oracle."""
     say(f"soft csr k={k}")
     n, d = 1200, 400

@@ -118,20 +118,39 @@ def serving(mcg, reqs, prec, rate, seed, workers):
     return np.array(lat), wall, np.array(stages)
 
 
-def fp64_check(ds, mcg, X, C0, iters, sample=16384):
-    """Error of the BF16-refined form against FP64, after `iters` iterations from C0:
-    the assignment of a row sample (BF16 E-step against the final centers) against
-    its FP64 argmin, and the sample's total cost (FP64 over the assignment) against
-    the FP64-optimal total.  Differing points are reported with their FP64 top-2 gap."""
+def fp64_check(ds, mcg, X, C0, iters, prec, sample=16384):
+    """Error of the 16-bit-refined form against FP64, after `iters` iterations from C0:
+    (1) the cost the last mcg_lloyd_step reported against the FP64 total over all rows
+    for the same assignment and input centers, and against the FP64-optimal total
+    (FP64 argmin of every row); (2) the assignment of a row sample (E-step against the
+    final centers) against its FP64 argmin, and the sample's total cost against the
+    FP64-optimal total.  Differing points are reported with their FP64 top-2 gap."""
     import numpy as np
     import torch
     C = C0.clone()
     a = torch.empty(X.shape[0], dtype=torch.int32, device="cuda")
-    for _ in range(iters):
-        ds.lloyd_step(mcg.SQRL2, C, a, precision=mcg.PREC_BF16)
+    reported = None
+    Cprev = None
+    for it in range(iters):
+        if it == iters - 1:
+            Cprev = C.clone()
+        reported = ds.lloyd_step(mcg.SQRL2, C, a, precision=prec)
     torch.cuda.synchronize()
+    # FP64 totals over all rows, in row chunks on the device
+    C64 = Cprev.double()
+    cn64 = (C64 * C64).sum(1)
+    tot_asn = 0.
+    tot_opt = 0.
+    for i in range(0, X.shape[0], 65536):
+        xs = X[i:i + 65536].double()
+        ai = a[i:i + 65536].long()
+        tot_asn += float(((xs - C64[ai]) ** 2).sum())
+        dd = (xs * xs).sum(1)[:, None] + cn64[None, :] - 2. * (xs @ C64.T)
+        bi = dd.argmin(1)
+        tot_opt += float(((xs - C64[bi]) ** 2).sum())
+    del C64
     Cf = C.cpu().numpy()
-    asn, _ = ds.assign(mcg.SQRL2, Cf, precision=mcg.PREC_BF16)
+    asn, _ = ds.assign(mcg.SQRL2, Cf, precision=prec)
     rows = np.random.default_rng(0).choice(X.shape[0], sample, replace=False)
     Xs = X[torch.as_tensor(rows, device="cuda")].cpu().numpy().astype(np.float64)
     C64 = Cf.astype(np.float64)
@@ -155,7 +174,10 @@ def fp64_check(ds, mcg, X, C0, iters, sample=16384):
         mine[i:i + 2048] = ((xs - C64[a_s]) ** 2).sum(-1)
     diff = arg != asn[rows]
     gap = (second - best) / np.maximum(best, 1e-300)
-    return {"sample_rows": sample, "iterations": iters,
+    return {"reported_cost_last_step": reported, "cost_fp64_all_rows": tot_asn,
+            "reported_cost_rel_err_vs_fp64": abs(reported - tot_asn) / tot_asn,
+            "cost_fp64_optimal_all_rows": tot_opt, "assignment_cost_rel_gap_vs_fp64_optimal": (tot_asn - tot_opt) / tot_opt,
+            "sample_rows": sample, "iterations": iters,
             "cost_rel_err_vs_fp64": float(abs(mine.sum() - best.sum()) / best.sum()),
             "assignments_differing": int(diff.sum()),
             "differing_with_gap_ge_1e-6": int((diff & (gap >= 1e-6)).sum())}
@@ -466,6 +488,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--form", choices=["fp16", "bf16"], default="fp16",
+                    help="16-bit operand format of the headline E-step (both give the exact argmin; "
+                         "the other one is timed too and reported beside it)")
     ap.add_argument("--config", choices=["c3", "c2", "c4", "minibatch", "median"], default="c3",
                     help="c3: the headline dense Lloyd workload; c2/c4: auxiliary sparse measurements; "
                          "minibatch: the minibatch driver on the headline rows; median: the L1 / TVD / L2 "
@@ -503,10 +528,16 @@ def main():
 
     ctx = mcg.Context(0)
     ds = mcg.Dataset.dense(X, ctx)
-    log("timing BF16 form")
-    bf = time_form(ds, ctx, mcg, C0, asn, mcg.PREC_BF16, args.steps, args.warmup)
+    form = args.form.upper()
+    other = "BF16" if form == "FP16" else "FP16"
+    prec = getattr(mcg, "PREC_" + form)
+    log(f"timing {form} form")
+    bf = time_form(ds, ctx, mcg, C0, asn, prec, args.steps, args.warmup)
     log(json.dumps(bf))
-    fp64 = fp64_check(ds, mcg, X, C0, args.steps + args.warmup)
+    log(f"timing {other} form")
+    ob = time_form(ds, ctx, mcg, C0, asn, getattr(mcg, "PREC_" + other), args.steps, args.warmup)
+    log(json.dumps(ob))
+    fp64 = fp64_check(ds, mcg, X, C0, args.steps + args.warmup, prec)
     log(json.dumps(fp64))
     fp = None
     if not args.no_fp32:
@@ -522,7 +553,7 @@ def main():
     kpad = (k + 63) // 64 * 64
     alg_bytes = n * dpad * 2 + n * 4 + n * 4 + kpad * dpad * 2
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_estep_bf16_n{n}_d{d}_k{k}.json")
+    tpath = os.path.join(ROOT, "profiles", f"traffic_estep_{args.form}_n{n}_d{d}_k{k}.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -545,7 +576,7 @@ def main():
         svc = []
         with mcg.Server(0, nworkers=1) as srv:
             for i, q in enumerate(reqs):
-                t = _submit(srv, mcg, q, mcg.PREC_BF16, i + 1)
+                t = _submit(srv, mcg, q, prec, i + 1)
                 r = srv.wait(t, 600.)
                 if r["state"] != mcg.REQ_DONE:
                     raise RuntimeError(f"request failed: {r}")
@@ -554,7 +585,7 @@ def main():
         svc_ms = float(np.mean(svc))
         rate = 0.7 / (svc_ms * 1e-3)
         log(f"serving: mean service {svc_ms:.2f} ms -> {rate:.1f} req/s")
-        lat_bf, wall_bf, st_bf = serving(mcg, reqs, mcg.PREC_BF16, rate, args.seed + 11, args.serve_workers)
+        lat_bf, wall_bf, st_bf = serving(mcg, reqs, prec, rate, args.seed + 11, args.serve_workers)
         lat_fp, wall_fp, _ = serving(mcg, reqs, mcg.PREC_FP32, rate, args.seed + 11, args.serve_workers)
         worst = int(np.argmax(lat_bf))
         pct = lambda a, q: float(np.percentile(a, q))
@@ -594,18 +625,21 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16",
+        "dtype": args.form,
         "data": "synthetic",
         "config": {"workload": f"dense sqL2 Lloyd iteration (E+M), n={n} d={d} k={k}, {k}-component Gaussian "
-                               f"mixture (means U[-10,10], scales U[0.5,2]), float32 rows in HBM; BF16 MFMA cross "
-                               f"term + FP32 near-tie re-score + FP64 centroid/cost sums",
+                               f"mixture (means U[-10,10], scales U[0.5,2]), float32 rows in HBM; {form} MFMA cross "
+                               f"term + FP32 near-tie re-score (exact FP32 argmin) + FP64 centroid/cost sums",
                    "n": n, "d": d, "k": k, "seed": args.seed, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_BF16_TFLOPS, "traffic": traffic,
                      "traffic_algorithmic_bytes": alg_bytes,
-                     "kernel": "k_estep_bf16<16,2>", "kernel_ms": bf["estep_ms"],
+                     "kernel": f"k_estep16<16,2,{'true' if form == 'FP16' else 'false'}>", "kernel_ms": bf["estep_ms"],
                      "algorithmic": "2*n*k*d flop per launch"},
         "latency_ms": {"p50": bf["p50_ms"], "p99": bf["p99_ms"]},
+        f"{other.lower()}_form": {"iters_per_s": 1.0 / ob["sec_per_step"], "ms_per_step": 1e3 * ob["sec_per_step"],
+                                  "estep_ms": ob["estep_ms"], "near_ties_last_step": {"pairs": ob["refine_pairs"],
+                                                                                     "full": ob["refine_full"]}},
         "near_ties_last_step": {"pairs": bf["refine_pairs"], "full": bf["refine_full"]},
         "error_vs_fp64": fp64,
         "mstep": {"partial_kernel_ms": bf["mstep_partial_ms"],
@@ -617,7 +651,7 @@ def main():
         fa = flops / (fp["estep_ms"] * 1e-3) / 1e12
         out["fp32_form"] = {"iters_per_s": 1.0 / fp["sec_per_step"], "ms_per_step": 1e3 * fp["sec_per_step"],
                             "estep_ms": fp["estep_ms"], "estep_tflops": fa, "estep_frac_fp32_peak": fa / PEAK_FP32_TFLOPS,
-                            "bf16_speedup": fp["sec_per_step"] / bf["sec_per_step"],
+                            f"{form.lower()}_speedup": fp["sec_per_step"] / bf["sec_per_step"],
                             "latency_ms": {"p50": fp["p50_ms"], "p99": fp["p99_ms"]}}
     print(json.dumps(out), flush=True)
 

@@ -89,6 +89,8 @@ extern "C" {
 #define MCG_PREC_FP32  0   /* FP32 MFMA cross term, FP32 compare ("plain FP32 form") */
 #define MCG_PREC_BF16  1   /* BF16 MFMA cross term + FP32 re-score of near ties */
 #define MCG_PREC_EXACT 2   /* direct FP32 sum of squared differences per pair */
+#define MCG_PREC_FP16  3   /* FP16 MFMA cross term (power-of-two scaled operands) + FP32
+                              re-score of near ties; same assignment as MCG_PREC_BF16 */
 
 /* coreset sensitivity methods: include/minicore/coreset/coreset.h:34-48 */
 #define MCG_SENS_BFL 0

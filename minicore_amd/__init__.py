@@ -89,7 +89,7 @@ def CoresetSampler(*args, **kwargs):
 
 
 def set_sqrl2_precision(p):
-    """'bf16' (default; exact argmin via near-tie re-scoring), 'fp32' or 'exact'."""
+    """'bf16' (default) or 'fp16' (exact argmin via near-tie re-scoring), 'fp32' or 'exact'."""
     _ext().set_sqrl2_precision(p)
 
 

@@ -14,7 +14,7 @@ LIBPATH = os.path.join(HERE, "libmcg.so")
 
 MCG_F32, MCG_F64, MCG_U8, MCG_U16, MCG_U32, MCG_U64, MCG_I32, MCG_I64 = range(8)
 SQRL2, JSD, MKL, REVERSE_MKL = 2, 4, 5, 11
-PREC_FP32, PREC_BF16, PREC_EXACT = 0, 1, 2
+PREC_FP32, PREC_BF16, PREC_EXACT, PREC_FP16 = 0, 1, 2, 3
 SENS_BFL, SENS_FL, SENS_VX, SENS_LBK = 0, 1, 3, 4
 
 EXPORTS = [

@@ -12,7 +12,13 @@
 //    center's position in its 32-center sub-tile packed into the low 5 mantissa bits
 //    (4 VALU ops per value: v_and_or, two v_med3_u32, v_min_u32).  Points whose
 //    best/second gap is not provably larger than the BF16 error bound get their
-//    candidate centers re-scored exactly (k_collect_bf16, k_exact_cands, k_rescore_full).
+//    candidate centers re-scored exactly (k_collect16, k_exact_cands, k_rescore_full).
+//  * FP16: the BF16 form with FP16 operands, power-of-two scaled into FP16's range
+//    (the rows by xscale, the centers by a per-E-step scale): 3 more mantissa bits,
+//    so an 8x smaller rounding error and far fewer near ties to re-score.  The
+//    rounding error that goes into the tie threshold is measured per row / center
+//    on the operands actually used (below FP16's normal range they are flushed to 0,
+//    so the bound does not depend on how the matrix cores treat denormals).
 //  * FP32: the same GEMM form on v_mfma_f32_32x32x2f32, plain compare/select.
 //  * EXACT: sum of squared differences per pair as a sequential FMA chain, FP32.
 #include "mcg_internal.h"
@@ -26,6 +32,7 @@
 namespace mcg {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ uint16_t f2bf_rne(float f) {
@@ -35,6 +42,28 @@ __device__ __forceinline__ uint16_t f2bf_rne(float f) {
 }
 
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// FP16 bits of an already scaled value: RNE, saturated to +-65504, 0 below the normal
+// range (2^-14); NaN stays NaN
+__device__ __forceinline__ uint16_t f2h_sat(float v) {
+    const float a = fabsf(v);
+    if (a != a) return 0x7E00u;
+    if (a < 0x1p-14f) return 0u;
+    const _Float16 h = (_Float16)copysignf(fminf(a, 65504.f), v);
+    return __builtin_bit_cast(uint16_t, h);
+}
+
+__device__ __forceinline__ float h2f(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
+
+// the 16-bit operand vector and its 32x32x16 MFMA
+template <bool F16> struct Op16 { typedef bf16x8 V; };
+template <> struct Op16<true> { typedef half8 V; };
+template <bool F16>
+__device__ __forceinline__ floatx16 mfma16(const typename Op16<F16>::V &a, const typename Op16<F16>::V &b,
+                                           const floatx16 &c) {
+    if constexpr (F16) return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    else return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
 
 // ---------------------------------------------------------------------------
 // Dataset preparation: column mean mu (FP64 sums), BF16 copy of X - mu padded
@@ -75,36 +104,48 @@ __global__ void k_colsum_final(const double *__restrict__ part, uint32_t nparts,
     if (threadIdx.x == 0) mu[j] = (float)(sm[0] / (double)n);
 }
 
-// one wave per row; lane l converts columns [4l, 4l + 4) of each 256-column pass.
-// xnorm = ||x - mu||, xdev = ||bf16(x - mu) - (x - mu)|| (each difference is exact
-// in FP32), both rounded up.
-__global__ void k_make_xb(const float *__restrict__ X, uint64_t n, uint32_t d, uint32_t dpad,
-                          const float *__restrict__ mu, uint16_t *__restrict__ Xb,
-                          float *__restrict__ xnorm, float *__restrict__ xdev) {
+// one wave per row: xnorm = ||x - mu||, rounded up
+__global__ void k_row_norms(const float *__restrict__ X, uint64_t n, uint32_t d, const float *__restrict__ mu,
+                            float *__restrict__ xnorm) {
     const uint64_t row = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (row >= n) return;
-    double s = 0., e = 0.;
+    double s = 0.;
+    for (uint32_t j = lane; j < d; j += 64) {
+        const float v = X[row * d + j] - mu[j];
+        s += (double)v * (double)v;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) xnorm[row] = (float)(sqrt(s) * (1. + 1e-6));
+}
+
+// one wave per row; lane l converts columns [4l, 4l + 4) of each 256-column pass.
+// F16: the operand is f2h_sat(scale * (x - mu)), else bf16(x - mu) (scale 1);
+// xdev = ||operand / scale - (x - mu)|| (each difference is exact in FP32), rounded up.
+template <bool F16>
+__global__ void k_make_x16(const float *__restrict__ X, uint64_t n, uint32_t d, uint32_t dpad,
+                           const float *__restrict__ mu, float scale, uint16_t *__restrict__ X16,
+                           float *__restrict__ xdev) {
+    const uint64_t row = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const float inv = 1.f / scale;
+    double e = 0.;
     for (uint32_t j0 = lane * 4; j0 < dpad; j0 += 256) {
-        float v[4];
         uint16_t bv[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const uint32_t j = j0 + i;
-            v[i] = j < d ? X[row * d + j] - mu[j] : 0.f;
-            bv[i] = f2bf_rne(v[i]);
-            s += (double)v[i] * (double)v[i];
-            const double dv = (double)bf2f(bv[i]) - (double)v[i];
+            const float v = j < d ? X[row * d + j] - mu[j] : 0.f;
+            bv[i] = F16 ? f2h_sat(v * scale) : f2bf_rne(v);
+            const double dv = (F16 ? (double)h2f(bv[i]) * (double)inv : (double)bf2f(bv[i])) - (double)v;
             e += dv * dv;
         }
         const uint2 packed = make_uint2((uint32_t)bv[0] | ((uint32_t)bv[1] << 16), (uint32_t)bv[2] | ((uint32_t)bv[3] << 16));
-        *(uint2 *)(Xb + row * dpad + j0) = packed;
+        *(uint2 *)(X16 + row * dpad + j0) = packed;
     }
-    for (int o = 32; o > 0; o >>= 1) { s += __shfl_xor(s, o); e += __shfl_xor(e, o); }
-    if (lane == 0) {
-        xnorm[row] = (float)(sqrt(s) * (1. + 1e-6));
-        xdev[row] = (float)(sqrt(e) * (1. + 1e-6));
-    }
+    for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o);
+    if (lane == 0) xdev[row] = (float)(sqrt(e) * (1. + 1e-6));
 }
 
 // max of non-negative floats (their bit patterns order like their values)
@@ -129,13 +170,13 @@ static uint32_t pad_dims(uint64_t d) {
     return 512;
 }
 
-void prepare_bf16(Data &ds) {
-    if (ds.bf16_ready) return;
+void prepare_base(Data &ds) {
+    if (ds.base_ready) return;
     Ctx &c = *ds.ctx;
     hipStream_t st = c.stream;
     const uint64_t n = ds.n;
     const uint32_t d = (uint32_t)ds.d;
-    REQUIRE(d <= 512, "BF16 form supports d <= 512");
+    REQUIRE(d <= 512, "16-bit MFMA forms support d <= 512");
     ds.dpad = pad_dims(d);
     const uint32_t rows_per = 1024;
     const uint32_t nparts = (uint32_t)cdiv(n, rows_per);
@@ -147,18 +188,52 @@ void prepare_bf16(Data &ds) {
     hipLaunchKernelGGL(k_colsum_partial, g1, dim3(256), 0, st, ds.X, n, d, rows_per, part.as<double>());
     hipLaunchKernelGGL(k_colsum_final, dim3(d), dim3(256), 0, st, part.as<double>(), nparts, d, n,
                        ds.mu.as<float>());
-    ds.Xb.reserve((size_t)n * ds.dpad * sizeof(uint16_t));
     ds.xnorm.reserve(n * sizeof(float));
-    ds.xdev.reserve(n * sizeof(float));
-    hipLaunchKernelGGL(k_make_xb, dim3((uint32_t)cdiv(n, 4)), dim3(256), 0, st, ds.X, n, d, ds.dpad,
-                       ds.mu.as<float>(), ds.Xb.as<uint16_t>(), ds.xnorm.as<float>(), ds.xdev.as<float>());
+    hipLaunchKernelGGL(k_row_norms, dim3((uint32_t)cdiv(n, 4)), dim3(256), 0, st, ds.X, n, d, ds.mu.as<const float>(),
+                       ds.xnorm.as<float>());
     unsigned int *mx = (unsigned int *)((char *)part.p + (size_t)nparts * d * sizeof(double));
     fill_u32(mx, 0u, 1, st);
     hipLaunchKernelGGL(k_max_f32, dim3((uint32_t)std::min<uint64_t>(cdiv(n, 256), 2048)), dim3(256), 0, st,
                        ds.xnorm.as<const float>(), n, mx);
     HIP_CHECK(hipMemcpyAsync(&ds.xmax, mx, sizeof(float), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
+    ds.base_ready = true;
+}
+
+void prepare_bf16(Data &ds) {
+    if (ds.bf16_ready) return;
+    prepare_base(ds);
+    Ctx &c = *ds.ctx;
+    hipStream_t st = c.stream;
+    ds.Xb.reserve((size_t)ds.n * ds.dpad * sizeof(uint16_t));
+    ds.xdev.reserve(ds.n * sizeof(float));
+    hipLaunchKernelGGL(k_make_x16<false>, dim3((uint32_t)cdiv(ds.n, 4)), dim3(256), 0, st, ds.X, ds.n, (uint32_t)ds.d,
+                       ds.dpad, ds.mu.as<const float>(), 1.f, ds.Xb.as<uint16_t>(), ds.xdev.as<float>());
+    HIP_CHECK(hipStreamSynchronize(st));
     ds.bf16_ready = true;
+}
+
+// largest power of two s with s * m <= lim (1 for m = 0), within [2^-100, 2^100]
+static double pow2_scale(double m, double lim) {
+    if (!(m > 0.)) return 1.;
+    int e;
+    frexp(lim / m, &e);
+    return ldexp(1., std::min(100, std::max(-100, e - 1)));
+}
+
+void prepare_f16(Data &ds) {
+    if (ds.f16_ready) return;
+    prepare_base(ds);
+    Ctx &c = *ds.ctx;
+    hipStream_t st = c.stream;
+    // every |x_j - mu_j| <= ||x - mu|| <= xmax, so the scaled rows stay below 2^14
+    ds.xscale = (float)pow2_scale(ds.xmax, 16384.);
+    ds.Xh.reserve((size_t)ds.n * ds.dpad * sizeof(uint16_t));
+    ds.xdev_h.reserve(ds.n * sizeof(float));
+    hipLaunchKernelGGL(k_make_x16<true>, dim3((uint32_t)cdiv(ds.n, 4)), dim3(256), 0, st, ds.X, ds.n, (uint32_t)ds.d,
+                       ds.dpad, ds.mu.as<const float>(), ds.xscale, ds.Xh.as<uint16_t>(), ds.xdev_h.as<float>());
+    HIP_CHECK(hipStreamSynchronize(st));
+    ds.f16_ready = true;
 }
 
 // ---------------------------------------------------------------------------
@@ -190,16 +265,70 @@ __global__ void k_prep_centers_bf16(const float *__restrict__ C, uint64_t k, uin
     }
 }
 
+// FP16 form, step 1 (one wave per padded center): cn = ||c - mu||^2 (FP64)
+__global__ void k_center_norms(const float *__restrict__ C, uint64_t k, uint64_t kpad, uint32_t d,
+                               const float *__restrict__ mu, double *__restrict__ cn) {
+    const uint64_t c = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (c >= kpad) return;
+    double s = 0.;
+    for (uint32_t j = lane; j < d; j += 64) {
+        const float v = c < k ? C[c * d + j] - mu[j] : 0.f;
+        s += (double)v * (double)v;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) cn[c] = s;
+}
+
+// FP16 form, step 3 (one wave per padded center, blocks of 4 waves, after
+// k_prep_params): the operand Ch = f2h_sat(-2 csc (c - mu)), cinfo = {||c - mu||,
+// ||Ch / (-2 csc) - (c - mu)||}, both rounded up, and dbmax = max over the real
+// centers of the latter (one atomic per block)
+__global__ __launch_bounds__(256) void k_prep_centers_h(const float *__restrict__ C, uint64_t k, uint64_t kpad,
+                                                        uint32_t d, uint32_t dpad, const float *__restrict__ mu,
+                                                        EParams *__restrict__ prm, uint16_t *__restrict__ Ch,
+                                                        float2 *__restrict__ cinfo) {
+    const uint64_t c = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const bool live = c < kpad;   // every wave reaches the block barrier below
+    const float sc = prm->csc;
+    const double inv = -0.5 / (double)sc;
+    double s = 0., e = 0.;
+    for (uint32_t j = lane; live && j < dpad; j += 64) {
+        const float v = (c < k && j < d) ? C[c * d + j] - mu[j] : 0.f;
+        const uint16_t hv = f2h_sat(-2.f * sc * v);
+        Ch[c * dpad + j] = hv;
+        s += (double)v * (double)v;
+        const double dv = (double)h2f(hv) * inv - (double)v;
+        e += dv * dv;
+    }
+    for (int o = 32; o > 0; o >>= 1) { s += __shfl_xor(s, o); e += __shfl_xor(e, o); }
+    __shared__ float dmax[4];
+    const float dev = (float)(sqrt(e) * (1. + 1e-6));
+    if (lane == 0) {
+        if (live) cinfo[c] = make_float2((float)(sqrt(s) * (1. + 1e-6)), dev);
+        dmax[threadIdx.x >> 6] = c < k ? dev : 0.f;
+    }
+    __syncthreads();
+    // one atomic per block; non-negative floats order like their bit patterns
+    if (threadIdx.x == 0)
+        atomicMax((unsigned int *)&prm->dbmax, __float_as_uint(fmaxf(fmaxf(dmax[0], dmax[1]), fmaxf(dmax[2], dmax[3]))));
+}
+
 // single block: cmax, max center rounding error, offset, near-tie coefficients,
-// cadd[c] = off + cn[c]
+// cadd[c] = S (off + cn[c]).  BF16 (xscale 0): S = 1 and the center errors come from
+// cinfo; FP16: the center scale csc (2 csc cmax <= 2^15, and S times the largest
+// approximate value stays below 2^100), S = xscale * csc, and dbmax is left at 0 for
+// k_prep_centers_h.
 __global__ void k_prep_params(const double *__restrict__ cn, const float2 *__restrict__ cinfo, uint64_t k,
-                              uint64_t kpad, float xmax, uint32_t d, float *__restrict__ cadd,
+                              uint64_t kpad, float xmax, uint32_t d, float xscale, float *__restrict__ cadd,
                               EParams *__restrict__ prm) {
     __shared__ double sm[256], se[256];
+    const bool f16 = xscale > 0.f;
     double m = 0., me = 0.;
     for (uint64_t i = threadIdx.x; i < k; i += blockDim.x) {
         m = fmax(m, cn[i]);
-        me = fmax(me, (double)cinfo[i].y);
+        if (!f16) me = fmax(me, (double)cinfo[i].y);
     }
     sm[threadIdx.x] = m;
     se[threadIdx.x] = me;
@@ -217,20 +346,32 @@ __global__ void k_prep_params(const double *__restrict__ cn, const float2 *__res
     const double vmax = off + cmax * cmax + 2.1 * (double)xmax * cmax;
     const double e_abs = 400. * vmax * 0x1p-24        // accumulation, offset, packing
                          + 4. * 0x1p-24 * ((double)xmax + cmax) * ((double)xmax + cmax);
+    double csc = 1., S = 1.;
+    if (f16) {
+        auto p2 = [](double mm, double lim) {
+            if (!(mm > 0.)) return 1.;
+            int e;
+            frexp(lim / mm, &e);
+            return ldexp(1., min(100, max(-100, e - 1)));
+        };
+        csc = fmin(p2(cmax, 16384.), p2((double)xscale * vmax, 0x1p100));
+        S = (double)xscale * csc;
+    }
     if (threadIdx.x == 0) {
         EParams p;
         p.off = (float)off;
         p.cmax = (float)cmax;
-        p.tau1 = 0.f;
+        p.csc = (float)csc;
         p.tau0 = (float)(2. * e_abs);
         p.taur = (float)(1e-4 + 4. * (double)d * 0x1p-24);
-        p.dbmax = (float)se[0];
-        p.pad[0] = p.pad[1] = 0.f;
+        p.dbmax = f16 ? 0.f : (float)se[0];
+        p.s = (float)S;
+        p.inv_s = (float)(1. / S);
         *prm = p;
     }
     __syncthreads();
     for (uint64_t i = threadIdx.x; i < kpad; i += blockDim.x)
-        cadd[i] = i < k ? (float)(off + cn[i]) : 3.0e38f;
+        cadd[i] = i < k ? (float)(S * (off + cn[i])) : 3.0e38f;
 }
 
 // ---------------------------------------------------------------------------
@@ -248,7 +389,7 @@ __global__ void k_prep_params(const double *__restrict__ cn, const float2 *__res
 //   certain : second - best > tau                 -> asn = best
 //   pair    : third - best > tau >= second - best -> candidates {best, second}
 //   full    : otherwise                           -> listed with threshold best + tau
-// The collect pass (k_collect_bf16) then gathers, for each listed point, every center
+// The collect pass (k_collect16) then gathers, for each listed point, every center
 // whose value is within the threshold; all candidates are scored exactly.
 // ---------------------------------------------------------------------------
 struct Cand {
@@ -333,8 +474,8 @@ __device__ __forceinline__ void cand_overflow(const RefineBufs &R, uint32_t p) {
     if (atomicExch(&R.oflag[p], 1u) == 0u) R.over[atomicAdd(&R.cnt[2], 1u)] = p;
 }
 
-template <int KS, int NB>
-__global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep_bf16(
+template <int KS, int NB, bool F16>
+__global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep16(
     const uint4 *__restrict__ Xb, const uint4 *__restrict__ Cb, const float *__restrict__ cadd,
     const float *__restrict__ xnorm, const float *__restrict__ xdev, const float2 *__restrict__ cinfo,
     const EParams *__restrict__ prm, uint32_t n, uint32_t ntiles, uint32_t kreal, uint32_t *__restrict__ asn,
@@ -353,14 +494,15 @@ __global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep_bf16(
     const int h = lane >> 5, r32 = lane & 31;
     const uint32_t pbase = blockIdx.x * (4u * NB * 32u) + wave * (NB * 32u);
 
-    bf16x8 b[NB][KS];
+    typedef typename Op16<F16>::V V8;
+    V8 b[NB][KS];
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
         uint32_t p = pbase + nb * 32 + r32;
         p = p < n ? p : n - 1;
         const uint4 *row = Xb + (size_t)p * CH + h * KS;
 #pragma unroll
-        for (int s = 0; s < KS; ++s) b[nb][s] = __builtin_bit_cast(bf16x8, row[s]);
+        for (int s = 0; s < KS; ++s) b[nb][s] = __builtin_bit_cast(V8, row[s]);
     }
 
     // Global->LDS copy of tile t into dst through a buffer resource over the center
@@ -453,10 +595,10 @@ __global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep_bf16(
         const char *tb = cur + mb * (32u * CH * 16u);
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
-            const bf16x8 a = *(const bf16x8 *)(tb + (ab ^ (uint32_t)(s * 16)));
+            const V8 a = *(const V8 *)(tb + (ab ^ (uint32_t)(s * 16)));
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb)
-                acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[nb][s], s == 0 ? init : acc[nb], 0, 0, 0);
+                acc[nb] = mfma16<F16>(a, b[nb][s], s == 0 ? init : acc[nb]);
 #pragma unroll
             for (int i = (s * E) / KS; i < ((s + 1) * E) / KS; ++i) epi_elem(pv, i);
         }
@@ -545,13 +687,14 @@ __global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep_bf16(
         pc2[nb] = c2;
         if (mine) {
             const float xn = xnorm[p], dxn = xdev[p];
-            const float dbest = xn * xn + (v1 - P.off);
+            // the values carry the scale P.s (a power of two; 1 for BF16)
+            const float dbest = xn * xn + (v1 * P.inv_s - P.off);
             // Cauchy-Schwarz on the rounding errors of the point (dxn) and of the centers:
             // |x~.c~ - x.c| <= |dx| |c| + |x| |dc| + |dx| |dc|, for the best center and
             // for any other (cmax, dbmax), doubled for the -2 x.c cross term
             const float2 ci = cinfo[c1 < kreal ? c1 : 0];
             const float tcs = 2.f * (dxn * (ci.x + P.cmax) + xn * (ci.y + P.dbmax) + dxn * (ci.y + P.dbmax));
-            const float tau = tcs * (1.f + 1e-3f) + P.tau0 + P.taur * fmaxf(dbest, 0.f);
+            const float tau = (tcs * (1.f + 1e-3f) + P.tau0 + P.taur * fmaxf(dbest, 0.f)) * P.s;
             const bool ok1 = c1 < kreal;
             asn[p] = ok1 ? c1 : 0u;
             if (!(ok1 && v2 - v1 > tau)) {
@@ -591,8 +734,8 @@ __global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep_bf16(
 // 128 listed points (4 waves x 32, the MFMA B operand in registers) over the tiles
 // of their share of the centers (blockIdx.y), streaming the tiles through two LDS
 // buffers with the main pass's DMA; one block-wide append per tile.
-template <int KS>
-__global__ __launch_bounds__(256) void k_collect_bf16(const uint4 *__restrict__ Xb, const uint4 *__restrict__ Cb,
+template <int KS, bool F16>
+__global__ __launch_bounds__(256) void k_collect16(const uint4 *__restrict__ Xb, const uint4 *__restrict__ Cb,
                                                       const float *__restrict__ cadd, uint32_t ntiles,
                                                       uint32_t kreal, RefineBufs R) {
     constexpr int CH = KS * 2;
@@ -624,7 +767,8 @@ __global__ __launch_bounds__(256) void k_collect_bf16(const uint4 *__restrict__ 
     };
     const uint32_t abase0 = (uint32_t)(r32 * (CH * 16) + ((h * KS) ^ swz<CH>(r32)) * 16);
 
-    bf16x8 b[KS];
+    typedef typename Op16<F16>::V V8;
+    V8 b[KS];
     uint32_t p = 0;
     float thr = 0.f;
     bool valid = false;
@@ -647,8 +791,8 @@ __global__ __launch_bounds__(256) void k_collect_bf16(const uint4 *__restrict__ 
             const char *tb = cur + mb * (32u * CH * 16u);
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
-                const bf16x8 a = *(const bf16x8 *)(tb + (abase0 ^ (uint32_t)(s * 16)));
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[s], s == 0 ? init : acc, 0, 0, 0);
+                const V8 a = *(const V8 *)(tb + (abase0 ^ (uint32_t)(s * 16)));
+                acc = mfma16<F16>(a, b[s], s == 0 ? init : acc);
             }
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
@@ -682,7 +826,7 @@ __global__ __launch_bounds__(256) void k_collect_bf16(const uint4 *__restrict__ 
         thr = R.thresh[qq];
         const uint4 *row = Xb + (size_t)p * CH + h * KS;
 #pragma unroll
-        for (int s = 0; s < KS; ++s) b[s] = __builtin_bit_cast(bf16x8, row[s]);
+        for (int s = 0; s < KS; ++s) b[s] = __builtin_bit_cast(V8, row[s]);
         __syncthreads();
         issue_tile(t_begin, lds0);
         if (tid < 64) ladd0[tid] = cadd[t_begin * 64 + tid];
@@ -698,34 +842,71 @@ __global__ __launch_bounds__(256) void k_collect_bf16(const uint4 *__restrict__ 
 
 // Exact cost of each candidate (sequential FMA chain over the dims, the oracle's
 // order), min-reduced per point as (cost bits << 32 | center): lowest cost, then
-// lowest index; the result does not depend on the order of the atomics.  One
-// thread per candidate; rows read straight from HBM / L2 with 16-byte loads when
-// vec (d % 4 == 0 and 16-byte aligned rows), so many candidates are in flight.
+// lowest index; the result does not depend on the order of the atomics.  The first
+// 2 * cnt[4] candidates are the main pass's pairs (a point's two candidates side by
+// side): one thread per pair point reads the row once and runs both chains together;
+// then one thread per remaining candidate.  Rows read straight from HBM / L2 with
+// 16-byte loads when vec (d % 4 == 0 and 16-byte aligned rows).
+__device__ __forceinline__ float exact_sq(const float *__restrict__ x, const float *__restrict__ c, uint32_t d,
+                                          int vec) {
+    float s = 0.f;
+    if (vec) {
+        const float4 *x4 = (const float4 *)x, *c4 = (const float4 *)c;
+        const uint32_t d4 = d >> 2;
+#pragma unroll 8
+        for (uint32_t j = 0; j < d4; ++j) {
+            const float4 a = x4[j], b = c4[j];
+            float t;
+            t = a.x - b.x; s = fmaf(t, t, s);
+            t = a.y - b.y; s = fmaf(t, t, s);
+            t = a.z - b.z; s = fmaf(t, t, s);
+            t = a.w - b.w; s = fmaf(t, t, s);
+        }
+    } else {
+        for (uint32_t j = 0; j < d; ++j) { const float t = x[j] - c[j]; s = fmaf(t, t, s); }
+    }
+    return s;
+}
+
 __global__ __launch_bounds__(256) void k_exact_cands(const float *__restrict__ X, uint32_t d, int vec,
                                                      const float *__restrict__ C, RefineBufs R) {
     const uint32_t cnt = min(R.cnt[0], R.cap);
-    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < cnt; i += gridDim.x * 256u) {
-        const Cand e = R.cand[i];
-        const float *x = X + (uint64_t)e.p * d, *c = C + (uint64_t)e.c * d;
-        float s = 0.f;
-        if (vec) {
-            const float4 *x4 = (const float4 *)x, *c4 = (const float4 *)c;
-            const uint32_t d4 = d >> 2;
-#pragma unroll 8
-            for (uint32_t j = 0; j < d4; ++j) {
-                const float4 a = x4[j], b = c4[j];
-                float t;
-                t = a.x - b.x; s = fmaf(t, t, s);
-                t = a.y - b.y; s = fmaf(t, t, s);
-                t = a.z - b.z; s = fmaf(t, t, s);
-                t = a.w - b.w; s = fmaf(t, t, s);
+    const uint32_t np = min(2u * R.cnt[4], cnt) / 2u;   // pair points stored in full
+    const uint32_t nwork = cnt - np;                      // np pairs + the single candidates
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < nwork; i += gridDim.x * 256u) {
+        if (i < np) {
+            const Cand e0 = R.cand[2 * i], e1 = R.cand[2 * i + 1];
+            const float *x = X + (uint64_t)e0.p * d, *c0 = C + (uint64_t)e0.c * d, *c1 = C + (uint64_t)e1.c * d;
+            float s0 = 0.f, s1 = 0.f;
+            if (vec) {
+                const float4 *x4 = (const float4 *)x, *a4 = (const float4 *)c0, *b4 = (const float4 *)c1;
+                const uint32_t d4 = d >> 2;
+#pragma unroll 4
+                for (uint32_t j = 0; j < d4; ++j) {
+                    const float4 a = x4[j], u = a4[j], v = b4[j];
+                    float t;
+                    t = a.x - u.x; s0 = fmaf(t, t, s0);
+                    t = a.x - v.x; s1 = fmaf(t, t, s1);
+                    t = a.y - u.y; s0 = fmaf(t, t, s0);
+                    t = a.y - v.y; s1 = fmaf(t, t, s1);
+                    t = a.z - u.z; s0 = fmaf(t, t, s0);
+                    t = a.z - v.z; s1 = fmaf(t, t, s1);
+                    t = a.w - u.w; s0 = fmaf(t, t, s0);
+                    t = a.w - v.w; s1 = fmaf(t, t, s1);
+                }
+            } else {
+                s0 = exact_sq(x, c0, d, 0);
+                s1 = exact_sq(x, c1, d, 0);
             }
+            // costs are >= 0 (or NaN), so their bit patterns order like their values
+            const unsigned long long k0 = ((unsigned long long)__float_as_uint(s0) << 32) | e0.c;
+            const unsigned long long k1 = ((unsigned long long)__float_as_uint(s1) << 32) | e1.c;
+            atomicMin(&R.key[e0.p], k0 < k1 ? k0 : k1);
         } else {
-            for (uint32_t j = 0; j < d; ++j) { const float t = x[j] - c[j]; s = fmaf(t, t, s); }
+            const Cand e = R.cand[i + np];
+            const float s = exact_sq(X + (uint64_t)e.p * d, C + (uint64_t)e.c * d, d, vec);
+            atomicMin(&R.key[e.p], ((unsigned long long)__float_as_uint(s) << 32) | e.c);
         }
-        // costs are >= 0 (or NaN), so their bit patterns order like their values
-        const unsigned long long key = ((unsigned long long)__float_as_uint(s) << 32) | e.c;
-        atomicMin(&R.key[e.p], key);
     }
 }
 
@@ -968,23 +1149,25 @@ __global__ void k_copy_counts(const uint32_t *__restrict__ cnt, uint32_t *__rest
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <int KS, int NB>
-static void launch_bf16(Data &ds, Ctx &c, uint64_t kpad, uint64_t kreal, uint32_t *asn, const RefineBufs &R,
+template <int KS, int NB, bool F16>
+static void launch16(Data &ds, Ctx &c, uint64_t kpad, uint64_t kreal, uint32_t *asn, const RefineBufs &R,
                         hipStream_t st) {
     const uint32_t n = (uint32_t)ds.n;
     const uint32_t per = 4u * NB * 32u;
     const dim3 grid((uint32_t)cdiv(n, per));
+    const uint4 *X16 = F16 ? ds.Xh.as<const uint4>() : ds.Xb.as<const uint4>();
+    const float *xdev = F16 ? ds.xdev_h.as<const float>() : ds.xdev.as<const float>();
     hipEvent_t ev;
     c.tic(0, ev);
-    hipLaunchKernelGGL((k_estep_bf16<KS, NB>), grid, dim3(256), 0, st, ds.Xb.as<const uint4>(),
-                       c.s[S_CB].as<const uint4>(), c.s[S_CADD].as<const float>(), ds.xnorm.as<const float>(),
-                       ds.xdev.as<const float>(), c.s[S_CINFO].as<const float2>(), c.s[S_PARAMS].as<const EParams>(),
-                       n, (uint32_t)(kpad / 64), (uint32_t)kreal, asn, R);
+    hipLaunchKernelGGL((k_estep16<KS, NB, F16>), grid, dim3(256), 0, st, X16, c.s[S_CB].as<const uint4>(),
+                       c.s[S_CADD].as<const float>(), ds.xnorm.as<const float>(), xdev,
+                       c.s[S_CINFO].as<const float2>(), c.s[S_PARAMS].as<const EParams>(), n, (uint32_t)(kpad / 64),
+                       (uint32_t)kreal, asn, R);
     c.toc(0, ev);
     // statistics: {pair points, listed points} before the collect pass adds candidates
     hipLaunchKernelGGL(k_copy_counts, dim3(1), dim3(1), 0, st, R.cnt, R.cnt + 4);
     // collect pass over the listed points
-    hipLaunchKernelGGL((k_collect_bf16<KS>), dim3(512, kpad >= 128 ? 2 : 1), dim3(256), 0, st, ds.Xb.as<const uint4>(),
+    hipLaunchKernelGGL((k_collect16<KS, F16>), dim3(512, kpad >= 128 ? 2 : 1), dim3(256), 0, st, X16,
                        c.s[S_CB].as<const uint4>(), c.s[S_CADD].as<const float>(), (uint32_t)(kpad / 64),
                        (uint32_t)kreal, R);
 }
@@ -1026,16 +1209,17 @@ static void dense_sqrl2_assign_mfma(Data &ds, const float *C, uint64_t k, int pr
 static std::mutex g_tune_mu;
 static std::vector<std::pair<uint64_t, int>> g_tuned;
 
-static int tuned_form(Data &ds, const float *C, uint64_t k, uint32_t *asn, hipStream_t st) {
+static int tuned_form(Data &ds, const float *C, uint64_t k, uint32_t *asn, hipStream_t st, int form16) {
     static const bool tune_on = [] { const char *e = getenv("MCG_TUNE"); return !(e && e[0] == '0'); }();
-    if (!tune_on || !ds.ctx->tuning) return MCG_PREC_BF16;
+    if (!tune_on || !ds.ctx->tuning) return form16;
     const double work = (double)ds.n * (double)k * (double)ds.d;
-    if (work >= 0x1p26) return MCG_PREC_BF16;
+    if (work >= 0x1p26) return form16;
     if (work <= 0x1p21) return MCG_PREC_EXACT;
-    prepare_bf16(ds);
+    prepare_base(ds);
     uint64_t lg = 0;
     while ((1ull << lg) < ds.n) ++lg;
-    const uint64_t key = ((uint64_t)(ds.ctx->device & 0xFF) << 56) | ((uint64_t)ds.dpad << 40) | (cdiv(k, 64) << 8) | lg;
+    const uint64_t key = ((uint64_t)(ds.ctx->device & 0xFF) << 56) | ((uint64_t)form16 << 52) |
+                         ((uint64_t)ds.dpad << 40) | (cdiv(k, 64) << 8) | lg;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         for (const auto &kv : g_tuned)
@@ -1050,7 +1234,7 @@ static int tuned_form(Data &ds, const float *C, uint64_t k, uint32_t *asn, hipSt
         for (int rep = 0; rep < 2; ++rep) {
             HIP_CHECK(hipEventRecord(e0, st));
             if (form == MCG_PREC_EXACT) dense_sqrl2_assign_exact(ds, C, k, asn, nullptr, st);
-            else dense_sqrl2_assign_mfma(ds, C, k, MCG_PREC_BF16, asn, st);
+            else dense_sqrl2_assign_mfma(ds, C, k, form16, asn, st);
             HIP_CHECK(hipEventRecord(e1, st));
             HIP_CHECK(hipEventSynchronize(e1));
             float ms = 0.f;
@@ -1059,10 +1243,10 @@ static int tuned_form(Data &ds, const float *C, uint64_t k, uint32_t *asn, hipSt
         }
         return best;
     };
-    const float t_mfma = time_form(MCG_PREC_BF16), t_exact = time_form(MCG_PREC_EXACT);
+    const float t_mfma = time_form(form16), t_exact = time_form(MCG_PREC_EXACT);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    const int form = t_exact < t_mfma ? MCG_PREC_EXACT : MCG_PREC_BF16;
+    const int form = t_exact < t_mfma ? MCG_PREC_EXACT : form16;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tuned.push_back({key, form});
     return form;
@@ -1071,7 +1255,8 @@ static int tuned_form(Data &ds, const float *C, uint64_t k, uint32_t *asn, hipSt
 void dense_sqrl2_assign(Data &ds, const float *C, uint64_t k, int precision, uint32_t *asn,
                         hipStream_t st) {
     REQUIRE(ds.n < (1ull << 31), "dense rows must be < 2^31");
-    if (precision == MCG_PREC_BF16 && ds.d <= 512) precision = tuned_form(ds, C, k, asn, st);
+    if ((precision == MCG_PREC_BF16 || precision == MCG_PREC_FP16) && ds.d <= 512)
+        precision = tuned_form(ds, C, k, asn, st, precision);
     if (precision == MCG_PREC_EXACT || ds.d > 512 || (precision == MCG_PREC_FP32 && ds.d > 256)) {
         dense_sqrl2_assign_exact(ds, C, k, asn, nullptr, st);
         return;
@@ -1082,7 +1267,12 @@ void dense_sqrl2_assign(Data &ds, const float *C, uint64_t k, int precision, uin
 static void dense_sqrl2_assign_mfma(Data &ds, const float *C, uint64_t k, int precision, uint32_t *asn,
                                     hipStream_t st) {
     Ctx &c = *ds.ctx;
-    prepare_bf16(ds);  // mu, dpad (and the BF16 copy) are shared by both MFMA forms
+    const bool f16 = precision == MCG_PREC_FP16;
+    prepare_base(ds);  // mu, dpad are shared by the MFMA forms
+    if (precision != MCG_PREC_FP32) {
+        if (f16) prepare_f16(ds);
+        else prepare_bf16(ds);
+    }
     const uint32_t d = (uint32_t)ds.d, dpad = ds.dpad;
     if (precision == MCG_PREC_FP32) {
         const uint64_t kpad = cdiv(k, 32) * 32;
@@ -1121,18 +1311,38 @@ static void dense_sqrl2_assign_mfma(Data &ds, const float *C, uint64_t k, int pr
         if (c.s[S_OFLAG].bytes != before) fill_u32(R.oflag, 0u, c.s[S_OFLAG].bytes / 4, st);
     }
     c.s[S_CINFO].reserve(kpad * sizeof(float2));
-    hipLaunchKernelGGL(k_prep_centers_bf16, dim3((uint32_t)cdiv(kpad, 4)), dim3(256), 0, st, C, k, kpad, d, dpad,
-                       ds.mu.as<const float>(), c.s[S_CB].as<uint16_t>(), c.s[S_CNORM].as<double>(),
-                       c.s[S_CINFO].as<float2>());
-    hipLaunchKernelGGL(k_prep_params, dim3(1), dim3(256), 0, st, c.s[S_CNORM].as<const double>(),
-                       c.s[S_CINFO].as<const float2>(), k, kpad, ds.xmax, d, c.s[S_CADD].as<float>(),
-                       c.s[S_PARAMS].as<EParams>());
+    if (f16) {
+        hipLaunchKernelGGL(k_center_norms, dim3((uint32_t)cdiv(kpad, 4)), dim3(256), 0, st, C, k, kpad, d,
+                           ds.mu.as<const float>(), c.s[S_CNORM].as<double>());
+        hipLaunchKernelGGL(k_prep_params, dim3(1), dim3(256), 0, st, c.s[S_CNORM].as<const double>(),
+                           c.s[S_CINFO].as<const float2>(), k, kpad, ds.xmax, d, ds.xscale, c.s[S_CADD].as<float>(),
+                           c.s[S_PARAMS].as<EParams>());
+        hipLaunchKernelGGL(k_prep_centers_h, dim3((uint32_t)cdiv(kpad, 4)), dim3(256), 0, st, C, k, kpad, d, dpad,
+                           ds.mu.as<const float>(), c.s[S_PARAMS].as<EParams>(), c.s[S_CB].as<uint16_t>(),
+                           c.s[S_CINFO].as<float2>());
+    } else {
+        hipLaunchKernelGGL(k_prep_centers_bf16, dim3((uint32_t)cdiv(kpad, 4)), dim3(256), 0, st, C, k, kpad, d, dpad,
+                           ds.mu.as<const float>(), c.s[S_CB].as<uint16_t>(), c.s[S_CNORM].as<double>(),
+                           c.s[S_CINFO].as<float2>());
+        hipLaunchKernelGGL(k_prep_params, dim3(1), dim3(256), 0, st, c.s[S_CNORM].as<const double>(),
+                           c.s[S_CINFO].as<const float2>(), k, kpad, ds.xmax, d, 0.f, c.s[S_CADD].as<float>(),
+                           c.s[S_PARAMS].as<EParams>());
+    }
     fill_u32(R.cnt, 0u, 4, st);
-    switch (dpad) {
-    case 64: launch_bf16<4, 2>(ds, c, kpad, k, asn, R, st); break;
-    case 128: launch_bf16<8, 2>(ds, c, kpad, k, asn, R, st); break;
-    case 256: launch_bf16<16, 2>(ds, c, kpad, k, asn, R, st); break;
-    default: launch_bf16<32, 1>(ds, c, kpad, k, asn, R, st); break;
+    if (f16) {
+        switch (dpad) {
+        case 64: launch16<4, 2, true>(ds, c, kpad, k, asn, R, st); break;
+        case 128: launch16<8, 2, true>(ds, c, kpad, k, asn, R, st); break;
+        case 256: launch16<16, 2, true>(ds, c, kpad, k, asn, R, st); break;
+        default: launch16<32, 1, true>(ds, c, kpad, k, asn, R, st); break;
+        }
+    } else {
+        switch (dpad) {
+        case 64: launch16<4, 2, false>(ds, c, kpad, k, asn, R, st); break;
+        case 128: launch16<8, 2, false>(ds, c, kpad, k, asn, R, st); break;
+        case 256: launch16<16, 2, false>(ds, c, kpad, k, asn, R, st); break;
+        default: launch16<32, 1, false>(ds, c, kpad, k, asn, R, st); break;
+        }
     }
     {
         const int vec = (d % 4 == 0) && ((uintptr_t)ds.X % 16 == 0) && ((uintptr_t)C % 16 == 0);

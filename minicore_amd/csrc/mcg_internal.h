@@ -108,12 +108,15 @@ struct Data {
     DBuf val, idx, ptr;
     // per-row FP64 sums (csc.h:1008-1016 / blaze sum<rowwise>)
     DBuf rowsum;
-    // BF16 copy for the MFMA cross term: rows centered on the column mean mu,
-    // padded to dpad columns; ||x - mu|| per row; max over rows.
-    bool bf16_ready = false;
+    // 16-bit copies (BF16, FP16) for the MFMA cross term: rows centered on the column
+    // mean mu, padded to dpad columns; ||x - mu|| per row; max over rows.
+    bool base_ready = false, bf16_ready = false, f16_ready = false;
     uint32_t dpad = 0;
     DBuf Xb, mu, xnorm;
     DBuf xdev;         // ||bf16(x - mu) - (x - mu)|| per row (rounded up)
+    DBuf Xh;           // FP16 copy of xscale * (x - mu), padded to dpad columns
+    DBuf xdev_h;       // ||Xh / xscale - (x - mu)|| per row (rounded up)
+    float xscale = 1.f;
     float xmax = 0.f;
     // per-prior CSR transforms (cached for the last prior seen)
     double tf_prior = -1.;
@@ -171,14 +174,17 @@ static_assert(S_NSLOTS <= 56, "scratch slots");
 struct EParams {
     float off;      // added to every approximate value to keep it positive
     float cmax;     // max_c ||c - mu||
-    float tau1;     // (unused; kept for layout)
+    float csc;      // FP16 form: power-of-two scale of the center operand (1 for BF16)
     float tau0;     // absolute part of the tie threshold
     float taur;     // relative part (of the approximate best distance)
-    float dbmax;    // max_c ||bf16(c - mu) - (c - mu)||
-    float pad[2];
+    float dbmax;    // max_c ||c~ - (c - mu)||, c~ the 16-bit center operand (unscaled)
+    float s;        // scale of the MFMA values: x scale * center scale (1 for BF16)
+    float inv_s;    // 1 / s (exact: powers of two)
 };
 
-void prepare_bf16(Data &ds);  // builds Xb, mu, xnorm, xmax (once per dataset)
+void prepare_base(Data &ds);  // mu, xnorm, xmax, dpad (once per dataset)
+void prepare_bf16(Data &ds);  // + Xb, xdev (once per dataset)
+void prepare_f16(Data &ds);   // + Xh, xdev_h, xscale (once per dataset)
 // Dense SQRL2 hard assignment into asn (device), by precision.
 void dense_sqrl2_assign(Data &ds, const float *C, uint64_t k, int precision, uint32_t *asn,
                         hipStream_t st);
@@ -315,7 +321,9 @@ void rows_into(const Data &S, const uint64_t *idx_h, const uint64_t *idx_d, uint
 
 // Drop the data derived from a dataset's rows (rebuilt on next use).
 inline void reset_derived(Data &D) {
+    D.base_ready = false;
     D.bf16_ready = false;
+    D.f16_ready = false;
     D.rtiles.ready = false;
     D.rtiles.cpl = 0;
     D.ctiles.ready = false;

@@ -1144,8 +1144,10 @@ PYBIND11_MODULE(_pyminicore, m) {
 
     m.def("set_sqrl2_precision", [](const std::string &p) {
         if (p == "bf16") g_sqrl2_precision = MCG_PREC_BF16;
+        else if (p == "fp16") g_sqrl2_precision = MCG_PREC_FP16;
         else if (p == "fp32") g_sqrl2_precision = MCG_PREC_FP32;
         else if (p == "exact") g_sqrl2_precision = MCG_PREC_EXACT;
-        else throw std::invalid_argument("precision must be bf16, fp32 or exact");
-    }, py::arg("precision"), "Cross-term form for dense squared-L2 clustering (default bf16, exact argmin)");
+        else throw std::invalid_argument("precision must be bf16, fp16, fp32 or exact");
+    }, py::arg("precision"),
+          "Cross-term form for dense squared-L2 clustering (default bf16; bf16 and fp16 give the exact argmin)");
 }

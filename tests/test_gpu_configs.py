@@ -18,9 +18,10 @@ def say(*a):
 
 def test_dense_headline_shape_assign_and_lloyd_step(mcg, ctx, mco):
     """32,768 rows of the headline generator (seed 2) with its k = 1024 starting
-    centers: BF16-refined and exact assignments bit-identical to the oracle's FP32
-    argmin; three mcg_lloyd_step calls (the benchmark's timed unit) equal the
-    oracle's E+M passes: assignment and centers bit for bit, cost within 1e-12."""
+    centers: BF16- and FP16-refined and exact assignments bit-identical to the
+    oracle's FP32 argmin; three mcg_lloyd_step calls (the benchmark's timed unit) in
+    each 16-bit form equal the oracle's E+M passes: assignment and centers bit for
+    bit, cost within 1e-12."""
     import torch
     say("headline shape: generating 1e6 x 256")
     n, d, k = 1_000_000, 256, 1024
@@ -33,24 +34,25 @@ def test_dense_headline_shape_assign_and_lloyd_step(mcg, ctx, mco):
     Xh, Ch = Xs.cpu().numpy(), C0.cpu().numpy()
     a_ref, c_ref, _ = mco.assign_sqrl2_f32(Xh, Ch)
     ds = mcg.Dataset.dense(Xs, ctx)
-    for prec in (mcg.PREC_BF16, mcg.PREC_EXACT):
+    for prec in (mcg.PREC_BF16, mcg.PREC_FP16, mcg.PREC_EXACT):
         say(f"headline shape: assign precision {prec}")
         a, c = ds.assign(mcg.SQRL2, Ch, precision=prec)
         assert np.array_equal(a, a_ref), (prec, np.nonzero(a != a_ref)[0][:10])
         assert np.array_equal(c, c_ref), prec
-    Ct = C0.clone()
-    at = torch.empty(32768, dtype=torch.int32, device="cuda")
-    Cr = Ch.copy()
-    reseeds = 0
-    for step in range(3):
-        say(f"headline shape: lloyd_step {step}")
-        cost = ds.lloyd_step(mcg.SQRL2, Ct, at, precision=mcg.PREC_BF16)
-        Cr, ar, cr, draws = mco.lloyd_step(Xh, Cr)
-        reseeds += draws
-        assert np.array_equal(at.cpu().numpy().astype(np.uint32), ar), step
-        assert np.array_equal(Ct.cpu().numpy(), Cr), step
-        assert cost == pytest.approx(cr, rel=1e-12), step
-    say(f"headline shape: {reseeds} empty-cluster reseeds in 3 steps")
+    for prec in (mcg.PREC_BF16, mcg.PREC_FP16):
+        Ct = C0.clone()
+        at = torch.empty(32768, dtype=torch.int32, device="cuda")
+        Cr = Ch.copy()
+        reseeds = 0
+        for step in range(3):
+            say(f"headline shape: lloyd_step {step} precision {prec}")
+            cost = ds.lloyd_step(mcg.SQRL2, Ct, at, precision=prec)
+            Cr, ar, cr, draws = mco.lloyd_step(Xh, Cr)
+            reseeds += draws
+            assert np.array_equal(at.cpu().numpy().astype(np.uint32), ar), step
+            assert np.array_equal(Ct.cpu().numpy(), Cr), step
+            assert cost == pytest.approx(cr, rel=1e-12), step
+        say(f"headline shape: {reseeds} empty-cluster reseeds in 3 steps")
     ds.close()
 
 

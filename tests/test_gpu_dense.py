@@ -23,12 +23,12 @@ CASES = [(20000, 50, 25, 0), (3000, 17, 3, 1), (4000, 64, 64, 2), (5000, 100, 65
 
 
 @pytest.mark.parametrize("n,d,k,seed", CASES)
-def test_assign_exact_and_bf16_bitexact(mcg, ctx, mco, n, d, k, seed):
+def test_assign_exact_and_16bit_bitexact(mcg, ctx, mco, n, d, k, seed):
     say(f"assign n={n} d={d} k={k} seed={seed}")
     X, C = mixture(n, d, k, seed)
     a_ref, c_ref, gap = mco.assign_sqrl2_f32(X, C)
     ds = mcg.Dataset.dense(X, ctx)
-    for prec in (mcg.PREC_EXACT, mcg.PREC_BF16):
+    for prec in (mcg.PREC_EXACT, mcg.PREC_BF16, mcg.PREC_FP16):
         a, c = ds.assign(mcg.SQRL2, C, precision=prec)
         assert np.array_equal(a, a_ref), (prec, np.nonzero(a != a_ref)[0][:10])
         assert np.array_equal(c, c_ref), prec
@@ -39,8 +39,9 @@ def test_assign_exact_and_bf16_bitexact(mcg, ctx, mco, n, d, k, seed):
     ds.close()
 
 
+@pytest.mark.parametrize("form", ["BF16", "FP16"])
 @pytest.mark.parametrize("n,d,k,seed", [(500, 20, 8, 11), (3000, 64, 40, 12), (20000, 128, 96, 13)])
-def test_tuned_form_bitexact(mcg, mco, n, d, k, seed):
+def test_tuned_form_bitexact(mcg, mco, n, d, k, seed, form):
     """With the E-step tuning on, whichever kernel the timing picks (exact for the
     smallest shape, measured for the others) gives the oracle's assignment; the
     second call reuses the cached choice."""
@@ -51,7 +52,7 @@ def test_tuned_form_bitexact(mcg, mco, n, d, k, seed):
     try:
         ds = mcg.Dataset.dense(X, tc)
         for _ in range(2):
-            a, c = ds.assign(mcg.SQRL2, C, precision=mcg.PREC_BF16)
+            a, c = ds.assign(mcg.SQRL2, C, precision=getattr(mcg, "PREC_" + form))
             assert np.array_equal(a, a_ref)
             assert np.array_equal(c, c_ref)
         ds.close()
@@ -59,10 +60,11 @@ def test_tuned_form_bitexact(mcg, mco, n, d, k, seed):
         tc.close()
 
 
-def test_bf16_near_ties_are_rescored(mcg, ctx, mco):
+@pytest.mark.parametrize("form", ["BF16", "FP16"])
+def test_16bit_near_ties_are_rescored(mcg, ctx, mco, form):
     """Points equidistant (in exact arithmetic) from two centers, or sitting on three
     duplicated centers: both refinement paths run and the result stays exact."""
-    say("near ties")
+    say(f"near ties {form}")
     rng = np.random.default_rng(0)
     d, k = 128, 40
     C = rng.uniform(-5, 5, (k, d)).astype(np.float32)
@@ -79,7 +81,7 @@ def test_bf16_near_ties_are_rescored(mcg, ctx, mco):
     X = np.array(pts, dtype=np.float32)
     a_ref, c_ref, _ = mco.assign_sqrl2_f32(X, C)
     ds = mcg.Dataset.dense(X, ctx)
-    a, c = ds.assign(mcg.SQRL2, C, precision=mcg.PREC_BF16)
+    a, c = ds.assign(mcg.SQRL2, C, precision=getattr(mcg, "PREC_" + form))
     pairs, full = ctx.refine_counts()
     say(f"refined pairs={pairs} full={full}")
     assert pairs > 0 and full > 0
@@ -88,10 +90,12 @@ def test_bf16_near_ties_are_rescored(mcg, ctx, mco):
     assert not np.any(a == 5) and not np.any(a == 9) and not np.any(a == 10)
 
 
-def test_bf16_candidate_overflow(mcg, ctx, mco):
+@pytest.mark.parametrize("form", ["BF16", "FP16"])
+def test_16bit_candidate_overflow(mcg, ctx, mco, form):
     """40 identical centers give every point 40 exact-tie candidates, more than the
     candidate buffer holds (8 per point): the overflow path re-scores over all centers."""
-    say("candidate overflow")
+    say(f"candidate overflow {form}")
+    prec = getattr(mcg, "PREC_" + form)
     rng = np.random.default_rng(1)
     d = 64
     base = rng.uniform(-3, 3, d).astype(np.float32)
@@ -99,20 +103,40 @@ def test_bf16_candidate_overflow(mcg, ctx, mco):
     X = (base + 0.5 * rng.standard_normal((2000, d))).astype(np.float32)
     a_ref, c_ref, _ = mco.assign_sqrl2_f32(X, C)
     ds = mcg.Dataset.dense(X, ctx)
-    a, c = ds.assign(mcg.SQRL2, C, precision=mcg.PREC_BF16)
+    a, c = ds.assign(mcg.SQRL2, C, precision=prec)
     assert np.array_equal(a, a_ref) and np.array_equal(c, c_ref)
-    a2, _ = ds.assign(mcg.SQRL2, C, precision=mcg.PREC_BF16)   # flags were reset
+    a2, _ = ds.assign(mcg.SQRL2, C, precision=prec)   # flags were reset
     assert np.array_equal(a2, a_ref)
 
 
-def test_far_from_origin(mcg, ctx, mco):
-    """Data offset far from the origin: centering keeps the BF16 cross term usable."""
-    say("offset data")
+@pytest.mark.parametrize("form", ["BF16", "FP16"])
+def test_far_from_origin(mcg, ctx, mco, form):
+    """Data offset far from the origin: centering keeps the 16-bit cross term usable."""
+    say(f"offset data {form}")
     X, C = mixture(4000, 64, 16, 9, lo=1000., hi=1010.)
     a_ref, c_ref, _ = mco.assign_sqrl2_f32(X, C)
     ds = mcg.Dataset.dense(X, ctx)
-    a, c = ds.assign(mcg.SQRL2, C, precision=mcg.PREC_BF16)
+    a, c = ds.assign(mcg.SQRL2, C, precision=getattr(mcg, "PREC_" + form))
     assert np.array_equal(a, a_ref)
+    assert np.array_equal(c, c_ref)
+
+
+@pytest.mark.parametrize("scale", [1e-12, 1e-3, 1e7])
+def test_fp16_scaling(mcg, ctx, mco, scale):
+    """FP16 operands are power-of-two scaled into FP16's range: tiny and huge data
+    give the oracle's assignment, also with centers far outside the data (whose
+    operands saturate) and a center at the data's mean (operand rows of zeros)."""
+    say(f"fp16 scaling {scale}")
+    X, C = mixture(3000, 48, 12, 21)
+    X = (X * scale).astype(np.float32)
+    C = (C * scale).astype(np.float32)
+    C[0] = X.mean(0)
+    C[1] = C[1] * 1e4
+    C[2] = -C[2] * 3e5
+    a_ref, c_ref, _ = mco.assign_sqrl2_f32(X, C)
+    ds = mcg.Dataset.dense(X, ctx)
+    a, c = ds.assign(mcg.SQRL2, C, precision=mcg.PREC_FP16)
+    assert np.array_equal(a, a_ref), np.nonzero(a != a_ref)[0][:10]
     assert np.array_equal(c, c_ref)
 
 

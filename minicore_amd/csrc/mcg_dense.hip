@@ -55,6 +55,14 @@ __device__ __forceinline__ uint16_t f2h_sat(float v) {
 
 __device__ __forceinline__ float h2f(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
 
+// largest power of two s with s * m <= lim (1 for m = 0), within [2^-100, 2^100]
+__device__ __forceinline__ double pow2_scale_d(double m, double lim) {
+    if (!(m > 0.)) return 1.;
+    int e;
+    frexp(lim / m, &e);
+    return ldexp(1., min(100, max(-100, e - 1)));
+}
+
 // the 16-bit operand vector and its 32x32x16 MFMA
 template <bool F16> struct Op16 { typedef bf16x8 V; };
 template <> struct Op16<true> { typedef half8 V; };
@@ -124,11 +132,12 @@ __global__ void k_row_norms(const float *__restrict__ X, uint64_t n, uint32_t d,
 // xdev = ||operand / scale - (x - mu)|| (each difference is exact in FP32), rounded up.
 template <bool F16>
 __global__ void k_make_x16(const float *__restrict__ X, uint64_t n, uint32_t d, uint32_t dpad,
-                           const float *__restrict__ mu, float scale, uint16_t *__restrict__ X16,
+                           const float *__restrict__ mu, const float *__restrict__ scale_p, uint16_t *__restrict__ X16,
                            float *__restrict__ xdev) {
     const uint64_t row = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (row >= n) return;
+    const float scale = F16 ? *scale_p : 1.f;
     const float inv = 1.f / scale;
     double e = 0.;
     for (uint32_t j0 = lane * 4; j0 < dpad; j0 += 256) {
@@ -191,12 +200,11 @@ void prepare_base(Data &ds) {
     ds.xnorm.reserve(n * sizeof(float));
     hipLaunchKernelGGL(k_row_norms, dim3((uint32_t)cdiv(n, 4)), dim3(256), 0, st, ds.X, n, d, ds.mu.as<const float>(),
                        ds.xnorm.as<float>());
-    unsigned int *mx = (unsigned int *)((char *)part.p + (size_t)nparts * d * sizeof(double));
-    fill_u32(mx, 0u, 1, st);
+    // xmax stays on the device (no host round trip): the center preparation reads it
+    ds.xstat.reserve(2 * sizeof(float));
+    fill_u32(ds.xstat.as<uint32_t>(), 0u, 2, st);
     hipLaunchKernelGGL(k_max_f32, dim3((uint32_t)std::min<uint64_t>(cdiv(n, 256), 2048)), dim3(256), 0, st,
-                       ds.xnorm.as<const float>(), n, mx);
-    HIP_CHECK(hipMemcpyAsync(&ds.xmax, mx, sizeof(float), hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipStreamSynchronize(st));
+                       ds.xnorm.as<const float>(), n, ds.xstat.as<unsigned int>());
     ds.base_ready = true;
 }
 
@@ -208,31 +216,26 @@ void prepare_bf16(Data &ds) {
     ds.Xb.reserve((size_t)ds.n * ds.dpad * sizeof(uint16_t));
     ds.xdev.reserve(ds.n * sizeof(float));
     hipLaunchKernelGGL(k_make_x16<false>, dim3((uint32_t)cdiv(ds.n, 4)), dim3(256), 0, st, ds.X, ds.n, (uint32_t)ds.d,
-                       ds.dpad, ds.mu.as<const float>(), 1.f, ds.Xb.as<uint16_t>(), ds.xdev.as<float>());
-    HIP_CHECK(hipStreamSynchronize(st));
+                       ds.dpad, ds.mu.as<const float>(), (const float *)nullptr, ds.Xb.as<uint16_t>(),
+                       ds.xdev.as<float>());
     ds.bf16_ready = true;
 }
 
-// largest power of two s with s * m <= lim (1 for m = 0), within [2^-100, 2^100]
-static double pow2_scale(double m, double lim) {
-    if (!(m > 0.)) return 1.;
-    int e;
-    frexp(lim / m, &e);
-    return ldexp(1., std::min(100, std::max(-100, e - 1)));
-}
+// xstat[1] = the largest power of two s with s * xmax <= 2^14: every |x_j - mu_j| <=
+// ||x - mu|| <= xmax, so the scaled rows stay inside FP16's range
+__global__ void k_xscale(float *__restrict__ xstat) { xstat[1] = (float)pow2_scale_d(xstat[0], 16384.); }
 
 void prepare_f16(Data &ds) {
     if (ds.f16_ready) return;
     prepare_base(ds);
     Ctx &c = *ds.ctx;
     hipStream_t st = c.stream;
-    // every |x_j - mu_j| <= ||x - mu|| <= xmax, so the scaled rows stay below 2^14
-    ds.xscale = (float)pow2_scale(ds.xmax, 16384.);
+    hipLaunchKernelGGL(k_xscale, dim3(1), dim3(1), 0, st, ds.xstat.as<float>());
     ds.Xh.reserve((size_t)ds.n * ds.dpad * sizeof(uint16_t));
     ds.xdev_h.reserve(ds.n * sizeof(float));
     hipLaunchKernelGGL(k_make_x16<true>, dim3((uint32_t)cdiv(ds.n, 4)), dim3(256), 0, st, ds.X, ds.n, (uint32_t)ds.d,
-                       ds.dpad, ds.mu.as<const float>(), ds.xscale, ds.Xh.as<uint16_t>(), ds.xdev_h.as<float>());
-    HIP_CHECK(hipStreamSynchronize(st));
+                       ds.dpad, ds.mu.as<const float>(), ds.xstat.as<const float>() + 1, ds.Xh.as<uint16_t>(),
+                       ds.xdev_h.as<float>());
     ds.f16_ready = true;
 }
 
@@ -316,15 +319,15 @@ __global__ __launch_bounds__(256) void k_prep_centers_h(const float *__restrict_
 }
 
 // single block: cmax, max center rounding error, offset, near-tie coefficients,
-// cadd[c] = S (off + cn[c]).  BF16 (xscale 0): S = 1 and the center errors come from
+// cadd[c] = S (off + cn[c]).  BF16 (f16 = 0): S = 1 and the center errors come from
 // cinfo; FP16: the center scale csc (2 csc cmax <= 2^15, and S times the largest
 // approximate value stays below 2^100), S = xscale * csc, and dbmax is left at 0 for
 // k_prep_centers_h.
 __global__ void k_prep_params(const double *__restrict__ cn, const float2 *__restrict__ cinfo, uint64_t k,
-                              uint64_t kpad, float xmax, uint32_t d, float xscale, float *__restrict__ cadd,
-                              EParams *__restrict__ prm) {
+                              uint64_t kpad, const float *__restrict__ xstat, uint32_t d, int f16,
+                              float *__restrict__ cadd, EParams *__restrict__ prm) {
     __shared__ double sm[256], se[256];
-    const bool f16 = xscale > 0.f;
+    const float xmax = xstat[0], xscale = f16 ? xstat[1] : 1.f;
     double m = 0., me = 0.;
     for (uint64_t i = threadIdx.x; i < k; i += blockDim.x) {
         m = fmax(m, cn[i]);
@@ -348,13 +351,7 @@ __global__ void k_prep_params(const double *__restrict__ cn, const float2 *__res
                          + 4. * 0x1p-24 * ((double)xmax + cmax) * ((double)xmax + cmax);
     double csc = 1., S = 1.;
     if (f16) {
-        auto p2 = [](double mm, double lim) {
-            if (!(mm > 0.)) return 1.;
-            int e;
-            frexp(lim / mm, &e);
-            return ldexp(1., min(100, max(-100, e - 1)));
-        };
-        csc = fmin(p2(cmax, 16384.), p2((double)xscale * vmax, 0x1p100));
+        csc = fmin(pow2_scale_d(cmax, 16384.), pow2_scale_d((double)xscale * vmax, 0x1p100));
         S = (double)xscale * csc;
     }
     if (threadIdx.x == 0) {
@@ -1200,11 +1197,12 @@ void dense_sqrl2_costs(Data &ds, const float *C, const uint32_t *asn, float *cos
 static void dense_sqrl2_assign_mfma(Data &ds, const float *C, uint64_t k, int precision, uint32_t *asn,
                                     hipStream_t st);
 
-// Tuning of the BF16-refined form: it and the exact kernel give the same assignment
-// bit for bit, so the faster one is used.  Clear cases are decided by the work
-// n*k*d (measured: the MFMA form is 1.5x faster at 2^24 and 4x at 2^28.5); in
-// between, both are timed once per device and (padded d, padded k, log2 n), min of
-// two runs each, and the choice is kept for every context of the process.
+// Tuning of the 16-bit refined forms: they and the exact kernel give the same
+// assignment bit for bit, so the faster one is used.  Clear cases are decided by the
+// work n*k*d (measured: the MFMA form is 1.5x faster at 2^24 and 4x at 2^28.5); in
+// between, both are timed once per device and (form, padded d, padded k, log2 n), min
+// of two runs each, and the choice is kept for every context of the process; server
+// workers take the crossover at 2^23 instead of timing inside a request.
 // mcg_tuning_enable(ctx, 0) or MCG_TUNE=0 always takes the MFMA form.
 static std::mutex g_tune_mu;
 static std::vector<std::pair<uint64_t, int>> g_tuned;
@@ -1215,6 +1213,7 @@ static int tuned_form(Data &ds, const float *C, uint64_t k, uint32_t *asn, hipSt
     const double work = (double)ds.n * (double)k * (double)ds.d;
     if (work >= 0x1p26) return form16;
     if (work <= 0x1p21) return MCG_PREC_EXACT;
+    if (ds.ctx->serving) return work < 0x1p23 ? MCG_PREC_EXACT : form16;   // the measured crossover
     prepare_base(ds);
     uint64_t lg = 0;
     while ((1ull << lg) < ds.n) ++lg;
@@ -1315,7 +1314,8 @@ static void dense_sqrl2_assign_mfma(Data &ds, const float *C, uint64_t k, int pr
         hipLaunchKernelGGL(k_center_norms, dim3((uint32_t)cdiv(kpad, 4)), dim3(256), 0, st, C, k, kpad, d,
                            ds.mu.as<const float>(), c.s[S_CNORM].as<double>());
         hipLaunchKernelGGL(k_prep_params, dim3(1), dim3(256), 0, st, c.s[S_CNORM].as<const double>(),
-                           c.s[S_CINFO].as<const float2>(), k, kpad, ds.xmax, d, ds.xscale, c.s[S_CADD].as<float>(),
+                           c.s[S_CINFO].as<const float2>(), k, kpad, ds.xstat.as<const float>(), d, 1,
+                           c.s[S_CADD].as<float>(),
                            c.s[S_PARAMS].as<EParams>());
         hipLaunchKernelGGL(k_prep_centers_h, dim3((uint32_t)cdiv(kpad, 4)), dim3(256), 0, st, C, k, kpad, d, dpad,
                            ds.mu.as<const float>(), c.s[S_PARAMS].as<EParams>(), c.s[S_CB].as<uint16_t>(),
@@ -1325,7 +1325,8 @@ static void dense_sqrl2_assign_mfma(Data &ds, const float *C, uint64_t k, int pr
                            ds.mu.as<const float>(), c.s[S_CB].as<uint16_t>(), c.s[S_CNORM].as<double>(),
                            c.s[S_CINFO].as<float2>());
         hipLaunchKernelGGL(k_prep_params, dim3(1), dim3(256), 0, st, c.s[S_CNORM].as<const double>(),
-                           c.s[S_CINFO].as<const float2>(), k, kpad, ds.xmax, d, 0.f, c.s[S_CADD].as<float>(),
+                           c.s[S_CINFO].as<const float2>(), k, kpad, ds.xstat.as<const float>(), d, 0,
+                           c.s[S_CADD].as<float>(),
                            c.s[S_PARAMS].as<EParams>());
     }
     fill_u32(R.cnt, 0u, 4, st);

@@ -1051,7 +1051,7 @@ void generic_costs(Data &ds, int msr, double prior, const float *C, uint64_t k, 
     const dim3 grid((uint32_t)cdiv(n, 4)), blk(256);
     const float fpv = (float)pv, lpv = (float)log(pv > 0 ? pv : 1.);
     if (!single && tiled_costs_ok(ds, msr, k) && tiled_costs_ready(ds, k, st)) {
-        tiled_costs(ds, msr, CT, kpad, k, fpv, lpv, rt, cs, G, slot, out_nk, asn, best, st);
+        tiled_costs(ds, msr, CT, kpad, k, fpv, lpv, rt, cs, out_nk, asn, best, st);
         return;
     }
     if (single) {

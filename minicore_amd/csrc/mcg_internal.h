@@ -116,8 +116,7 @@ struct Data {
     DBuf xdev;         // ||bf16(x - mu) - (x - mu)|| per row (rounded up)
     DBuf Xh;           // FP16 copy of xscale * (x - mu), padded to dpad columns
     DBuf xdev_h;       // ||Xh / xscale - (x - mu)|| per row (rounded up)
-    float xscale = 1.f;
-    float xmax = 0.f;
+    DBuf xstat;        // device floats: [0] xmax = max ||x - mu||, [1] xscale (FP16 form)
     // per-prior CSR transforms (cached for the last prior seen)
     double tf_prior = -1.;
     DBuf tf_logx;      // log1p(x / pv) per stored value (MKL)
@@ -150,6 +149,9 @@ struct Ctx {
     uint64_t timed_n[2] = {0, 0};
     // E-step kernel chosen per problem shape by timing (see dense_sqrl2_assign)
     bool tuning = true;
+    // a server worker: shape choices come from the fixed model, never from timing runs
+    // inside a request
+    bool serving = false;
     // Lloyd iterations after the first replay one captured HIP graph (off by default:
     // while a capture is open, HIP rejects legacy-stream calls from other threads)
     bool graphs = false;
@@ -260,8 +262,7 @@ inline bool msr_mean_centroid(int msr) {
 bool tiled_costs_ok(const Data &ds, int msr, uint64_t k);
 bool tiled_costs_ready(Data &ds, uint64_t k, hipStream_t st);
 void tiled_costs(Data &ds, int msr, const float *CT, uint32_t kpad, uint64_t k, float pv, float logpv,
-                 const float4 *rt, const double *cs, const double *G, const uint32_t *slot, float *out_nk,
-                 uint32_t *asn, float *best, hipStream_t st);
+                 const float4 *rt, const double *cs, float *out_nk, uint32_t *asn, float *best, hipStream_t st);
 bool tiled_mstep_ok(const Data &ds, uint64_t k);
 bool tiled_mstep_ready(Data &ds, uint64_t k, const uint64_t *cptr, const uint32_t *crow, const float *cval,
                        hipStream_t st);

@@ -467,6 +467,7 @@ int mcg_server_create(int device, int nworkers, uint64_t hbm_budget, mcg_server 
                 for (auto *x : s->ctxs) mcg_ctx_destroy(x);
                 MCG_THROW(rc, mcg_last_error());
             }
+            c->c.serving = true;
             s->ctxs.push_back(c);
         }
         mcg_server *raw = s.get();

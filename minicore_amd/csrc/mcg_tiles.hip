@@ -313,7 +313,7 @@ __device__ __forceinline__ void stage_loop(const Stage<CPL, ALIGN16> &S, const u
 // Cross terms of TG = 16 * TWR rows x 64 * CPL centers per block (TWR = 64 / CPL rows a
 // wave, CPL centers a lane), then the per-row / per-center terms of the decomposition
 // (as k_row_costs) and the cost matrix and/or the row minimum (64-bit keys: cost bits
-// above, center below; lowest cost, then lowest center).
+// above, center below; lowest cost, then lowest center).  SQRL2, MKL, REVERSE_MKL.
 template <int MSR, int CPL>
 __global__ __launch_bounds__(1024) void k_tile_estep(const uint8_t *__restrict__ cnt, const uint16_t *__restrict__ wb,
                                                      const uint32_t *__restrict__ off, const uint2 *__restrict__ ent,
@@ -321,7 +321,6 @@ __global__ __launch_bounds__(1024) void k_tile_estep(const uint8_t *__restrict__
                                                      const float *__restrict__ CT, uint32_t kpad, uint32_t k,
                                                      uint32_t nslices, float pv, float logpv,
                                                      const float4 *__restrict__ rt, const double *__restrict__ cs,
-                                                     const double *__restrict__ G, const uint32_t *__restrict__ slot,
                                                      float *__restrict__ out, unsigned long long *__restrict__ keys,
                                                      uint8_t *__restrict__ nanflag) {
     constexpr int TWR = 64 / CPL, TG = TWR * TNW, CW = 64 * CPL;
@@ -336,13 +335,6 @@ __global__ __launch_bounds__(1024) void k_tile_estep(const uint8_t *__restrict__
     for (int a = 0; a < TWR; ++a)
 #pragma unroll
         for (int u = 0; u < CPL; ++u) acc[a][u] = 0.f;
-    // per-row terms the entry terms need (JSD): li and 1/L, lane a holding row a's
-    float li_l = 0.f, Li_l = 0.f;
-    if (MSR == MCG_JSD && lane < TWR && row0 + lane < n) {
-        const float4 r = rt[row0 + lane];
-        li_l = r.w;
-        Li_l = 1.f / r.x;
-    }
     const Stage<CPL, true> S{CT, kpad, d, c0, ts, (uint32_t)TG, ent, cnt, wb};
     auto walk = [&](const float *Tw, const uint2 *Es, const uint32_t *Cs, const uint32_t *Ws) {
         stage_walk<TWR>(Tw + lane * CPL, Es, Cs, Ws, wave, lane, pv, MSR == MCG_MKL,
@@ -354,19 +346,9 @@ __global__ __launch_bounds__(1024) void k_tile_estep(const uint8_t *__restrict__
             if (MSR == MCG_SQRL2) {
 #pragma unroll
                 for (int u = 0; u < CPL; ++u) acc[a][u] = fmaf(x, x - 2.f * t[u], acc[a][u]);
-            } else if (MSR == MCG_MKL || MSR == MCG_REVERSE_MKL) {
+            } else {
 #pragma unroll
                 for (int u = 0; u < CPL; ++u) acc[a][u] = fmaf(x, t[u], acc[a][u]);
-            } else {
-                const float li = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(li_l), a));
-                const float Li = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Li_l), a));
-                const float xt = (x + pv) * Li;
-                const float hd = xt * __logf(xt) - li * __logf(li);
-#pragma unroll
-                for (int u = 0; u < CPL; ++u) {
-                    const float m1 = xt + t[u], m0 = li + t[u];
-                    acc[a][u] += 0.5f * (hd - m1 * __logf(0.5f * m1) + m0 * __logf(0.5f * m0));
-                }
             }
         });
     };
@@ -379,7 +361,6 @@ __global__ __launch_bounds__(1024) void k_tile_estep(const uint8_t *__restrict__
         if (row >= n) break;
         float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
         if (MSR != MCG_SQRL2) r = rt[row];
-        const double *Grow = MSR == MCG_JSD ? G + (uint64_t)slot[row] * kpad : nullptr;
         // a NaN cost never wins, except at center 0, which the reference's scan keeps
         // (then the row's flag marks the NaN minimum); the lane's centers ascend, so a
         // strict comparison keeps the lowest center among equal costs
@@ -398,9 +379,7 @@ __global__ __launch_bounds__(1024) void k_tile_estep(const uint8_t *__restrict__
                 w = w <= 0.f ? 0.f : w;
             } else {
                 if (MSR == MCG_MKL) w = (float)(cs[2 * cc] - cs[2 * cc + 1] * (double)(logpv - r.y) - (double)av);
-                else if (MSR == MCG_REVERSE_MKL)
-                    w = (float)((double)r.z - (double)r.w * cs[2 * cc] - (double)av * (double)(1.f / r.x));
-                else w = (float)(Grow[cc] + (double)av);
+                else w = (float)((double)r.z - (double)r.w * cs[2 * cc] - (double)av * (double)(1.f / r.x));
                 w = w <= 0.f ? 0.f : w;   // also -0 -> +0, so the key bits order like the values
                 if (w == INFINITY) w = FLT_MAX;
             }
@@ -448,9 +427,17 @@ static bool tiles_enabled() {
 // centers per lane of the E-step tiles (the row kernel serves k <= 64)
 static int estep_cpl(uint64_t k) { return k > 128 ? 4 : 2; }
 
+// The tiles pay for their build (a few passes over the entries) and their staging
+// when the product is large; JSD's entry terms are bound by their logarithms (two
+// per entry and center either way), so it stays with the row kernel.
+// (with the context's tuning off they are used whenever they apply)
+static bool tiles_worth(const Data &ds, uint64_t k) {
+    return !ds.ctx->tuning || (double)ds.nnz * (double)k >= 4e9;
+}
+
 bool tiled_costs_ok(const Data &ds, int msr, uint64_t k) {
     return tiles_enabled() && ds.csr && ds.nnz < (1ull << 32) && ds.n < (1ull << 32) && k > 64 &&
-           (msr == MCG_SQRL2 || msr == MCG_MKL || msr == MCG_REVERSE_MKL || msr == MCG_JSD);
+           tiles_worth(ds, k) && (msr == MCG_SQRL2 || msr == MCG_MKL || msr == MCG_REVERSE_MKL);
 }
 
 // the row tiles for k centers (built on first use); false when no stage size fits
@@ -464,8 +451,7 @@ bool tiled_costs_ready(Data &ds, uint64_t k, hipStream_t st) {
 }
 
 void tiled_costs(Data &ds, int msr, const float *CT, uint32_t kpad, uint64_t k, float pv, float logpv,
-                 const float4 *rt, const double *cs, const double *G, const uint32_t *slot, float *out_nk,
-                 uint32_t *asn, float *best, hipStream_t st) {
+                 const float4 *rt, const double *cs, float *out_nk, uint32_t *asn, float *best, hipStream_t st) {
     Ctx &c = *ds.ctx;
     const Tiles &T = ds.rtiles;
     REQUIRE(T.ready && T.cpl == estep_cpl(k), "internal: row tiles not ready");
@@ -490,20 +476,18 @@ void tiled_costs(Data &ds, int msr, const float *CT, uint32_t kpad, uint64_t k, 
     c.tic(0, ev);
 #define TILE_E(M, P)                                                                                              \
     hipLaunchKernelGGL((k_tile_estep<M, P>), grid, blk, 0, st, cn, w, o, e, ds.n, T.nstages, T.ts, d, CT, kpad,      \
-                       (uint32_t)k, nslices, pv, logpv, rt, cs, G, slot, out_nk, keys, nanflag)
+                       (uint32_t)k, nslices, pv, logpv, rt, cs, out_nk, keys, nanflag)
     if (T.cpl == 4) {
         switch (msr) {
         case MCG_SQRL2: TILE_E(MCG_SQRL2, 4); break;
         case MCG_MKL: TILE_E(MCG_MKL, 4); break;
-        case MCG_REVERSE_MKL: TILE_E(MCG_REVERSE_MKL, 4); break;
-        default: TILE_E(MCG_JSD, 4); break;
+        default: TILE_E(MCG_REVERSE_MKL, 4); break;
         }
     } else {
         switch (msr) {
         case MCG_SQRL2: TILE_E(MCG_SQRL2, 2); break;
         case MCG_MKL: TILE_E(MCG_MKL, 2); break;
-        case MCG_REVERSE_MKL: TILE_E(MCG_REVERSE_MKL, 2); break;
-        default: TILE_E(MCG_JSD, 2); break;
+        default: TILE_E(MCG_REVERSE_MKL, 2); break;
         }
     }
 #undef TILE_E
@@ -580,7 +564,7 @@ static int mstep_cpl(uint64_t k) {
 }
 
 bool tiled_mstep_ok(const Data &ds, uint64_t k) {
-    return tiles_enabled() && ds.csr && ds.nnz < (1ull << 32) && ds.n < (1ull << 32) && k > 64;
+    return tiles_enabled() && ds.csr && ds.nnz < (1ull << 32) && ds.n < (1ull << 32) && k > 64 && tiles_worth(ds, k);
 }
 
 // the column tiles (built on first use); false when no stage size fits

@@ -486,7 +486,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--form", choices=["fp16", "bf16"], default="fp16",
                     help="16-bit operand format of the headline E-step (both give the exact argmin; "

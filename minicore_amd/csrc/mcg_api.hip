@@ -1654,11 +1654,11 @@ int mcg_lloyd_step(mcg_data *ds, int msr, double prior, float *centers, uint64_t
             copy_sync(hp.data(), pc, n * 4, hipMemcpyDeviceToHost, st);
             std::vector<double> wv(n);
             for (uint64_t i = 0; i < n; ++i) wv[i] = wh.empty() ? (double)hp[i] : wh[i] * (double)hp[i];
+            FixedSeq seq(std::move(wv));
             std::vector<uint64_t> rows(ids.size());
             for (size_t e = 0; e < ids.size(); ++e) {
-                const uint64_t r = wyrand_at(0, j++);
-                draw_fixed_many(wv, &r, 1, &rows[e]);
-                wv[rows[e]] = 0.;
+                rows[e] = seq.draw(wyrand_at(0, j++));
+                seq.zero(rows[e]);
             }
             copy_rows_to_centers(D, rows.data(), ids.data(), ids.size(), centers, st);
             char *buf = (char *)c.s[S_MISC2].reserve(ids.size() * 12);

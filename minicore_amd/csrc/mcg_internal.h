@@ -313,6 +313,19 @@ uint64_t restart_draw_device(Ctx &ctx, const float *costs_d, uint64_t n, uint64_
 // Fixed-point inverse-CDF draws over host weights wv for raw draws r[0..m): the
 // weights are converted once, each draw is a search of their prefix sums.
 void draw_fixed_many(const std::vector<double> &wv, const uint64_t *r, uint64_t m, uint64_t *out);
+class FixedSeq {
+  public:
+    explicit FixedSeq(std::vector<double> w);
+    uint64_t draw(uint64_t r) const;
+    void zero(uint64_t i);
+  private:
+    void build();
+    std::vector<double> w_;
+    std::vector<uint64_t> fx_, tree_;
+    double mx_ = 0.;
+    int s_ = 0;
+    uint64_t T_ = 0;
+};
 // Exact sum of w_i * v_i (v float on the device, w nullable): each term in fixed
 // point with the sampler's shift, integer sums; independent of the order.
 double fixed_wsum(Ctx &ctx, const float *v, const double *w, uint64_t n, hipStream_t st);

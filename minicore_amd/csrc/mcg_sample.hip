@@ -197,6 +197,50 @@ void draw_fixed_many(const std::vector<double> &wv, const uint64_t *r, uint64_t 
     }
 }
 
+// Draws without replacement: draw(r) is draw_fixed_many's draw on the current
+// weights, and zero(i) removes row i before the next draw.  A Fenwick tree over the
+// fixed-point weights keeps both O(log n); when the removed weight was the maximum,
+// the shift and the tree are rebuilt (draw_fixed_many recomputes them per call).
+FixedSeq::FixedSeq(std::vector<double> w) : w_(std::move(w)) { build(); }
+
+void FixedSeq::build() {
+    const uint64_t n = w_.size();
+    mx_ = 0.;
+    for (double x : w_) if (x > mx_) mx_ = x;
+    s_ = fixed_shift(mx_, n);
+    fx_.assign(n, 0);
+    tree_.assign(n + 1, 0);
+    T_ = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        fx_[i] = to_fixed(w_[i], s_);
+        T_ += fx_[i];
+        tree_[i + 1] += fx_[i];
+        const uint64_t p = (i + 1) + ((i + 1) & (~(i + 1) + 1));
+        if (p <= n) tree_[p] += tree_[i + 1];
+    }
+}
+
+uint64_t FixedSeq::draw(uint64_t r) const {
+    const uint64_t n = w_.size();
+    if (T_ == 0) return 0;
+    uint64_t rem = mulhi64(r, T_), pos = 0, step = 1;
+    while (step * 2 <= n) step *= 2;
+    // the largest pos whose prefix is <= u: the first row whose inclusive prefix exceeds u
+    for (; step; step >>= 1)
+        if (pos + step <= n && tree_[pos + step] <= rem) { pos += step; rem -= tree_[pos]; }
+    return pos < n ? pos : n - 1;
+}
+
+void FixedSeq::zero(uint64_t i) {
+    const double old = w_[i];
+    w_[i] = 0.;
+    if (old >= mx_) { build(); return; }
+    const uint64_t f = fx_[i];
+    fx_[i] = 0;
+    T_ -= f;
+    for (uint64_t p = i + 1; p < tree_.size(); p += p & (~p + 1)) tree_[p] -= f;
+}
+
 double fixed_wsum(Ctx &c, const float *v, const double *w, uint64_t n, hipStream_t st) {
     const uint32_t nblk = (uint32_t)cdiv(n, FX_BLK);
     char *buf = (char *)c.s[S_MISC0].reserve(64 + nblk * 8);

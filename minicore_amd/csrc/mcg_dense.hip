@@ -85,13 +85,17 @@ __global__ void k_colsum_partial(const float *__restrict__ X, uint64_t n, uint32
         // four rows in flight; the four running sums are added in a fixed order
         double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
         uint64_t r = r0;
+        // non-finite values are left out: mu only centres the 16-bit operands (any
+        // finite shift is exact for the bounds), and one such value must not make every
+        // row's operand non-finite
+        auto fin = [](float v) { return fabsf(v) <= FLT_MAX ? (double)v : 0.; };
         for (; r + 4 <= r1; r += 4) {
-            s0 += (double)X[r * d + j];
-            s1 += (double)X[(r + 1) * d + j];
-            s2 += (double)X[(r + 2) * d + j];
-            s3 += (double)X[(r + 3) * d + j];
+            s0 += fin(X[r * d + j]);
+            s1 += fin(X[(r + 1) * d + j]);
+            s2 += fin(X[(r + 2) * d + j]);
+            s3 += fin(X[(r + 3) * d + j]);
         }
-        for (; r < r1; ++r) s0 += (double)X[r * d + j];
+        for (; r < r1; ++r) s0 += fin(X[r * d + j]);
         part[(uint64_t)blockIdx.y * d + j] = (s0 + s1) + (s2 + s3);
     }
 }
@@ -157,12 +161,12 @@ __global__ void k_make_x16(const float *__restrict__ X, uint64_t n, uint32_t d, 
     if (lane == 0) xdev[row] = (float)(sqrt(e) * (1. + 1e-6));
 }
 
-// max of non-negative floats (their bit patterns order like their values)
+// max of the finite non-negative floats (their bit patterns order like their values)
 __global__ void k_max_f32(const float *__restrict__ v, uint64_t n, unsigned int *__restrict__ out_bits) {
     __shared__ float sm[256];
     float m = 0.f;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        m = fmaxf(m, v[i]);
+        if (v[i] <= FLT_MAX) m = fmaxf(m, v[i]);   // rows with non-finite values are left out
     sm[threadIdx.x] = m;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
@@ -330,8 +334,8 @@ __global__ void k_prep_params(const double *__restrict__ cn, const float2 *__res
     const float xmax = xstat[0], xscale = f16 ? xstat[1] : 1.f;
     double m = 0., me = 0.;
     for (uint64_t i = threadIdx.x; i < k; i += blockDim.x) {
-        m = fmax(m, cn[i]);
-        if (!f16) me = fmax(me, (double)cinfo[i].y);
+        if (cn[i] <= DBL_MAX) m = fmax(m, cn[i]);   // non-finite centers never win (cadd below)
+        if (!f16 && cn[i] <= DBL_MAX) me = fmax(me, (double)cinfo[i].y);
     }
     sm[threadIdx.x] = m;
     se[threadIdx.x] = me;
@@ -364,11 +368,14 @@ __global__ void k_prep_params(const double *__restrict__ cn, const float2 *__res
         p.dbmax = f16 ? 0.f : (float)se[0];
         p.s = (float)S;
         p.inv_s = (float)(1. / S);
+        // a NaN in center 0 makes every row's first cost NaN, which the reference's scan keeps
+        p.c0nan = k > 0 && cn[0] != cn[0];
+        p.pad[0] = p.pad[1] = p.pad[2] = 0;
         *prm = p;
     }
     __syncthreads();
     for (uint64_t i = threadIdx.x; i < kpad; i += blockDim.x)
-        cadd[i] = i < k ? (float)(S * (off + cn[i])) : 3.0e38f;
+        cadd[i] = (i < k && cn[i] <= DBL_MAX) ? (float)(S * (off + cn[i])) : 3.0e38f;
 }
 
 // ---------------------------------------------------------------------------
@@ -682,7 +689,12 @@ __global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep16(
         pthr[nb] = FLT_MAX;
         pc1[nb] = c1;
         pc2[nb] = c2;
-        if (mine) {
+        if (mine && (P.c0nan || !(xnorm[p] <= FLT_MAX))) {
+            // center 0 NaN: every row keeps center 0 (the scan's first cost is NaN); a row
+            // with non-finite values: the exact scan over all centers (overflow list)
+            asn[p] = 0u;
+            if (!P.c0nan) cand_overflow(R, p);
+        } else if (mine) {
             const float xn = xnorm[p], dxn = xdev[p];
             // the values carry the scale P.s (a power of two; 1 for BF16)
             const float dbest = xn * xn + (v1 * P.inv_s - P.off);
@@ -1072,17 +1084,27 @@ __global__ void k_prep_centers_f32(const float *__restrict__ C, uint64_t k, uint
 // ---------------------------------------------------------------------------
 constexpr int EX_TC = 16;  // centers per register tile
 
+// keys == nullptr: the block's 256 rows against all k centers, written to asn / cost.
+// Otherwise blockIdx.y takes the centers [y * per, (y + 1) * per) and min-reduces
+// (cost bits << 32 | center) per row into keys (a NaN cost never wins, except at
+// center 0, which the scan above keeps: then key 0 and the row's flag), so rows of a
+// small n are spread over more blocks; k_exact_keys_out writes the result.
 __global__ __launch_bounds__(256) void k_assign_exact(const float *__restrict__ X, uint64_t n, uint32_t d,
-                                                      const float *__restrict__ C, uint32_t k,
-                                                      uint32_t *__restrict__ asn, float *__restrict__ cost) {
+                                                      const float *__restrict__ C, uint32_t k, uint32_t per,
+                                                      uint32_t *__restrict__ asn, float *__restrict__ cost,
+                                                      unsigned long long *__restrict__ keys,
+                                                      uint8_t *__restrict__ nanflag) {
     __shared__ float xs[256][33];
     __shared__ float cs[EX_TC][32];
     const int tid = threadIdx.x;
     const uint64_t p0 = (uint64_t)blockIdx.x * 256;
     const uint64_t p = p0 + tid;
+    const uint32_t cb = keys ? blockIdx.y * per : 0u;
+    const uint32_t ce = keys ? min(k, cb + per) : k;
     float best = 0.f;
     uint32_t bi = 0xFFFFFFFFu;
-    for (uint32_t c0 = 0; c0 < k; c0 += EX_TC) {
+    bool nan0 = false;
+    for (uint32_t c0 = cb; c0 < ce; c0 += EX_TC) {
         float acc[EX_TC];
 #pragma unroll
         for (int c = 0; c < EX_TC; ++c) acc[c] = 0.f;
@@ -1096,7 +1118,7 @@ __global__ __launch_bounds__(256) void k_assign_exact(const float *__restrict__ 
             }
             for (int q = tid; q < EX_TC * 32; q += 256) {
                 int c = q >> 5, jj = q & 31;
-                cs[c][jj] = (c0 + c < k && j0 + jj < d) ? C[(uint64_t)(c0 + c) * d + j0 + jj] : 0.f;
+                cs[c][jj] = (c0 + c < ce && j0 + jj < d) ? C[(uint64_t)(c0 + c) * d + j0 + jj] : 0.f;
             }
             __syncthreads();
             const int jn = (int)min<uint32_t>(32, d - j0);
@@ -1108,10 +1130,47 @@ __global__ __launch_bounds__(256) void k_assign_exact(const float *__restrict__ 
         }
 #pragma unroll
         for (int c = 0; c < EX_TC; ++c) {
-            if (c0 + c < k && (bi == 0xFFFFFFFFu || acc[c] < best)) { best = acc[c]; bi = c0 + c; }
+            if (c0 + c >= ce) continue;
+            if (keys) {
+                if (c0 + c == 0 && acc[c] != acc[c]) nan0 = true;
+                if (acc[c] == acc[c] && (bi == 0xFFFFFFFFu || acc[c] < best)) { best = acc[c]; bi = c0 + c; }
+            } else if (bi == 0xFFFFFFFFu || acc[c] < best) {
+                best = acc[c];
+                bi = c0 + c;
+            }
         }
     }
-    if (p < n) { asn[p] = bi; if (cost) cost[p] = best; }
+    if (p >= n) return;
+    if (!keys) {
+        asn[p] = bi;
+        if (cost) cost[p] = best;
+        return;
+    }
+    if (nan0) {
+        nanflag[p] = 1;
+        atomicMin(keys + p, 0ull);
+    } else if (bi != 0xFFFFFFFFu) {
+        // costs are >= 0, so their bit patterns order like their values
+        atomicMin(keys + p, ((unsigned long long)__float_as_uint(best) << 32) | bi);
+    }
+}
+
+__global__ void k_exact_keys_out(const unsigned long long *__restrict__ keys, const uint8_t *__restrict__ nanflag,
+                                 uint64_t n, uint32_t *__restrict__ asn, float *__restrict__ cost) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const unsigned long long kk = keys[i];
+        // every center NaN (and center 0 not): the scan keeps center 0 with its cost
+        const bool none = kk == ~0ull;
+        asn[i] = none ? 0u : (uint32_t)kk;
+        if (cost) cost[i] = (nanflag[i] || none) ? NAN : __uint_as_float((uint32_t)(kk >> 32));
+    }
+}
+
+__global__ void k_fill_keys(unsigned long long *__restrict__ keys, uint8_t *__restrict__ nanflag, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        keys[i] = ~0ull;
+        nanflag[i] = 0;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_point_costs(const float *__restrict__ X, uint64_t n, uint32_t d,
@@ -1184,8 +1243,26 @@ void dense_sqrl2_assign_exact(Data &ds, const float *C, uint64_t k, uint32_t *as
                               hipStream_t st) {
     hipEvent_t ev;
     ds.ctx->tic(0, ev);
-    hipLaunchKernelGGL(k_assign_exact, dim3((uint32_t)cdiv(ds.n, 256)), dim3(256), 0, st, ds.X, ds.n,
-                       (uint32_t)ds.d, C, (uint32_t)k, asn, costs);
+    const uint32_t nbx = (uint32_t)cdiv(ds.n, 256);
+    // fewer than 256 row blocks: the centers are split over up to 512 / nbx slices
+    const uint64_t tiles = cdiv(k, EX_TC);
+    const uint64_t S = nbx >= 256 ? 1 : std::min<uint64_t>(cdiv(512, nbx), tiles);
+    if (S <= 1) {
+        hipLaunchKernelGGL(k_assign_exact, dim3(nbx), dim3(256), 0, st, ds.X, ds.n, (uint32_t)ds.d, C, (uint32_t)k,
+                           (uint32_t)k, asn, costs, (unsigned long long *)nullptr, (uint8_t *)nullptr);
+    } else {
+        const uint32_t per = (uint32_t)(cdiv(tiles, S) * EX_TC);
+        const uint32_t ny = (uint32_t)cdiv(k, per);
+        char *kb = (char *)ds.ctx->s[S_KEYS].reserve(ds.n * 9 + 8);
+        auto *keys = (unsigned long long *)kb;
+        auto *nanflag = (uint8_t *)(kb + ds.n * 8);
+        const dim3 g1((uint32_t)std::min<uint64_t>(cdiv(ds.n, 256), 4096));
+        hipLaunchKernelGGL(k_fill_keys, g1, dim3(256), 0, st, keys, nanflag, ds.n);
+        hipLaunchKernelGGL(k_assign_exact, dim3(nbx, ny), dim3(256), 0, st, ds.X, ds.n, (uint32_t)ds.d, C, (uint32_t)k,
+                           per, asn, costs, keys, nanflag);
+        hipLaunchKernelGGL(k_exact_keys_out, g1, dim3(256), 0, st, (const unsigned long long *)keys,
+                           (const uint8_t *)nanflag, ds.n, asn, costs);
+    }
     ds.ctx->toc(0, ev);
 }
 

@@ -182,6 +182,8 @@ struct EParams {
     float dbmax;    // max_c ||c~ - (c - mu)||, c~ the 16-bit center operand (unscaled)
     float s;        // scale of the MFMA values: x scale * center scale (1 for BF16)
     float inv_s;    // 1 / s (exact: powers of two)
+    int c0nan;      // center 0 has a NaN: every row's first cost is NaN
+    int pad[3];
 };
 
 void prepare_base(Data &ds);  // mu, xnorm, xmax, dpad (once per dataset)

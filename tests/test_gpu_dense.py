@@ -331,3 +331,26 @@ def test_lloyd_graph_replay_matches_eager(mcg, kind):
         for key in ("centers", "asn", "costs"):
             assert np.array_equal(a[key], b[key]), (t, key)
     assert eager[-1]["numiter"] >= 3
+
+
+@pytest.mark.parametrize("n", [3000, 70000])
+def test_nan_rows_and_centers(mcg, ctx, mco, n):
+    """NaN in a row makes all its costs NaN: the oracle's scan keeps center 0 with a NaN
+    cost.  A center with a NaN never wins.  A NaN center 0 keeps every row (its cost is
+    NaN for all of them).  Exact form (centers split over blocks at the smaller n) and
+    the 16-bit forms against the oracle."""
+    say(f"nan rows/centers n={n}")
+    X, C = mixture(n, 32, 40, 31)
+    X[5, 3] = np.nan
+    X[n - 1, 0] = np.nan
+    C[7, 2] = np.nan
+    C0 = C.copy()
+    C0[0, 1] = np.nan
+    for cc in (C, C0):
+        a_ref, c_ref, _ = mco.assign_sqrl2_f32(X, cc)
+        ds = mcg.Dataset.dense(X, ctx)
+        for prec in (mcg.PREC_EXACT, mcg.PREC_BF16, mcg.PREC_FP16):
+            a, c = ds.assign(mcg.SQRL2, cc, precision=prec)
+            assert np.array_equal(a, a_ref), (prec, np.nonzero(a != a_ref)[0][:10])
+            np.testing.assert_array_equal(c, c_ref)
+        ds.close()

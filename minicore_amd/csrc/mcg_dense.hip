@@ -1009,24 +1009,32 @@ __global__ __launch_bounds__(256) void k_estep_f32(const float *__restrict__ X, 
         b[s] = j < d ? xr[j] - mu[j] : 0.f;
     }
 
-    float4 pf[CPT > 0 ? CPT : 1];
+    // the next tile in registers (CPT float4 per thread), then into LDS; written out
+    // without an array captured by a lambda, which would live in scratch memory
+    float4 pf0, pf1, pf2, pf3, pf4, pf5, pf6, pf7;
     float addn = 0.f;
-    auto load_tile = [&](uint32_t t) {
-#pragma unroll
-        for (int q = 0; q < CPT; ++q) {
-            int g = tid + q * 256;
-            pf[q] = ((const float4 *)Cc)[(size_t)(t * 32 + g / (DP / 4)) * (DP / 4) + (g % (DP / 4))];
-        }
-        if (tid < 32) addn = cadd[t * 32 + tid];
+#define F32_LD(q, v)                                                                                       \
+    if (q < CPT) {                                                                                         \
+        const int g = tid + q * 256;                                                                       \
+        v = ((const float4 *)Cc)[(size_t)(t1 * 32 + g / (DP / 4)) * (DP / 4) + (g % (DP / 4))];          \
+    }
+#define F32_ST(q, v)                                                                                       \
+    if (q < CPT) {                                                                                         \
+        const int g = tid + q * 256;                                                                       \
+        *(float4 *)(&lds[buf1][(g / (DP / 4)) * ROWF + (g % (DP / 4)) * 4]) = v;                          \
+    }
+    auto load_tile = [&](uint32_t t1) __attribute__((always_inline)) {
+        F32_LD(0, pf0) F32_LD(1, pf1) F32_LD(2, pf2) F32_LD(3, pf3)
+        F32_LD(4, pf4) F32_LD(5, pf5) F32_LD(6, pf6) F32_LD(7, pf7)
+        if (tid < 32) addn = cadd[t1 * 32 + tid];
     };
-    auto store_tile = [&](int buf) {
-#pragma unroll
-        for (int q = 0; q < CPT; ++q) {
-            int g = tid + q * 256;
-            *(float4 *)(&lds[buf][(g / (DP / 4)) * ROWF + (g % (DP / 4)) * 4]) = pf[q];
-        }
-        if (tid < 32) ladd[buf][tid] = addn;
+    auto store_tile = [&](int buf1) __attribute__((always_inline)) {
+        F32_ST(0, pf0) F32_ST(1, pf1) F32_ST(2, pf2) F32_ST(3, pf3)
+        F32_ST(4, pf4) F32_ST(5, pf5) F32_ST(6, pf6) F32_ST(7, pf7)
+        if (tid < 32) ladd[buf1][tid] = addn;
     };
+#undef F32_LD
+#undef F32_ST
 
     float best = 0.f;
     uint32_t bi = 0xFFFFFFFFu;

@@ -908,6 +908,141 @@ void csr_transforms(Data &ds, double pv, hipStream_t st) {
     ds.tf_prior = pv;
 }
 
+// ---- dense MKL / REVERSE_MKL cross term on MFMA ------------------------------------------
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// The decomposed costs need, per (row, center), the cross term sum_j a_j * t_cj over all
+// d coordinates: a = x (REVERSE_MKL) or log1p(x / pv) (MKL), t = log c~ or c~ (the
+// center tables).  For dense rows with d <= 256 it is an FP32 GEMM on
+// v_mfma_f32_32x32x2f32 (FP32 products and sums, as the row kernel; the sum is
+// formed in the matrix core's order): 32 rows per wave, their operand in registers
+// (lane half h supplies coordinates h*DP/2 + s), the centers' tables in LDS in tiles
+// of 32 (center-major, copied from the d-major table by k_ct_transpose, so the values
+// equal the row kernel's).  The epilogue adds the per-row and per-center terms as
+// k_row_costs does (FP64), clamps, writes the cost matrix and/or min-reduces 64-bit keys
+// (cost bits above, center below: lowest cost, then lowest center; a NaN cost never
+// wins except at center 0, which the row kernel's scan keeps).
+__global__ void k_ct_transpose(const float *__restrict__ CT, uint32_t kpad, uint32_t k, uint32_t d, uint32_t dpad,
+                               float *__restrict__ Ct) {
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (uint64_t)kpad * dpad) return;
+    const uint32_t c = (uint32_t)(e / dpad), j = (uint32_t)(e % dpad);
+    Ct[e] = (c < k && j < d) ? CT[(uint64_t)j * kpad + c] : 0.f;
+}
+
+// MKL's row operand log1p(x / pv), the row kernel's expression, once per dataset and prior
+__global__ void k_log1p_div(const float *__restrict__ X, uint64_t m, float pv, float *__restrict__ out) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = log1pf(X[i] / pv);
+}
+
+template <int MSR, int DP>
+__global__ __launch_bounds__(256) void k_breg_mfma(const float *__restrict__ X, uint32_t d, uint64_t n, float pv,
+                                                   float logpv, const float4 *__restrict__ rt,
+                                                   const float *__restrict__ Ct, const double *__restrict__ cs,
+                                                   uint32_t k, uint32_t ntiles, float *__restrict__ out,
+                                                   uint32_t *__restrict__ asn, float *__restrict__ best_out) {
+    constexpr int KS = DP / 2;
+    constexpr int ROWF = DP + 4;              // padded LDS row stride (floats)
+    constexpr int CPT = (32 * DP / 4) / 256;  // float4 chunks per thread per tile (2, 4, 8)
+    static_assert(CPT <= 8, "tile chunks");
+    __shared__ __attribute__((aligned(16))) float lds[2][32 * ROWF];
+    __shared__ double lcs[2][64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 5, r32 = lane & 31;
+    const uint64_t p = (uint64_t)blockIdx.x * 128u + wave * 32u + r32;
+    const uint64_t pc = p < n ? p : n - 1;
+    float b[KS];
+    const float *xr = X + pc * d;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        const uint32_t j = h * KS + s;
+        b[s] = j < d ? xr[j] : 0.f;   // MKL: X holds log1p(x / pv) already (k_log1p_div)
+    }
+    const float4 r = rt[pc];
+    // the next tile in registers (CPT float4 per thread), then into LDS; written out
+    // without an array captured by a lambda, which would live in scratch memory
+    float4 pf0, pf1, pf2, pf3, pf4, pf5, pf6, pf7;
+    double csn = 0.;
+#define BREG_LD(q, v)                                                                                      \
+    if (q < CPT) {                                                                                         \
+        const int g = tid + q * 256;                                                                       \
+        v = ((const float4 *)Ct)[(size_t)(t1 * 32 + g / (DP / 4)) * (DP / 4) + (g % (DP / 4))];          \
+    }
+#define BREG_ST(q, v)                                                                                      \
+    if (q < CPT) {                                                                                         \
+        const int g = tid + q * 256;                                                                       \
+        *(float4 *)(&lds[buf1][(g / (DP / 4)) * ROWF + (g % (DP / 4)) * 4]) = v;                          \
+    }
+    auto load_tile = [&](uint32_t t1) __attribute__((always_inline)) {
+        BREG_LD(0, pf0) BREG_LD(1, pf1) BREG_LD(2, pf2) BREG_LD(3, pf3)
+        BREG_LD(4, pf4) BREG_LD(5, pf5) BREG_LD(6, pf6) BREG_LD(7, pf7)
+        if (tid < 64) csn = cs[(uint64_t)t1 * 64 + tid];   // {a, b} of the tile's 32 centers
+    };
+    auto store_tile = [&](int buf1) __attribute__((always_inline)) {
+        BREG_ST(0, pf0) BREG_ST(1, pf1) BREG_ST(2, pf2) BREG_ST(3, pf3)
+        BREG_ST(4, pf4) BREG_ST(5, pf5) BREG_ST(6, pf6) BREG_ST(7, pf7)
+        if (tid < 64) lcs[buf1][tid] = csn;
+    };
+#undef BREG_LD
+#undef BREG_ST
+    unsigned long long key = ~0ull;
+    bool nan0 = false;
+    load_tile(0);
+    store_tile(0);
+    __syncthreads();
+    for (uint32_t t = 0; t < ntiles; ++t) {
+        const int buf = t & 1;
+        if (t + 1 < ntiles) load_tile(t + 1);
+        floatx16 acc;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[e] = 0.f;
+        const float *arow = &lds[buf][r32 * ROWF + h * KS];
+#pragma unroll
+        for (int s = 0; s < KS; s += 4) {
+            const float4 a = *(const float4 *)(arow + s);
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b[s + 0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b[s + 1], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, b[s + 2], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, b[s + 3], acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const uint32_t cl = 8u * (e >> 2) + 4u * h + (e & 3), c = t * 32u + cl;
+            if (c >= k) continue;
+            // an opaque copy keeps each element's FP64 epilogue with the element (hoisted, the
+            // widened values of the whole tile would not fit the registers)
+            float av = acc[e];
+            asm volatile("" : "+v"(av));
+            const double c0 = lcs[buf][2 * cl], c1 = lcs[buf][2 * cl + 1];
+            float v;
+            if (MSR == MCG_MKL) v = (float)(c0 - c1 * (double)(logpv - r.y) - (double)av);
+            else v = (float)((double)r.z - (double)r.w * c0 - (double)av * (double)(1.f / r.x));
+            v = v < 0.f ? 0.f : v;
+            if (v == INFINITY) v = FLT_MAX;
+            if (out && p < n) out[p * k + c] = v;
+            if (v != v) {
+                if (c == 0) nan0 = true;
+            } else {
+                const unsigned long long kc = ((unsigned long long)__float_as_uint(v) << 32) | c;
+                key = kc < key ? kc : key;
+            }
+        }
+        if (t + 1 < ntiles) store_tile(buf ^ 1);
+        __syncthreads();
+    }
+    if (asn || best_out) {
+        const unsigned long long ok = __shfl_xor(key, 32);
+        key = ok < key ? ok : key;
+        const bool on = __shfl_xor((int)nan0, 32) != 0;
+        nan0 = nan0 || on;
+        if (h == 0 && p < n) {
+            if (asn) asn[p] = nan0 ? 0u : (key == ~0ull ? 0u : (uint32_t)key);
+            if (best_out) best_out[p] = (nan0 || key == ~0ull) ? NAN : __uint_as_float((uint32_t)(key >> 32));
+        }
+    }
+}
+
 // Per-center tables (CT d x kpad, per-center scalars, JSD's G per distinct li) and
 // the per-row terms for k centers C; returns them with the row arrays of ds.
 // Center tables with row stride kpad (>= k): a multiple of 64 for the kernels whose
@@ -991,6 +1126,12 @@ bool generic_pp_round(Data &ds, int msr, double prior, const float *center_row, 
     return true;
 }
 
+// MCG_BREG_MFMA=0 keeps dense MKL / REVERSE_MKL on the row kernel (for comparisons)
+static bool dense_breg_mfma_on() {
+    static const bool on = [] { const char *e = getenv("MCG_BREG_MFMA"); return !(e && e[0] == '0'); }();
+    return on;
+}
+
 void generic_costs(Data &ds, int msr, double prior, const float *C, uint64_t k, float *out_nk, uint32_t *asn,
                    float *best, hipStream_t st) {
     Ctx &c = *ds.ctx;
@@ -1050,6 +1191,39 @@ void generic_costs(Data &ds, int msr, double prior, const float *C, uint64_t k, 
     const uint32_t *slot = T.slot;
     const dim3 grid((uint32_t)cdiv(n, 4)), blk(256);
     const float fpv = (float)pv, lpv = (float)log(pv > 0 ? pv : 1.);
+    if (!single && !ds.csr && (msr == MCG_MKL || msr == MCG_REVERSE_MKL) && d <= 256 && dense_breg_mfma_on()) {
+        const uint32_t dpad = d <= 64 ? 64u : d <= 128 ? 128u : 256u;
+        float *Ct = (float *)c.s[S_CTM].reserve((uint64_t)kpad * dpad * sizeof(float));
+        hipLaunchKernelGGL(k_ct_transpose, dim3((uint32_t)cdiv((uint64_t)kpad * dpad, 256)), dim3(256), 0, st, CT,
+                           kpad, (uint32_t)k, d, dpad, Ct);
+        const float *A = X;
+        if (msr == MCG_MKL) {
+            if (ds.tf_logx_pv != pv) {
+                ds.tf_logx.reserve(n * d * sizeof(float));
+                hipLaunchKernelGGL(k_log1p_div, dim3((uint32_t)std::min<uint64_t>(cdiv(n * d, 256), 8192)), dim3(256),
+                                   0, st, X, n * d, fpv, ds.tf_logx.as<float>());
+                ds.tf_logx_pv = pv;
+            }
+            A = ds.tf_logx.as<const float>();
+        }
+        const dim3 g((uint32_t)cdiv(n, 128));
+        const uint32_t nt = kpad / 32;
+#define BREG(M, D)                                                                                                 \
+    hipLaunchKernelGGL((k_breg_mfma<M, D>), g, dim3(256), 0, st, A, d, n, fpv, lpv, rt, (const float *)Ct, cs,     \
+                       (uint32_t)k, nt, out_nk, asn, best)
+        hipEvent_t ev;
+        c.tic(0, ev);
+        if (msr == MCG_MKL) {
+            if (dpad == 64) BREG(MCG_MKL, 64); else if (dpad == 128) BREG(MCG_MKL, 128); else BREG(MCG_MKL, 256);
+        } else {
+            if (dpad == 64) BREG(MCG_REVERSE_MKL, 64);
+            else if (dpad == 128) BREG(MCG_REVERSE_MKL, 128);
+            else BREG(MCG_REVERSE_MKL, 256);
+        }
+#undef BREG
+        c.toc(0, ev);
+        return;
+    }
     if (!single && tiled_costs_ok(ds, msr, k) && tiled_costs_ready(ds, k, st)) {
         tiled_costs(ds, msr, CT, kpad, k, fpv, lpv, rt, cs, out_nk, asn, best, st);
         return;

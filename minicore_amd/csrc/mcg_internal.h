@@ -119,7 +119,8 @@ struct Data {
     DBuf xstat;        // device floats: [0] xmax = max ||x - mu||, [1] xscale (FP16 form)
     // per-prior CSR transforms (cached for the last prior seen)
     double tf_prior = -1.;
-    DBuf tf_logx;      // log1p(x / pv) per stored value (MKL)
+    DBuf tf_logx;      // dense rows: log1p(x / pv) per value (MKL on MFMA), for the prior tf_logx_pv
+    double tf_logx_pv = -1.;
     DBuf rowinfo;      // per-row float4 {L, log L, Hx, li}
     std::vector<float> li_unique;  // distinct li values (JSD)
     DBuf li_dev;       // li_unique on the device
@@ -167,7 +168,7 @@ enum Slot {
     S_HIST, S_BASE, S_PERM, S_CNT, S_ITEMOFF, S_PART, S_PARTW, S_PARTQ,
     S_CCOST, S_TOTAL, S_CENT_A, S_CENT_B, S_CENT_C, S_COSTS, S_CT, S_CPARAM,
     S_MISC0, S_MISC1, S_MISC2, S_MISC3, S_W, S_STAGE0, S_STAGE1, S_STAGE2,
-    S_LIS, S_TOT, S_JSDG, S_PAIRS, S_THRESH, S_OVER, S_KEYS, S_OFLAG, S_CINFO, S_CENT_D, S_LCTL, S_PREP, S_ROWLEN, S_MED0, S_MED1, S_MED2, S_MED3, S_NSLOTS
+    S_LIS, S_TOT, S_JSDG, S_PAIRS, S_THRESH, S_OVER, S_KEYS, S_OFLAG, S_CINFO, S_CENT_D, S_LCTL, S_PREP, S_ROWLEN, S_MED0, S_MED1, S_MED2, S_MED3, S_CTM, S_NSLOTS
 };
 static_assert(S_NSLOTS <= 56, "scratch slots");
 
@@ -346,6 +347,7 @@ inline void reset_derived(Data &D) {
     D.ctiles.cpl = 0;
     D.rowsum.release();
     D.tf_prior = -1.;
+    D.tf_logx_pv = -1.;
     D.li_unique.clear();
     D.csc_ptr.release();
     D.csc_row.release();

@@ -70,6 +70,30 @@ def test_cost_matrix_dense_rows(mcg, ctx, mco, name, code):
     np.testing.assert_allclose(got, ref, rtol=2e-4, atol=2e-5 * np.abs(ref).max())
 
 
+@pytest.mark.parametrize("d,k", [(50, 40), (100, 130), (256, 70)])
+@pytest.mark.parametrize("name,code", [("MKL", 5), ("REVERSE_MKL", 11)])
+def test_dense_bregman_mfma(mcg, ctx, mco, name, code, d, k):
+    """Dense MKL / REVERSE_MKL with d <= 256 take the FP32 MFMA cross term
+    (k_breg_mfma, 32x32x2 tiles): the cost matrix within 2e-4 of the FP64 oracle; the
+    assignment equals the FP64 argmin where the FP64 top-2 gap exceeds 1e-4, and its
+    costs equal the cost matrix's row minima."""
+    say(f"dense bregman mfma {name} d={d} k={k}")
+    rng = np.random.default_rng(d + k)
+    X = rng.poisson(2.0, (3000, d)).astype(np.float32)
+    C = (rng.poisson(2.0, (k, d)) + rng.uniform(0.1, 1., (k, d))).astype(np.float32)
+    ref = mco.cost_matrix(code, X, C, 0.05)
+    ds = mcg.Dataset.dense(X, ctx)
+    got = ds.cost_matrix(code, C, prior=0.05)
+    np.testing.assert_allclose(got, ref, rtol=2e-4, atol=2e-5 * np.abs(ref).max())
+    a, c = ds.assign(code, C, prior=0.05)
+    a_ref, _, gap = mco.assign_f64(code, X, C, 0.05)
+    bad = a != a_ref
+    assert np.all(gap[bad] < 1e-4), gap[bad][:10]
+    np.testing.assert_array_equal(c, got[np.arange(len(a)), a])
+    assert np.array_equal(a, np.argmin(got, axis=1))
+    ds.close()
+
+
 @pytest.mark.parametrize("name,code", MEASURES[1:])
 def test_tiny_prior_direct_path(mcg, ctx, mco, name, code):
     """prior 0 takes the reference's per-pair smoothing (7.0069e-42 floor)."""

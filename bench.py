@@ -638,7 +638,7 @@ def main():
         rate = 0.7 / (svc_ms * 1e-3)
         log(f"serving: mean service {svc_ms:.2f} ms -> {rate:.1f} req/s")
         lat_bf, wall_bf, st_bf = serving(mcg, reqs, prec, rate, args.seed + 11, args.serve_workers)
-        lat_fp, wall_fp, _ = serving(mcg, reqs, mcg.PREC_FP32, rate, args.seed + 11, args.serve_workers)
+        lat_fp, wall_fp, st_fp = serving(mcg, reqs, mcg.PREC_FP32, rate, args.seed + 11, args.serve_workers)
         worst = int(np.argmax(lat_bf))
         pct = lambda a, q: float(np.percentile(a, q))
         serve = {"requests": len(reqs), "workers": args.serve_workers, "rate_per_s": rate,
@@ -648,6 +648,8 @@ def main():
                  "fp32_throughput_per_s": len(reqs) / wall_fp,
                  "mean_stage_ms": dict(zip(["seeding", "sensitivities", "coreset", "lloyd", "assign"],
                                            [float(x) for x in st_bf.mean(axis=0)])),
+                 "fp32_mean_stage_ms": dict(zip(["seeding", "sensitivities", "coreset", "lloyd", "assign"],
+                                                [float(x) for x in st_fp.mean(axis=0)])),
                  "slowest_request": {"n": reqs[worst]["n"], "d": reqs[worst]["d"], "k": reqs[worst]["k"],
                                      "msr": reqs[worst]["msr"], "latency_ms": float(lat_bf[worst]),
                                      "stage_ms": [float(x) for x in st_bf[worst]]},

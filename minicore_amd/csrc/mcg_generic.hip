@@ -26,61 +26,15 @@ namespace mcg {
 
 constexpr double DIRECT_PRIOR = 1e-20;
 
-// ---- center tables -----------------------------------------------------------
+// ---- center tables ----------------------------------------------------------
 // One block per center.  CT is d x kpad (transposed so lanes over centers read
 // contiguous memory); cs = {a, b} per-center scalars (FP64).
-__global__ void k_center_tables(int msr, const float *__restrict__ C, uint32_t k, uint32_t kpad, uint32_t d,
-                                double pv, float *__restrict__ CT, double *__restrict__ cs) {
+__global__ __launch_bounds__(256) void k_center_tables(int msr, const float *__restrict__ C, uint32_t k,
+                                                       uint32_t kpad, uint32_t d, double pv, float *__restrict__ CT,
+                                                       double *__restrict__ cs) {
     __shared__ double red[2][256];
     const uint32_t c = blockIdx.x;
-    const float *cr = C + (uint64_t)c * d;
-    // center sum
-    double s = 0.;
-    for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) s += (double)cr[j];
-    red[0][threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[0][threadIdx.x] += red[0][threadIdx.x + o];
-        __syncthreads();
-    }
-    const double csum = red[0][0];
-    __syncthreads();
-    const double M = csum + (double)d * pv, Mi = 1. / M;
-    double a = 0., b = 0.;
-    for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) {
-        const double cj = (double)cr[j];
-        float t;
-        if (msr == MCG_SQRL2) {
-            t = (float)cj;
-            a += cj * cj;
-        } else {
-            const double ct = (cj + pv) * Mi;
-            if (msr == MCG_REVERSE_MKL) {
-                const double l = log(ct);
-                t = (float)l;
-                a += l;
-            } else {
-                t = (float)ct;
-                a += ct * log(ct);
-                b += ct;
-            }
-        }
-        CT[(uint64_t)j * kpad + c] = t;
-    }
-    red[0][threadIdx.x] = a;
-    red[1][threadIdx.x] = b;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + o];
-            red[1][threadIdx.x] += red[1][threadIdx.x + o];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        cs[2 * c] = red[0][0];
-        cs[2 * c + 1] = red[1][0];
-    }
+    center_table_body<256>(msr, C + (uint64_t)c * d, c, kpad, d, pv, CT, cs, red);
 }
 
 // per-row terms: {L, log L, H_x, li}
@@ -119,27 +73,15 @@ __device__ __forceinline__ float jsd_h(float a, float b) {
     return 0.5f * (a * __logf(2.f * a / m) + b * __logf(2.f * b / m));
 }
 
-// G[u][c] = sum_j h(li_u, c~_j), FP64 accumulation.  One wave per (u, c).
+// G[u][c] = sum_j h(li_u, c~_j), FP64 accumulation.  One block per (u, c), blocks
+// striding over the pairs.
 __global__ __launch_bounds__(256) void k_jsd_g(const float *__restrict__ CT, uint32_t kpad, uint32_t k, uint32_t d,
                                                const float *__restrict__ lis, uint32_t U, double *__restrict__ G) {
-    // one block per (u, c), blocks striding over the pairs; lanes over the columns,
-    // then the four wave sums in a fixed order
     __shared__ double ws[4];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint64_t pid = blockIdx.x; pid < (uint64_t)U * k; pid += gridDim.x) {
         const uint32_t u = (uint32_t)(pid / k), c = (uint32_t)(pid % k);
-        const double l = lis[u];
-        double s = 0.;
-        for (uint32_t j = threadIdx.x; j < d; j += 256) {
-            const double b = CT[(uint64_t)j * kpad + c];
-            const double m = l + b;
-            s += 0.5 * (l * log(2. * l / m) + b * log(2. * b / m));
-        }
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-        if (lane == 0) ws[wave] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) G[(uint64_t)u * kpad + c] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
-        __syncthreads();
+        const double g = jsd_g_body<256>(CT, kpad, c, d, (double)lis[u], ws);
+        if (threadIdx.x == 0) G[(uint64_t)u * kpad + c] = g;
     }
 }
 
@@ -363,27 +305,29 @@ __global__ __launch_bounds__(256) void k_pp_cc_gen(Cost1 P, const uint32_t *__re
     if (lane == 0) cc[a] = v;
 }
 
-// D^2 seeding update (kmeans.h:189-204).  A wave takes 64 rows at a time: each lane
-// tests its row for pruning and loads its row's extent and per-row terms, then the
-// rows still needed are costed by groups of GS lanes (64 / GS rows at once).  With
+// D^2 seeding update (kmeans.h:189-204).  A wave takes ROWS (64, or 64 / GS when the
+// rows are few, so that more waves share them) rows at a time: each lane tests its
+// row for pruning and loads its row's extent and per-row terms, then the rows still
+// needed are costed by groups of GS lanes (64 / GS rows at once).  With
 // PRUNE (JSD, whose square root is a metric on the smoothed distributions): a row
 // whose center is far from the new one, JSD(c_a, c_new) > 4 dist (1 + 2^-6) + 2^-17,
 // cannot get closer (triangle inequality for sqrt(JSD); the margins cover the FP32
 // evaluation of both costs), so its entries are not read and nothing changes.
-template <int MSR, bool PRUNE, int GS>
+template <int MSR, bool PRUNE, int GS, int ROWS>
 __global__ __launch_bounds__(256) void k_pp_update_gen(Cost1 P, uint64_t n, const uint64_t *__restrict__ newc_p,
                                                        uint32_t ci, const float *__restrict__ cc,
                                                        float *__restrict__ dist, uint32_t *__restrict__ asn,
                                                        int first, const double *__restrict__ w,
                                                        unsigned long long *__restrict__ mx) {
-    // waves stride over tiles of 64 rows; one block-wide maximum and one atomic per block
+    // waves stride over tiles of ROWS rows; one block-wide maximum and one atomic per block
+    static_assert(ROWS == 64 || ROWS == 64 / GS, "a tile is 64 rows or one row per lane group");
     const uint64_t nwaves = (uint64_t)gridDim.x * 4;
     const int lane = threadIdx.x & 63, g = lane / GS, gl = lane % GS;
     const uint64_t newc = *newc_p;
     double a = 0.;
-    for (uint64_t base = ((uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64; base < n; base += nwaves * 64) {
+    for (uint64_t base = ((uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * ROWS; base < n; base += nwaves * ROWS) {
         const uint64_t row = base + lane;
-        bool need = row < n;
+        bool need = lane < ROWS && row < n;
         if (PRUNE && need && !first && row != newc) {
             const float v = dist[row];
             need = v > 0.f && !(cc[asn[row]] > 4.f * v * (1.f + 1.f / 64.f) + 0x1p-17f);
@@ -1081,34 +1025,44 @@ static Cost1 center_tables(Data &ds, int msr, double pv, const float *C, uint64_
     return P;
 }
 
-template <int GS>
-static void pp_update_launch(const Cost1 &P, uint64_t n, int msr, const uint64_t *newc, uint32_t ci, const float *cc,
-                             float *dist, uint32_t *asn, int first, const double *w, unsigned long long *mx,
-                             hipStream_t st) {
-    const dim3 grid((uint32_t)std::min<uint64_t>(cdiv(n, 256), 2048)), blk(256);
+template <int GS, int ROWS>
+static void pp_update_launch_r(const Cost1 &P, uint64_t n, int msr, const uint64_t *newc, uint32_t ci,
+                               const float *cc, float *dist, uint32_t *asn, int first, const double *w,
+                               unsigned long long *mx, hipStream_t st) {
+    const dim3 grid((uint32_t)std::min<uint64_t>(cdiv(n, 4 * ROWS), ROWS == 64 ? 2048 : 8192)), blk(256);
     if (msr == MCG_JSD && !first && ci > 0) {
-        hipLaunchKernelGGL((k_pp_update_gen<MCG_JSD, true, GS>), grid, blk, 0, st, P, n, newc, ci, cc, dist, asn,
-                           first, w, mx);
+        hipLaunchKernelGGL((k_pp_update_gen<MCG_JSD, true, GS, ROWS>), grid, blk, 0, st, P, n, newc, ci, cc, dist,
+                           asn, first, w, mx);
         return;
     }
     switch (msr) {
     case MCG_SQRL2:
-        hipLaunchKernelGGL((k_pp_update_gen<MCG_SQRL2, false, GS>), grid, blk, 0, st, P, n, newc, ci, cc, dist, asn,
-                           first, w, mx);
-        break;
-    case MCG_MKL:
-        hipLaunchKernelGGL((k_pp_update_gen<MCG_MKL, false, GS>), grid, blk, 0, st, P, n, newc, ci, cc, dist, asn,
-                           first, w, mx);
-        break;
-    case MCG_REVERSE_MKL:
-        hipLaunchKernelGGL((k_pp_update_gen<MCG_REVERSE_MKL, false, GS>), grid, blk, 0, st, P, n, newc, ci, cc, dist,
+        hipLaunchKernelGGL((k_pp_update_gen<MCG_SQRL2, false, GS, ROWS>), grid, blk, 0, st, P, n, newc, ci, cc, dist,
                            asn, first, w, mx);
         break;
+    case MCG_MKL:
+        hipLaunchKernelGGL((k_pp_update_gen<MCG_MKL, false, GS, ROWS>), grid, blk, 0, st, P, n, newc, ci, cc, dist,
+                           asn, first, w, mx);
+        break;
+    case MCG_REVERSE_MKL:
+        hipLaunchKernelGGL((k_pp_update_gen<MCG_REVERSE_MKL, false, GS, ROWS>), grid, blk, 0, st, P, n, newc, ci, cc,
+                           dist, asn, first, w, mx);
+        break;
     default:
-        hipLaunchKernelGGL((k_pp_update_gen<MCG_JSD, false, GS>), grid, blk, 0, st, P, n, newc, ci, cc, dist, asn,
-                           first, w, mx);
+        hipLaunchKernelGGL((k_pp_update_gen<MCG_JSD, false, GS, ROWS>), grid, blk, 0, st, P, n, newc, ci, cc, dist,
+                           asn, first, w, mx);
         break;
     }
+}
+
+// Fewer than 2^18 rows: a wave takes one row per lane group at a time, so that the
+// rows' dependent loads are spread over more waves.
+template <int GS>
+static void pp_update_launch(const Cost1 &P, uint64_t n, int msr, const uint64_t *newc, uint32_t ci, const float *cc,
+                             float *dist, uint32_t *asn, int first, const double *w, unsigned long long *mx,
+                             hipStream_t st) {
+    if (n < (1ull << 18)) pp_update_launch_r<GS, 64 / GS>(P, n, msr, newc, ci, cc, dist, asn, first, w, mx, st);
+    else pp_update_launch_r<GS, 64>(P, n, msr, newc, ci, cc, dist, asn, first, w, mx, st);
 }
 
 bool generic_pp_round(Data &ds, int msr, double prior, const float *center_row, const uint32_t *cent, uint32_t ci,
@@ -1124,6 +1078,53 @@ bool generic_pp_round(Data &ds, int msr, double prior, const float *center_row, 
     if (ds.csr) pp_update_launch<8>(P, n, msr, newc, ci, cc, dist, asn, first, w, mx, st);
     else pp_update_launch<64>(P, n, msr, newc, ci, cc, dist, asn, first, w, mx, st);
     return true;
+}
+
+bool generic_pp_tables(Data &ds, int msr, double prior, PPTables &t, hipStream_t st) {
+    if (!msr_decomposed(msr) || (msr != MCG_SQRL2 && prior < DIRECT_PRIOR)) return false;
+    Ctx &c = *ds.ctx;
+    const double pv = msr == MCG_SQRL2 ? 0. : prior;
+    t.pv = pv;
+    t.CT = (float *)c.s[S_CT].reserve(ds.d * sizeof(float));
+    t.cs = (double *)c.s[S_CPARAM].reserve(2 * sizeof(double));
+    t.G = nullptr;
+    t.lis = nullptr;
+    t.U = 0;
+    if (msr != MCG_SQRL2) csr_transforms(ds, pv, st);
+    if (msr == MCG_JSD) {
+        t.U = (uint32_t)ds.li_unique.size();
+        REQUIRE(t.U <= (1u << 20), "JSD: too many distinct row sums");
+        t.G = (double *)c.s[S_JSDG].reserve((uint64_t)t.U * sizeof(double));
+        t.lis = ds.li_dev.as<const float>();
+    }
+    return true;
+}
+
+void generic_pp_next(Data &ds, int msr, const PPTables &t, const uint32_t *cent, uint32_t ci, const uint64_t *newc,
+                     float *cc, float *dist, uint32_t *asn, const double *w, unsigned long long *mx,
+                     hipStream_t st) {
+    Cost1 P{};
+    P.X = ds.csr ? nullptr : ds.X;
+    P.val = ds.csr ? ds.val.as<const float>() : nullptr;
+    P.idx = ds.csr ? ds.idx.as<const uint32_t>() : nullptr;
+    P.ptr = ds.csr ? ds.ptr.as<const uint64_t>() : nullptr;
+    P.d = (uint32_t)ds.d;
+    P.pv = (float)t.pv;
+    P.logpv = (float)log(t.pv > 0 ? t.pv : 1.);
+    P.kpad = 1;
+    P.CT = t.CT;
+    P.cs = t.cs;
+    if (msr != MCG_SQRL2) P.rt = ds.rowinfo.as<const float4>();
+    if (msr == MCG_JSD) {
+        if (t.U > PP_GMAX)
+            hipLaunchKernelGGL(k_jsd_g, dim3((uint32_t)std::min<uint64_t>(t.U, 65536)), dim3(256), 0, st, t.CT, 1u, 1u,
+                               (uint32_t)ds.d, t.lis, t.U, t.G);
+        P.G = t.G;
+        P.slot = ds.li_slot.as<const uint32_t>();
+        hipLaunchKernelGGL(k_pp_cc_gen<MCG_JSD>, dim3((uint32_t)cdiv(ci, 4)), dim3(256), 0, st, P, cent, ci, cc);
+    }
+    if (ds.csr) pp_update_launch<8>(P, ds.n, msr, newc, ci, cc, dist, asn, 0, w, mx, st);
+    else pp_update_launch<64>(P, ds.n, msr, newc, ci, cc, dist, asn, 0, w, mx, st);
 }
 
 // MCG_BREG_MFMA=0 keeps dense MKL / REVERSE_MKL on the row kernel (for comparisons)

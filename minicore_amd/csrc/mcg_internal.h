@@ -111,6 +111,7 @@ struct Data {
     // 16-bit copies (BF16, FP16) for the MFMA cross term: rows centered on the column
     // mean mu, padded to dpad columns; ||x - mu|| per row; max over rows.
     bool base_ready = false, bf16_ready = false, f16_ready = false;
+    bool oneshot = false;   // one E-step is asked of these rows (a server's final assignment)
     uint32_t dpad = 0;
     DBuf Xb, mu, xnorm;
     DBuf xdev;         // ||bf16(x - mu) - (x - mu)|| per row (rounded up)
@@ -454,6 +455,119 @@ __device__ __forceinline__ double pp_weighted(float v, const double *__restrict_
 bool generic_pp_round(Data &ds, int msr, double prior, const float *center_row, const uint32_t *cent, uint32_t ci,
                       const uint64_t *newc, float *cc, float *dist, uint32_t *asn, int first, const double *w,
                       unsigned long long *mx, hipStream_t st);
+
+// The single-center tables of the seeding rounds after the first, filled on the
+// device by the round's choice kernel (the table of the new center row; JSD's G for
+// U <= PP_GMAX distinct li values, else by k_jsd_g in generic_pp_next).
+constexpr uint32_t PP_GMAX = 8;
+struct PPTables {
+    float *CT;           // d floats
+    double *cs;          // 2 doubles
+    double *G;           // U doubles (JSD)
+    const float *lis;    // U distinct li values (JSD)
+    uint32_t U;
+    double pv;
+};
+// false when the measure and prior take the general cost path (no tables)
+bool generic_pp_tables(Data &ds, int msr, double prior, PPTables &t, hipStream_t st);
+// a round after the first whose tables the choice kernel filled: JSD's G when U >
+// PP_GMAX, the center-to-center costs (JSD), the update of dist / asn / mx
+void generic_pp_next(Data &ds, int msr, const PPTables &t, const uint32_t *cent, uint32_t ci, const uint64_t *newc,
+                     float *cc, float *dist, uint32_t *asn, const double *w, unsigned long long *mx,
+                     hipStream_t st);
+
+// Table of one center row cr for the decomposed measures (applicator.h:1632-1915 through
+// the decompositions): CT[j * kpad + c] and cs[2c], cs[2c + 1].  Computed by threads
+// 0..NT-1 of the block (larger blocks: the other threads only take part in the
+// barriers); red: 2 x NT doubles of LDS.
+template <int NT>
+__device__ __forceinline__ void center_table_body(int msr, const float *__restrict__ cr, uint32_t c, uint32_t kpad,
+                                                  uint32_t d, double pv, float *__restrict__ CT,
+                                                  double *__restrict__ cs, double (*red)[NT]) {
+    const uint32_t t = threadIdx.x;
+    const bool on = t < NT;
+    double s = 0.;
+    if (on)
+        for (uint32_t j = t; j < d; j += NT) s += (double)cr[j];
+    if (on) red[0][t] = s;
+    __syncthreads();
+    for (int o = NT / 2; o > 0; o >>= 1) {
+        if ((int)t < o) red[0][t] += red[0][t + o];
+        __syncthreads();
+    }
+    const double csum = red[0][0];
+    __syncthreads();
+    const double M = csum + (double)d * pv, Mi = 1. / M;
+    double a = 0., b = 0.;
+    if (on)
+        for (uint32_t j = t; j < d; j += NT) {
+            const double cj = (double)cr[j];
+            float v;
+            // explicit fma: the same arithmetic in every translation unit, whatever its
+            // contraction setting
+            if (msr == MCG_SQRL2) {
+                v = (float)cj;
+                a = fma(cj, cj, a);
+            } else {
+                const double ct = (cj + pv) * Mi;
+                if (msr == MCG_REVERSE_MKL) {
+                    const double l = log(ct);
+                    v = (float)l;
+                    a += l;
+                } else {
+                    v = (float)ct;
+                    if (msr == MCG_MKL) {   // JSD reads only the table
+                        a = fma(ct, log(ct), a);
+                        b += ct;
+                    }
+                }
+            }
+            CT[(uint64_t)j * kpad + c] = v;
+        }
+    if (on) {
+        red[0][t] = a;
+        red[1][t] = b;
+    }
+    __syncthreads();
+    for (int o = NT / 2; o > 0; o >>= 1) {
+        if ((int)t < o) {
+            red[0][t] += red[0][t + o];
+            red[1][t] += red[1][t + o];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        cs[2 * c] = red[0][0];
+        cs[2 * c + 1] = red[1][0];
+    }
+    __syncthreads();
+}
+
+// JSD's G = sum_j h(l, CT[j * kpad + c]) in FP64 by threads 0..NT-1 of the block:
+// lanes over the columns, then the NT / 64 wave sums in a fixed order (pairs, then
+// pairs of pairs); ws: NT / 64 doubles of LDS.
+template <int NT>
+__device__ __forceinline__ double jsd_g_body(const float *__restrict__ CT, uint32_t kpad, uint32_t c, uint32_t d,
+                                             double l, double *ws) {
+    static_assert(NT == 256 || NT == 1024, "256 or 1024 threads");
+    const uint32_t t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    double s = 0.;
+    if (t < NT)
+        for (uint32_t j = t; j < d; j += NT) {
+            const double b = CT[(uint64_t)j * kpad + c];
+            const double m = l + b;
+            s = fma(0.5, fma(l, log(2. * l / m), b * log(2. * b / m)), s);
+        }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0 && t < NT) ws[wave] = s;
+    __syncthreads();
+    double g = ((ws[0] + ws[1]) + ws[2]) + ws[3];
+    if (NT == 1024)
+        for (int q = 4; q < 16; q += 4) g += ((ws[q] + ws[q + 1]) + ws[q + 2]) + ws[q + 3];
+    __syncthreads();
+    return g;
+}
 
 }  // namespace mcg
 

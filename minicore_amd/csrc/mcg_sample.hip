@@ -106,18 +106,33 @@ __global__ void k_fixed_blocksum(const T *__restrict__ v, const double *__restri
     if (threadIdx.x == 0) bsum[blockIdx.x] = sm[0];
 }
 
-// Inclusive scan of one u64 per thread over a 1024-thread block (Hillis-Steele in
-// LDS); integer sums, so the association does not matter.
+// Inclusive scan of one u64 per thread over a 1024-thread block: a shuffle scan in
+// each wave, then the 16 wave totals scanned by the first wave; integer sums, so the
+// association does not matter.  sm: 1024 u64 of LDS; sm[1023] holds the block total
+// on return.
 __device__ __forceinline__ unsigned long long block_scan_u64(unsigned long long x, unsigned long long *sm) {
-    sm[threadIdx.x] = x;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        const unsigned long long t = (int)threadIdx.x >= o ? sm[threadIdx.x - o] : 0ull;
-        __syncthreads();
-        sm[threadIdx.x] += t;
-        __syncthreads();
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned long long y = __shfl_up(x, o);
+        if (lane >= o) x += y;
     }
-    return sm[threadIdx.x];
+    if (lane == 63) sm[wave] = x;
+    __syncthreads();
+    if (wave == 0) {
+        unsigned long long v = lane < 16 ? sm[lane] : 0ull;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            const unsigned long long y = __shfl_up(v, o);
+            if (lane >= o) v += y;
+        }
+        if (lane < 16) sm[16 + lane] = v;
+    }
+    __syncthreads();
+    if (wave > 0) x += sm[16 + wave - 1];
+    if (t == 1023) sm[1023] = x;
+    __syncthreads();
+    return x;
 }
 
 // Single block of 1024 threads: choose the index for raw draw r; writes *out.  The
@@ -347,27 +362,128 @@ __global__ void k_pp_fallback(const float *__restrict__ dist, uint64_t n, uint64
     if (threadIdx.x == 0) { *newc_p = bi[0]; centers[ci] = (uint32_t)bi[0]; }
 }
 
-// One seeding round's choice (kmeans.h:115-186), one block of 1024 threads: the
-// fixed-point draw r over w * dist (as k_fixed_select); if the drawn point has
-// distance 0, the first index of the maximum distance instead; writes *newc_p and
-// centers[ci], then clears *mx for the next update's maximum.
-__global__ __launch_bounds__(1024) void k_pp_select(const float *__restrict__ dist, const double *__restrict__ w,
-                                                    uint64_t n, unsigned long long *__restrict__ mx,
-                                                    const uint64_t *__restrict__ bsum, uint32_t nblk, uint64_t r,
-                                                    uint64_t *__restrict__ newc_p, uint32_t *__restrict__ centers,
-                                                    uint32_t ci) {
+// FP64 squared distance between dense rows ra and rb by a group of 16 lanes (l16 =
+// the lane's index in it; on: the group has a pair): lanes over the dims (float4
+// when vec), then a shuffle tree inside the group.  Every lane of the group returns
+// the value; all lanes of the wave take part in the shuffles.
+__device__ __forceinline__ double row_dist_l2_16(const float *__restrict__ X, uint32_t d, int vec, uint64_t ra,
+                                                 uint64_t rb, int l16, bool on) {
+    double s = 0.;
+    if (on) {
+        const float *x = X + ra * d, *y = X + rb * d;
+        if (vec) {
+            const float4 *x4 = (const float4 *)x, *y4 = (const float4 *)y;
+            for (uint32_t j = l16; j < (d >> 2); j += 16) {
+                const float4 p = x4[j], q = y4[j];
+                double t;
+                t = (double)p.x - (double)q.x; s += t * t;
+                t = (double)p.y - (double)q.y; s += t * t;
+                t = (double)p.z - (double)q.z; s += t * t;
+                t = (double)p.w - (double)q.w; s += t * t;
+            }
+        } else {
+            for (uint32_t j = l16; j < d; j += 16) {
+                const double t = (double)x[j] - (double)y[j];
+                s += t * t;
+            }
+        }
+    }
+    for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    return s;
+}
+
+// Squared distances (FP64) between the new center's row and each earlier center's
+// row, for rounds with many earlier centers: 16 lanes per earlier center q < ci.
+__global__ __launch_bounds__(256) void k_pp_cc(const float *__restrict__ X, uint32_t d, int vec,
+                                               const uint32_t *__restrict__ centers, uint32_t ci,
+                                               const uint64_t *__restrict__ newc_p, double *__restrict__ cc) {
+    const uint32_t q = blockIdx.x * 16 + (threadIdx.x >> 4);
+    const int l16 = threadIdx.x & 15;
+    const bool on = q < ci;
+    const double v = row_dist_l2_16(X, d, vec, on ? centers[q] : 0u, *newc_p, l16, on);
+    if (on && l16 == 0) cc[q] = v;
+}
+
+// Arguments of the fused choice kernel of a seeding round.
+struct PickArgs {
+    const float *dist;
+    const double *w;
+    uint64_t n;
+    unsigned long long *mx;   // weighted maximum of the last update; cleared for the next
+    uint64_t *bsum;           // per-block fixed-point sums
+    uint32_t nblk;
+    uint32_t *ctr;            // blocks done (the last block resets it)
+    uint64_t *newc;
+    uint32_t *centers;
+    const float *X;           // dense rows (MODE 0; MODE 1 with val == nullptr)
+    uint32_t d;
+    double *cc;               // MODE 0: FP64 center-to-center squared distances (here when ci <= ccmax)
+    uint32_t ccmax;
+    int vec;                  // rows as float4
+    const float *val;         // MODE 1: CSR rows
+    const uint32_t *idx;
+    const uint64_t *ptr;
+    float *row;               // MODE 1: the new center as a dense row
+    int msr;                  // MODE 1 with tables: the measure, prior and tables
+    int tables;
+    double pv;
+    float *CT;
+    double *cs;
+    const float *lis;
+    uint32_t gU;              // JSD: G for these many li values here (0: none)
+    double *G;
+};
+
+// One seeding round's choice (kmeans.h:115-186) fused with the block sums it needs.
+// Each block of 1024 threads adds its FX_BLK fixed-point weights w * dist into
+// bsum[b]; the last block to finish (a device-scope counter) reads the block sums
+// back through the device-coherent path and draws: the fixed-point draw r over w *
+// dist, the first index of the maximum distance instead if the drawn point has
+// distance 0.  It writes *newc and centers[ci], clears *mx and the counter, and
+// prepares the round's update: MODE 0 (dense SQRL2 rows) the FP64 squared distances
+// from the new center's row to the earlier centers' rows; MODE 1 (other measures)
+// the new center as a dense row and, with tables, its single-center table and JSD's
+// G for the first gU distinct li values.  The draw is that of the block sums and
+// select of separate kernels (integer sums: the same index).
+template <int MODE>
+__global__ __launch_bounds__(1024) void k_pp_pick(PickArgs a, uint64_t r, uint32_t ci) {
     __shared__ unsigned long long sm[1024];
     __shared__ unsigned long long sel_u;
-    __shared__ uint32_t sel_b;
+    __shared__ uint32_t sel_b, last;
     __shared__ uint64_t chosen;
     __shared__ float bv[1024];
     __shared__ uint64_t bi[1024];
-    const int s = fixed_shift(__longlong_as_double((long long)*mx), n);
+    __shared__ double red[2][1024];
+    __shared__ double ws[16];
     const uint32_t t = threadIdx.x;
-    const uint32_t per = (nblk + 1023) / 1024;
-    const uint32_t b0 = t * per, b1 = min(nblk, b0 + per);
+    const int s = fixed_shift(__longlong_as_double((long long)*a.mx), a.n);
+    {
+        const uint64_t i = (uint64_t)blockIdx.x * FX_BLK + t;
+        unsigned long long f = 0;
+        if (i < a.n) {
+            double v = (double)a.dist[i];
+            if (a.w) v = a.w[i] * v;
+            f = to_fixed(v, s);
+        }
+        for (int o = 32; o > 0; o >>= 1) f += __shfl_xor(f, o);
+        if ((t & 63) == 0) sm[t >> 6] = f;
+        __syncthreads();
+        if (t == 0) {
+            unsigned long long tot = 0;
+            for (int q = 0; q < 16; ++q) tot += sm[q];
+            a.bsum[blockIdx.x] = tot;
+            __threadfence();
+            last = atomicAdd(a.ctr, 1u) == a.nblk - 1 ? 1u : 0u;
+        }
+        __syncthreads();
+        if (!last) return;
+        __threadfence();
+    }
+    // the last block: every block's sum is written
+    const uint32_t per = (a.nblk + 1023) / 1024;
+    const uint32_t b0 = t * per, b1 = min(a.nblk, b0 + per);
     unsigned long long acc = 0;
-    for (uint32_t b = b0; b < b1; ++b) acc += bsum[b];
+    for (uint32_t b = b0; b < b1; ++b) acc += __hip_atomic_load(&a.bsum[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned long long incl = block_scan_u64(acc, sm);
     const unsigned long long tot = sm[1023];
     if (t == 0) chosen = 0;
@@ -378,8 +494,9 @@ __global__ __launch_bounds__(1024) void k_pp_select(const float *__restrict__ di
             unsigned long long run = incl - acc;
             uint32_t blk = b1 - 1;
             for (uint32_t b = b0; b < b1; ++b) {
-                if (run + bsum[b] > u) { blk = b; break; }
-                run += bsum[b];
+                const unsigned long long bs = __hip_atomic_load(&a.bsum[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (run + bs > u) { blk = b; break; }
+                run += bs;
             }
             sel_b = blk;
             sel_u = u - run;
@@ -387,21 +504,21 @@ __global__ __launch_bounds__(1024) void k_pp_select(const float *__restrict__ di
         __syncthreads();
         const uint64_t i = (uint64_t)sel_b * FX_BLK + t;
         unsigned long long f = 0;
-        if (i < n) {
-            double a = (double)dist[i];
-            if (w) a = w[i] * a;
-            f = to_fixed(a, s);
+        if (i < a.n) {
+            double v = (double)a.dist[i];
+            if (a.w) v = a.w[i] * v;
+            f = to_fixed(v, s);
         }
         const unsigned long long pre = block_scan_u64(f, sm);
         if (pre > sel_u && pre - f <= sel_u) chosen = i;
         __syncthreads();
     }
     // distance 0 (or all distances 0): first index of the maximum distance
-    if (dist[chosen] == 0.f) {
+    if (a.dist[chosen] == 0.f) {
         float m = -1.f;
         uint64_t mi = 0;
-        for (uint64_t i = t; i < n; i += 1024)
-            if (dist[i] > m) { m = dist[i]; mi = i; }
+        for (uint64_t i = t; i < a.n; i += 1024)
+            if (a.dist[i] > m) { m = a.dist[i]; mi = i; }
         bv[t] = m;
         bi[t] = mi;
         __syncthreads();
@@ -416,29 +533,39 @@ __global__ __launch_bounds__(1024) void k_pp_select(const float *__restrict__ di
         if (t == 0) chosen = bi[0];
         __syncthreads();
     }
+    const uint64_t c = chosen;
+    if (MODE == 0) {
+        // 16 lanes per earlier center q < ci (with more, k_pp_cc follows)
+        if (ci <= a.ccmax)
+            for (uint32_t q0 = 0; q0 < ci; q0 += 64) {
+                const uint32_t q = q0 + (t >> 4);
+                const bool on = q < ci;
+                const double v = row_dist_l2_16(a.X, a.d, a.vec, on ? a.centers[q] : 0u, c, (int)(t & 15), on);
+                if (on && (t & 15) == 0) a.cc[q] = v;
+            }
+    } else {
+        if (!a.val) {
+            for (uint32_t j = t; j < a.d; j += 1024) a.row[j] = a.X[c * a.d + j];
+        } else {
+            for (uint32_t j = t; j < a.d; j += 1024) a.row[j] = 0.f;
+            __syncthreads();
+            for (uint64_t p = a.ptr[c] + t; p < a.ptr[c + 1]; p += 1024) a.row[a.idx[p]] = a.val[p];
+        }
+        __syncthreads();
+        if (a.tables) {
+            center_table_body<1024>(a.msr, a.row, 0u, 1u, a.d, a.pv, a.CT, a.cs, red);
+            for (uint32_t u = 0; u < a.gU; ++u) {
+                const double g = jsd_g_body<1024>(a.CT, 1u, 0u, a.d, (double)a.lis[u], ws);
+                if (t == 0) a.G[u] = g;
+            }
+        }
+    }
     if (t == 0) {
-        *newc_p = chosen;
-        centers[ci] = (uint32_t)chosen;
-        *mx = 0ull;
+        *a.newc = c;
+        a.centers[ci] = (uint32_t)c;
+        *a.mx = 0ull;
+        *a.ctr = 0u;
     }
-}
-
-// Squared distance (FP64) between the new center's row and each earlier center's
-// row: one wave per earlier center a < ci.
-__global__ __launch_bounds__(256) void k_pp_cc(const float *__restrict__ X, uint32_t d,
-                                               const uint32_t *__restrict__ centers, uint32_t ci,
-                                               const uint64_t *__restrict__ newc_p, double *__restrict__ cc) {
-    const uint32_t a = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (a >= ci) return;
-    const float *x = X + (uint64_t)centers[a] * d, *y = X + *newc_p * d;
-    double s = 0.;
-    for (uint32_t j = lane; j < d; j += 64) {
-        const double t = (double)x[j] - (double)y[j];
-        s += t * t;
-    }
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (lane == 0) cc[a] = s;
 }
 
 // Dense SQRL2 update after the first round, one thread per point.  A point whose
@@ -540,29 +667,71 @@ void kmeanspp_run(Data &ds, int msr, double prior, uint64_t k, uint64_t seed, co
                                dist_d, asn_d, first, w, mx);
         }
     };
+    uint32_t *ctr = (uint32_t *)(misc + 16);
     uint64_t j = 0;
     const uint64_t fc = wyrand_at(seed, j++) % n;
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, newc, fc);
-    fill_u32((uint32_t *)mx, 0, 2, st);   // later rounds: k_pp_select clears it
+    fill_u32((uint32_t *)mx, 0, 3, st);   // mx and the choice kernel's counter; later rounds: it clears both
     update(0, 1);
     centers_h[0] = (uint32_t)fc;
     fill_u32(cent, (uint32_t)fc, 1, st);
-    // dense SQRL2 rounds after the first prune by the center-center distances
+    // rounds after the first: the choice kernel (block sums, draw, the round's
+    // preparation), then the update; dense SQRL2 prunes by the center-center distances
+    PickArgs pa{};
+    pa.dist = dist_d;
+    pa.w = w;
+    pa.n = n;
+    pa.mx = mx;
+    pa.bsum = bsum;
+    pa.nblk = nblk;
+    pa.ctr = ctr;
+    pa.newc = newc;
+    pa.centers = cent;
+    pa.X = ds.csr ? nullptr : ds.X;
+    pa.d = d;
+    PPTables tb{};
+    bool tables = false;
     const int vec = (d % 4 == 0) && ((uintptr_t)ds.X % 16 == 0);
-    double *cc = (double *)ccbuf;
+    // the choice kernel's one block computes the center-center distances of the
+    // first rounds; later rounds spread them over a kernel of their own
+    constexpr uint32_t CC_IN_PICK = 128;
+    if (l2) {
+        pa.cc = (double *)ccbuf;
+        pa.ccmax = CC_IN_PICK;
+        pa.vec = vec;
+    } else {
+        pa.val = ds.csr ? ds.val.as<const float>() : nullptr;
+        pa.idx = ds.csr ? ds.idx.as<const uint32_t>() : nullptr;
+        pa.ptr = ds.csr ? ds.ptr.as<const uint64_t>() : nullptr;
+        pa.row = row;
+        tables = generic_pp_tables(ds, msr, prior, tb, st);
+        pa.tables = tables ? 1 : 0;
+        pa.msr = msr;
+        pa.pv = tb.pv;
+        pa.CT = tb.CT;
+        pa.cs = tb.cs;
+        pa.lis = tb.lis;
+        pa.gU = tb.U <= PP_GMAX ? tb.U : 0u;
+        pa.G = tb.G;
+    }
     for (uint64_t ci = 1; ci < k; ++ci) {
         const uint64_t r = wyrand_at(seed, j++);
-        hipLaunchKernelGGL(k_fixed_blocksum<float>, dim3(nblk), dim3(256), 0, st, (const float *)dist_d, w, n,
-                           (const unsigned long long *)mx, bsum);
-        hipLaunchKernelGGL(k_pp_select, dim3(1), dim3(1024), 0, st, (const float *)dist_d, w, n, mx,
-                           (const uint64_t *)bsum, nblk, r, newc, cent, (uint32_t)ci);
         if (l2) {
-            hipLaunchKernelGGL(k_pp_cc, dim3((uint32_t)cdiv(ci, 4)), dim3(256), 0, st, ds.X, d,
-                               (const uint32_t *)cent, (uint32_t)ci, newc, cc);
+            hipLaunchKernelGGL(k_pp_pick<0>, dim3(nblk), dim3(1024), 0, st, pa, r, (uint32_t)ci);
+            if (ci > CC_IN_PICK)
+                hipLaunchKernelGGL(k_pp_cc, dim3((uint32_t)cdiv(ci, 16)), dim3(256), 0, st, ds.X, d, vec,
+                                   (const uint32_t *)cent, (uint32_t)ci, (const uint64_t *)newc, (double *)ccbuf);
             hipLaunchKernelGGL(k_pp_update_l2_pruned, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, ds.X, n, d,
-                               vec, newc, (uint32_t)ci, cc, dist_d, asn_d, w, mx);
+                               vec, newc, (uint32_t)ci, (const double *)ccbuf, dist_d, asn_d, w, mx);
         } else {
-            update((uint32_t)ci, 0);
+            hipLaunchKernelGGL(k_pp_pick<1>, dim3(nblk), dim3(1024), 0, st, pa, r, (uint32_t)ci);
+            if (tables) {
+                generic_pp_next(ds, msr, tb, cent, (uint32_t)ci, newc, (float *)ccbuf, dist_d, asn_d, w, mx, st);
+            } else {
+                generic_costs(ds, msr, prior, row, 1, nullptr, asn1, cost1, st);
+                hipLaunchKernelGGL(k_pp_update_from, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, cost1, n, newc,
+                                   (uint32_t)ci, dist_d, asn_d, 0, w, mx);
+            }
         }
     }
     HIP_CHECK(hipMemcpyAsync(centers_h, cent, k * 4, hipMemcpyDeviceToHost, st));

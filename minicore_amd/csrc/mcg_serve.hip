@@ -289,7 +289,8 @@ static bool run_request(Req &r, mcg_ctx *ctx) {
     stage_done(3);
     if (cancelled()) return false;
 
-    // 5. all rows to the final centers
+    // 5. all rows to the final centers (one E-step on these rows)
+    ds.get()->d.oneshot = true;
     Afin.reserve(n * 4);
     Cfin.reserve(n * 4);
     check(mcg_assign_hard(ds.get(), q.msr, q.prior, Cent.as<const float>(), k, q.precision, Afin.as<uint32_t>(),

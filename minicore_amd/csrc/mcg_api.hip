@@ -57,16 +57,7 @@ void Ctx::collect() {
     }
 }
 
-uint32_t *Ctx::refine_host() {
-    if (!refine_counts) {
-        HIP_CHECK(hipHostMalloc((void **)&refine_counts, 4 * sizeof(uint32_t)));
-        refine_counts[0] = refine_counts[1] = 0;
-    }
-    return refine_counts;
-}
-
 Ctx::~Ctx() {
-    if (refine_counts) (void)hipHostFree(refine_counts);
     for (int w = 0; w < 2; ++w)
         for (auto &pr : pending[w]) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
     if (stream) (void)hipStreamDestroy(stream);
@@ -1617,9 +1608,12 @@ int mcg_lloyd_step(mcg_data *ds, int msr, double prior, float *centers, uint64_t
         float *pc = nullptr;
         MStepOut mo;
         const bool l2 = dense_l2(D, msr);
+        // {cost, clusters without members}: written by the M-step's last block straight
+        // into pinned host memory (dense SQRL2), else summarised and copied
+        double *hout = (double *)c.pinned(2 * sizeof(double));
         if (l2) {
             dense_sqrl2_assign(D, centers, k, precision, asn, st);
-            mo = dense_mstep(D, asn, W, k, centers, centers, true, st);
+            mo = dense_mstep(D, asn, W, k, centers, centers, true, st, hout);
         } else {
             pc = (float *)c.s[S_COSTS].reserve(n * 4);
             double *tot = (double *)c.s[S_TOT].reserve(4 * sizeof(double));
@@ -1627,11 +1621,11 @@ int mcg_lloyd_step(mcg_data *ds, int msr, double prior, float *centers, uint64_t
             hipLaunchKernelGGL(k_wsum, dim3(1), dim3(1024), 0, st, pc, W, n, tot);
             mo = D.csr ? csr_mstep(D, asn, W, k, centers, st) : dense_mstep(D, asn, W, k, centers, centers, false, st);
             mo.total = tot;
+            hipLaunchKernelGGL(k_step_summary, dim3(1), dim3(256), 0, st, mo.total, mo.counts, k, res);
+            HIP_CHECK(hipMemcpyAsync(hout, res, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
         }
-        hipLaunchKernelGGL(k_step_summary, dim3(1), dim3(256), 0, st, mo.total, mo.counts, k, res);
-        double h[2] = {0., 0.};
-        HIP_CHECK(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, st));
         sync(c);
+        const double h[2] = {hout[0], hout[1]};
         if (cost) *cost = h[0];
         if (h[1] == 0.) return;
         // reseed empty clusters (rare): host draws, device moves and means
@@ -1675,10 +1669,12 @@ int mcg_lloyd_step(mcg_data *ds, int msr, double prior, float *centers, uint64_t
 int mcg_refine_counts(mcg_ctx *ctx, uint64_t *pairs, uint64_t *full) {
     return guard([&] {
         REQUIRE(ctx, "null ctx");
-        sync(ctx->c);
-        const uint32_t *r = ctx->c.refine_host();
-        if (pairs) *pairs = r[0];
-        if (full) *full = r[1];
+        Ctx &c = ctx->c;
+        uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (c.refine_cnt) copy_sync(h, c.refine_cnt, sizeof(h), hipMemcpyDeviceToHost, c.stream);
+        else sync(c);
+        if (pairs) *pairs = h[4];
+        if (full) *full = h[1];
     });
 }
 

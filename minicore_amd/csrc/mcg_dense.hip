@@ -12,7 +12,7 @@
 //    center's position in its 32-center sub-tile packed into the low 5 mantissa bits
 //    (4 VALU ops per value: v_and_or, two v_med3_u32, v_min_u32).  Points whose
 //    best/second gap is not provably larger than the BF16 error bound get their
-//    candidate centers re-scored exactly (k_collect16, k_exact_cands, k_rescore_full).
+//    candidate centers re-scored exactly (k_collect16, k_exact_cands, k_refine_final).
 //  * FP16: the BF16 form with FP16 operands, power-of-two scaled into FP16's range
 //    (the rows by xscale, the centers by a per-E-step scale): 3 more mantissa bits,
 //    so an 8x smaller rounding error and far fewer near ties to re-score.  The
@@ -329,8 +329,9 @@ __global__ __launch_bounds__(256) void k_prep_centers_h(const float *__restrict_
 // k_prep_centers_h.
 __global__ void k_prep_params(const double *__restrict__ cn, const float2 *__restrict__ cinfo, uint64_t k,
                               uint64_t kpad, const float *__restrict__ xstat, uint32_t d, int f16,
-                              float *__restrict__ cadd, EParams *__restrict__ prm) {
+                              float *__restrict__ cadd, EParams *__restrict__ prm, uint32_t *__restrict__ rcnt) {
     __shared__ double sm[256], se[256];
+    if (threadIdx.x < 8) rcnt[threadIdx.x] = 0u;   // refinement counters of the E-step that follows
     const float xmax = xstat[0], xscale = f16 ? xstat[1] : 1.f;
     double m = 0., me = 0.;
     for (uint64_t i = threadIdx.x; i < k; i += blockDim.x) {
@@ -437,8 +438,10 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
     return x;
 }
 
-// two lists at once: cnt2 = count A (low 16 bits) | count B << 16, block totals < 2^16
-__device__ __forceinline__ uint2 block_append2(uint32_t cnt2, uint32_t *ctr_a, uint32_t *ctr_b, uint32_t *sm) {
+// two lists at once: cnt2 = count A (low 16 bits) | count B << 16, block totals < 2^16;
+// ctr_p counts the pairs of list A (two entries each)
+__device__ __forceinline__ uint2 block_append2(uint32_t cnt2, uint32_t *ctr_a, uint32_t *ctr_b, uint32_t *ctr_p,
+                                               uint32_t *sm) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t x = wave_incl_scan(cnt2);
     if (lane == 63) sm[wave] = x;
@@ -449,6 +452,7 @@ __device__ __forceinline__ uint2 block_append2(uint32_t cnt2, uint32_t *ctr_a, u
         const uint32_t ta = run & 0xFFFFu, tb = run >> 16;
         sm[4] = ta ? atomicAdd(ctr_a, ta) : 0u;
         sm[5] = tb ? atomicAdd(ctr_b, tb) : 0u;
+        if (ta) atomicAdd(ctr_p, ta / 2u);
     }
     __syncthreads();
     const uint32_t excl = sm[wave] + x - cnt2;
@@ -494,7 +498,9 @@ __global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep16(
     __shared__ __attribute__((aligned(16))) char lds1[TILE];
     __shared__ float ladd0[64], ladd1[64];
     __shared__ uint32_t sm_app[8];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // the wave index is uniform: kept scalar, so the per-wave LDS destinations of the
+    // tile copy are SGPRs rather than VGPRs read back with readfirstlane
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int h = lane >> 5, r32 = lane & 31;
     const uint32_t pbase = blockIdx.x * (4u * NB * 32u) + wave * (NB * 32u);
 
@@ -597,33 +603,25 @@ __global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep16(
         uint32_t ab = abase0;
         asm volatile("" : "+v"(ab));
         const char *tb = cur + mb * (32u * CH * 16u);
+        // Each A fragment is read FD steps ahead of its MFMAs, so its LDS latency is
+        // covered by the MFMAs and epilogue work of the steps in between.  The
+        // scheduling barrier after each step keeps that order: without it the
+        // scheduler sinks every read down to its first use and the wave waits for
+        // the LDS at every step.
+        constexpr int FD = 2;
+        V8 fr[FD + 1];
+#pragma unroll
+        for (int s = 0; s < FD && s < KS; ++s) fr[s] = *(const V8 *)(tb + (ab ^ (uint32_t)(s * 16)));
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
-            const V8 a = *(const V8 *)(tb + (ab ^ (uint32_t)(s * 16)));
+            if (s + FD < KS) fr[(s + FD) % (FD + 1)] = *(const V8 *)(tb + (ab ^ (uint32_t)((s + FD) * 16)));
+            const V8 a = fr[s % (FD + 1)];
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb)
                 acc[nb] = mfma16<F16>(a, b[nb][s], s == 0 ? init : acc[nb]);
 #pragma unroll
             for (int i = (s * E) / KS; i < ((s + 1) * E) / KS; ++i) epi_elem(pv, i);
-        }
-        // order: per step one fragment read, then the step's MFMAs, each followed by
-        // its share of the VALU work (epilogue: 4 ops per element; 1 address op)
-        // The 4 reads of the chain's initial values and fragments 0..DEPTH-1 go first;
-        // then each step issues the read of fragment s + DEPTH, so a fragment is in
-        // flight for DEPTH steps before its MFMAs.
-        constexpr int VPS = (4 * E) / KS + 1;   // VALU ops per step
-        constexpr int VA = (VPS + NB - 1) / NB, VB = VPS - VA > 0 ? VPS - VA : 1;
-        constexpr int DEPTH = 3;
-        __builtin_amdgcn_sched_group_barrier(0x100, 4 + DEPTH, 0);
-#pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x002, VA, 0);
-            if constexpr (NB == 2) {
-                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                __builtin_amdgcn_sched_group_barrier(0x002, VB, 0);
-            }
-            if (s + DEPTH < KS) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+            __builtin_amdgcn_sched_barrier(0);
         }
         epi_close(upv);
     };
@@ -693,30 +691,52 @@ __global__ __launch_bounds__(256, (KS <= 16 ? 2 : 1)) void k_estep16(
             // center 0 NaN: every row keeps center 0 (the scan's first cost is NaN); a row
             // with non-finite values: the exact scan over all centers (overflow list)
             asn[p] = 0u;
+            R.oflag[p] = 0u;
             if (!P.c0nan) cand_overflow(R, p);
         } else if (mine) {
             const float xn = xnorm[p], dxn = xdev[p];
             // the values carry the scale P.s (a power of two; 1 for BF16)
             const float dbest = xn * xn + (v1 * P.inv_s - P.off);
-            // Cauchy-Schwarz on the rounding errors of the point (dxn) and of the centers:
-            // |x~.c~ - x.c| <= |dx| |c| + |x| |dc| + |dx| |dc|, for the best center and
-            // for any other (cmax, dbmax), doubled for the -2 x.c cross term
+            // Rounding of the 16-bit operands: with a = x - mu, b = c - mu and their
+            // errors da, db, the approximate difference D~_j - D~_1 is off from the
+            // exact one by at most
+            //   A |b_j - b_1| + Bc,  A = 2 |da|,  Bc = 2 (|a| + |da|) (|db_1| + |db_j|)
+            // (Cauchy-Schwarz on -2 a.b), plus the accumulation terms tau0 and taur D~_1.
+            // Loose form: |b_j - b_1| <= |b_1| + cmax and |db_j| <= dbmax.
             const float2 ci = cinfo[c1 < kreal ? c1 : 0];
-            const float tcs = 2.f * (dxn * (ci.x + P.cmax) + xn * (ci.y + P.dbmax) + dxn * (ci.y + P.dbmax));
-            const float tau = (tcs * (1.f + 1e-3f) + P.tau0 + P.taur * fmaxf(dbest, 0.f)) * P.s;
+            const float A = 2.f * (1.f + 1e-3f) * dxn;
+            const float Bc = 2.f * (1.f + 1e-3f) * (xn + dxn) * (ci.y + P.dbmax) + P.tau0 + P.taur * fmaxf(dbest, 0.f);
+            const float tl = A * (ci.x + P.cmax) + Bc;
+            // Tight form: |b_j - b_1| = |c_j - c_1| <= sqrt(D_1) + sqrt(D_j), each
+            // exact distance at most its approximate value plus tl.  Every center
+            // j ranked at or beyond a value D~ with D~_j - D~_1 - A (R_1 + R_j)
+            // non-decreasing in D~_j (true when 4 tl >= A^2) is then decided by its
+            // bound at D~.
+            const float R1 = sqrtf(fmaxf(dbest, 0.f) + tl);
+            const bool mono = 4.f * tl >= A * A;
+            auto tau_at = [&](float v) __attribute__((always_inline)) {
+                const float dj = xn * xn + (v * P.inv_s - P.off);
+                const float t = A * (R1 + sqrtf(fmaxf(dj, 0.f) + tl)) + Bc;
+                return (mono ? fminf(t, tl) : tl) * P.s;
+            };
+            const float tau2 = tau_at(v2), tau3 = tau_at(v3);
+            // listed points: a center beyond v1 + tau_list cannot win (its bound at
+            // D~_1 + tau_list, with tau_list <= tl)
+            const float tlist = mono ? fminf(A * (R1 + sqrtf(fmaxf(dbest, 0.f) + 2.f * tl)) + Bc, tl) : tl;
             const bool ok1 = c1 < kreal;
             asn[p] = ok1 ? c1 : 0u;
-            if (!(ok1 && v2 - v1 > tau)) {
+            if (!(ok1 && v2 - v1 > tau2)) {
                 R.key[p] = ~0ull;
-                is_pair[nb] = ok1 && c2 < kreal && v3 - v1 > tau;
+                R.oflag[p] = 0u;   // set again below / by the collect pass if its candidates overflow
+                is_pair[nb] = ok1 && c2 < kreal && v3 - v1 > tau3;
                 is_full[nb] = !is_pair[nb];
-                pthr[nb] = ok1 ? v1 + tau : FLT_MAX;   // no finite best: every center is a candidate
+                pthr[nb] = ok1 ? v1 + tlist * P.s : FLT_MAX;   // no finite best: every center is a candidate
             }
         }
         cnt2 += (is_pair[nb] ? 2u : 0u) + (is_full[nb] ? (1u << 16) : 0u);
     }
     // pairs -> two candidates each; the rest -> the list of the collect pass
-    const uint2 base = block_append2(cnt2, &R.cnt[0], &R.cnt[1], sm_app);
+    const uint2 base = block_append2(cnt2, &R.cnt[0], &R.cnt[1], &R.cnt[4], sm_app);
     uint32_t ia = base.x, ib = base.y;
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
@@ -754,7 +774,7 @@ __global__ __launch_bounds__(256) void k_collect16(const uint4 *__restrict__ Xb,
     __shared__ __attribute__((aligned(16))) char lds1[TILE];
     __shared__ float ladd0[64], ladd1[64];
     __shared__ uint32_t sm_app[8];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int h = lane >> 5, r32 = lane & 31;
     const uint32_t count = R.cnt[1];
     const uint32_t t_begin = blockIdx.y * ntiles / gridDim.y, t_end = (blockIdx.y + 1) * ntiles / gridDim.y;
@@ -849,13 +869,12 @@ __global__ __launch_bounds__(256) void k_collect16(const uint4 *__restrict__ Xb,
     }
 }
 
-// Exact cost of each candidate (sequential FMA chain over the dims, the oracle's
-// order), min-reduced per point as (cost bits << 32 | center): lowest cost, then
-// lowest index; the result does not depend on the order of the atomics.  The first
-// 2 * cnt[4] candidates are the main pass's pairs (a point's two candidates side by
-// side): one thread per pair point reads the row once and runs both chains together;
-// then one thread per remaining candidate.  Rows read straight from HBM / L2 with
-// 16-byte loads when vec (d % 4 == 0 and 16-byte aligned rows).
+// Exact cost of each candidate: s = fma(x_j - c_j, x_j - c_j, s) over j = 0..d-1 in
+// FP32 (the oracle's sequential chain), min-reduced per point as (cost bits << 32 |
+// center): lowest cost, then lowest index; the result does not depend on the order
+// of the atomics.  The first 2 * cnt[4] candidates are the main pass's pairs (a
+// point's two candidates side by side), the rest single candidates of the collect
+// pass.
 __device__ __forceinline__ float exact_sq(const float *__restrict__ x, const float *__restrict__ c, uint32_t d,
                                           int vec) {
     float s = 0.f;
@@ -877,68 +896,134 @@ __device__ __forceinline__ float exact_sq(const float *__restrict__ x, const flo
     return s;
 }
 
-__global__ __launch_bounds__(256) void k_exact_cands(const float *__restrict__ X, uint32_t d, int vec,
-                                                     const float *__restrict__ C, RefineBufs R) {
+
+// Pair points, blocks [0, gridDim.x / 2): one wave per 64 pair points (one per lane),
+// grid-stride.  The rows and both centers of the wave's points go through LDS in
+// slabs of 32 dims, loaded cooperatively (8 lanes per 128-byte row piece when vec,
+// else one float per lane) while the previous slab is scored; each lane then runs
+// its two FMA chains from LDS (the operations of exact_sq).  Single candidates, the
+// other blocks: one thread each, grid-stride (a listed point's candidates share its
+// row, which these threads find in cache).
+constexpr int XS_SD = 32;   // dims per slab
+__global__ __launch_bounds__(64) void k_exact_cands(const float *__restrict__ X, uint32_t d, int vec,
+                                                    const float *__restrict__ C, RefineBufs R) {
     const uint32_t cnt = min(R.cnt[0], R.cap);
     const uint32_t np = min(2u * R.cnt[4], cnt) / 2u;   // pair points stored in full
-    const uint32_t nwork = cnt - np;                      // np pairs + the single candidates
-    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < nwork; i += gridDim.x * 256u) {
-        if (i < np) {
-            const Cand e0 = R.cand[2 * i], e1 = R.cand[2 * i + 1];
-            const float *x = X + (uint64_t)e0.p * d, *c0 = C + (uint64_t)e0.c * d, *c1 = C + (uint64_t)e1.c * d;
-            float s0 = 0.f, s1 = 0.f;
+    const uint32_t half = gridDim.x / 2;
+    const int lane = threadIdx.x;
+    if (blockIdx.x >= half) {
+        const uint32_t nsingle = cnt - 2u * np;
+        for (uint32_t i = (blockIdx.x - half) * 64u + lane; i < nsingle; i += half * 64u) {
+            const Cand e = R.cand[2u * np + i];
+            const float s = exact_sq(X + (uint64_t)e.p * d, C + (uint64_t)e.c * d, d, vec);
+            atomicMin(&R.key[e.p], ((unsigned long long)__float_as_uint(s) << 32) | e.c);
+        }
+        return;
+    }
+    __shared__ float sx[64][XS_SD + 1], sa[64][XS_SD + 1], sb[64][XS_SD + 1];
+    const uint32_t nslab = (d + XS_SD - 1) / XS_SD;
+    for (uint32_t base = blockIdx.x * 64u; base < np; base += half * 64u) {   // uniform over the wave
+        const uint32_t q = min(base + (uint32_t)lane, np - 1u);
+        const Cand e0 = R.cand[2u * q], e1 = R.cand[2u * q + 1u];
+        // staging: vec -> 8 instructions per operand, lane l takes row 8i + l/8, float4 l%8;
+        // else 32 instructions, lane l takes row 2i + l/32, dim l%32
+        float4 g[3][8];
+        auto load_slab = [&](uint32_t j0) __attribute__((always_inline)) {
             if (vec) {
-                const float4 *x4 = (const float4 *)x, *a4 = (const float4 *)c0, *b4 = (const float4 *)c1;
-                const uint32_t d4 = d >> 2;
-#pragma unroll 4
-                for (uint32_t j = 0; j < d4; ++j) {
-                    const float4 a = x4[j], u = a4[j], v = b4[j];
-                    float t;
-                    t = a.x - u.x; s0 = fmaf(t, t, s0);
-                    t = a.x - v.x; s1 = fmaf(t, t, s1);
-                    t = a.y - u.y; s0 = fmaf(t, t, s0);
-                    t = a.y - v.y; s1 = fmaf(t, t, s1);
-                    t = a.z - u.z; s0 = fmaf(t, t, s0);
-                    t = a.z - v.z; s1 = fmaf(t, t, s1);
-                    t = a.w - u.w; s0 = fmaf(t, t, s0);
-                    t = a.w - v.w; s1 = fmaf(t, t, s1);
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int r = 8 * i + (lane >> 3);
+                    const uint32_t j = j0 + 4u * (lane & 7);
+                    const uint32_t pr = __shfl(e0.p, r), ca = __shfl(e0.c, r), cb = __shfl(e1.c, r);
+                    const bool in = j < d;
+                    g[0][i] = in ? *(const float4 *)(X + (uint64_t)pr * d + j) : make_float4(0.f, 0.f, 0.f, 0.f);
+                    g[1][i] = in ? *(const float4 *)(C + (uint64_t)ca * d + j) : make_float4(0.f, 0.f, 0.f, 0.f);
+                    g[2][i] = in ? *(const float4 *)(C + (uint64_t)cb * d + j) : make_float4(0.f, 0.f, 0.f, 0.f);
                 }
             } else {
-                s0 = exact_sq(x, c0, d, 0);
-                s1 = exact_sq(x, c1, d, 0);
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    float v[3][4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int r = 2 * (4 * i + u) + (lane >> 5);
+                        const uint32_t j = j0 + (uint32_t)(lane & 31);
+                        const uint32_t pr = __shfl(e0.p, r), ca = __shfl(e0.c, r), cb = __shfl(e1.c, r);
+                        const bool in = j < d;
+                        v[0][u] = in ? X[(uint64_t)pr * d + j] : 0.f;
+                        v[1][u] = in ? C[(uint64_t)ca * d + j] : 0.f;
+                        v[2][u] = in ? C[(uint64_t)cb * d + j] : 0.f;
+                    }
+#pragma unroll
+                    for (int o = 0; o < 3; ++o) g[o][i] = make_float4(v[o][0], v[o][1], v[o][2], v[o][3]);
+                }
             }
+        };
+        auto store_slab = [&]() __attribute__((always_inline)) {
+            float(*dst[3])[XS_SD + 1] = {sx, sa, sb};
+#pragma unroll
+            for (int o = 0; o < 3; ++o) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    if (vec) {
+                        float *t = &dst[o][8 * i + (lane >> 3)][4 * (lane & 7)];
+                        t[0] = g[o][i].x; t[1] = g[o][i].y; t[2] = g[o][i].z; t[3] = g[o][i].w;
+                    } else {
+                        const float vv[4] = {g[o][i].x, g[o][i].y, g[o][i].z, g[o][i].w};
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) dst[o][2 * (4 * i + u) + (lane >> 5)][lane & 31] = vv[u];
+                    }
+                }
+            }
+        };
+        float s0 = 0.f, s1 = 0.f;
+        load_slab(0);
+        for (uint32_t sl = 0; sl < nslab; ++sl) {
+            __syncthreads();   // the previous slab's reads are done
+            store_slab();
+            __syncthreads();
+            if (sl + 1 < nslab) load_slab((sl + 1) * XS_SD);   // in flight while this slab is scored
+            const uint32_t jn = min<uint32_t>(XS_SD, d - sl * XS_SD);
+            for (uint32_t j = 0; j < jn; ++j) {
+                const float xv = sx[lane][j];
+                float t = xv - sa[lane][j];
+                s0 = fmaf(t, t, s0);
+                t = xv - sb[lane][j];
+                s1 = fmaf(t, t, s1);
+            }
+        }
+        if (base + (uint32_t)lane < np) {
             // costs are >= 0 (or NaN), so their bit patterns order like their values
             const unsigned long long k0 = ((unsigned long long)__float_as_uint(s0) << 32) | e0.c;
             const unsigned long long k1 = ((unsigned long long)__float_as_uint(s1) << 32) | e1.c;
             atomicMin(&R.key[e0.p], k0 < k1 ? k0 : k1);
-        } else {
-            const Cand e = R.cand[i + np];
-            const float s = exact_sq(X + (uint64_t)e.p * d, C + (uint64_t)e.c * d, d, vec);
-            atomicMin(&R.key[e.p], ((unsigned long long)__float_as_uint(s) << 32) | e.c);
         }
     }
 }
 
-__global__ void k_apply_keys(RefineBufs R, uint32_t *__restrict__ asn) {
-    const uint32_t cnt = min(R.cnt[0], R.cap);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
-        const uint32_t p = R.cand[i].p;
-        asn[p] = (uint32_t)(R.key[p] & 0xFFFFFFFFull);
-    }
-}
-
-// Exact argmin over all centers for the listed points (candidate overflow only):
-// 256 listed points per block, rows staged through LDS in 32-dim slabs, 16 centers
-// per register tile.
-__global__ __launch_bounds__(256) void k_rescore_full(const float *__restrict__ X, uint32_t d,
+// Final assignment of the points the main pass left open, in one launch:
+//  * every candidate's point that is not on the overflow list takes the center of its
+//    min-reduced key (k_exact_cands has finished: stream order);
+//  * the overflow points (candidates did not fit, or non-finite rows) get the exact
+//    argmin over all centers: 256 points per block, rows staged through LDS in 32-dim
+//    slabs, 16 centers per register tile.
+// The two parts write disjoint points (oflag is 1 exactly for the overflow points and
+// stays so until the next main pass resets it), so they need no ordering.
+__global__ __launch_bounds__(256) void k_refine_final(const float *__restrict__ X, uint32_t d,
                                                       const float *__restrict__ C, uint32_t k, RefineBufs R,
                                                       uint32_t *__restrict__ asn) {
+    {
+        const uint32_t cnt = min(R.cnt[0], R.cap);
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+            const uint32_t p = R.cand[i].p;
+            if (R.oflag[p] == 0u) asn[p] = (uint32_t)(R.key[p] & 0xFFFFFFFFull);
+        }
+    }
     const uint32_t *list = R.over;
-    const uint32_t *nlist = R.cnt + 2;
     __shared__ float xs[256][33];
     __shared__ float cs[16][32];
     __shared__ uint32_t pts[256];
-    const uint32_t cnt = *nlist;
+    const uint32_t cnt = R.cnt[2];
     const int tid = threadIdx.x;
     for (uint32_t base = blockIdx.x * 256u; base < cnt; base += gridDim.x * 256u) {
         __syncthreads();
@@ -973,10 +1058,7 @@ __global__ __launch_bounds__(256) void k_rescore_full(const float *__restrict__ 
             for (int c = 0; c < 16; ++c)
                 if (c0 + c < k && (bi == 0xFFFFFFFFu || acc[c] < best)) { best = acc[c]; bi = c0 + c; }
         }
-        if (pts[tid] != 0xFFFFFFFFu) {
-            asn[pts[tid]] = bi;
-            R.oflag[pts[tid]] = 0u;
-        }
+        if (pts[tid] != 0xFFFFFFFFu) asn[pts[tid]] = bi;
     }
 }
 
@@ -1205,11 +1287,6 @@ __global__ __launch_bounds__(256) void k_point_costs(const float *__restrict__ X
     if (p < n) cost[p] = s;
 }
 
-__global__ void k_copy_counts(const uint32_t *__restrict__ cnt, uint32_t *__restrict__ out) {
-    out[0] = cnt[0] / 2;   // after the main pass every candidate belongs to a pair point
-    out[1] = cnt[1];
-}
-
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -1228,10 +1305,10 @@ static void launch16(Data &ds, Ctx &c, uint64_t kpad, uint64_t kreal, uint32_t *
                        c.s[S_CINFO].as<const float2>(), c.s[S_PARAMS].as<const EParams>(), n, (uint32_t)(kpad / 64),
                        (uint32_t)kreal, asn, R);
     c.toc(0, ev);
-    // statistics: {pair points, listed points} before the collect pass adds candidates
-    hipLaunchKernelGGL(k_copy_counts, dim3(1), dim3(1), 0, st, R.cnt, R.cnt + 4);
-    // collect pass over the listed points
-    hipLaunchKernelGGL((k_collect16<KS, F16>), dim3(512, kpad >= 128 ? 2 : 1), dim3(256), 0, st, X16,
+    // collect pass over the listed points: the center tiles split over up to 4 slices
+    // (grid y), so that a few thousand listed points still fill the GPU
+    const uint32_t nt = (uint32_t)(kpad / 64), gy = std::min<uint32_t>(nt, 4u);
+    hipLaunchKernelGGL((k_collect16<KS, F16>), dim3(2048 / gy, gy), dim3(256), 0, st, X16,
                        c.s[S_CB].as<const uint4>(), c.s[S_CADD].as<const float>(), (uint32_t)(kpad / 64),
                        (uint32_t)kreal, R);
 }
@@ -1407,8 +1484,7 @@ static void dense_sqrl2_assign_mfma(Data &ds, const float *C, uint64_t k, int pr
                            ds.mu.as<const float>(), c.s[S_CNORM].as<double>());
         hipLaunchKernelGGL(k_prep_params, dim3(1), dim3(256), 0, st, c.s[S_CNORM].as<const double>(),
                            c.s[S_CINFO].as<const float2>(), k, kpad, ds.xstat.as<const float>(), d, 1,
-                           c.s[S_CADD].as<float>(),
-                           c.s[S_PARAMS].as<EParams>());
+                           c.s[S_CADD].as<float>(), c.s[S_PARAMS].as<EParams>(), R.cnt);
         hipLaunchKernelGGL(k_prep_centers_h, dim3((uint32_t)cdiv(kpad, 4)), dim3(256), 0, st, C, k, kpad, d, dpad,
                            ds.mu.as<const float>(), c.s[S_PARAMS].as<EParams>(), c.s[S_CB].as<uint16_t>(),
                            c.s[S_CINFO].as<float2>());
@@ -1418,10 +1494,8 @@ static void dense_sqrl2_assign_mfma(Data &ds, const float *C, uint64_t k, int pr
                            c.s[S_CINFO].as<float2>());
         hipLaunchKernelGGL(k_prep_params, dim3(1), dim3(256), 0, st, c.s[S_CNORM].as<const double>(),
                            c.s[S_CINFO].as<const float2>(), k, kpad, ds.xstat.as<const float>(), d, 0,
-                           c.s[S_CADD].as<float>(),
-                           c.s[S_PARAMS].as<EParams>());
+                           c.s[S_CADD].as<float>(), c.s[S_PARAMS].as<EParams>(), R.cnt);
     }
-    fill_u32(R.cnt, 0u, 4, st);
     if (f16) {
         switch (dpad) {
         case 64: launch16<4, 2, true>(ds, c, kpad, k, asn, R, st); break;
@@ -1439,13 +1513,12 @@ static void dense_sqrl2_assign_mfma(Data &ds, const float *C, uint64_t k, int pr
     }
     {
         const int vec = (d % 4 == 0) && ((uintptr_t)ds.X % 16 == 0) && ((uintptr_t)C % 16 == 0);
-        hipLaunchKernelGGL(k_exact_cands, dim3(2048), dim3(256), 0, st, ds.X, d, vec, C, R);
+        hipLaunchKernelGGL(k_exact_cands, dim3(4096), dim3(64), 0, st, ds.X, d, vec, C, R);
     }
-    hipLaunchKernelGGL(k_apply_keys, dim3(1024), dim3(256), 0, st, R, asn);
-    hipLaunchKernelGGL(k_rescore_full, dim3(256), dim3(256), 0, st, ds.X, d, C, (uint32_t)k, R, asn);
-    // near-tie statistics (not kept for iterations replayed from a graph)
-    if (!stream_capturing(st))
-        HIP_CHECK(hipMemcpyAsync(c.refine_host(), R.cnt + 4, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(k_refine_final, dim3(512), dim3(256), 0, st, ds.X, d, C, (uint32_t)k, R, asn);
+    // R.cnt[4] (pair points) and R.cnt[1] (listed points) stay on the device for
+    // mcg_refine_counts
+    c.refine_cnt = R.cnt;
 }
 
 }  // namespace mcg

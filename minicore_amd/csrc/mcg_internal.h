@@ -141,11 +141,10 @@ struct Ctx {
     void *host_pinned = nullptr;
     size_t host_pinned_bytes = 0;
     void *pinned(size_t nb);
-    uint32_t *refine_host();
     // kernel timing: pending (start, stop) event pairs per timed kernel
     bool timing = false;
-    // near-tie statistics of the last BF16 E-step {pairs, full re-scores}, pinned host memory
-    uint32_t *refine_counts = nullptr;
+    // device counters of the last 16-bit E-step: [4] pair points, [1] listed points
+    const uint32_t *refine_cnt = nullptr;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[2];
     double timed_ms[2] = {0., 0.};
     uint64_t timed_n[2] = {0, 0};
@@ -205,15 +204,16 @@ void dense_sqrl2_assign_exact(Data &ds, const float *C, uint64_t k, uint32_t *as
 // (f64 at S_TOTAL) on the device.
 struct MStepOut {
     uint32_t *counts;
-    double *total;   // [0] = cost vs Cold (SQRL2 dense only), [1] = total weight
+    double *total;   // [0] = cost vs Cold (SQRL2 dense only), [1] = total weight; dense: [2] = empty clusters
 };
+// host_out: pinned host memory that receives {total[0], total[2]} (or null)
 MStepOut dense_mstep(Data &ds, const uint32_t *asn, const double *w, uint64_t k, const float *Cold,
-                     float *Cnew, bool want_cost, hipStream_t st);
+                     float *Cnew, bool want_cost, hipStream_t st, double *host_out = nullptr);
 // Stable counting sort of point ids by assignment (ids >= k left out): perm holds
 // each cluster's members contiguous and ascending, at off[c], counts[c] of them;
 // itemoff the prefix of 256-member chunks.  k <= 16384.
 struct SortOut {
-    uint32_t *perm, *counts, *off, *itemoff;
+    uint32_t *perm, *counts, *off, *itemoff, *imap;   // imap: cluster of each chunk item
 };
 SortOut sort_by_cluster(Ctx &c, const uint32_t *asn, uint64_t n, uint64_t k, hipStream_t st);
 // Exclusive prefix sum of L u32 values (tmp: scratch for block totals).
@@ -357,6 +357,7 @@ inline void reset_derived(Data &D) {
 
 // Small helpers.
 void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
+
 void copy_rows_to_centers(Data &ds, const uint64_t *rows_h, const uint32_t *ids_h, uint64_t m,
                           float *C, hipStream_t st);
 void rowsums(Data &ds, hipStream_t st);

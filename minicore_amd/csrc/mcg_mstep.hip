@@ -123,17 +123,41 @@ void scan_exclusive(const uint32_t *in, uint32_t *out, uint64_t L, DBuf &tmp, hi
     hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)nb), dim3(256), 0, st, in, L, tmp.as<const uint32_t>(), out);
 }
 
-// counts[c] = sum_b hist[c][b]; off[c] = base[c][0]; items[c] = ceil(counts/MCHUNK)
-__global__ void k_counts(const uint32_t *__restrict__ hist, const uint32_t *__restrict__ base, uint64_t k,
-                         uint32_t nblk, uint32_t *__restrict__ counts, uint32_t *__restrict__ off,
-                         uint32_t *__restrict__ items) {
-    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= k) return;
-    const uint32_t b0 = base[c * nblk];
-    const uint32_t last = base[c * nblk + nblk - 1] + hist[c * nblk + nblk - 1];
-    counts[c] = last - b0;
-    off[c] = b0;
-    items[c] = (last - b0 + MCHUNK - 1) / MCHUNK;
+// One block of 1024 threads: counts[c] = sum_b hist[c][b]; off[c] = base[c][0];
+// itemoff = exclusive prefix of the partial-sum chunks ceil(counts / MCHUNK) (k + 1
+// entries, the last the total); imap[i] = the cluster of chunk item i.
+__global__ __launch_bounds__(1024) void k_counts_items(const uint32_t *__restrict__ hist,
+                                                       const uint32_t *__restrict__ base, uint32_t k, uint32_t nblk,
+                                                       uint32_t *__restrict__ counts, uint32_t *__restrict__ off,
+                                                       uint32_t *__restrict__ itemoff, uint32_t *__restrict__ imap) {
+    __shared__ uint32_t sm[1024];
+    const uint32_t per = (k + 1023) / 1024;
+    const uint32_t c0 = threadIdx.x * per, c1 = min(k, c0 + per);
+    uint32_t s = 0;
+    for (uint32_t c = c0; c < c1; ++c) {
+        const uint64_t r = (uint64_t)c * nblk;
+        const uint32_t b0 = base[r];
+        const uint32_t cnt = base[r + nblk - 1] + hist[r + nblk - 1] - b0;
+        counts[c] = cnt;
+        off[c] = b0;
+        s += (cnt + MCHUNK - 1) / MCHUNK;
+    }
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const uint32_t t = threadIdx.x >= (unsigned)o ? sm[threadIdx.x - o] : 0u;
+        __syncthreads();
+        sm[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? sm[threadIdx.x - 1] : 0u;
+    for (uint32_t c = c0; c < c1; ++c) {
+        const uint32_t it = (counts[c] + MCHUNK - 1) / MCHUNK;
+        itemoff[c] = run;
+        for (uint32_t i = 0; i < it; ++i) imap[run + i] = c;
+        run += it;
+    }
+    if (threadIdx.x == 1023) itemoff[k] = sm[1023];
 }
 
 // Stable scatter: one wave per histogram block, LDS cursors for all clusters.
@@ -177,17 +201,19 @@ __global__ void k_scatter(const uint32_t *__restrict__ asn, uint64_t n, uint32_t
     }
 }
 
-// Partial sums: one wave per (cluster, chunk) item.  Lane l owns dims
-// [4l, 4l+4) of each 256-dim pass (16-byte loads when d % 4 == 0, else one dim
-// per lane).  Members are walked in order, so each dim's FP64 sum is sequential:
-// part[item*d + j] = sum_m w*x (fma) or sum_m x;  partw[item] = sum_m w;
+// Partial sums: one wave per (cluster, chunk) item; imap[item] is its cluster.
+// Lane l owns dims [4l, 4l+4) of each 256-dim pass (16-byte loads when d % 4 == 0,
+// else one dim per lane).  Members are walked in order, so each dim's FP64 sum is
+// sequential: part[item*d + j] = sum_m w*x (fma) or sum_m x;  partw[item] = sum_m w
+// (added in the same member order by every lane; the member count when unweighted);
 // partq[item] = sum_m w * ||x - cold||^2 (FP64) when cold != nullptr.
 template <bool VEC4>
 __global__ __launch_bounds__(64) void k_partial(const float *__restrict__ X, uint32_t d,
                                                 const uint32_t *__restrict__ perm,
                                                 const uint32_t *__restrict__ counts,
                                                 const uint32_t *__restrict__ off,
-                                                const uint32_t *__restrict__ itemoff, uint32_t k,
+                                                const uint32_t *__restrict__ itemoff,
+                                                const uint32_t *__restrict__ imap, uint32_t k,
                                                 const double *__restrict__ w, const float *__restrict__ cold,
                                                 double *__restrict__ part, double *__restrict__ partw,
                                                 double *__restrict__ partq) {
@@ -196,9 +222,7 @@ __global__ __launch_bounds__(64) void k_partial(const float *__restrict__ X, uin
     const uint32_t item = blockIdx.x;
     const uint32_t total_items = itemoff[k];
     if (item >= total_items) return;
-    uint32_t lo = 0, hi = k;
-    while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (itemoff[mid] <= item) lo = mid; else hi = mid; }
-    const uint32_t c = lo;
+    const uint32_t c = imap[item];
     const uint32_t mb = off[c] + (item - itemoff[c]) * MCHUNK;
     const uint32_t me = min(off[c] + counts[c], mb + MCHUNK);
     const uint32_t nm = me - mb;
@@ -209,7 +233,7 @@ __global__ __launch_bounds__(64) void k_partial(const float *__restrict__ X, uin
         wl[q] = w ? w[pnt] : 1.;
     }
     __syncthreads();
-    double qsum = 0.;
+    double qsum = 0., wsum = 0.;
     constexpr uint32_t DPL = VEC4 ? 4 : 1;   // dims per lane
     for (uint32_t j0 = 0; j0 < d; j0 += 64 * DPL) {
         const uint32_t j = j0 + lane * DPL;
@@ -221,6 +245,7 @@ __global__ __launch_bounds__(64) void k_partial(const float *__restrict__ X, uin
             cj[i] = cold ? (double)cold[(uint64_t)c * d + j + i] : 0.;
         }
         auto accumulate = [&](const float (&xv)[DPL], double wq) __attribute__((always_inline)) {
+            if (j0 == 0 && w) wsum = __dadd_rn(wsum, wq);
 #pragma unroll
             for (uint32_t i = 0; i < DPL; ++i) {
                 const double x = (double)xv[i];
@@ -267,12 +292,11 @@ __global__ __launch_bounds__(64) void k_partial(const float *__restrict__ X, uin
         for (int o = 32; o > 0; o >>= 1) qsum += __shfl_xor(qsum, o);
         if (lane == 0) partq[item] = qsum;
     }
-    if (lane == 0) {
-        double ws = 0.;
-        for (uint32_t q = 0; q < nm; ++q) ws = __dadd_rn(ws, wl[q]);
-        partw[item] = ws;
-    }
+    // every lane added the same weights in member order (lanes past d skipped the
+    // member loop, so lane 0 writes)
+    if (lane == 0) partw[item] = w ? wsum : (double)nm;
 }
+
 
 // One block per cluster: add chunk partials in order; mean = sum / weight.
 __global__ __launch_bounds__(256) void k_finalize(uint32_t d, const uint32_t *__restrict__ counts,
@@ -303,21 +327,35 @@ __global__ __launch_bounds__(256) void k_finalize(uint32_t d, const uint32_t *__
     }
 }
 
-// Single block: total[0] = sum_c ccost[c], total[1] = sum_c cw[c] (fixed order);
-// total[2] (as u32 bit pattern in a double slot) unused.
-__global__ void k_total(const double *__restrict__ ccost, const double *__restrict__ cw, uint64_t k,
-                        double *__restrict__ total) {
+// Single block: total[0] = sum_c ccost[c], total[1] = sum_c cw[c] (fixed order),
+// total[2] = clusters without members; host_out (pinned host memory, or null)
+// receives {total[0], total[2]}.
+__global__ void k_total(const double *__restrict__ ccost, const double *__restrict__ cw,
+                        const uint32_t *__restrict__ counts, uint64_t k, double *__restrict__ total,
+                        double *__restrict__ host_out) {
     __shared__ double a[256], b[256];
+    __shared__ uint32_t z[256];
     double s = 0., t = 0.;
-    for (uint64_t i = threadIdx.x; i < k; i += blockDim.x) { s += ccost[i]; t += cw[i]; }
+    uint32_t e = 0;
+    for (uint64_t i = threadIdx.x; i < k; i += blockDim.x) { s += ccost[i]; t += cw[i]; e += counts[i] == 0; }
     a[threadIdx.x] = s;
     b[threadIdx.x] = t;
+    z[threadIdx.x] = e;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) { a[threadIdx.x] += a[threadIdx.x + o]; b[threadIdx.x] += b[threadIdx.x + o]; }
+        if ((int)threadIdx.x < o) {
+            a[threadIdx.x] += a[threadIdx.x + o];
+            b[threadIdx.x] += b[threadIdx.x + o];
+            z[threadIdx.x] += z[threadIdx.x + o];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) { total[0] = a[0]; total[1] = b[0]; }
+    if (threadIdx.x == 0) {
+        total[0] = a[0];
+        total[1] = b[0];
+        total[2] = (double)z[0];
+        if (host_out) { host_out[0] = a[0]; host_out[1] = (double)z[0]; }
+    }
 }
 
 SortOut sort_by_cluster(Ctx &c, const uint32_t *asn, uint64_t n, uint64_t k, hipStream_t st) {
@@ -333,20 +371,18 @@ SortOut sort_by_cluster(Ctx &c, const uint32_t *asn, uint64_t n, uint64_t k, hip
     hipLaunchKernelGGL(k_hist, dim3(nblk), dim3(256), (size_t)k * sizeof(uint32_t), st, asn, n, nblk, (uint32_t)k,
                        hist);
     scan_exclusive(hist, base, L, c.s[S_MISC0], st);
-    hipLaunchKernelGGL(k_counts, dim3((uint32_t)cdiv(k, 256)), dim3(256), 0, st, hist, base, k, nblk, cnt, off,
-                       items);
-    // items -> exclusive prefix (k+1 entries: the last one becomes the total)
-    fill_u32(items + k, 0, 1, st);
-    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, items, k + 1, (uint32_t *)nullptr);
+    uint32_t *imap = (uint32_t *)c.s[S_ITEMOFF].reserve((n / MCHUNK + k + 1) * sizeof(uint32_t));
+    hipLaunchKernelGGL(k_counts_items, dim3(1), dim3(1024), 0, st, hist, base, (uint32_t)k, nblk, cnt, off, items,
+                       imap);
     uint32_t kbits = 1;   // bits of the largest cluster id
     while ((1ull << kbits) < k) ++kbits;
     hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(64), (size_t)k * sizeof(uint32_t), st, asn, n, nblk,
                        (uint32_t)k, kbits, base, perm);
-    return SortOut{perm, cnt, off, items};
+    return SortOut{perm, cnt, off, items, imap};
 }
 
 MStepOut dense_mstep(Data &ds, const uint32_t *asn, const double *w, uint64_t k, const float *Cold,
-                     float *Cnew, bool want_cost, hipStream_t st) {
+                     float *Cnew, bool want_cost, hipStream_t st, double *host_out) {
     Ctx &c = *ds.ctx;
     const uint64_t n = ds.n;
     const uint32_t d = (uint32_t)ds.d;
@@ -357,18 +393,19 @@ MStepOut dense_mstep(Data &ds, const uint32_t *asn, const double *w, uint64_t k,
     double *partq = (double *)c.s[S_PARTQ].reserve(max_items * sizeof(double));
     double *ccost = (double *)c.s[S_CCOST].reserve(2 * k * sizeof(double));
     double *total = (double *)c.s[S_TOTAL].reserve(4 * sizeof(double));
+    const uint32_t *imap = so.imap;
     hipEvent_t ev;
     c.tic(1, ev);
     if (d % 4 == 0)
         hipLaunchKernelGGL(k_partial<true>, dim3((uint32_t)max_items), dim3(64), 0, st, ds.X, d, so.perm, so.counts,
-                           so.off, so.itemoff, (uint32_t)k, w, want_cost ? Cold : nullptr, part, partw, partq);
+                           so.off, so.itemoff, imap, (uint32_t)k, w, want_cost ? Cold : nullptr, part, partw, partq);
     else
         hipLaunchKernelGGL(k_partial<false>, dim3((uint32_t)max_items), dim3(64), 0, st, ds.X, d, so.perm, so.counts,
-                           so.off, so.itemoff, (uint32_t)k, w, want_cost ? Cold : nullptr, part, partw, partq);
+                           so.off, so.itemoff, imap, (uint32_t)k, w, want_cost ? Cold : nullptr, part, partw, partq);
     c.toc(1, ev);
     hipLaunchKernelGGL(k_finalize, dim3((uint32_t)k), dim3(256), 0, st, d, so.counts, so.itemoff, part, partw,
                        want_cost ? partq : nullptr, Cold, Cnew, ccost, ccost + k);
-    hipLaunchKernelGGL(k_total, dim3(1), dim3(256), 0, st, ccost, ccost + k, k, total);
+    hipLaunchKernelGGL(k_total, dim3(1), dim3(256), 0, st, ccost, ccost + k, so.counts, k, total, host_out);
     return MStepOut{so.counts, total};
 }
 

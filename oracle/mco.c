@@ -516,7 +516,7 @@ uint64_t mco_centroids_dense(const float *X, uint64_t n, uint64_t d, const uint3
     memcpy(cur, cnt, k * sizeof(uint64_t));
     for (uint64_t i = 0; i < n; ++i) perm[cur[asn[i]]++] = i;
     uint64_t empty = 0;
-    if (chunk == 0) chunk = 256;
+    if (chunk == 0) chunk = MCO_MEAN_CHUNK;
     #pragma omp parallel for schedule(dynamic, 1) reduction(+:empty)
     for (int64_t c = 0; c < (int64_t)k; ++c) {
         uint64_t b = cnt[c], e = cnt[c + 1];
@@ -545,7 +545,7 @@ uint64_t mco_centroids_dense(const float *X, uint64_t n, uint64_t d, const uint3
 /* One pass of lloyd_iteration (kmeans.h:359-479) on dense rows, squared L2, in the
  * order E-step then M-step: assign every row to its nearest center (FP32 FMA chain,
  * lowest index on ties), *cost = sum w_i ||x_i - c_asn(i)||^2 against the input
- * centers (FP64), then the weighted means (chunk 256 association).  An empty
+ * centers (FP64), then the weighted means (chunks of MCO_MEAN_CHUNK members).  An empty
  * cluster (ascending id) takes a row drawn with probability ~ w_j * cost_j, cost_j
  * the FP32 cost of row j to its new mean (kmeans.h:433-440), by the fixed-point
  * sampler with output e of the wyrand stream of 0 for the e-th draw of the call;
@@ -569,7 +569,7 @@ uint64_t mco_lloyd_step(const float *X, uint64_t n, uint64_t d, float *C, uint64
     uint64_t *fw = (uint64_t *)malloc(n * sizeof(uint64_t));
     uint64_t *cnt = (uint64_t *)malloc(k * sizeof(uint64_t));
     for (uint64_t round = 0; round <= k; ++round) {
-        uint64_t empty = mco_centroids_dense(X, n, d, asn, w, k, 256, C);
+        uint64_t empty = mco_centroids_dense(X, n, d, asn, w, k, MCO_MEAN_CHUNK, C);
         if (!empty) break;
         memset(cnt, 0, k * sizeof(uint64_t));
         for (uint64_t i = 0; i < n; ++i) cnt[asn[i]]++;
@@ -668,7 +668,7 @@ static int centroids_with_restart(int msr, const float *X, const float *val, con
     }
     free(cnt);
     if (val) mco_centroids_csr(val, idx, indptr, n, d, asn, w, k, C);
-    else     mco_centroids_dense(X, n, d, asn, w, k, 256, C);
+    else     mco_centroids_dense(X, n, d, asn, w, k, MCO_MEAN_CHUNK, C);
     return restarted;
 }
 
@@ -850,7 +850,7 @@ int mco_minibatch(int msr, const float *X, const float *val, const uint32_t *idx
             assign_generic(msr, Xs, NULL, NULL, NULL, mbsize, d, C, k, prior, sa, sc, nthreads);
             if (median_policy(msr)) median_core(median_policy(msr), Xs, NULL, NULL, NULL, mbsize, d, sw, w, k, sa, C,
                                                 0., 0, 1);
-            else mco_centroids_dense(Xs, mbsize, d, sa, sw, k, 256, C);
+            else mco_centroids_dense(Xs, mbsize, d, sa, sw, k, MCO_MEAN_CHUNK, C);
         }
     }
     memcpy(C, save, k * d * sizeof(float));

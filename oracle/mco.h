@@ -23,6 +23,9 @@ extern "C" {
 #endif
 
 /* Measure codes: include/minicore/dist/distance.h:24-64 */
+/* members per fixed-order FP64 chunk of the dense hard M-step's sums */
+#define MCO_MEAN_CHUNK 256
+
 enum { MCO_SQRL2 = 2, MCO_JSD = 4, MCO_MKL = 5, MCO_REVERSE_MKL = 11 };
 /* further measures (distance.h:24-64): applicator.h:1673-1749 (dense), 1778-1891 (sparse) */
 enum {

@@ -30,7 +30,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TIMING_EVERY = 4   # kernel durations are sampled on every 4th timed step
+
+
 def time_form(ds, ctx, mcg, C0, asn, prec, steps, warmup):
+    """Times `steps` Lloyd steps after `warmup` untimed ones.  The E-step and M-step
+    kernels are timed with HIP events on the context stream on every TIMING_EVERY-th
+    step of the timed region (each event record idles the queue for a few
+    microseconds, so recording them on every step would inflate ms_per_step)."""
     import torch
     C = C0.clone()
     ctx.timing(False)
@@ -42,7 +49,8 @@ def time_form(ds, ctx, mcg, C0, asn, prec, steps, warmup):
     lat = []
     t0 = time.perf_counter()
     cost = None
-    for _ in range(steps):
+    for i in range(steps):
+        ctx.timing_sample(i % TIMING_EVERY == 0)
         s0 = time.perf_counter()
         cost = ds.lloyd_step(mcg.SQRL2, C, asn, precision=prec)
         lat.append(time.perf_counter() - s0)

@@ -201,8 +201,13 @@ class Context:
         check(load().mcg_graphs_enable(self.h, 1 if on else 0))
 
     def timing(self, on=True):
+        """Switch kernel timing on or off and clear the accumulated times."""
         check(load().mcg_timing_enable(self.h, 1 if on else 0))
         check(load().mcg_timing_reset(self.h))
+
+    def timing_sample(self, on):
+        """Switch kernel timing on or off for the next calls, keeping the totals."""
+        check(load().mcg_timing_enable(self.h, 1 if on else 0))
 
     def refine_counts(self):
         a, b = C.c_uint64(), C.c_uint64()

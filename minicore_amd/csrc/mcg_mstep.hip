@@ -160,23 +160,34 @@ __global__ __launch_bounds__(1024) void k_counts_items(const uint32_t *__restric
     if (threadIdx.x == 1023) itemoff[k] = sm[1023];
 }
 
-// Stable scatter: one wave per histogram block, LDS cursors for all clusters.
-__global__ void k_scatter(const uint32_t *__restrict__ asn, uint64_t n, uint32_t nblk, uint32_t k, uint32_t kbits,
-                          const uint32_t *__restrict__ base, uint32_t *__restrict__ perm) {
+// Stable scatter: one wave per histogram block, LDS cursors for all clusters.  The
+// block's assignments are loaded up front (16 per lane) and the cursors set from
+// them, so the 16 ranking rounds run without memory waits.
+__global__ __launch_bounds__(64) void k_scatter(const uint32_t *__restrict__ asn, uint64_t n, uint32_t nblk,
+                                                uint32_t k, uint32_t kbits, const uint32_t *__restrict__ base,
+                                                uint32_t *__restrict__ perm) {
     extern __shared__ uint32_t cur[];
+    constexpr int R = SORT_BLK / 64;
     const uint32_t b = blockIdx.x;
     const int lane = threadIdx.x;
     const uint64_t i0 = (uint64_t)b * SORT_BLK;
-    for (uint32_t q = lane; q < SORT_BLK; q += 64) {
-        uint64_t i = i0 + q;
-        if (i < n) { uint32_t a = asn[i]; if (a < k) cur[a] = base[(uint64_t)a * nblk + b]; }
+    uint32_t av[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint64_t i = i0 + (uint64_t)r * 64 + lane;
+        av[r] = i < n ? asn[i] : 0xFFFFFFFFu;
     }
+    // lanes with the same cluster write the same cursor
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (av[r] < k) cur[av[r]] = base[(uint64_t)av[r] * nblk + b];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    for (uint32_t q0 = 0; q0 < SORT_BLK; q0 += 64) {
-        const uint64_t i = i0 + q0 + lane;
-        const uint32_t a = i < n ? asn[i] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint64_t i = i0 + (uint64_t)r * 64 + lane;
+        const uint32_t a = av[r];
         const bool valid = a < k;
         const uint32_t key = valid ? a : 0xFFFFFFFFu;
         // lanes holding the same key: intersect the ballots of each key bit (and of

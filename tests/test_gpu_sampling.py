@@ -13,10 +13,12 @@ def say(*a):
 
 
 @pytest.mark.parametrize("weighted", [False, True])
-@pytest.mark.parametrize("n,d,k", [(20000, 50, 25), (30000, 64, 200)])
+@pytest.mark.parametrize("n,d,k", [(20000, 50, 25), (30000, 64, 200), (200, 8, 5), (1001, 17, 12),
+                                   (8192, 64, 40), (70000, 64, 30)])
 def test_kmeanspp_dense_bitexact(mcg, ctx, mco, weighted, n, d, k):
     """Rounds after the first skip points whose center is far from the new one; the
-    result must still equal the oracle, which evaluates every point."""
+    result must still equal the oracle, which evaluates every point.  200 rows is
+    less than one block of the update, d = 17 takes scalar loads."""
     say(f"kmeans++ dense n={n} d={d} k={k} weighted={weighted}")
     X, _, _ = gaussian_mixture(n, d, 25, 0)
     w = np.random.default_rng(3).uniform(0.5, 2, n) if weighted else None

@@ -190,6 +190,20 @@ int mcg_lloyd(mcg_data *ds, int msr, double prior, float *centers, uint64_t k,
               const double *weights, double eps, uint64_t maxiter, int precision,
               uint32_t *asn, float *costs, double *initcost, double *finalcost,
               uint64_t *numiter);
+/* perform_hard_clustering (solve.h:98-175) on dense rows given as float64 for
+ * SQRL2, with every value in FP64 as the reference's FT = double instantiation for
+ * float64 arrays (python/pycluster_dense.cpp:85-90): per pair the sequential FP64
+ * FMA chain, FP64 means (the chunked member order of mcg_lloyd), FP64 costs in the
+ * control flow (each the exact fixed-point sum of w_i cost_i), the empty-cluster
+ * restart of centroid.h:687-752.  ds must come from mcg_data_dense with MCG_F64;
+ * centers f64[k*d] in/out, costs f64[n] or NULL, weights f64[n] or NULL.
+ * MCG_ENAN when a center becomes NaN. */
+int mcg_lloyd_f64(mcg_data *ds, double *centers, uint64_t k, const double *weights, double eps,
+                  uint64_t maxiter, uint32_t *asn, double *costs, double *initcost,
+                  double *finalcost, uint64_t *numiter);
+/* assign_points_hard (solve.h:234-285) for SQRL2 on rows given as float64, in
+ * FP64 (the sequential FMA chain; lowest index on ties).  costs f64[n] or NULL. */
+int mcg_assign_f64(mcg_data *ds, const double *centers, uint64_t k, uint32_t *asn, double *costs);
 /* One Lloyd iteration on device-resident state: asn = argmin over centers
  * (E-step), then centers = weighted cluster means (M-step, in place; empty
  * clusters keep their center); *cost = the assignment's cost against the input

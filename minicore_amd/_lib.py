@@ -21,7 +21,7 @@ EXPORTS = [
     "mcg_last_error", "mcg_device_count", "mcg_ctx_create", "mcg_ctx_destroy", "mcg_ctx_synchronize",
     "mcg_ctx_stream", "mcg_data_dense", "mcg_data_csr", "mcg_data_free", "mcg_data_shape",
     "mcg_cost_matrix", "mcg_assign_hard", "mcg_centroids_hard", "mcg_lloyd", "mcg_soft_em",
-    "mcg_kmeanspp", "mcg_coreset_probs", "mcg_coreset_sample", "mcg_lloyd_step", "mcg_timing_enable",
+    "mcg_lloyd_f64", "mcg_assign_f64", "mcg_kmeanspp", "mcg_coreset_probs", "mcg_coreset_sample", "mcg_lloyd_step", "mcg_timing_enable",
     "mcg_timing_reset", "mcg_timing_read", "mcg_refine_counts", "mcg_data_rows", "mcg_server_create",
     "mcg_server_destroy", "mcg_submit", "mcg_poll", "mcg_wait", "mcg_cancel", "mcg_release", "mcg_tuning_enable",
     "mcg_graphs_enable", "mcg_minibatch", "mcg_cs_minibatch", "mcg_kmeanspp_ls", "mcg_kmeanspp_local", "mcg_sampler_write",
@@ -104,6 +104,8 @@ def load():
         "mcg_assign_hard": (I, [P, I, D, P, U64, I, P, P]),
         "mcg_centroids_hard": (I, [P, P, P, U64, P, P]),
         "mcg_lloyd": (I, [P, I, D, P, U64, P, D, U64, I, P, P, P, P, P]),
+        "mcg_lloyd_f64": (I, [P, P, U64, P, D, U64, P, P, P, P, P]),
+        "mcg_assign_f64": (I, [P, P, U64, P, P]),
         "mcg_soft_em": (I, [P, I, D, D, P, U64, P, D, U64, P, P, P, P, P]),
         "mcg_kmeanspp": (I, [P, I, D, U64, U64, P, P, P, P]),
         "mcg_coreset_probs": (I, [P, I, U64, U64, P, P, P, P]),
@@ -343,6 +345,26 @@ class Dataset:
         check(load().mcg_lloyd(self.h, msr, prior, ptr(Cc), k, ptr(w), eps, maxiter, precision, ptr(asn),
                                ptr(costs), C.byref(ic), C.byref(fc), C.byref(it)))
         return dict(centers=Cc, asn=asn, costs=costs, initcost=ic.value, finalcost=fc.value, numiter=it.value)
+
+    def lloyd_f64(self, centers, weights=None, eps=1e-10, maxiter=100):
+        """Dense SQRL2 hard clustering of rows given as float64, every value in FP64."""
+        Cc = np.ascontiguousarray(centers, dtype=np.float64).copy()
+        k = Cc.shape[0]
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        asn = np.empty(self.n, dtype=np.uint32)
+        costs = np.empty(self.n, dtype=np.float64)
+        ic, fc, it = C.c_double(), C.c_double(), C.c_uint64()
+        check(load().mcg_lloyd_f64(self.h, ptr(Cc), k, ptr(w), eps, maxiter, ptr(asn), ptr(costs), C.byref(ic),
+                                   C.byref(fc), C.byref(it)))
+        return dict(centers=Cc, asn=asn, costs=costs, initcost=ic.value, finalcost=fc.value, numiter=it.value)
+
+    def assign_f64(self, centers):
+        """SQRL2 assignment of rows given as float64, in FP64."""
+        Cc = np.ascontiguousarray(centers, dtype=np.float64)
+        asn = np.empty(self.n, dtype=np.uint32)
+        costs = np.empty(self.n, dtype=np.float64)
+        check(load().mcg_assign_f64(self.h, ptr(Cc), Cc.shape[0], ptr(asn), ptr(costs)))
+        return asn, costs
 
     def minibatch(self, msr, centers, prior=0., weights=None, mbsize=1000, maxiter=100, freq=10, reseed_after=1,
                   with_replacement=False, seed=0, precision=PREC_BF16):

@@ -21,7 +21,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MCG_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["mcg_api.hip", "mcg_dense.hip", "mcg_mstep.hip", "mcg_generic.hip", "mcg_sample.hip", "mcg_serve.hip",
-           "mcg_soft.hip", "mcg_format.hip", "mcg_median.hip", "mcg_noalloc.hip", "mcg_tiles.hip"]
+           "mcg_soft.hip", "mcg_format.hip", "mcg_median.hip", "mcg_noalloc.hip", "mcg_tiles.hip", "mcg_f64.hip"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",

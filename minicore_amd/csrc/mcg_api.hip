@@ -75,7 +75,7 @@ void *Ctx::pinned(size_t nb) {
 }
 
 // Device view of a caller buffer (borrow device memory, stage host memory).
-static const void *in_dev(Ctx &c, const void *p, size_t bytes, DBuf &stage) {
+const void *in_dev(Ctx &c, const void *p, size_t bytes, DBuf &stage) {
     if (!p) return nullptr;
     if (is_device_ptr(p)) return p;
     stage.reserve(bytes);
@@ -105,7 +105,7 @@ static void out_finish(Ctx &c, Out &o) {
     if (o.host && o.user) HIP_CHECK(hipMemcpyAsync(o.user, o.dev, o.bytes, hipMemcpyDeviceToHost, c.stream));
 }
 
-static void sync(Ctx &c) {
+void sync(Ctx &c) {
     HIP_CHECK(hipStreamSynchronize(c.stream));
     c.collect();
 }
@@ -647,6 +647,9 @@ int mcg_data_dense(mcg_ctx *ctx, const void *x, int dtype, uint64_t n, uint64_t 
                 }
                 hipLaunchKernelGGL(k_f64_to_f32, dim3((uint32_t)std::min<uint64_t>(cdiv(n * d, 256), 65536)), dim3(256),
                                    0, c.stream, (const double *)src, n, d, ld, D.X);
+                // the values as given, for the float64 path (mcg_lloyd_f64, mcg_assign_f64)
+                D.Xd.reserve(n * d * 8);
+                HIP_CHECK(hipMemcpy2DAsync(D.Xd.p, d * 8, src, ld * 8, d * 8, n, hipMemcpyDeviceToDevice, c.stream));
                 sync(c);
             }
         }

@@ -104,6 +104,9 @@ struct Data {
     // dense rows (float32, row-major, stride d)
     float *X = nullptr;
     DBuf Xown;
+    // rows given as float64: the values as given (row-major, stride d), for the
+    // float64 path of dense SQRL2 hard clustering (mcg_f64.hip)
+    DBuf Xd;
     // CSR rows (canonical: f32 data, u32 indices, u64 indptr)
     DBuf val, idx, ptr;
     // per-row FP64 sums (csc.h:1008-1016 / blaze sum<rowwise>)
@@ -317,6 +320,13 @@ uint64_t restart_draw_device(Ctx &ctx, const float *costs_d, uint64_t n, uint64_
 // Fixed-point inverse-CDF draws over host weights wv for raw draws r[0..m): the
 // weights are converted once, each draw is a search of their prefix sums.
 void draw_fixed_many(const std::vector<double> &wv, const uint64_t *r, uint64_t m, uint64_t *out);
+// exact sum of w_i v_i (v_i >= 0) at the sampler's fixed-point scale (one host read)
+double fixed_wsum(Ctx &c, const float *v, const double *w, uint64_t n, hipStream_t st);
+double fixed_wsum(Ctx &c, const double *v, const double *w, uint64_t n, hipStream_t st);
+// device view of a caller buffer: device memory borrowed, host memory staged in `stage`
+const void *in_dev(Ctx &c, const void *p, size_t bytes, DBuf &stage);
+// stream synchronize, then fold pending kernel timings into the totals
+void sync(Ctx &c);
 class FixedSeq {
   public:
     explicit FixedSeq(std::vector<double> w);

@@ -256,15 +256,16 @@ void FixedSeq::zero(uint64_t i) {
     for (uint64_t p = i + 1; p < tree_.size(); p += p & (~p + 1)) tree_[p] -= f;
 }
 
-double fixed_wsum(Ctx &c, const float *v, const double *w, uint64_t n, hipStream_t st) {
+template <class V>
+static double fixed_wsum_t(Ctx &c, const V *v, const double *w, uint64_t n, hipStream_t st) {
     const uint32_t nblk = (uint32_t)cdiv(n, FX_BLK);
     char *buf = (char *)c.s[S_MISC0].reserve(64 + nblk * 8);
     unsigned long long *mx = (unsigned long long *)buf;
     uint64_t *bsum = (uint64_t *)(buf + 64);
     fill_u32((uint32_t *)mx, 0, 2, st);
-    hipLaunchKernelGGL(k_absmax<float>, dim3((uint32_t)std::min<uint64_t>(cdiv(n, 256), 1024)), dim3(256), 0, st, v, w,
+    hipLaunchKernelGGL(k_absmax<V>, dim3((uint32_t)std::min<uint64_t>(cdiv(n, 256), 1024)), dim3(256), 0, st, v, w,
                        n, mx);
-    hipLaunchKernelGGL(k_fixed_blocksum<float>, dim3(nblk), dim3(256), 0, st, v, w, n,
+    hipLaunchKernelGGL(k_fixed_blocksum<V>, dim3(nblk), dim3(256), 0, st, v, w, n,
                        (const unsigned long long *)mx, bsum);
     std::vector<uint64_t> h(nblk + 1);
     HIP_CHECK(hipMemcpyAsync(h.data(), mx, 8, hipMemcpyDeviceToHost, st));
@@ -275,6 +276,14 @@ double fixed_wsum(Ctx &c, const float *v, const double *w, uint64_t n, hipStream
     uint64_t T = 0;
     for (uint32_t b = 0; b < nblk; ++b) T += h[b + 1];
     return ldexp((double)T, -fixed_shift(mxv, n));
+}
+
+double fixed_wsum(Ctx &c, const float *v, const double *w, uint64_t n, hipStream_t st) {
+    return fixed_wsum_t(c, v, w, n, st);
+}
+
+double fixed_wsum(Ctx &c, const double *v, const double *w, uint64_t n, hipStream_t st) {
+    return fixed_wsum_t(c, v, w, n, st);
 }
 
 void restart_draws(Ctx &c, const float *costs_d, uint64_t n, uint64_t nempty, uint64_t *rows_h, hipStream_t st) {

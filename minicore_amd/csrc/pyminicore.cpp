@@ -233,9 +233,11 @@ std::vector<int64_t> index_list(py::object centers) {
 }
 
 // k x d center block from an array, a list of arrays or indices (pyhelpers.h:122-218)
-std::vector<float> center_block(py::object centers, uint64_t d, const std::function<std::vector<float>(const std::vector<int64_t> &)> &by_index,
-                                uint64_t *k_out) {
-    std::vector<float> C;
+template <class T>
+std::vector<T> center_block_t(py::object centers, uint64_t d,
+                              const std::function<std::vector<T>(const std::vector<int64_t> &)> &by_index,
+                              uint64_t *k_out) {
+    std::vector<T> C;
     if (is_index_list(centers)) {
         auto ids = index_list(centers);
         C = by_index(ids);
@@ -243,7 +245,7 @@ std::vector<float> center_block(py::object centers, uint64_t d, const std::funct
         return C;
     }
     if (py::isinstance<py::array>(centers)) {
-        auto a = py::array_t<float, py::array::c_style | py::array::forcecast>::ensure(centers);
+        auto a = py::array_t<T, py::array::c_style | py::array::forcecast>::ensure(centers);
         if (!a || a.ndim() != 2 || (uint64_t)a.shape(1) != d)
             throw std::invalid_argument("centers must be k x d (d = number of columns)");
         *k_out = (uint64_t)a.shape(0);
@@ -253,7 +255,7 @@ std::vector<float> center_block(py::object centers, uint64_t d, const std::funct
     if (py::isinstance<py::sequence>(centers)) {
         uint64_t k = 0;
         for (auto item : py::cast<py::sequence>(centers)) {
-            auto a = py::array_t<float, py::array::c_style | py::array::forcecast>::ensure(item);
+            auto a = py::array_t<T, py::array::c_style | py::array::forcecast>::ensure(item);
             if (!a || (uint64_t)a.size() != d) throw std::invalid_argument("each center must have d entries");
             C.insert(C.end(), a.data(), a.data() + d);
             ++k;
@@ -262,6 +264,12 @@ std::vector<float> center_block(py::object centers, uint64_t d, const std::funct
         return C;
     }
     throw std::invalid_argument("centers must be a 2d numpy array or sequence containing numpy arrays of the full vectors or center ids");
+}
+
+std::vector<float> center_block(py::object centers, uint64_t d,
+                                const std::function<std::vector<float>(const std::vector<int64_t> &)> &by_index,
+                                uint64_t *k_out) {
+    return center_block_t<float>(centers, d, by_index, k_out);
 }
 
 // centers2pylist (pyhelpers.h:66-112): ((data f32, idx u32, indptr u64), shape u64[2]) of the non-zeros
@@ -332,6 +340,31 @@ py::object hcluster_dense(py::array dataset, py::object centers, double prior, p
     const int m = measure_code(msr);
     Dense D = dense_of(dataset);
     uint64_t k = 0;
+    if (D.f64 && m == MCG_SQRL2 && mbsize < 0) {
+        // float64 rows, SQRL2: the FT = double instantiation (pycluster_dense.cpp:85-90),
+        // every value in FP64 (mcg_lloyd_f64)
+        auto C = center_block_t<double>(centers, D.d, [&](const std::vector<int64_t> &ids) {
+            std::vector<double> out;
+            for (auto r : ids) {
+                if (r < 0 || (uint64_t)r >= D.n) throw std::invalid_argument("center index out of range");
+                const double *p = (const double *)D.arr.data() + (uint64_t)r * D.d;
+                out.insert(out.end(), p, p + D.d);
+            }
+            return out;
+        }, &k);
+        if (k == 0) throw std::invalid_argument("no centers");
+        auto w = weights_f64(weights, D.n);
+        py::array_t<uint32_t> asn(D.n);
+        py::array_t<double> costs(D.n);
+        double ic = 0, fc = 0;
+        uint64_t it = 0;
+        check(mcg_lloyd_f64(D.dev->h, C.data(), k, wptr(w), eps, maxiter, asn.mutable_data(), costs.mutable_data(),
+                            &ic, &fc, &it));
+        py::array_t<double> ctr({(py::ssize_t)k, (py::ssize_t)D.d});
+        std::memcpy(ctr.mutable_data(), C.data(), k * D.d * 8);
+        return py::dict("initcost"_a = ic, "finalcost"_a = fc, "numiter"_a = it, "costs"_a = costs, "asn"_a = asn,
+                        "centers"_a = ctr);
+    }
     auto C = center_block(centers, D.d, [&](const std::vector<int64_t> &ids) {
         std::vector<float> out;
         for (auto r : ids) { auto v = row_f32(D, r); out.insert(out.end(), v.begin(), v.end()); }

@@ -1836,3 +1836,151 @@ uint64_t mco_kmeanspp_local(int msr, const float *X, const float *val, const uin
     if (rounds > 0 && k >= 2) ls_rounds(msr, X, val, idx, indptr, n, d, k, prior, seed, w, rounds, L + k - 1, centers, asn, distf);
     return k;
 }
+
+/* ======================================================================
+ * float64 rows (pycluster_dense.cpp:85-90 instantiates the dense path with
+ * FT = double for float64 arrays): perform_hard_clustering (solve.h:98-175) for
+ * SQRL2 with every value in FP64.  Per pair the sequential FMA chain
+ * s = fma(x_j - c_j, x_j - c_j, s) over j (FP64); the lowest cost wins, ties to
+ * the lower index, the first center's cost sticks when it is NaN.  Means: the
+ * chunked member order of mco_centroids_dense, on double rows, kept in double.
+ * The control flow compares FP64 costs (no rounding to float: FT = double);
+ * each cost is the exact sum of w_i * cost_i at the fixed-point scale of the
+ * sampler (ldexp of an integer sum), so it does not depend on summation order.
+ * An empty cluster takes the row drawn over the per-point costs (RNG seeded
+ * with n), every row is re-assigned and the drawn rows are forced onto their
+ * clusters (centroid.h:687-752), as in centroids_with_restart.
+ * ====================================================================== */
+double mco_sqrl2_f64(const double *x, const double *c, uint64_t d) {
+    double s = 0.;
+    for (uint64_t j = 0; j < d; ++j) { const double t = x[j] - c[j]; s = fma(t, t, s); }
+    return s;
+}
+
+void mco_assign_sqrl2_f64(const double *X, uint64_t n, uint64_t d, const double *C, uint64_t k,
+                          uint32_t *asn, double *cost) {
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const double *x = X + (uint64_t)i * d;
+        double best = mco_sqrl2_f64(x, C, d);
+        uint32_t bi = 0;
+        for (uint64_t c = 1; c < k; ++c) {
+            const double v = mco_sqrl2_f64(x, C + c * d, d);
+            if (v < best) { best = v; bi = (uint32_t)c; }
+        }
+        asn[i] = bi; cost[i] = best;
+    }
+}
+
+uint64_t mco_centroids_dense_f64(const double *X, uint64_t n, uint64_t d, const uint32_t *asn, const double *w,
+                                 uint64_t k, uint64_t chunk, double *C) {
+    uint64_t *cnt = (uint64_t *)calloc(k + 1, sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; ++i) cnt[asn[i] + 1]++;
+    for (uint64_t c = 0; c < k; ++c) cnt[c + 1] += cnt[c];
+    uint64_t *perm = (uint64_t *)malloc((n ? n : 1) * sizeof(uint64_t));
+    uint64_t *cur = (uint64_t *)malloc(k * sizeof(uint64_t));
+    memcpy(cur, cnt, k * sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; ++i) perm[cur[asn[i]]++] = i;
+    uint64_t empty = 0;
+    if (chunk == 0) chunk = MCO_MEAN_CHUNK;
+    #pragma omp parallel for schedule(dynamic, 1) reduction(+:empty)
+    for (int64_t c = 0; c < (int64_t)k; ++c) {
+        uint64_t b = cnt[c], e = cnt[c + 1];
+        if (b == e) { ++empty; continue; }
+        double *tot = (double *)calloc(d, sizeof(double)), *part = (double *)malloc(d * sizeof(double));
+        double wt = 0., wpart;
+        for (uint64_t s = b; s < e; s += chunk) {
+            uint64_t se = s + chunk < e ? s + chunk : e;
+            memset(part, 0, d * sizeof(double)); wpart = 0.;
+            for (uint64_t p = s; p < se; ++p) {
+                const double *x = X + perm[p] * d;
+                double wi = w ? w[perm[p]] : 1.;
+                for (uint64_t j = 0; j < d; ++j) part[j] = w ? fma(wi, x[j], part[j]) : part[j] + x[j];
+                wpart += wi;
+            }
+            for (uint64_t j = 0; j < d; ++j) tot[j] += part[j];
+            wt += wpart;
+        }
+        for (uint64_t j = 0; j < d; ++j) C[c * d + j] = tot[j] / wt;
+        free(tot); free(part);
+    }
+    free(cnt); free(perm); free(cur);
+    return empty;
+}
+
+/* exact sum of w_i * cost_i (cost_i >= 0) at the sampler's fixed-point scale */
+double mco_fixed_total(const double *cost, const double *w, uint64_t n) {
+    double mx = 0.;
+    for (uint64_t i = 0; i < n; ++i) {
+        const double t = fabs(w ? w[i] * cost[i] : cost[i]);
+        if (t > mx) mx = t;
+    }
+    const int s = mco_fixed_shift(mx, n);
+    uint64_t T = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        const double t = w ? w[i] * cost[i] : cost[i];
+        if (t > 0.) T += (uint64_t)floor(ldexp(t, s));
+    }
+    return ldexp((double)T, -s);
+}
+
+static int centroids_with_restart_f64(const double *X, uint64_t n, uint64_t d, double *C, uint64_t k,
+                                      const double *w, uint32_t *asn, double *cost) {
+    uint64_t *cnt = (uint64_t *)calloc(k, sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; ++i) cnt[asn[i]]++;
+    int restarted = 0;
+    uint64_t draw = 0, ne = 0;
+    for (uint64_t c = 0; c < k; ++c) if (!cnt[c]) ++ne;
+    if (ne) {
+        restarted = 1;
+        uint32_t *rs = (uint32_t *)malloc(ne * sizeof(uint32_t)), *sa = (uint32_t *)malloc(ne * sizeof(uint32_t));
+        uint64_t *fw = (uint64_t *)malloc(n * sizeof(uint64_t));
+        mco_fixed_weights(cost, n, fw);
+        uint64_t e = 0;
+        for (uint64_t c = 0; c < k; ++c) if (!cnt[c]) {
+            const uint64_t r = mco_fixed_draw(fw, n, mco_wyrand_at(n, draw++));
+            rs[e] = (uint32_t)r; sa[e] = (uint32_t)c; ++e;
+            memcpy(C + c * d, X + r * d, d * sizeof(double));
+        }
+        free(fw);
+        mco_assign_sqrl2_f64(X, n, d, C, k, asn, cost);
+        for (uint64_t i = 0; i < ne; ++i) if (asn[rs[i]] != sa[i]) { asn[rs[i]] = sa[i]; cost[rs[i]] = 0.; }
+        free(rs); free(sa);
+    }
+    free(cnt);
+    mco_centroids_dense_f64(X, n, d, asn, w, k, MCO_MEAN_CHUNK, C);
+    return restarted;
+}
+
+int mco_lloyd_f64(const double *X, uint64_t n, uint64_t d, double *C, uint64_t k, const double *w, double eps,
+                  uint64_t maxiter, uint32_t *asn, double *cost, double *initcost, double *finalcost,
+                  uint64_t *numiter) {
+    mco_assign_sqrl2_f64(X, n, d, C, k, asn, cost);
+    double fcost = mco_fixed_total(cost, w, n);
+    *initcost = fcost;
+    if (fcost == 0.) { *finalcost = 0.; *numiter = 0; return 0; }
+    double *Cc = (double *)malloc(k * d * sizeof(double));
+    memcpy(Cc, C, k * d * sizeof(double));
+    uint64_t iter = 0;
+    int rc = 0;
+    for (;;) {
+        const int res = centroids_with_restart_f64(X, n, d, Cc, k, w, asn, cost);
+        int nan = 0;
+        for (uint64_t q = 0; q < k * d; ++q) if (isnan(Cc[q])) { nan = 1; break; }
+        if (nan) { rc = -1; break; }
+        mco_assign_sqrl2_f64(X, n, d, Cc, k, asn, cost);
+        const double newcost = mco_fixed_total(cost, w, n);
+        if (newcost > fcost && !res) {
+            mco_assign_sqrl2_f64(X, n, d, C, k, asn, cost);
+            break;
+        }
+        memcpy(C, Cc, k * d * sizeof(double));
+        ++iter;
+        const double old = fcost;
+        fcost = newcost;
+        if (old - newcost < eps * (newcost > old ? newcost : old) || iter > maxiter) break;
+    }
+    free(Cc);
+    *finalcost = fcost; *numiter = iter;
+    return rc;
+}

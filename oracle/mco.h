@@ -168,4 +168,15 @@ void mco_coreset_sample(uint64_t n, const float *probs, const double *w, uint64_
 #ifdef __cplusplus
 }
 #endif
+/* float64 rows, dense SQRL2 (FT = double): FP64 chains, means and control flow */
+double   mco_sqrl2_f64(const double *x, const double *c, uint64_t d);
+void     mco_assign_sqrl2_f64(const double *X, uint64_t n, uint64_t d, const double *C, uint64_t k,
+                              uint32_t *asn, double *cost);
+uint64_t mco_centroids_dense_f64(const double *X, uint64_t n, uint64_t d, const uint32_t *asn, const double *w,
+                                 uint64_t k, uint64_t chunk, double *C);
+double   mco_fixed_total(const double *cost, const double *w, uint64_t n);
+int      mco_lloyd_f64(const double *X, uint64_t n, uint64_t d, double *C, uint64_t k, const double *w, double eps,
+                       uint64_t maxiter, uint32_t *asn, double *cost, double *initcost, double *finalcost,
+                       uint64_t *numiter);
+
 #endif

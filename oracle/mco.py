@@ -77,6 +77,11 @@ def _setup(L):
         "mco_kmeanspp_local": (U64, [I, P, P, P, P, U64, U64, U64, D, U64, P, U64, U64, P, P, P]),
         "mco_coreset_probs": (I, [I, U64, U64, P, P, P, P]),
         "mco_coreset_sample": (None, [U64, P, P, U64, U64, U64, P, P]),
+        "mco_sqrl2_f64": (D, [P, P, U64]),
+        "mco_assign_sqrl2_f64": (None, [P, U64, U64, P, U64, P, P]),
+        "mco_centroids_dense_f64": (U64, [P, U64, U64, P, P, U64, U64, P]),
+        "mco_fixed_total": (D, [P, P, U64]),
+        "mco_lloyd_f64": (I, [P, U64, U64, P, U64, P, D, U64, P, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -280,6 +285,54 @@ def lloyd(msr, X, C0, prior=0., w=None, eps=1e-10, maxiter=100, nthreads=0):
                          maxiter, _p(asn), _p(cost), _p(ic), _p(fc), _p(it), nthreads)
     if rc != 0:
         raise RuntimeError("oracle lloyd failed (NaN center)")
+    return dict(centers=C, asn=asn, costs=cost, initcost=float(ic[0]), finalcost=float(fc[0]),
+                numiter=int(it[0]))
+
+
+def _f64c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def assign_sqrl2_f64(X, C):
+    """float64 rows: FP64 sequential-FMA argmin (lowest index on ties)."""
+    X, C = _f64c(X), _f64c(C)
+    n, d = X.shape
+    asn = np.empty(n, dtype=np.uint32)
+    cost = np.empty(n, dtype=np.float64)
+    lib().mco_assign_sqrl2_f64(_p(X), n, d, _p(C), C.shape[0], _p(asn), _p(cost))
+    return asn, cost
+
+
+def centroids_f64(X, asn, k, C_prev, w=None, chunk=256):
+    X = _f64c(X)
+    n, d = X.shape
+    C = _f64c(C_prev).copy()
+    w = _f64(w)
+    e = lib().mco_centroids_dense_f64(_p(X), n, d, _p(np.ascontiguousarray(asn, dtype=np.uint32)), _p(w), k,
+                                      chunk, _p(C))
+    return C, int(e)
+
+
+def fixed_total(cost, w=None):
+    cost = _f64c(cost)
+    return lib().mco_fixed_total(_p(cost), _p(_f64(w)), cost.shape[0])
+
+
+def lloyd_f64(X, C0, w=None, eps=1e-10, maxiter=100):
+    """perform_hard_clustering on float64 rows, dense SQRL2, everything in FP64."""
+    X = _f64c(X)
+    n, d = X.shape
+    C = _f64c(C0).copy()
+    k = C.shape[0]
+    w = _f64(w)
+    asn = np.empty(n, dtype=np.uint32)
+    cost = np.empty(n, dtype=np.float64)
+    ic, fc = np.zeros(1), np.zeros(1)
+    it = np.zeros(1, dtype=np.uint64)
+    rc = lib().mco_lloyd_f64(_p(X), n, d, _p(C), k, _p(w), eps, maxiter, _p(asn), _p(cost), _p(ic), _p(fc),
+                             _p(it))
+    if rc != 0:
+        raise RuntimeError("oracle lloyd_f64 failed (NaN center)")
     return dict(centers=C, asn=asn, costs=cost, initcost=float(ic[0]), finalcost=float(fc[0]),
                 numiter=int(it[0]))
 

@@ -246,13 +246,19 @@ __device__ __forceinline__ void stage_walk(const float *Tw, const uint2 *Es, con
     for (int i = 0; i < TWR / 4; ++i) cw[i] = __builtin_amdgcn_readfirstlane(Cs[wave * (TWR / 4) + i]);
     const char *T = (const char *)Tw;
     uint32_t e = 0;
+    // the next entry's offset and value are read out while this entry's operand row
+    // loads (after the last entry, lane (e + 1) & 63 is read out and goes unused)
+    uint32_t bo = __builtin_amdgcn_readlane(ev.x, 0);
+    float x = __int_as_float(__builtin_amdgcn_readlane((int)ev.y, 0));
 #pragma unroll
     for (int a = 0; a < TWR; ++a) {
         const uint32_t end = (cw[a >> 2] >> (8 * (a & 3))) & 0xFFu;
         for (; e < end; ++e) {
-            const uint32_t bo = __builtin_amdgcn_readlane(ev.x, e);
-            const float x = __int_as_float(__builtin_amdgcn_readlane((int)ev.y, e));
-            fn(a, x, (const float *)(T + bo));
+            const float *tp = (const float *)(T + bo);
+            const float xc = x;
+            bo = __builtin_amdgcn_readlane(ev.x, (e + 1) & 63);
+            x = __int_as_float(__builtin_amdgcn_readlane((int)ev.y, (e + 1) & 63));
+            fn(a, xc, tp);
         }
     }
 }
@@ -425,7 +431,12 @@ static bool tiles_enabled() {
 }
 
 // centers per lane of the E-step tiles (the row kernel serves k <= 64)
-static int estep_cpl(uint64_t k) { return k > 128 ? 4 : 2; }
+// (MCG_TILES_ECPL = 2 or 4 overrides)
+static int estep_cpl(uint64_t k) {
+    static const int forced = [] { const char *e = getenv("MCG_TILES_ECPL"); return e ? atoi(e) : 0; }();
+    if (forced == 2 || forced == 4) return forced;
+    return k > 128 ? 4 : 2;
+}
 
 // The tiles pay for their build (a few passes over the entries) and their staging
 // when the product is large; JSD's entry terms are bound by their logarithms (two

@@ -265,6 +265,8 @@ __device__ __forceinline__ void stage_walk(const float *Tw, const uint2 *Es, con
 
 // The stage loop shared by both products: buffers A / B are separate LDS objects and
 // the loop is unrolled by two, so a DMA into one buffer never waits on reads of the other.
+// One barrier per stage: the barrier after a stage's DMA has landed is also the point
+// every wave passes only after walking the buffer the next DMA refills.
 template <int CPL, int TWR, bool ALIGN16, class Walk>
 __device__ __forceinline__ void stage_loop(const Stage<CPL, ALIGN16> &S, const uint32_t *__restrict__ off, uint64_t t0,
                                            uint32_t nstages, float *TwA, float *TwB, uint2 *EsA, uint2 *EsB,
@@ -291,7 +293,6 @@ __device__ __forceinline__ void stage_loop(const Stage<CPL, ALIGN16> &S, const u
             offv = load_off(s + 2);
         }
         walk(TwA, EsA, CsA, WsA);
-        __syncthreads();   // A may be refilled
         if (s + 1 >= nstages) break;
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
@@ -301,7 +302,6 @@ __device__ __forceinline__ void stage_loop(const Stage<CPL, ALIGN16> &S, const u
             offv = load_off(s + 3);
         }
         walk(TwB, EsB, CsB, WsB);
-        __syncthreads();
     }
 }
 

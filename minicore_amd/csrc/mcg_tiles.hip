@@ -194,10 +194,13 @@ __device__ __forceinline__ void dma(const void *base, uint32_t nbytes, void *dst
 
 // The block's staging of one stage: the operand rows [s*ts, s*ts + ts) x columns
 // [c0, c0 + CW) into Tw (row-major, CW floats a row), the stage's entries into Es, the
-// end offsets into Cs and the wave bases into Ws.  ALIGN16: ld and c0 are multiples of
-// 4, so 16-byte pieces (1 KB per wave instruction), else dwords (256 bytes).
-template <int CPL, bool ALIGN16>
+// end offsets into Cs and the wave bases into Ws.  MODE 1 and 2: ld and c0 are multiples
+// of 4, so 16-byte pieces (1 KB per wave instruction); MODE 2 also: the operand (and the
+// last stage's rows past its end) spans under 4 GiB, so one buffer resource covers all of
+// it; MODE 0: dwords (256 bytes per wave instruction).
+template <int CPL, int MODE>
 struct Stage {
+    static constexpr bool ALIGN16 = MODE >= 1;
     static constexpr int CW = 64 * CPL;
     const float *op;
     uint64_t ld, nop;
@@ -205,13 +208,47 @@ struct Stage {
     const uint2 *ent;
     const uint8_t *cnt;
     const uint16_t *wb;
+    // MODE 2 stages through one buffer resource over the whole operand: the lane's part
+    // of the offset is fixed and the stage's and instruction's parts are scalar
+    // (soffset), so an instruction costs little more than its LDS destination.  Rows
+    // past the operand's end read 0 either way.
+    __amdgpu_buffer_rsrc_t wr;
+    uint32_t wlane;
+    __device__ __forceinline__ Stage(const float *op_, uint64_t ld_, uint64_t nop_, uint32_t c0_, uint32_t ts_,
+                                     uint32_t tg_, const uint2 *ent_, const uint8_t *cnt_, const uint16_t *wb_,
+                                     int lane)
+        : op(op_), ld(ld_), nop(nop_), c0(c0_), ts(ts_), tg(tg_), ent(ent_), cnt(cnt_), wb(wb_) {
+        wr = __builtin_amdgcn_make_buffer_rsrc((void *)op, (short)0, (int)(MODE == 2 ? nop * ld * 4 : 0), 0x00020000);
+        constexpr int RPI = 4 / CPL, LPR = 64 / RPI;
+        wlane = (uint32_t)(((lane / LPR) * ld + c0 + (lane % LPR) * 4) * 4);
+    }
     __device__ __forceinline__ void issue(uint64_t t, uint32_t s, uint32_t seg, uint32_t sz, float *Tw, uint2 *Es,
                                           uint32_t *Cs, uint32_t *Ws, int wave, int lane) const {
         const uint64_t r0 = (uint64_t)s * ts;
         const uint32_t rows = (uint32_t)min<uint64_t>(ts, nop - r0);
         const float *base = op + r0 * ld;
         const uint32_t nb = (uint32_t)((uint64_t)rows * ld * 4);
-        if (ALIGN16) {
+        if constexpr (MODE == 2) {
+            constexpr int RPI = 4 / CPL;
+            const uint32_t ni = ts / RPI;
+            const uint32_t step = (uint32_t)(TNW * RPI * ld * 4);
+            uint32_t so = (uint32_t)((r0 + (uint64_t)wave * RPI) * ld * 4);
+            auto *l = (__attribute__((address_space(3))) char *)(Tw + wave * RPI * CW);
+            constexpr uint32_t LSTEP = TNW * RPI * CW * 4;
+            if (ni == 4 * TNW) {   // a full stage: four instructions a wave, unrolled
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(wr, (__attribute__((address_space(3))) void *)(l + j * LSTEP),
+                                                             16, wlane, so + (uint32_t)j * step, 0, 0);
+            } else {
+                for (uint32_t q = wave; q < ni; q += TNW) {
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(wr, (__attribute__((address_space(3))) void *)l, 16, wlane, so,
+                                                             0, 0);
+                    so += step;
+                    l += LSTEP;
+                }
+            }
+        } else if constexpr (MODE == 1) {
             constexpr int RPI = 4 / CPL, LPR = 64 / RPI;   // operand rows per instruction, lanes per row
             const uint32_t ni = ts / RPI;
             const int rl = lane / LPR, ch = lane % LPR;
@@ -245,20 +282,22 @@ __device__ __forceinline__ void stage_walk(const float *Tw, const uint2 *Es, con
 #pragma unroll
     for (int i = 0; i < TWR / 4; ++i) cw[i] = __builtin_amdgcn_readfirstlane(Cs[wave * (TWR / 4) + i]);
     const char *T = (const char *)Tw;
-    uint32_t e = 0;
-    // the next entry's offset and value are read out while this entry's operand row
-    // loads (after the last entry, lane (e + 1) & 63 is read out and goes unused)
+    // the next entry's offset is read out while this entry's operand row loads (after
+    // the last entry, lane e & 63 is read out and goes unused); each accumulator's range
+    // comes straight from the end offsets, so nothing but the entry index is carried
+    // from entry to entry (the walk is bound by its scalar instructions)
     uint32_t bo = __builtin_amdgcn_readlane(ev.x, 0);
-    float x = __int_as_float(__builtin_amdgcn_readlane((int)ev.y, 0));
+    auto endof = [&](int a) -> uint32_t { return (cw[a >> 2] >> (8 * (a & 3))) & 0xFFu; };
 #pragma unroll
     for (int a = 0; a < TWR; ++a) {
-        const uint32_t end = (cw[a >> 2] >> (8 * (a & 3))) & 0xFFu;
-        for (; e < end; ++e) {
+        uint32_t e = a ? endof(a - 1) : 0u;
+        const uint32_t end = endof(a);
+        while (e < end) {
             const float *tp = (const float *)(T + bo);
-            const float xc = x;
-            bo = __builtin_amdgcn_readlane(ev.x, (e + 1) & 63);
-            x = __int_as_float(__builtin_amdgcn_readlane((int)ev.y, (e + 1) & 63));
-            fn(a, xc, tp);
+            const float x = __int_as_float(__builtin_amdgcn_readlane((int)ev.y, e));
+            ++e;
+            bo = __builtin_amdgcn_readlane(ev.x, e & 63);
+            fn(a, x, tp);
         }
     }
 }
@@ -267,8 +306,8 @@ __device__ __forceinline__ void stage_walk(const float *Tw, const uint2 *Es, con
 // the loop is unrolled by two, so a DMA into one buffer never waits on reads of the other.
 // One barrier per stage: the barrier after a stage's DMA has landed is also the point
 // every wave passes only after walking the buffer the next DMA refills.
-template <int CPL, int TWR, bool ALIGN16, class Walk>
-__device__ __forceinline__ void stage_loop(const Stage<CPL, ALIGN16> &S, const uint32_t *__restrict__ off, uint64_t t0,
+template <int CPL, int TWR, int MODE, class Walk>
+__device__ __forceinline__ void stage_loop(const Stage<CPL, MODE> &S, const uint32_t *__restrict__ off, uint64_t t0,
                                            uint32_t nstages, float *TwA, float *TwB, uint2 *EsA, uint2 *EsB,
                                            uint32_t *CsA, uint32_t *CsB, uint32_t *WsA, uint32_t *WsB, Walk &&walk) {
     const int lane = threadIdx.x & 63;
@@ -320,7 +359,7 @@ __device__ __forceinline__ void stage_loop(const Stage<CPL, ALIGN16> &S, const u
 // wave, CPL centers a lane), then the per-row / per-center terms of the decomposition
 // (as k_row_costs) and the cost matrix and/or the row minimum (64-bit keys: cost bits
 // above, center below; lowest cost, then lowest center).  SQRL2, MKL, REVERSE_MKL.
-template <int MSR, int CPL>
+template <int MSR, int CPL, int MODE>
 __global__ __launch_bounds__(1024) void k_tile_estep(const uint8_t *__restrict__ cnt, const uint16_t *__restrict__ wb,
                                                      const uint32_t *__restrict__ off, const uint2 *__restrict__ ent,
                                                      uint64_t n, uint32_t nstages, uint32_t ts, uint32_t d,
@@ -341,7 +380,7 @@ __global__ __launch_bounds__(1024) void k_tile_estep(const uint8_t *__restrict__
     for (int a = 0; a < TWR; ++a)
 #pragma unroll
         for (int u = 0; u < CPL; ++u) acc[a][u] = 0.f;
-    const Stage<CPL, true> S{CT, kpad, d, c0, ts, (uint32_t)TG, ent, cnt, wb};
+    const Stage<CPL, MODE> S(CT, kpad, d, c0, ts, (uint32_t)TG, ent, cnt, wb, lane);
     auto walk = [&](const float *Tw, const uint2 *Es, const uint32_t *Cs, const uint32_t *Ws) {
         stage_walk<TWR>(Tw + lane * CPL, Es, Cs, Ws, wave, lane, pv, MSR == MCG_MKL,
                         [&](int a, float x, const float *tp) {
@@ -485,9 +524,13 @@ void tiled_costs(Data &ds, int msr, const float *CT, uint32_t kpad, uint64_t k, 
     const uint32_t d = (uint32_t)ds.d;
     hipEvent_t ev;
     c.tic(0, ev);
-#define TILE_E(M, P)                                                                                              \
-    hipLaunchKernelGGL((k_tile_estep<M, P>), grid, blk, 0, st, cn, w, o, e, ds.n, T.nstages, T.ts, d, CT, kpad,      \
+    // one buffer resource over the whole table when it (and the last stage's rows past
+    // its end) spans under 4 GiB
+    const bool whole = ((uint64_t)d + T.ts) * kpad * 4 < (1ull << 32);
+#define TILE_E1(M, P, W)                                                                                          \
+    hipLaunchKernelGGL((k_tile_estep<M, P, W>), grid, blk, 0, st, cn, w, o, e, ds.n, T.nstages, T.ts, d, CT, kpad,   \
                        (uint32_t)k, nslices, pv, logpv, rt, cs, out_nk, keys, nanflag)
+#define TILE_E(M, P) do { if (whole) TILE_E1(M, P, 2); else TILE_E1(M, P, 1); } while (0)
     if (T.cpl == 4) {
         switch (msr) {
         case MCG_SQRL2: TILE_E(MCG_SQRL2, 4); break;
@@ -502,6 +545,7 @@ void tiled_costs(Data &ds, int msr, const float *CT, uint32_t kpad, uint64_t k, 
         }
     }
 #undef TILE_E
+#undef TILE_E1
     c.toc(0, ev);
     if (keys)
         hipLaunchKernelGGL(k_keys_out, dim3((uint32_t)std::min<uint64_t>(cdiv(ds.n, 256), 4096)), dim3(256), 0, st,
@@ -511,7 +555,7 @@ void tiled_costs(Data &ds, int msr, const float *CT, uint32_t kpad, uint64_t k, 
 // ---- soft M-step numerator -----------------------------------------------------------------
 // num[j][c] = sum over column j's entries (rows ascending) of R[i][c] * x (FP64): TG =
 // 16 * TWR columns (TWR = 32 / CPL a wave) x 64 * CPL centers per block.
-template <int CPL, bool ALIGN16>
+template <int CPL, int MODE>
 __global__ __launch_bounds__(1024) void k_tile_mstep(const uint8_t *__restrict__ cnt, const uint16_t *__restrict__ wb,
                                                      const uint32_t *__restrict__ off, const uint2 *__restrict__ ent,
                                                      uint32_t d, uint32_t nstages, uint32_t ts, uint64_t n,
@@ -528,7 +572,7 @@ __global__ __launch_bounds__(1024) void k_tile_mstep(const uint8_t *__restrict__
     for (int a = 0; a < TWR; ++a)
 #pragma unroll
         for (int u = 0; u < CPL; ++u) acc[a][u] = 0.;
-    const Stage<CPL, ALIGN16> S{R, k, n, c0, ts, (uint32_t)TG, ent, cnt, wb};
+    const Stage<CPL, MODE> S(R, k, n, c0, ts, (uint32_t)TG, ent, cnt, wb, lane);
     auto walk = [&](const float *Tw, const uint2 *Es, const uint32_t *Cs, const uint32_t *Ws) {
         stage_walk<TWR>(Tw + lane * CPL, Es, Cs, Ws, wave, lane, 0.f, false, [&](int a, float x, const float *tp) {
             const double xd = (double)x;
@@ -603,13 +647,14 @@ void tiled_soft_numer(Data &ds, const float *R, uint64_t k, const double *den, f
     const bool a16 = k % 4 == 0;
     hipEvent_t ev;
     c.tic(1, ev);
+    const int mode = !a16 ? 0 : ((ds.n + T.ts) * k * 4 < (1ull << 32) ? 2 : 1);
 #define TILE_M(P, A)                                                                                             \
     hipLaunchKernelGGL((k_tile_mstep<P, A>), grid, blk, 0, st, cn, w, o, e, d, T.nstages, T.ts, ds.n, R, (uint32_t)k, \
                        nslices, kpad, num)
     switch (T.cpl) {
-    case 4: if (a16) TILE_M(4, true); else TILE_M(4, false); break;
-    case 2: if (a16) TILE_M(2, true); else TILE_M(2, false); break;
-    default: if (a16) TILE_M(1, true); else TILE_M(1, false); break;
+    case 4: if (mode == 2) TILE_M(4, 2); else if (mode == 1) TILE_M(4, 1); else TILE_M(4, 0); break;
+    case 2: if (mode == 2) TILE_M(2, 2); else if (mode == 1) TILE_M(2, 1); else TILE_M(2, 0); break;
+    default: if (mode == 2) TILE_M(1, 2); else if (mode == 1) TILE_M(1, 1); else TILE_M(1, 0); break;
     }
 #undef TILE_M
     c.toc(1, ev);

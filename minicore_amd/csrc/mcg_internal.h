@@ -89,7 +89,7 @@ struct DevView {
 struct Tiles {
     bool ready = false;
     int cpl = 0;
-    uint32_t twr = 0, tg = 0, ts = 0;
+    uint32_t twr = 0, tg = 0, ts = 0, nw = 0;   // indices a wave, a block; stage rows; waves a block
     uint64_t nacc = 0, nop = 0;
     uint32_t ngroups = 0, nstages = 0;
     uint32_t maxseg = 0, maxwave = 0;   // largest stage segment of a block / of a wave

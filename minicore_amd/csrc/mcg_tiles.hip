@@ -31,7 +31,7 @@
 
 namespace mcg {
 
-constexpr int TNW = 16;                        // waves per block
+constexpr int TNW = 16;                        // waves per block (at most; the M-step's count)
 constexpr uint32_t WCAP = 64;                  // entries per wave and stage (one per lane)
 constexpr uint32_t ECAP = 1024;                // entries per block and stage
 constexpr uint32_t STAGE_BYTES = 64 * 1024;    // one operand stage buffer
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void k_tiles_count(const uint64_t *__restrict_
 // per tile (one block of tg threads): the waves' base offsets (exclusive prefix of the
 // entry counts of the 16 groups of twr indices), the tile's size and the largest group
 __global__ __launch_bounds__(1024) void k_tiles_sizes(const uint8_t *__restrict__ cnt, uint32_t tg, uint32_t twr,
-                                                       uint16_t *__restrict__ wb, uint32_t *__restrict__ size,
+                                                       uint32_t nw, uint16_t *__restrict__ wb, uint32_t *__restrict__ size,
                                                        uint32_t *__restrict__ maxwave) {
     __shared__ uint32_t ws[TNW];
     const uint64_t t = blockIdx.x;
@@ -80,8 +80,8 @@ __global__ __launch_bounds__(1024) void k_tiles_sizes(const uint8_t *__restrict_
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t run = 0, mx = 0;
-        for (int w = 0; w < TNW; ++w) {
-            wb[t * TNW + w] = (uint16_t)min<uint32_t>(run, 0xFFFFu);
+        for (uint32_t w = 0; w < nw; ++w) {
+            wb[t * nw + w] = (uint16_t)min<uint32_t>(run, 0xFFFFu);
             run += ws[w];
             mx = max(mx, ws[w]);
         }
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(1024) void k_tiles_sizes(const uint8_t *__restrict_
 // The counts are then replaced by the end offsets inside the wave's group.
 __global__ __launch_bounds__(1024) void k_tiles_fill(const uint64_t *__restrict__ ptr, const uint32_t *__restrict__ idx,
                                                       const float *__restrict__ val, uint64_t nacc, uint32_t nstages,
-                                                      uint32_t ts, uint32_t tg, uint32_t twr, uint32_t rowbytes,
+                                                      uint32_t ts, uint32_t tg, uint32_t twr, uint32_t nw, uint32_t rowbytes,
                                                       uint8_t *__restrict__ cnt, const uint16_t *__restrict__ wb,
                                                       const uint32_t *__restrict__ off, uint2 *__restrict__ ent) {
     const uint64_t t = blockIdx.x;
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(1024) void k_tiles_fill(const uint64_t *__restrict_
     }
     cnt[t * tg + threadIdx.x] = (uint8_t)min<uint32_t>(inc, 255);
     if (a >= nacc || c == 0) return;
-    const uint64_t pos = (uint64_t)off[t] + wb[t * TNW + grp] + (inc - c);
+    const uint64_t pos = (uint64_t)off[t] + wb[t * nw + grp] + (inc - c);
     const uint64_t b = ptr[a], m = ptr[a + 1] - b;
     const uint64_t lo = b + lbound(idx + b, m, s * ts);
     for (uint32_t q = 0; q < c; ++q)
@@ -131,12 +131,14 @@ __global__ void k_max_u32(const uint32_t *__restrict__ a, uint64_t n, uint32_t *
 // twr indices per wave and rowbytes bytes per staged operand row; TS starts at the
 // largest value the stage buffer holds and halves while a stage overflows.
 static void build_tiles(Tiles &T, const uint64_t *ptr, const uint32_t *idx, const float *val, uint64_t nacc,
-                        uint64_t nop, uint64_t nnz, int cpl, uint32_t twr, uint32_t rowbytes, hipStream_t st) {
+                        uint64_t nop, uint64_t nnz, int cpl, uint32_t twr, uint32_t nw, uint32_t rowbytes,
+                        hipStream_t st) {
     REQUIRE(nnz < (1ull << 32), "tiled product: more than 2^32 stored entries");
     T.ready = false;
     T.cpl = cpl;
     T.twr = twr;
-    T.tg = twr * TNW;
+    T.nw = nw;
+    T.tg = twr * nw;
     T.nacc = nacc;
     T.nop = nop;
     T.ngroups = (uint32_t)cdiv(nacc, T.tg);
@@ -148,7 +150,7 @@ static void build_tiles(Tiles &T, const uint64_t *ptr, const uint32_t *idx, cons
         const uint64_t nt = (uint64_t)T.ngroups * T.nstages;
         T.cnt.reserve(nt * T.tg);
         fill_u32(T.cnt.as<uint32_t>(), 0u, nt * T.tg / 4, st);   // indices beyond nacc count 0
-        T.wb.reserve(nt * TNW * sizeof(uint16_t));
+        T.wb.reserve(nt * nw * sizeof(uint16_t));
         T.off.reserve((nt + 1) * sizeof(uint32_t));
         size.reserve((nt + 1) * sizeof(uint32_t));
         fill_u32(mx.as<uint32_t>(), 0u, 2, st);
@@ -156,7 +158,7 @@ static void build_tiles(Tiles &T, const uint64_t *ptr, const uint32_t *idx, cons
                            ts, T.tg, T.cnt.as<uint8_t>());
         fill_u32(size.as<uint32_t>() + nt, 0u, 1, st);
         hipLaunchKernelGGL(k_tiles_sizes, dim3((uint32_t)nt), dim3(T.tg), 0, st, T.cnt.as<const uint8_t>(), T.tg, twr,
-                           T.wb.as<uint16_t>(), size.as<uint32_t>(), mx.as<uint32_t>());
+                           nw, T.wb.as<uint16_t>(), size.as<uint32_t>(), mx.as<uint32_t>());
         scan_exclusive(size.as<const uint32_t>(), T.off.as<uint32_t>(), nt + 1, tmp, st);
         hipLaunchKernelGGL(k_max_u32, dim3((uint32_t)std::min<uint64_t>(cdiv(nt, 256), 1024)), dim3(256), 0, st,
                            size.as<const uint32_t>(), nt, mx.as<uint32_t>() + 1);
@@ -167,7 +169,7 @@ static void build_tiles(Tiles &T, const uint64_t *ptr, const uint32_t *idx, cons
         if (T.maxwave > WCAP || T.maxseg > ECAP) continue;
         T.ent.reserve(std::max<uint64_t>(nnz, 1) * sizeof(uint2));
         hipLaunchKernelGGL(k_tiles_fill, dim3((uint32_t)nt), dim3(T.tg), 0, st, ptr, idx, val, nacc, T.nstages, ts, T.tg,
-                           twr, rowbytes, T.cnt.as<uint8_t>(), T.wb.as<const uint16_t>(), T.off.as<const uint32_t>(),
+                           twr, nw, rowbytes, T.cnt.as<uint8_t>(), T.wb.as<const uint16_t>(), T.off.as<const uint32_t>(),
                            T.ent.as<uint2>());
         HIP_CHECK(hipStreamSynchronize(st));   // the temporaries are freed on return
         T.ready = true;
@@ -198,7 +200,7 @@ __device__ __forceinline__ void dma(const void *base, uint32_t nbytes, void *dst
 // of 4, so 16-byte pieces (1 KB per wave instruction); MODE 2 also: the operand (and the
 // last stage's rows past its end) spans under 4 GiB, so one buffer resource covers all of
 // it; MODE 0: dwords (256 bytes per wave instruction).
-template <int CPL, int MODE>
+template <int CPL, int MODE, int NW>
 struct Stage {
     static constexpr bool ALIGN16 = MODE >= 1;
     static constexpr int CW = 64 * CPL;
@@ -231,17 +233,17 @@ struct Stage {
         if constexpr (MODE == 2) {
             constexpr int RPI = 4 / CPL;
             const uint32_t ni = ts / RPI;
-            const uint32_t step = (uint32_t)(TNW * RPI * ld * 4);
+            const uint32_t step = (uint32_t)(NW * RPI * ld * 4);
             uint32_t so = (uint32_t)((r0 + (uint64_t)wave * RPI) * ld * 4);
             auto *l = (__attribute__((address_space(3))) char *)(Tw + wave * RPI * CW);
-            constexpr uint32_t LSTEP = TNW * RPI * CW * 4;
-            if (ni == 4 * TNW) {   // a full stage: four instructions a wave, unrolled
+            constexpr uint32_t LSTEP = NW * RPI * CW * 4;
+            if (ni == 4 * NW) {   // a full stage of 16 waves: four instructions a wave, unrolled
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
                     __builtin_amdgcn_raw_ptr_buffer_load_lds(wr, (__attribute__((address_space(3))) void *)(l + j * LSTEP),
                                                              16, wlane, so + (uint32_t)j * step, 0, 0);
             } else {
-                for (uint32_t q = wave; q < ni; q += TNW) {
+                for (uint32_t q = wave; q < ni; q += NW) {
                     __builtin_amdgcn_raw_ptr_buffer_load_lds(wr, (__attribute__((address_space(3))) void *)l, 16, wlane, so,
                                                              0, 0);
                     so += step;
@@ -252,20 +254,20 @@ struct Stage {
             constexpr int RPI = 4 / CPL, LPR = 64 / RPI;   // operand rows per instruction, lanes per row
             const uint32_t ni = ts / RPI;
             const int rl = lane / LPR, ch = lane % LPR;
-            for (uint32_t q = wave; q < ni; q += TNW)
+            for (uint32_t q = wave; q < ni; q += NW)
                 dma<16>(base, nb, Tw + q * RPI * CW, (uint32_t)(((uint64_t)(q * RPI + rl) * ld + c0 + ch * 4) * 4));
         } else {
             const uint32_t ni = ts * CPL;                  // one instruction: 64 floats of one row
-            for (uint32_t q = wave; q < ni; q += TNW) {
+            for (uint32_t q = wave; q < ni; q += NW) {
                 const uint32_t r = q / CPL, part = q % CPL;
                 dma<4>(base, nb, Tw + q * 64, (uint32_t)(((uint64_t)r * ld + c0 + part * 64 + lane) * 4));
             }
         }
         const uint32_t ninstr = (2 * sz + 63) / 64;
-        for (uint32_t i = wave; i < ninstr; i += TNW) dma<4>(ent + seg, sz * 8, (uint32_t *)Es + i * 64, (i * 64 + lane) * 4);
-        if (wave == TNW - 1)
+        for (uint32_t i = wave; i < ninstr; i += NW) dma<4>(ent + seg, sz * 8, (uint32_t *)Es + i * 64, (i * 64 + lane) * 4);
+        if (wave == NW - 1)
             for (uint32_t i = 0; i < (tg / 4 + 63) / 64; ++i) dma<4>(cnt + t * tg, tg, Cs + i * 64, (i * 64 + lane) * 4);
-        if (wave == TNW - 2) dma<4>(wb + t * TNW, TNW * 2, Ws, lane * 4);
+        if (wave == NW - 2) dma<4>(wb + t * NW, NW * 2, Ws, lane * 4);
     }
 };
 
@@ -306,8 +308,8 @@ __device__ __forceinline__ void stage_walk(const float *Tw, const uint2 *Es, con
 // the loop is unrolled by two, so a DMA into one buffer never waits on reads of the other.
 // One barrier per stage: the barrier after a stage's DMA has landed is also the point
 // every wave passes only after walking the buffer the next DMA refills.
-template <int CPL, int TWR, int MODE, class Walk>
-__device__ __forceinline__ void stage_loop(const Stage<CPL, MODE> &S, const uint32_t *__restrict__ off, uint64_t t0,
+template <int CPL, int TWR, int MODE, int NW, class Walk>
+__device__ __forceinline__ void stage_loop(const Stage<CPL, MODE, NW> &S, const uint32_t *__restrict__ off, uint64_t t0,
                                            uint32_t nstages, float *TwA, float *TwB, uint2 *EsA, uint2 *EsB,
                                            uint32_t *CsA, uint32_t *CsB, uint32_t *WsA, uint32_t *WsB, Walk &&walk) {
     const int lane = threadIdx.x & 63;
@@ -359,8 +361,8 @@ __device__ __forceinline__ void stage_loop(const Stage<CPL, MODE> &S, const uint
 // wave, CPL centers a lane), then the per-row / per-center terms of the decomposition
 // (as k_row_costs) and the cost matrix and/or the row minimum (64-bit keys: cost bits
 // above, center below; lowest cost, then lowest center).  SQRL2, MKL, REVERSE_MKL.
-template <int MSR, int CPL, int MODE>
-__global__ __launch_bounds__(1024) void k_tile_estep(const uint8_t *__restrict__ cnt, const uint16_t *__restrict__ wb,
+template <int MSR, int CPL, int MODE, int TWR, int NW>
+__global__ __launch_bounds__(NW * 64) void k_tile_estep(const uint8_t *__restrict__ cnt, const uint16_t *__restrict__ wb,
                                                      const uint32_t *__restrict__ off, const uint2 *__restrict__ ent,
                                                      uint64_t n, uint32_t nstages, uint32_t ts, uint32_t d,
                                                      const float *__restrict__ CT, uint32_t kpad, uint32_t k,
@@ -368,7 +370,7 @@ __global__ __launch_bounds__(1024) void k_tile_estep(const uint8_t *__restrict__
                                                      const float4 *__restrict__ rt, const double *__restrict__ cs,
                                                      float *__restrict__ out, unsigned long long *__restrict__ keys,
                                                      uint8_t *__restrict__ nanflag) {
-    constexpr int TWR = 64 / CPL, TG = TWR * TNW, CW = 64 * CPL;
+    constexpr int TG = TWR * NW, CW = 64 * CPL;
     TILE_LDS
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -380,7 +382,7 @@ __global__ __launch_bounds__(1024) void k_tile_estep(const uint8_t *__restrict__
     for (int a = 0; a < TWR; ++a)
 #pragma unroll
         for (int u = 0; u < CPL; ++u) acc[a][u] = 0.f;
-    const Stage<CPL, MODE> S(CT, kpad, d, c0, ts, (uint32_t)TG, ent, cnt, wb, lane);
+    const Stage<CPL, MODE, NW> S(CT, kpad, d, c0, ts, (uint32_t)TG, ent, cnt, wb, lane);
     auto walk = [&](const float *Tw, const uint2 *Es, const uint32_t *Cs, const uint32_t *Ws) {
         stage_walk<TWR>(Tw + lane * CPL, Es, Cs, Ws, wave, lane, pv, MSR == MCG_MKL,
                         [&](int a, float x, const float *tp) {
@@ -469,12 +471,17 @@ static bool tiles_enabled() {
     return on;
 }
 
-// centers per lane of the E-step tiles (the row kernel serves k <= 64)
-// (MCG_TILES_ECPL = 2 or 4 overrides)
-static int estep_cpl(uint64_t k) {
-    static const int forced = [] { const char *e = getenv("MCG_TILES_ECPL"); return e ? atoi(e) : 0; }();
-    if (forced == 2 || forced == 4) return forced;
-    return k > 128 ? 4 : 2;
+// Shape of the E-step tiles (the row kernel serves k <= 64): centers per lane, rows per
+// wave (64 accumulators), waves per block.  (MCG_TILES_ECPL = 2 or 4 overrides the
+// centers per lane.)  Twelve waves of 32 rows at four centers a lane (a third less table
+// staging, twice the entries a wave and stage) measured the same at 10^6 x 32768 rows,
+// k = 256: the walk is bound by its per-entry instruction stream, not by the staging or
+// the barrier.
+struct EShape { int cpl; uint32_t twr, nw; };
+static EShape estep_shape(uint64_t k) {
+    static const int fc = [] { const char *e = getenv("MCG_TILES_ECPL"); return e ? atoi(e) : 0; }();
+    const int cpl = (fc == 2 || fc == 4) ? fc : (k > 128 ? 4 : 2);
+    return {cpl, (uint32_t)(64 / cpl), (uint32_t)TNW};
 }
 
 // The tiles pay for their build (a few passes over the entries) and their staging
@@ -492,11 +499,11 @@ bool tiled_costs_ok(const Data &ds, int msr, uint64_t k) {
 
 // the row tiles for k centers (built on first use); false when no stage size fits
 bool tiled_costs_ready(Data &ds, uint64_t k, hipStream_t st) {
-    const int cpl = estep_cpl(k);
+    const EShape sh = estep_shape(k);
     Tiles &T = ds.rtiles;
-    if (T.cpl != cpl)
+    if (T.cpl != sh.cpl || T.twr != sh.twr || T.nw != sh.nw)
         build_tiles(T, ds.ptr.as<const uint64_t>(), ds.idx.as<const uint32_t>(), ds.val.as<const float>(), ds.n, ds.d,
-                    ds.nnz, cpl, 64 / cpl, 64 * cpl * 4, st);
+                    ds.nnz, sh.cpl, sh.twr, sh.nw, 64 * sh.cpl * 4, st);
     return T.ready;
 }
 
@@ -504,7 +511,8 @@ void tiled_costs(Data &ds, int msr, const float *CT, uint32_t kpad, uint64_t k, 
                  const float4 *rt, const double *cs, float *out_nk, uint32_t *asn, float *best, hipStream_t st) {
     Ctx &c = *ds.ctx;
     const Tiles &T = ds.rtiles;
-    REQUIRE(T.ready && T.cpl == estep_cpl(k), "internal: row tiles not ready");
+    const EShape sh = estep_shape(k);
+    REQUIRE(T.ready && T.cpl == sh.cpl && T.twr == sh.twr && T.nw == sh.nw, "internal: row tiles not ready");
     const uint32_t nslices = (uint32_t)cdiv(k, 64 * T.cpl);
     unsigned long long *keys = nullptr;
     uint8_t *nanflag = nullptr;
@@ -516,7 +524,7 @@ void tiled_costs(Data &ds, int msr, const float *CT, uint32_t kpad, uint64_t k, 
                            ~0ull, ds.n);
         fill_u32((uint32_t *)nanflag, 0u, cdiv(ds.n, 4), st);
     }
-    const dim3 grid(T.ngroups * nslices), blk(TNW * 64);
+    const dim3 grid(T.ngroups * nslices), blk(T.nw * 64);
     const auto *cn = T.cnt.as<const uint8_t>();
     const auto *w = T.wb.as<const uint16_t>();
     const auto *o = T.off.as<const uint32_t>();
@@ -527,25 +535,27 @@ void tiled_costs(Data &ds, int msr, const float *CT, uint32_t kpad, uint64_t k, 
     // one buffer resource over the whole table when it (and the last stage's rows past
     // its end) spans under 4 GiB
     const bool whole = ((uint64_t)d + T.ts) * kpad * 4 < (1ull << 32);
-#define TILE_E1(M, P, W)                                                                                          \
-    hipLaunchKernelGGL((k_tile_estep<M, P, W>), grid, blk, 0, st, cn, w, o, e, ds.n, T.nstages, T.ts, d, CT, kpad,   \
+#define TILE_E2(M, P, W, R, N)                                                                                    \
+    hipLaunchKernelGGL((k_tile_estep<M, P, W, R, N>), grid, blk, 0, st, cn, w, o, e, ds.n, T.nstages, T.ts, d, CT, kpad,   \
                        (uint32_t)k, nslices, pv, logpv, rt, cs, out_nk, keys, nanflag)
-#define TILE_E(M, P) do { if (whole) TILE_E1(M, P, 2); else TILE_E1(M, P, 1); } while (0)
+#define TILE_E1(M, P, R, N) do { if (whole) TILE_E2(M, P, 2, R, N); else TILE_E2(M, P, 1, R, N); } while (0)
+#define TILE_E4(M) TILE_E1(M, 4, 16, TNW)
     if (T.cpl == 4) {
         switch (msr) {
-        case MCG_SQRL2: TILE_E(MCG_SQRL2, 4); break;
-        case MCG_MKL: TILE_E(MCG_MKL, 4); break;
-        default: TILE_E(MCG_REVERSE_MKL, 4); break;
+        case MCG_SQRL2: TILE_E4(MCG_SQRL2); break;
+        case MCG_MKL: TILE_E4(MCG_MKL); break;
+        default: TILE_E4(MCG_REVERSE_MKL); break;
         }
     } else {
         switch (msr) {
-        case MCG_SQRL2: TILE_E(MCG_SQRL2, 2); break;
-        case MCG_MKL: TILE_E(MCG_MKL, 2); break;
-        default: TILE_E(MCG_REVERSE_MKL, 2); break;
+        case MCG_SQRL2: TILE_E1(MCG_SQRL2, 2, 32, TNW); break;
+        case MCG_MKL: TILE_E1(MCG_MKL, 2, 32, TNW); break;
+        default: TILE_E1(MCG_REVERSE_MKL, 2, 32, TNW); break;
         }
     }
-#undef TILE_E
+#undef TILE_E4
 #undef TILE_E1
+#undef TILE_E2
     c.toc(0, ev);
     if (keys)
         hipLaunchKernelGGL(k_keys_out, dim3((uint32_t)std::min<uint64_t>(cdiv(ds.n, 256), 4096)), dim3(256), 0, st,
@@ -562,6 +572,7 @@ __global__ __launch_bounds__(1024) void k_tile_mstep(const uint8_t *__restrict__
                                                      const float *__restrict__ R, uint32_t k, uint32_t nslices,
                                                      uint32_t kpad, double *__restrict__ num) {
     constexpr int TWR = 32 / CPL, TG = TWR * TNW, CW = 64 * CPL;
+    constexpr int NW = TNW;
     TILE_LDS
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -572,7 +583,7 @@ __global__ __launch_bounds__(1024) void k_tile_mstep(const uint8_t *__restrict__
     for (int a = 0; a < TWR; ++a)
 #pragma unroll
         for (int u = 0; u < CPL; ++u) acc[a][u] = 0.;
-    const Stage<CPL, MODE> S(R, k, n, c0, ts, (uint32_t)TG, ent, cnt, wb, lane);
+    const Stage<CPL, MODE, NW> S(R, k, n, c0, ts, (uint32_t)TG, ent, cnt, wb, lane);
     auto walk = [&](const float *Tw, const uint2 *Es, const uint32_t *Cs, const uint32_t *Ws) {
         stage_walk<TWR>(Tw + lane * CPL, Es, Cs, Ws, wave, lane, 0.f, false, [&](int a, float x, const float *tp) {
             const double xd = (double)x;
@@ -627,7 +638,7 @@ bool tiled_mstep_ready(Data &ds, uint64_t k, const uint64_t *cptr, const uint32_
                        hipStream_t st) {
     const int cpl = mstep_cpl(k);
     Tiles &T = ds.ctiles;
-    if (T.cpl != cpl) build_tiles(T, cptr, crow, cval, ds.d, ds.n, ds.nnz, cpl, 32 / cpl, 64 * cpl * 4, st);
+    if (T.cpl != cpl) build_tiles(T, cptr, crow, cval, ds.d, ds.n, ds.nnz, cpl, 32 / cpl, TNW, 64 * cpl * 4, st);
     return T.ready;
 }
 

@@ -1,8 +1,10 @@
 // Internal declarations shared by the libmcg translation units.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <exception>
 #include <new>
 #include <string>
@@ -95,6 +97,15 @@ struct Tiles {
     uint32_t maxseg = 0, maxwave = 0;   // largest stage segment of a block / of a wave
     DBuf cnt, wb, off, ent;
 };
+
+// Chunks per cluster / column of the CSR M-steps: up to 8, while the Z x k x d FP64
+// partial sums stay under about 2 GB (MCG_CSR_CHUNKS = 1..8 overrides, read per call).
+inline uint32_t csr_chunks(uint64_t kd) {
+    const char *e = getenv("MCG_CSR_CHUNKS");
+    const int f = e ? atoi(e) : 0;
+    if (f >= 1 && f <= 8) return (uint32_t)f;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(8, (uint64_t)(2e9 / 8.) / kd));
+}
 
 // ---- dataset ----------------------------------------------------------------
 struct Data {

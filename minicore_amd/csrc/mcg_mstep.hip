@@ -434,7 +434,8 @@ __global__ __launch_bounds__(256) void k_csr_mean(const float *__restrict__ val,
                                                   const uint32_t *__restrict__ perm,
                                                   const uint32_t *__restrict__ counts,
                                                   const uint32_t *__restrict__ off, const double *__restrict__ w,
-                                                  uint32_t k, double *__restrict__ part) {
+                                                  uint32_t k, double *__restrict__ part,
+                                                  const double *__restrict__ cw, float *__restrict__ cnew) {
     __shared__ double acc[4][CSR_CW];
     const uint32_t c = blockIdx.y, z = blockIdx.z, Z = gridDim.z;
     const uint32_t col0 = blockIdx.x * CSR_CW;
@@ -498,6 +499,18 @@ __global__ __launch_bounds__(256) void k_csr_mean(const float *__restrict__ val,
         }
     }
     __syncthreads();
+    // one chunk (cnew given): the sum is the center's numerator, divided here as the
+    // final pass would (empty clusters are left as they are)
+    if (cnew) {
+        if (cm == 0) return;
+        const double wc = cw[c];
+        for (uint32_t j = col0 + tid; j < col1; j += 256) {
+            const uint32_t jj = j - col0;
+            const double s = __dadd_rn(__dadd_rn(acc[0][jj], acc[1][jj]), __dadd_rn(acc[2][jj], acc[3][jj]));
+            cnew[(uint64_t)c * d + j] = (float)__ddiv_rn(s, wc);
+        }
+        return;
+    }
     double *pz = part + ((uint64_t)z * k + c) * d;
     for (uint32_t j = col0 + tid; j < col1; j += 256) {
         const uint32_t jj = j - col0;
@@ -548,15 +561,18 @@ MStepOut csr_mstep(Data &ds, const uint32_t *asn, const double *w, uint64_t k, f
     hipLaunchKernelGGL(k_cluster_weight, dim3((uint32_t)k), dim3(256), 0, st, so.perm, so.counts, so.off, w, k,
                        ccost + k);
     // up to 8 member chunks per cluster, so a large cluster is spread over blocks
-    // (partial sums: Z x k x d FP64)
+    // (partial sums: Z x k x d FP64, at most about 2 GB); one chunk writes the centers
+    // directly
     const uint64_t kd = k * ds.d;
-    const uint32_t Z = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(8, (uint64_t)(2e9 / 8.) / kd));
-    double *part = (double *)c.s[S_PART].reserve((uint64_t)Z * kd * sizeof(double));
+    const uint32_t Z = csr_chunks(kd);
+    double *part = Z > 1 ? (double *)c.s[S_PART].reserve((uint64_t)Z * kd * sizeof(double)) : nullptr;
     dim3 g((uint32_t)cdiv(ds.d, CSR_CW), (uint32_t)k, Z);
     hipLaunchKernelGGL(k_csr_mean, g, dim3(256), 0, st, ds.val.as<const float>(), ds.idx.as<const uint32_t>(),
-                       ds.ptr.as<const uint64_t>(), (uint32_t)ds.d, so.perm, so.counts, so.off, w, (uint32_t)k, part);
-    hipLaunchKernelGGL(k_csr_mean_final, dim3((uint32_t)cdiv(kd, 256)), dim3(256), 0, st, (const double *)part, Z,
-                       (uint32_t)k, (uint32_t)ds.d, so.counts, (const double *)(ccost + k), Cnew);
+                       ds.ptr.as<const uint64_t>(), (uint32_t)ds.d, so.perm, so.counts, so.off, w, (uint32_t)k, part,
+                       (const double *)(ccost + k), Z > 1 ? nullptr : Cnew);
+    if (Z > 1)
+        hipLaunchKernelGGL(k_csr_mean_final, dim3((uint32_t)cdiv(kd, 256)), dim3(256), 0, st, (const double *)part, Z,
+                           (uint32_t)k, (uint32_t)ds.d, so.counts, (const double *)(ccost + k), Cnew);
     return MStepOut{so.counts, total};
 }
 

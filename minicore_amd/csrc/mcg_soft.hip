@@ -134,7 +134,8 @@ template <int Q>
 __global__ __launch_bounds__(256) void k_soft_csc(const uint64_t *__restrict__ cptr, const uint32_t *__restrict__ crow,
                                                   const float *__restrict__ cval, uint32_t d,
                                                   const float *__restrict__ R, uint32_t k, uint32_t c0,
-                                                  double *__restrict__ part) {
+                                                  double *__restrict__ part, const double *__restrict__ den,
+                                                  float *__restrict__ C) {
     const uint64_t j = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (j >= d) return;
@@ -170,10 +171,13 @@ __global__ __launch_bounds__(256) void k_soft_csc(const uint64_t *__restrict__ c
         for (int q = 0; q < Q; ++q)
             if (ok[q]) acc[q] += (double)R[(uint64_t)i0 * k + c0 + lane + 64u * q] * x0;
     }
+    // one chunk (C given): the sum is the numerator, divided here as the final pass would
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const uint64_t c = c0 + lane + 64u * q;
-        if (ok[q]) part[((uint64_t)blockIdx.y * k + c) * d + j] = acc[q];
+        if (!ok[q]) continue;
+        if (C) C[c * d + j] = (float)(acc[q] / den[c]);
+        else part[((uint64_t)blockIdx.y * k + c) * d + j] = acc[q];
     }
 }
 
@@ -251,10 +255,12 @@ void soft_mstep(Data &ds, const float *R, const double *w, uint64_t k, float *Cn
         return;
     }
     // up to 8 entry chunks per column, so the columns with many entries are spread
-    // over waves (partial sums: Z x k x d FP64)
+    // over waves (partial sums: Z x k x d FP64, at most about 2 GB); one chunk writes
+    // the centers directly
     const uint64_t kd = k * (uint64_t)d;
-    const uint32_t Z = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(8, (uint64_t)(2e9 / 8.) / kd));
-    double *part = (double *)c.s[S_PART].reserve((uint64_t)Z * kd * sizeof(double));
+    const uint32_t Z = csr_chunks(kd);
+    double *part = Z > 1 ? (double *)c.s[S_PART].reserve((uint64_t)Z * kd * sizeof(double)) : nullptr;
+    float *Cd = Z > 1 ? nullptr : Cnew;
     for (uint32_t c0 = 0; c0 < k; c0 += 256) {
         const uint32_t q = (uint32_t)std::min<uint64_t>(4, cdiv(k - c0, 64));
         const dim3 g((uint32_t)cdiv(d, 4), Z), blk(256);
@@ -262,14 +268,15 @@ void soft_mstep(Data &ds, const float *R, const double *w, uint64_t k, float *Cn
         const uint32_t *cr = ds.csc_row.as<const uint32_t>();
         const float *cv = ds.csc_val.as<const float>();
         switch (q) {
-        case 1: hipLaunchKernelGGL(k_soft_csc<1>, g, blk, 0, st, cp, cr, cv, d, R, (uint32_t)k, c0, part); break;
-        case 2: hipLaunchKernelGGL(k_soft_csc<2>, g, blk, 0, st, cp, cr, cv, d, R, (uint32_t)k, c0, part); break;
-        case 3: hipLaunchKernelGGL(k_soft_csc<3>, g, blk, 0, st, cp, cr, cv, d, R, (uint32_t)k, c0, part); break;
-        default: hipLaunchKernelGGL(k_soft_csc<4>, g, blk, 0, st, cp, cr, cv, d, R, (uint32_t)k, c0, part); break;
+        case 1: hipLaunchKernelGGL(k_soft_csc<1>, g, blk, 0, st, cp, cr, cv, d, R, (uint32_t)k, c0, part, den, Cd); break;
+        case 2: hipLaunchKernelGGL(k_soft_csc<2>, g, blk, 0, st, cp, cr, cv, d, R, (uint32_t)k, c0, part, den, Cd); break;
+        case 3: hipLaunchKernelGGL(k_soft_csc<3>, g, blk, 0, st, cp, cr, cv, d, R, (uint32_t)k, c0, part, den, Cd); break;
+        default: hipLaunchKernelGGL(k_soft_csc<4>, g, blk, 0, st, cp, cr, cv, d, R, (uint32_t)k, c0, part, den, Cd); break;
         }
     }
-    hipLaunchKernelGGL(k_soft_csc_final, dim3((uint32_t)cdiv(kd, 256)), dim3(256), 0, st, (const double *)part, Z,
-                       (uint32_t)k, d, (const double *)den, Cnew);
+    if (Z > 1)
+        hipLaunchKernelGGL(k_soft_csc_final, dim3((uint32_t)cdiv(kd, 256)), dim3(256), 0, st, (const double *)part, Z,
+                           (uint32_t)k, d, (const double *)den, Cnew);
 }
 
 // ---- soft L1 / L2 centers (set_centroids_full_mean, centroid.h:815-838) ----------

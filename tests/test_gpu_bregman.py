@@ -173,6 +173,35 @@ def test_csr_centroids_skewed(mcg, ctx, mco, weighted):
     np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-9)
 
 
+@pytest.mark.parametrize("chunks", ["1", "3"])
+def test_csr_mstep_chunk_counts(mcg, ctx, mco, monkeypatch, chunks):
+    """The CSR M-steps with one member / entry chunk (the centers written by the
+    chunk kernel itself, no partial sums) and with three: hard centers within 1e-6 of
+    the oracle's FP64 means (weighted and not), soft EM centers within the soft tests'
+    bound."""
+    say(f"csr m-step chunks={chunks}")
+    monkeypatch.setenv("MCG_CSR_CHUNKS", chunks)
+    n, d, k = 3000, 2000, 9
+    data, idx, ptr = csr_data(n, d, 0.02, 31, False)
+    X = mco.CSR(data, idx, ptr, (n, d))
+    asn = np.random.default_rng(4).integers(0, k, n).astype(np.uint32)
+    asn[asn == 3] = 4                                       # cluster 3 empty: left as it was
+    w = np.random.default_rng(5).uniform(0.5, 2, n)
+    prev = np.full((k, d), 0.5, np.float32)
+    ds = mcg.Dataset.csr(data, idx, ptr, (n, d), ctx)
+    for ww in (None, w):
+        ref, _ = mco.centroids(X, asn, k, prev, ww)
+        got, _ = ds.centroids(asn, k, prev, ww)
+        np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-9)
+        assert np.array_equal(got[3], prev[3])
+    C0 = X.todense()[np.random.default_rng(6).choice(n, k, replace=False)] + 0.25
+    ref = mco.soft(11, X, C0, prior=1e-3, temp=1., eps=-1., maxiter=2)
+    got = ds.soft(11, C0, prior=1e-3, temp=1., eps=-1., maxiter=2)
+    assert got["finalcost"] == pytest.approx(ref["finalcost"], rel=2e-4)
+    np.testing.assert_allclose(got["centers"], ref["centers"], rtol=2e-4, atol=1e-5)
+    ds.close()
+
+
 @pytest.mark.parametrize("k", [100, 130, 300])
 def test_soft_em_csr_many_centers(mcg, ctx, mco, k):
     """Soft CSR EM with k = 100 (two centers per lane in the tiled E-step), k = 130

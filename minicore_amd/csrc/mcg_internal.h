@@ -442,7 +442,12 @@ __device__ __forceinline__ void block_max_into(double a, unsigned long long *mx)
         if ((int)threadIdx.x < o) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicMax(mx, (unsigned long long)__double_as_longlong(sm[0]));
+    // (the maximum only grows within a round: a block whose maximum is not above the
+    // current one skips the atomic, so the blocks' same-address atomics do not queue)
+    if (threadIdx.x == 0) {
+        const unsigned long long v = (unsigned long long)__double_as_longlong(sm[0]);
+        if (v > __hip_atomic_load(mx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(mx, v);
+    }
 }
 
 // new distance of point p given its cost s to the new center (kmeans.h:189-204)

@@ -444,9 +444,11 @@ struct PickArgs {
 };
 
 // One seeding round's choice (kmeans.h:115-186) fused with the block sums it needs.
-// Each block of 1024 threads adds its FX_BLK fixed-point weights w * dist into
-// bsum[b]; the last block to finish (a device-scope counter) reads the block sums
-// back through the device-coherent path and draws: the fixed-point draw r over w *
+// The blocks of 1024 threads add the fixed-point weights w * dist of chunks of FX_BLK
+// rows, chunk b into bsum[b] (a block takes chunks blockIdx.x, + gridDim.x, ...: at
+// most a few hundred blocks, since every block's device-scope release and counter
+// update are serialised across the GPU); the last block to finish (a device-scope
+// counter) reads the chunk sums back through the device-coherent path and draws: the fixed-point draw r over w *
 // dist, the first index of the maximum distance instead if the drawn point has
 // distance 0.  It writes *newc and centers[ci], clears *mx and the counter, and
 // prepares the round's update: MODE 0 (dense SQRL2 rows) the FP64 squared distances
@@ -454,6 +456,8 @@ struct PickArgs {
 // the new center as a dense row and, with tables, its single-center table and JSD's
 // G for the first gU distinct li values.  The draw is that of the block sums and
 // select of separate kernels (integer sums: the same index).
+constexpr uint32_t PICK_GRID = 128;   // blocks of the choice kernel (at most)
+
 template <int MODE>
 __global__ __launch_bounds__(1024) void k_pp_pick(PickArgs a, uint64_t r, uint32_t ci) {
     __shared__ unsigned long long sm[1024];
@@ -467,22 +471,27 @@ __global__ __launch_bounds__(1024) void k_pp_pick(PickArgs a, uint64_t r, uint32
     const uint32_t t = threadIdx.x;
     const int s = fixed_shift(__longlong_as_double((long long)*a.mx), a.n);
     {
-        const uint64_t i = (uint64_t)blockIdx.x * FX_BLK + t;
-        unsigned long long f = 0;
-        if (i < a.n) {
-            double v = (double)a.dist[i];
-            if (a.w) v = a.w[i] * v;
-            f = to_fixed(v, s);
+        for (uint32_t cb = blockIdx.x; cb < a.nblk; cb += gridDim.x) {
+            const uint64_t i = (uint64_t)cb * FX_BLK + t;
+            unsigned long long f = 0;
+            if (i < a.n) {
+                double v = (double)a.dist[i];
+                if (a.w) v = a.w[i] * v;
+                f = to_fixed(v, s);
+            }
+            for (int o = 32; o > 0; o >>= 1) f += __shfl_xor(f, o);
+            if ((t & 63) == 0) sm[t >> 6] = f;
+            __syncthreads();
+            if (t == 0) {
+                unsigned long long tot = 0;
+                for (int q = 0; q < 16; ++q) tot += sm[q];
+                a.bsum[cb] = tot;
+            }
+            __syncthreads();
         }
-        for (int o = 32; o > 0; o >>= 1) f += __shfl_xor(f, o);
-        if ((t & 63) == 0) sm[t >> 6] = f;
-        __syncthreads();
         if (t == 0) {
-            unsigned long long tot = 0;
-            for (int q = 0; q < 16; ++q) tot += sm[q];
-            a.bsum[blockIdx.x] = tot;
             __threadfence();
-            last = atomicAdd(a.ctr, 1u) == a.nblk - 1 ? 1u : 0u;
+            last = atomicAdd(a.ctr, 1u) == gridDim.x - 1 ? 1u : 0u;
         }
         __syncthreads();
         if (!last) return;
@@ -723,17 +732,18 @@ void kmeanspp_run(Data &ds, int msr, double prior, uint64_t k, uint64_t seed, co
         pa.gU = tb.U <= PP_GMAX ? tb.U : 0u;
         pa.G = tb.G;
     }
+    const uint32_t pgrid = std::min<uint32_t>(nblk, PICK_GRID);
     for (uint64_t ci = 1; ci < k; ++ci) {
         const uint64_t r = wyrand_at(seed, j++);
         if (l2) {
-            hipLaunchKernelGGL(k_pp_pick<0>, dim3(nblk), dim3(1024), 0, st, pa, r, (uint32_t)ci);
+            hipLaunchKernelGGL(k_pp_pick<0>, dim3(pgrid), dim3(1024), 0, st, pa, r, (uint32_t)ci);
             if (ci > CC_IN_PICK)
                 hipLaunchKernelGGL(k_pp_cc, dim3((uint32_t)cdiv(ci, 16)), dim3(256), 0, st, ds.X, d, vec,
                                    (const uint32_t *)cent, (uint32_t)ci, (const uint64_t *)newc, (double *)ccbuf);
             hipLaunchKernelGGL(k_pp_update_l2_pruned, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, ds.X, n, d,
                                vec, newc, (uint32_t)ci, (const double *)ccbuf, dist_d, asn_d, w, mx);
         } else {
-            hipLaunchKernelGGL(k_pp_pick<1>, dim3(nblk), dim3(1024), 0, st, pa, r, (uint32_t)ci);
+            hipLaunchKernelGGL(k_pp_pick<1>, dim3(pgrid), dim3(1024), 0, st, pa, r, (uint32_t)ci);
             if (tables) {
                 generic_pp_next(ds, msr, tb, cent, (uint32_t)ci, newc, (float *)ccbuf, dist_d, asn_d, w, mx, st);
             } else {
@@ -1118,6 +1128,52 @@ __global__ void k_cs_sums(const double *__restrict__ cost, const uint32_t *__res
     }
 }
 
+// The same sums for up to CS_LDS clusters: each block adds its rows into LDS copies
+// of the per-cluster sums and its totals by a block reduction, then adds them to the
+// global sums (one atomic per block and non-empty cluster instead of five per row,
+// two of them on a single address).  Integer sums: the same totals in any order.
+constexpr uint32_t CS_LDS = 2048;
+__global__ __launch_bounds__(256) void k_cs_sums_lds(const double *__restrict__ cost, const uint32_t *__restrict__ asn,
+                                                     const double *__restrict__ w, uint64_t n, uint32_t ncenters,
+                                                     CsSums *__restrict__ S, unsigned long long *__restrict__ ws,
+                                                     unsigned long long *__restrict__ cs,
+                                                     unsigned long long *__restrict__ cnt) {
+    __shared__ unsigned long long lw[CS_LDS], lc[CS_LDS], ln[CS_LDS];
+    __shared__ unsigned long long rw[256], rc[256];
+    for (uint32_t c = threadIdx.x; c < ncenters; c += 256) { lw[c] = 0; lc[c] = 0; ln[c] = 0; }
+    __syncthreads();
+    const int sw = fixed_shift(__longlong_as_double((long long)S->maxw), n);
+    const int sc = fixed_shift(__longlong_as_double((long long)S->maxc), n);
+    unsigned long long tw = 0, tc = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const double wf = wf_of(w, i);
+        const int64_t a = to_fixed_signed(wf, sw), b = to_fixed_signed(wf * cost[i], sc);
+        const uint32_t c = asn[i];
+        atomicAdd(&lw[c], (unsigned long long)a);
+        atomicAdd(&lc[c], (unsigned long long)b);
+        atomicAdd(&ln[c], 1ull);
+        tw += (unsigned long long)a;
+        tc += (unsigned long long)b;
+    }
+    rw[threadIdx.x] = tw;
+    rc[threadIdx.x] = tc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) { rw[threadIdx.x] += rw[threadIdx.x + o]; rc[threadIdx.x] += rc[threadIdx.x + o]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(&S->wtot, rw[0]);
+        atomicAdd(&S->ctot, rc[0]);
+    }
+    for (uint32_t c = threadIdx.x; c < ncenters; c += 256)
+        if (ln[c]) {
+            atomicAdd(&ws[c], lw[c]);
+            atomicAdd(&cs[c], lc[c]);
+            atomicAdd(&cnt[c], ln[c]);
+        }
+}
+
 // unnormalised sensitivities p_i (double), plus max |p|
 __global__ void k_cs_sens(int sens, const double *__restrict__ cost, const uint32_t *__restrict__ asn,
                           const double *__restrict__ w, uint64_t n, double alpha, CsSums *__restrict__ S,
@@ -1213,7 +1269,11 @@ void coreset_probs_core(int sens, uint64_t n, uint64_t ncenters, const double *c
     double *p = (double *)(buf + head);
     const uint32_t g = (uint32_t)std::min<uint64_t>(cdiv(n, 256), 2048);
     hipLaunchKernelGGL(k_cs_max, dim3(g), dim3(256), 0, st, cost, w, n, S);
-    hipLaunchKernelGGL(k_cs_sums, dim3(g), dim3(256), 0, st, cost, asn, w, n, S, ws, cs, cnt);
+    if (ncenters <= CS_LDS)
+        hipLaunchKernelGGL(k_cs_sums_lds, dim3((uint32_t)std::min<uint64_t>(cdiv(n, 2048), 128)), dim3(256), 0, st,
+                           cost, asn, w, n, (uint32_t)ncenters, S, ws, cs, cnt);
+    else
+        hipLaunchKernelGGL(k_cs_sums, dim3(g), dim3(256), 0, st, cost, asn, w, n, S, ws, cs, cnt);
     const double alpha = 16. * std::log((double)ncenters) + 32.;
     hipLaunchKernelGGL(k_cs_sens, dim3(g), dim3(256), 0, st, sens, cost, asn, w, n, alpha, S, ws, cs, cnt, p, n,
                        probs);
@@ -1224,12 +1284,18 @@ void coreset_probs_core(int sens, uint64_t n, uint64_t ncenters, const double *c
 
 // ---- coreset draws ----------------------------------------------------------------
 // inclusive prefix of fixed weights: per-block sums -> host scan (nblk small) -> per-block
-// prefixes; each draw binary-searches blocks then walks its block.
-__global__ void k_draws(const float *__restrict__ probs, const double *__restrict__ w, uint64_t n,
-                        const unsigned long long *__restrict__ maxbits, const uint64_t *__restrict__ bpre,
-                        uint32_t nblk, uint64_t T, uint64_t m, uint64_t seed, uint64_t first,
-                        uint64_t *__restrict__ idx, float *__restrict__ wout) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// prefixes; each draw binary-searches blocks, then one wave finds the first index of
+// its block whose inclusive prefix exceeds the draw: every lane adds 16 consecutive
+// weights, a wave scan places the lanes' runs, and the first lane whose run passes
+// the draw walks its 16.  (Integer sums: the index of a sequential walk.)
+__global__ __launch_bounds__(256) void k_draws(const float *__restrict__ probs, const double *__restrict__ w,
+                                               uint64_t n, const unsigned long long *__restrict__ maxbits,
+                                               const uint64_t *__restrict__ bpre, uint32_t nblk, uint64_t T,
+                                               uint64_t m, uint64_t seed, uint64_t first, uint64_t *__restrict__ idx,
+                                               float *__restrict__ wout) {
+    static_assert(FX_BLK == 64 * 16, "16 weights per lane");
+    const uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
     if (t >= m) return;
     const int s = fixed_shift(__longlong_as_double((long long)*maxbits), n);
     const uint64_t u = mulhi64(wyrand_at(seed, first + t), T);
@@ -1237,15 +1303,41 @@ __global__ void k_draws(const float *__restrict__ probs, const double *__restric
     uint32_t lo = 0, hi = nblk;
     while (lo < hi) { const uint32_t mid = (lo + hi) >> 1; if (bpre[mid] > u) hi = mid; else lo = mid + 1; }
     const uint32_t b = lo < nblk ? lo : nblk - 1;
-    uint64_t run = b ? bpre[b - 1] : 0;
-    uint64_t ans = min<uint64_t>(n - 1, (uint64_t)(b + 1) * FX_BLK - 1);
-    for (uint64_t i = (uint64_t)b * FX_BLK; i < min<uint64_t>(n, (uint64_t)(b + 1) * FX_BLK); ++i) {
-        run += to_fixed((double)probs[i], s);
-        if (run > u) { ans = i; break; }
+    const uint64_t run0 = b ? bpre[b - 1] : 0;
+    const uint64_t i0 = (uint64_t)b * FX_BLK + (uint64_t)lane * 16;
+    unsigned long long f[16], sum = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        f[j] = i0 + j < n ? to_fixed((double)probs[i0 + j], s) : 0ull;
+        sum += f[j];
     }
-    idx[t] = ans;
-    const double wi = w ? (double)(float)w[ans] : 1.;
-    wout[t] = (float)(wi / ((double)m * (double)probs[ans]));
+    unsigned long long incl = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned long long y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    const uint64_t hit = __builtin_amdgcn_ballot_w64(run0 + incl > u);
+    uint64_t ans = min<uint64_t>(n - 1, (uint64_t)(b + 1) * FX_BLK - 1);
+    if (hit) {
+        const int L = __builtin_ctzll(hit);
+        uint64_t mine = 0;
+        if (lane == L) {
+            uint64_t run = run0 + incl - sum;
+            mine = i0 + 15;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                run += f[j];
+                if (run > u) { mine = i0 + j; break; }
+            }
+        }
+        ans = __shfl(mine, L);
+    }
+    if (lane == 0) {
+        idx[t] = ans;
+        const double wi = w ? (double)(float)w[ans] : 1.;
+        wout[t] = (float)(wi / ((double)m * (double)probs[ans]));
+    }
 }
 
 void coreset_sample_run(Ctx &c, uint64_t n, const float *probs, const double *w, uint64_t m, uint64_t seed,
@@ -1271,7 +1363,7 @@ void coreset_sample_from(Ctx &c, uint64_t n, const float *probs, const double *w
     for (uint32_t b = 0; b < nblk; ++b) { T += h[b]; h[b] = T; }
     REQUIRE(T > 0, "all sampling probabilities are zero");
     HIP_CHECK(hipMemcpyAsync(bsum, h.data(), nblk * 8, hipMemcpyHostToDevice, st));
-    if (m) hipLaunchKernelGGL(k_draws, dim3((uint32_t)cdiv(m, 256)), dim3(256), 0, st, probs, w, n,
+    if (m) hipLaunchKernelGGL(k_draws, dim3((uint32_t)cdiv(m, 4)), dim3(256), 0, st, probs, w, n,
                               (const unsigned long long *)mx, (const uint64_t *)bsum, nblk, T, m, seed, first, idx, wout);
     HIP_CHECK(hipStreamSynchronize(st));
 }

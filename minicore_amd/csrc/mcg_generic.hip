@@ -85,6 +85,22 @@ __global__ __launch_bounds__(256) void k_jsd_g(const float *__restrict__ CT, uin
     }
 }
 
+// JSD with few distinct li (rows of nearly equal sums, e.g. normalised rows): the
+// zero-baseline term of each (li, column, center), m0 log(m0 / 2) with m0 = li + c~_j,
+// is the same for every row of that li, so it is tabulated beside c~_j, one table per
+// li (one logarithm per entry and center instead of two).
+constexpr uint32_t JSD_TAB_U = 4;
+__global__ void k_jsd_pair_table(const float *__restrict__ CT, uint64_t nel, const float *__restrict__ lis,
+                                 uint32_t U, float2 *__restrict__ CT2) {
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nel * U; e += (uint64_t)gridDim.x * blockDim.x) {
+        const float li = lis[e / nel];
+        const float ct = CT[e % nel];
+        const float m0 = li + ct;
+        const float l0 = __logf(0.5f * m0);
+        CT2[e] = make_float2(ct, m0 * l0);
+    }
+}
+
 __device__ __forceinline__ float post_kl(float r) {
     r = r < 0.f ? 0.f : r;
     if (r == INFINITY) r = FLT_MAX;
@@ -100,8 +116,8 @@ __global__ __launch_bounds__(256) void k_row_costs(const float *__restrict__ X, 
                                                    const float4 *__restrict__ rt, const float *__restrict__ CT,
                                                    uint32_t kpad, uint32_t k, const double *__restrict__ cs,
                                                    const double *__restrict__ G, const uint32_t *__restrict__ slot,
-                                                   float *__restrict__ out, uint32_t *__restrict__ asn,
-                                                   float *__restrict__ best_out) {
+                                                   const float2 *__restrict__ CT2, float *__restrict__ out,
+                                                   uint32_t *__restrict__ asn, float *__restrict__ best_out) {
     const uint64_t row = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (row >= n) return;
@@ -136,6 +152,35 @@ __global__ __launch_bounds__(256) void k_row_costs(const float *__restrict__ X, 
             // then accumulated in entry order (the same sums as one at a time)
             constexpr int BT = MSR == MCG_JSD ? 8 : 16;
             int t0 = 0;
+            if (MSR == MCG_JSD && CT2) {
+                // the tabulated zero-baseline term of this row's li: one logarithm per
+                // entry and center
+                const float2 *CTr = CT2 + (uint64_t)slot[row] * ((uint64_t)d * kpad);
+                for (; t0 + BT <= cnt; t0 += BT) {
+                    float xv[BT], xl[BT];
+                    float2 ct[BT];
+#pragma unroll
+                    for (int u = 0; u < BT; ++u) {
+                        const uint32_t j = (uint32_t)__shfl((int)jq, t0 + u);
+                        xv[u] = __shfl(tq, t0 + u);
+                        xl[u] = __shfl(tq2, t0 + u);
+                        ct[u] = CTr[(uint64_t)j * kpad + cc];
+                    }
+#pragma unroll
+                    for (int u = 0; u < BT; ++u) {
+                        const float m1 = xv[u] + ct[u].x;
+                        acc += 0.5f * ((xl[u] - hli) - m1 * __logf(0.5f * m1) + ct[u].y);
+                    }
+                }
+                for (int t = t0; t < cnt; ++t) {
+                    const uint32_t j = (uint32_t)__shfl((int)jq, t);
+                    const float xv = __shfl(tq, t), xl = __shfl(tq2, t);
+                    const float2 ct = CTr[(uint64_t)j * kpad + cc];
+                    const float m1 = xv + ct.x;
+                    acc += 0.5f * ((xl - hli) - m1 * __logf(0.5f * m1) + ct.y);
+                }
+                continue;
+            }
             for (; t0 + BT <= cnt; t0 += BT) {
                 float xv[BT], ct[BT], xl[BT];
 #pragma unroll
@@ -1191,6 +1236,15 @@ void generic_costs(Data &ds, int msr, double prior, const float *C, uint64_t k, 
     const float4 *rt = T.rt;
     const uint32_t *slot = T.slot;
     const dim3 grid((uint32_t)cdiv(n, 4)), blk(256);
+    // JSD rows that all share one li: the zero-baseline terms tabulated beside the table
+    const float2 *CT2 = nullptr;
+    if (msr == MCG_JSD && !single && !ds.li_unique.empty() && ds.li_unique.size() <= JSD_TAB_U) {
+        const uint64_t nel = (uint64_t)d * kpad, U = ds.li_unique.size();
+        float2 *t2 = (float2 *)c.s[S_JSDT].reserve(U * nel * sizeof(float2));
+        hipLaunchKernelGGL(k_jsd_pair_table, dim3((uint32_t)std::min<uint64_t>(cdiv(U * nel, 256), 4096)), dim3(256), 0,
+                           st, CT, nel, ds.li_dev.as<const float>(), (uint32_t)U, t2);
+        CT2 = t2;
+    }
     const float fpv = (float)pv, lpv = (float)log(pv > 0 ? pv : 1.);
     if (!single && !ds.csr && (msr == MCG_MKL || msr == MCG_REVERSE_MKL) && d <= 256 && dense_breg_mfma_on()) {
         const uint32_t dpad = d <= 64 ? 64u : d <= 128 ? 128u : 256u;
@@ -1244,19 +1298,19 @@ void generic_costs(Data &ds, int msr, double prior, const float *C, uint64_t k, 
     switch (msr) {
     case MCG_SQRL2:
         hipLaunchKernelGGL(k_row_costs<MCG_SQRL2>, grid, blk, 0, st, X, val, idx, ptr, n, d, fpv, lpv, rt, CT, kpad,
-                           (uint32_t)k, cs, G, slot, out_nk, asn, best);
+                           (uint32_t)k, cs, G, slot, CT2, out_nk, asn, best);
         break;
     case MCG_MKL:
         hipLaunchKernelGGL(k_row_costs<MCG_MKL>, grid, blk, 0, st, X, val, idx, ptr, n, d, fpv, lpv, rt, CT, kpad,
-                           (uint32_t)k, cs, G, slot, out_nk, asn, best);
+                           (uint32_t)k, cs, G, slot, CT2, out_nk, asn, best);
         break;
     case MCG_REVERSE_MKL:
         hipLaunchKernelGGL(k_row_costs<MCG_REVERSE_MKL>, grid, blk, 0, st, X, val, idx, ptr, n, d, fpv, lpv, rt, CT,
-                           kpad, (uint32_t)k, cs, G, slot, out_nk, asn, best);
+                           kpad, (uint32_t)k, cs, G, slot, CT2, out_nk, asn, best);
         break;
     default:
         hipLaunchKernelGGL(k_row_costs<MCG_JSD>, grid, blk, 0, st, X, val, idx, ptr, n, d, fpv, lpv, rt, CT, kpad,
-                           (uint32_t)k, cs, G, slot, out_nk, asn, best);
+                           (uint32_t)k, cs, G, slot, CT2, out_nk, asn, best);
         break;
     }
 }

@@ -3,6 +3,7 @@
 // the other translation units.
 #include "mcg_internal.h"
 #include <algorithm>
+#include <mutex>
 #include <cfloat>
 #include <cmath>
 #include <cstdlib>
@@ -60,6 +61,8 @@ void Ctx::collect() {
 Ctx::~Ctx() {
     for (int w = 0; w < 2; ++w)
         for (auto &pr : pending[w]) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+    // the scratch buffers are freed in the stream's order: before the stream goes
+    for (DBuf &b : s) b.release();
     if (stream) (void)hipStreamDestroy(stream);
     if (host_pinned) (void)hipHostFree(host_pinned);
 }
@@ -569,6 +572,20 @@ int mcg_device_count(int *count) {
     });
 }
 
+// The device's default memory pool keeps freed memory for reuse instead of returning
+// it to the driver at synchronisations (once per device).
+static void mempool_setup(int device) {
+    static std::mutex mu;
+    static std::vector<int> done;
+    std::lock_guard<std::mutex> lk(mu);
+    if (std::find(done.begin(), done.end(), device) != done.end()) return;
+    hipMemPool_t pool;
+    HIP_CHECK(hipDeviceGetDefaultMemPool(&pool, device));
+    uint64_t thr = ~0ull;
+    HIP_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
+    done.push_back(device);
+}
+
 int mcg_ctx_create(int device, mcg_ctx **out) {
     return guard([&] {
         REQUIRE(out, "null out");
@@ -581,6 +598,10 @@ int mcg_ctx_create(int device, mcg_ctx **out) {
         std::unique_ptr<mcg_ctx> c(new mcg_ctx);
         c->c.device = device;
         HIP_CHECK(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
+        // scratch buffers come from the device's memory pool in the stream's order (their
+        // frees and regrowth wait for this stream only); the pool keeps what is freed
+        mempool_setup(device);
+        for (DBuf &b : c->c.s) b.st = c->c.stream;
         *out = c.release();
     });
 }

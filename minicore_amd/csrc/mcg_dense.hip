@@ -1364,9 +1364,7 @@ static void dense_sqrl2_assign_mfma(Data &ds, const float *C, uint64_t k, int pr
 // work n*k*d (measured: the MFMA form is 1.5x faster at 2^24 and 4x at 2^28.5); in
 // between, both are timed once per device and (form, padded d, padded k, log2 n), min
 // of two runs each, and the choice is kept for every context of the process; server
-// workers take the crossover at 2^23 instead of timing inside a request, and their
-// final assignments (one E-step on rows without a 16-bit copy) the exact kernel below
-// 2^30.
+// workers never time inside a request and take the exact kernel below 2^30.
 // mcg_tuning_enable(ctx, 0) or MCG_TUNE=0 always takes the MFMA form.
 static std::mutex g_tune_mu;
 static std::vector<std::pair<uint64_t, int>> g_tuned;
@@ -1380,9 +1378,13 @@ static int tuned_form(Data &ds, const float *C, uint64_t k, uint32_t *asn, hipSt
     // to assignment)
     const bool have16 = form16 == MCG_PREC_FP16 ? ds.f16_ready : ds.bf16_ready;
     if (ds.oneshot && !have16) return work < 0x1p30 ? MCG_PREC_EXACT : form16;
+    // a server's datasets are used for a few E-steps (a coreset's Lloyd iterations), so
+    // the 16-bit copy and its preparation are paid per request: the exact kernel below
+    // n*k*d = 2^30 (measured on the serving mix: the Lloyd stage 2.8 ms per request
+    // with the exact kernel against 3.2 ms with the 16-bit form from 2^23)
+    if (ds.ctx->serving) return work < 0x1p30 ? MCG_PREC_EXACT : form16;
     if (work >= 0x1p26) return form16;
     if (work <= 0x1p21) return MCG_PREC_EXACT;
-    if (ds.ctx->serving) return work < 0x1p23 ? MCG_PREC_EXACT : form16;   // the measured crossover
     prepare_base(ds);
     uint64_t lg = 0;
     while ((1ull << lg) < ds.n) ++lg;

@@ -50,19 +50,45 @@ int guard(F &&f) {
 }
 
 // ---- device buffers -------------------------------------------------------
+// Stream that device buffers created on this thread are bound to (a server worker
+// sets its context's stream for its lifetime; elsewhere none).
+inline thread_local hipStream_t g_dbuf_stream = nullptr;
+
+// A device buffer.  Bound to a stream (st), it is allocated and freed in that stream's
+// order from the device's memory pool (hipMallocAsync / hipFreeAsync), so freeing it
+// does not wait for the work of other streams: hipFree waits for the whole device,
+// which would make concurrent server workers wait for each other at every request's
+// end.  A bound buffer may only be used by work on its stream, and the stream must
+// outlive it.  Unbound, it uses hipMalloc / hipFree.
 struct DBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    hipStream_t st = g_dbuf_stream;
     DBuf() = default;
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
     ~DBuf() { release(); }
-    void release() { if (p) (void)hipFree(p); p = nullptr; bytes = 0; }
+    // (while st is being captured into a graph, the plain calls: a stream-ordered
+    // allocation there would belong to the graph)
+    bool ordered() const {
+        if (!st) return false;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        return !(hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone);
+    }
+    void release() {
+        if (p) {
+            if (ordered()) (void)hipFreeAsync(p, st);
+            else (void)hipFree(p);
+        }
+        p = nullptr;
+        bytes = 0;
+    }
     // grow-only; contents are not preserved
     void *reserve(size_t nb) {
         if (nb > bytes) {
             release();
-            HIP_CHECK(hipMalloc(&p, nb ? nb : 1));
+            if (ordered()) HIP_CHECK(hipMallocAsync(&p, nb ? nb : 1, st));
+            else HIP_CHECK(hipMalloc(&p, nb ? nb : 1));
             bytes = nb;
         }
         return p;

@@ -327,6 +327,10 @@ static double ms_between(Clock::time_point a, Clock::time_point b) {
 
 static void worker_main(mcg_server *s, mcg_ctx *ctx) {
     (void)hipSetDevice(s->device);
+    // the device buffers of this worker's requests are allocated and freed in its
+    // stream's order: a request's end then waits for its own stream only (hipFree
+    // would wait for every worker's work); the context outlives the worker
+    g_dbuf_stream = ctx->c.stream;
     for (;;) {
         std::shared_ptr<Req> r;
         {

@@ -1238,7 +1238,8 @@ void generic_costs(Data &ds, int msr, double prior, const float *C, uint64_t k, 
     const dim3 grid((uint32_t)cdiv(n, 4)), blk(256);
     // JSD rows that all share one li: the zero-baseline terms tabulated beside the table
     const float2 *CT2 = nullptr;
-    if (msr == MCG_JSD && !single && !ds.li_unique.empty() && ds.li_unique.size() <= JSD_TAB_U) {
+    if (msr == MCG_JSD && !single && !ds.li_unique.empty() && ds.li_unique.size() <= JSD_TAB_U &&
+        (uint64_t)ds.li_unique.size() * d * kpad * sizeof(float2) <= (1ull << 30)) {   // tables up to 1 GiB
         const uint64_t nel = (uint64_t)d * kpad, U = ds.li_unique.size();
         float2 *t2 = (float2 *)c.s[S_JSDT].reserve(U * nel * sizeof(float2));
         hipLaunchKernelGGL(k_jsd_pair_table, dim3((uint32_t)std::min<uint64_t>(cdiv(U * nel, 256), 4096)), dim3(256), 0,

@@ -123,36 +123,64 @@ void scan_exclusive(const uint32_t *in, uint32_t *out, uint64_t L, DBuf &tmp, hi
     hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)nb), dim3(256), 0, st, in, L, tmp.as<const uint32_t>(), out);
 }
 
-// One block of 1024 threads: counts[c] = sum_b hist[c][b]; off[c] = base[c][0];
-// itemoff = exclusive prefix of the partial-sum chunks ceil(counts / MCHUNK) (k + 1
-// entries, the last the total); imap[i] = the cluster of chunk item i.
-__global__ __launch_bounds__(1024) void k_counts_items(const uint32_t *__restrict__ hist,
-                                                       const uint32_t *__restrict__ base, uint32_t k, uint32_t nblk,
-                                                       uint32_t *__restrict__ counts, uint32_t *__restrict__ off,
-                                                       uint32_t *__restrict__ itemoff, uint32_t *__restrict__ imap) {
-    __shared__ uint32_t sm[1024];
-    const uint32_t per = (k + 1023) / 1024;
-    const uint32_t c0 = threadIdx.x * per, c1 = min(k, c0 + per);
+// One block per cluster c: base[c][b] = the exclusive prefix over the histogram blocks
+// b of hist[c][b] (the position of block b's first member inside the cluster), and
+// counts[c] = the cluster's total.
+__global__ __launch_bounds__(256) void k_colscan(const uint32_t *__restrict__ hist, uint32_t nblk,
+                                                 uint32_t *__restrict__ base, uint32_t *__restrict__ counts) {
+    __shared__ uint32_t sm[256];
+    const uint32_t c = blockIdx.x;
+    const uint64_t r = (uint64_t)c * nblk;
+    const uint32_t per = (nblk + 255) / 256;
+    const uint32_t b0 = threadIdx.x * per, b1 = min(nblk, b0 + per);
     uint32_t s = 0;
-    for (uint32_t c = c0; c < c1; ++c) {
-        const uint64_t r = (uint64_t)c * nblk;
-        const uint32_t b0 = base[r];
-        const uint32_t cnt = base[r + nblk - 1] + hist[r + nblk - 1] - b0;
-        counts[c] = cnt;
-        off[c] = b0;
-        s += (cnt + MCHUNK - 1) / MCHUNK;
-    }
+    for (uint32_t b = b0; b < b1; ++b) s += hist[r + b];
     sm[threadIdx.x] = s;
     __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
+    for (int o = 1; o < 256; o <<= 1) {
         const uint32_t t = threadIdx.x >= (unsigned)o ? sm[threadIdx.x - o] : 0u;
         __syncthreads();
         sm[threadIdx.x] += t;
         __syncthreads();
     }
     uint32_t run = threadIdx.x ? sm[threadIdx.x - 1] : 0u;
+    for (uint32_t b = b0; b < b1; ++b) { const uint32_t t = hist[r + b]; base[r + b] = run; run += t; }
+    if (threadIdx.x == 255) counts[c] = sm[255];
+}
+
+// One block of 1024 threads: off[c] = exclusive prefix of the cluster counts;
+// itemoff = exclusive prefix of the partial-sum chunks ceil(counts / MCHUNK) (k + 1
+// entries, the last the total); imap[i] = the cluster of chunk item i.
+__global__ __launch_bounds__(1024) void k_counts_items(uint32_t k, const uint32_t *__restrict__ counts,
+                                                       uint32_t *__restrict__ off, uint32_t *__restrict__ itemoff,
+                                                       uint32_t *__restrict__ imap) {
+    __shared__ uint32_t sm[1024], sn[1024];
+    const uint32_t per = (k + 1023) / 1024;
+    const uint32_t c0 = threadIdx.x * per, c1 = min(k, c0 + per);
+    uint32_t s = 0, sc = 0;
     for (uint32_t c = c0; c < c1; ++c) {
-        const uint32_t it = (counts[c] + MCHUNK - 1) / MCHUNK;
+        const uint32_t cnt = counts[c];
+        sc += cnt;
+        s += (cnt + MCHUNK - 1) / MCHUNK;
+    }
+    sm[threadIdx.x] = s;
+    sn[threadIdx.x] = sc;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const uint32_t t = threadIdx.x >= (unsigned)o ? sm[threadIdx.x - o] : 0u;
+        const uint32_t tn = threadIdx.x >= (unsigned)o ? sn[threadIdx.x - o] : 0u;
+        __syncthreads();
+        sm[threadIdx.x] += t;
+        sn[threadIdx.x] += tn;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? sm[threadIdx.x - 1] : 0u;
+    uint32_t rn = threadIdx.x ? sn[threadIdx.x - 1] : 0u;
+    for (uint32_t c = c0; c < c1; ++c) {
+        const uint32_t cnt = counts[c];
+        const uint32_t it = (cnt + MCHUNK - 1) / MCHUNK;
+        off[c] = rn;
+        rn += cnt;
         itemoff[c] = run;
         for (uint32_t i = 0; i < it; ++i) imap[run + i] = c;
         run += it;
@@ -160,12 +188,13 @@ __global__ __launch_bounds__(1024) void k_counts_items(const uint32_t *__restric
     if (threadIdx.x == 1023) itemoff[k] = sm[1023];
 }
 
-// Stable scatter: one wave per histogram block, LDS cursors for all clusters.  The
-// block's assignments are loaded up front (16 per lane) and the cursors set from
-// them, so the 16 ranking rounds run without memory waits.
+// Stable scatter: one wave per histogram block, LDS cursors for all clusters (the
+// cluster's offset plus the block's position inside the cluster).  The block's
+// assignments are loaded up front (16 per lane) and the cursors set from them, so the
+// 16 ranking rounds run without memory waits.
 __global__ __launch_bounds__(64) void k_scatter(const uint32_t *__restrict__ asn, uint64_t n, uint32_t nblk,
-                                                uint32_t k, uint32_t kbits, const uint32_t *__restrict__ base,
-                                                uint32_t *__restrict__ perm) {
+                                                uint32_t k, uint32_t kbits, const uint32_t *__restrict__ off,
+                                                const uint32_t *__restrict__ base, uint32_t *__restrict__ perm) {
     extern __shared__ uint32_t cur[];
     constexpr int R = SORT_BLK / 64;
     const uint32_t b = blockIdx.x;
@@ -180,7 +209,7 @@ __global__ __launch_bounds__(64) void k_scatter(const uint32_t *__restrict__ asn
     // lanes with the same cluster write the same cursor
 #pragma unroll
     for (int r = 0; r < R; ++r)
-        if (av[r] < k) cur[av[r]] = base[(uint64_t)av[r] * nblk + b];
+        if (av[r] < k) cur[av[r]] = off[av[r]] + base[(uint64_t)av[r] * nblk + b];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -381,14 +410,14 @@ SortOut sort_by_cluster(Ctx &c, const uint32_t *asn, uint64_t n, uint64_t k, hip
     uint32_t *off = cnt + (k + 1), *items = cnt + 2 * (k + 1);
     hipLaunchKernelGGL(k_hist, dim3(nblk), dim3(256), (size_t)k * sizeof(uint32_t), st, asn, n, nblk, (uint32_t)k,
                        hist);
-    scan_exclusive(hist, base, L, c.s[S_MISC0], st);
+    hipLaunchKernelGGL(k_colscan, dim3((uint32_t)k), dim3(256), 0, st, (const uint32_t *)hist, nblk, base, cnt);
     uint32_t *imap = (uint32_t *)c.s[S_ITEMOFF].reserve((n / MCHUNK + k + 1) * sizeof(uint32_t));
-    hipLaunchKernelGGL(k_counts_items, dim3(1), dim3(1024), 0, st, hist, base, (uint32_t)k, nblk, cnt, off, items,
+    hipLaunchKernelGGL(k_counts_items, dim3(1), dim3(1024), 0, st, (uint32_t)k, (const uint32_t *)cnt, off, items,
                        imap);
     uint32_t kbits = 1;   // bits of the largest cluster id
     while ((1ull << kbits) < k) ++kbits;
     hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(64), (size_t)k * sizeof(uint32_t), st, asn, n, nblk,
-                       (uint32_t)k, kbits, base, perm);
+                       (uint32_t)k, kbits, (const uint32_t *)off, (const uint32_t *)base, perm);
     return SortOut{perm, cnt, off, items, imap};
 }
 

@@ -348,21 +348,31 @@ __global__ __launch_bounds__(256) void k_finalize(uint32_t d, const uint32_t *__
                                                   double *__restrict__ cw) {
     const uint32_t c = blockIdx.x;
     const uint32_t i0 = itemoff[c], i1 = itemoff[c + 1];
-    double wt = 0.;
-    for (uint32_t it = i0; it < i1; ++it) wt = __dadd_rn(wt, partw[it]);
+    // the chunks' values are loaded eight at a time, then added in chunk order
+    constexpr uint32_t B = 8;
+    auto ordered_sum = [&](const double *__restrict__ v, uint64_t stride, uint64_t o) {
+        double s = 0.;
+        for (uint32_t it = i0; it < i1; it += B) {
+            double t[B];
+#pragma unroll
+            for (uint32_t u = 0; u < B; ++u) t[u] = it + u < i1 ? v[(uint64_t)(it + u) * stride + o] : 0.;
+#pragma unroll
+            for (uint32_t u = 0; u < B; ++u)
+                if (it + u < i1) s = __dadd_rn(s, t[u]);
+        }
+        return s;
+    };
+    const double wt = ordered_sum(partw, 1, 0);
     for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) {
         if (counts[c] == 0) {
             if (cold && cnew != cold) cnew[(uint64_t)c * d + j] = cold[(uint64_t)c * d + j];
             continue;
         }
-        double s = 0.;
-        for (uint32_t it = i0; it < i1; ++it) s = __dadd_rn(s, part[(uint64_t)it * d + j]);
+        const double s = ordered_sum(part, d, j);
         cnew[(uint64_t)c * d + j] = (float)__ddiv_rn(s, wt);
     }
     if (threadIdx.x == 0) {
-        double q = 0.;
-        if (partq) for (uint32_t it = i0; it < i1; ++it) q += partq[it];
-        ccost[c] = q;
+        ccost[c] = partq ? ordered_sum(partq, 1, 0) : 0.;
         cw[c] = wt;
     }
 }

@@ -208,7 +208,7 @@ enum Slot {
     S_HIST, S_BASE, S_PERM, S_CNT, S_ITEMOFF, S_PART, S_PARTW, S_PARTQ,
     S_CCOST, S_TOTAL, S_CENT_A, S_CENT_B, S_CENT_C, S_COSTS, S_CT, S_CPARAM,
     S_MISC0, S_MISC1, S_MISC2, S_MISC3, S_W, S_STAGE0, S_STAGE1, S_STAGE2,
-    S_LIS, S_TOT, S_JSDG, S_PAIRS, S_THRESH, S_OVER, S_KEYS, S_OFLAG, S_CINFO, S_CENT_D, S_LCTL, S_PREP, S_ROWLEN, S_MED0, S_MED1, S_MED2, S_MED3, S_CTM, S_JSDT, S_NSLOTS
+    S_LIS, S_TOT, S_JSDG, S_PAIRS, S_THRESH, S_OVER, S_KEYS, S_OFLAG, S_CINFO, S_CENT_D, S_LCTL, S_PREP, S_ROWLEN, S_MED0, S_MED1, S_MED2, S_MED3, S_CTM, S_JSDT, S_WSTART, S_NSLOTS
 };
 static_assert(S_NSLOTS <= 56, "scratch slots");
 

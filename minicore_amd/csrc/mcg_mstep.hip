@@ -464,9 +464,29 @@ MStepOut dense_mstep(Data &ds, const uint32_t *asn, const double *w, uint64_t k,
 // CSR_CW columns, chunk of the cluster's members); 4 waves each own an LDS FP64
 // accumulator and take members in groups of 64 in turn; a wave walks its rows in
 // order (row entries have distinct columns, so lanes never collide), the 4
-// accumulators are added in a fixed order, and the chunks in chunk order.
+// accumulators are added in a fixed order, and the chunks in chunk order.  A row's
+// entries in the block's window come from the dataset's window starts (ws), else
+// from a binary search of the row.
 // ---------------------------------------------------------------------------
 constexpr uint32_t CSR_CW = 1024;
+
+// Window starts of every CSR row (column indices ascending within a row): ws[r][w] =
+// the number of the row's entries with column < w * CSR_CW, for w = 0..nw.  One wave
+// per row; the lane holding an entry writes the starts of the windows between the
+// previous entry's window and its own, the first lane past the row's end the rest.
+__global__ __launch_bounds__(256) void k_csr_wstart(const uint32_t *__restrict__ idx, const uint64_t *__restrict__ ptr,
+                                                    uint64_t n, uint32_t nw, uint32_t *__restrict__ ws) {
+    const uint64_t r = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (r >= n) return;
+    const uint64_t p0 = ptr[r], len = ptr[r + 1] - p0;
+    uint32_t *o = ws + r * (nw + 1);
+    for (uint64_t q = lane; q <= len; q += 64) {
+        const uint32_t wq = q < len ? idx[p0 + q] / CSR_CW : nw;
+        const uint32_t wp = q ? idx[p0 + q - 1] / CSR_CW + 1 : 0;
+        for (uint32_t w = wp; w <= wq; ++w) o[w] = (uint32_t)q;
+    }
+}
 
 __global__ __launch_bounds__(256) void k_csr_mean(const float *__restrict__ val, const uint32_t *__restrict__ idx,
                                                   const uint64_t *__restrict__ ptr, uint32_t d,
@@ -474,7 +494,8 @@ __global__ __launch_bounds__(256) void k_csr_mean(const float *__restrict__ val,
                                                   const uint32_t *__restrict__ counts,
                                                   const uint32_t *__restrict__ off, const double *__restrict__ w,
                                                   uint32_t k, double *__restrict__ part,
-                                                  const double *__restrict__ cw, float *__restrict__ cnew) {
+                                                  const double *__restrict__ cw, float *__restrict__ cnew,
+                                                  const uint32_t *__restrict__ ws) {
     __shared__ double acc[4][CSR_CW];
     const uint32_t c = blockIdx.y, z = blockIdx.z, Z = gridDim.z;
     const uint32_t col0 = blockIdx.x * CSR_CW;
@@ -493,10 +514,17 @@ __global__ __launch_bounds__(256) void k_csr_mean(const float *__restrict__ val,
         double wq = 1.;
         if (q < nm) {
             const uint32_t r = perm[m0 + q];
-            uint64_t lo = ptr[r], hi = ptr[r + 1];
-            re = hi;
-            while (lo < hi) { uint64_t mid = (lo + hi) >> 1; if (idx[mid] < col0) lo = mid + 1; else hi = mid; }
-            rb = lo;
+            if (ws) {   // the window's entries [rb, re) from the row's window starts
+                const uint64_t p0 = ptr[r];
+                const uint32_t *o = ws + (uint64_t)r * (gridDim.x + 1) + blockIdx.x;
+                rb = p0 + o[0];
+                re = p0 + o[1];
+            } else {
+                uint64_t lo = ptr[r], hi = ptr[r + 1];
+                re = hi;
+                while (lo < hi) { uint64_t mid = (lo + hi) >> 1; if (idx[mid] < col0) lo = mid + 1; else hi = mid; }
+                rb = lo;
+            }
             if (w) wq = w[r];
         }
         const uint32_t nrow = min(64u, nm - g);
@@ -604,11 +632,23 @@ MStepOut csr_mstep(Data &ds, const uint32_t *asn, const double *w, uint64_t k, f
     // directly
     const uint64_t kd = k * ds.d;
     const uint32_t Z = csr_chunks(kd);
+    const uint32_t nw = (uint32_t)cdiv(ds.d, CSR_CW);
+    // MCG_CSR_WSTART=0 (read per call) keeps the per-block binary search instead
+    const char *wse = getenv("MCG_CSR_WSTART");
+    const bool use_ws = !(wse && wse[0] == '0');
+    const uint64_t wsbytes = ds.n * (nw + 1) * sizeof(uint32_t);
+    const uint32_t *ws = nullptr;
+    if (use_ws && wsbytes <= (1ull << 30)) {   // window starts of the rows as they are now
+        uint32_t *wsb = (uint32_t *)c.s[S_WSTART].reserve(wsbytes);
+        hipLaunchKernelGGL(k_csr_wstart, dim3((uint32_t)cdiv(ds.n, 4)), dim3(256), 0, st, ds.idx.as<const uint32_t>(),
+                           ds.ptr.as<const uint64_t>(), ds.n, nw, wsb);
+        ws = wsb;
+    }
     double *part = Z > 1 ? (double *)c.s[S_PART].reserve((uint64_t)Z * kd * sizeof(double)) : nullptr;
     dim3 g((uint32_t)cdiv(ds.d, CSR_CW), (uint32_t)k, Z);
     hipLaunchKernelGGL(k_csr_mean, g, dim3(256), 0, st, ds.val.as<const float>(), ds.idx.as<const uint32_t>(),
                        ds.ptr.as<const uint64_t>(), (uint32_t)ds.d, so.perm, so.counts, so.off, w, (uint32_t)k, part,
-                       (const double *)(ccost + k), Z > 1 ? nullptr : Cnew);
+                       (const double *)(ccost + k), Z > 1 ? nullptr : Cnew, ws);
     if (Z > 1)
         hipLaunchKernelGGL(k_csr_mean_final, dim3((uint32_t)cdiv(kd, 256)), dim3(256), 0, st, (const double *)part, Z,
                            (uint32_t)k, (uint32_t)ds.d, so.counts, (const double *)(ccost + k), Cnew);

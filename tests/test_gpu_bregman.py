@@ -194,6 +194,12 @@ def test_csr_mstep_chunk_counts(mcg, ctx, mco, monkeypatch, chunks):
         got, _ = ds.centroids(asn, k, prev, ww)
         np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-9)
         assert np.array_equal(got[3], prev[3])
+        # the rows' window entries found by binary search instead of the window
+        # starts: the same entries in the same order, so the same centers bit for bit
+        monkeypatch.setenv("MCG_CSR_WSTART", "0")
+        got_bs, _ = ds.centroids(asn, k, prev, ww)
+        monkeypatch.delenv("MCG_CSR_WSTART")
+        assert np.array_equal(got_bs, got)
     C0 = X.todense()[np.random.default_rng(6).choice(n, k, replace=False)] + 0.25
     ref = mco.soft(11, X, C0, prior=1e-3, temp=1., eps=-1., maxiter=2)
     got = ds.soft(11, C0, prior=1e-3, temp=1., eps=-1., maxiter=2)
